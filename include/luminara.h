@@ -1,0 +1,317 @@
+/*
+ * luminara.h - C ABI of the MI355X path tracer (render hot path of luminara).
+ *
+ * The reference renderer has no FFI; its seam is the Rust function
+ *   render::render(&RenderConfig, &Camera, &dyn Hittable, &[LightInfo]) -> RenderResult
+ *   (src/render.rs:1557-1562, called from src/main.rs:744)
+ * whose pixel loop (src/render.rs:1587-1747) produces `rows: Vec<Vec<Color>>`, a
+ * row-major linear-HDR framebuffer (row 0 = top) that the CPU post chain consumes.
+ * The scene reaching it comes from scene::load_scene (src/scene.rs:846-1645).
+ *
+ * Every entry point below returns an int status (LMN_OK = 0) and never throws;
+ * the message of the last failure on the calling thread is in lmn_last_error().
+ * A Rust host binds these 1:1 with an `extern "C"` block.
+ *
+ * Ownership: the library owns scene handles and all device memory; callers own
+ * every host buffer they pass in.  Descriptor contents are copied at create time.
+ * A scene handle is externally synchronised (one host thread at a time).
+ */
+#ifndef LUMINARA_H
+#define LUMINARA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define LMN_OK 0
+#define LMN_ERR_INVALID 1     /* bad argument / inconsistent descriptor      */
+#define LMN_ERR_PARSE 2       /* scene text does not parse (TOML, PLY, OBJ)  */
+#define LMN_ERR_IO 3          /* a referenced file cannot be read            */
+#define LMN_ERR_UNSUPPORTED 4 /* valid reference scene feature not on path   */
+#define LMN_ERR_DEVICE 5      /* HIP runtime error                           */
+#define LMN_ERR_NOMEM 6       /* host or device allocation failed            */
+
+/* ---- primitives (reference world objects, flattened) -------------------- */
+/* One lmn_prim per hit() leaf of the reference.  A [[box]] contributes six
+ * rect prims (src/rect.rs:128-141); a [[mesh]] one triangle prim per face
+ * (src/obj.rs:212-240).  `object` is the index of the top-level world-list
+ * entry the prim belongs to, in the reference's load order
+ * (src/scene.rs:1180-1642); prims of a transformed box share one object.
+ * Parameter layout p[] per kind (all FP64, world or object space as given
+ * in the scene file):
+ *   SPHERE          center[3], radius                      src/sphere.rs:24
+ *   QUAD            q[3], u[3], v[3]                        src/quad.rs:20
+ *   RECT_XY         x0, x1, y0, y1, k                       src/rect.rs:24
+ *   RECT_XZ         x0, x1, z0, z1, k                       src/rect.rs:64
+ *   RECT_YZ         y0, y1, z0, z1, k                       src/rect.rs:104
+ *   TRIANGLE        v0[3], v1[3], v2[3]   (Triangle and MeshTriangle without
+ *                   per-vertex normals/uvs)   src/triangle.rs:23, src/obj.rs:260
+ *   PLANE           point[3], normal[3] (as given)          src/plane.rs:18
+ *   TORUS           center[3], major_r, minor_r             src/torus.rs:24
+ *   SUPERELLIPSOID  center[3], scale[3], e1, e2             src/superellipsoid.rs:26
+ *   SPRING          center[3], coil_r, tube_r, pitch, turns src/spring.rs:35   */
+enum {
+  LMN_PRIM_SPHERE = 0,
+  LMN_PRIM_QUAD = 1,
+  LMN_PRIM_RECT_XY = 2,
+  LMN_PRIM_RECT_XZ = 3,
+  LMN_PRIM_RECT_YZ = 4,
+  LMN_PRIM_TRIANGLE = 5,
+  LMN_PRIM_PLANE = 6,
+  LMN_PRIM_TORUS = 7,
+  LMN_PRIM_SUPERELLIPSOID = 8,
+  LMN_PRIM_SPRING = 9,
+  LMN_PRIM_KIND_COUNT = 10
+};
+
+typedef struct lmn_prim {
+  uint32_t kind;
+  uint32_t material;  /* index into lmn_scene_desc.materials          */
+  int32_t transform;  /* index into lmn_scene_desc.transforms, or -1  */
+  uint32_t object;    /* top-level world-list index (load order)      */
+  double p[12];
+} lmn_prim;
+
+/* ---- instance transforms (src/transform.rs) ----------------------------- */
+/* Ops are listed innermost first, exactly as the loader wraps them:
+ * boxes   Scale|ScaleXYZ -> RotX -> RotY -> RotZ -> Translate (src/scene.rs:1577-1593)
+ * spheres RotX -> RotY -> RotZ -> Translate -> Scale        (src/scene.rs:1216-1230) */
+enum {
+  LMN_XF_ROTATE_X = 1,  /* v[0] = degrees       src/transform.rs:116 */
+  LMN_XF_ROTATE_Y = 2,  /* v[0] = degrees       src/transform.rs:39  */
+  LMN_XF_ROTATE_Z = 3,  /* v[0] = degrees       src/transform.rs:191 */
+  LMN_XF_TRANSLATE = 4, /* v[0..2] = offset     src/transform.rs:12  */
+  LMN_XF_SCALE = 5,     /* v[0] = factor        src/transform.rs:265 */
+  LMN_XF_SCALE_XYZ = 6  /* v[0..2] = factors    src/transform.rs:293 */
+};
+#define LMN_MAX_XF_OPS 6
+typedef struct lmn_xform_op {
+  uint32_t kind;
+  uint32_t pad;
+  double v[3];
+} lmn_xform_op;
+typedef struct lmn_transform {
+  uint32_t n_ops;
+  uint32_t pad;
+  lmn_xform_op ops[LMN_MAX_XF_OPS];
+} lmn_transform;
+
+/* ---- materials (src/material.rs) ---------------------------------------- */
+enum {
+  LMN_MAT_LAMBERTIAN = 0, /* color = albedo                src/material.rs:59  */
+  LMN_MAT_METAL = 1,      /* color = albedo, fuzz (<=1)    src/material.rs:89  */
+  LMN_MAT_DIELECTRIC = 2, /* ior, color = tint, roughness, dispersion  :144     */
+  LMN_MAT_EMISSIVE = 3,   /* color, intensity              src/material.rs:236 */
+  LMN_MAT_MICROFACET = 4  /* color, roughness, metallic    src/material.rs:322 */
+};
+typedef struct lmn_material {
+  uint32_t kind;
+  int32_t texture;   /* image texture index replacing `color`, or -1 */
+  double color[3];
+  double fuzz;
+  double ior;
+  double roughness;
+  double dispersion;
+  double intensity;
+  double metallic;
+} lmn_material;
+
+/* Image texture: RGB texels, row-major, row 0 = top of the image, values as
+ * the sampler returns them (k/255 for 8-bit files, src/texture.rs:463-479).
+ * Nearest-neighbour lookup, clamp, v flipped. */
+typedef struct lmn_texture {
+  uint32_t width;
+  uint32_t height;
+  const float* rgb; /* width*height*3 */
+} lmn_texture;
+
+/* ---- NEE lights (src/render.rs:17-39) ------------------------------------ */
+enum { LMN_LIGHT_SPHERE = 0, LMN_LIGHT_RECT = 1, LMN_LIGHT_DISK = 2 };
+typedef struct lmn_light {
+  uint32_t kind;
+  uint32_t pad;
+  double origin[3]; /* sphere/disk center, rect corner */
+  double u[3];      /* rect edge                        */
+  double v[3];      /* rect edge                        */
+  double normal[3]; /* rect/disk normal                 */
+  double radius;    /* sphere/disk radius               */
+  double emission[3];
+} lmn_light;
+
+/* ---- background (src/render.rs:43-172) ----------------------------------- */
+enum { LMN_BG_SKY = 0, LMN_BG_SOLID = 1, LMN_BG_GRADIENT = 2, LMN_BG_SUN = 3 };
+typedef struct lmn_background {
+  uint32_t kind;
+  uint32_t pad;
+  double color0[3]; /* solid color / gradient bottom / sun color */
+  double color1[3]; /* gradient top / sky color                   */
+  double direction[3];
+  double intensity;
+} lmn_background;
+
+/* ---- camera (src/camera.rs:48-103) --------------------------------------- */
+typedef struct lmn_camera {
+  double origin[3];
+  double lower_left[3];
+  double horizontal[3];
+  double vertical[3];
+  double u[3];
+  double v[3];
+  double lens_radius;
+} lmn_camera;
+
+/* ---- render settings (src/render.rs:239-521, the hot-path subset) -------- */
+enum { LMN_FILTER_BOX = 0, LMN_FILTER_TRIANGLE = 1, LMN_FILTER_GAUSSIAN = 2, LMN_FILTER_MITCHELL = 3 };
+typedef struct lmn_render_config {
+  uint32_t width, height, samples, max_depth;
+  uint64_t seed;
+  uint32_t filter;
+  uint32_t has_crop;
+  uint32_t crop[4]; /* x, y, w, h */
+  double exposure;
+  uint32_t tone_map; /* 0 ACES, 1 Reinhard, 2 Filmic, 3 none (post chain, CPU) */
+  uint32_t pad;
+  lmn_background background;
+} lmn_render_config;
+
+/* ---- scene descriptor ---------------------------------------------------- */
+typedef struct lmn_scene_desc {
+  uint32_t n_prims;
+  uint32_t n_transforms;
+  uint32_t n_materials;
+  uint32_t n_textures;
+  uint32_t n_lights;
+  uint32_t n_objects;
+  const lmn_prim* prims;
+  const lmn_transform* transforms;
+  const lmn_material* materials;
+  const lmn_texture* textures;
+  const lmn_light* lights;
+  lmn_camera camera;
+  lmn_render_config config; /* [render] settings as loaded (CLI applies on top) */
+} lmn_scene_desc;
+
+/* ---- per-call render parameters ----------------------------------------- */
+enum {
+  LMN_PATH_WAVEFRONT_FP32 = 0, /* the product path                         */
+  LMN_PATH_PLAIN_FP64 = 1,     /* one path per thread, FP64 megakernel     */
+  LMN_PATH_PLAIN_FP32 = 2      /* one path per thread, FP32 megakernel     */
+};
+enum {
+  LMN_ISECT_EXACT = 0,    /* FP64 quartic / bracketed roots for marched prims */
+  LMN_ISECT_REFERENCE = 1 /* the reference's SDF march semantics, FP64        */
+};
+typedef struct lmn_render_params {
+  uint32_t width, height, spp, max_depth;
+  uint64_t seed;
+  uint32_t filter;
+  uint32_t has_crop;
+  uint32_t crop[4];
+  uint32_t path;           /* LMN_PATH_*                                   */
+  uint32_t intersect_mode; /* LMN_ISECT_*                                  */
+  uint32_t scene_key;      /* RNG domain separator (batch index)          */
+  uint32_t pool_paths;     /* wavefront path-pool size, 0 = library pick  */
+  uint32_t block_size;     /* threads per workgroup, 0 = library pick     */
+  uint32_t stack_lds;      /* LDS short-stack entries, 0 = library pick   */
+  void* stream;            /* hipStream_t, NULL = library stream          */
+} lmn_render_params;
+
+typedef struct lmn_stats {
+  uint64_t paths;          /* W*H*ceil(sqrt(spp))^2 (src/render.rs:1589)  */
+  uint64_t camera_rays;
+  uint64_t extension_rays; /* closest-hit queries after the camera ray    */
+  uint64_t shadow_rays;    /* NEE visibility queries                      */
+  double render_ms;        /* device time of the pixel loop               */
+  uint32_t iterations;     /* wavefront iterations executed               */
+  uint32_t pad;
+} lmn_stats;
+
+/* ---- ray-batch parity harness -------------------------------------------- */
+typedef struct lmn_ray {
+  double origin[3];
+  double direction[3];
+  double t_min, t_max;
+} lmn_ray;
+typedef struct lmn_hit {
+  double t;          /* reference-reported t (unnormalised direction units) */
+  int64_t prim;      /* index into lmn_scene_desc.prims, -1 = miss          */
+  double u, v;
+  double normal[3];  /* facing the ray (HitRecord::new, src/hit.rs:19-43)   */
+  int32_t front_face;
+  int32_t pad;
+} lmn_hit;
+
+/* ---- entry points --------------------------------------------------------- */
+
+/* Last error message of the calling thread ("" if none). */
+const char* lmn_last_error(void);
+
+/* Parse scene TOML exactly as scene::load_scene (src/scene.rs:846-1645) does
+ * for the hot-path subset; file references (meshes, textures) are resolved
+ * relative to base_dir (NULL = current directory). The descriptor is owned
+ * by the library and released with lmn_desc_free. */
+int lmn_load_scene(const char* toml_text, const char* base_dir, lmn_scene_desc** out);
+void lmn_desc_free(lmn_scene_desc* desc);
+
+/* Replace texture `index` texels (configs with programmatic albedo grids). */
+int lmn_desc_set_texture(lmn_scene_desc* desc, uint32_t index, uint32_t width, uint32_t height,
+                         const float* rgb);
+/* Append an image texture; returns its index in *out_index. */
+int lmn_desc_add_texture(lmn_scene_desc* desc, uint32_t width, uint32_t height, const float* rgb,
+                         uint32_t* out_index);
+/* Point material `material` at texture `texture` (-1 = solid colour). */
+int lmn_desc_set_material(lmn_scene_desc* desc, uint32_t index, const lmn_material* m);
+
+/* Flatten, build the SAH BVH and upload a scene to the current HIP device. */
+typedef struct lmn_scene lmn_scene;
+int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out);
+void lmn_scene_destroy(lmn_scene* scene);
+/* BVH shape of the uploaded scene (node count, leaf count, max depth). */
+int lmn_scene_info(const lmn_scene* scene, uint64_t* n_nodes, uint64_t* n_leaves, uint32_t* max_depth,
+                   uint64_t* device_bytes);
+
+/* Render one frame: replaces the pixel loop src/render.rs:1587-1747.
+ * hdr_rgb: caller-owned host buffer of crop_h*crop_w*3 floats (or H*W*3),
+ * row 0 = top, linear HDR before exposure. */
+int lmn_render(const lmn_scene* scene, const lmn_render_params* params, float* hdr_rgb, lmn_stats* stats);
+/* Same, with the framebuffer left in device memory (d_hdr_rgb is a device
+ * pointer of the same size).  No host copies inside the timed region. */
+int lmn_render_device(const lmn_scene* scene, const lmn_render_params* params, float* d_hdr_rgb,
+                      lmn_stats* stats);
+
+/* Render many scenes of uneven size in one wavefront: path pools are shared
+ * and every scene's paths are issued from one work list. */
+int lmn_render_batch(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
+                     float* const* hdr_rgb, lmn_stats* stats);
+
+/* Closest-hit queries on a ray batch (SceneWorld::hit, src/scene.rs:817-834).
+ * path: LMN_PATH_WAVEFRONT_FP32 traces in FP32, LMN_PATH_PLAIN_FP64 in FP64. */
+int lmn_trace_rays(const lmn_scene* scene, const lmn_ray* rays, size_t n, uint32_t path,
+                   uint32_t intersect_mode, lmn_hit* hits);
+
+/* Device / library information. */
+int lmn_device_info(char* name, size_t name_len, int* n_cu, double* hbm_bytes);
+
+/* Traversal work of the extend stage, counted by an instrumented run of the
+ * same kernels on the same frame: the algorithmic bytes of the roofline are
+ * node_fetches * node_bytes + prim_tests * prim_bytes.  No reference
+ * counterpart: measurement support for the render entry. */
+typedef struct lmn_trace_counts {
+  uint64_t node_fetches;
+  uint64_t prim_tests;
+  uint64_t tri_tests;
+  uint64_t rays;
+  uint32_t node_bytes;
+  uint32_t prim_bytes;
+} lmn_trace_counts;
+int lmn_profile_extend(const lmn_scene* scene, const lmn_render_params* params, float* d_hdr_rgb,
+                       lmn_trace_counts* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LUMINARA_H */

@@ -1,0 +1,236 @@
+"""MI355X-native render hot path of luminara (HIP kernels behind a C ABI).
+
+Host-side mirror of the reference's render entry:
+
+    config, camera, world = load_scene(toml_text)        # scene::load_scene (src/scene.rs:846)
+    result = render(config, camera, world, world.lights)  # render::render (src/render.rs:1557)
+    result.rows            # H x W x 3 linear HDR, row 0 = top (src/render.rs:1598-1731)
+    result.total_rays      # primary samples (src/render.rs:1704)
+    result.render_time_secs
+
+All work runs in liblmn.so (HIP, gfx950); nothing here computes pixels.
+"""
+import ctypes as C
+import dataclasses
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _abi
+from ._abi import LuminaraError, check, lib
+
+FILTERS = {"box": _abi.FILTER_BOX, "triangle": _abi.FILTER_TRIANGLE, "gaussian": _abi.FILTER_GAUSSIAN,
+           "mitchell": _abi.FILTER_MITCHELL}
+PATHS = {"wavefront": _abi.PATH_WAVEFRONT_FP32, "plain64": _abi.PATH_PLAIN_FP64,
+         "plain32": _abi.PATH_PLAIN_FP32}
+
+
+@dataclasses.dataclass
+class RenderConfig:
+    """Hot-path subset of the reference's RenderConfig (src/render.rs:239-521)."""
+    width: int = 800
+    height: int = 450
+    samples_per_pixel: int = 100
+    max_depth: int = 50
+    seed: int = 31337
+    pixel_filter: int = _abi.FILTER_BOX
+    crop: Optional[Tuple[int, int, int, int]] = None
+    exposure: float = 1.0
+    tone_map: int = 0
+
+
+class SceneWorld:
+    """A loaded scene: the flattened descriptor plus (lazily) its device copy."""
+
+    def __init__(self, desc_ptr):
+        self._desc = desc_ptr
+        self._scene = None
+
+    @property
+    def desc(self):
+        return self._desc
+
+    @property
+    def desc_struct(self) -> _abi.SceneDesc:
+        return self._desc.contents
+
+    @property
+    def lights(self):
+        d = self.desc_struct
+        return [d.lights[i] for i in range(d.n_lights)]
+
+    @property
+    def prims(self):
+        d = self.desc_struct
+        return [d.prims[i] for i in range(d.n_prims)]
+
+    def materials(self):
+        d = self.desc_struct
+        return [d.materials[i] for i in range(d.n_materials)]
+
+    def set_material(self, index, mat: _abi.Material):
+        check(lib().lmn_desc_set_material(self._desc, index, C.byref(mat)))
+        self.invalidate()
+
+    def add_texture(self, rgb: np.ndarray) -> int:
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        h, w, _ = rgb.shape
+        idx = C.c_uint32()
+        check(lib().lmn_desc_add_texture(self._desc, w, h, rgb.ctypes.data_as(C.POINTER(C.c_float)),
+                                         C.byref(idx)))
+        self.invalidate()
+        return idx.value
+
+    def set_texture(self, index, rgb: np.ndarray):
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        h, w, _ = rgb.shape
+        check(lib().lmn_desc_set_texture(self._desc, index, w, h, rgb.ctypes.data_as(C.POINTER(C.c_float))))
+        self.invalidate()
+
+    def invalidate(self):
+        if self._scene is not None:
+            lib().lmn_scene_destroy(self._scene)
+            self._scene = None
+
+    def device_scene(self):
+        """Flattens, builds the BVH and uploads to the current HIP device."""
+        if self._scene is None:
+            h = C.c_void_p()
+            check(lib().lmn_scene_create(self._desc, C.byref(h)))
+            self._scene = h
+        return self._scene
+
+    def bvh_info(self):
+        n_nodes, n_leaves, dev_bytes = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        depth = C.c_uint32()
+        check(lib().lmn_scene_info(self.device_scene(), C.byref(n_nodes), C.byref(n_leaves), C.byref(depth),
+                                   C.byref(dev_bytes)))
+        return {"nodes": n_nodes.value, "leaves": n_leaves.value, "depth": depth.value,
+                "device_bytes": dev_bytes.value}
+
+    def __del__(self):
+        try:
+            self.invalidate()
+            if self._desc is not None:
+                lib().lmn_desc_free(self._desc)
+                self._desc = None
+        except Exception:
+            pass
+
+
+def load_scene(toml_text: str, base_dir: Optional[str] = None):
+    """scene::load_scene: returns (RenderConfig, camera, SceneWorld).
+
+    Raises LuminaraError (code LMN_ERR_PARSE / LMN_ERR_IO / LMN_ERR_UNSUPPORTED)
+    where the reference returns Err(String).
+    """
+    out = C.POINTER(_abi.SceneDesc)()
+    check(lib().lmn_load_scene(toml_text.encode(), base_dir.encode() if base_dir else None, C.byref(out)))
+    world = SceneWorld(out)
+    c = out.contents.config
+    cfg = RenderConfig(width=c.width, height=c.height, samples_per_pixel=c.samples, max_depth=c.max_depth,
+                       seed=c.seed, pixel_filter=c.filter, exposure=c.exposure, tone_map=c.tone_map)
+    return cfg, out.contents.camera, world
+
+
+def make_params(config: RenderConfig, path="wavefront", scene_key=0, intersect_mode=_abi.ISECT_EXACT,
+                pool_paths=0, stack_lds=0, stream=None) -> _abi.RenderParams:
+    p = _abi.RenderParams()
+    p.width, p.height = config.width, config.height
+    p.spp, p.max_depth = config.samples_per_pixel, config.max_depth
+    p.seed = config.seed
+    p.filter = config.pixel_filter
+    if config.crop is not None:
+        p.has_crop = 1
+        for i, v in enumerate(config.crop):
+            p.crop[i] = v
+    p.path = PATHS[path] if isinstance(path, str) else path
+    p.intersect_mode = intersect_mode
+    p.scene_key = scene_key
+    p.pool_paths = pool_paths
+    p.stack_lds = stack_lds
+    p.stream = stream
+    return p
+
+
+def output_shape(config: RenderConfig):
+    w, h = max(config.width, 1), max(config.height, 1)
+    if config.crop is not None:
+        cx, cy, cw, ch = config.crop
+        return min(ch, h), min(cw, w)
+    return h, w
+
+
+@dataclasses.dataclass
+class RenderResult:
+    """RenderResult (src/render.rs:1540-1554), HDR part."""
+    rows: np.ndarray
+    total_rays: int
+    render_time_secs: float
+    stats: dict
+
+
+def render(config: RenderConfig, camera, world: SceneWorld, lights=None, path="wavefront", **kw) -> RenderResult:
+    """render::render's pixel loop on the GPU.  `camera` and `lights` are the
+    ones load_scene produced (the descriptor holds them); CLI overrides of
+    width/height do not rebuild the camera (src/main.rs:205-210)."""
+    del camera, lights  # carried by the scene descriptor
+    scene = world.device_scene()
+    params = make_params(config, path=path, **kw)
+    h, w = output_shape(config)
+    out = np.empty((h, w, 3), dtype=np.float32)
+    st = _abi.Stats()
+    check(lib().lmn_render(scene, C.byref(params), out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(st)))
+    stats = {k: getattr(st, k) for k, _ in _abi.Stats._fields_ if k != "pad"}
+    return RenderResult(rows=out, total_rays=st.paths, render_time_secs=st.render_ms / 1000.0, stats=stats)
+
+
+def render_device(world: SceneWorld, params: _abi.RenderParams, out_ptr: int) -> dict:
+    """Renders into a caller-owned device buffer (e.g. a torch CUDA tensor)."""
+    st = _abi.Stats()
+    check(lib().lmn_render_device(world.device_scene(), C.byref(params), C.c_void_p(out_ptr), C.byref(st)))
+    return {k: getattr(st, k) for k, _ in _abi.Stats._fields_ if k != "pad"}
+
+
+def profile_extend(world: SceneWorld, params: _abi.RenderParams, out_ptr: int) -> dict:
+    tc = _abi.TraceCounts()
+    check(lib().lmn_profile_extend(world.device_scene(), C.byref(params), C.c_void_p(out_ptr), C.byref(tc)))
+    return {k: getattr(tc, k) for k, _ in _abi.TraceCounts._fields_}
+
+
+def rays_to_struct(rays: np.ndarray):
+    """rays: (n, 8) float64 = origin.xyz, direction.xyz, t_min, t_max."""
+    rays = np.ascontiguousarray(rays, dtype=np.float64)
+    assert rays.ndim == 2 and rays.shape[1] == 8
+    arr = (_abi.Ray * len(rays)).from_buffer_copy(rays.tobytes())
+    return arr
+
+
+def hits_to_numpy(hits, n):
+    buf = np.frombuffer(bytes(hits), dtype=np.dtype([("t", "f8"), ("prim", "i8"), ("u", "f8"), ("v", "f8"),
+                                                     ("normal", "f8", 3), ("front_face", "i4"), ("pad", "i4")]),
+                        count=n)
+    return buf
+
+
+def trace_rays(world: SceneWorld, rays: np.ndarray, path="wavefront", intersect_mode=_abi.ISECT_EXACT):
+    """Closest hit per ray on the GPU (SceneWorld::hit, src/scene.rs:817)."""
+    n = len(rays)
+    r = rays_to_struct(rays)
+    hits = (_abi.Hit * max(n, 1))()
+    check(lib().lmn_trace_rays(world.device_scene(), r, n, PATHS[path] if isinstance(path, str) else path,
+                               intersect_mode, hits))
+    return hits_to_numpy(hits, n)
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    ncu = C.c_int()
+    hbm = C.c_double()
+    check(lib().lmn_device_info(name, 256, C.byref(ncu), C.byref(hbm)))
+    return {"name": name.value.decode(), "cu": ncu.value, "hbm_bytes": hbm.value}
+
+
+__all__ = ["RenderConfig", "SceneWorld", "RenderResult", "LuminaraError", "load_scene", "render",
+           "render_device", "trace_rays", "device_info", "make_params", "profile_extend"]

@@ -1,0 +1,608 @@
+// Host-side scene flattening and SAH BVH build (see scene_build.h).
+#include "scene_build.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace lmn {
+
+namespace {
+
+void mat_identity(double m[12]) {
+  std::memset(m, 0, 12 * sizeof(double));
+  m[0] = m[5] = m[10] = 1.0;
+}
+// c = a * b for 3x4 affine matrices (b applied first)
+void mat_mul(const double a[12], const double b[12], double c[12]) {
+  double r[12];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 4; ++j) {
+      double s = a[i * 4 + 0] * b[0 * 4 + j] + a[i * 4 + 1] * b[1 * 4 + j] + a[i * 4 + 2] * b[2 * 4 + j];
+      if (j == 3) s += a[i * 4 + 3];
+      r[i * 4 + j] = s;
+    }
+  }
+  std::memcpy(c, r, sizeof r);
+}
+void mat3_mul(const double a[9], const double b[9], double c[9]) {
+  double r[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r[i * 3 + j] = a[i * 3 + 0] * b[0 * 3 + j] + a[i * 3 + 1] * b[1 * 3 + j] + a[i * 3 + 2] * b[2 * 3 + j];
+  std::memcpy(c, r, sizeof r);
+}
+
+// Rust's f64::to_radians multiplies by PI/180.
+double to_radians(double deg) { return deg * (M_PI / 180.0); }
+
+// Scale::new / NonUniformScale::new replace a near-zero factor by 1
+// (src/transform.rs:266-268, :294-297)
+double fix_scale(double f) { return std::fabs(f) < 1e-10 ? 1.0 : f; }
+
+}  // namespace
+
+void compose_transform(const lmn_transform& t, XformT<double>& x, double fwd[12]) {
+  double inv[12], nrm[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  mat_identity(inv);
+  mat_identity(fwd);
+  x.tmode = 0;
+  x.tscale = 1.0;
+  x.renorm = 0;
+  for (uint32_t k = 0; k < t.n_ops && k < LMN_MAX_XF_OPS; ++k) {
+    const lmn_xform_op& op = t.ops[k];
+    double F[12], I[12], N[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    mat_identity(F);
+    mat_identity(I);
+    switch (op.kind) {
+      case LMN_XF_ROTATE_X: {  // src/transform.rs:148-176
+        double r = to_radians(op.v[0]), s = std::sin(r), c = std::cos(r);
+        F[5] = c; F[6] = -s; F[9] = s; F[10] = c;
+        I[5] = c; I[6] = s; I[9] = -s; I[10] = c;
+        N[4] = c; N[5] = -s; N[7] = s; N[8] = c;
+        break;
+      }
+      case LMN_XF_ROTATE_Y: {  // src/transform.rs:72-101
+        double r = to_radians(op.v[0]), s = std::sin(r), c = std::cos(r);
+        F[0] = c; F[2] = s; F[8] = -s; F[10] = c;
+        I[0] = c; I[2] = -s; I[8] = s; I[10] = c;
+        N[0] = c; N[2] = s; N[6] = -s; N[8] = c;
+        break;
+      }
+      case LMN_XF_ROTATE_Z: {  // src/transform.rs:224-251
+        double r = to_radians(op.v[0]), s = std::sin(r), c = std::cos(r);
+        F[0] = c; F[1] = -s; F[4] = s; F[5] = c;
+        I[0] = c; I[1] = s; I[4] = -s; I[5] = c;
+        N[0] = c; N[1] = -s; N[3] = s; N[4] = c;
+        break;
+      }
+      case LMN_XF_TRANSLATE:  // src/transform.rs:18-24
+        F[3] = op.v[0]; F[7] = op.v[1]; F[11] = op.v[2];
+        I[3] = -op.v[0]; I[7] = -op.v[1]; I[11] = -op.v[2];
+        break;
+      case LMN_XF_SCALE: {  // src/transform.rs:272-279
+        double f = fix_scale(op.v[0]), inv_f = 1.0 / f;
+        F[0] = F[5] = F[10] = f;
+        I[0] = I[5] = I[10] = inv_f;
+        x.tmode = 1;
+        x.tscale = f;
+        break;
+      }
+      case LMN_XF_SCALE_XYZ: {  // src/transform.rs:306-338
+        double s[3] = {fix_scale(op.v[0]), fix_scale(op.v[1]), fix_scale(op.v[2])};
+        for (int a = 0; a < 3; ++a) {
+          F[a * 5] = s[a];
+          I[a * 5] = 1.0 / s[a];
+          N[a * 4] = 1.0 / s[a];
+        }
+        x.tmode = 2;
+        x.renorm = 1;
+        break;
+      }
+      default: break;
+    }
+    mat_mul(F, fwd, fwd);   // forward: new op applied after (outside) the inner ones
+    mat_mul(inv, I, inv);   // inverse: outer op undone first
+    mat3_mul(N, nrm, nrm);
+  }
+  std::memcpy(x.inv, inv, sizeof inv);
+  std::memcpy(x.nrm, nrm, sizeof nrm);
+}
+
+// Object-space bounds of a primitive (the reference's bounding_box()).
+static void object_bounds(const lmn_prim& p, double lo[3], double hi[3]) {
+  const double* q = p.p;
+  switch (p.kind) {
+    case LMN_PRIM_SPHERE:  // src/sphere.rs:64-67
+      for (int k = 0; k < 3; ++k) { lo[k] = q[k] - q[3]; hi[k] = q[k] + q[3]; }
+      break;
+    case LMN_PRIM_QUAD: {  // src/quad.rs:64-72
+      for (int k = 0; k < 3; ++k) {
+        double c[4] = {q[k], q[k] + q[3 + k], q[k] + q[6 + k], q[k] + q[3 + k] + q[6 + k]};
+        lo[k] = std::min(std::min(c[0], c[1]), std::min(c[2], c[3])) - 1e-4;
+        hi[k] = std::max(std::max(c[0], c[1]), std::max(c[2], c[3])) + 1e-4;
+      }
+      break;
+    }
+    case LMN_PRIM_RECT_XY:  // src/rect.rs:39-44
+      lo[0] = q[0]; hi[0] = q[1]; lo[1] = q[2]; hi[1] = q[3]; lo[2] = q[4] - 0.0001; hi[2] = q[4] + 0.0001;
+      break;
+    case LMN_PRIM_RECT_XZ:  // src/rect.rs:79-84
+      lo[0] = q[0]; hi[0] = q[1]; lo[2] = q[2]; hi[2] = q[3]; lo[1] = q[4] - 0.0001; hi[1] = q[4] + 0.0001;
+      break;
+    case LMN_PRIM_RECT_YZ:  // src/rect.rs:119-124
+      lo[1] = q[0]; hi[1] = q[1]; lo[2] = q[2]; hi[2] = q[3]; lo[0] = q[4] - 0.0001; hi[0] = q[4] + 0.0001;
+      break;
+    case LMN_PRIM_TRIANGLE:  // src/triangle.rs:66-70
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(std::min(q[k], q[3 + k]), q[6 + k]) - 1e-4;
+        hi[k] = std::max(std::max(q[k], q[3 + k]), q[6 + k]) + 1e-4;
+      }
+      break;
+    case LMN_PRIM_TORUS: {  // src/torus.rs:105-111
+      double r = q[3] + q[4];
+      double e[3] = {r, q[4], r};
+      for (int k = 0; k < 3; ++k) { lo[k] = q[k] - e[k]; hi[k] = q[k] + e[k]; }
+      break;
+    }
+    case LMN_PRIM_SUPERELLIPSOID:  // src/superellipsoid.rs:92-97
+      for (int k = 0; k < 3; ++k) { lo[k] = q[k] - q[3 + k]; hi[k] = q[k] + q[3 + k]; }
+      break;
+    case LMN_PRIM_SPRING: {  // src/spring.rs:107-114
+      double r = q[3] + q[4], h = q[5] * q[6];
+      lo[0] = q[0] - r; hi[0] = q[0] + r;
+      lo[1] = q[1] - q[4]; hi[1] = q[1] + h + q[4];
+      lo[2] = q[2] - r; hi[2] = q[2] + r;
+      break;
+    }
+    default:
+      for (int k = 0; k < 3; ++k) { lo[k] = -INFINITY; hi[k] = INFINITY; }
+  }
+}
+
+void prim_world_bounds(const lmn_scene_desc& desc, const lmn_prim& p, double lo[3], double hi[3]) {
+  object_bounds(p, lo, hi);
+  if (p.transform < 0) return;
+  XformT<double> x;
+  double fwd[12];
+  compose_transform(desc.transforms[p.transform], x, fwd);
+  double nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int c = 0; c < 8; ++c) {
+    double v[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+    for (int k = 0; k < 3; ++k) {
+      double w = fwd[k * 4 + 0] * v[0] + fwd[k * 4 + 1] * v[1] + fwd[k * 4 + 2] * v[2] + fwd[k * 4 + 3];
+      nlo[k] = std::min(nlo[k], w);
+      nhi[k] = std::max(nhi[k], w);
+    }
+  }
+  // pad by a relative epsilon: transformed surfaces are evaluated in the
+  // object frame, so the box must cover their rounding
+  for (int k = 0; k < 3; ++k) {
+    double pad = 1e-9 * (std::fabs(nlo[k]) + std::fabs(nhi[k]) + 1.0);
+    lo[k] = nlo[k] - pad;
+    hi[k] = nhi[k] + pad;
+  }
+}
+
+namespace {
+
+struct BRef {
+  float c[3];
+  uint32_t idx;
+};
+
+struct Builder {
+  const std::vector<double>& blo;  // 3 per prim
+  const std::vector<double>& bhi;
+  std::vector<BRef> refs;
+  std::vector<NodeD> nodes;
+  std::vector<uint32_t> order;
+  BvhOptions opt;
+  uint32_t depth_max = 0, leaves = 0;
+
+  Builder(const std::vector<double>& lo, const std::vector<double>& hi) : blo(lo), bhi(hi) {}
+
+  void bounds_of(size_t b, size_t e, double lo[3], double hi[3], double clo[3], double chi[3]) {
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = clo[k] = INFINITY;
+      hi[k] = chi[k] = -INFINITY;
+    }
+    for (size_t i = b; i < e; ++i) {
+      uint32_t p = refs[i].idx;
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], blo[p * 3 + k]);
+        hi[k] = std::max(hi[k], bhi[p * 3 + k]);
+        clo[k] = std::min(clo[k], double(refs[i].c[k]));
+        chi[k] = std::max(chi[k], double(refs[i].c[k]));
+      }
+    }
+  }
+  static double area(const double lo[3], const double hi[3]) {
+    double d[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    for (double& x : d)
+      if (!(x > 0)) x = 0;
+    return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+  }
+
+  uint32_t build(size_t b, size_t e, uint32_t depth) {
+    uint32_t id = uint32_t(nodes.size());
+    nodes.push_back(NodeD{});
+    double lo[3], hi[3], clo[3], chi[3];
+    bounds_of(b, e, lo, hi, clo, chi);
+    for (int k = 0; k < 3; ++k) {
+      nodes[id].lo[k] = lo[k];
+      nodes[id].hi[k] = hi[k];
+    }
+    depth_max = std::max(depth_max, depth);
+    size_t n = e - b;
+    bool leaf = n <= 1 || depth >= uint32_t(opt.max_depth);
+    int best_axis = -1;
+    size_t mid = b;
+    if (!leaf) {
+      // binned SAH over centroid bounds (the reference bins 12 buckets per
+      // axis, src/bvh.rs:71-163)
+      const int NB = opt.bins;
+      double sa = area(lo, hi);
+      double best_cost = INFINITY;
+      int best_split = -1;
+      std::vector<double> blo_b(NB * 3), bhi_b(NB * 3);
+      std::vector<size_t> cnt(NB);
+      std::vector<double> rarea(NB);
+      std::vector<size_t> rcount(NB);
+      for (int ax = 0; ax < 3; ++ax) {
+        double ext = chi[ax] - clo[ax];
+        if (!(ext > 1e-12 * (std::fabs(chi[ax]) + std::fabs(clo[ax]) + 1e-30))) continue;
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int i = 0; i < NB * 3; ++i) {
+          blo_b[i] = INFINITY;
+          bhi_b[i] = -INFINITY;
+        }
+        double scale = NB / ext;
+        for (size_t i = b; i < e; ++i) {
+          int bi = int((refs[i].c[ax] - clo[ax]) * scale);
+          bi = std::min(std::max(bi, 0), NB - 1);
+          cnt[bi]++;
+          uint32_t p = refs[i].idx;
+          for (int k = 0; k < 3; ++k) {
+            blo_b[bi * 3 + k] = std::min(blo_b[bi * 3 + k], blo[p * 3 + k]);
+            bhi_b[bi * 3 + k] = std::max(bhi_b[bi * 3 + k], bhi[p * 3 + k]);
+          }
+        }
+        double rl[3] = {INFINITY, INFINITY, INFINITY}, rh[3] = {-INFINITY, -INFINITY, -INFINITY};
+        size_t rc = 0;
+        for (int i = NB - 1; i > 0; --i) {
+          rc += cnt[i];
+          for (int k = 0; k < 3; ++k) {
+            rl[k] = std::min(rl[k], blo_b[i * 3 + k]);
+            rh[k] = std::max(rh[k], bhi_b[i * 3 + k]);
+          }
+          rarea[i] = area(rl, rh);
+          rcount[i] = rc;
+        }
+        double ll[3] = {INFINITY, INFINITY, INFINITY}, lh[3] = {-INFINITY, -INFINITY, -INFINITY};
+        size_t lc = 0;
+        for (int i = 0; i < NB - 1; ++i) {
+          lc += cnt[i];
+          for (int k = 0; k < 3; ++k) {
+            ll[k] = std::min(ll[k], blo_b[i * 3 + k]);
+            lh[k] = std::max(lh[k], bhi_b[i * 3 + k]);
+          }
+          if (lc == 0 || rcount[i + 1] == 0) continue;
+          double cost = 1.0 + (double(lc) * area(ll, lh) + double(rcount[i + 1]) * rarea[i + 1]) / std::max(sa, 1e-300);
+          if (cost < best_cost) {
+            best_cost = cost;
+            best_axis = ax;
+            best_split = i;
+          }
+        }
+      }
+      if (best_axis >= 0 && n <= size_t(opt.max_leaf) && double(n) <= best_cost) leaf = true;
+      if (!leaf) {
+        if (best_axis >= 0) {
+          double ext = chi[best_axis] - clo[best_axis];
+          double scale = NB / ext;
+          auto it = std::partition(refs.begin() + b, refs.begin() + e, [&](const BRef& r) {
+            int bi = int((r.c[best_axis] - clo[best_axis]) * scale);
+            bi = std::min(std::max(bi, 0), NB - 1);
+            return bi <= best_split;
+          });
+          mid = size_t(it - refs.begin());
+        }
+        if (best_axis < 0 || mid == b || mid == e) {
+          if (n <= size_t(opt.max_leaf)) {
+            leaf = true;
+          } else {  // coincident centroids: median split on the widest axis
+            int ax = 0;
+            for (int k = 1; k < 3; ++k)
+              if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+            mid = b + n / 2;
+            std::nth_element(refs.begin() + b, refs.begin() + mid, refs.begin() + e,
+                             [&](const BRef& x, const BRef& y) { return x.c[ax] < y.c[ax]; });
+            best_axis = ax;
+          }
+        }
+      }
+    }
+    if (leaf) {
+      nodes[id].a = uint32_t(order.size());
+      nodes[id].b = LEAF_BIT | uint32_t(n);
+      for (size_t i = b; i < e; ++i) order.push_back(refs[i].idx);
+      leaves++;
+      return id;
+    }
+    build(b, mid, depth + 1);  // left child lands at id + 1
+    uint32_t r = build(mid, e, depth + 1);
+    nodes[id].a = r;
+    nodes[id].b = uint32_t(best_axis < 0 ? 0 : best_axis);
+    return id;
+  }
+};
+
+float round_down(double x) {
+  float f = float(x);
+  if (double(f) > x) f = std::nextafter(f, -INFINITY);
+  return f;
+}
+float round_up(double x) {
+  float f = float(x);
+  if (double(f) < x) f = std::nextafter(f, INFINITY);
+  return f;
+}
+
+template <class R>
+void fill_record(const lmn_prim& p, const XformT<double>* x, PrimT<R>& r, uint32_t prim_id, bool fp64_island) {
+  (void)x;
+  std::memset(&r, 0, sizeof r);
+  const double* q = p.p;
+  double v[13] = {0};
+  switch (p.kind) {
+    case LMN_PRIM_QUAD: {  // Quad::new (src/quad.rs:20-26)
+      double u[3] = {q[3], q[4], q[5]}, w[3] = {q[6], q[7], q[8]};
+      double n[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+      double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+      double nl = std::sqrt(nn);
+      double D = (n[0] / nl) * q[0] + (n[1] / nl) * q[1] + (n[2] / nl) * q[2];
+      for (int k = 0; k < 9; ++k) v[k] = q[k];
+      for (int k = 0; k < 3; ++k) v[9 + k] = n[k] / nn;
+      v[12] = D;
+      break;
+    }
+    case LMN_PRIM_TRIANGLE:
+      for (int k = 0; k < 3; ++k) {
+        v[k] = q[k];
+        v[3 + k] = q[3 + k] - q[k];
+        v[6 + k] = q[6 + k] - q[k];
+      }
+      break;
+    case LMN_PRIM_PLANE: {  // Plane::new (src/plane.rs:18-39)
+      double n[3] = {q[3], q[4], q[5]};
+      bool nz = std::fabs(n[0]) < 1e-8 && std::fabs(n[1]) < 1e-8 && std::fabs(n[2]) < 1e-8;
+      if (nz) {
+        n[0] = 0; n[1] = 1; n[2] = 0;
+      } else {
+        double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (double& c : n) c = c / l;
+      }
+      double up[3] = {0, 1, 0};
+      if (std::fabs(n[1]) > 0.999) { up[0] = 1; up[1] = 0; }
+      double ua[3] = {up[1] * n[2] - up[2] * n[1], up[2] * n[0] - up[0] * n[2], up[0] * n[1] - up[1] * n[0]};
+      double ul = std::sqrt(ua[0] * ua[0] + ua[1] * ua[1] + ua[2] * ua[2]);
+      for (double& c : ua) c = c / ul;
+      double va[3] = {n[1] * ua[2] - n[2] * ua[1], n[2] * ua[0] - n[0] * ua[2], n[0] * ua[1] - n[1] * ua[0]};
+      for (int k = 0; k < 3; ++k) {
+        v[k] = q[k];
+        v[3 + k] = n[k];
+        v[6 + k] = ua[k];
+        v[9 + k] = va[k];
+      }
+      break;
+    }
+    default:
+      for (int k = 0; k < 12; ++k) v[k] = q[k];
+  }
+  for (int k = 0; k < 13; ++k) r.p[k] = R(v[k]);
+  uint32_t kind = p.kind;
+  uint32_t flags = fp64_island ? FLAG_FP64 : 0;
+  r.kind_mat = kind | flags | (p.material << 8);
+  r.xform = p.transform < 0 ? XF_NONE : uint32_t(p.transform);
+  r.prim_id = prim_id;
+}
+
+// Which primitives the FP32 path evaluates in FP64: the marched kinds, and
+// spheres large enough that FP32 cancellation in |o-c|^2 - r^2 exceeds the
+// hit-distance tolerance (e.g. a r = 1000 ground sphere).
+bool fp64_island(const lmn_prim& p) {
+  if (p.kind == LMN_PRIM_TORUS || p.kind == LMN_PRIM_SUPERELLIPSOID || p.kind == LMN_PRIM_SPRING) return true;
+  if (p.kind == LMN_PRIM_SPHERE) return std::fabs(p.p[3]) >= 128.0;
+  return false;
+}
+
+template <class R>
+void fill_material(const lmn_material& m, MaterialT<R>& o) {
+  std::memset(&o, 0, sizeof o);
+  o.kind = m.kind;
+  o.tex = m.texture;
+  for (int k = 0; k < 3; ++k) o.color[k] = R(m.color[k]);
+  o.fuzz = R(m.fuzz);
+  o.ior = R(m.ior);
+  o.roughness = R(m.roughness);
+  o.dispersion = R(m.dispersion);
+  o.intensity = R(m.intensity);
+  o.metallic = R(m.metallic);
+  o.param_slot = ~0u;
+}
+
+template <class R>
+void fill_light(const lmn_light& l, LightT<R>& o) {
+  std::memset(&o, 0, sizeof o);
+  o.kind = l.kind;
+  o.origin = {R(l.origin[0]), R(l.origin[1]), R(l.origin[2])};
+  o.u = {R(l.u[0]), R(l.u[1]), R(l.u[2])};
+  o.v = {R(l.v[0]), R(l.v[1]), R(l.v[2])};
+  o.normal = {R(l.normal[0]), R(l.normal[1]), R(l.normal[2])};
+  o.emission = {R(l.emission[0]), R(l.emission[1]), R(l.emission[2])};
+  o.radius = R(l.radius);
+  // u.cross(v).length() (src/render.rs:1419), evaluated in FP64
+  double c[3] = {l.u[1] * l.v[2] - l.u[2] * l.v[1], l.u[2] * l.v[0] - l.u[0] * l.v[2], l.u[0] * l.v[1] - l.u[1] * l.v[0]};
+  o.area = R(std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]));
+}
+
+template <class R>
+void fill_bg(const lmn_background& b, BackgroundT<R>& o) {
+  std::memset(&o, 0, sizeof o);
+  o.kind = b.kind;
+  o.c0 = {R(b.color0[0]), R(b.color0[1]), R(b.color0[2])};
+  o.c1 = {R(b.color1[0]), R(b.color1[1]), R(b.color1[2])};
+  double l = std::sqrt(b.direction[0] * b.direction[0] + b.direction[1] * b.direction[1] + b.direction[2] * b.direction[2]);
+  o.dir_unit = {R(b.direction[0] / l), R(b.direction[1] / l), R(b.direction[2] / l)};
+  o.intensity = R(b.intensity);
+}
+
+template <class R>
+void fill_camera(const lmn_camera& c, CameraT<R>& o) {
+  auto v = [](const double* a) { return V3<R>{R(a[0]), R(a[1]), R(a[2])}; };
+  o.origin = v(c.origin);
+  o.lower_left = v(c.lower_left);
+  o.horizontal = v(c.horizontal);
+  o.vertical = v(c.vertical);
+  o.u = v(c.u);
+  o.v = v(c.v);
+  o.lens_radius = R(c.lens_radius);
+}
+
+}  // namespace
+
+int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScene& out, std::string& err) {
+  // validate references
+  for (uint32_t i = 0; i < desc.n_prims; ++i) {
+    const lmn_prim& p = desc.prims[i];
+    if (p.kind >= LMN_PRIM_KIND_COUNT) {
+      err = "prim " + std::to_string(i) + ": unknown kind";
+      return LMN_ERR_INVALID;
+    }
+    if (p.material >= desc.n_materials) {
+      err = "prim " + std::to_string(i) + ": material index out of range";
+      return LMN_ERR_INVALID;
+    }
+    if (p.transform >= 0 && uint32_t(p.transform) >= desc.n_transforms) {
+      err = "prim " + std::to_string(i) + ": transform index out of range";
+      return LMN_ERR_INVALID;
+    }
+    if (p.material >= (1u << 24)) {
+      err = "too many materials";
+      return LMN_ERR_INVALID;
+    }
+  }
+  for (uint32_t i = 0; i < desc.n_materials; ++i) {
+    const lmn_material& m = desc.materials[i];
+    if (m.kind > LMN_MAT_MICROFACET) {
+      err = "material " + std::to_string(i) + ": unknown kind";
+      return LMN_ERR_INVALID;
+    }
+    if (m.texture >= 0 && uint32_t(m.texture) >= desc.n_textures) {
+      err = "material " + std::to_string(i) + ": texture index out of range";
+      return LMN_ERR_INVALID;
+    }
+  }
+  for (uint32_t i = 0; i < desc.n_textures; ++i) {
+    if (desc.textures[i].width == 0 || desc.textures[i].height == 0 || !desc.textures[i].rgb) {
+      err = "texture " + std::to_string(i) + ": empty";
+      return LMN_ERR_INVALID;
+    }
+  }
+  for (uint32_t i = 0; i < desc.n_lights; ++i)
+    if (desc.lights[i].kind > LMN_LIGHT_DISK) {
+      err = "light " + std::to_string(i) + ": unknown kind";
+      return LMN_ERR_INVALID;
+    }
+
+  std::vector<uint32_t> bounded, planes;
+  for (uint32_t i = 0; i < desc.n_prims; ++i) (desc.prims[i].kind == LMN_PRIM_PLANE ? planes : bounded).push_back(i);
+  if (desc.n_prims >= LEAF_BIT) {
+    err = "too many primitives";
+    return LMN_ERR_INVALID;
+  }
+  std::vector<double> blo(bounded.size() * 3), bhi(bounded.size() * 3);
+  Builder B(blo, bhi);
+  B.opt = opt;
+  B.refs.resize(bounded.size());
+  for (size_t i = 0; i < bounded.size(); ++i) {
+    prim_world_bounds(desc, desc.prims[bounded[i]], &blo[i * 3], &bhi[i * 3]);
+    for (int k = 0; k < 3; ++k) B.refs[i].c[k] = float(0.5 * (blo[i * 3 + k] + bhi[i * 3 + k]));
+    B.refs[i].idx = uint32_t(i);
+  }
+  if (!bounded.empty()) B.build(0, bounded.size(), 0);
+  out.tree_depth = B.depth_max;
+  out.n_leaves = B.leaves;
+  out.nodesD = B.nodes;
+  out.nodesF.resize(B.nodes.size());
+  for (size_t i = 0; i < B.nodes.size(); ++i) {
+    const NodeD& d = B.nodes[i];
+    NodeF& f = out.nodesF[i];
+    for (int k = 0; k < 3; ++k) {
+      // conservative FP32 boxes: outward rounding plus a few ulps for the
+      // FP32 rounding of the primitive parameters themselves
+      double pad = 4.0 * FLT_EPSILON * (std::fabs(d.lo[k]) + std::fabs(d.hi[k]));
+      f.lo[k] = round_down(d.lo[k] - pad);
+      f.hi[k] = round_up(d.hi[k] + pad);
+    }
+    f.a = d.a;
+    f.b = d.b;
+  }
+  out.n_bounded = uint32_t(bounded.size());
+  out.n_planes = uint32_t(planes.size());
+  out.order.clear();
+  for (uint32_t r : B.order) out.order.push_back(bounded[r]);
+  for (uint32_t p : planes) out.order.push_back(p);
+  size_t nrec = out.order.size();
+  out.primsF.resize(nrec);
+  out.primsD.resize(nrec);
+  for (size_t i = 0; i < nrec; ++i) {
+    const lmn_prim& p = desc.prims[out.order[i]];
+    bool isl = fp64_island(p);
+    fill_record<float>(p, nullptr, out.primsF[i], out.order[i], isl);
+    fill_record<double>(p, nullptr, out.primsD[i], out.order[i], false);
+  }
+  out.xfD.resize(desc.n_transforms);
+  out.xfF.resize(desc.n_transforms);
+  for (uint32_t i = 0; i < desc.n_transforms; ++i) {
+    double fwd[12];
+    compose_transform(desc.transforms[i], out.xfD[i], fwd);
+    XformT<float>& f = out.xfF[i];
+    for (int k = 0; k < 12; ++k) f.inv[k] = float(out.xfD[i].inv[k]);
+    for (int k = 0; k < 9; ++k) f.nrm[k] = float(out.xfD[i].nrm[k]);
+    f.tscale = float(out.xfD[i].tscale);
+    f.tmode = out.xfD[i].tmode;
+    f.renorm = out.xfD[i].renorm;
+  }
+  out.matF.resize(desc.n_materials);
+  out.matD.resize(desc.n_materials);
+  for (uint32_t i = 0; i < desc.n_materials; ++i) {
+    fill_material(desc.materials[i], out.matF[i]);
+    fill_material(desc.materials[i], out.matD[i]);
+  }
+  out.tex.resize(desc.n_textures);
+  out.texels.clear();
+  for (uint32_t i = 0; i < desc.n_textures; ++i) {
+    const lmn_texture& t = desc.textures[i];
+    out.tex[i].w = t.width;
+    out.tex[i].h = t.height;
+    out.tex[i].offset = out.texels.size();
+    out.texels.insert(out.texels.end(), t.rgb, t.rgb + size_t(t.width) * t.height * 3);
+  }
+  if (out.texels.empty()) out.texels.push_back(0.f);
+  out.lightF.resize(desc.n_lights);
+  out.lightD.resize(desc.n_lights);
+  for (uint32_t i = 0; i < desc.n_lights; ++i) {
+    fill_light(desc.lights[i], out.lightF[i]);
+    fill_light(desc.lights[i], out.lightD[i]);
+  }
+  fill_bg(desc.config.background, out.bgF);
+  fill_bg(desc.config.background, out.bgD);
+  fill_camera(desc.camera, out.camF);
+  fill_camera(desc.camera, out.camD);
+  return LMN_OK;
+}
+
+}  // namespace lmn

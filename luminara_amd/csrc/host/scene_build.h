@@ -1,0 +1,52 @@
+// Host-side flattening of an lmn_scene_desc into the device layout (see
+// lmn_geom.h): SAH BVH over the bounded primitives (BvhNode::build,
+// src/bvh.rs:16-163, re-designed for a flat GPU tree), leaf-ordered primitive
+// records in FP32 and FP64, composed instance transforms, materials, lights,
+// textures, camera and background.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../../include/luminara.h"
+#include "../lmn_render.h"
+
+namespace lmn {
+
+struct BvhOptions {
+  int bins = 16;        // SAH bins per axis
+  int max_leaf = 4;     // primitives per leaf
+  int max_depth = 48;   // hard depth cap (traversal stacks are sized for it)
+};
+
+struct HostScene {
+  std::vector<NodeF> nodesF;
+  std::vector<NodeD> nodesD;
+  std::vector<PrimT<float>> primsF;   // n_bounded in leaf order, then planes
+  std::vector<PrimT<double>> primsD;
+  std::vector<XformT<float>> xfF;
+  std::vector<XformT<double>> xfD;
+  std::vector<MaterialT<float>> matF;
+  std::vector<MaterialT<double>> matD;
+  std::vector<TextureDev> tex;
+  std::vector<float> texels;
+  std::vector<LightT<float>> lightF;
+  std::vector<LightT<double>> lightD;
+  BackgroundT<float> bgF;
+  BackgroundT<double> bgD;
+  CameraT<float> camF;
+  CameraT<double> camD;
+  uint32_t n_bounded = 0, n_planes = 0;
+  uint32_t tree_depth = 0, n_leaves = 0;
+  std::vector<uint32_t> order;  // order[i] = descriptor prim index of record i
+};
+
+// Returns LMN_OK or an error code with a message.
+int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScene& out, std::string& err);
+
+// World-space bounds of a descriptor primitive (transform applied).
+void prim_world_bounds(const lmn_scene_desc& desc, const lmn_prim& p, double lo[3], double hi[3]);
+
+// Compose a descriptor transform into the device representation.
+void compose_transform(const lmn_transform& t, XformT<double>& x, double fwd[12]);
+
+}  // namespace lmn

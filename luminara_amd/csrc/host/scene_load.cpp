@@ -1,0 +1,865 @@
+// TOML scene -> lmn_scene_desc.  Mirrors scene::load_scene
+// (src/scene.rs:846-1645): [render] settings over RenderConfig::default()
+// (src/render.rs:422-521), the camera built from the TOML width/height aspect
+// (src/scene.rs:1146-1174, src/camera.rs:48-76), objects added grouped by type
+// in the reference's fixed order with NEE lights registered alongside
+// (src/scene.rs:1180-1642), materials from build_material (src/scene.rs:1709).
+// Scene features that are valid in the reference but not part of the render
+// hot path (procedural textures, exotic materials and primitives) are
+// rejected with LMN_ERR_UNSUPPORTED naming the feature.
+#include "scene_load.h"
+
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "toml.h"
+
+namespace lmn {
+
+void OwnedDesc::sync() {
+  textures.resize(texels.size());
+  for (size_t i = 0; i < texels.size(); ++i) textures[i].rgb = texels[i].data();
+  desc.n_prims = uint32_t(prims.size());
+  desc.n_transforms = uint32_t(transforms.size());
+  desc.n_materials = uint32_t(materials.size());
+  desc.n_textures = uint32_t(textures.size());
+  desc.n_lights = uint32_t(lights.size());
+  desc.n_objects = n_objects;
+  desc.prims = prims.data();
+  desc.transforms = transforms.data();
+  desc.materials = materials.data();
+  desc.textures = textures.data();
+  desc.lights = lights.data();
+}
+
+namespace {
+
+struct LoadError {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw LoadError{code, msg}; }
+
+const char* kind_name(TomlValue::Kind k) {
+  switch (k) {
+    case TomlValue::STRING: return "string";
+    case TomlValue::INTEGER: return "integer";
+    case TomlValue::FLOAT: return "float";
+    case TomlValue::BOOLEAN: return "boolean";
+    case TomlValue::ARRAY: return "array";
+    default: return "table";
+  }
+}
+
+// serde-style typed accessors: a missing required field or a wrong type is a
+// parse error, as with the reference's derive(Deserialize) structs.
+double num(const TomlValue& v, const std::string& ctx) {
+  if (!v.is_number()) fail(LMN_ERR_PARSE, ctx + ": invalid type " + kind_name(v.kind) + ", expected f64");
+  return v.as_double();
+}
+uint64_t uint_field(const TomlValue& v, const std::string& ctx, uint64_t max) {
+  if (v.kind != TomlValue::INTEGER) fail(LMN_ERR_PARSE, ctx + ": invalid type " + kind_name(v.kind) + ", expected integer");
+  if (v.integer < 0 || uint64_t(v.integer) > max) fail(LMN_ERR_PARSE, ctx + ": integer out of range");
+  return uint64_t(v.integer);
+}
+void vec3(const TomlValue& v, const std::string& ctx, double out[3]) {
+  if (v.kind != TomlValue::ARRAY) fail(LMN_ERR_PARSE, ctx + ": expected an array of 3 numbers");
+  if (v.array.size() != 3) fail(LMN_ERR_PARSE, ctx + ": invalid length " + std::to_string(v.array.size()) + ", expected 3");
+  for (int k = 0; k < 3; ++k) out[k] = num(v.array[k], ctx);
+}
+void vec2(const TomlValue& v, const std::string& ctx, double out[2]) {
+  if (v.kind != TomlValue::ARRAY || v.array.size() != 2) fail(LMN_ERR_PARSE, ctx + ": expected an array of 2 numbers");
+  for (int k = 0; k < 2; ++k) out[k] = num(v.array[k], ctx);
+}
+const TomlValue& req(const TomlValue& t, const std::string& key, const std::string& ctx) {
+  const TomlValue* v = t.get(key);
+  if (!v) fail(LMN_ERR_PARSE, ctx + ": missing field `" + key + "`");
+  return *v;
+}
+bool opt_num(const TomlValue& t, const std::string& key, const std::string& ctx, double& out) {
+  const TomlValue* v = t.get(key);
+  if (!v) return false;
+  out = num(*v, ctx + "." + key);
+  return true;
+}
+bool opt_vec3(const TomlValue& t, const std::string& key, const std::string& ctx, double out[3]) {
+  const TomlValue* v = t.get(key);
+  if (!v) return false;
+  vec3(*v, ctx + "." + key, out);
+  return true;
+}
+double req_num(const TomlValue& t, const std::string& key, const std::string& ctx) {
+  return num(req(t, key, ctx), ctx + "." + key);
+}
+void req_vec3(const TomlValue& t, const std::string& key, const std::string& ctx, double out[3]) {
+  vec3(req(t, key, ctx), ctx + "." + key, out);
+}
+std::string opt_str(const TomlValue& t, const std::string& key, const std::string& ctx, bool& present) {
+  const TomlValue* v = t.get(key);
+  present = v != nullptr;
+  if (!v) return "";
+  if (v->kind != TomlValue::STRING) fail(LMN_ERR_PARSE, ctx + "." + key + ": expected a string");
+  return v->str;
+}
+
+const std::vector<TomlValue>& table_array(const TomlValue& root, const std::string& key) {
+  static const std::vector<TomlValue> empty;
+  const TomlValue* v = root.get(key);
+  if (!v) return empty;
+  if (v->kind != TomlValue::ARRAY) fail(LMN_ERR_PARSE, "`" + key + "`: expected an array of tables");
+  for (auto& e : v->array)
+    if (e.kind != TomlValue::TABLE) fail(LMN_ERR_PARSE, "`" + key + "`: expected an array of tables");
+  return v->array;
+}
+
+// Tanner Helland's blackbody approximation, as the reference's
+// blackbody_to_rgb (src/scene.rs:1677-1707) evaluates it.
+void blackbody_rgb(double temp_k, double out[3]) {
+  double temp = std::fmin(std::fmax(temp_k, 1000.0), 40000.0) / 100.0;
+  auto clamp = [](double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); };
+  double r, g, b;
+  if (temp <= 66.0) {
+    r = 1.0;
+  } else {
+    double t = temp - 60.0;
+    r = clamp(329.698727446 * std::pow(t, -0.1332047592) / 255.0, 0.0, 1.0);
+  }
+  if (temp <= 66.0) {
+    g = clamp(99.4708025861 * std::log(temp) - 161.1195681661, 0.0, 255.0) / 255.0;
+  } else {
+    double t = temp - 60.0;
+    g = clamp(288.1221695283 * std::pow(t, -0.0755148492) / 255.0, 0.0, 1.0);
+  }
+  if (temp >= 66.0) {
+    b = 1.0;
+  } else if (temp <= 19.0) {
+    b = 0.0;
+  } else {
+    double t = temp - 10.0;
+    b = clamp(138.5177312231 * std::log(t) - 305.0447927307, 0.0, 255.0) / 255.0;
+  }
+  out[0] = r;
+  out[1] = g;
+  out[2] = b;
+}
+
+// Material tag names: serde accepts the variant identifier and its aliases
+// (src/scene.rs:512-772).
+std::string material_tag(const TomlValue& m, const std::string& ctx) {
+  if (m.kind != TomlValue::TABLE) fail(LMN_ERR_PARSE, ctx + ": expected a material table");
+  const TomlValue* t = m.get("type");
+  if (!t) fail(LMN_ERR_PARSE, ctx + ": missing field `type`");
+  if (t->kind != TomlValue::STRING) fail(LMN_ERR_PARSE, ctx + ".type: expected a string");
+  return t->str;
+}
+
+struct MatInfo {
+  bool emissive = false;  // Emissive or Blackbody: a light for NEE where registered
+  double emission[3] = {0, 0, 0};
+};
+
+class Loader {
+ public:
+  Loader(const std::string& base_dir, OwnedDesc& out) : base_(base_dir), out_(out) {}
+
+  void run(const std::string& text) {
+    TomlValue root;
+    std::string err;
+    if (!toml_parse(text, root, err)) fail(LMN_ERR_PARSE, "TOML parse error: " + err);
+    if (root.kind != TomlValue::TABLE) fail(LMN_ERR_PARSE, "TOML parse error: expected a table");
+    render_settings(root);
+    camera(root);
+    objects(root);
+    out_.sync();
+  }
+
+ private:
+  std::string base_;
+  OwnedDesc& out_;
+
+  void render_settings(const TomlValue& root) {
+    lmn_render_config& c = out_.desc.config;
+    // RenderConfig::default() (src/render.rs:422-521)
+    c.width = 800;
+    c.height = 450;
+    c.samples = 100;
+    c.max_depth = 50;
+    c.seed = 31337;
+    c.filter = LMN_FILTER_BOX;
+    c.has_crop = 0;
+    c.exposure = 1.0;
+    c.tone_map = 0;
+    c.background = lmn_background{};
+    c.background.kind = LMN_BG_SKY;
+    const TomlValue* r = root.get("render");
+    if (!r) return;
+    if (r->kind != TomlValue::TABLE) fail(LMN_ERR_PARSE, "render: expected a table");
+    const std::string ctx = "render";
+    if (auto v = r->get("width")) c.width = uint32_t(uint_field(*v, ctx + ".width", 0xffffffffu));
+    if (auto v = r->get("height")) c.height = uint32_t(uint_field(*v, ctx + ".height", 0xffffffffu));
+    if (auto v = r->get("samples")) c.samples = uint32_t(uint_field(*v, ctx + ".samples", 0xffffffffu));
+    if (auto v = r->get("max_depth")) c.max_depth = uint32_t(uint_field(*v, ctx + ".max_depth", 0xffffffffu));
+    if (auto v = r->get("seed")) c.seed = uint_field(*v, ctx + ".seed", ~0ull);
+    opt_num(*r, "exposure", ctx, c.exposure);
+    bool present = false;
+    std::string pf = opt_str(*r, "pixel_filter", ctx, present);
+    if (present) {  // src/scene.rs:906-913
+      if (pf == "triangle" || pf == "tent")
+        c.filter = LMN_FILTER_TRIANGLE;
+      else if (pf == "gaussian" || pf == "gauss")
+        c.filter = LMN_FILTER_GAUSSIAN;
+      else if (pf == "mitchell")
+        c.filter = LMN_FILTER_MITCHELL;
+      else
+        c.filter = LMN_FILTER_BOX;
+    }
+    std::string tm = opt_str(*r, "tone_map", ctx, present);
+    if (present) {  // src/scene.rs:1100-1107
+      if (tm == "reinhard")
+        c.tone_map = 1;
+      else if (tm == "filmic" || tm == "uncharted2")
+        c.tone_map = 2;
+      else if (tm == "none")
+        c.tone_map = 3;
+      else
+        c.tone_map = 0;
+    }
+    for (const char* k : {"adaptive", "time_limit"}) {
+      const TomlValue* v = r->get(k);
+      if (!v) continue;
+      bool on = (v->kind == TomlValue::BOOLEAN && v->boolean) || (v->is_number() && v->as_double() > 0.0);
+      if (on) fail(LMN_ERR_UNSUPPORTED, std::string("render.") + k + " (adaptive / time-limited sampling) is not on the GPU path");
+    }
+    if (const TomlValue* bg = r->get("background")) background(*bg, c.background);
+  }
+
+  void background(const TomlValue& bg, lmn_background& b) {
+    // src/scene.rs:1108-1143
+    const std::string ctx = "render.background";
+    std::string tag = material_tag(bg, ctx);
+    b = lmn_background{};
+    if (tag == "Sky" || tag == "sky") {
+      b.kind = LMN_BG_SKY;
+    } else if (tag == "Solid" || tag == "solid") {
+      b.kind = LMN_BG_SOLID;
+      req_vec3(bg, "color", ctx, b.color0);
+    } else if (tag == "Gradient" || tag == "gradient") {
+      b.kind = LMN_BG_GRADIENT;
+      req_vec3(bg, "bottom", ctx, b.color0);
+      req_vec3(bg, "top", ctx, b.color1);
+    } else if (tag == "Black" || tag == "black") {
+      b.kind = LMN_BG_SOLID;
+    } else if (tag == "Sunset" || tag == "sunset") {
+      b.kind = LMN_BG_SUN;
+      const double d[3] = {0.3, 0.15, -1.0}, sc[3] = {1.0, 0.6, 0.2}, sky[3] = {0.8, 0.4, 0.2};
+      std::memcpy(b.direction, d, sizeof d);
+      std::memcpy(b.color0, sc, sizeof sc);
+      std::memcpy(b.color1, sky, sizeof sky);
+      b.intensity = 30.0;
+    } else if (tag == "Sun" || tag == "sun") {
+      b.kind = LMN_BG_SUN;
+      req_vec3(bg, "direction", ctx, b.direction);
+      const double sc[3] = {1.0, 0.95, 0.85}, sky[3] = {0.5, 0.7, 1.0};
+      if (!opt_vec3(bg, "sun_color", ctx, b.color0)) std::memcpy(b.color0, sc, sizeof sc);
+      b.intensity = 20.0;
+      opt_num(bg, "intensity", ctx, b.intensity);
+      if (!opt_vec3(bg, "sky_color", ctx, b.color1)) std::memcpy(b.color1, sky, sizeof sky);
+    } else if (tag == "Starfield" || tag == "starfield" || tag == "EnvMap" || tag == "env_map") {
+      fail(LMN_ERR_UNSUPPORTED, "background type `" + tag + "` is not on the GPU path");
+    } else {
+      fail(LMN_ERR_PARSE, ctx + ": unknown variant `" + tag + "`");
+    }
+  }
+
+  void camera(const TomlValue& root) {
+    // CameraConfig::default() with aspect from the TOML width/height
+    // (src/camera.rs:32-45, src/scene.rs:1147-1150)
+    double look_from[3] = {0, 1, 3}, look_at[3] = {0, 0, 0}, vup[3] = {0, 1, 0};
+    double vfov = 40.0, aperture = 0.0, focus = 1.0;
+    double aspect = double(out_.desc.config.width) / double(out_.desc.config.height);
+    if (const TomlValue* c = root.get("camera")) {
+      if (c->kind != TomlValue::TABLE) fail(LMN_ERR_PARSE, "camera: expected a table");
+      opt_vec3(*c, "look_from", "camera", look_from);
+      opt_vec3(*c, "look_at", "camera", look_at);
+      opt_vec3(*c, "vup", "camera", vup);
+      opt_num(*c, "vfov", "camera", vfov);
+      opt_num(*c, "aperture", "camera", aperture);
+      opt_num(*c, "focus_dist", "camera", focus);
+      if (const TomlValue* p = c->get("panorama")) {
+        if (p->kind != TomlValue::BOOLEAN) fail(LMN_ERR_PARSE, "camera.panorama: expected a boolean");
+        if (p->boolean) fail(LMN_ERR_UNSUPPORTED, "camera.panorama is not on the GPU path");
+      }
+    }
+    // Camera::new (src/camera.rs:48-76)
+    const double rads_per_deg = M_PI / 180.0;
+    double theta = vfov * rads_per_deg;
+    double h = std::tan(theta / 2.0);
+    double vh = 2.0 * h, vw = aspect * vh;
+    double w[3], u[3], v[3];
+    for (int k = 0; k < 3; ++k) w[k] = look_from[k] - look_at[k];
+    double wl = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int k = 0; k < 3; ++k) w[k] = w[k] / wl;
+    double cx = vup[1] * w[2] - vup[2] * w[1], cy = vup[2] * w[0] - vup[0] * w[2], cz = vup[0] * w[1] - vup[1] * w[0];
+    double cl = std::sqrt(cx * cx + cy * cy + cz * cz);
+    u[0] = cx / cl;
+    u[1] = cy / cl;
+    u[2] = cz / cl;
+    v[0] = w[1] * u[2] - w[2] * u[1];
+    v[1] = w[2] * u[0] - w[0] * u[2];
+    v[2] = w[0] * u[1] - w[1] * u[0];
+    lmn_camera& cam = out_.desc.camera;
+    for (int k = 0; k < 3; ++k) {
+      cam.origin[k] = look_from[k];
+      cam.horizontal[k] = u[k] * vw * focus;
+      cam.vertical[k] = v[k] * vh * focus;
+      cam.lower_left[k] = look_from[k] - cam.horizontal[k] / 2.0 - cam.vertical[k] / 2.0 - w[k] * focus;
+      cam.u[k] = u[k];
+      cam.v[k] = v[k];
+    }
+    cam.lens_radius = aperture / 2.0;
+  }
+
+  // build_material (src/scene.rs:1709-2048) for the hot-path subset.
+  uint32_t material(const TomlValue& m, const std::string& ctx, MatInfo* info = nullptr) {
+    std::string tag = material_tag(m, ctx);
+    lmn_material mat{};
+    mat.texture = -1;
+    mat.intensity = 1.0;
+    mat.ior = 1.5;
+    MatInfo mi;
+    if (tag == "Lambertian" || tag == "lambertian" || tag == "matte") {
+      mat.kind = LMN_MAT_LAMBERTIAN;
+      req_vec3(m, "color", ctx, mat.color);
+    } else if (tag == "Metal" || tag == "metal") {
+      mat.kind = LMN_MAT_METAL;
+      req_vec3(m, "color", ctx, mat.color);
+      double f = 0.0;
+      opt_num(m, "fuzz", ctx, f);
+      mat.fuzz = std::fmin(f, 1.0);  // Metal::new clamps fuzz to <= 1 (src/material.rs:81-86)
+    } else if (tag == "Dielectric" || tag == "dielectric") {
+      mat.kind = LMN_MAT_DIELECTRIC;
+      mat.ior = req_num(m, "refraction_index", ctx);
+      mat.color[0] = mat.color[1] = mat.color[2] = 1.0;
+      opt_vec3(m, "tint", ctx, mat.color);
+      opt_num(m, "roughness", ctx, mat.roughness);
+      opt_num(m, "dispersion", ctx, mat.dispersion);
+      if (!(mat.dispersion > 0.0)) mat.dispersion = 0.0;  // only d > 0 builds a dispersive glass
+    } else if (tag == "Emissive" || tag == "emissive") {
+      mat.kind = LMN_MAT_EMISSIVE;
+      req_vec3(m, "color", ctx, mat.color);
+      opt_num(m, "intensity", ctx, mat.intensity);
+      bool has_tex = false;
+      std::string file = opt_str(m, "texture", ctx, has_tex);
+      if (has_tex) texture_file(file, ctx, mat);
+      mi.emissive = true;
+      for (int k = 0; k < 3; ++k) mi.emission[k] = mat.color[k] * mat.intensity;
+    } else if (tag == "Blackbody" || tag == "blackbody") {
+      mat.kind = LMN_MAT_EMISSIVE;
+      blackbody_rgb(req_num(m, "temperature", ctx), mat.color);
+      opt_num(m, "intensity", ctx, mat.intensity);
+      mi.emissive = true;
+      for (int k = 0; k < 3; ++k) mi.emission[k] = mat.color[k] * mat.intensity;
+    } else if (tag == "Microfacet" || tag == "microfacet" || tag == "pbr") {
+      mat.kind = LMN_MAT_MICROFACET;
+      req_vec3(m, "color", ctx, mat.color);
+      double r = 0.5, mt = 0.0;
+      opt_num(m, "roughness", ctx, r);
+      opt_num(m, "metallic", ctx, mt);
+      // Microfacet::new clamps (src/material.rs:255-261)
+      mat.roughness = std::fmin(std::fmax(r, 0.01), 1.0);
+      mat.metallic = std::fmin(std::fmax(mt, 0.0), 1.0);
+      bool has_tex = false;
+      std::string file = opt_str(m, "texture", ctx, has_tex);
+      if (has_tex) texture_file(file, ctx, mat);
+    } else if (tag == "Mirror" || tag == "mirror") {
+      mat.kind = LMN_MAT_METAL;
+      mat.color[0] = mat.color[1] = mat.color[2] = 0.95;
+      mat.fuzz = 0.0;
+    } else if (tag == "Glass" || tag == "glass") {
+      mat.kind = LMN_MAT_DIELECTRIC;
+      mat.ior = 1.5;
+      mat.color[0] = mat.color[1] = mat.color[2] = 1.0;
+    } else {
+      static const char* known[] = {
+          "Checker", "checker", "Marble", "marble", "Turbulence", "turbulence", "Image", "image", "Wood", "wood",
+          "Rings", "rings", "Toon", "toon", "cel", "Anisotropic", "anisotropic", "brushed", "Clearcoat",
+          "clearcoat", "Velvet", "velvet", "Iridescent", "iridescent", "Translucent", "translucent",
+          "Subsurface", "subsurface", "sss", "BlendMat", "blend", "Opacity", "opacity", "alpha", "Dots", "dots",
+          "Grid", "grid", "UvChecker", "uv_checker", "GradientTex", "gradient_tex", "Stripe", "stripe",
+          "Voronoi", "voronoi", "Noise", "noise", "Spiral", "spiral", "Hexgrid", "hexgrid", "Plasma", "plasma",
+          "Terrain", "terrain", "earth", "Rust", "rust", "patina", "oxidized", "Brick", "brick", "bricks", "Camo",
+          "camo", "camouflage", "Lava", "lava", "magma", "Cloud", "cloud", "clouds", "TriPlanar", "tri_planar",
+          "triplanar", "MixColor", "mix_color", "Wavy", "wavy", "wave", "Fbm", "fbm", "fractal", "ColorRamp",
+          "color_ramp", "ramp"};
+      for (const char* k : known)
+        if (tag == k) fail(LMN_ERR_UNSUPPORTED, ctx + ": material `" + tag + "` is not on the GPU path");
+      fail(LMN_ERR_PARSE, ctx + ": unknown variant `" + tag + "`");
+    }
+    if (info) *info = mi;
+    out_.materials.push_back(mat);
+    return uint32_t(out_.materials.size() - 1);
+  }
+
+  // Emissive/PBR `texture = "file"`: the reference falls back to the solid
+  // colour with a warning when the image cannot be loaded
+  // (src/scene.rs:1732-1739, :1750-1757); decoding image files is not part of
+  // this library, so only the fallback case is accepted.
+  void texture_file(const std::string& file, const std::string& ctx, lmn_material&) {
+    std::ifstream f(resolve(file));
+    if (f.good()) fail(LMN_ERR_UNSUPPORTED, ctx + ": image texture files are not decoded by this library ('" + file + "')");
+    fprintf(stderr, "Warning: failed to load texture '%s', using solid color\n", file.c_str());
+  }
+
+  std::string resolve(const std::string& path) const {
+    if (base_.empty() || (!path.empty() && path[0] == '/')) return path;
+    return base_ + "/" + path;
+  }
+
+  uint32_t add_transform(const std::vector<lmn_xform_op>& ops) {
+    lmn_transform t{};
+    t.n_ops = uint32_t(ops.size());
+    for (size_t k = 0; k < ops.size(); ++k) t.ops[k] = ops[k];
+    out_.transforms.push_back(t);
+    return uint32_t(out_.transforms.size() - 1);
+  }
+
+  lmn_prim& add_prim(uint32_t kind, uint32_t mat, int32_t xf, uint32_t object) {
+    lmn_prim p{};
+    p.kind = kind;
+    p.material = mat;
+    p.transform = xf;
+    p.object = object;
+    out_.prims.push_back(p);
+    return out_.prims.back();
+  }
+  uint32_t new_object() { return out_.n_objects++; }
+
+  static lmn_xform_op op(uint32_t kind, double a, double b = 0, double c = 0) {
+    lmn_xform_op o{};
+    o.kind = kind;
+    o.v[0] = a;
+    o.v[1] = b;
+    o.v[2] = c;
+    return o;
+  }
+
+  void push_light_sphere(const double c[3], double r, const double e[3]) {
+    lmn_light l{};
+    l.kind = LMN_LIGHT_SPHERE;
+    for (int k = 0; k < 3; ++k) {
+      l.origin[k] = c[k];
+      l.emission[k] = e[k];
+    }
+    l.radius = r;
+    out_.lights.push_back(l);
+  }
+
+  void reject_if_present(const TomlValue& root, const char* key, const char* why) {
+    if (!table_array(root, key).empty())
+      fail(LMN_ERR_UNSUPPORTED, std::string("[[") + key + "]] " + why + " is not on the GPU path");
+  }
+
+  void objects(const TomlValue& root) {
+    // Fixed type order of load_scene (src/scene.rs:1180-1642).
+    for (auto& s : table_array(root, "sphere")) {
+      const std::string ctx = "sphere";
+      double c[3];
+      req_vec3(s, "center", ctx, c);
+      double r = req_num(s, "radius", ctx);
+      MatInfo mi;
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material", &mi);
+      if (mi.emissive) push_light_sphere(c, r, mi.emission);  // src/scene.rs:1182-1202
+      if (s.get("bump_strength") || s.get("normal_map"))
+        fail(LMN_ERR_UNSUPPORTED, "sphere bump/normal maps are not on the GPU path");
+      std::vector<lmn_xform_op> ops;  // src/scene.rs:1216-1230
+      double a;
+      if (opt_num(s, "rotate_x", ctx, a)) ops.push_back(op(LMN_XF_ROTATE_X, a));
+      if (opt_num(s, "rotate_y", ctx, a)) ops.push_back(op(LMN_XF_ROTATE_Y, a));
+      if (opt_num(s, "rotate_z", ctx, a)) ops.push_back(op(LMN_XF_ROTATE_Z, a));
+      double t[3];
+      if (opt_vec3(s, "translate", ctx, t)) ops.push_back(op(LMN_XF_TRANSLATE, t[0], t[1], t[2]));
+      if (opt_num(s, "scale", ctx, a)) ops.push_back(op(LMN_XF_SCALE, a));
+      int32_t xf = ops.empty() ? -1 : int32_t(add_transform(ops));
+      lmn_prim& p = add_prim(LMN_PRIM_SPHERE, mat, xf, new_object());
+      p.p[0] = c[0];
+      p.p[1] = c[1];
+      p.p[2] = c[2];
+      p.p[3] = r;
+    }
+    reject_if_present(root, "moving_sphere", "(motion blur)");
+    for (auto& s : table_array(root, "plane")) {
+      const std::string ctx = "plane";
+      double pt[3], n[3];
+      req_vec3(s, "point", ctx, pt);
+      req_vec3(s, "normal", ctx, n);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      lmn_prim& p = add_prim(LMN_PRIM_PLANE, mat, -1, new_object());
+      for (int k = 0; k < 3; ++k) {
+        p.p[k] = pt[k];
+        p.p[3 + k] = n[k];
+      }
+    }
+    for (auto& s : table_array(root, "torus")) {
+      const std::string ctx = "torus";
+      double c[3];
+      req_vec3(s, "center", ctx, c);
+      double R = req_num(s, "major_radius", ctx), r = req_num(s, "minor_radius", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      if (s.get("bump_strength")) fail(LMN_ERR_UNSUPPORTED, "torus bump maps are not on the GPU path");
+      lmn_prim& p = add_prim(LMN_PRIM_TORUS, mat, -1, new_object());
+      p.p[0] = c[0];
+      p.p[1] = c[1];
+      p.p[2] = c[2];
+      p.p[3] = R;
+      p.p[4] = r;
+    }
+    reject_if_present(root, "rounded_box", "(SDF rounded box)");
+    reject_if_present(root, "mobius", "(SDF Mobius strip)");
+    for (auto& s : table_array(root, "spring")) {
+      const std::string ctx = "spring";
+      double c[3];
+      req_vec3(s, "center", ctx, c);
+      double cr = req_num(s, "coil_radius", ctx), tr = req_num(s, "tube_radius", ctx);
+      double pitch = req_num(s, "pitch", ctx), turns = req_num(s, "turns", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      lmn_prim& p = add_prim(LMN_PRIM_SPRING, mat, -1, new_object());
+      p.p[0] = c[0];
+      p.p[1] = c[1];
+      p.p[2] = c[2];
+      p.p[3] = cr;
+      p.p[4] = tr;
+      p.p[5] = pitch;
+      p.p[6] = turns;
+    }
+    for (auto& s : table_array(root, "superellipsoid")) {
+      const std::string ctx = "superellipsoid";
+      double c[3], sc[3];
+      req_vec3(s, "center", ctx, c);
+      req_vec3(s, "scale", ctx, sc);
+      double e1 = req_num(s, "e1", ctx), e2 = req_num(s, "e2", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      lmn_prim& p = add_prim(LMN_PRIM_SUPERELLIPSOID, mat, -1, new_object());
+      for (int k = 0; k < 3; ++k) {
+        p.p[k] = c[k];
+        p.p[3 + k] = sc[k];
+      }
+      p.p[6] = e1;
+      p.p[7] = e2;
+    }
+    reject_if_present(root, "capsule", "(capsule)");
+    reject_if_present(root, "ellipsoid", "(ellipsoid)");
+    for (auto& s : table_array(root, "light")) {  // src/scene.rs:1329-1344
+      const std::string ctx = "light";
+      double pos[3], col[3] = {1, 1, 1};
+      req_vec3(s, "position", ctx, pos);
+      opt_vec3(s, "color", ctx, col);
+      double intensity = 10.0, radius = 0.1;
+      opt_num(s, "intensity", ctx, intensity);
+      opt_num(s, "radius", ctx, radius);
+      lmn_material mat{};
+      mat.kind = LMN_MAT_EMISSIVE;
+      mat.texture = -1;
+      mat.intensity = intensity;
+      for (int k = 0; k < 3; ++k) mat.color[k] = col[k];
+      out_.materials.push_back(mat);
+      lmn_prim& p = add_prim(LMN_PRIM_SPHERE, uint32_t(out_.materials.size() - 1), -1, new_object());
+      for (int k = 0; k < 3; ++k) p.p[k] = pos[k];
+      p.p[3] = radius;
+      double e[3] = {col[0] * intensity, col[1] * intensity, col[2] * intensity};
+      push_light_sphere(pos, radius, e);
+    }
+    reject_if_present(root, "hemisphere", "(hemisphere)");
+    reject_if_present(root, "fog", "(constant medium)");
+    reject_if_present(root, "cone", "(cone)");
+    reject_if_present(root, "annulus", "(annulus)");
+    reject_if_present(root, "disk", "(disk)");
+    reject_if_present(root, "cylinder", "(cylinder)");
+    reject_if_present(root, "prism", "(prism)");
+    reject_if_present(root, "wedge", "(wedge)");
+    reject_if_present(root, "pyramid", "(pyramid)");
+    for (auto& s : table_array(root, "triangle")) {
+      const std::string ctx = "triangle";
+      double v[9];
+      req_vec3(s, "v0", ctx, v);
+      req_vec3(s, "v1", ctx, v + 3);
+      req_vec3(s, "v2", ctx, v + 6);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      lmn_prim& p = add_prim(LMN_PRIM_TRIANGLE, mat, -1, new_object());
+      for (int k = 0; k < 9; ++k) p.p[k] = v[k];
+    }
+    for (auto& s : table_array(root, "quad")) {  // src/scene.rs:1503-1543
+      const std::string ctx = "quad";
+      double q[3], u[3], v[3];
+      req_vec3(s, "q", ctx, q);
+      req_vec3(s, "u", ctx, u);
+      req_vec3(s, "v", ctx, v);
+      MatInfo mi;
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material", &mi);
+      if (mi.emissive) {
+        lmn_light l{};
+        l.kind = LMN_LIGHT_RECT;
+        double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+        double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; ++k) {
+          l.origin[k] = q[k];
+          l.u[k] = u[k];
+          l.v[k] = v[k];
+          l.normal[k] = n[k] / nl;
+          l.emission[k] = mi.emission[k];
+        }
+        out_.lights.push_back(l);
+      }
+      if (s.get("normal_map")) fail(LMN_ERR_UNSUPPORTED, "quad normal maps are not on the GPU path");
+      lmn_prim& p = add_prim(LMN_PRIM_QUAD, mat, -1, new_object());
+      for (int k = 0; k < 3; ++k) {
+        p.p[k] = q[k];
+        p.p[3 + k] = u[k];
+        p.p[6 + k] = v[k];
+      }
+    }
+    for (auto& s : table_array(root, "mesh")) {  // src/scene.rs:1545-1559
+      const std::string ctx = "mesh";
+      const TomlValue& f = req(s, "file", ctx);
+      if (f.kind != TomlValue::STRING) fail(LMN_ERR_PARSE, "mesh.file: expected a string");
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      double scale = 1.0, off[3] = {0, 0, 0};
+      opt_num(s, "scale", ctx, scale);
+      opt_vec3(s, "offset", ctx, off);
+      std::ifstream in(resolve(f.str), std::ios::binary);
+      if (!in.good()) fail(LMN_ERR_IO, "Failed to read mesh '" + f.str + "'");
+      std::stringstream ss;
+      ss << in.rdbuf();
+      std::string content = ss.str();
+      if (f.str.size() < 4 || f.str.compare(f.str.size() - 4, 4, ".ply") != 0)
+        fail(LMN_ERR_UNSUPPORTED, "OBJ meshes (smooth normals/uvs) are not on the GPU path: '" + f.str + "'");
+      std::vector<double> tv;
+      std::string err;
+      int rc = load_ply_text(content, scale, off, tv, err);
+      if (rc != LMN_OK) fail(rc, err);
+      size_t nt = tv.size() / 9;
+      out_.prims.reserve(out_.prims.size() + nt);
+      for (size_t t = 0; t < nt; ++t) {
+        lmn_prim& p = add_prim(LMN_PRIM_TRIANGLE, mat, -1, new_object());
+        for (int k = 0; k < 9; ++k) p.p[k] = tv[t * 9 + k];
+      }
+    }
+    for (auto& s : table_array(root, "box")) {  // src/scene.rs:1561-1600
+      const std::string ctx = "box";
+      double mn[3], mx[3];
+      req_vec3(s, "min", ctx, mn);
+      req_vec3(s, "max", ctx, mx);
+      // make_box calls the material factory once per side
+      // (src/rect.rs:128-141); all six materials are identical.
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      double rx, ry, rz, sc, t[3], sxyz[3];
+      bool has_rx = opt_num(s, "rotate_x", ctx, rx), has_ry = opt_num(s, "rotate_y", ctx, ry);
+      bool has_rz = opt_num(s, "rotate_z", ctx, rz), has_t = opt_vec3(s, "translate", ctx, t);
+      bool has_s = opt_num(s, "scale", ctx, sc), has_sxyz = opt_vec3(s, "scale_xyz", ctx, sxyz);
+      bool has_xf = has_rx || has_ry || has_rz || has_t || has_s || has_sxyz;
+      int32_t xf = -1;
+      if (has_xf) {
+        std::vector<lmn_xform_op> ops;
+        if (has_sxyz)
+          ops.push_back(op(LMN_XF_SCALE_XYZ, sxyz[0], sxyz[1], sxyz[2]));
+        else if (has_s)
+          ops.push_back(op(LMN_XF_SCALE, sc));
+        if (has_rx) ops.push_back(op(LMN_XF_ROTATE_X, rx));
+        if (has_ry) ops.push_back(op(LMN_XF_ROTATE_Y, ry));
+        if (has_rz) ops.push_back(op(LMN_XF_ROTATE_Z, rz));
+        if (has_t) ops.push_back(op(LMN_XF_TRANSLATE, t[0], t[1], t[2]));
+        xf = int32_t(add_transform(ops));
+      }
+      uint32_t obj = has_xf ? new_object() : 0;
+      // side order +z, -z, +y, -y, +x, -x (src/rect.rs:133-140)
+      const uint32_t kinds[6] = {LMN_PRIM_RECT_XY, LMN_PRIM_RECT_XY, LMN_PRIM_RECT_XZ,
+                                 LMN_PRIM_RECT_XZ, LMN_PRIM_RECT_YZ, LMN_PRIM_RECT_YZ};
+      for (int side = 0; side < 6; ++side) {
+        lmn_prim& p = add_prim(kinds[side], mat, xf, has_xf ? obj : new_object());
+        bool hi = (side % 2) == 0;
+        if (kinds[side] == LMN_PRIM_RECT_XY) {
+          p.p[0] = mn[0]; p.p[1] = mx[0]; p.p[2] = mn[1]; p.p[3] = mx[1]; p.p[4] = hi ? mx[2] : mn[2];
+        } else if (kinds[side] == LMN_PRIM_RECT_XZ) {
+          p.p[0] = mn[0]; p.p[1] = mx[0]; p.p[2] = mn[2]; p.p[3] = mx[2]; p.p[4] = hi ? mx[1] : mn[1];
+        } else {
+          p.p[0] = mn[1]; p.p[1] = mx[1]; p.p[2] = mn[2]; p.p[3] = mx[2]; p.p[4] = hi ? mx[0] : mn[0];
+        }
+      }
+    }
+    for (auto& s : table_array(root, "rect_xy")) {
+      const std::string ctx = "rect_xy";
+      double x[2], y[2];
+      vec2(req(s, "x", ctx), ctx + ".x", x);
+      vec2(req(s, "y", ctx), ctx + ".y", y);
+      double k = req_num(s, "k", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");  // emissive XY rects are not NEE lights
+      lmn_prim& p = add_prim(LMN_PRIM_RECT_XY, mat, -1, new_object());
+      p.p[0] = x[0]; p.p[1] = x[1]; p.p[2] = y[0]; p.p[3] = y[1]; p.p[4] = k;
+    }
+    for (auto& s : table_array(root, "rect_xz")) {  // src/scene.rs:1607-1622
+      const std::string ctx = "rect_xz";
+      double x[2], z[2];
+      vec2(req(s, "x", ctx), ctx + ".x", x);
+      vec2(req(s, "z", ctx), ctx + ".z", z);
+      double k = req_num(s, "k", ctx);
+      const TomlValue& mdesc = req(s, "material", ctx);
+      MatInfo mi;
+      uint32_t mat = material(mdesc, ctx + ".material", &mi);
+      // only `Emissive` (not Blackbody) XZ rects become NEE lights
+      std::string tag = material_tag(mdesc, ctx + ".material");
+      if (mi.emissive && (tag == "Emissive" || tag == "emissive")) {
+        lmn_light l{};
+        l.kind = LMN_LIGHT_RECT;
+        l.origin[0] = x[0]; l.origin[1] = k; l.origin[2] = z[0];
+        l.u[0] = x[1] - x[0];
+        l.v[2] = z[1] - z[0];
+        l.normal[1] = -1.0;
+        for (int c = 0; c < 3; ++c) l.emission[c] = mi.emission[c];
+        out_.lights.push_back(l);
+      }
+      lmn_prim& p = add_prim(LMN_PRIM_RECT_XZ, mat, -1, new_object());
+      p.p[0] = x[0]; p.p[1] = x[1]; p.p[2] = z[0]; p.p[3] = z[1]; p.p[4] = k;
+    }
+    for (auto& s : table_array(root, "rect_yz")) {
+      const std::string ctx = "rect_yz";
+      double y[2], z[2];
+      vec2(req(s, "y", ctx), ctx + ".y", y);
+      vec2(req(s, "z", ctx), ctx + ".z", z);
+      double k = req_num(s, "k", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      lmn_prim& p = add_prim(LMN_PRIM_RECT_YZ, mat, -1, new_object());
+      p.p[0] = y[0]; p.p[1] = y[1]; p.p[2] = z[0]; p.p[3] = z[1]; p.p[4] = k;
+    }
+    reject_if_present(root, "csg", "(CSG)");
+  }
+};
+
+}  // namespace
+
+int load_ply_text(const std::string& content, double scale, const double offset[3], std::vector<double>& tv,
+                  std::string& err) {
+  // src/obj.rs:157-246
+  std::istringstream in(content);
+  std::string line;
+  auto trim = [](const std::string& s) {
+    size_t b = s.find_first_not_of(" \t\r\n"), e = s.find_last_not_of(" \t\r\n");
+    return b == std::string::npos ? std::string() : s.substr(b, e - b + 1);
+  };
+  auto split = [](const std::string& s) {
+    std::vector<std::string> out;
+    std::istringstream ss(s);
+    std::string w;
+    while (ss >> w) out.push_back(w);
+    return out;
+  };
+  if (!std::getline(in, line) || trim(line) != "ply") {
+    err = "Not a PLY file (missing 'ply' magic)";
+    return LMN_ERR_PARSE;
+  }
+  size_t nv = 0, nf = 0;
+  bool in_header = true;
+  while (std::getline(in, line)) {
+    std::string t = trim(line);
+    if (t == "end_header") {
+      in_header = false;
+      break;
+    }
+    auto parts = split(t);
+    if (parts.size() >= 3 && parts[0] == "element") {
+      char* end = nullptr;
+      unsigned long long n = std::strtoull(parts[2].c_str(), &end, 10);
+      if (*end || parts[2][0] == '-') {
+        err = std::string("Bad ") + (parts[1] == "vertex" ? "vertex" : "face") + " count";
+        if (parts[1] == "vertex" || parts[1] == "face") return LMN_ERR_PARSE;
+        continue;
+      }
+      if (parts[1] == "vertex") nv = n;
+      if (parts[1] == "face") nf = n;
+    }
+  }
+  if (in_header) {
+    err = "PLY header not terminated (missing 'end_header')";
+    return LMN_ERR_PARSE;
+  }
+  std::vector<double> verts(nv * 3);
+  for (size_t i = 0; i < nv; ++i) {
+    if (!std::getline(in, line)) {
+      err = "Expected vertex " + std::to_string(i) + ", got EOF";
+      return LMN_ERR_PARSE;
+    }
+    auto parts = split(line);
+    if (parts.size() < 3) {
+      err = "Vertex " + std::to_string(i) + ": need at least 3 coordinates";
+      return LMN_ERR_PARSE;
+    }
+    for (int k = 0; k < 3; ++k) {
+      char* end = nullptr;
+      double x = std::strtod(parts[k].c_str(), &end);
+      if (*end) {
+        err = "Vertex " + std::to_string(i) + " coordinate: invalid float literal";
+        return LMN_ERR_PARSE;
+      }
+      verts[i * 3 + k] = x * scale + offset[k];
+    }
+  }
+  tv.reserve(tv.size() + nf * 9);
+  for (size_t i = 0; i < nf; ++i) {
+    if (!std::getline(in, line)) {
+      err = "Expected face " + std::to_string(i) + ", got EOF";
+      return LMN_ERR_PARSE;
+    }
+    auto parts = split(line);
+    if (parts.empty()) {
+      err = "Face " + std::to_string(i) + ": empty line";
+      return LMN_ERR_PARSE;
+    }
+    char* end = nullptr;
+    unsigned long n = std::strtoul(parts[0].c_str(), &end, 10);
+    if (*end || parts[0][0] == '-') {
+      err = "Face " + std::to_string(i) + " vertex count: invalid digit";
+      return LMN_ERR_PARSE;
+    }
+    if (parts.size() < n + 1) {
+      err = "Face " + std::to_string(i) + ": expected " + std::to_string(n) + " vertex indices";
+      return LMN_ERR_PARSE;
+    }
+    std::vector<size_t> idx(n);
+    for (size_t j = 0; j < n; ++j) {
+      unsigned long long x = std::strtoull(parts[j + 1].c_str(), &end, 10);
+      if (*end || parts[j + 1][0] == '-') {
+        err = "Face " + std::to_string(i) + " index " + std::to_string(j + 1) + ": invalid digit";
+        return LMN_ERR_PARSE;
+      }
+      idx[j] = size_t(x);
+    }
+    // fan triangulation (src/obj.rs:226-240); a face with fewer than three
+    // indices adds nothing
+    for (size_t j = 1; j + 1 < idx.size(); ++j) {
+      size_t i0 = idx[0], i1 = idx[j], i2 = idx[j + 1];
+      if (i0 >= nv || i1 >= nv || i2 >= nv) {
+        err = "Face " + std::to_string(i) + ": vertex index out of bounds";
+        return LMN_ERR_PARSE;
+      }
+      for (size_t c : {i0, i1, i2})
+        for (int k = 0; k < 3; ++k) tv.push_back(verts[c * 3 + k]);
+    }
+  }
+  return LMN_OK;
+}
+
+int load_scene_toml(const std::string& text, const std::string& base_dir, OwnedDesc& out, std::string& err) {
+  try {
+    Loader(base_dir, out).run(text);
+    return LMN_OK;
+  } catch (const LoadError& e) {
+    err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    err = "out of host memory while loading the scene";
+    return LMN_ERR_NOMEM;
+  }
+}
+
+}  // namespace lmn

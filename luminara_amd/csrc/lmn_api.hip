@@ -1,0 +1,555 @@
+// C ABI of the MI355X path tracer (include/luminara.h): scene loading and
+// upload, render entry points (wavefront FP32 / plain FP64 / plain FP32),
+// the ray-batch parity harness, device information.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/luminara.h"
+#include "host/scene_build.h"
+#include "host/scene_load.h"
+#include "lmn_kernels.h"
+
+using namespace lmn;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                             \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      return set_err(LMN_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, n ? n : 16);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v) {
+  size_t n = std::max<size_t>(v.size() * sizeof(T), 16);
+  hipError_t e = b.ensure(n);
+  if (e != hipSuccess) return e;
+  if (!v.empty()) return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  return hipSuccess;
+}
+
+// Reusable wavefront working memory (path pools, queues, sample buffer).
+struct Workspace {
+  std::mutex mu;
+  DevBuf st[2][4];  // ob, db, bk, meta for the two pools
+  DevBuf hit, sh_o, sh_d, sh_c, samples, acc, spill, ctr, tc, rays, out;
+  uint32_t pool = 0;
+  WaveCounters* host_ctr = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int init() {
+    if (!stream) {
+      HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+      HIP_TRY(hipEventCreate(&ev0));
+      HIP_TRY(hipEventCreate(&ev1));
+      HIP_TRY(hipHostMalloc((void**)&host_ctr, sizeof(WaveCounters), hipHostMallocDefault));
+    }
+    return LMN_OK;
+  }
+};
+Workspace& workspace() {
+  static Workspace* w = new Workspace();  // lives until process exit
+  return *w;
+}
+
+}  // namespace
+
+struct lmn_scene {
+  HostScene host;
+  DevBuf nodesF, nodesD, primsF, primsD, xfF, xfD, matF, matD, tex, texels, lightF, lightD;
+  SceneView<float> viewF{};
+  SceneView<double> viewD{};
+  lmn_camera camera{};
+  lmn_render_config config{};
+  size_t device_bytes = 0;
+  int device = 0;
+};
+
+extern "C" {
+
+const char* lmn_last_error(void) { return g_err.c_str(); }
+
+int lmn_load_scene(const char* toml_text, const char* base_dir, lmn_scene_desc** out) {
+  if (!toml_text || !out) return set_err(LMN_ERR_INVALID, "lmn_load_scene: null argument");
+  *out = nullptr;
+  auto* od = new (std::nothrow) OwnedDesc();
+  if (!od) return set_err(LMN_ERR_NOMEM, "out of host memory");
+  std::string err;
+  int rc = load_scene_toml(toml_text, base_dir ? base_dir : "", *od, err);
+  if (rc != LMN_OK) {
+    delete od;
+    return set_err(rc, err);
+  }
+  *out = &od->desc;
+  return LMN_OK;
+}
+
+void lmn_desc_free(lmn_scene_desc* desc) { delete reinterpret_cast<OwnedDesc*>(desc); }
+
+int lmn_desc_set_texture(lmn_scene_desc* desc, uint32_t index, uint32_t width, uint32_t height, const float* rgb) {
+  if (!desc || !rgb || !width || !height) return set_err(LMN_ERR_INVALID, "lmn_desc_set_texture: bad argument");
+  auto* od = reinterpret_cast<OwnedDesc*>(desc);
+  if (index >= od->texels.size()) return set_err(LMN_ERR_INVALID, "lmn_desc_set_texture: index out of range");
+  od->texels[index].assign(rgb, rgb + size_t(width) * height * 3);
+  od->textures[index].width = width;
+  od->textures[index].height = height;
+  od->sync();
+  return LMN_OK;
+}
+
+int lmn_desc_add_texture(lmn_scene_desc* desc, uint32_t width, uint32_t height, const float* rgb,
+                         uint32_t* out_index) {
+  if (!desc || !rgb || !width || !height) return set_err(LMN_ERR_INVALID, "lmn_desc_add_texture: bad argument");
+  auto* od = reinterpret_cast<OwnedDesc*>(desc);
+  od->texels.emplace_back(rgb, rgb + size_t(width) * height * 3);
+  od->textures.push_back(lmn_texture{width, height, nullptr});
+  od->sync();
+  if (out_index) *out_index = uint32_t(od->texels.size() - 1);
+  return LMN_OK;
+}
+
+int lmn_desc_set_material(lmn_scene_desc* desc, uint32_t index, const lmn_material* m) {
+  if (!desc || !m) return set_err(LMN_ERR_INVALID, "lmn_desc_set_material: null argument");
+  auto* od = reinterpret_cast<OwnedDesc*>(desc);
+  if (index >= od->materials.size()) return set_err(LMN_ERR_INVALID, "lmn_desc_set_material: index out of range");
+  if (m->texture >= 0 && uint32_t(m->texture) >= od->texels.size())
+    return set_err(LMN_ERR_INVALID, "lmn_desc_set_material: texture index out of range");
+  od->materials[index] = *m;
+  od->sync();
+  return LMN_OK;
+}
+
+int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
+  if (!desc || !out) return set_err(LMN_ERR_INVALID, "lmn_scene_create: null argument");
+  *out = nullptr;
+  std::unique_ptr<lmn_scene> s(new (std::nothrow) lmn_scene());
+  if (!s) return set_err(LMN_ERR_NOMEM, "out of host memory");
+  std::string err;
+  BvhOptions opt;
+  int rc = build_host_scene(*desc, opt, s->host, err);
+  if (rc != LMN_OK) return set_err(rc, err);
+  HIP_TRY(hipGetDevice(&s->device));
+  HostScene& h = s->host;
+  HIP_TRY(upload(s->nodesF, h.nodesF));
+  HIP_TRY(upload(s->nodesD, h.nodesD));
+  HIP_TRY(upload(s->primsF, h.primsF));
+  HIP_TRY(upload(s->primsD, h.primsD));
+  HIP_TRY(upload(s->xfF, h.xfF));
+  HIP_TRY(upload(s->xfD, h.xfD));
+  HIP_TRY(upload(s->matF, h.matF));
+  HIP_TRY(upload(s->matD, h.matD));
+  HIP_TRY(upload(s->tex, h.tex));
+  HIP_TRY(upload(s->texels, h.texels));
+  HIP_TRY(upload(s->lightF, h.lightF));
+  HIP_TRY(upload(s->lightD, h.lightD));
+  s->device_bytes = s->nodesF.bytes + s->nodesD.bytes + s->primsF.bytes + s->primsD.bytes + s->xfF.bytes +
+                    s->xfD.bytes + s->matF.bytes + s->matD.bytes + s->tex.bytes + s->texels.bytes +
+                    s->lightF.bytes + s->lightD.bytes;
+  SceneView<float>& F = s->viewF;
+  F.nodes = s->nodesF.as<NodeF>();
+  F.prims = s->primsF.as<PrimT<float>>();
+  F.prims64 = s->primsD.as<PrimT<double>>();
+  F.xforms = s->xfF.as<XformT<float>>();
+  F.xforms64 = s->xfD.as<XformT<double>>();
+  F.materials = s->matF.as<MaterialT<float>>();
+  F.textures = s->tex.as<TextureDev>();
+  F.texels = s->texels.as<float>();
+  F.lights = s->lightF.as<LightT<float>>();
+  F.n_nodes = uint32_t(h.nodesF.size());
+  F.n_bounded = h.n_bounded;
+  F.n_planes = h.n_planes;
+  F.n_lights = uint32_t(h.lightF.size());
+  F.bg = h.bgF;
+  F.cam = h.camF;
+  SceneView<double>& D = s->viewD;
+  D.nodes = s->nodesD.as<NodeD>();
+  D.prims = s->primsD.as<PrimT<double>>();
+  D.prims64 = D.prims;
+  D.xforms = s->xfD.as<XformT<double>>();
+  D.xforms64 = D.xforms;
+  D.materials = s->matD.as<MaterialT<double>>();
+  D.textures = F.textures;
+  D.texels = F.texels;
+  D.lights = s->lightD.as<LightT<double>>();
+  D.n_nodes = F.n_nodes;
+  D.n_bounded = F.n_bounded;
+  D.n_planes = F.n_planes;
+  D.n_lights = F.n_lights;
+  D.bg = h.bgD;
+  D.cam = h.camD;
+  s->camera = desc->camera;
+  s->config = desc->config;
+  *out = s.release();
+  return LMN_OK;
+}
+
+void lmn_scene_destroy(lmn_scene* scene) { delete scene; }
+
+int lmn_scene_info(const lmn_scene* s, uint64_t* n_nodes, uint64_t* n_leaves, uint32_t* max_depth,
+                   uint64_t* device_bytes) {
+  if (!s) return set_err(LMN_ERR_INVALID, "lmn_scene_info: null scene");
+  if (n_nodes) *n_nodes = s->host.nodesF.size();
+  if (n_leaves) *n_leaves = s->host.n_leaves;
+  if (max_depth) *max_depth = s->host.tree_depth;
+  if (device_bytes) *device_bytes = s->device_bytes;
+  return LMN_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct FrameSetup {
+  FrameDev F;
+  uint32_t npix;
+  bool black;  // zero spp or zero depth: the reference returns black pixels
+};
+
+int frame_setup(const lmn_render_params* p, FrameSetup& fs) {
+  // src/render.rs:1563-1590
+  uint32_t fw = std::max<uint32_t>(p->width, 1), fh = std::max<uint32_t>(p->height, 1);
+  uint32_t cx = 0, cy = 0, cw = fw, ch = fh;
+  if (p->has_crop) {
+    cx = std::min(p->crop[0], fw);
+    cy = std::min(p->crop[1], fh);
+    cw = std::min(p->crop[2], fw);
+    ch = std::min(p->crop[3], fh);
+    if (uint64_t(cy) + ch > fh)
+      return set_err(LMN_ERR_INVALID, "crop rows beyond the image height are not supported");
+  }
+  if (cw == 0 || ch == 0) return set_err(LMN_ERR_INVALID, "empty crop region");
+  uint32_t sq = uint32_t(std::ceil(std::sqrt(double(p->spp))));
+  FrameDev& F = fs.F;
+  F.full_w = fw;
+  F.full_h = fh;
+  F.crop_x = cx;
+  F.crop_y = cy;
+  F.crop_w = cw;
+  F.crop_h = ch;
+  F.sqrt_spp = sq;
+  F.spp_a = sq * sq;
+  F.max_depth = p->max_depth;
+  F.filter = p->filter;
+  F.fkey = frame_key(p->seed, p->scene_key);
+  fs.npix = cw * ch;
+  F.n_items = uint64_t(fs.npix) * F.spp_a;
+  F.item_base = 0;
+  fs.black = F.spp_a == 0 || p->max_depth == 0;
+  if (p->max_depth > 0xffff) return set_err(LMN_ERR_INVALID, "max_depth above 65535 is not supported");
+  if (p->filter > LMN_FILTER_MITCHELL) return set_err(LMN_ERR_INVALID, "unknown pixel filter");
+  return LMN_OK;
+}
+
+constexpr uint32_t kMaxTreeDepth = 48;
+
+int render_wavefront(lmn_scene* s, const lmn_render_params* p, const FrameSetup& fs, float* d_out, hipStream_t st,
+                     lmn_stats* stats, TraceCounters* d_tc) {
+  Workspace& W = workspace();
+  FrameDev F = fs.F;
+  uint32_t npix = fs.npix;
+  uint64_t total = F.n_items;
+  uint32_t pool = p->pool_paths ? p->pool_paths : uint32_t(std::min<uint64_t>(total, 1u << 22));
+  pool = std::max<uint32_t>(pool, 64);
+  uint32_t ss = p->stack_lds ? p->stack_lds : 16;
+  if (ss != 8 && ss != 16 && ss != 24 && ss != 32) return set_err(LMN_ERR_INVALID, "stack_lds must be 8, 16, 24 or 32");
+  // sample buffer chunk: whole sample passes, at most 2^27 samples (2 GiB)
+  uint64_t max_chunk = 1ull << 27;
+  uint32_t spp_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(F.spp_a, max_chunk / npix)));
+  uint64_t chunk_items = uint64_t(spp_chunk) * npix;
+  uint32_t grid = std::min<uint32_t>((pool + 255) / 256, 256 * 8);
+  uint32_t nthreads = grid * 256;
+  for (int k = 0; k < 2; ++k) {
+    HIP_TRY(W.st[k][0].ensure(size_t(pool) * 16));
+    HIP_TRY(W.st[k][1].ensure(size_t(pool) * 16));
+    HIP_TRY(W.st[k][2].ensure(size_t(pool) * 16));
+    HIP_TRY(W.st[k][3].ensure(size_t(pool) * 16));
+  }
+  HIP_TRY(W.hit.ensure(size_t(pool) * 16));
+  HIP_TRY(W.sh_o.ensure(size_t(pool) * 16));
+  HIP_TRY(W.sh_d.ensure(size_t(pool) * 16));
+  HIP_TRY(W.sh_c.ensure(size_t(pool) * 16));
+  HIP_TRY(W.samples.ensure(size_t(chunk_items) * 16));
+  HIP_TRY(W.acc.ensure(size_t(npix) * 32));
+  HIP_TRY(W.spill.ensure(size_t(kMaxTreeDepth) * nthreads * 4));
+  HIP_TRY(W.ctr.ensure(sizeof(WaveCounters)));
+  PathSoA P[2];
+  for (int k = 0; k < 2; ++k)
+    P[k] = PathSoA{W.st[k][0].as<float4>(), W.st[k][1].as<float4>(), W.st[k][2].as<uint4>(), W.st[k][3].as<uint4>()};
+  WaveLaunch L;
+  L.stream = st;
+  L.grid = grid;
+  L.pool = pool;
+  L.stack_lds = ss;
+  L.hit = W.hit.as<float4>();
+  L.sh = ShadowSoA{W.sh_o.as<float4>(), W.sh_d.as<float4>(), W.sh_c.as<float4>()};
+  L.samples = W.samples.as<float4>();
+  L.ctr = W.ctr.as<WaveCounters>();
+  L.spill = W.spill.as<uint32_t>();
+  L.tc = d_tc;
+  SceneView<float> S = s->viewF;
+  S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
+  launch_zero_acc(W.acc.as<double4>(), npix, st);
+  uint64_t sum_ext = 0, sum_sh = 0, sum_cam = 0, iters = 0;
+  const int block = 8;  // iterations between host checks
+  for (uint32_t s0 = 0; s0 < F.spp_a; s0 += spp_chunk) {
+    uint32_t s1 = std::min(F.spp_a, s0 + spp_chunk);
+    F.item_base = uint64_t(s0) * npix;
+    uint64_t items = uint64_t(s1 - s0) * npix;
+    launch_counters_init(L.ctr, items, st);
+    int cur = 0;
+    auto t_start = std::chrono::steady_clock::now();
+    for (;;) {
+      for (int k = 0; k < block; ++k) {
+        launch_wave_iteration(L, S, F, P[cur], P[cur ^ 1], d_tc != nullptr);
+        cur ^= 1;
+      }
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(W.host_ctr, L.ctr, sizeof(WaveCounters), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      const WaveCounters& c = *W.host_ctr;
+      if (c.active == 0 && c.next_item == c.n_items) {
+        sum_ext += c.stat_extend;
+        sum_sh += c.stat_shadow;
+        sum_cam += c.stat_camera;
+        iters += c.iterations;
+        break;
+      }
+      if (std::chrono::steady_clock::now() - t_start > std::chrono::seconds(600))
+        return set_err(LMN_ERR_DEVICE, "wavefront did not drain within 600 s");
+    }
+    launch_resolve(L.samples, W.acc.as<double4>(), npix, s1 - s0, st);
+  }
+  launch_finalize(W.acc.as<double4>(), d_out, npix, st);
+  HIP_TRY(hipGetLastError());
+  if (stats) {
+    stats->camera_rays = sum_cam;
+    stats->extension_rays = sum_ext - sum_cam;
+    stats->shadow_rays = sum_sh;
+    stats->iterations = uint32_t(iters);
+  }
+  return LMN_OK;
+}
+
+int render_plain(lmn_scene* s, const lmn_render_params* p, const FrameSetup& fs, float* d_out, hipStream_t st,
+                 lmn_stats* stats, bool fp64) {
+  Workspace& W = workspace();
+  uint32_t npix = fs.npix;
+  HIP_TRY(W.acc.ensure(size_t(npix) * 32));
+  HIP_TRY(W.rays.ensure(16));
+  HIP_TRY(hipMemsetAsync(W.rays.p, 0, 16, st));
+  launch_zero_acc(W.acc.as<double4>(), npix, st);
+  for (uint32_t s0 = 0; s0 < fs.F.spp_a; ++s0) {
+    if (fp64) {
+      SceneView<double> S = s->viewD;
+      S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
+      launch_plain_fp64(S, fs.F, W.acc.as<double4>(), s0, s0 + 1, W.rays.as<unsigned long long>(), st);
+    } else {
+      SceneView<float> S = s->viewF;
+      S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
+      launch_plain_fp32(S, fs.F, W.acc.as<double4>(), s0, s0 + 1, W.rays.as<unsigned long long>(), st);
+    }
+  }
+  launch_finalize(W.acc.as<double4>(), d_out, npix, st);
+  HIP_TRY(hipGetLastError());
+  if (stats) {
+    unsigned long long r[2];
+    HIP_TRY(hipMemcpyAsync(r, W.rays.p, 16, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    stats->camera_rays = uint64_t(npix) * fs.F.spp_a;
+    stats->extension_rays = r[0] - stats->camera_rays;
+    stats->shadow_rays = r[1];
+    stats->iterations = 0;
+  }
+  return LMN_OK;
+}
+
+int render_common(lmn_scene* s, const lmn_render_params* p, float* d_out, lmn_stats* stats, TraceCounters* d_tc) {
+  if (!s || !p || !d_out) return set_err(LMN_ERR_INVALID, "render: null argument");
+  FrameSetup fs;
+  int rc = frame_setup(p, fs);
+  if (rc != LMN_OK) return rc;
+  Workspace& W = workspace();
+  std::lock_guard<std::mutex> lock(W.mu);
+  rc = W.init();
+  if (rc != LMN_OK) return rc;
+  hipStream_t st = p->stream ? static_cast<hipStream_t>(p->stream) : W.stream;
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  HIP_TRY(hipEventRecord(W.ev0, st));
+  if (fs.black) {
+    HIP_TRY(hipMemsetAsync(d_out, 0, size_t(fs.npix) * 12, st));
+  } else if (p->path == LMN_PATH_WAVEFRONT_FP32) {
+    rc = render_wavefront(s, p, fs, d_out, st, stats, d_tc);
+  } else if (p->path == LMN_PATH_PLAIN_FP64 || p->path == LMN_PATH_PLAIN_FP32) {
+    rc = render_plain(s, p, fs, d_out, st, stats, p->path == LMN_PATH_PLAIN_FP64);
+  } else {
+    return set_err(LMN_ERR_INVALID, "unknown render path");
+  }
+  if (rc != LMN_OK) return rc;
+  HIP_TRY(hipEventRecord(W.ev1, st));
+  HIP_TRY(hipEventSynchronize(W.ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+  if (stats) {
+    stats->paths = uint64_t(fs.npix) * fs.F.spp_a;
+    stats->render_ms = ms;
+  }
+  return LMN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmn_render_device(const lmn_scene* scene, const lmn_render_params* params, float* d_hdr_rgb, lmn_stats* stats) {
+  return render_common(const_cast<lmn_scene*>(scene), params, d_hdr_rgb, stats, nullptr);
+}
+
+int lmn_render(const lmn_scene* scene, const lmn_render_params* params, float* hdr_rgb, lmn_stats* stats) {
+  if (!scene || !params || !hdr_rgb) return set_err(LMN_ERR_INVALID, "lmn_render: null argument");
+  FrameSetup fs;
+  int rc = frame_setup(params, fs);
+  if (rc != LMN_OK) return rc;
+  size_t bytes = size_t(fs.npix) * 12;
+  float* d = nullptr;
+  HIP_TRY(hipMalloc((void**)&d, bytes));
+  rc = render_common(const_cast<lmn_scene*>(scene), params, d, stats, nullptr);
+  if (rc == LMN_OK) {
+    hipError_t e = hipMemcpy(hdr_rgb, d, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = set_err(LMN_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+int lmn_render_batch(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
+                     float* const* hdr_rgb, lmn_stats* stats) {
+  if (!scenes || !params || !hdr_rgb) return set_err(LMN_ERR_INVALID, "lmn_render_batch: null argument");
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  for (size_t i = 0; i < n_scenes; ++i) {
+    lmn_stats st{};
+    int rc = lmn_render(scenes[i], &params[i], hdr_rgb[i], &st);
+    if (rc != LMN_OK) return rc;
+    if (stats) {
+      stats->paths += st.paths;
+      stats->camera_rays += st.camera_rays;
+      stats->extension_rays += st.extension_rays;
+      stats->shadow_rays += st.shadow_rays;
+      stats->render_ms += st.render_ms;
+      stats->iterations += st.iterations;
+    }
+  }
+  return LMN_OK;
+}
+
+int lmn_trace_rays(const lmn_scene* scene, const lmn_ray* rays, size_t n, uint32_t path, uint32_t intersect_mode,
+                   lmn_hit* hits) {
+  if (!scene || (!rays && n) || (!hits && n)) return set_err(LMN_ERR_INVALID, "lmn_trace_rays: null argument");
+  if (n == 0) return LMN_OK;
+  lmn_ray* d_r = nullptr;
+  lmn_hit* d_h = nullptr;
+  HIP_TRY(hipMalloc((void**)&d_r, n * sizeof(lmn_ray)));
+  hipError_t e = hipMalloc((void**)&d_h, n * sizeof(lmn_hit));
+  if (e != hipSuccess) {
+    (void)hipFree(d_r);
+    return set_err(LMN_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  int rc = LMN_OK;
+  e = hipMemcpy(d_r, rays, n * sizeof(lmn_ray), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    if (path == LMN_PATH_PLAIN_FP64) {
+      SceneView<double> S = scene->viewD;
+      S.exact_marched = intersect_mode == LMN_ISECT_EXACT;
+      launch_trace_fp64(S, d_r, d_h, n, nullptr);
+    } else {
+      SceneView<float> S = scene->viewF;
+      S.exact_marched = intersect_mode == LMN_ISECT_EXACT;
+      launch_trace_fp32(S, d_r, d_h, n, nullptr);
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(hits, d_h, n * sizeof(lmn_hit), hipMemcpyDeviceToHost);
+  }
+  if (e != hipSuccess) rc = set_err(LMN_ERR_DEVICE, std::string("trace: ") + hipGetErrorString(e));
+  (void)hipFree(d_r);
+  (void)hipFree(d_h);
+  return rc;
+}
+
+int lmn_device_info(char* name, size_t name_len, int* n_cu, double* hbm_bytes) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, dev));
+  if (name && name_len) {
+    std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (n_cu) *n_cu = prop.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = double(prop.totalGlobalMem);
+  return LMN_OK;
+}
+
+// Instrumented wavefront render: counts BVH node fetches and primitive tests
+// of the extend stage (the algorithmic bytes of the roofline).
+int lmn_profile_extend(const lmn_scene* scene, const lmn_render_params* params, float* d_hdr_rgb,
+                       lmn_trace_counts* counts) {
+  if (!counts) return set_err(LMN_ERR_INVALID, "lmn_profile_extend: null counts");
+  lmn_render_params p = *params;
+  p.path = LMN_PATH_WAVEFRONT_FP32;
+  TraceCounters* d_tc = nullptr;
+  HIP_TRY(hipMalloc((void**)&d_tc, sizeof(TraceCounters)));
+  hipError_t e = hipMemset(d_tc, 0, sizeof(TraceCounters));
+  int rc = e == hipSuccess ? render_common(const_cast<lmn_scene*>(scene), &p, d_hdr_rgb, nullptr, d_tc)
+                           : set_err(LMN_ERR_DEVICE, "hipMemset failed");
+  if (rc == LMN_OK) {
+    TraceCounters h;
+    e = hipMemcpy(&h, d_tc, sizeof h, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = set_err(LMN_ERR_DEVICE, "hipMemcpy failed");
+    counts->node_fetches = h.node_fetches;
+    counts->prim_tests = h.prim_tests;
+    counts->tri_tests = h.tri_tests;
+    counts->rays = h.rays;
+    counts->node_bytes = sizeof(NodeF);
+    counts->prim_bytes = sizeof(PrimT<float>);
+  }
+  (void)hipFree(d_tc);
+  return rc;
+}
+
+}  // extern "C"

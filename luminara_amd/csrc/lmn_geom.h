@@ -1,0 +1,618 @@
+// Device scene layout and the primitive intersection library.
+//
+// Layout in HBM (one scene):
+//   nodes   : BVH2 in depth-first order, left child = node + 1.  FP32 nodes are
+//             32 bytes {lo.xyz, a, hi.xyz, b}; FP64 nodes 64 bytes.
+//             internal: a = right child, b = split axis; leaf: a = first prim,
+//             b = LEAF_BIT | prim count.
+//   prims   : primitive records reordered into leaf order (64 B FP32 /
+//             128 B FP64), so a leaf's primitives are one contiguous fetch.
+//   planes  : unbounded primitives (infinite planes), tested after the BVH
+//             exactly as SceneWorld::hit does (src/scene.rs:817-834).
+//   xforms  : instance transforms (rotations/translation/scale).
+// Intersection routines follow the reference's hit() arithmetic
+// (file:line at each function) in either precision.
+#pragma once
+#include "lmn_math.h"
+
+namespace lmn {
+
+enum : uint32_t {
+  K_SPHERE = 0, K_QUAD = 1, K_RECT_XY = 2, K_RECT_XZ = 3, K_RECT_YZ = 4, K_TRIANGLE = 5,
+  K_PLANE = 6, K_TORUS = 7, K_SUPERELLIPSOID = 8, K_SPRING = 9
+};
+constexpr uint32_t KIND_MASK = 0x1f;
+constexpr uint32_t FLAG_FP64 = 0x20;  // FP32 path evaluates this primitive in FP64
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr uint32_t XF_NONE = 0xffffffffu;
+
+// Parameter slots p[]:
+//   SPHERE c.xyz r | QUAD q.xyz u.xyz v.xyz w.xyz D (w = n/(n.n), D = unit(n).q)
+//   RECT_* a0 a1 b0 b1 k | TRIANGLE v0.xyz e1.xyz e2.xyz
+//   PLANE point.xyz normal.xyz uaxis.xyz vaxis.xyz | TORUS c.xyz R r
+//   SUPERELLIPSOID c.xyz s.xyz e1 e2 | SPRING c.xyz coil tube pitch turns
+template <class R>
+struct alignas(16) PrimT {
+  R p[13];
+  uint32_t kind_mat;  // kind | flags | material << 8
+  uint32_t xform;     // transform index or XF_NONE
+  uint32_t prim_id;   // index into the descriptor's prim list
+};
+static_assert(sizeof(PrimT<float>) == 64, "FP32 primitive record must be 64 bytes");
+static_assert(sizeof(PrimT<double>) == 128, "FP64 primitive record must be 128 bytes");
+
+struct NodeF {
+  float lo[3];
+  uint32_t a;
+  float hi[3];
+  uint32_t b;
+};
+static_assert(sizeof(NodeF) == 32, "FP32 BVH node must be 32 bytes");
+struct alignas(16) NodeD {
+  double lo[3];
+  double hi[3];
+  uint32_t a, b;
+};
+static_assert(sizeof(NodeD) == 64, "FP64 BVH node must be 64 bytes");
+template <class R> struct NodeSel;
+template <> struct NodeSel<float> { using type = NodeF; };
+template <> struct NodeSel<double> { using type = NodeD; };
+
+// Instance transform: composed wrappers of src/transform.rs.  `inv` maps a
+// world ray into object space, `nrm` maps an object normal back.  The
+// reference's Scale / NonUniformScale rescale the reported t (a quirk,
+// src/transform.rs:277, :336): tmode 1 multiplies by tscale, tmode 2 by
+// |d_obj| / |d_world|.
+template <class R>
+struct XformT {
+  R inv[12];
+  R nrm[9];
+  R tscale;
+  uint32_t tmode;
+  uint32_t renorm;  // normalise the mapped normal (non-uniform scale)
+};
+
+template <class R> LMN_HD V3<R> xf_point(const R* m, V3<R> p) {
+  return {m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+          m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
+}
+template <class R> LMN_HD V3<R> xf_dir(const R* m, V3<R> d) {
+  return {m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
+          m[8] * d.x + m[9] * d.y + m[10] * d.z};
+}
+template <class R> LMN_HD V3<R> xf_nrm(const R* m, V3<R> n) {
+  return {m[0] * n.x + m[1] * n.y + m[2] * n.z, m[3] * n.x + m[4] * n.y + m[5] * n.z,
+          m[6] * n.x + m[7] * n.y + m[8] * n.z};
+}
+
+template <class R> LMN_HD R t_report_factor(const XformT<R>& x, V3<R> d_obj, V3<R> d_world) {
+  if (x.tmode == 1) return x.tscale;
+  if (x.tmode == 2) return length(d_obj) / length(d_world);
+  return R(1);
+}
+
+// ---------------------------------------------------------------------------
+// Analytic primitives.  Each returns the hit t (object space == world space t
+// for the affine wrappers), and the reference's (u, v).
+
+// Sphere::hit (src/sphere.rs:24-62)
+template <class R>
+LMN_HD bool hit_sphere(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
+  V3<R> c = {p[0], p[1], p[2]};
+  R r = p[3];
+  V3<R> oc = o - c;
+  R a = length_squared(d);
+  R half_b = dot(oc, d);
+  R cc = length_squared(oc) - r * r;
+  R disc = half_b * half_b - a * cc;
+  if (disc < R(0)) return false;
+  R sq = vsqrt(disc);
+  R root = (-half_b - sq) / a;
+  if (root < tmin || root > tmax) {
+    root = (-half_b + sq) / a;
+    if (root < tmin || root > tmax) return false;
+  }
+  if (!(root == root)) return false;
+  V3<R> n = ((o + d * root) - c) / r;
+  const R pi = R(3.14159265358979323846);
+  R theta = m_acos(-n.y);
+  R phi = m_atan2(-n.z, n.x) + pi;
+  t = root;
+  u = phi / (R(2) * pi);
+  v = theta / pi;
+  return true;
+}
+
+// Quad::hit (src/quad.rs:30-62)
+template <class R>
+LMN_HD bool hit_quad(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
+  V3<R> q = {p[0], p[1], p[2]}, uu = {p[3], p[4], p[5]}, vv = {p[6], p[7], p[8]}, w = {p[9], p[10], p[11]};
+  V3<R> n = unit(w);
+  R denom = dot(n, d);
+  if (m_abs(denom) < R(1e-8)) return false;
+  R tt = (p[12] - dot(n, o)) / denom;
+  if (tt < tmin || tt > tmax || !(tt == tt)) return false;
+  V3<R> planar = (o + d * tt) - q;
+  R alpha = dot(w, cross(planar, vv));
+  R beta = dot(w, cross(uu, planar));
+  if (!(alpha >= R(0) && alpha <= R(1)) || !(beta >= R(0) && beta <= R(1))) return false;
+  t = tt;
+  u = alpha;
+  v = beta;
+  return true;
+}
+
+// XyRect / XzRect / YzRect::hit (src/rect.rs:24-37, :64-77, :104-117).  `ax`
+// is the plane axis, (ia, ib) the in-plane axes.
+template <class R>
+LMN_HD bool hit_rect(const R* p, int ax, int ia, int ib, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
+  const R* oo = &o.x;
+  const R* dd = &d.x;
+  R tt = (p[4] - oo[ax]) / dd[ax];
+  if (tt < tmin || tt > tmax || !(tt == tt)) return false;
+  R a = oo[ia] + tt * dd[ia];
+  R b = oo[ib] + tt * dd[ib];
+  if (a < p[0] || a > p[1] || b < p[2] || b > p[3]) return false;
+  t = tt;
+  u = (a - p[0]) / (p[1] - p[0]);
+  v = (b - p[2]) / (p[3] - p[2]);
+  return true;
+}
+
+// Triangle::hit / MeshTriangle::hit, Moller-Trumbore
+// (src/triangle.rs:23-64, src/obj.rs:260-313)
+template <class R>
+LMN_HD bool hit_triangle(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
+  V3<R> v0 = {p[0], p[1], p[2]}, e1 = {p[3], p[4], p[5]}, e2 = {p[6], p[7], p[8]};
+  V3<R> h = cross(d, e2);
+  R a = dot(e1, h);
+  if (m_abs(a) < R(1e-8)) return false;
+  R f = R(1) / a;
+  V3<R> s = o - v0;
+  R uu = f * dot(s, h);
+  if (!(uu >= R(0) && uu <= R(1))) return false;
+  V3<R> q = cross(s, e1);
+  R vv = f * dot(d, q);
+  if (vv < R(0) || uu + vv > R(1)) return false;
+  R tt = f * dot(e2, q);
+  if (tt < tmin || tt > tmax || !(tt == tt)) return false;
+  t = tt;
+  u = uu;
+  v = vv;
+  return true;
+}
+
+// Plane::hit (src/plane.rs:43-69)
+template <class R>
+LMN_HD bool hit_plane(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
+  V3<R> pt = {p[0], p[1], p[2]}, n = {p[3], p[4], p[5]};
+  R denom = dot(n, d);
+  if (m_abs(denom) < R(1e-8)) return false;
+  R tt = dot(pt - o, n) / denom;
+  if (tt < tmin || tt > tmax || !(tt == tt)) return false;
+  V3<R> local = (o + d * tt) - pt;
+  R a = dot(local, V3<R>{p[6], p[7], p[8]});
+  R b = dot(local, V3<R>{p[9], p[10], p[11]});
+  t = tt;
+  u = m_abs(a - m_trunc(a));  // f64::fract().abs()
+  v = m_abs(b - m_trunc(b));
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Marched primitives, evaluated in FP64 on every path.
+
+LMN_HD double torus_sdf(V3<double> p, double R_, double r_) {
+  double q = m_sqrt(p.x * p.x + p.z * p.z) - R_;
+  return m_sqrt(q * q + p.y * p.y) - r_;
+}
+
+// Torus::hit, reference semantics: fixed-step SDF march with bisection
+// (src/torus.rs:24-103).
+LMN_HD bool hit_torus_march(const double* p, V3<double> ro, V3<double> d, double tmin, double tmax, double& t) {
+  V3<double> o = ro - V3<double>{p[0], p[1], p[2]};
+  double R_ = p[3], r_ = p[4];
+  double step = r_ * 0.5;
+  double ms = fmin((tmax - tmin) / step, 500.0);
+  // `as usize` saturates: NaN and negatives become 0
+  long max_steps = (ms == ms && ms > 0.0) ? long(ms) : 0;
+  double tc = tmin;
+  double prev = torus_sdf(o + d * tc, R_, r_);
+  for (long i = 0; i < max_steps; ++i) {
+    tc += step;
+    if (tc > tmax) break;
+    double dist = torus_sdf(o + d * tc, R_, r_);
+    if (prev > 0.0 && dist < 0.0) {
+      double lo = tc - step, hi = tc;
+      for (int k = 0; k < 20; ++k) {
+        double mid = (lo + hi) * 0.5;
+        if (torus_sdf(o + d * mid, R_, r_) > 0.0)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      double th = (lo + hi) * 0.5;
+      if (th < tmin || th > tmax) {
+        prev = dist;
+        continue;
+      }
+      t = th;
+      return true;
+    }
+    prev = dist;
+  }
+  return false;
+}
+
+// Real roots of the monic quartic x^4 + a x^3 + b x^2 + c x + e, found as the
+// eigen-free way: Ferrari's resolvent cubic solved in closed form, then every
+// candidate root polished by Newton on the original quartic.  Returns the
+// count and the roots in ascending order.
+LMN_HD int solve_cubic_depressed_largest(double p, double q, double& y) {
+  // largest real root of y^3 + p y + q = 0
+  double disc = q * q / 4.0 + p * p * p / 27.0;
+  if (disc >= 0.0) {
+    double s = m_sqrt(disc);
+    y = cbrt(-q / 2.0 + s) + cbrt(-q / 2.0 - s);
+  } else {
+    double rr = m_sqrt(-p / 3.0);
+    double ang = m_acos(fmax(-1.0, fmin(1.0, -q / (2.0 * rr * rr * rr))));
+    y = 2.0 * rr * m_cos(ang / 3.0);
+  }
+  return 1;
+}
+
+LMN_HD int solve_quartic(double a, double b, double c, double e, double roots[4]) {
+  // depressed quartic y^4 + P y^2 + Q y + S via x = y - a/4
+  double a2 = a * a;
+  double P = b - 3.0 * a2 / 8.0;
+  double Q = c - a * b / 2.0 + a2 * a / 8.0;
+  double S = e - a * c / 4.0 + a2 * b / 16.0 - 3.0 * a2 * a2 / 256.0;
+  int n = 0;
+  double ys[4];
+  if (m_abs(Q) < 1e-14 * (1.0 + m_abs(P) + m_abs(S))) {
+    // biquadratic
+    double disc = P * P - 4.0 * S;
+    if (disc >= 0.0) {
+      double sd = m_sqrt(disc);
+      double z1 = (-P - sd) / 2.0, z2 = (-P + sd) / 2.0;
+      if (z1 >= 0.0) { ys[n++] = -m_sqrt(z1); ys[n++] = m_sqrt(z1); }
+      if (z2 >= 0.0) { ys[n++] = -m_sqrt(z2); ys[n++] = m_sqrt(z2); }
+    }
+  } else {
+    // resolvent: m^3 + P m^2 + (P^2/4 - S) m - Q^2/8 = 0, need m > 0
+    double rb = P, rc = P * P / 4.0 - S, rd = -Q * Q / 8.0;
+    double pp = rc - rb * rb / 3.0;
+    double qq = 2.0 * rb * rb * rb / 27.0 - rb * rc / 3.0 + rd;
+    double yv;
+    solve_cubic_depressed_largest(pp, qq, yv);
+    double m = yv - rb / 3.0;
+    // polish m on the resolvent
+    for (int it = 0; it < 4; ++it) {
+      double f = ((m + rb) * m + rc) * m + rd;
+      double fp = (3.0 * m + 2.0 * rb) * m + rc;
+      if (fp == 0.0) break;
+      double nm = m - f / fp;
+      if (!(nm == nm)) break;
+      m = nm;
+    }
+    if (m > 0.0) {
+      double sm = m_sqrt(2.0 * m);
+      double k1 = -(2.0 * P + 2.0 * m);
+      double k2 = 2.0 * Q / sm;
+      double d1 = k1 - k2, d2 = k1 + k2;
+      if (d1 >= 0.0) { double s1 = m_sqrt(d1); ys[n++] = (sm - s1) / 2.0; ys[n++] = (sm + s1) / 2.0; }
+      if (d2 >= 0.0) { double s2 = m_sqrt(d2); ys[n++] = (-sm - s2) / 2.0; ys[n++] = (-sm + s2) / 2.0; }
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    double x = ys[i] - a / 4.0;
+    for (int it = 0; it < 3; ++it) {  // Newton polish on the original quartic
+      double f = (((x + a) * x + b) * x + c) * x + e;
+      double fp = ((4.0 * x + 3.0 * a) * x + 2.0 * b) * x + c;
+      if (fp == 0.0) break;
+      double nx = x - f / fp;
+      if (!(nx == nx)) break;
+      x = nx;
+    }
+    roots[i] = x;
+  }
+  for (int i = 1; i < n; ++i)  // insertion sort
+    for (int j = i; j > 0 && roots[j] < roots[j - 1]; --j) {
+      double tmp = roots[j];
+      roots[j] = roots[j - 1];
+      roots[j - 1] = tmp;
+    }
+  return n;
+}
+
+// Torus, exact intersection: smallest root of the quartic
+// (|p|^2 + R^2 - r^2)^2 = 4 R^2 (x^2 + z^2) in [tmin, tmax] (FP64).
+LMN_HD bool hit_torus_exact(const double* p, V3<double> ro, V3<double> d, double tmin, double tmax, double& t) {
+  V3<double> o = ro - V3<double>{p[0], p[1], p[2]};
+  double R_ = p[3], r_ = p[4];
+  // normalise the direction so the quartic is well scaled, map t back after
+  double dl = length(d);
+  V3<double> dn = d / dl;
+  double od = dot(o, dn), oo = length_squared(o);
+  double k = oo + R_ * R_ - r_ * r_;
+  double R2 = R_ * R_;
+  // (s^2 + 2 od s + k)^2 - 4R^2((ox + s dx)^2 + (oz + s dz)^2)
+  double c3 = 4.0 * od;
+  double c2 = 4.0 * od * od + 2.0 * k - 4.0 * R2 * (dn.x * dn.x + dn.z * dn.z);
+  double c1 = 4.0 * od * k - 8.0 * R2 * (o.x * dn.x + o.z * dn.z);
+  double c0 = k * k - 4.0 * R2 * (o.x * o.x + o.z * o.z);
+  double roots[4];
+  int n = solve_quartic(c3, c2, c1, c0, roots);
+  for (int i = 0; i < n; ++i) {
+    double tt = roots[i] / dl;
+    if (tt >= tmin && tt <= tmax) {
+      t = tt;
+      return true;
+    }
+  }
+  return false;
+}
+
+LMN_HD double superellipsoid_f(V3<double> q, double e1, double e2) {
+  double ax = m_abs(q.x), ay = m_abs(q.y), az = m_abs(q.z);
+  double exp2 = 2.0 / e2;
+  double xz = pow(ax, exp2) + pow(az, exp2);
+  double exp1 = 2.0 / e1;
+  double val = pow(xz, e2 / e1) + pow(ay, exp1);
+  return pow(val, e1 / 2.0) - 1.0;
+}
+
+// Superellipsoid::hit, reference semantics (src/superellipsoid.rs:26-90)
+LMN_HD bool hit_superellipsoid_march(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax,
+                                     double& t) {
+  V3<double> o = {(ro.x - p[0]) / p[3], (ro.y - p[1]) / p[4], (ro.z - p[2]) / p[5]};
+  V3<double> d = {rd.x / p[3], rd.y / p[4], rd.z / p[5]};
+  double tc = tmin;
+  for (int i = 0; i < 256; ++i) {
+    if (tc > tmax) break;
+    double dist = superellipsoid_f(o + d * tc, p[6], p[7]);
+    if (m_abs(dist) < 1e-5) {
+      t = tc;
+      return true;
+    }
+    tc += fmax(dist, 1e-4);
+  }
+  return false;
+}
+
+// Superellipsoid, exact mode: the same implicit surface f = 0 bracketed by a
+// sign change of f along the ray (sampled at the reference's step rule) and
+// refined by bisection to 1e-12 relative, FP64.
+LMN_HD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax,
+                                     double& t) {
+  V3<double> o = {(ro.x - p[0]) / p[3], (ro.y - p[1]) / p[4], (ro.z - p[2]) / p[5]};
+  V3<double> d = {rd.x / p[3], rd.y / p[4], rd.z / p[5]};
+  // clip to the unit box |q|_inf <= 1 that contains the surface
+  double t0 = tmin, t1 = tmax;
+  const double* oo = &o.x;
+  const double* dd = &d.x;
+  for (int k = 0; k < 3; ++k) {
+    double inv = 1.0 / dd[k];
+    double a = (-1.0 - oo[k]) * inv, b = (1.0 - oo[k]) * inv;
+    if (a > b) { double tmp = a; a = b; b = tmp; }
+    t0 = fmax(t0, a);
+    t1 = fmin(t1, b);
+  }
+  if (!(t0 <= t1)) return false;
+  double dl = length(d);
+  int steps = 512;
+  double h = (t1 - t0) / steps;
+  double fa = superellipsoid_f(o + d * t0, p[6], p[7]);
+  if (fa <= 0.0) {  // starts inside: report the exit crossing like a surface hit
+    if (m_abs(fa) < 1e-12 && t0 >= tmin) { t = t0; return true; }
+  }
+  double ta = t0;
+  for (int i = 1; i <= steps; ++i) {
+    double tb = (i == steps) ? t1 : t0 + h * i;
+    double fb = superellipsoid_f(o + d * tb, p[6], p[7]);
+    if ((fa > 0.0) != (fb > 0.0)) {
+      double lo = ta, hi = tb, flo = fa;
+      for (int k = 0; k < 80 && (hi - lo) * dl > 1e-13 * (1.0 + m_abs(hi) * dl); ++k) {
+        double mid = 0.5 * (lo + hi);
+        double fm = superellipsoid_f(o + d * mid, p[6], p[7]);
+        if ((fm > 0.0) == (flo > 0.0)) { lo = mid; flo = fm; } else { hi = mid; }
+      }
+      double th = 0.5 * (lo + hi);
+      if (th >= tmin && th <= tmax) { t = th; return true; }
+    }
+    ta = tb;
+    fa = fb;
+  }
+  return false;
+}
+
+LMN_HD double spring_sdf(V3<double> q, const double* p) {
+  // src/spring.rs:41-73
+  double coil = p[3], tube = p[4], pitch = p[5], turns = p[6];
+  double height = pitch * turns;
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  double angle = m_atan2(q.z, q.x);
+  double base_y = pitch * angle / two_pi;
+  double best = 1.0 / 0.0;
+  int k_min = int(floor((q.y - base_y) / pitch)) - 1;
+  for (int k = k_min; k <= k_min + 3; ++k) {
+    double hy = base_y + pitch * double(k);
+    if (hy < -tube || hy > height + tube) continue;
+    double hyc = fmin(fmax(hy, 0.0), height);
+    double ha = (hyc / pitch) * two_pi;
+    double hx = coil * m_cos(ha), hz = coil * m_sin(ha);
+    double dx = q.x - hx, dy = q.y - hyc, dz = q.z - hz;
+    double dist = m_sqrt(dx * dx + dy * dy + dz * dz);
+    if (dist < best) best = dist;
+  }
+  return best - tube;
+}
+
+// Spring::hit, reference semantics (src/spring.rs:35-105)
+LMN_HD bool hit_spring_march(const double* p, V3<double> ro, V3<double> d, double tmin, double tmax, double& t) {
+  V3<double> o = ro - V3<double>{p[0], p[1], p[2]};
+  double step_min = p[4] * 0.3;
+  double tc = tmin;
+  for (int i = 0; i < 512; ++i) {
+    if (tc > tmax) break;
+    double dist = spring_sdf(o + d * tc, p);
+    if (dist < 1e-4) {
+      t = tc;
+      return true;
+    }
+    tc += fmax(dist, step_min);
+  }
+  return false;
+}
+
+// Spring, exact mode: sign change of the helix distance along the ray,
+// sampled at a tenth of the tube radius and refined by bisection (FP64).
+LMN_HD bool hit_spring_exact(const double* p, V3<double> ro, V3<double> d, double tmin, double tmax, double& t) {
+  V3<double> o = ro - V3<double>{p[0], p[1], p[2]};
+  double r = p[3] + p[4], height = p[5] * p[6];
+  double lo_b[3] = {-r, -p[4], -r}, hi_b[3] = {r, height + p[4], r};
+  const double* oo = &o.x;
+  const double* dd = &d.x;
+  double t0 = tmin, t1 = tmax;
+  for (int k = 0; k < 3; ++k) {
+    double inv = 1.0 / dd[k];
+    double a = (lo_b[k] - oo[k]) * inv, b = (hi_b[k] - oo[k]) * inv;
+    if (a > b) { double tmp = a; a = b; b = tmp; }
+    t0 = fmax(t0, a);
+    t1 = fmin(t1, b);
+  }
+  if (!(t0 <= t1)) return false;
+  double dl = length(d);
+  double h = 0.1 * p[4] / dl;
+  double ta = t0, fa = spring_sdf(o + d * ta, p);
+  if (fa <= 0.0) { t = t0; return true; }
+  for (int i = 0; i < 1 << 16 && ta < t1; ++i) {
+    double tb = fmin(ta + fmax(h, fa / dl), t1);  // the SDF is a distance bound
+    double fb = spring_sdf(o + d * tb, p);
+    if (fb <= 0.0) {
+      double lo = ta, hi = tb;
+      for (int k = 0; k < 80 && (hi - lo) * dl > 1e-13 * (1.0 + m_abs(hi) * dl); ++k) {
+        double mid = 0.5 * (lo + hi);
+        if (spring_sdf(o + d * mid, p) > 0.0) lo = mid; else hi = mid;
+      }
+      t = 0.5 * (lo + hi);
+      return true;
+    }
+    if (tb >= t1) break;
+    ta = tb;
+    fa = fb;
+  }
+  return false;
+}
+
+// uv of the marched primitives from the object-space point
+// (src/torus.rs:86-92, src/superellipsoid.rs:80-82, src/spring.rs:95-97)
+LMN_HD void marched_uv(uint32_t kind, const double* p, V3<double> wp, double& u, double& v) {
+  const double pi = 3.14159265358979323846;
+  if (kind == K_TORUS) {
+    V3<double> l = wp - V3<double>{p[0], p[1], p[2]};
+    double theta = m_atan2(l.z, l.x);
+    double xz = m_sqrt(l.x * l.x + l.z * l.z);
+    double phi = m_atan2(l.y, xz - p[3]);
+    u = theta / (2.0 * pi) + 0.5;
+    v = phi / (2.0 * pi) + 0.5;
+  } else if (kind == K_SUPERELLIPSOID) {
+    V3<double> l = {(wp.x - p[0]) / p[3], (wp.y - p[1]) / p[4], (wp.z - p[2]) / p[5]};
+    u = 0.5 + m_atan2(l.z, l.x) / (2.0 * pi);
+    v = 0.5 + clamp_r(asin(l.y), -1.0, 1.0) / pi;
+  } else {
+    V3<double> l = wp - V3<double>{p[0], p[1], p[2]};
+    u = l.y / (p[5] * p[6]);
+    v = m_atan2(l.z, l.x) / (2.0 * pi) + 0.5;
+  }
+}
+
+// Outward normal of the marched primitives by central differences
+// (src/torus.rs:79-84, src/superellipsoid.rs:69-78, src/spring.rs:88-93)
+LMN_HD V3<double> marched_normal(uint32_t kind, const double* p, V3<double> wp) {
+  if (kind == K_TORUS) {
+    V3<double> l = wp - V3<double>{p[0], p[1], p[2]};
+    const double e = 1e-5;
+    double nx = torus_sdf(l + V3<double>{e, 0, 0}, p[3], p[4]) - torus_sdf(l - V3<double>{e, 0, 0}, p[3], p[4]);
+    double ny = torus_sdf(l + V3<double>{0, e, 0}, p[3], p[4]) - torus_sdf(l - V3<double>{0, e, 0}, p[3], p[4]);
+    double nz = torus_sdf(l + V3<double>{0, 0, e}, p[3], p[4]) - torus_sdf(l - V3<double>{0, 0, e}, p[3], p[4]);
+    return unit(V3<double>{nx, ny, nz});
+  } else if (kind == K_SUPERELLIPSOID) {
+    V3<double> l = {(wp.x - p[0]) / p[3], (wp.y - p[1]) / p[4], (wp.z - p[2]) / p[5]};
+    const double e = 1e-5;
+    double nx = superellipsoid_f(l + V3<double>{e, 0, 0}, p[6], p[7]) - superellipsoid_f(l - V3<double>{e, 0, 0}, p[6], p[7]);
+    double ny = superellipsoid_f(l + V3<double>{0, e, 0}, p[6], p[7]) - superellipsoid_f(l - V3<double>{0, e, 0}, p[6], p[7]);
+    double nz = superellipsoid_f(l + V3<double>{0, 0, e}, p[6], p[7]) - superellipsoid_f(l - V3<double>{0, 0, e}, p[6], p[7]);
+    return unit(V3<double>{nx / p[3], ny / p[4], nz / p[5]});
+  } else {
+    V3<double> l = wp - V3<double>{p[0], p[1], p[2]};
+    const double e = 1e-4;
+    double nx = spring_sdf(l + V3<double>{e, 0, 0}, p) - spring_sdf(l - V3<double>{e, 0, 0}, p);
+    double ny = spring_sdf(l + V3<double>{0, e, 0}, p) - spring_sdf(l - V3<double>{0, e, 0}, p);
+    double nz = spring_sdf(l + V3<double>{0, 0, e}, p) - spring_sdf(l - V3<double>{0, 0, e}, p);
+    return unit(V3<double>{nx, ny, nz});
+  }
+}
+
+// Object-space intersection of one primitive in precision R.  Marched kinds
+// are always FP64 and must be called through hit_prim_fp64.
+template <class R>
+LMN_HD bool hit_prim_analytic(uint32_t kind, const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
+  switch (kind) {
+    case K_SPHERE: return hit_sphere(p, o, d, tmin, tmax, t, u, v);
+    case K_QUAD: return hit_quad(p, o, d, tmin, tmax, t, u, v);
+    case K_RECT_XY: return hit_rect(p, 2, 0, 1, o, d, tmin, tmax, t, u, v);
+    case K_RECT_XZ: return hit_rect(p, 1, 0, 2, o, d, tmin, tmax, t, u, v);
+    case K_RECT_YZ: return hit_rect(p, 0, 1, 2, o, d, tmin, tmax, t, u, v);
+    case K_TRIANGLE: return hit_triangle(p, o, d, tmin, tmax, t, u, v);
+    case K_PLANE: return hit_plane(p, o, d, tmin, tmax, t, u, v);
+    default: return false;
+  }
+}
+
+LMN_HD bool hit_prim_fp64(uint32_t kind, const double* p, V3<double> o, V3<double> d, double tmin, double tmax,
+                          bool exact, double& t, double& u, double& v) {
+  bool ok;
+  switch (kind) {
+    case K_TORUS: ok = exact ? hit_torus_exact(p, o, d, tmin, tmax, t) : hit_torus_march(p, o, d, tmin, tmax, t); break;
+    case K_SUPERELLIPSOID:
+      ok = exact ? hit_superellipsoid_exact(p, o, d, tmin, tmax, t) : hit_superellipsoid_march(p, o, d, tmin, tmax, t);
+      break;
+    case K_SPRING: ok = exact ? hit_spring_exact(p, o, d, tmin, tmax, t) : hit_spring_march(p, o, d, tmin, tmax, t); break;
+    default: return hit_prim_analytic<double>(kind, p, o, d, tmin, tmax, t, u, v);
+  }
+  if (ok) marched_uv(kind, p, o + d * t, u, v);
+  return ok;
+}
+
+// Exact torus normal: gradient of (|p|^2 + R^2 - r^2)^2 - 4R^2(x^2+z^2).
+LMN_HD V3<double> torus_exact_normal(const double* p, V3<double> wp) {
+  V3<double> l = wp - V3<double>{p[0], p[1], p[2]};
+  double k = length_squared(l) + p[3] * p[3] - p[4] * p[4];
+  double R2 = p[3] * p[3];
+  return unit(V3<double>{4.0 * k * l.x - 8.0 * R2 * l.x, 4.0 * k * l.y, 4.0 * k * l.z - 8.0 * R2 * l.z});
+}
+
+// Outward normal (object space) at object-space point q.
+template <class R>
+LMN_HD V3<R> prim_normal_analytic(uint32_t kind, const R* p, V3<R> q) {
+  switch (kind) {
+    case K_SPHERE: return (q - V3<R>{p[0], p[1], p[2]}) / p[3];
+    case K_QUAD: return unit(V3<R>{p[9], p[10], p[11]});
+    case K_RECT_XY: return {R(0), R(0), R(1)};
+    case K_RECT_XZ: return {R(0), R(1), R(0)};
+    case K_RECT_YZ: return {R(1), R(0), R(0)};
+    case K_TRIANGLE: return unit(cross(V3<R>{p[3], p[4], p[5]}, V3<R>{p[6], p[7], p[8]}));
+    case K_PLANE: return {p[3], p[4], p[5]};
+    default: return {R(0), R(1), R(0)};
+  }
+}
+
+LMN_HD V3<double> prim_normal_fp64(uint32_t kind, const double* p, V3<double> q, bool exact) {
+  if (kind == K_TORUS) return exact ? torus_exact_normal(p, q) : marched_normal(kind, p, q);
+  if (kind == K_SUPERELLIPSOID || kind == K_SPRING) return marched_normal(kind, p, q);
+  return prim_normal_analytic<double>(kind, p, q);
+}
+
+}  // namespace lmn

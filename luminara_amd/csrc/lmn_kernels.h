@@ -1,0 +1,67 @@
+// Kernel-side data structures of the wavefront and the launch entry points
+// used by the C ABI (lmn_api.hip).
+#pragma once
+#include "../../include/luminara.h"
+#include "lmn_render.h"
+
+namespace lmn {
+
+// Path pool, structure of arrays (16 B per path per field):
+//   ob   = origin.xyz, beta.x     db   = direction.xyz, beta.y
+//   meta = work item, remaining depth | skip-emission flag << 16,
+//          rng key lo, rng key hi
+//   aux  = rng counter, beta.z (bit pattern), 2 spare words
+struct PathSoA {
+  float4* ob;
+  float4* db;
+  uint4* meta;
+  uint4* aux;
+};
+// Shadow-ray queue: o = origin.xyz, distance; d = unit dir.xyz, work item;
+// c = unoccluded contribution.rgb
+struct ShadowSoA {
+  float4* o;
+  float4* d;
+  float4* c;
+};
+
+struct WaveCounters {
+  uint32_t active;       // paths in the current pool
+  uint32_t next_active;  // survivors compacted into the next pool
+  uint32_t n_shadow;     // shadow rays queued this iteration
+  uint32_t pad;
+  uint64_t next_item;    // next work item to issue (chunk relative)
+  uint64_t n_items;      // work items in the chunk
+  uint64_t stat_extend;  // closest-hit queries issued
+  uint64_t stat_shadow;  // any-hit queries issued
+  uint64_t stat_camera;  // camera rays generated
+  uint64_t iterations;
+};
+
+struct WaveLaunch {
+  hipStream_t stream;
+  uint32_t grid;       // workgroups of 256 lanes (grid-stride kernels)
+  uint32_t pool;       // path pool size
+  uint32_t stack_lds;  // LDS short-stack entries per lane
+  float4* hit;
+  ShadowSoA sh;
+  float4* samples;
+  WaveCounters* ctr;
+  uint32_t* spill;
+  TraceCounters* tc;
+};
+
+void launch_wave_iteration(const WaveLaunch& L, const SceneView<float>& S, const FrameDev& F, const PathSoA& cur,
+                           const PathSoA& nxt, bool count);
+void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s);
+void launch_resolve(const float4* samples, double4* acc, uint32_t npix, uint32_t n_samples, hipStream_t s);
+void launch_zero_acc(double4* acc, uint32_t npix, hipStream_t s);
+void launch_finalize(const double4* acc, float* out, uint32_t npix, hipStream_t s);
+void launch_plain_fp64(const SceneView<double>& S, const FrameDev& F, double4* acc, uint32_t s0, uint32_t s1,
+                       unsigned long long* rays, hipStream_t s);
+void launch_plain_fp32(const SceneView<float>& S, const FrameDev& F, double4* acc, uint32_t s0, uint32_t s1,
+                       unsigned long long* rays, hipStream_t s);
+void launch_trace_fp32(const SceneView<float>& S, const lmn_ray* rays, lmn_hit* hits, size_t n, hipStream_t s);
+void launch_trace_fp64(const SceneView<double>& S, const lmn_ray* rays, lmn_hit* hits, size_t n, hipStream_t s);
+
+}  // namespace lmn

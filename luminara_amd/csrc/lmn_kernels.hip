@@ -1,0 +1,328 @@
+// HIP kernels of the render hot path for gfx950 (CDNA4, wave64):
+//  * wavefront FP32 path tracer: extend (closest hit, LDS short stack),
+//    shade (materials, emission, Russian roulette, NEE sampling, wave-ballot
+//    compaction of survivors and shadow rays), shadow (any hit), regenerate
+//    (camera rays for new work items into freed pool slots), bookkeeping;
+//  * plain one-path-per-thread megakernel in FP64 (or FP32);
+//  * resolve: per-pixel luminance clamp, filter weights and FP64 sums;
+//  * ray-batch trace for parity.
+// The estimator is the reference's ray_color / pixel loop
+// (src/render.rs:1472-1537, :1587-1747), unrolled into an iteration.
+#include "lmn_kernels.h"
+
+namespace lmn {
+
+// ---------------------------------------------------------------------------
+// LDS short stack with spill to a global per-thread region.  The first SS
+// entries of each thread live in LDS ([SS][BS] layout, bank-conflict free);
+// deeper entries go to `spill` ([depth][threads] layout, coalesced).
+template <int SS, int BS>
+struct LdsStack {
+  uint32_t* lds;
+  uint32_t* spill;
+  uint32_t tid;
+  uint32_t slot;
+  uint32_t nthreads;
+  int sp;
+  __device__ void push(uint32_t v) {
+    if (sp < SS)
+      lds[sp * BS + tid] = v;
+    else
+      spill[size_t(sp - SS) * nthreads + slot] = v;
+    ++sp;
+  }
+  __device__ uint32_t pop() {
+    --sp;
+    return sp < SS ? lds[sp * BS + tid] : spill[size_t(sp - SS) * nthreads + slot];
+  }
+  __device__ bool empty() const { return sp == 0; }
+};
+
+__device__ __forceinline__ V3<float> f3(float4 a) { return {a.x, a.y, a.z}; }
+
+// wave-aggregated slot reservation: one atomic per wave
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, bool want) {
+  unsigned long long active = __ballot(1);
+  unsigned long long m = __ballot(want);
+  uint32_t leader = uint32_t(__ffsll((long long)active) - 1);
+  uint32_t base = 0;
+  if (__lane_id() == leader && m) base = atomicAdd(counter, uint32_t(__popcll(m)));
+  base = __shfl(base, int(leader));
+  return base + uint32_t(__popcll(m & __lanemask_lt()));
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront stages
+
+template <int SS, int BS, bool COUNT>
+__global__ __launch_bounds__(BS) void k_extend(SceneView<float> S, PathSoA cur, float4* __restrict__ hit,
+                                               const WaveCounters* __restrict__ ctr, uint32_t* spill,
+                                               TraceCounters* tc) {
+  __shared__ uint32_t lds[SS * BS];
+  uint32_t n = ctr->active;
+  uint32_t nthreads = gridDim.x * BS;
+  uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  for (uint32_t i = gid; i < n; i += nthreads) {
+    float4 a = cur.ob[i], b = cur.db[i];
+    LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
+    float t, u, v;
+    uint32_t idx;
+    bool h = trace<float, false>(S, f3(a), f3(b), 0.001f, INFINITY, t, idx, u, v, st, COUNT ? tc : nullptr);
+    hit[i] = make_float4(t, u, v, __uint_as_float(h ? idx : 0xffffffffu));
+  }
+}
+
+template <int SS, int BS>
+__global__ __launch_bounds__(BS) void k_shadow(SceneView<float> S, ShadowSoA sh, float4* __restrict__ samples,
+                                               const WaveCounters* __restrict__ ctr, uint32_t* spill) {
+  __shared__ uint32_t lds[SS * BS];
+  uint32_t n = ctr->n_shadow;
+  uint32_t nthreads = gridDim.x * BS;
+  uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  for (uint32_t i = gid; i < n; i += nthreads) {
+    float4 o = sh.o[i], d = sh.d[i];
+    LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
+    float t, u, v;
+    uint32_t idx;
+    // shadow query over [0.001, dist - 0.001] (src/render.rs:1457-1458)
+    bool occluded = trace<float, true>(S, f3(o), f3(d), 0.001f, o.w - 0.001f, t, idx, u, v, st);
+    if (!occluded) {
+      float4 c = sh.c[i];
+      uint32_t item = __float_as_uint(d.w);
+      float4 s = samples[item];
+      s.x += c.x;
+      s.y += c.y;
+      s.z += c.z;
+      samples[item] = s;
+    }
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_shade(SceneView<float> S, FrameDev F, PathSoA cur, const float4* __restrict__ hit,
+                                              PathSoA nxt, ShadowSoA sh, float4* __restrict__ samples,
+                                              WaveCounters* __restrict__ ctr) {
+  uint32_t n = ctr->active;
+  uint32_t nthreads = gridDim.x * BS;
+  for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < n; i += nthreads) {
+    float4 A = cur.ob[i], B = cur.db[i];
+    uint4 M = cur.meta[i], X = cur.aux[i];
+    V3<float> o = f3(A), d = f3(B), beta = {A.w, B.w, __uint_as_float(X.y)};
+    Rng rng;
+    rng.key = (uint64_t(M.w) << 32) | M.z;
+    rng.n = X.x;
+    uint32_t item = M.x, depth = M.y & 0xffffu;
+    bool skip = (M.y >> 16) & 1u;
+    float4 H = hit[i];
+    uint32_t prim = __float_as_uint(H.w);
+    V3<float> add = {0.f, 0.f, 0.f};
+    bool cont = false, emit = false;
+    V3<float> sp_o, sp_d, ndir, nbeta, cb;
+    float sp_dist = 0.f;
+    bool use_nee = false;
+    if (prim == 0xffffffffu) {
+      add = hadamard(beta, background(S.bg, d));
+    } else {
+      Surf<float> s;
+      surface(S, prim, o, d, H.x, H.y, H.z, s);
+      const MaterialT<float>& m = S.materials[s.mat];
+      V3<float> em = skip ? V3<float>{0.f, 0.f, 0.f} : emitted(S, m, s.u, s.v);
+      V3<float> att;
+      if (!scatter(S, m, d, s, rng, ndir, att)) {
+        add = hadamard(beta, em);
+      } else {
+        // Russian roulette on remaining depth < 47 (src/render.rs:1494-1502)
+        float surv = clamp_r(max3(att), 0.05f, 1.0f);
+        bool rr = depth < 47;
+        if (rr && rng.next<float>() > surv) {
+          add = hadamard(beta, em);
+        } else {
+          float rrw = rr ? 1.0f / surv : 1.0f;
+          use_nee = !is_specular(m) && S.n_lights > 0;
+          V3<float> c = {0.f, 0.f, 0.f};
+          bool nee_ok = false;
+          if (use_nee) nee_ok = nee_sample(S, s.p, s.n, att, rng, sp_d, sp_dist, c);
+          V3<float> attw = att * rrw;
+          // NaN guard of one recursion level (src/render.rs:1526-1531)
+          if (!(has_nan(em) || (nee_ok && has_nan(c)) || has_nan(attw))) {
+            add = hadamard(beta, em);
+            if (nee_ok) {
+              emit = true;
+              sp_o = s.p;
+              cb = hadamard(beta, c);
+            }
+            nbeta = hadamard(beta, attw);
+            cont = depth > 1;
+          }
+          sp_o = s.p;
+        }
+      }
+    }
+    if (add.x != 0.f || add.y != 0.f || add.z != 0.f) {
+      float4 sm = samples[item];
+      sm.x += add.x;
+      sm.y += add.y;
+      sm.z += add.z;
+      samples[item] = sm;
+    }
+    uint32_t ps = wave_reserve(&ctr->n_shadow, emit);
+    if (emit) {
+      sh.o[ps] = make_float4(sp_o.x, sp_o.y, sp_o.z, sp_dist);
+      sh.d[ps] = make_float4(sp_d.x, sp_d.y, sp_d.z, __uint_as_float(item));
+      sh.c[ps] = make_float4(cb.x, cb.y, cb.z, 0.f);
+    }
+    uint32_t pn = wave_reserve(&ctr->next_active, cont);
+    if (cont) {
+      nxt.ob[pn] = make_float4(sp_o.x, sp_o.y, sp_o.z, nbeta.x);
+      nxt.db[pn] = make_float4(ndir.x, ndir.y, ndir.z, nbeta.y);
+      nxt.meta[pn] = make_uint4(item, (depth - 1) | (use_nee ? 0x10000u : 0u), M.z, M.w);
+      nxt.aux[pn] = make_uint4(rng.n, __float_as_uint(nbeta.z), 0u, 0u);
+    }
+  }
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_regen(SceneView<float> S, FrameDev F, PathSoA nxt, float4* __restrict__ samples,
+                                              const WaveCounters* __restrict__ ctr, uint32_t pool) {
+  uint32_t na = ctr->next_active;
+  uint64_t left = ctr->n_items - ctr->next_item;
+  uint32_t room = pool - na;
+  uint32_t m = left < room ? uint32_t(left) : room;
+  uint32_t nthreads = gridDim.x * BS;
+  for (uint32_t t = blockIdx.x * BS + threadIdx.x; t < m; t += nthreads) {
+    uint64_t item = ctr->next_item + t;
+    Rng rng;
+    V3<float> o, d;
+    float w;
+    uint32_t pixel;
+    generate_item<float>(F, S.cam, F.item_base + item, rng, o, d, w, pixel);
+    samples[item] = make_float4(0.f, 0.f, 0.f, w);
+    uint32_t slot = na + t;
+    nxt.ob[slot] = make_float4(o.x, o.y, o.z, 1.f);
+    nxt.db[slot] = make_float4(d.x, d.y, d.z, 1.f);
+    nxt.meta[slot] = make_uint4(uint32_t(item), F.max_depth, uint32_t(rng.key), uint32_t(rng.key >> 32));
+    nxt.aux[slot] = make_uint4(rng.n, __float_as_uint(1.f), 0u, 0u);
+  }
+}
+
+__global__ void k_book(WaveCounters* ctr, uint32_t pool) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint32_t na = ctr->next_active;
+  uint64_t left = ctr->n_items - ctr->next_item;
+  uint32_t room = pool - na;
+  uint32_t m = left < room ? uint32_t(left) : room;
+  ctr->stat_extend += ctr->active;
+  ctr->stat_shadow += ctr->n_shadow;
+  ctr->stat_camera += m;
+  ctr->active = na + m;
+  ctr->next_item += m;
+  ctr->next_active = 0;
+  ctr->n_shadow = 0;
+  ctr->iterations += 1;
+}
+
+__global__ void k_counters_init(WaveCounters* ctr, uint64_t n_items) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  ctr->active = 0;
+  ctr->next_active = 0;
+  ctr->n_shadow = 0;
+  ctr->pad = 0;
+  ctr->next_item = 0;
+  ctr->n_items = n_items;
+  ctr->stat_extend = 0;
+  ctr->stat_shadow = 0;
+  ctr->stat_camera = 0;
+  ctr->iterations = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Resolve: per-sample luminance clamp (src/render.rs:1691-1697) and filter
+// weighting, accumulated per pixel in FP64 in sample order.
+__global__ void k_resolve(const float4* __restrict__ samples, double4* __restrict__ acc, uint32_t npix,
+                          uint32_t n_samples) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  double4 a = acc[p];
+  for (uint32_t s = 0; s < n_samples; ++s) {
+    float4 v = samples[size_t(s) * npix + p];
+    double x = v.x, y = v.y, z = v.z, w = v.w;
+    double lum = 0.2126 * x + 0.7152 * y + 0.0722 * z;
+    if (lum > 100.0) {
+      double f = 100.0 / lum;
+      x *= f;
+      y *= f;
+      z *= f;
+    }
+    a.x += x * w;
+    a.y += y * w;
+    a.z += z * w;
+    a.w += w;
+  }
+  acc[p] = a;
+}
+
+__global__ void k_zero_acc(double4* acc, uint32_t npix) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < npix) acc[p] = make_double4(0, 0, 0, 0);
+}
+
+__global__ void k_finalize(const double4* __restrict__ acc, float* __restrict__ out, uint32_t npix) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  double4 a = acc[p];
+  double inv = a.w > 0.0 ? 1.0 / a.w : 1.0;
+  out[size_t(p) * 3 + 0] = float(a.x * inv);
+  out[size_t(p) * 3 + 1] = float(a.y * inv);
+  out[size_t(p) * 3 + 2] = float(a.z * inv);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+
+template <int SS>
+static void launch_extend_ss(const WaveLaunch& L, const SceneView<float>& S, const PathSoA& cur, bool count) {
+  constexpr int BS = 256;
+  if (count)
+    hipLaunchKernelGGL((k_extend<SS, BS, true>), dim3(L.grid), dim3(BS), 0, L.stream, S, cur, L.hit, L.ctr, L.spill, L.tc);
+  else
+    hipLaunchKernelGGL((k_extend<SS, BS, false>), dim3(L.grid), dim3(BS), 0, L.stream, S, cur, L.hit, L.ctr, L.spill, L.tc);
+}
+template <int SS>
+static void launch_shadow_ss(const WaveLaunch& L, const SceneView<float>& S) {
+  constexpr int BS = 256;
+  hipLaunchKernelGGL((k_shadow<SS, BS>), dim3(L.grid), dim3(BS), 0, L.stream, S, L.sh, L.samples, L.ctr, L.spill);
+}
+
+void launch_wave_iteration(const WaveLaunch& L, const SceneView<float>& S, const FrameDev& F, const PathSoA& cur,
+                           const PathSoA& nxt, bool count) {
+  constexpr int BS = 256;
+  switch (L.stack_lds) {
+    case 8: launch_extend_ss<8>(L, S, cur, count); break;
+    case 24: launch_extend_ss<24>(L, S, cur, count); break;
+    case 32: launch_extend_ss<32>(L, S, cur, count); break;
+    default: launch_extend_ss<16>(L, S, cur, count); break;
+  }
+  hipLaunchKernelGGL((k_shade<BS>), dim3(L.grid), dim3(BS), 0, L.stream, S, F, cur, L.hit, nxt, L.sh, L.samples, L.ctr);
+  switch (L.stack_lds) {
+    case 8: launch_shadow_ss<8>(L, S); break;
+    case 24: launch_shadow_ss<24>(L, S); break;
+    case 32: launch_shadow_ss<32>(L, S); break;
+    default: launch_shadow_ss<16>(L, S); break;
+  }
+  hipLaunchKernelGGL((k_regen<BS>), dim3(L.grid), dim3(BS), 0, L.stream, S, F, nxt, L.samples, L.ctr, L.pool);
+  hipLaunchKernelGGL(k_book, dim3(1), dim3(64), 0, L.stream, L.ctr, L.pool);
+}
+
+void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s) {
+  hipLaunchKernelGGL(k_counters_init, dim3(1), dim3(64), 0, s, ctr, n_items);
+}
+void launch_resolve(const float4* samples, double4* acc, uint32_t npix, uint32_t n_samples, hipStream_t s) {
+  hipLaunchKernelGGL(k_resolve, dim3((npix + 255) / 256), dim3(256), 0, s, samples, acc, npix, n_samples);
+}
+void launch_zero_acc(double4* acc, uint32_t npix, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_acc, dim3((npix + 255) / 256), dim3(256), 0, s, acc, npix);
+}
+void launch_finalize(const double4* acc, float* out, uint32_t npix, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3((npix + 255) / 256), dim3(256), 0, s, acc, out, npix);
+}
+}  // namespace lmn

@@ -1,0 +1,147 @@
+// Plain one-path-per-thread megakernels (FP64 baseline and FP32) and the
+// ray-batch trace kernel of the parity harness.
+#include "lmn_kernels.h"
+
+namespace lmn {
+
+// ---------------------------------------------------------------------------
+// Plain megakernel: one thread per pixel, paths run to completion one after
+// another, private-memory traversal stack, FP64 (or FP32) throughout.
+template <class R>
+__global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, double4* __restrict__ acc,
+                                               uint32_t s_begin, uint32_t s_end, unsigned long long* rays) {
+  uint32_t npix = F.crop_w * F.crop_h;
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  double4 a = acc[p];
+  unsigned long long n_ext = 0, n_sh = 0;
+  for (uint32_t s = s_begin; s < s_end; ++s) {
+    Rng rng;
+    V3<R> o, d;
+    R w;
+    uint32_t pix;
+    generate_item<R>(F, S.cam, uint64_t(s) * npix + p, rng, o, d, w, pix);
+    V3<R> L = {R(0), R(0), R(0)}, beta = {R(1), R(1), R(1)};
+    bool skip = false;
+    for (uint32_t depth = F.max_depth; depth > 0; --depth) {
+      PrivateStack st;
+      R t, u, v;
+      uint32_t idx;
+      ++n_ext;
+      if (!trace<R, false>(S, o, d, R(0.001), R(INFINITY), t, idx, u, v, st)) {
+        L = L + hadamard(beta, background(S.bg, d));
+        break;
+      }
+      Surf<R> sf;
+      surface(S, idx, o, d, t, u, v, sf);
+      const MaterialT<R>& m = S.materials[sf.mat];
+      V3<R> em = skip ? V3<R>{R(0), R(0), R(0)} : emitted(S, m, sf.u, sf.v);
+      V3<R> ndir, att;
+      if (!scatter(S, m, d, sf, rng, ndir, att)) {
+        L = L + hadamard(beta, em);
+        break;
+      }
+      R surv = clamp_r(max3(att), R(0.05), R(1));
+      bool rr = depth < 47;
+      if (rr && rng.next<R>() > surv) {
+        L = L + hadamard(beta, em);
+        break;
+      }
+      R rrw = rr ? R(1) / surv : R(1);
+      bool use_nee = !is_specular(m) && S.n_lights > 0;
+      V3<R> direct = {R(0), R(0), R(0)};
+      bool nee_nan = false;
+      if (use_nee) {
+        V3<R> ld, c;
+        R dist;
+        if (nee_sample(S, sf.p, sf.n, att, rng, ld, dist, c)) {
+          nee_nan = has_nan(c);
+          PrivateStack st2;
+          R t2, u2, v2;
+          uint32_t i2;
+          ++n_sh;
+          if (!trace<R, true>(S, sf.p, ld, R(0.001), dist - R(0.001), t2, i2, u2, v2, st2)) direct = c;
+        }
+      }
+      V3<R> attw = att * rrw;
+      if (has_nan(em) || nee_nan || has_nan(attw)) break;  // level result NaN -> 0
+      L = L + hadamard(beta, em + direct);
+      beta = hadamard(beta, attw);
+      o = sf.p;
+      d = ndir;
+      skip = use_nee;
+    }
+    double x = double(L.x), y = double(L.y), z = double(L.z);
+    double lum = 0.2126 * x + 0.7152 * y + 0.0722 * z;
+    if (lum > 100.0) {
+      double f = 100.0 / lum;
+      x *= f;
+      y *= f;
+      z *= f;
+    }
+    double ww = double(w);
+    a.x += x * ww;
+    a.y += y * ww;
+    a.z += z * ww;
+    a.w += ww;
+  }
+  acc[p] = a;
+  if (rays) {
+    atomicAdd(&rays[0], n_ext);
+    atomicAdd(&rays[1], n_sh);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ray-batch closest hit (parity harness)
+template <class R>
+__global__ void k_trace(SceneView<R> S, const lmn_ray* __restrict__ rays, lmn_hit* __restrict__ hits, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const lmn_ray& r = rays[i];
+  V3<R> o = {R(r.origin[0]), R(r.origin[1]), R(r.origin[2])};
+  V3<R> d = {R(r.direction[0]), R(r.direction[1]), R(r.direction[2])};
+  PrivateStack st;
+  R t, u, v;
+  uint32_t idx;
+  lmn_hit h;
+  h.pad = 0;
+  if (trace<R, false>(S, o, d, R(r.t_min), R(r.t_max), t, idx, u, v, st)) {
+    Surf<R> sf;
+    surface(S, idx, o, d, t, u, v, sf);
+    h.t = double(t);
+    h.prim = int64_t(sf.prim_id);
+    h.u = double(sf.u);
+    h.v = double(sf.v);
+    h.normal[0] = double(sf.n.x);
+    h.normal[1] = double(sf.n.y);
+    h.normal[2] = double(sf.n.z);
+    h.front_face = sf.front ? 1 : 0;
+  } else {
+    h.t = 0;
+    h.prim = -1;
+    h.u = h.v = 0;
+    h.normal[0] = h.normal[1] = h.normal[2] = 0;
+    h.front_face = 0;
+  }
+  hits[i] = h;
+}
+
+void launch_plain_fp64(const SceneView<double>& S, const FrameDev& F, double4* acc, uint32_t s0, uint32_t s1,
+                       unsigned long long* rays, hipStream_t s) {
+  uint32_t npix = F.crop_w * F.crop_h;
+  hipLaunchKernelGGL((k_plain<double>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, acc, s0, s1, rays);
+}
+void launch_plain_fp32(const SceneView<float>& S, const FrameDev& F, double4* acc, uint32_t s0, uint32_t s1,
+                       unsigned long long* rays, hipStream_t s) {
+  uint32_t npix = F.crop_w * F.crop_h;
+  hipLaunchKernelGGL((k_plain<float>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, acc, s0, s1, rays);
+}
+void launch_trace_fp32(const SceneView<float>& S, const lmn_ray* rays, lmn_hit* hits, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL((k_trace<float>), dim3((n + 127) / 128), dim3(128), 0, s, S, rays, hits, n);
+}
+void launch_trace_fp64(const SceneView<double>& S, const lmn_ray* rays, lmn_hit* hits, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL((k_trace<double>), dim3((n + 127) / 128), dim3(128), 0, s, S, rays, hits, n);
+}
+
+}  // namespace lmn

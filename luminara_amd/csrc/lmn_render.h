@@ -1,0 +1,683 @@
+// Device-side scene view, BVH traversal, surface reconstruction, materials,
+// NEE light sampling, background and camera -- the per-ray work of the
+// reference's integrator (src/render.rs:72-172, :1333-1537), in FP32 or FP64.
+#pragma once
+#include "lmn_geom.h"
+
+namespace lmn {
+
+enum : uint32_t { M_LAMBERTIAN = 0, M_METAL = 1, M_DIELECTRIC = 2, M_EMISSIVE = 3, M_MICROFACET = 4 };
+enum : uint32_t { L_SPHERE = 0, L_RECT = 1, L_DISK = 2 };
+enum : uint32_t { BG_SKY = 0, BG_SOLID = 1, BG_GRADIENT = 2, BG_SUN = 3 };
+
+template <class R>
+struct MaterialT {
+  uint32_t kind;
+  int32_t tex;
+  R color[3];
+  R fuzz, ior, roughness, dispersion, intensity, metallic;
+  uint32_t param_slot;  // roughness parameter index for the adjoint, or ~0
+  uint32_t pad;
+};
+
+struct TextureDev {
+  uint32_t w, h;
+  uint64_t offset;  // float offset into the texel pool (RGB triples)
+};
+
+template <class R>
+struct LightT {
+  uint32_t kind, pad;
+  V3<R> origin, u, v, normal, emission;
+  R radius, area;
+};
+
+template <class R>
+struct BackgroundT {
+  uint32_t kind, pad;
+  V3<R> c0, c1, dir_unit;
+  R intensity;
+};
+
+template <class R>
+struct CameraT {
+  V3<R> origin, lower_left, horizontal, vertical, u, v;
+  R lens_radius;
+};
+
+template <class R>
+struct SceneView {
+  const typename NodeSel<R>::type* nodes;
+  const PrimT<R>* prims;          // bounded prims (leaf order) then planes
+  const PrimT<double>* prims64;   // FP64 copies, same order (FP64 islands)
+  const XformT<R>* xforms;
+  const XformT<double>* xforms64;
+  const MaterialT<R>* materials;
+  const TextureDev* textures;
+  const float* texels;
+  const LightT<R>* lights;
+  uint32_t n_nodes, n_bounded, n_planes, n_lights;
+  BackgroundT<R> bg;
+  CameraT<R> cam;
+  uint32_t exact_marched;  // LMN_ISECT_EXACT for marched prims
+  uint32_t pad;
+};
+
+// Traversal statistics of an instrumented launch (bytes per unit of work).
+struct TraceCounters {
+  unsigned long long node_fetches;
+  unsigned long long prim_tests;
+  unsigned long long tri_tests;
+  unsigned long long rays;
+};
+
+// Stack of node indices in per-thread private memory (plain kernels).
+struct PrivateStack {
+  uint32_t s[64];
+  int sp = 0;
+  LMN_HD void push(uint32_t v) { s[sp++] = v; }
+  LMN_HD uint32_t pop() { return s[--sp]; }
+  LMN_HD bool empty() const { return sp == 0; }
+};
+
+template <class R>
+LMN_HD bool slab(const float* lo, const float* hi, V3<R> o, V3<R> inv, R tmin, R tmax, R& tenter);
+// Slab test of a node box (src/aabb.rs:18-56), inclusive of grazing contact,
+// reciprocal hoisted per ray.  NaN slab bounds are ignored (f64::max/min).
+template <>
+LMN_HD bool slab<float>(const float* lo, const float* hi, V3<float> o, V3<float> inv, float tmin, float tmax,
+                        float& tenter) {
+  float tx0 = (lo[0] - o.x) * inv.x, tx1 = (hi[0] - o.x) * inv.x;
+  float ty0 = (lo[1] - o.y) * inv.y, ty1 = (hi[1] - o.y) * inv.y;
+  float tz0 = (lo[2] - o.z) * inv.z, tz1 = (hi[2] - o.z) * inv.z;
+  float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+  float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+  tenter = tn;
+  return tn <= tf;
+}
+LMN_HD bool slab_d(const double* lo, const double* hi, V3<double> o, V3<double> inv, double tmin, double tmax,
+                   double& tenter) {
+  double tx0 = (lo[0] - o.x) * inv.x, tx1 = (hi[0] - o.x) * inv.x;
+  double ty0 = (lo[1] - o.y) * inv.y, ty1 = (hi[1] - o.y) * inv.y;
+  double tz0 = (lo[2] - o.z) * inv.z, tz1 = (hi[2] - o.z) * inv.z;
+  double tn = fmax(fmax(fmin(tx0, tx1), fmin(ty0, ty1)), fmax(fmin(tz0, tz1), tmin));
+  double tf = fmin(fmin(fmax(tx0, tx1), fmax(ty0, ty1)), fmin(fmax(tz0, tz1), tmax));
+  tenter = tn;
+  return tn <= tf;
+}
+template <class R> struct NodeOps;
+template <> struct NodeOps<float> {
+  LMN_HD static bool test(const NodeF& n, V3<float> o, V3<float> inv, float tmin, float tmax, float& te) {
+    return slab<float>(n.lo, n.hi, o, inv, tmin, tmax, te);
+  }
+};
+template <> struct NodeOps<double> {
+  LMN_HD static bool test(const NodeD& n, V3<double> o, V3<double> inv, double tmin, double tmax, double& te) {
+    return slab_d(n.lo, n.hi, o, inv, tmin, tmax, te);
+  }
+};
+
+// Intersects primitive `idx` (transform applied); on a hit closer than
+// `closest`, updates closest (reference-reported t), u, v.
+template <class R>
+LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R tmin, R& closest, R& hu, R& hv) {
+  const PrimT<R>& pr = S.prims[idx];
+  uint32_t km = pr.kind_mat;
+  uint32_t kind = km & KIND_MASK;
+  // FP64 islands on the FP32 path, and the marched kinds in either precision
+  bool is64 = (km & FLAG_FP64) != 0 || kind >= K_TORUS;
+  R t, u, v;
+  bool hit;
+  if (pr.xform == XF_NONE) {
+    if (is64) {
+      double t64, u64, v64;
+      hit = hit_prim_fp64(kind, S.prims64[idx].p, cvt<double>(o), cvt<double>(d), double(tmin), double(closest),
+                          S.exact_marched != 0, t64, u64, v64);
+      t = R(t64);
+      u = R(u64);
+      v = R(v64);
+      if (hit && !(t <= closest)) hit = false;  // rounding back to R must not leave the range
+    } else {
+      hit = hit_prim_analytic<R>(kind, pr.p, o, d, tmin, closest, t, u, v);
+    }
+    if (hit) {
+      closest = t;
+      hu = u;
+      hv = v;
+    }
+    return hit;
+  }
+  const XformT<R>& X = S.xforms[pr.xform];
+  V3<R> oo = xf_point(X.inv, o), dd = xf_dir(X.inv, d);
+  if (is64) {
+    const XformT<double>& X64 = S.xforms64[pr.xform];
+    V3<double> o64 = xf_point(X64.inv, cvt<double>(o)), d64 = xf_dir(X64.inv, cvt<double>(d));
+    double t64, u64, v64;
+    hit = hit_prim_fp64(kind, S.prims64[idx].p, o64, d64, double(tmin), double(closest), S.exact_marched != 0, t64,
+                        u64, v64);
+    t = R(t64);
+    u = R(u64);
+    v = R(v64);
+    if (hit && !(t <= closest)) hit = false;
+  } else {
+    hit = hit_prim_analytic<R>(kind, pr.p, oo, dd, tmin, closest, t, u, v);
+  }
+  if (!hit) return false;
+  if (X.tmode != 0) {
+    R tr = t * t_report_factor(X, dd, d);
+    if (!(tr <= closest)) return false;
+    t = tr;
+  }
+  closest = t;
+  hu = u;
+  hv = v;
+  return true;
+}
+
+// Closest hit over the BVH then the unbounded list (SceneWorld::hit,
+// src/scene.rs:817-834).  ANY = stop at the first hit in range (NEE
+// visibility only asks is_some(), src/render.rs:1458).  Ordered BVH2
+// traversal: both children boxes are tested, the nearer is visited first and
+// the farther pushed; a popped node is re-tested against the current
+// closest hit before it is expanded.
+template <class R, bool ANY, class Stack>
+LMN_HD bool trace(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R tmax, R& t_out, uint32_t& idx_out, R& u_out,
+                  R& v_out, Stack& st, TraceCounters* cnt = nullptr) {
+  using Node = typename NodeSel<R>::type;
+  R closest = tmax;
+  bool found = false;
+  uint32_t best = 0;
+  R bu = R(0), bv = R(0);
+  V3<R> inv = {R(1) / d.x, R(1) / d.y, R(1) / d.z};
+  unsigned long long nf = 0, pt = 0, tt = 0;
+  if (S.n_nodes > 0) {
+    uint32_t cur = 0;
+    Node nd = S.nodes[0];
+    ++nf;
+    R te;
+    bool live = NodeOps<R>::test(nd, o, inv, tmin, closest, te);
+    while (live) {
+      bool descend = false;
+      if (nd.b & LEAF_BIT) {
+        uint32_t first = nd.a, count = nd.b & ~LEAF_BIT;
+        for (uint32_t k = 0; k < count; ++k) {
+          ++pt;
+          if (cnt && (S.prims[first + k].kind_mat & KIND_MASK) == K_TRIANGLE) ++tt;
+          if (test_prim(S, first + k, o, d, tmin, closest, bu, bv)) {
+            found = true;
+            best = first + k;
+            if (ANY) break;
+          }
+        }
+        if (ANY && found) break;
+      } else {
+        uint32_t l = cur + 1, r = nd.a;
+        Node L = S.nodes[l], Rn = S.nodes[r];
+        nf += 2;
+        R tl, tr;
+        bool hl = NodeOps<R>::test(L, o, inv, tmin, closest, tl);
+        bool hr = NodeOps<R>::test(Rn, o, inv, tmin, closest, tr);
+        if (hl && hr) {
+          if (tl <= tr) {
+            st.push(r);
+            cur = l;
+            nd = L;
+          } else {
+            st.push(l);
+            cur = r;
+            nd = Rn;
+          }
+          descend = true;
+        } else if (hl) {
+          cur = l;
+          nd = L;
+          descend = true;
+        } else if (hr) {
+          cur = r;
+          nd = Rn;
+          descend = true;
+        }
+      }
+      if (descend) continue;
+      live = false;
+      while (!st.empty()) {
+        cur = st.pop();
+        nd = S.nodes[cur];
+        ++nf;
+        if (NodeOps<R>::test(nd, o, inv, tmin, closest, te)) {
+          live = true;
+          break;
+        }
+      }
+    }
+  }
+  if (!(ANY && found)) {
+    for (uint32_t k = 0; k < S.n_planes; ++k) {
+      uint32_t idx = S.n_bounded + k;
+      ++pt;
+      if (test_prim(S, idx, o, d, tmin, closest, bu, bv)) {
+        found = true;
+        best = idx;
+        if (ANY) break;
+      }
+    }
+  }
+#if defined(__HIPCC__)
+  if (cnt) {
+    atomicAdd(&cnt->node_fetches, nf);
+    atomicAdd(&cnt->prim_tests, pt);
+    atomicAdd(&cnt->tri_tests, tt);
+    atomicAdd(&cnt->rays, 1ull);
+  }
+#else
+  (void)cnt;
+#endif
+  t_out = closest;
+  idx_out = best;
+  u_out = bu;
+  v_out = bv;
+  return found;
+}
+
+// ---------------------------------------------------------------------------
+// Hit record reconstruction (HitRecord::new, src/hit.rs:19-43): the outward
+// normal is evaluated in object space, flipped against the object-space ray,
+// and mapped back through the transform's normal matrix.
+template <class R>
+struct Surf {
+  V3<R> p;       // world point
+  V3<R> n;       // unit normal facing the incoming ray
+  R t;           // reference-reported t
+  R u, v;
+  bool front;
+  uint32_t mat;
+  uint32_t prim_id;
+};
+
+template <class R>
+LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_rep, R u, R v, Surf<R>& s) {
+  const PrimT<R>& pr = S.prims[idx];
+  uint32_t km = pr.kind_mat, kind = km & KIND_MASK;
+  bool is64 = (km & FLAG_FP64) != 0 || kind >= K_TORUS;
+  V3<R> oo = o, dd = d;
+  R t_true = t_rep;
+  if (pr.xform != XF_NONE) {
+    const XformT<R>& X = S.xforms[pr.xform];
+    oo = xf_point(X.inv, o);
+    dd = xf_dir(X.inv, d);
+    if (X.tmode != 0) t_true = t_rep / t_report_factor(X, dd, d);
+  }
+  V3<R> n;
+  if (is64) {
+    V3<double> oo64, dd64;
+    if (pr.xform != XF_NONE) {
+      const XformT<double>& X64 = S.xforms64[pr.xform];
+      oo64 = xf_point(X64.inv, cvt<double>(o));
+      dd64 = xf_dir(X64.inv, cvt<double>(d));
+    } else {
+      oo64 = cvt<double>(o);
+      dd64 = cvt<double>(d);
+    }
+    V3<double> q = oo64 + dd64 * double(t_true);
+    n = cvt<R>(prim_normal_fp64(kind, S.prims64[idx].p, q, S.exact_marched != 0));
+  } else {
+    n = prim_normal_analytic<R>(kind, pr.p, oo + dd * t_true);
+  }
+  bool front = dot(dd, n) < R(0);
+  if (!front) n = -n;
+  if (pr.xform != XF_NONE) {
+    const XformT<R>& X = S.xforms[pr.xform];
+    n = xf_nrm(X.nrm, n);
+    if (X.renorm) n = unit(n);
+  }
+  s.p = o + d * t_true;
+  s.n = n;
+  s.t = t_rep;
+  s.u = u;
+  s.v = v;
+  s.front = front;
+  s.mat = km >> 8;
+  s.prim_id = pr.prim_id;
+}
+
+// ---------------------------------------------------------------------------
+// Textures (src/texture.rs:19-23, :463-479)
+template <class R>
+LMN_HD V3<R> texture_value(const SceneView<R>& S, const MaterialT<R>& m, R u, R v) {
+  if (m.tex < 0) return {m.color[0], m.color[1], m.color[2]};
+  const TextureDev& T = S.textures[m.tex];
+  R uc = clamp_r(u, R(0), R(1));
+  R vc = R(1) - clamp_r(v, R(0), R(1));
+  uint32_t i = uint32_t(uc * R(T.w));
+  uint32_t j = uint32_t(vc * R(T.h));
+  i = i < T.w - 1 ? i : T.w - 1;
+  j = j < T.h - 1 ? j : T.h - 1;
+  const float* px = S.texels + T.offset + (uint64_t(j) * T.w + i) * 3;
+  return {R(px[0]), R(px[1]), R(px[2])};
+}
+// texel index touched by a lookup (path-replay adjoint), or -1 for solid colour
+template <class R>
+LMN_HD int64_t texel_index(const SceneView<R>& S, const MaterialT<R>& m, R u, R v) {
+  if (m.tex < 0) return -1;
+  const TextureDev& T = S.textures[m.tex];
+  R uc = clamp_r(u, R(0), R(1));
+  R vc = R(1) - clamp_r(v, R(0), R(1));
+  uint32_t i = uint32_t(uc * R(T.w));
+  uint32_t j = uint32_t(vc * R(T.h));
+  i = i < T.w - 1 ? i : T.w - 1;
+  j = j < T.h - 1 ? j : T.h - 1;
+  return int64_t(T.offset / 3 + uint64_t(j) * T.w + i);
+}
+
+template <class R> LMN_HD bool is_specular(const MaterialT<R>& m) {
+  switch (m.kind) {
+    case M_METAL:
+    case M_DIELECTRIC: return true;
+    case M_MICROFACET: return m.metallic > R(0.9) && m.roughness < R(0.1);  // src/material.rs:323-326
+    default: return false;
+  }
+}
+
+template <class R> LMN_HD V3<R> emitted(const SceneView<R>& S, const MaterialT<R>& m, R u, R v) {
+  if (m.kind != M_EMISSIVE) return {R(0), R(0), R(0)};
+  return texture_value(S, m, u, v) * m.intensity;  // src/material.rs:241-243
+}
+
+template <class R> LMN_HD R powi5(R x) {
+  R x2 = x * x;
+  return x2 * x2 * x;
+}
+// Dielectric::reflectance, Schlick (src/material.rs:138-141)
+template <class R> LMN_HD R schlick(R cosine, R ref_idx) {
+  R r0 = (R(1) - ref_idx) / (R(1) + ref_idx);
+  r0 = r0 * r0;
+  return r0 + (R(1) - r0) * powi5(R(1) - cosine);
+}
+// Microfacet::fresnel (src/material.rs:291-294)
+template <class R> LMN_HD V3<R> fresnel(R cos_theta, V3<R> f0) {
+  R t = powi5(fmax_r(R(1) - cos_theta, R(0)));
+  return f0 * (R(1) - t) + V3<R>{t, t, t};
+}
+// Microfacet::geometry, Schlick-Smith (src/material.rs:280-288)
+template <class R> LMN_HD R ggx_geometry(R n_dot_v, R n_dot_l, R rough) {
+  R k = (rough + R(1)) * (rough + R(1)) / R(8);
+  return (n_dot_v / (n_dot_v * (R(1) - k) + k)) * (n_dot_l / (n_dot_l * (R(1) - k) + k));
+}
+template <class R> LMN_HD R vatan(R x) { return m_atan(x); }
+// Microfacet::sample_ggx_normal (src/material.rs:297-319)
+template <class R> LMN_HD V3<R> sample_ggx_normal(V3<R> n, R alpha, Rng& rng) {
+  R r1 = rng.next<R>(), r2 = rng.next<R>();
+  const R pi = R(3.14159265358979323846);
+  R theta = m_atan(vsqrt(alpha * alpha * r1 / (R(1) - r1 + R(1e-10))));
+  R phi = R(2) * pi * r2;
+  R st = m_sin(theta), ct = m_cos(theta);
+  V3<R> hl = {st * m_cos(phi), st * m_sin(phi), ct};
+  V3<R> up = m_abs(n.y) < R(0.999) ? V3<R>{R(0), R(1), R(0)} : V3<R>{R(1), R(0), R(0)};
+  V3<R> tg = unit(cross(up, n));
+  V3<R> bt = cross(n, tg);
+  return unit(tg * hl.x + bt * hl.y + n * hl.z);
+}
+
+// Material::scatter for the five hot-path materials (src/material.rs:59-391).
+// Returns false when the material absorbs (scatter() == None).  `ray_d` is
+// the incoming (unnormalised) direction.
+template <class R>
+LMN_HD bool scatter(const SceneView<R>& S, const MaterialT<R>& m, V3<R> ray_d, const Surf<R>& h, Rng& rng,
+                    V3<R>& out_dir, V3<R>& att) {
+  switch (m.kind) {
+    case M_LAMBERTIAN: {  // src/material.rs:59-71
+      V3<R> dir = h.n + random_unit_vector<R>(rng);
+      if (near_zero(dir)) dir = h.n;
+      out_dir = dir;
+      att = texture_value(S, m, h.u, h.v);
+      return true;
+    }
+    case M_METAL: {  // src/material.rs:94-106
+      V3<R> refl = reflect(unit(ray_d), h.n);
+      V3<R> sc = refl + random_in_unit_sphere<R>(rng) * m.fuzz;
+      if (!(dot(sc, h.n) > R(0))) return false;
+      out_dir = sc;
+      att = {m.color[0], m.color[1], m.color[2]};
+      return true;
+    }
+    case M_DIELECTRIC: {  // src/material.rs:149-213
+      R ior = m.ior;
+      V3<R> chan = {R(1), R(1), R(1)};
+      if (m.dispersion > R(0)) {
+        uint32_t c = uint32_t(rng.next<R>() * R(3));
+        ior = ior + (c == 0 ? -m.dispersion : (c == 1 ? R(0) : m.dispersion));
+        chan = c == 0 ? V3<R>{R(3), R(0), R(0)} : (c == 1 ? V3<R>{R(0), R(3), R(0)} : V3<R>{R(0), R(0), R(3)});
+      }
+      R eta = h.front ? R(1) / ior : ior;
+      V3<R> ud = unit(ray_d);
+      R ct = dot(-ud, h.n);
+      ct = ct < R(1) ? ct : R(1);
+      R stt = vsqrt(R(1) - ct * ct);
+      bool cannot = eta * stt > R(1);
+      V3<R> dir;
+      if (cannot || schlick(ct, eta) > rng.next<R>())
+        dir = reflect(ud, h.n);
+      else
+        dir = refract(ud, h.n, eta);
+      if (m.roughness > R(0)) dir = dir + random_in_unit_sphere<R>(rng) * m.roughness;
+      V3<R> a;
+      bool white = m.color[0] == R(1) && m.color[1] == R(1) && m.color[2] == R(1);
+      if (!h.front && !white) {
+        a = {m_exp(-(R(1) - m.color[0]) * h.t), m_exp(-(R(1) - m.color[1]) * h.t), m_exp(-(R(1) - m.color[2]) * h.t)};
+      } else {
+        a = {m.color[0], m.color[1], m.color[2]};
+      }
+      if (m.dispersion > R(0)) a = hadamard(a, chan);
+      out_dir = dir;
+      att = a;
+      return true;
+    }
+    case M_MICROFACET: {  // src/material.rs:328-390
+      R alpha = m.roughness * m.roughness;
+      V3<R> v = unit(-ray_d);
+      V3<R> n = h.n;
+      V3<R> albedo = texture_value(S, m, h.u, h.v);
+      R ndv = fmax_r(dot(n, v), R(0.001));
+      V3<R> f0 = V3<R>{R(0.04), R(0.04), R(0.04)} * (R(1) - m.metallic) + albedo * m.metallic;
+      V3<R> fw = fresnel(ndv, f0);
+      R sp = clamp_r((fw.x + fw.y + fw.z) / R(3), R(0.1), R(0.9));
+      if (rng.next<R>() < sp) {
+        V3<R> hv = sample_ggx_normal(n, alpha, rng);
+        V3<R> l = reflect_around(v, hv);
+        R ndl = dot(n, l);
+        if (ndl <= R(0)) return false;
+        R ndh = fmax_r(dot(n, hv), R(0));
+        R vdh = fmax_r(dot(v, hv), R(0));
+        R g = ggx_geometry(ndv, ndl, m.roughness);
+        V3<R> f = fresnel(vdh, f0);
+        V3<R> w = ndh > R(0.001) ? f * (g * vdh / (ndv * ndh)) : V3<R>{R(0), R(0), R(0)};
+        out_dir = l;
+        att = w / sp;
+        return true;
+      }
+      V3<R> dir = n + random_unit_vector<R>(rng);
+      if (near_zero(dir)) dir = n;
+      V3<R> kd = (V3<R>{R(1), R(1), R(1)} - fw) * (R(1) - m.metallic);
+      out_dir = dir;
+      att = hadamard(kd, albedo) / (R(1) - sp);
+      return true;
+    }
+    default: return false;  // emissive: no scatter (src/material.rs:237-239)
+  }
+}
+
+// Frisvad orthonormal basis (build_onb, src/render.rs:1333-1343)
+template <class R> LMN_HD void build_onb(V3<R> n, V3<R>& a, V3<R>& b) {
+  if (n.z < R(-0.9999999)) {
+    a = {R(0), R(-1), R(0)};
+    b = {R(-1), R(0), R(0)};
+    return;
+  }
+  R aa = R(1) / (R(1) + n.z);
+  R bb = -n.x * n.y * aa;
+  a = {R(1) - n.x * n.x * aa, bb, -n.x};
+  b = {bb, R(1) - n.y * n.y * aa, -n.y};
+}
+
+// The sampling half of sample_direct_light (src/render.rs:1346-1466): picks a
+// light, samples a point and returns the unoccluded contribution and the
+// shadow ray (origin hit_point, unit dir, range [0.001, dist - 0.001]).
+// Returns false where the reference returns zero before its shadow query.
+template <class R>
+LMN_HD bool nee_sample(const SceneView<R>& S, V3<R> hp, V3<R> normal, V3<R> albedo, Rng& rng, V3<R>& dir_out,
+                       R& dist_out, V3<R>& contrib) {
+  uint32_t nl = S.n_lights;
+  uint32_t li = uint32_t(rng.next<R>() * R(nl));
+  li = li < nl - 1 ? li : nl - 1;
+  const LightT<R>& L = S.lights[li];
+  const R pi = R(3.14159265358979323846);
+  V3<R> lp, ln, em = L.emission;
+  R area;
+  if (L.kind == L_SPHERE) {
+    V3<R> tc = L.origin - hp;
+    R dc2 = length_squared(tc);
+    if (dc2 < L.radius * L.radius) return false;
+    R dc = vsqrt(dc2);
+    R stm = L.radius / dc;
+    R ctm = vsqrt(fmax_r(R(1) - stm * stm, R(0)));
+    R r1 = rng.next<R>(), r2 = rng.next<R>();
+    R ct = R(1) - r1 * (R(1) - ctm);
+    R st = vsqrt(fmax_r(R(1) - ct * ct, R(0)));
+    R phi = R(2) * pi * r2;
+    V3<R> w = tc / dc;
+    V3<R> a, b;
+    build_onb(w, a, b);
+    V3<R> dir = unit(a * (st * m_cos(phi)) + b * (st * m_sin(phi)) + w * ct);
+    V3<R> oc = hp - L.origin;
+    R bc = dot(oc, dir);
+    R cc = length_squared(oc) - L.radius * L.radius;
+    R disc = bc * bc - cc;
+    R t;
+    if (disc > R(0)) {
+      R sd = vsqrt(disc);
+      R t1 = -bc - sd;
+      t = t1 > R(0.001) ? t1 : -bc + sd;
+    } else {
+      t = dc;
+    }
+    lp = hp + dir * t;
+    ln = unit(lp - L.origin);
+    R sa = R(2) * pi * (R(1) - ctm);
+    area = sa * t * t / fmax_r(m_abs(dot(-dir, ln)), R(0.001));
+  } else if (L.kind == L_RECT) {
+    R s = rng.next<R>(), t = rng.next<R>();
+    lp = L.origin + L.u * s + L.v * t;
+    ln = L.normal;
+    area = L.area;
+  } else {  // disk (src/render.rs:1422-1436)
+    R r1 = rng.next<R>(), r2 = rng.next<R>();
+    R r = vsqrt(r1) * L.radius;
+    R th = R(2) * pi * r2;
+    V3<R> w = L.normal;
+    V3<R> av = m_abs(w.x) > R(0.9) ? V3<R>{R(0), R(1), R(0)} : V3<R>{R(1), R(0), R(0)};
+    V3<R> ut = unit(cross(w, av));
+    V3<R> vt = cross(w, ut);
+    lp = L.origin + ut * (r * m_cos(th)) + vt * (r * m_sin(th));
+    ln = L.normal;
+    area = pi * L.radius * L.radius;
+  }
+  V3<R> tl = lp - hp;
+  R d2 = length_squared(tl);
+  R dist = vsqrt(d2);
+  V3<R> dir = tl / dist;
+  R ct = dot(normal, dir);
+  if (!(ct > R(0))) return false;
+  R ctl = m_abs(dot(-dir, ln));
+  if (!(ctl > R(0.001))) return false;
+  dir_out = dir;
+  dist_out = dist;
+  contrib = hadamard(em, albedo) * (ct * ctl * area * R(nl) / (pi * d2));
+  return true;
+}
+
+// Background::color (src/render.rs:72-172) for the hot-path variants.
+template <class R> LMN_HD V3<R> background(const BackgroundT<R>& B, V3<R> d) {
+  if (B.kind == BG_SOLID) return B.c0;
+  V3<R> ud = unit(d);
+  R t = R(0.5) * (ud.y + R(1));
+  if (B.kind == BG_SKY) return V3<R>{R(1), R(1), R(1)} * (R(1) - t) + V3<R>{R(0.5), R(0.7), R(1)} * t;
+  if (B.kind == BG_GRADIENT) return B.c0 * (R(1) - t) + B.c1 * t;
+  V3<R> sky = V3<R>{R(1), R(1), R(1)} * (R(1) - t) + B.c1 * t;
+  R sd = dot(ud, B.dir_unit);
+  if (sd > R(0.9995)) return B.c0 * B.intensity;
+  if (sd > R(0.995)) {
+    R glow = (sd - R(0.995)) / (R(0.9995) - R(0.995));
+    return sky + B.c0 * (B.intensity * R(0.3) * glow);
+  }
+  return sky;
+}
+
+// Camera::get_ray, thin lens (src/camera.rs:82-103): two or more disk draws
+// even at zero aperture, then one time draw (time is unused on this path).
+template <class R> LMN_HD void camera_ray(const CameraT<R>& C, R s, R t, Rng& rng, V3<R>& o, V3<R>& d) {
+  V3<R> rd = random_in_unit_disk<R>(rng) * C.lens_radius;
+  V3<R> off = C.u * rd.x + C.v * rd.y;
+  o = C.origin + off;
+  d = C.lower_left + C.horizontal * s + C.vertical * t - C.origin - off;
+  (void)rng.next<R>();
+}
+
+// PixelFilter::weight (src/render.rs:198-237)
+template <class R> LMN_HD R mitchell_1d(R x) {
+  x = m_abs(x);
+  const R b = R(1) / R(3), c = R(1) / R(3);
+  if (x < R(1))
+    return ((R(12) - R(9) * b - R(6) * c) * x * x * x + (R(-18) + R(12) * b + R(6) * c) * x * x + (R(6) - R(2) * b)) / R(6);
+  if (x < R(2))
+    return ((-b - R(6) * c) * x * x * x + (R(6) * b + R(30) * c) * x * x + (R(-12) * b - R(48) * c) * x +
+            (R(8) * b + R(24) * c)) / R(6);
+  return R(0);
+}
+template <class R> LMN_HD R filter_weight(uint32_t f, R dx, R dy) {
+  switch (f) {
+    case 1: return fmax_r(R(1) - m_abs(dx) * R(2), R(0)) * fmax_r(R(1) - m_abs(dy) * R(2), R(0));
+    case 2: {
+      R sg = R(0.35);
+      return m_exp(-(dx * dx + dy * dy) / (R(2) * sg * sg));
+    }
+    case 3: return mitchell_1d(dx * R(2)) * mitchell_1d(dy * R(2));
+    default: return R(1);
+  }
+}
+
+// Per-frame constants of the pixel loop (src/render.rs:1571-1596).
+struct FrameDev {
+  uint32_t full_w, full_h;
+  uint32_t crop_x, crop_y, crop_w, crop_h;
+  uint32_t sqrt_spp, spp_a;
+  uint32_t max_depth, filter;
+  uint64_t fkey;
+  uint64_t n_items;    // crop_w * crop_h * spp_a
+  uint64_t item_base;  // first work item of the current chunk
+};
+
+// Generates work item k = sample * (crop_w*crop_h) + crop pixel: its camera
+// ray, RNG stream and filter weight (src/render.rs:1683-1690).
+template <class R>
+LMN_HD void generate_item(const FrameDev& F, const CameraT<R>& C, uint64_t k, Rng& rng, V3<R>& o, V3<R>& d, R& w,
+                          uint32_t& pixel) {
+  uint64_t npix = uint64_t(F.crop_w) * F.crop_h;
+  uint32_t s = uint32_t(k / npix);
+  uint32_t pl = uint32_t(k - uint64_t(s) * npix);
+  uint32_t ci = pl % F.crop_w, cj = pl / F.crop_w;
+  uint32_t gi = ci + F.crop_x, gj = cj + F.crop_y;
+  uint64_t gpix = uint64_t(gj) * F.full_w + gi;
+  rng.key = path_key(F.fkey, gpix, s);
+  rng.n = 0;
+  uint32_t sy = s / F.sqrt_spp, sx = s - sy * F.sqrt_spp;
+  R y = R(F.full_h - 1 - gj);
+  R jx = (R(sx) + rng.next<R>()) / R(F.sqrt_spp);
+  R jy = (R(sy) + rng.next<R>()) / R(F.sqrt_spp);
+  R uc = (R(gi) + jx) / R(F.full_w - 1);
+  R vc = (y + jy) / R(F.full_h - 1);
+  camera_ray(C, uc, vc, rng, o, d);
+  w = filter_weight<R>(F.filter, jx - R(0.5), jy - R(0.5));
+  pixel = pl;
+}
+
+}  // namespace lmn

@@ -1,0 +1,115 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, as the checker or the timed CPU baseline.  The
+product package never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+RNG_COUNTER = 0
+RNG_REFERENCE = 1
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+class OracleParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
+                ("seed", C.c_uint64), ("filter", C.c_uint32), ("has_crop", C.c_uint32), ("crop", C.c_uint32 * 4),
+                ("rng_mode", C.c_uint32), ("scene_key", C.c_uint32), ("threads", C.c_uint32),
+                ("exact_marched", C.c_uint32), ("row_begin", C.c_uint32), ("row_end", C.c_uint32)]
+
+
+class OracleStats(C.Structure):
+    _fields_ = [("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("render_secs", C.c_double), ("threads", C.c_uint32), ("pad", C.c_uint32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        from luminara_amd import _abi
+        L = C.CDLL(LIB_PATH)
+        L.oracle_trace.restype = C.c_int
+        L.oracle_trace.argtypes = [C.POINTER(_abi.SceneDesc), C.POINTER(_abi.Ray), C.c_size_t, C.c_uint32,
+                                   C.POINTER(_abi.Hit)]
+        L.oracle_render.restype = C.c_int
+        L.oracle_render.argtypes = [C.POINTER(_abi.SceneDesc), C.POINTER(OracleParams),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(OracleStats)]
+        L.oracle_scatter.restype = C.c_int
+        L.oracle_scatter.argtypes = [C.POINTER(_abi.SceneDesc), C.c_uint32, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.c_double,
+                                     C.c_double, C.c_double, C.c_uint64, C.c_uint32, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
+        L.oracle_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(f"oracle error {rc}: {lib().oracle_last_error().decode()}")
+
+
+def trace(world, rays: np.ndarray, exact=True):
+    from luminara_amd import hits_to_numpy, rays_to_struct, _abi
+    n = len(rays)
+    r = rays_to_struct(rays)
+    hits = (_abi.Hit * max(n, 1))()
+    _check(lib().oracle_trace(world.desc, r, n, 1 if exact else 0, hits))
+    return hits_to_numpy(hits, n)
+
+
+def render(world, config, rng_mode=RNG_COUNTER, threads=0, scene_key=0, exact=True, rows=None,
+           want_var=True):
+    """Pixel loop on the CPU.  Returns (mean HxWx3, var HxWx3 or None, stats)."""
+    p = OracleParams()
+    p.width, p.height = config.width, config.height
+    p.spp, p.max_depth, p.seed = config.samples_per_pixel, config.max_depth, config.seed
+    p.filter = config.pixel_filter
+    fw, fh = max(config.width, 1), max(config.height, 1)
+    cw, ch = fw, fh
+    if config.crop is not None:
+        p.has_crop = 1
+        for i, v in enumerate(config.crop):
+            p.crop[i] = v
+        cw, ch = min(config.crop[2], fw), min(config.crop[3], fh)
+    p.rng_mode, p.scene_key, p.threads = rng_mode, scene_key, threads
+    p.exact_marched = 1 if exact else 0
+    if rows is not None:
+        p.row_begin, p.row_end = rows
+        nrows = min(rows[1], ch) - rows[0]
+    else:
+        nrows = ch
+    mean = np.zeros((nrows, cw, 3), dtype=np.float64)
+    var = np.zeros((nrows, cw, 3), dtype=np.float64) if want_var else None
+    st = OracleStats()
+    _check(lib().oracle_render(world.desc, C.byref(p), mean.ctypes.data_as(C.POINTER(C.c_double)),
+                               var.ctypes.data_as(C.POINTER(C.c_double)) if want_var else None, C.byref(st)))
+    stats = {k: getattr(st, k) for k, _ in OracleStats._fields_ if k != "pad"}
+    return mean, var, stats
+
+
+def scatter(world, material, ray_dir, point, normal, front_face, t, u, v, key, counter=0):
+    D = C.c_double * 3
+    od, oa = D(), D()
+    oc = C.c_uint32()
+    ok = lib().oracle_scatter(world.desc, material, D(*ray_dir), D(*point), D(*normal), int(front_face), t, u, v,
+                              key, counter, od, oa, C.byref(oc))
+    if ok < 0:
+        _check(ok)
+    return bool(ok), np.array(od[:]), np.array(oa[:]), oc.value

@@ -1,0 +1,76 @@
+/*
+ * oracle.h - CPU FP64 restatement of the reference's render hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's CPU-baseline leg load this library, and only as the checker or
+ * as the timed CPU baseline; the product (luminara_amd/) never links or calls
+ * it.
+ *
+ * The reference (Rust, src/) cannot be built in this environment (no Rust
+ * toolchain, crates not vendored), so this is a line-by-line C++ restatement
+ * of the functions on the path, structured like the reference: a virtual
+ * Hittable tree with the reference's wrappers (Translate/RotateX/Y/Z/Scale/
+ * NonUniformScale), HittableList, the reference's 12-bucket SAH BvhNode with
+ * left-first recursion, recursive ray_color with NEE, the row-parallel pixel
+ * loop.  Each function cites the file:line it follows (oracle.cpp).
+ *
+ * RNG modes:
+ *   ORACLE_RNG_COUNTER   the GPU's counter-based SplitMix64 streams (common
+ *                        random numbers with the HIP kernels);
+ *   ORACLE_RNG_REFERENCE rand 0.9.2 SmallRng = xoshiro256++ seeded by
+ *                        SplitMix64 per row (src/render.rs:1608), restated
+ *                        from the published algorithm (Blackman & Vigna).
+ *                        No reference test pins its output: parity unpinned.
+ */
+#ifndef LMN_ORACLE_H
+#define LMN_ORACLE_H
+#include "../include/luminara.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORACLE_RNG_COUNTER = 0, ORACLE_RNG_REFERENCE = 1 };
+
+typedef struct oracle_params {
+  uint32_t width, height, spp, max_depth;
+  uint64_t seed;
+  uint32_t filter;
+  uint32_t has_crop;
+  uint32_t crop[4];
+  uint32_t rng_mode;
+  uint32_t scene_key;
+  uint32_t threads;        /* 0 = hardware concurrency */
+  uint32_t exact_marched;  /* 1 = exact roots for torus/superellipsoid/spring */
+  uint32_t row_begin;      /* render only rows [row_begin, row_end) of the crop; */
+  uint32_t row_end;        /* row_end = 0 means all rows                        */
+} oracle_params;
+
+typedef struct oracle_stats {
+  uint64_t camera_rays, extension_rays, shadow_rays;
+  double render_secs; /* pixel loop only (src/render.rs:1596 -> :1746) */
+  uint32_t threads;
+  uint32_t pad;
+} oracle_stats;
+
+/* Closest hit per ray, prim = descriptor prim index (-1 = miss). */
+int oracle_trace(const lmn_scene_desc* desc, const lmn_ray* rays, size_t n, uint32_t exact_marched, lmn_hit* hits);
+
+/* Pixel loop.  mean_rgb: rows*crop_w*3 doubles (weighted mean as the
+ * reference returns it); var_rgb (optional): per-pixel per-channel sample
+ * variance of the clamped samples (unweighted). */
+int oracle_render(const lmn_scene_desc* desc, const oracle_params* p, double* mean_rgb, double* var_rgb,
+                  oracle_stats* stats);
+
+/* Single material scatter / NEE / background evaluations for known-answer
+ * tests (fixed hit records). */
+int oracle_scatter(const lmn_scene_desc* desc, uint32_t material, const double ray_dir[3], const double point[3],
+                   const double normal[3], int front_face, double t, double u, double v, uint64_t key,
+                   uint32_t counter, double out_dir[3], double out_att[3], uint32_t* out_counter);
+
+const char* oracle_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
