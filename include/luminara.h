@@ -217,8 +217,11 @@ typedef struct lmn_render_params {
   uint32_t pool_paths;     /* wavefront path-pool size, 0 = library pick  */
   uint32_t block_size;     /* threads per workgroup, 0 = library pick     */
   uint32_t stack_lds;      /* LDS short-stack entries, 0 = library pick   */
+  uint32_t flags;          /* LMN_FLAG_*                                  */
+  uint32_t pad;
   void* stream;            /* hipStream_t, NULL = library stream          */
 } lmn_render_params;
+#define LMN_FLAG_STAGE_TIMING 1u /* time each wavefront stage with HIP events */
 
 typedef struct lmn_stats {
   uint64_t paths;          /* W*H*ceil(sqrt(spp))^2 (src/render.rs:1589)  */
@@ -227,7 +230,11 @@ typedef struct lmn_stats {
   uint64_t shadow_rays;    /* NEE visibility queries                      */
   double render_ms;        /* device time of the pixel loop               */
   uint32_t iterations;     /* wavefront iterations executed               */
-  uint32_t pad;
+  uint32_t extend_launches;
+  double extend_ms;        /* summed stage times (LMN_FLAG_STAGE_TIMING)  */
+  double shade_ms;
+  double shadow_ms;
+  double regen_ms;
 } lmn_stats;
 
 /* ---- ray-batch parity harness -------------------------------------------- */

@@ -135,7 +135,7 @@ def load_scene(toml_text: str, base_dir: Optional[str] = None):
 
 
 def make_params(config: RenderConfig, path="wavefront", scene_key=0, intersect_mode=_abi.ISECT_EXACT,
-                pool_paths=0, stack_lds=0, stream=None) -> _abi.RenderParams:
+                pool_paths=0, stack_lds=0, stream=None, flags=0) -> _abi.RenderParams:
     p = _abi.RenderParams()
     p.width, p.height = config.width, config.height
     p.spp, p.max_depth = config.samples_per_pixel, config.max_depth
@@ -150,6 +150,7 @@ def make_params(config: RenderConfig, path="wavefront", scene_key=0, intersect_m
     p.scene_key = scene_key
     p.pool_paths = pool_paths
     p.stack_lds = stack_lds
+    p.flags = flags
     p.stream = stream
     return p
 
@@ -182,7 +183,7 @@ def render(config: RenderConfig, camera, world: SceneWorld, lights=None, path="w
     out = np.empty((h, w, 3), dtype=np.float32)
     st = _abi.Stats()
     check(lib().lmn_render(scene, C.byref(params), out.ctypes.data_as(C.POINTER(C.c_float)), C.byref(st)))
-    stats = {k: getattr(st, k) for k, _ in _abi.Stats._fields_ if k != "pad"}
+    stats = {k: getattr(st, k) for k, _ in _abi.Stats._fields_}
     return RenderResult(rows=out, total_rays=st.paths, render_time_secs=st.render_ms / 1000.0, stats=stats)
 
 
@@ -190,7 +191,7 @@ def render_device(world: SceneWorld, params: _abi.RenderParams, out_ptr: int) ->
     """Renders into a caller-owned device buffer (e.g. a torch CUDA tensor)."""
     st = _abi.Stats()
     check(lib().lmn_render_device(world.device_scene(), C.byref(params), C.c_void_p(out_ptr), C.byref(st)))
-    return {k: getattr(st, k) for k, _ in _abi.Stats._fields_ if k != "pad"}
+    return {k: getattr(st, k) for k, _ in _abi.Stats._fields_}
 
 
 def profile_extend(world: SceneWorld, params: _abi.RenderParams, out_ptr: int) -> dict:

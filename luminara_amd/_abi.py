@@ -90,13 +90,17 @@ class RenderParams(C.Structure):
                 ("max_depth", C.c_uint32), ("seed", C.c_uint64), ("filter", C.c_uint32),
                 ("has_crop", C.c_uint32), ("crop", C.c_uint32 * 4), ("path", C.c_uint32),
                 ("intersect_mode", C.c_uint32), ("scene_key", C.c_uint32), ("pool_paths", C.c_uint32),
-                ("block_size", C.c_uint32), ("stack_lds", C.c_uint32), ("stream", C.c_void_p)]
+                ("block_size", C.c_uint32), ("stack_lds", C.c_uint32), ("flags", C.c_uint32),
+                ("pad", C.c_uint32), ("stream", C.c_void_p)]
+
+FLAG_STAGE_TIMING = 1
 
 
 class Stats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("render_ms", C.c_double), ("iterations", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("extend_launches", C.c_uint32), ("extend_ms", C.c_double), ("shade_ms", C.c_double),
+                ("shadow_ms", C.c_double), ("regen_ms", C.c_double)]
 
 
 class Ray(C.Structure):

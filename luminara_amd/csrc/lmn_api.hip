@@ -75,11 +75,15 @@ struct Workspace {
   WaveCounters* host_ctr = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  static constexpr int kBlock = 8;  // wavefront iterations between host checks
+  hipEvent_t stage_ev[kBlock][5] = {};
   int init() {
     if (!stream) {
       HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
       HIP_TRY(hipEventCreate(&ev0));
       HIP_TRY(hipEventCreate(&ev1));
+      for (auto& it : stage_ev)
+        for (auto& e : it) HIP_TRY(hipEventCreate(&e));
       HIP_TRY(hipHostMalloc((void**)&host_ctr, sizeof(WaveCounters), hipHostMallocDefault));
     }
     return LMN_OK;
@@ -328,7 +332,10 @@ int render_wavefront(lmn_scene* s, const lmn_render_params* p, const FrameSetup&
   S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
   launch_zero_acc(W.acc.as<double4>(), npix, st);
   uint64_t sum_ext = 0, sum_sh = 0, sum_cam = 0, iters = 0;
-  const int block = 8;  // iterations between host checks
+  const int block = Workspace::kBlock;
+  const bool timing = (p->flags & LMN_FLAG_STAGE_TIMING) != 0;
+  double t_stage[4] = {0, 0, 0, 0};
+  uint32_t n_launch = 0;
   for (uint32_t s0 = 0; s0 < F.spp_a; s0 += spp_chunk) {
     uint32_t s1 = std::min(F.spp_a, s0 + spp_chunk);
     F.item_base = uint64_t(s0) * npix;
@@ -338,12 +345,21 @@ int render_wavefront(lmn_scene* s, const lmn_render_params* p, const FrameSetup&
     auto t_start = std::chrono::steady_clock::now();
     for (;;) {
       for (int k = 0; k < block; ++k) {
-        launch_wave_iteration(L, S, F, P[cur], P[cur ^ 1], d_tc != nullptr);
+        launch_wave_iteration(L, S, F, P[cur], P[cur ^ 1], d_tc != nullptr, timing ? W.stage_ev[k] : nullptr);
         cur ^= 1;
       }
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync(W.host_ctr, L.ctr, sizeof(WaveCounters), hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
+      if (timing) {
+        for (int k = 0; k < block; ++k)
+          for (int g = 0; g < 4; ++g) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, W.stage_ev[k][g], W.stage_ev[k][g + 1]));
+            t_stage[g] += ms;
+          }
+        n_launch += block;
+      }
       const WaveCounters& c = *W.host_ctr;
       if (c.active == 0 && c.next_item == c.n_items) {
         sum_ext += c.stat_extend;
@@ -364,6 +380,11 @@ int render_wavefront(lmn_scene* s, const lmn_render_params* p, const FrameSetup&
     stats->extension_rays = sum_ext - sum_cam;
     stats->shadow_rays = sum_sh;
     stats->iterations = uint32_t(iters);
+    stats->extend_launches = n_launch;
+    stats->extend_ms = t_stage[0];
+    stats->shade_ms = t_stage[1];
+    stats->shadow_ms = t_stage[2];
+    stats->regen_ms = t_stage[3];
   }
   return LMN_OK;
 }

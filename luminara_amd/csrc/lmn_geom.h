@@ -104,13 +104,35 @@ LMN_HD bool hit_sphere(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u,
   R a = length_squared(d);
   R half_b = dot(oc, d);
   R cc = length_squared(oc) - r * r;
-  R disc = half_b * half_b - a * cc;
-  if (disc < R(0)) return false;
-  R sq = vsqrt(disc);
-  R root = (-half_b - sq) / a;
-  if (root < tmin || root > tmax) {
-    root = (-half_b + sq) / a;
-    if (root < tmin || root > tmax) return false;
+  R root;
+  if constexpr (sizeof(R) == 4) {
+    // FP32: the discriminant from the ray's perpendicular distance to the
+    // centre and the roots as c/q, q/a (q = -b -+ sqrt(disc)) avoid the
+    // cancellation of half_b^2 - a c for small or distant spheres
+    // (Haines et al., "Precision Improvements for Ray/Sphere Intersection",
+    // Ray Tracing Gems 2019).  Same roots, same near-then-far selection.
+    V3<R> l = oc - d * (half_b / a);
+    R disc = a * (r * r - length_squared(l));
+    if (disc < R(0)) return false;
+    R sq = vsqrt(disc);
+    R q = half_b >= R(0) ? -half_b - sq : -half_b + sq;
+    R t0 = cc / q, t1 = q / a;
+    R tn = t0 < t1 ? t0 : t1, tf = t0 < t1 ? t1 : t0;
+    if (q == R(0)) tn = tf = -half_b / a;
+    root = tn;
+    if (root < tmin || root > tmax) {
+      root = tf;
+      if (root < tmin || root > tmax) return false;
+    }
+  } else {  // the reference's half-b quadratic, unchanged
+    R disc = half_b * half_b - a * cc;
+    if (disc < R(0)) return false;
+    R sq = vsqrt(disc);
+    root = (-half_b - sq) / a;
+    if (root < tmin || root > tmax) {
+      root = (-half_b + sq) / a;
+      if (root < tmin || root > tmax) return false;
+    }
   }
   if (!(root == root)) return false;
   V3<R> n = ((o + d * root) - c) / r;

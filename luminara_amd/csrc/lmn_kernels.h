@@ -51,8 +51,10 @@ struct WaveLaunch {
   TraceCounters* tc;
 };
 
+// One wavefront iteration: extend, shade, shadow, regenerate, bookkeeping.
+// With `ev` (5 events), ev[0..4] bracket the stages for per-stage timing.
 void launch_wave_iteration(const WaveLaunch& L, const SceneView<float>& S, const FrameDev& F, const PathSoA& cur,
-                           const PathSoA& nxt, bool count);
+                           const PathSoA& nxt, bool count, hipEvent_t* ev = nullptr);
 void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s);
 void launch_resolve(const float4* samples, double4* acc, uint32_t npix, uint32_t n_samples, hipStream_t s);
 void launch_zero_acc(double4* acc, uint32_t npix, hipStream_t s);

@@ -294,23 +294,28 @@ static void launch_shadow_ss(const WaveLaunch& L, const SceneView<float>& S) {
 }
 
 void launch_wave_iteration(const WaveLaunch& L, const SceneView<float>& S, const FrameDev& F, const PathSoA& cur,
-                           const PathSoA& nxt, bool count) {
+                           const PathSoA& nxt, bool count, hipEvent_t* ev) {
   constexpr int BS = 256;
+  if (ev) (void)hipEventRecord(ev[0], L.stream);
   switch (L.stack_lds) {
     case 8: launch_extend_ss<8>(L, S, cur, count); break;
     case 24: launch_extend_ss<24>(L, S, cur, count); break;
     case 32: launch_extend_ss<32>(L, S, cur, count); break;
     default: launch_extend_ss<16>(L, S, cur, count); break;
   }
+  if (ev) (void)hipEventRecord(ev[1], L.stream);
   hipLaunchKernelGGL((k_shade<BS>), dim3(L.grid), dim3(BS), 0, L.stream, S, F, cur, L.hit, nxt, L.sh, L.samples, L.ctr);
+  if (ev) (void)hipEventRecord(ev[2], L.stream);
   switch (L.stack_lds) {
     case 8: launch_shadow_ss<8>(L, S); break;
     case 24: launch_shadow_ss<24>(L, S); break;
     case 32: launch_shadow_ss<32>(L, S); break;
     default: launch_shadow_ss<16>(L, S); break;
   }
+  if (ev) (void)hipEventRecord(ev[3], L.stream);
   hipLaunchKernelGGL((k_regen<BS>), dim3(L.grid), dim3(BS), 0, L.stream, S, F, nxt, L.samples, L.ctr, L.pool);
   hipLaunchKernelGGL(k_book, dim3(1), dim3(64), 0, L.stream, L.ctr, L.pool);
+  if (ev) (void)hipEventRecord(ev[4], L.stream);
 }
 
 void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s) {

@@ -35,6 +35,6 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(_abi.Light) == 136
     assert C.sizeof(_abi.Background) == 88
     assert C.sizeof(_abi.Camera) == 152
-    assert C.sizeof(_abi.RenderParams) == 80
+    assert C.sizeof(_abi.RenderParams) == 88
     assert C.sizeof(_abi.Ray) == 64
     assert C.sizeof(_abi.Hit) == 64
