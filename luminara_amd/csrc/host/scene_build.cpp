@@ -225,9 +225,10 @@ struct Builder {
     return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
   }
 
-  uint32_t build(size_t b, size_t e, uint32_t depth) {
-    uint32_t id = uint32_t(nodes.size());
-    nodes.push_back(NodeD{});
+  // Builds the subtree of refs[b, e) into the already allocated node `id`.
+  // Children are allocated as adjacent pairs at even indices (root at 0,
+  // index 1 unused), so a pair of FP32 nodes is one aligned 64-byte block.
+  void build(uint32_t id, size_t b, size_t e, uint32_t depth) {
     double lo[3], hi[3], clo[3], chi[3];
     bounds_of(b, e, lo, hi, clo, chi);
     for (int k = 0; k < 3; ++k) {
@@ -329,13 +330,14 @@ struct Builder {
       nodes[id].b = LEAF_BIT | uint32_t(n);
       for (size_t i = b; i < e; ++i) order.push_back(refs[i].idx);
       leaves++;
-      return id;
+      return;
     }
-    build(b, mid, depth + 1);  // left child lands at id + 1
-    uint32_t r = build(mid, e, depth + 1);
-    nodes[id].a = r;
+    uint32_t c = uint32_t(nodes.size());
+    nodes.resize(c + 2);
+    nodes[id].a = c;
     nodes[id].b = uint32_t(best_axis < 0 ? 0 : best_axis);
-    return id;
+    build(c, b, mid, depth + 1);
+    build(c + 1, mid, e, depth + 1);
   }
 };
 
@@ -532,7 +534,11 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
     for (int k = 0; k < 3; ++k) B.refs[i].c[k] = float(0.5 * (blo[i * 3 + k] + bhi[i * 3 + k]));
     B.refs[i].idx = uint32_t(i);
   }
-  if (!bounded.empty()) B.build(0, bounded.size(), 0);
+  if (!bounded.empty()) {
+    B.nodes.resize(2);  // root + unused slot keeps child pairs 64-byte aligned
+    B.nodes[1].b = LEAF_BIT;  // empty leaf, never referenced
+    B.build(0, 0, bounded.size(), 0);
+  }
   out.tree_depth = B.depth_max;
   out.n_leaves = B.leaves;
   out.nodesD = B.nodes;
@@ -558,11 +564,15 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
   size_t nrec = out.order.size();
   out.primsF.resize(nrec);
   out.primsD.resize(nrec);
+  out.featF = out.featD = 0;
   for (size_t i = 0; i < nrec; ++i) {
     const lmn_prim& p = desc.prims[out.order[i]];
     bool isl = fp64_island(p);
     fill_record<float>(p, nullptr, out.primsF[i], out.order[i], isl);
     fill_record<double>(p, nullptr, out.primsD[i], out.order[i], false);
+    if (p.transform >= 0) out.featF |= FEAT_XF, out.featD |= FEAT_XF;
+    if (isl) out.featF |= FEAT_F64;
+    if (p.kind >= LMN_PRIM_TORUS) out.featF |= FEAT_MARCH, out.featD |= FEAT_MARCH;
   }
   out.xfD.resize(desc.n_transforms);
   out.xfF.resize(desc.n_transforms);

@@ -36,6 +36,7 @@ struct HostScene {
   CameraT<float> camF;
   CameraT<double> camD;
   uint32_t n_bounded = 0, n_planes = 0;
+  uint32_t featF = 0, featD = 0;  // FEAT_* of the FP32 and FP64 paths
   uint32_t tree_depth = 0, n_leaves = 0;
   std::vector<uint32_t> order;  // order[i] = descriptor prim index of record i
 };

@@ -293,7 +293,7 @@ int render_wavefront(lmn_scene* s, const lmn_render_params* p, const FrameSetup&
   uint32_t pool = p->pool_paths ? p->pool_paths : uint32_t(std::min<uint64_t>(total, 1u << 22));
   pool = std::max<uint32_t>(pool, 64);
   uint32_t ss = p->stack_lds ? p->stack_lds : 16;
-  if (ss != 8 && ss != 16 && ss != 24 && ss != 32) return set_err(LMN_ERR_INVALID, "stack_lds must be 8, 16, 24 or 32");
+  if (ss != 16 && ss != 32) return set_err(LMN_ERR_INVALID, "stack_lds must be 16 or 32");
   // sample buffer chunk: whole sample passes, at most 2^27 samples (2 GiB)
   uint64_t max_chunk = 1ull << 27;
   uint32_t spp_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(F.spp_a, max_chunk / npix)));
@@ -306,7 +306,7 @@ int render_wavefront(lmn_scene* s, const lmn_render_params* p, const FrameSetup&
     HIP_TRY(W.st[k][2].ensure(size_t(pool) * 16));
     HIP_TRY(W.st[k][3].ensure(size_t(pool) * 16));
   }
-  HIP_TRY(W.hit.ensure(size_t(pool) * 16));
+  HIP_TRY(W.hit.ensure(size_t(pool) * 8));
   HIP_TRY(W.sh_o.ensure(size_t(pool) * 16));
   HIP_TRY(W.sh_d.ensure(size_t(pool) * 16));
   HIP_TRY(W.sh_c.ensure(size_t(pool) * 16));
@@ -322,7 +322,8 @@ int render_wavefront(lmn_scene* s, const lmn_render_params* p, const FrameSetup&
   L.grid = grid;
   L.pool = pool;
   L.stack_lds = ss;
-  L.hit = W.hit.as<float4>();
+  L.feat = s->host.featF;
+  L.hit = W.hit.as<float2>();
   L.sh = ShadowSoA{W.sh_o.as<float4>(), W.sh_d.as<float4>(), W.sh_c.as<float4>()};
   L.samples = W.samples.as<float4>();
   L.ctr = W.ctr.as<WaveCounters>();
@@ -401,11 +402,11 @@ int render_plain(lmn_scene* s, const lmn_render_params* p, const FrameSetup& fs,
     if (fp64) {
       SceneView<double> S = s->viewD;
       S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
-      launch_plain_fp64(S, fs.F, W.acc.as<double4>(), s0, s0 + 1, W.rays.as<unsigned long long>(), st);
+      launch_plain_fp64(S, s->host.featD, fs.F, W.acc.as<double4>(), s0, s0 + 1, W.rays.as<unsigned long long>(), st);
     } else {
       SceneView<float> S = s->viewF;
       S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
-      launch_plain_fp32(S, fs.F, W.acc.as<double4>(), s0, s0 + 1, W.rays.as<unsigned long long>(), st);
+      launch_plain_fp32(S, s->host.featF, fs.F, W.acc.as<double4>(), s0, s0 + 1, W.rays.as<unsigned long long>(), st);
     }
   }
   launch_finalize(W.acc.as<double4>(), d_out, npix, st);
@@ -518,11 +519,11 @@ int lmn_trace_rays(const lmn_scene* scene, const lmn_ray* rays, size_t n, uint32
     if (path == LMN_PATH_PLAIN_FP64) {
       SceneView<double> S = scene->viewD;
       S.exact_marched = intersect_mode == LMN_ISECT_EXACT;
-      launch_trace_fp64(S, d_r, d_h, n, nullptr);
+      launch_trace_fp64(S, scene->host.featD, d_r, d_h, n, nullptr);
     } else {
       SceneView<float> S = scene->viewF;
       S.exact_marched = intersect_mode == LMN_ISECT_EXACT;
-      launch_trace_fp32(S, d_r, d_h, n, nullptr);
+      launch_trace_fp32(S, scene->host.featF, d_r, d_h, n, nullptr);
     }
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(hits, d_h, n * sizeof(lmn_hit), hipMemcpyDeviceToHost);

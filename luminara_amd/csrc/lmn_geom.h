@@ -92,12 +92,14 @@ template <class R> LMN_HD R t_report_factor(const XformT<R>& x, V3<R> d_obj, V3<
 }
 
 // ---------------------------------------------------------------------------
-// Analytic primitives.  Each returns the hit t (object space == world space t
-// for the affine wrappers), and the reference's (u, v).
+// Analytic primitives.  The hit routines return only t (object-space t ==
+// world t for the affine wrappers); the reference's (u, v) are recomputed from
+// the hit point when a hit is shaded (prim_uv), which keeps the traversal
+// loop's register footprint small.
 
 // Sphere::hit (src/sphere.rs:24-62)
 template <class R>
-LMN_HD bool hit_sphere(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
+LMN_HD bool hit_sphere(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t) {
   V3<R> c = {p[0], p[1], p[2]};
   R r = p[3];
   V3<R> oc = o - c;
@@ -135,39 +137,36 @@ LMN_HD bool hit_sphere(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u,
     }
   }
   if (!(root == root)) return false;
-  V3<R> n = ((o + d * root) - c) / r;
-  const R pi = R(3.14159265358979323846);
-  R theta = m_acos(-n.y);
-  R phi = m_atan2(-n.z, n.x) + pi;
   t = root;
-  u = phi / (R(2) * pi);
-  v = theta / pi;
   return true;
 }
 
-// Quad::hit (src/quad.rs:30-62)
+// Quad::hit (src/quad.rs:30-62); alpha/beta from the planar hit vector
 template <class R>
-LMN_HD bool hit_quad(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
-  V3<R> q = {p[0], p[1], p[2]}, uu = {p[3], p[4], p[5]}, vv = {p[6], p[7], p[8]}, w = {p[9], p[10], p[11]};
+LMN_HD void quad_ab(const R* p, V3<R> planar, R& alpha, R& beta) {
+  V3<R> uu = {p[3], p[4], p[5]}, vv = {p[6], p[7], p[8]}, w = {p[9], p[10], p[11]};
+  alpha = dot(w, cross(planar, vv));
+  beta = dot(w, cross(uu, planar));
+}
+template <class R>
+LMN_HD bool hit_quad(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t) {
+  V3<R> q = {p[0], p[1], p[2]}, w = {p[9], p[10], p[11]};
   V3<R> n = unit(w);
   R denom = dot(n, d);
   if (m_abs(denom) < R(1e-8)) return false;
   R tt = (p[12] - dot(n, o)) / denom;
   if (tt < tmin || tt > tmax || !(tt == tt)) return false;
-  V3<R> planar = (o + d * tt) - q;
-  R alpha = dot(w, cross(planar, vv));
-  R beta = dot(w, cross(uu, planar));
+  R alpha, beta;
+  quad_ab(p, (o + d * tt) - q, alpha, beta);
   if (!(alpha >= R(0) && alpha <= R(1)) || !(beta >= R(0) && beta <= R(1))) return false;
   t = tt;
-  u = alpha;
-  v = beta;
   return true;
 }
 
 // XyRect / XzRect / YzRect::hit (src/rect.rs:24-37, :64-77, :104-117).  `ax`
 // is the plane axis, (ia, ib) the in-plane axes.
 template <class R>
-LMN_HD bool hit_rect(const R* p, int ax, int ia, int ib, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
+LMN_HD bool hit_rect(const R* p, int ax, int ia, int ib, V3<R> o, V3<R> d, R tmin, R tmax, R& t) {
   const R* oo = &o.x;
   const R* dd = &d.x;
   R tt = (p[4] - oo[ax]) / dd[ax];
@@ -176,15 +175,13 @@ LMN_HD bool hit_rect(const R* p, int ax, int ia, int ib, V3<R> o, V3<R> d, R tmi
   R b = oo[ib] + tt * dd[ib];
   if (a < p[0] || a > p[1] || b < p[2] || b > p[3]) return false;
   t = tt;
-  u = (a - p[0]) / (p[1] - p[0]);
-  v = (b - p[2]) / (p[3] - p[2]);
   return true;
 }
 
 // Triangle::hit / MeshTriangle::hit, Moller-Trumbore
-// (src/triangle.rs:23-64, src/obj.rs:260-313)
+// (src/triangle.rs:23-64, src/obj.rs:260-313); u, v are the barycentrics.
 template <class R>
-LMN_HD bool hit_triangle(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
+LMN_HD bool hit_triangle_uv(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
   V3<R> v0 = {p[0], p[1], p[2]}, e1 = {p[3], p[4], p[5]}, e2 = {p[6], p[7], p[8]};
   V3<R> h = cross(d, e2);
   R a = dot(e1, h);
@@ -203,22 +200,64 @@ LMN_HD bool hit_triangle(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& 
   v = vv;
   return true;
 }
+template <class R>
+LMN_HD bool hit_triangle(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t) {
+  R u, v;
+  return hit_triangle_uv(p, o, d, tmin, tmax, t, u, v);
+}
 
 // Plane::hit (src/plane.rs:43-69)
 template <class R>
-LMN_HD bool hit_plane(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
+LMN_HD bool hit_plane(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t) {
   V3<R> pt = {p[0], p[1], p[2]}, n = {p[3], p[4], p[5]};
   R denom = dot(n, d);
   if (m_abs(denom) < R(1e-8)) return false;
   R tt = dot(pt - o, n) / denom;
   if (tt < tmin || tt > tmax || !(tt == tt)) return false;
-  V3<R> local = (o + d * tt) - pt;
-  R a = dot(local, V3<R>{p[6], p[7], p[8]});
-  R b = dot(local, V3<R>{p[9], p[10], p[11]});
   t = tt;
-  u = m_abs(a - m_trunc(a));  // f64::fract().abs()
-  v = m_abs(b - m_trunc(b));
   return true;
+}
+
+// (u, v) of an analytic hit from the object-space ray and its t, with the
+// reference's formulas: spherical map (src/sphere.rs:48-52), quad alpha/beta
+// (src/quad.rs:43-46), rect fractions (src/rect.rs:34-35), triangle
+// barycentrics (src/triangle.rs:35-42), plane fract (src/plane.rs:57-59).
+template <class R>
+LMN_HD void prim_uv_analytic(uint32_t kind, const R* p, V3<R> o, V3<R> d, R t, R& u, R& v) {
+  const R pi = R(3.14159265358979323846);
+  switch (kind) {
+    case K_SPHERE: {
+      V3<R> n = ((o + d * t) - V3<R>{p[0], p[1], p[2]}) / p[3];
+      u = (m_atan2(-n.z, n.x) + pi) / (R(2) * pi);
+      v = m_acos(-n.y) / pi;
+      break;
+    }
+    case K_QUAD: quad_ab(p, (o + d * t) - V3<R>{p[0], p[1], p[2]}, u, v); break;
+    case K_RECT_XY:
+    case K_RECT_XZ:
+    case K_RECT_YZ: {
+      int ia = kind == K_RECT_YZ ? 1 : 0, ib = kind == K_RECT_XY ? 1 : 2;
+      const R* oo = &o.x;
+      const R* dd = &d.x;
+      u = (oo[ia] + t * dd[ia] - p[0]) / (p[1] - p[0]);
+      v = (oo[ib] + t * dd[ib] - p[2]) / (p[3] - p[2]);
+      break;
+    }
+    case K_TRIANGLE: {
+      R tt;
+      if (!hit_triangle_uv(p, o, d, R(-INFINITY), R(INFINITY), tt, u, v)) u = v = R(0);
+      break;
+    }
+    case K_PLANE: {
+      V3<R> local = (o + d * t) - V3<R>{p[0], p[1], p[2]};
+      R a = dot(local, V3<R>{p[6], p[7], p[8]});
+      R b = dot(local, V3<R>{p[9], p[10], p[11]});
+      u = m_abs(a - m_trunc(a));  // f64::fract().abs()
+      v = m_abs(b - m_trunc(b));
+      break;
+    }
+    default: u = v = R(0);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -578,34 +617,39 @@ LMN_HD V3<double> marched_normal(uint32_t kind, const double* p, V3<double> wp) 
 }
 
 // Object-space intersection of one primitive in precision R.  Marched kinds
-// are always FP64 and must be called through hit_prim_fp64.
+// are always FP64 and go through hit_marched_fp64.
 template <class R>
-LMN_HD bool hit_prim_analytic(uint32_t kind, const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, R& u, R& v) {
+LMN_HD bool hit_prim_analytic(uint32_t kind, const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t) {
   switch (kind) {
-    case K_SPHERE: return hit_sphere(p, o, d, tmin, tmax, t, u, v);
-    case K_QUAD: return hit_quad(p, o, d, tmin, tmax, t, u, v);
-    case K_RECT_XY: return hit_rect(p, 2, 0, 1, o, d, tmin, tmax, t, u, v);
-    case K_RECT_XZ: return hit_rect(p, 1, 0, 2, o, d, tmin, tmax, t, u, v);
-    case K_RECT_YZ: return hit_rect(p, 0, 1, 2, o, d, tmin, tmax, t, u, v);
-    case K_TRIANGLE: return hit_triangle(p, o, d, tmin, tmax, t, u, v);
-    case K_PLANE: return hit_plane(p, o, d, tmin, tmax, t, u, v);
+    case K_SPHERE: return hit_sphere(p, o, d, tmin, tmax, t);
+    case K_QUAD: return hit_quad(p, o, d, tmin, tmax, t);
+    case K_RECT_XY: return hit_rect(p, 2, 0, 1, o, d, tmin, tmax, t);
+    case K_RECT_XZ: return hit_rect(p, 1, 0, 2, o, d, tmin, tmax, t);
+    case K_RECT_YZ: return hit_rect(p, 0, 1, 2, o, d, tmin, tmax, t);
+    case K_TRIANGLE: return hit_triangle(p, o, d, tmin, tmax, t);
+    case K_PLANE: return hit_plane(p, o, d, tmin, tmax, t);
     default: return false;
   }
 }
 
-LMN_HD bool hit_prim_fp64(uint32_t kind, const double* p, V3<double> o, V3<double> d, double tmin, double tmax,
-                          bool exact, double& t, double& u, double& v) {
-  bool ok;
+// Marched kinds (torus, superellipsoid, spring), FP64, out of line.
+LMN_COLD bool hit_marched_fp64(uint32_t kind, const double* p, V3<double> o, V3<double> d, double tmin, double tmax,
+                               bool exact, double& t) {
   switch (kind) {
-    case K_TORUS: ok = exact ? hit_torus_exact(p, o, d, tmin, tmax, t) : hit_torus_march(p, o, d, tmin, tmax, t); break;
+    case K_TORUS: return exact ? hit_torus_exact(p, o, d, tmin, tmax, t) : hit_torus_march(p, o, d, tmin, tmax, t);
     case K_SUPERELLIPSOID:
-      ok = exact ? hit_superellipsoid_exact(p, o, d, tmin, tmax, t) : hit_superellipsoid_march(p, o, d, tmin, tmax, t);
-      break;
-    case K_SPRING: ok = exact ? hit_spring_exact(p, o, d, tmin, tmax, t) : hit_spring_march(p, o, d, tmin, tmax, t); break;
-    default: return hit_prim_analytic<double>(kind, p, o, d, tmin, tmax, t, u, v);
+      return exact ? hit_superellipsoid_exact(p, o, d, tmin, tmax, t) : hit_superellipsoid_march(p, o, d, tmin, tmax, t);
+    default: return exact ? hit_spring_exact(p, o, d, tmin, tmax, t) : hit_spring_march(p, o, d, tmin, tmax, t);
   }
-  if (ok) marched_uv(kind, p, o + d * t, u, v);
-  return ok;
+}
+
+// FP64 evaluation of one primitive: spheres inline (the FP32 path's large
+// spheres), the marched kinds out of line.
+LMN_HD bool hit_prim_fp64(uint32_t kind, const double* p, V3<double> o, V3<double> d, double tmin, double tmax,
+                          bool exact, double& t) {
+  if (kind == K_SPHERE) return hit_sphere<double>(p, o, d, tmin, tmax, t);
+  if (kind >= K_TORUS) return hit_marched_fp64(kind, p, o, d, tmin, tmax, exact, t);
+  return hit_prim_analytic<double>(kind, p, o, d, tmin, tmax, t);
 }
 
 // Exact torus normal: gradient of (|p|^2 + R^2 - r^2)^2 - 4R^2(x^2+z^2).
@@ -631,10 +675,9 @@ LMN_HD V3<R> prim_normal_analytic(uint32_t kind, const R* p, V3<R> q) {
   }
 }
 
-LMN_HD V3<double> prim_normal_fp64(uint32_t kind, const double* p, V3<double> q, bool exact) {
-  if (kind == K_TORUS) return exact ? torus_exact_normal(p, q) : marched_normal(kind, p, q);
-  if (kind == K_SUPERELLIPSOID || kind == K_SPRING) return marched_normal(kind, p, q);
-  return prim_normal_analytic<double>(kind, p, q);
+LMN_COLD V3<double> marched_normal_fp64(uint32_t kind, const double* p, V3<double> q, bool exact) {
+  if (kind == K_TORUS && exact) return torus_exact_normal(p, q);
+  return marched_normal(kind, p, q);
 }
 
 }  // namespace lmn

@@ -43,7 +43,8 @@ struct WaveLaunch {
   uint32_t grid;       // workgroups of 256 lanes (grid-stride kernels)
   uint32_t pool;       // path pool size
   uint32_t stack_lds;  // LDS short-stack entries per lane
-  float4* hit;
+  uint32_t feat;       // FEAT_* specialisation of the scene
+  float2* hit;         // per path: t, primitive record index (bits)
   ShadowSoA sh;
   float4* samples;
   WaveCounters* ctr;
@@ -59,11 +60,13 @@ void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s);
 void launch_resolve(const float4* samples, double4* acc, uint32_t npix, uint32_t n_samples, hipStream_t s);
 void launch_zero_acc(double4* acc, uint32_t npix, hipStream_t s);
 void launch_finalize(const double4* acc, float* out, uint32_t npix, hipStream_t s);
-void launch_plain_fp64(const SceneView<double>& S, const FrameDev& F, double4* acc, uint32_t s0, uint32_t s1,
-                       unsigned long long* rays, hipStream_t s);
-void launch_plain_fp32(const SceneView<float>& S, const FrameDev& F, double4* acc, uint32_t s0, uint32_t s1,
-                       unsigned long long* rays, hipStream_t s);
-void launch_trace_fp32(const SceneView<float>& S, const lmn_ray* rays, lmn_hit* hits, size_t n, hipStream_t s);
-void launch_trace_fp64(const SceneView<double>& S, const lmn_ray* rays, lmn_hit* hits, size_t n, hipStream_t s);
+void launch_plain_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, double4* acc, uint32_t s0,
+                       uint32_t s1, unsigned long long* rays, hipStream_t s);
+void launch_plain_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, double4* acc, uint32_t s0,
+                       uint32_t s1, unsigned long long* rays, hipStream_t s);
+void launch_trace_fp32(const SceneView<float>& S, uint32_t feat, const lmn_ray* rays, lmn_hit* hits, size_t n,
+                       hipStream_t s);
+void launch_trace_fp64(const SceneView<double>& S, uint32_t feat, const lmn_ray* rays, lmn_hit* hits, size_t n,
+                       hipStream_t s);
 
 }  // namespace lmn

@@ -54,8 +54,8 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, bool want) {
 // ---------------------------------------------------------------------------
 // Wavefront stages
 
-template <int SS, int BS, bool COUNT>
-__global__ __launch_bounds__(BS) void k_extend(SceneView<float> S, PathSoA cur, float4* __restrict__ hit,
+template <int SS, int BS, uint32_t FEAT>
+__global__ __launch_bounds__(BS) void k_extend(SceneView<float> S, PathSoA cur, float2* __restrict__ hit,
                                                const WaveCounters* __restrict__ ctr, uint32_t* spill,
                                                TraceCounters* tc) {
   __shared__ uint32_t lds[SS * BS];
@@ -65,14 +65,14 @@ __global__ __launch_bounds__(BS) void k_extend(SceneView<float> S, PathSoA cur, 
   for (uint32_t i = gid; i < n; i += nthreads) {
     float4 a = cur.ob[i], b = cur.db[i];
     LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
-    float t, u, v;
+    float t;
     uint32_t idx;
-    bool h = trace<float, false>(S, f3(a), f3(b), 0.001f, INFINITY, t, idx, u, v, st, COUNT ? tc : nullptr);
-    hit[i] = make_float4(t, u, v, __uint_as_float(h ? idx : 0xffffffffu));
+    bool h = trace<float, FEAT, false>(S, f3(a), f3(b), 0.001f, INFINITY, t, idx, st, tc);
+    hit[i] = make_float2(t, __uint_as_float(h ? idx : 0xffffffffu));
   }
 }
 
-template <int SS, int BS>
+template <int SS, int BS, uint32_t FEAT>
 __global__ __launch_bounds__(BS) void k_shadow(SceneView<float> S, ShadowSoA sh, float4* __restrict__ samples,
                                                const WaveCounters* __restrict__ ctr, uint32_t* spill) {
   __shared__ uint32_t lds[SS * BS];
@@ -82,10 +82,10 @@ __global__ __launch_bounds__(BS) void k_shadow(SceneView<float> S, ShadowSoA sh,
   for (uint32_t i = gid; i < n; i += nthreads) {
     float4 o = sh.o[i], d = sh.d[i];
     LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
-    float t, u, v;
+    float t;
     uint32_t idx;
     // shadow query over [0.001, dist - 0.001] (src/render.rs:1457-1458)
-    bool occluded = trace<float, true>(S, f3(o), f3(d), 0.001f, o.w - 0.001f, t, idx, u, v, st);
+    bool occluded = trace<float, FEAT, true>(S, f3(o), f3(d), 0.001f, o.w - 0.001f, t, idx, st);
     if (!occluded) {
       float4 c = sh.c[i];
       uint32_t item = __float_as_uint(d.w);
@@ -98,8 +98,8 @@ __global__ __launch_bounds__(BS) void k_shadow(SceneView<float> S, ShadowSoA sh,
   }
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_shade(SceneView<float> S, FrameDev F, PathSoA cur, const float4* __restrict__ hit,
+template <int BS, uint32_t FEAT>
+__global__ __launch_bounds__(BS) void k_shade(SceneView<float> S, FrameDev F, PathSoA cur, const float2* __restrict__ hit,
                                               PathSoA nxt, ShadowSoA sh, float4* __restrict__ samples,
                                               WaveCounters* __restrict__ ctr) {
   uint32_t n = ctr->active;
@@ -113,8 +113,8 @@ __global__ __launch_bounds__(BS) void k_shade(SceneView<float> S, FrameDev F, Pa
     rng.n = X.x;
     uint32_t item = M.x, depth = M.y & 0xffffu;
     bool skip = (M.y >> 16) & 1u;
-    float4 H = hit[i];
-    uint32_t prim = __float_as_uint(H.w);
+    float2 H = hit[i];
+    uint32_t prim = __float_as_uint(H.y);
     V3<float> add = {0.f, 0.f, 0.f};
     bool cont = false, emit = false;
     V3<float> sp_o, sp_d, ndir, nbeta, cb;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(BS) void k_shade(SceneView<float> S, FrameDev F, Pa
       add = hadamard(beta, background(S.bg, d));
     } else {
       Surf<float> s;
-      surface(S, prim, o, d, H.x, H.y, H.z, s);
+      surface<float, FEAT>(S, prim, o, d, H.x, s);
       const MaterialT<float>& m = S.materials[s.mat];
       V3<float> em = skip ? V3<float>{0.f, 0.f, 0.f} : emitted(S, m, s.u, s.v);
       V3<float> att;
@@ -279,39 +279,58 @@ __global__ void k_finalize(const double4* __restrict__ acc, float* __restrict__ 
 // ---------------------------------------------------------------------------
 // Launchers
 
-template <int SS>
-static void launch_extend_ss(const WaveLaunch& L, const SceneView<float>& S, const PathSoA& cur, bool count) {
+template <int SS, uint32_t FEAT>
+static void launch_stage_ss(const WaveLaunch& L, const SceneView<float>& S, const PathSoA& cur, bool shadow) {
   constexpr int BS = 256;
-  if (count)
-    hipLaunchKernelGGL((k_extend<SS, BS, true>), dim3(L.grid), dim3(BS), 0, L.stream, S, cur, L.hit, L.ctr, L.spill, L.tc);
+  if (shadow)
+    hipLaunchKernelGGL((k_shadow<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, S, L.sh, L.samples, L.ctr, L.spill);
   else
-    hipLaunchKernelGGL((k_extend<SS, BS, false>), dim3(L.grid), dim3(BS), 0, L.stream, S, cur, L.hit, L.ctr, L.spill, L.tc);
+    hipLaunchKernelGGL((k_extend<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, S, cur, L.hit, L.ctr, L.spill,
+                       L.tc);
 }
-template <int SS>
-static void launch_shadow_ss(const WaveLaunch& L, const SceneView<float>& S) {
+template <uint32_t FEAT>
+static void launch_stage(const WaveLaunch& L, const SceneView<float>& S, const PathSoA& cur, bool shadow) {
+  if (L.stack_lds == 32)
+    launch_stage_ss<32, FEAT>(L, S, cur, shadow);
+  else
+    launch_stage_ss<16, FEAT>(L, S, cur, shadow);
+}
+// The FP32 path's feature sets: marched primitives are FP64 records too, so
+// FEAT_MARCH always comes with FEAT_F64.
+#define LMN_FEAT32_SWITCH(feat, CALL) \
+  switch (feat) {                      \
+    case 0: CALL(0); break;            \
+    case 1: CALL(1); break;            \
+    case 2: CALL(2); break;            \
+    case 3: CALL(3); break;            \
+    case 6: CALL(6); break;            \
+    default: CALL(7); break;           \
+  }
+
+static void launch_trace_stage(const WaveLaunch& L, const SceneView<float>& S, const PathSoA& cur, bool shadow) {
+#define CALL(f) launch_stage<f>(L, S, cur, shadow)
+  LMN_FEAT32_SWITCH(L.feat, CALL)
+#undef CALL
+}
+static void launch_shade(const WaveLaunch& L, const SceneView<float>& S, const FrameDev& F, const PathSoA& cur,
+                         const PathSoA& nxt) {
   constexpr int BS = 256;
-  hipLaunchKernelGGL((k_shadow<SS, BS>), dim3(L.grid), dim3(BS), 0, L.stream, S, L.sh, L.samples, L.ctr, L.spill);
+#define CALL(f) \
+  hipLaunchKernelGGL((k_shade<BS, f>), dim3(L.grid), dim3(BS), 0, L.stream, S, F, cur, L.hit, nxt, L.sh, L.samples, L.ctr)
+  LMN_FEAT32_SWITCH(L.feat, CALL)
+#undef CALL
 }
 
 void launch_wave_iteration(const WaveLaunch& L, const SceneView<float>& S, const FrameDev& F, const PathSoA& cur,
                            const PathSoA& nxt, bool count, hipEvent_t* ev) {
+  (void)count;
   constexpr int BS = 256;
   if (ev) (void)hipEventRecord(ev[0], L.stream);
-  switch (L.stack_lds) {
-    case 8: launch_extend_ss<8>(L, S, cur, count); break;
-    case 24: launch_extend_ss<24>(L, S, cur, count); break;
-    case 32: launch_extend_ss<32>(L, S, cur, count); break;
-    default: launch_extend_ss<16>(L, S, cur, count); break;
-  }
+  launch_trace_stage(L, S, cur, false);
   if (ev) (void)hipEventRecord(ev[1], L.stream);
-  hipLaunchKernelGGL((k_shade<BS>), dim3(L.grid), dim3(BS), 0, L.stream, S, F, cur, L.hit, nxt, L.sh, L.samples, L.ctr);
+  launch_shade(L, S, F, cur, nxt);
   if (ev) (void)hipEventRecord(ev[2], L.stream);
-  switch (L.stack_lds) {
-    case 8: launch_shadow_ss<8>(L, S); break;
-    case 24: launch_shadow_ss<24>(L, S); break;
-    case 32: launch_shadow_ss<32>(L, S); break;
-    default: launch_shadow_ss<16>(L, S); break;
-  }
+  launch_trace_stage(L, S, cur, true);
   if (ev) (void)hipEventRecord(ev[3], L.stream);
   hipLaunchKernelGGL((k_regen<BS>), dim3(L.grid), dim3(BS), 0, L.stream, S, F, nxt, L.samples, L.ctr, L.pool);
   hipLaunchKernelGGL(k_book, dim3(1), dim3(64), 0, L.stream, L.ctr, L.pool);

@@ -6,9 +6,13 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define LMN_HD __host__ __device__ __forceinline__
+// Rarely taken FP64 paths stay out of line so that the register allocation of
+// the traversal and shading loops is not sized for them.
+#define LMN_COLD inline __host__ __device__ __attribute__((noinline))
 #else
 #include <cmath>
 #define LMN_HD inline
+#define LMN_COLD inline
 #endif
 
 namespace lmn {

@@ -7,7 +7,7 @@ namespace lmn {
 // ---------------------------------------------------------------------------
 // Plain megakernel: one thread per pixel, paths run to completion one after
 // another, private-memory traversal stack, FP64 (or FP32) throughout.
-template <class R>
+template <class R, uint32_t FEAT>
 __global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, double4* __restrict__ acc,
                                                uint32_t s_begin, uint32_t s_end, unsigned long long* rays) {
   uint32_t npix = F.crop_w * F.crop_h;
@@ -25,15 +25,15 @@ __global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, doubl
     bool skip = false;
     for (uint32_t depth = F.max_depth; depth > 0; --depth) {
       PrivateStack st;
-      R t, u, v;
+      R t;
       uint32_t idx;
       ++n_ext;
-      if (!trace<R, false>(S, o, d, R(0.001), R(INFINITY), t, idx, u, v, st)) {
+      if (!trace<R, FEAT, false>(S, o, d, R(0.001), R(INFINITY), t, idx, st)) {
         L = L + hadamard(beta, background(S.bg, d));
         break;
       }
       Surf<R> sf;
-      surface(S, idx, o, d, t, u, v, sf);
+      surface<R, FEAT>(S, idx, o, d, t, sf);
       const MaterialT<R>& m = S.materials[sf.mat];
       V3<R> em = skip ? V3<R>{R(0), R(0), R(0)} : emitted(S, m, sf.u, sf.v);
       V3<R> ndir, att;
@@ -57,10 +57,10 @@ __global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, doubl
         if (nee_sample(S, sf.p, sf.n, att, rng, ld, dist, c)) {
           nee_nan = has_nan(c);
           PrivateStack st2;
-          R t2, u2, v2;
+          R t2;
           uint32_t i2;
           ++n_sh;
-          if (!trace<R, true>(S, sf.p, ld, R(0.001), dist - R(0.001), t2, i2, u2, v2, st2)) direct = c;
+          if (!trace<R, FEAT, true>(S, sf.p, ld, R(0.001), dist - R(0.001), t2, i2, st2)) direct = c;
         }
       }
       V3<R> attw = att * rrw;
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, doubl
 
 // ---------------------------------------------------------------------------
 // Ray-batch closest hit (parity harness)
-template <class R>
+template <class R, uint32_t FEAT>
 __global__ void k_trace(SceneView<R> S, const lmn_ray* __restrict__ rays, lmn_hit* __restrict__ hits, size_t n) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -102,13 +102,13 @@ __global__ void k_trace(SceneView<R> S, const lmn_ray* __restrict__ rays, lmn_hi
   V3<R> o = {R(r.origin[0]), R(r.origin[1]), R(r.origin[2])};
   V3<R> d = {R(r.direction[0]), R(r.direction[1]), R(r.direction[2])};
   PrivateStack st;
-  R t, u, v;
+  R t;
   uint32_t idx;
   lmn_hit h;
   h.pad = 0;
-  if (trace<R, false>(S, o, d, R(r.t_min), R(r.t_max), t, idx, u, v, st)) {
+  if (trace<R, FEAT, false>(S, o, d, R(r.t_min), R(r.t_max), t, idx, st)) {
     Surf<R> sf;
-    surface(S, idx, o, d, t, u, v, sf);
+    surface<R, FEAT>(S, idx, o, d, t, sf);
     h.t = double(t);
     h.prim = int64_t(sf.prim_id);
     h.u = double(sf.u);
@@ -127,21 +127,50 @@ __global__ void k_trace(SceneView<R> S, const lmn_ray* __restrict__ rays, lmn_hi
   hits[i] = h;
 }
 
-void launch_plain_fp64(const SceneView<double>& S, const FrameDev& F, double4* acc, uint32_t s0, uint32_t s1,
-                       unsigned long long* rays, hipStream_t s) {
+// FP32 feature sets (FEAT_MARCH implies FEAT_F64) and FP64 feature sets
+// (FEAT_F64 is meaningless in FP64: only transforms and marched kinds).
+#define LMN_FEAT_SWITCH(feat, CALL) \
+  switch (feat) {                    \
+    case 0: CALL(0); break;          \
+    case 1: CALL(1); break;          \
+    case 2: CALL(2); break;          \
+    case 3: CALL(3); break;          \
+    case 6: CALL(6); break;          \
+    default: CALL(7); break;         \
+  }
+#define LMN_FEAT64_SWITCH(feat, CALL) \
+  switch (feat & 5u) {                 \
+    case 0: CALL(0); break;            \
+    case 1: CALL(1); break;            \
+    case 4: CALL(4); break;            \
+    default: CALL(5); break;           \
+  }
+
+void launch_plain_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, double4* acc, uint32_t s0,
+                       uint32_t s1, unsigned long long* rays, hipStream_t s) {
   uint32_t npix = F.crop_w * F.crop_h;
-  hipLaunchKernelGGL((k_plain<double>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, acc, s0, s1, rays);
+#define CALL(f) hipLaunchKernelGGL((k_plain<double, f>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, acc, s0, s1, rays)
+  LMN_FEAT64_SWITCH(feat, CALL)
+#undef CALL
 }
-void launch_plain_fp32(const SceneView<float>& S, const FrameDev& F, double4* acc, uint32_t s0, uint32_t s1,
-                       unsigned long long* rays, hipStream_t s) {
+void launch_plain_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, double4* acc, uint32_t s0,
+                       uint32_t s1, unsigned long long* rays, hipStream_t s) {
   uint32_t npix = F.crop_w * F.crop_h;
-  hipLaunchKernelGGL((k_plain<float>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, acc, s0, s1, rays);
+#define CALL(f) hipLaunchKernelGGL((k_plain<float, f>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, acc, s0, s1, rays)
+  LMN_FEAT_SWITCH(feat, CALL)
+#undef CALL
 }
-void launch_trace_fp32(const SceneView<float>& S, const lmn_ray* rays, lmn_hit* hits, size_t n, hipStream_t s) {
-  hipLaunchKernelGGL((k_trace<float>), dim3((n + 127) / 128), dim3(128), 0, s, S, rays, hits, n);
+void launch_trace_fp32(const SceneView<float>& S, uint32_t feat, const lmn_ray* rays, lmn_hit* hits, size_t n,
+                       hipStream_t s) {
+#define CALL(f) hipLaunchKernelGGL((k_trace<float, f>), dim3((n + 127) / 128), dim3(128), 0, s, S, rays, hits, n)
+  LMN_FEAT_SWITCH(feat, CALL)
+#undef CALL
 }
-void launch_trace_fp64(const SceneView<double>& S, const lmn_ray* rays, lmn_hit* hits, size_t n, hipStream_t s) {
-  hipLaunchKernelGGL((k_trace<double>), dim3((n + 127) / 128), dim3(128), 0, s, S, rays, hits, n);
+void launch_trace_fp64(const SceneView<double>& S, uint32_t feat, const lmn_ray* rays, lmn_hit* hits, size_t n,
+                       hipStream_t s) {
+#define CALL(f) hipLaunchKernelGGL((k_trace<double, f>), dim3((n + 127) / 128), dim3(128), 0, s, S, rays, hits, n)
+  LMN_FEAT64_SWITCH(feat, CALL)
+#undef CALL
 }
 
 }  // namespace lmn

@@ -117,93 +117,98 @@ template <> struct NodeOps<double> {
   }
 };
 
-// Intersects primitive `idx` (transform applied); on a hit closer than
-// `closest`, updates closest (reference-reported t), u, v.
-template <class R>
-LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R tmin, R& closest, R& hu, R& hv) {
+// Scene features that select a kernel specialisation: instance transforms,
+// FP64 spheres on the FP32 path (large spheres), marched primitives (torus,
+// superellipsoid, spring; FP64, out of line).  A specialisation without a
+// feature carries none of its code or registers.
+enum : uint32_t { FEAT_XF = 1, FEAT_F64 = 2, FEAT_MARCH = 4 };
+
+// Whether record `km` is evaluated in FP64 by the R-precision path.
+template <class R, uint32_t FEAT>
+LMN_HD bool prim_is64(uint32_t km) {
+  if constexpr (sizeof(R) == 4) {
+    if (!(FEAT & (FEAT_F64 | FEAT_MARCH))) return false;
+    return (km & FLAG_FP64) != 0;
+  } else {
+    if (!(FEAT & FEAT_MARCH)) return false;
+    return (km & KIND_MASK) >= K_TORUS;
+  }
+}
+
+// Intersects primitive `idx` (transform applied); on a hit in
+// [tmin, closest] updates closest to the reference-reported t.
+template <class R, uint32_t FEAT>
+LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R tmin, R& closest) {
   const PrimT<R>& pr = S.prims[idx];
   uint32_t km = pr.kind_mat;
   uint32_t kind = km & KIND_MASK;
-  // FP64 islands on the FP32 path, and the marched kinds in either precision
-  bool is64 = (km & FLAG_FP64) != 0 || kind >= K_TORUS;
-  R t, u, v;
-  bool hit;
-  if (pr.xform == XF_NONE) {
-    if (is64) {
-      double t64, u64, v64;
-      hit = hit_prim_fp64(kind, S.prims64[idx].p, cvt<double>(o), cvt<double>(d), double(tmin), double(closest),
-                          S.exact_marched != 0, t64, u64, v64);
-      t = R(t64);
-      u = R(u64);
-      v = R(v64);
-      if (hit && !(t <= closest)) hit = false;  // rounding back to R must not leave the range
-    } else {
-      hit = hit_prim_analytic<R>(kind, pr.p, o, d, tmin, closest, t, u, v);
-    }
-    if (hit) {
-      closest = t;
-      hu = u;
-      hv = v;
-    }
-    return hit;
+  bool has_xf = (FEAT & FEAT_XF) && pr.xform != XF_NONE;
+  V3<R> oo = o, dd = d;
+  if (has_xf) {
+    const XformT<R>& X = S.xforms[pr.xform];
+    oo = xf_point(X.inv, o);
+    dd = xf_dir(X.inv, d);
   }
-  const XformT<R>& X = S.xforms[pr.xform];
-  V3<R> oo = xf_point(X.inv, o), dd = xf_dir(X.inv, d);
-  if (is64) {
-    const XformT<double>& X64 = S.xforms64[pr.xform];
-    V3<double> o64 = xf_point(X64.inv, cvt<double>(o)), d64 = xf_dir(X64.inv, cvt<double>(d));
-    double t64, u64, v64;
-    hit = hit_prim_fp64(kind, S.prims64[idx].p, o64, d64, double(tmin), double(closest), S.exact_marched != 0, t64,
-                        u64, v64);
+  R t;
+  bool hit;
+  if (prim_is64<R, FEAT>(km)) {
+    V3<double> o64 = cvt<double>(o), d64 = cvt<double>(d);
+    if (has_xf) {
+      const XformT<double>& X64 = S.xforms64[pr.xform];
+      o64 = xf_point(X64.inv, o64);
+      d64 = xf_dir(X64.inv, d64);
+    }
+    double t64;
+    if constexpr ((FEAT & FEAT_MARCH) != 0)
+      hit = hit_prim_fp64(kind, S.prims64[idx].p, o64, d64, double(tmin), double(closest), S.exact_marched != 0, t64);
+    else  // without marched kinds only spheres are FP64 records
+      hit = hit_sphere<double>(S.prims64[idx].p, o64, d64, double(tmin), double(closest), t64);
     t = R(t64);
-    u = R(u64);
-    v = R(v64);
-    if (hit && !(t <= closest)) hit = false;
+    if (hit && !(t <= closest)) hit = false;  // rounding back to R must not leave the range
   } else {
-    hit = hit_prim_analytic<R>(kind, pr.p, oo, dd, tmin, closest, t, u, v);
+    hit = hit_prim_analytic<R>(kind, pr.p, oo, dd, tmin, closest, t);
   }
   if (!hit) return false;
-  if (X.tmode != 0) {
-    R tr = t * t_report_factor(X, dd, d);
-    if (!(tr <= closest)) return false;
-    t = tr;
+  if (has_xf) {
+    const XformT<R>& X = S.xforms[pr.xform];
+    if (X.tmode != 0) {
+      R tr = t * t_report_factor(X, dd, d);
+      if (!(tr <= closest)) return false;
+      t = tr;
+    }
   }
   closest = t;
-  hu = u;
-  hv = v;
   return true;
 }
 
 // Closest hit over the BVH then the unbounded list (SceneWorld::hit,
 // src/scene.rs:817-834).  ANY = stop at the first hit in range (NEE
 // visibility only asks is_some(), src/render.rs:1458).  Ordered BVH2
-// traversal: both children boxes are tested, the nearer is visited first and
-// the farther pushed; a popped node is re-tested against the current
-// closest hit before it is expanded.
-template <class R, bool ANY, class Stack>
-LMN_HD bool trace(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R tmax, R& t_out, uint32_t& idx_out, R& u_out,
-                  R& v_out, Stack& st, TraceCounters* cnt = nullptr) {
+// traversal: the two children of a node are stored side by side, so one
+// contiguous 64-byte fetch (FP32) brings both boxes; both are tested, the
+// nearer is visited first and the farther pushed; a popped node is
+// re-tested against the current closest hit before it is expanded.
+template <class R, uint32_t FEAT, bool ANY, class Stack>
+LMN_HD bool trace(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R tmax, R& t_out, uint32_t& idx_out, Stack& st,
+                  TraceCounters* cnt = nullptr) {
   using Node = typename NodeSel<R>::type;
   R closest = tmax;
   bool found = false;
   uint32_t best = 0;
-  R bu = R(0), bv = R(0);
   V3<R> inv = {R(1) / d.x, R(1) / d.y, R(1) / d.z};
-  unsigned long long nf = 0, pt = 0, tt = 0;
+  uint32_t nf = 0, pt = 0, tt = 0;
   if (S.n_nodes > 0) {
-    uint32_t cur = 0;
     Node nd = S.nodes[0];
     ++nf;
     R te;
     bool live = NodeOps<R>::test(nd, o, inv, tmin, closest, te);
     while (live) {
-      bool descend = false;
       if (nd.b & LEAF_BIT) {
         uint32_t first = nd.a, count = nd.b & ~LEAF_BIT;
         for (uint32_t k = 0; k < count; ++k) {
           ++pt;
           if (cnt && (S.prims[first + k].kind_mat & KIND_MASK) == K_TRIANGLE) ++tt;
-          if (test_prim(S, first + k, o, d, tmin, closest, bu, bv)) {
+          if (test_prim<R, FEAT>(S, first + k, o, d, tmin, closest)) {
             found = true;
             best = first + k;
             if (ANY) break;
@@ -211,38 +216,26 @@ LMN_HD bool trace(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R tmax, R& t_
         }
         if (ANY && found) break;
       } else {
-        uint32_t l = cur + 1, r = nd.a;
-        Node L = S.nodes[l], Rn = S.nodes[r];
+        uint32_t c = nd.a;  // children at c, c + 1
+        Node L = S.nodes[c], Rn = S.nodes[c + 1];
         nf += 2;
         R tl, tr;
         bool hl = NodeOps<R>::test(L, o, inv, tmin, closest, tl);
         bool hr = NodeOps<R>::test(Rn, o, inv, tmin, closest, tr);
         if (hl && hr) {
-          if (tl <= tr) {
-            st.push(r);
-            cur = l;
-            nd = L;
-          } else {
-            st.push(l);
-            cur = r;
-            nd = Rn;
-          }
-          descend = true;
-        } else if (hl) {
-          cur = l;
-          nd = L;
-          descend = true;
-        } else if (hr) {
-          cur = r;
-          nd = Rn;
-          descend = true;
+          bool left_first = tl <= tr;
+          st.push(left_first ? c + 1 : c);
+          nd = left_first ? L : Rn;
+          continue;
+        }
+        if (hl || hr) {
+          nd = hl ? L : Rn;
+          continue;
         }
       }
-      if (descend) continue;
       live = false;
       while (!st.empty()) {
-        cur = st.pop();
-        nd = S.nodes[cur];
+        nd = S.nodes[st.pop()];
         ++nf;
         if (NodeOps<R>::test(nd, o, inv, tmin, closest, te)) {
           live = true;
@@ -255,7 +248,7 @@ LMN_HD bool trace(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R tmax, R& t_
     for (uint32_t k = 0; k < S.n_planes; ++k) {
       uint32_t idx = S.n_bounded + k;
       ++pt;
-      if (test_prim(S, idx, o, d, tmin, closest, bu, bv)) {
+      if (test_prim<R, FEAT>(S, idx, o, d, tmin, closest)) {
         found = true;
         best = idx;
         if (ANY) break;
@@ -264,25 +257,25 @@ LMN_HD bool trace(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R tmax, R& t_
   }
 #if defined(__HIPCC__)
   if (cnt) {
-    atomicAdd(&cnt->node_fetches, nf);
-    atomicAdd(&cnt->prim_tests, pt);
-    atomicAdd(&cnt->tri_tests, tt);
+    atomicAdd(&cnt->node_fetches, (unsigned long long)nf);
+    atomicAdd(&cnt->prim_tests, (unsigned long long)pt);
+    atomicAdd(&cnt->tri_tests, (unsigned long long)tt);
     atomicAdd(&cnt->rays, 1ull);
   }
 #else
   (void)cnt;
+  (void)tt;
 #endif
   t_out = closest;
   idx_out = best;
-  u_out = bu;
-  v_out = bv;
   return found;
 }
 
 // ---------------------------------------------------------------------------
 // Hit record reconstruction (HitRecord::new, src/hit.rs:19-43): the outward
-// normal is evaluated in object space, flipped against the object-space ray,
-// and mapped back through the transform's normal matrix.
+// normal and (u, v) are evaluated in object space, the normal is flipped
+// against the object-space ray and mapped back through the transform's
+// normal matrix.
 template <class R>
 struct Surf {
   V3<R> p;       // world point
@@ -294,38 +287,49 @@ struct Surf {
   uint32_t prim_id;
 };
 
-template <class R>
-LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_rep, R u, R v, Surf<R>& s) {
+template <class R, uint32_t FEAT>
+LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_rep, Surf<R>& s) {
   const PrimT<R>& pr = S.prims[idx];
   uint32_t km = pr.kind_mat, kind = km & KIND_MASK;
-  bool is64 = (km & FLAG_FP64) != 0 || kind >= K_TORUS;
+  bool has_xf = (FEAT & FEAT_XF) && pr.xform != XF_NONE;
   V3<R> oo = o, dd = d;
   R t_true = t_rep;
-  if (pr.xform != XF_NONE) {
+  if (has_xf) {
     const XformT<R>& X = S.xforms[pr.xform];
     oo = xf_point(X.inv, o);
     dd = xf_dir(X.inv, d);
     if (X.tmode != 0) t_true = t_rep / t_report_factor(X, dd, d);
   }
   V3<R> n;
-  if (is64) {
-    V3<double> oo64, dd64;
-    if (pr.xform != XF_NONE) {
+  if (prim_is64<R, FEAT>(km)) {
+    V3<double> oo64 = cvt<double>(o), dd64 = cvt<double>(d);
+    if (has_xf) {
       const XformT<double>& X64 = S.xforms64[pr.xform];
-      oo64 = xf_point(X64.inv, cvt<double>(o));
-      dd64 = xf_dir(X64.inv, cvt<double>(d));
-    } else {
-      oo64 = cvt<double>(o);
-      dd64 = cvt<double>(d);
+      oo64 = xf_point(X64.inv, oo64);
+      dd64 = xf_dir(X64.inv, dd64);
     }
+    const double* p64 = S.prims64[idx].p;
     V3<double> q = oo64 + dd64 * double(t_true);
-    n = cvt<R>(prim_normal_fp64(kind, S.prims64[idx].p, q, S.exact_marched != 0));
+    double u64, v64;
+    if ((FEAT & FEAT_MARCH) && kind >= K_TORUS) {
+      n = cvt<R>(marched_normal_fp64(kind, p64, q, S.exact_marched != 0));
+      marched_uv(kind, p64, q, u64, v64);
+    } else {  // FP64 sphere: only the centre offset needs FP64, the map does not
+      V3<R> nn = cvt<R>(prim_normal_analytic<double>(kind, p64, q));
+      n = nn;
+      const R pi = R(3.14159265358979323846);
+      u64 = double((m_atan2(-nn.z, nn.x) + pi) / (R(2) * pi));
+      v64 = double(m_acos(-nn.y) / pi);
+    }
+    s.u = R(u64);
+    s.v = R(v64);
   } else {
     n = prim_normal_analytic<R>(kind, pr.p, oo + dd * t_true);
+    prim_uv_analytic<R>(kind, pr.p, oo, dd, t_true, s.u, s.v);
   }
   bool front = dot(dd, n) < R(0);
   if (!front) n = -n;
-  if (pr.xform != XF_NONE) {
+  if (has_xf) {
     const XformT<R>& X = S.xforms[pr.xform];
     n = xf_nrm(X.nrm, n);
     if (X.renorm) n = unit(n);
@@ -333,8 +337,6 @@ LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_r
   s.p = o + d * t_true;
   s.n = n;
   s.t = t_rep;
-  s.u = u;
-  s.v = v;
   s.front = front;
   s.mat = km >> 8;
   s.prim_id = pr.prim_id;
