@@ -56,8 +56,7 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, bool want) {
 
 template <int SS, int BS, uint32_t FEAT>
 __global__ __launch_bounds__(BS) void k_extend(SceneView<float> S, PathSoA cur, float2* __restrict__ hit,
-                                               const WaveCounters* __restrict__ ctr, uint32_t* spill,
-                                               TraceCounters* tc) {
+                                               const WaveCounters* __restrict__ ctr, uint32_t* spill) {
   __shared__ uint32_t lds[SS * BS];
   uint32_t n = ctr->active;
   uint32_t nthreads = gridDim.x * BS;
@@ -67,7 +66,7 @@ __global__ __launch_bounds__(BS) void k_extend(SceneView<float> S, PathSoA cur, 
     LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
     float t;
     uint32_t idx;
-    bool h = trace<float, FEAT, false>(S, f3(a), f3(b), 0.001f, INFINITY, t, idx, st, tc);
+    bool h = trace<float, FEAT, false>(S, f3(a), f3(b), 0.001f, INFINITY, t, idx, st);
     hit[i] = make_float2(t, __uint_as_float(h ? idx : 0xffffffffu));
   }
 }
@@ -279,14 +278,36 @@ __global__ void k_finalize(const double4* __restrict__ acc, float* __restrict__ 
 // ---------------------------------------------------------------------------
 // Launchers
 
+// Instrumented extend (counts node fetches and primitive tests): a kernel of
+// its own name so that profiles keep it apart from the timed k_extend.
+template <int SS, int BS, uint32_t FEAT>
+__global__ __launch_bounds__(BS) void k_extend_count(SceneView<float> S, PathSoA cur, float2* __restrict__ hit,
+                                                     const WaveCounters* __restrict__ ctr, uint32_t* spill,
+                                                     TraceCounters* tc) {
+  __shared__ uint32_t lds[SS * BS];
+  uint32_t n = ctr->active;
+  uint32_t nthreads = gridDim.x * BS;
+  uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  for (uint32_t i = gid; i < n; i += nthreads) {
+    float4 a = cur.ob[i], b = cur.db[i];
+    LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
+    float t;
+    uint32_t idx;
+    bool h = trace<float, FEAT, false>(S, f3(a), f3(b), 0.001f, INFINITY, t, idx, st, tc);
+    hit[i] = make_float2(t, __uint_as_float(h ? idx : 0xffffffffu));
+  }
+}
+
 template <int SS, uint32_t FEAT>
 static void launch_stage_ss(const WaveLaunch& L, const SceneView<float>& S, const PathSoA& cur, bool shadow) {
   constexpr int BS = 256;
   if (shadow)
     hipLaunchKernelGGL((k_shadow<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, S, L.sh, L.samples, L.ctr, L.spill);
+  else if (L.tc)
+    hipLaunchKernelGGL((k_extend_count<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, S, cur, L.hit, L.ctr,
+                       L.spill, L.tc);
   else
-    hipLaunchKernelGGL((k_extend<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, S, cur, L.hit, L.ctr, L.spill,
-                       L.tc);
+    hipLaunchKernelGGL((k_extend<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, S, cur, L.hit, L.ctr, L.spill);
 }
 template <uint32_t FEAT>
 static void launch_stage(const WaveLaunch& L, const SceneView<float>& S, const PathSoA& cur, bool shadow) {
