@@ -270,8 +270,18 @@ int lmn_desc_set_texture(lmn_scene_desc* desc, uint32_t index, uint32_t width, u
 /* Append an image texture; returns its index in *out_index. */
 int lmn_desc_add_texture(lmn_scene_desc* desc, uint32_t width, uint32_t height, const float* rgb,
                          uint32_t* out_index);
-/* Point material `material` at texture `texture` (-1 = solid colour). */
+/* Replace material `index`. */
 int lmn_desc_set_material(lmn_scene_desc* desc, uint32_t index, const lmn_material* m);
+/* Append a material; returns its index in *out_index. */
+int lmn_desc_add_material(lmn_scene_desc* desc, const lmn_material* m, uint32_t* out_index);
+/* Append an indexed triangle mesh as the loader appends a [[mesh]]: one
+ * triangle prim (and one world object) per face, in face order, vertices
+ * already scaled/offset (src/obj.rs:205, :232-239).  verts: n_verts*3
+ * doubles; tris: n_tris*3 vertex indices; materials: one material index per
+ * triangle, or NULL to use `material` for all.  Binary counterpart of the
+ * ASCII PLY path for large meshes. */
+int lmn_desc_add_triangles(lmn_scene_desc* desc, const double* verts, uint64_t n_verts, const uint32_t* tris,
+                           uint64_t n_tris, uint32_t material, const uint32_t* materials);
 
 /* Flatten, build the SAH BVH and upload a scene to the current HIP device. */
 typedef struct lmn_scene lmn_scene;
@@ -294,6 +304,9 @@ int lmn_render_device(const lmn_scene* scene, const lmn_render_params* params, f
  * and every scene's paths are issued from one work list. */
 int lmn_render_batch(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
                      float* const* hdr_rgb, lmn_stats* stats);
+/* Same with device framebuffers (d_hdr_rgb[i] device pointers). */
+int lmn_render_batch_device(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
+                            float* const* d_hdr_rgb, lmn_stats* stats);
 
 /* Closest-hit queries on a ray batch (SceneWorld::hit, src/scene.rs:817-834).
  * path: LMN_PATH_WAVEFRONT_FP32 traces in FP32, LMN_PATH_PLAIN_FP64 in FP64. */

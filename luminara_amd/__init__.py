@@ -82,6 +82,22 @@ class SceneWorld:
         self.invalidate()
         return idx.value
 
+    def add_material(self, mat: _abi.Material) -> int:
+        idx = C.c_uint32()
+        check(lib().lmn_desc_add_material(self._desc, C.byref(mat), C.byref(idx)))
+        self.invalidate()
+        return idx.value
+
+    def add_triangles(self, verts: np.ndarray, tris: np.ndarray, material: int = 0, materials=None):
+        """Appends a triangle mesh (one prim and world object per face)."""
+        v = np.ascontiguousarray(verts, dtype=np.float64)
+        t = np.ascontiguousarray(tris, dtype=np.uint32)
+        m = None if materials is None else np.ascontiguousarray(materials, dtype=np.uint32)
+        check(lib().lmn_desc_add_triangles(
+            self._desc, v.ctypes.data_as(C.POINTER(C.c_double)), len(v), t.ctypes.data_as(C.POINTER(C.c_uint32)),
+            len(t), material, None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint32))))
+        self.invalidate()
+
     def set_texture(self, index, rgb: np.ndarray):
         rgb = np.ascontiguousarray(rgb, dtype=np.float32)
         h, w, _ = rgb.shape
@@ -191,6 +207,29 @@ def render_device(world: SceneWorld, params: _abi.RenderParams, out_ptr: int) ->
     """Renders into a caller-owned device buffer (e.g. a torch CUDA tensor)."""
     st = _abi.Stats()
     check(lib().lmn_render_device(world.device_scene(), C.byref(params), C.c_void_p(out_ptr), C.byref(st)))
+    return {k: getattr(st, k) for k, _ in _abi.Stats._fields_}
+
+
+def render_batch(configs, worlds, path="wavefront", **kw):
+    """Renders several scenes in one wavefront (lmn_render_batch); returns
+    the list of framebuffers and the batch statistics."""
+    n = len(worlds)
+    scenes = (C.c_void_p * n)(*[w.device_scene().value for w in worlds])
+    params = (_abi.RenderParams * n)(*[make_params(c, path=path, scene_key=i, **kw) for i, c in enumerate(configs)])
+    outs = [np.empty(output_shape(c) + (3,), dtype=np.float32) for c in configs]
+    ptrs = (C.POINTER(C.c_float) * n)(*[o.ctypes.data_as(C.POINTER(C.c_float)) for o in outs])
+    st = _abi.Stats()
+    check(lib().lmn_render_batch(scenes, n, params, ptrs, C.byref(st)))
+    return outs, {k: getattr(st, k) for k, _ in _abi.Stats._fields_}
+
+
+def render_batch_device(worlds, params_list, out_ptrs) -> dict:
+    n = len(worlds)
+    scenes = (C.c_void_p * n)(*[w.device_scene().value for w in worlds])
+    params = (_abi.RenderParams * n)(*params_list)
+    ptrs = (C.c_void_p * n)(*out_ptrs)
+    st = _abi.Stats()
+    check(lib().lmn_render_batch_device(scenes, n, params, ptrs, C.byref(st)))
     return {k: getattr(st, k) for k, _ in _abi.Stats._fields_}
 
 

@@ -127,6 +127,9 @@ SIGNATURES = {
     "lmn_desc_add_texture": (C.c_int, [C.POINTER(SceneDesc), C.c_uint32, C.c_uint32, C.POINTER(C.c_float),
                                        C.POINTER(C.c_uint32)]),
     "lmn_desc_set_material": (C.c_int, [C.POINTER(SceneDesc), C.c_uint32, C.POINTER(Material)]),
+    "lmn_desc_add_material": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(Material), C.POINTER(C.c_uint32)]),
+    "lmn_desc_add_triangles": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_double), C.c_uint64,
+                                         C.POINTER(C.c_uint32), C.c_uint64, C.c_uint32, C.POINTER(C.c_uint32)]),
     "lmn_scene_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
     "lmn_scene_destroy": (None, [C.c_void_p]),
     "lmn_scene_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
@@ -135,6 +138,8 @@ SIGNATURES = {
     "lmn_render_device": (C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_void_p, C.POINTER(Stats)]),
     "lmn_render_batch": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(RenderParams),
                                    C.POINTER(C.POINTER(C.c_float)), C.POINTER(Stats)]),
+    "lmn_render_batch_device": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(RenderParams),
+                                          C.POINTER(C.c_void_p), C.POINTER(Stats)]),
     "lmn_trace_rays": (C.c_int, [C.c_void_p, C.POINTER(Ray), C.c_size_t, C.c_uint32, C.c_uint32,
                                  C.POINTER(Hit)]),
     "lmn_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_double)]),

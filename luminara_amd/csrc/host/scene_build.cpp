@@ -519,47 +519,66 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
       return LMN_ERR_INVALID;
     }
 
-  std::vector<uint32_t> bounded, planes;
-  for (uint32_t i = 0; i < desc.n_prims; ++i) (desc.prims[i].kind == LMN_PRIM_PLANE ? planes : bounded).push_back(i);
+  // main BVH over the analytic kinds, a second BVH over the marched kinds
+  // (torus, superellipsoid, spring), the unbounded planes last
+  std::vector<uint32_t> bounded, marched, planes;
+  for (uint32_t i = 0; i < desc.n_prims; ++i) {
+    uint32_t k = desc.prims[i].kind;
+    if (k == LMN_PRIM_PLANE)
+      planes.push_back(i);
+    else if (k >= LMN_PRIM_TORUS)
+      marched.push_back(i);
+    else
+      bounded.push_back(i);
+  }
   if (desc.n_prims >= LEAF_BIT) {
     err = "too many primitives";
     return LMN_ERR_INVALID;
   }
-  std::vector<double> blo(bounded.size() * 3), bhi(bounded.size() * 3);
-  Builder B(blo, bhi);
-  B.opt = opt;
-  B.refs.resize(bounded.size());
-  for (size_t i = 0; i < bounded.size(); ++i) {
-    prim_world_bounds(desc, desc.prims[bounded[i]], &blo[i * 3], &bhi[i * 3]);
-    for (int k = 0; k < 3; ++k) B.refs[i].c[k] = float(0.5 * (blo[i * 3 + k] + bhi[i * 3 + k]));
-    B.refs[i].idx = uint32_t(i);
-  }
-  if (!bounded.empty()) {
-    B.nodes.resize(2);  // root + unused slot keeps child pairs 64-byte aligned
-    B.nodes[1].b = LEAF_BIT;  // empty leaf, never referenced
-    B.build(0, 0, bounded.size(), 0);
-  }
-  out.tree_depth = B.depth_max;
-  out.n_leaves = B.leaves;
-  out.nodesD = B.nodes;
-  out.nodesF.resize(B.nodes.size());
-  for (size_t i = 0; i < B.nodes.size(); ++i) {
-    const NodeD& d = B.nodes[i];
-    NodeF& f = out.nodesF[i];
-    for (int k = 0; k < 3; ++k) {
-      // conservative FP32 boxes: outward rounding plus a few ulps for the
-      // FP32 rounding of the primitive parameters themselves
-      double pad = 4.0 * FLT_EPSILON * (std::fabs(d.lo[k]) + std::fabs(d.hi[k]));
-      f.lo[k] = round_down(d.lo[k] - pad);
-      f.hi[k] = round_up(d.hi[k] + pad);
+  auto build_tree = [&](const std::vector<uint32_t>& ids, uint32_t rec_base, std::vector<NodeD>& nodesD,
+                        std::vector<NodeF>& nodesF) {
+    std::vector<double> blo(ids.size() * 3), bhi(ids.size() * 3);
+    Builder B(blo, bhi);
+    B.opt = opt;
+    B.refs.resize(ids.size());
+    for (size_t i = 0; i < ids.size(); ++i) {
+      prim_world_bounds(desc, desc.prims[ids[i]], &blo[i * 3], &bhi[i * 3]);
+      for (int k = 0; k < 3; ++k) B.refs[i].c[k] = float(0.5 * (blo[i * 3 + k] + bhi[i * 3 + k]));
+      B.refs[i].idx = uint32_t(i);
     }
-    f.a = d.a;
-    f.b = d.b;
-  }
-  out.n_bounded = uint32_t(bounded.size());
-  out.n_planes = uint32_t(planes.size());
+    if (!ids.empty()) {
+      B.nodes.resize(2);  // root + unused slot keeps child pairs 64-byte aligned
+      B.nodes[1].b = LEAF_BIT;  // empty leaf, never referenced
+      B.build(0, 0, ids.size(), 0);
+    }
+    for (NodeD& n : B.nodes)
+      if (n.b & LEAF_BIT) n.a += rec_base;
+    out.tree_depth = std::max(out.tree_depth, B.depth_max);
+    out.n_leaves += B.leaves;
+    nodesD = B.nodes;
+    nodesF.resize(B.nodes.size());
+    for (size_t i = 0; i < B.nodes.size(); ++i) {
+      const NodeD& d = B.nodes[i];
+      NodeF& f = nodesF[i];
+      for (int k = 0; k < 3; ++k) {
+        // conservative FP32 boxes: outward rounding plus a few ulps for the
+        // FP32 rounding of the primitive parameters themselves
+        double pad = 4.0 * FLT_EPSILON * (std::fabs(d.lo[k]) + std::fabs(d.hi[k]));
+        f.lo[k] = round_down(d.lo[k] - pad);
+        f.hi[k] = round_up(d.hi[k] + pad);
+      }
+      f.a = d.a;
+      f.b = d.b;
+    }
+    for (uint32_t r : B.order) out.order.push_back(ids[r]);
+  };
   out.order.clear();
-  for (uint32_t r : B.order) out.order.push_back(bounded[r]);
+  out.tree_depth = 0;
+  out.n_leaves = 0;
+  build_tree(bounded, 0, out.nodesD, out.nodesF);
+  build_tree(marched, uint32_t(bounded.size()), out.mnodesD, out.mnodesF);
+  out.n_bounded = uint32_t(bounded.size() + marched.size());
+  out.n_planes = uint32_t(planes.size());
   for (uint32_t p : planes) out.order.push_back(p);
   size_t nrec = out.order.size();
   out.primsF.resize(nrec);

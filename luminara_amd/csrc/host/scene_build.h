@@ -19,8 +19,10 @@ struct BvhOptions {
 };
 
 struct HostScene {
-  std::vector<NodeF> nodesF;
+  std::vector<NodeF> nodesF;   // main BVH
   std::vector<NodeD> nodesD;
+  std::vector<NodeF> mnodesF;  // BVH over the marched kinds
+  std::vector<NodeD> mnodesD;
   std::vector<PrimT<float>> primsF;   // n_bounded in leaf order, then planes
   std::vector<PrimT<double>> primsD;
   std::vector<XformT<float>> xfF;

@@ -70,7 +70,7 @@ hipError_t upload(DevBuf& b, const std::vector<T>& v) {
 struct Workspace {
   std::mutex mu;
   DevBuf st[2][4];  // ob, db, bk, meta for the two pools
-  DevBuf hit, sh_o, sh_d, sh_c, samples, acc, spill, ctr, tc, rays, out;
+  DevBuf hit, sh_o, sh_d, sh_c, samples, acc, spill, ctr, tc, rays, out, views, parts, part_begin;
   uint32_t pool = 0;
   WaveCounters* host_ctr = nullptr;
   hipStream_t stream = nullptr;
@@ -98,7 +98,7 @@ Workspace& workspace() {
 
 struct lmn_scene {
   HostScene host;
-  DevBuf nodesF, nodesD, primsF, primsD, xfF, xfD, matF, matD, tex, texels, lightF, lightD;
+  DevBuf nodesF, nodesD, mnodesF, mnodesD, primsF, primsD, xfF, xfD, matF, matD, tex, texels, lightF, lightD;
   SceneView<float> viewF{};
   SceneView<double> viewD{};
   lmn_camera camera{};
@@ -161,6 +161,46 @@ int lmn_desc_set_material(lmn_scene_desc* desc, uint32_t index, const lmn_materi
   return LMN_OK;
 }
 
+int lmn_desc_add_material(lmn_scene_desc* desc, const lmn_material* m, uint32_t* out_index) {
+  if (!desc || !m) return set_err(LMN_ERR_INVALID, "lmn_desc_add_material: null argument");
+  auto* od = reinterpret_cast<OwnedDesc*>(desc);
+  if (m->texture >= 0 && uint32_t(m->texture) >= od->texels.size())
+    return set_err(LMN_ERR_INVALID, "lmn_desc_add_material: texture index out of range");
+  od->materials.push_back(*m);
+  od->sync();
+  if (out_index) *out_index = uint32_t(od->materials.size() - 1);
+  return LMN_OK;
+}
+
+int lmn_desc_add_triangles(lmn_scene_desc* desc, const double* verts, uint64_t n_verts, const uint32_t* tris,
+                           uint64_t n_tris, uint32_t material, const uint32_t* materials) {
+  if (!desc || (n_tris && (!verts || !tris))) return set_err(LMN_ERR_INVALID, "lmn_desc_add_triangles: null argument");
+  auto* od = reinterpret_cast<OwnedDesc*>(desc);
+  if (od->prims.size() + n_tris >= (1ull << 31)) return set_err(LMN_ERR_INVALID, "too many primitives");
+  try {
+    od->prims.reserve(od->prims.size() + n_tris);
+    for (uint64_t t = 0; t < n_tris; ++t) {
+      uint32_t m = materials ? materials[t] : material;
+      if (m >= od->materials.size()) return set_err(LMN_ERR_INVALID, "lmn_desc_add_triangles: material out of range");
+      lmn_prim p{};
+      p.kind = LMN_PRIM_TRIANGLE;
+      p.material = m;
+      p.transform = -1;
+      p.object = od->n_objects++;
+      for (int c = 0; c < 3; ++c) {
+        uint32_t vi = tris[t * 3 + c];
+        if (vi >= n_verts) return set_err(LMN_ERR_INVALID, "lmn_desc_add_triangles: vertex index out of range");
+        for (int k = 0; k < 3; ++k) p.p[c * 3 + k] = verts[size_t(vi) * 3 + k];
+      }
+      od->prims.push_back(p);
+    }
+  } catch (const std::bad_alloc&) {
+    return set_err(LMN_ERR_NOMEM, "out of host memory");
+  }
+  od->sync();
+  return LMN_OK;
+}
+
 int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   if (!desc || !out) return set_err(LMN_ERR_INVALID, "lmn_scene_create: null argument");
   *out = nullptr;
@@ -174,6 +214,8 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   HostScene& h = s->host;
   HIP_TRY(upload(s->nodesF, h.nodesF));
   HIP_TRY(upload(s->nodesD, h.nodesD));
+  HIP_TRY(upload(s->mnodesF, h.mnodesF));
+  HIP_TRY(upload(s->mnodesD, h.mnodesD));
   HIP_TRY(upload(s->primsF, h.primsF));
   HIP_TRY(upload(s->primsD, h.primsD));
   HIP_TRY(upload(s->xfF, h.xfF));
@@ -184,11 +226,13 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   HIP_TRY(upload(s->texels, h.texels));
   HIP_TRY(upload(s->lightF, h.lightF));
   HIP_TRY(upload(s->lightD, h.lightD));
-  s->device_bytes = s->nodesF.bytes + s->nodesD.bytes + s->primsF.bytes + s->primsD.bytes + s->xfF.bytes +
+  s->device_bytes = s->nodesF.bytes + s->nodesD.bytes + s->mnodesF.bytes + s->mnodesD.bytes + s->primsF.bytes + s->primsD.bytes + s->xfF.bytes +
                     s->xfD.bytes + s->matF.bytes + s->matD.bytes + s->tex.bytes + s->texels.bytes +
                     s->lightF.bytes + s->lightD.bytes;
   SceneView<float>& F = s->viewF;
   F.nodes = s->nodesF.as<NodeF>();
+  F.mnodes = s->mnodesF.as<NodeF>();
+  F.n_mnodes = uint32_t(h.mnodesF.size());
   F.prims = s->primsF.as<PrimT<float>>();
   F.prims64 = s->primsD.as<PrimT<double>>();
   F.xforms = s->xfF.as<XformT<float>>();
@@ -205,6 +249,8 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   F.cam = h.camF;
   SceneView<double>& D = s->viewD;
   D.nodes = s->nodesD.as<NodeD>();
+  D.mnodes = s->mnodesD.as<NodeD>();
+  D.n_mnodes = F.n_mnodes;
   D.prims = s->primsD.as<PrimT<double>>();
   D.prims64 = D.prims;
   D.xforms = s->xfD.as<XformT<double>>();
@@ -230,7 +276,7 @@ void lmn_scene_destroy(lmn_scene* scene) { delete scene; }
 int lmn_scene_info(const lmn_scene* s, uint64_t* n_nodes, uint64_t* n_leaves, uint32_t* max_depth,
                    uint64_t* device_bytes) {
   if (!s) return set_err(LMN_ERR_INVALID, "lmn_scene_info: null scene");
-  if (n_nodes) *n_nodes = s->host.nodesF.size();
+  if (n_nodes) *n_nodes = s->host.nodesF.size() + s->host.mnodesF.size();
   if (n_leaves) *n_leaves = s->host.n_leaves;
   if (max_depth) *max_depth = s->host.tree_depth;
   if (device_bytes) *device_bytes = s->device_bytes;
@@ -284,36 +330,79 @@ int frame_setup(const lmn_render_params* p, FrameSetup& fs) {
 
 constexpr uint32_t kMaxTreeDepth = 48;
 
-int render_wavefront(lmn_scene* s, const lmn_render_params* p, const FrameSetup& fs, float* d_out, hipStream_t st,
-                     lmn_stats* stats, TraceCounters* d_tc) {
+// Wavefront over a batch of scenes: every scene's work items are issued from
+// one list (scene by scene, sample pass by sample pass), so one set of
+// kernel launches per iteration serves the whole batch.  Items are grouped in
+// chunks of whole sample passes (at most kMaxChunkItems samples buffered).
+int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* params, const FrameSetup* fs,
+                     float* const* d_out, size_t n, hipStream_t st, lmn_stats* stats, TraceCounters* d_tc) {
   Workspace& W = workspace();
-  FrameDev F = fs.F;
-  uint32_t npix = fs.npix;
-  uint64_t total = F.n_items;
-  uint32_t pool = p->pool_paths ? p->pool_paths : uint32_t(std::min<uint64_t>(total, 1u << 22));
+  const lmn_render_params* p0 = params[0];
+  uint64_t total = 0;
+  uint32_t max_npix = 0, feat = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (fs[i].black) continue;
+    total += fs[i].F.n_items;
+    max_npix = std::max(max_npix, fs[i].npix);
+    feat |= scenes[i]->host.featF;
+  }
+  if (feat & FEAT_MARCH) feat |= FEAT_F64;
+  uint32_t pool = p0->pool_paths ? p0->pool_paths : uint32_t(std::min<uint64_t>(std::max<uint64_t>(total, 64), 1u << 22));
   pool = std::max<uint32_t>(pool, 64);
-  uint32_t ss = p->stack_lds ? p->stack_lds : 16;
+  uint32_t ss = p0->stack_lds ? p0->stack_lds : 16;
   if (ss != 16 && ss != 32) return set_err(LMN_ERR_INVALID, "stack_lds must be 16 or 32");
-  // sample buffer chunk: whole sample passes, at most 2^27 samples (2 GiB)
-  uint64_t max_chunk = 1ull << 27;
-  uint32_t spp_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(F.spp_a, max_chunk / npix)));
-  uint64_t chunk_items = uint64_t(spp_chunk) * npix;
+  const uint64_t kMaxChunkItems = 1ull << 27;  // 2 GiB of float4 samples
+  // parts: whole sample passes of one scene
+  std::vector<FrameDev> parts;
+  for (size_t i = 0; i < n; ++i) {
+    if (fs[i].black) continue;
+    FrameDev F = fs[i].F;
+    F.scene = uint32_t(i);
+    uint32_t npix = fs[i].npix;
+    uint32_t spp_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(F.spp_a, kMaxChunkItems / npix)));
+    for (uint32_t s0 = 0; s0 < F.spp_a; s0 += spp_chunk) {
+      uint32_t s1 = std::min(F.spp_a, s0 + spp_chunk);
+      FrameDev P = F;
+      P.item_base = uint64_t(s0) * npix;
+      P.n_items = uint64_t(s1 - s0) * npix;
+      parts.push_back(P);
+    }
+  }
+  // chunks: consecutive parts up to kMaxChunkItems items
+  std::vector<std::pair<size_t, size_t>> chunks;
+  uint64_t chunk_max = 0;
+  for (size_t b = 0; b < parts.size();) {
+    size_t e = b;
+    uint64_t items = 0;
+    while (e < parts.size() && (e == b || items + parts[e].n_items <= kMaxChunkItems)) items += parts[e++].n_items;
+    chunks.push_back({b, e});
+    chunk_max = std::max(chunk_max, items);
+    b = e;
+  }
   uint32_t grid = std::min<uint32_t>((pool + 255) / 256, 256 * 8);
   uint32_t nthreads = grid * 256;
-  for (int k = 0; k < 2; ++k) {
-    HIP_TRY(W.st[k][0].ensure(size_t(pool) * 16));
-    HIP_TRY(W.st[k][1].ensure(size_t(pool) * 16));
-    HIP_TRY(W.st[k][2].ensure(size_t(pool) * 16));
-    HIP_TRY(W.st[k][3].ensure(size_t(pool) * 16));
-  }
+  for (int k = 0; k < 2; ++k)
+    for (int f = 0; f < 4; ++f) HIP_TRY(W.st[k][f].ensure(size_t(pool) * 16));
   HIP_TRY(W.hit.ensure(size_t(pool) * 8));
   HIP_TRY(W.sh_o.ensure(size_t(pool) * 16));
   HIP_TRY(W.sh_d.ensure(size_t(pool) * 16));
   HIP_TRY(W.sh_c.ensure(size_t(pool) * 16));
-  HIP_TRY(W.samples.ensure(size_t(chunk_items) * 16));
-  HIP_TRY(W.acc.ensure(size_t(npix) * 32));
+  HIP_TRY(W.samples.ensure(size_t(std::max<uint64_t>(chunk_max, 1)) * 16));
   HIP_TRY(W.spill.ensure(size_t(kMaxTreeDepth) * nthreads * 4));
   HIP_TRY(W.ctr.ensure(sizeof(WaveCounters)));
+  // per-scene accumulators, scene views, parts
+  std::vector<size_t> acc_off(n + 1, 0);
+  for (size_t i = 0; i < n; ++i) acc_off[i + 1] = acc_off[i] + (fs[i].black ? 0 : fs[i].npix);
+  HIP_TRY(W.acc.ensure(std::max<size_t>(acc_off[n], 1) * 32));
+  std::vector<SceneView<float>> views(n);
+  for (size_t i = 0; i < n; ++i) {
+    views[i] = scenes[i]->viewF;
+    views[i].exact_marched = params[i]->intersect_mode == LMN_ISECT_EXACT;
+  }
+  HIP_TRY(W.views.ensure(n * sizeof(SceneView<float>)));
+  HIP_TRY(hipMemcpyAsync(W.views.p, views.data(), n * sizeof(SceneView<float>), hipMemcpyHostToDevice, st));
+  HIP_TRY(W.parts.ensure(std::max<size_t>(parts.size(), 1) * sizeof(FrameDev)));
+  HIP_TRY(W.part_begin.ensure((parts.size() + 1) * sizeof(uint64_t)));
   PathSoA P[2];
   for (int k = 0; k < 2; ++k)
     P[k] = PathSoA{W.st[k][0].as<float4>(), W.st[k][1].as<float4>(), W.st[k][2].as<uint4>(), W.st[k][3].as<uint4>()};
@@ -322,31 +411,34 @@ int render_wavefront(lmn_scene* s, const lmn_render_params* p, const FrameSetup&
   L.grid = grid;
   L.pool = pool;
   L.stack_lds = ss;
-  L.feat = s->host.featF;
+  L.feat = feat;
   L.hit = W.hit.as<float2>();
   L.sh = ShadowSoA{W.sh_o.as<float4>(), W.sh_d.as<float4>(), W.sh_c.as<float4>()};
   L.samples = W.samples.as<float4>();
   L.ctr = W.ctr.as<WaveCounters>();
   L.spill = W.spill.as<uint32_t>();
   L.tc = d_tc;
-  SceneView<float> S = s->viewF;
-  S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
-  launch_zero_acc(W.acc.as<double4>(), npix, st);
+  for (size_t i = 0; i < n; ++i)
+    if (!fs[i].black) launch_zero_acc(W.acc.as<double4>() + acc_off[i], fs[i].npix, st);
   uint64_t sum_ext = 0, sum_sh = 0, sum_cam = 0, iters = 0;
   const int block = Workspace::kBlock;
-  const bool timing = (p->flags & LMN_FLAG_STAGE_TIMING) != 0;
+  const bool timing = (p0->flags & LMN_FLAG_STAGE_TIMING) != 0;
   double t_stage[4] = {0, 0, 0, 0};
   uint32_t n_launch = 0;
-  for (uint32_t s0 = 0; s0 < F.spp_a; s0 += spp_chunk) {
-    uint32_t s1 = std::min(F.spp_a, s0 + spp_chunk);
-    F.item_base = uint64_t(s0) * npix;
-    uint64_t items = uint64_t(s1 - s0) * npix;
-    launch_counters_init(L.ctr, items, st);
+  // parts of a chunk: uploaded in one copy, chunk after chunk
+  std::vector<uint64_t> begins;
+  for (auto [pb, pe] : chunks) {
+    begins.assign(1, 0);
+    for (size_t q = pb; q < pe; ++q) begins.push_back(begins.back() + parts[q].n_items);
+    HIP_TRY(hipMemcpyAsync(W.parts.p, parts.data() + pb, (pe - pb) * sizeof(FrameDev), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(W.part_begin.p, begins.data(), begins.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    BatchDev B{W.views.as<SceneView<float>>(), W.parts.as<FrameDev>(), W.part_begin.as<uint64_t>(), uint32_t(pe - pb), 0};
+    launch_counters_init(L.ctr, begins.back(), st);
     int cur = 0;
     auto t_start = std::chrono::steady_clock::now();
     for (;;) {
       for (int k = 0; k < block; ++k) {
-        launch_wave_iteration(L, S, F, P[cur], P[cur ^ 1], d_tc != nullptr, timing ? W.stage_ev[k] : nullptr);
+        launch_wave_iteration(L, B, P[cur], P[cur ^ 1], timing ? W.stage_ev[k] : nullptr);
         cur ^= 1;
       }
       HIP_TRY(hipGetLastError());
@@ -372,9 +464,15 @@ int render_wavefront(lmn_scene* s, const lmn_render_params* p, const FrameSetup&
       if (std::chrono::steady_clock::now() - t_start > std::chrono::seconds(600))
         return set_err(LMN_ERR_DEVICE, "wavefront did not drain within 600 s");
     }
-    launch_resolve(L.samples, W.acc.as<double4>(), npix, s1 - s0, st);
+    for (size_t q = pb; q < pe; ++q) {
+      const FrameDev& F = parts[q];
+      uint32_t npix = fs[F.scene].npix;
+      launch_resolve(L.samples + begins[q - pb], W.acc.as<double4>() + acc_off[F.scene], npix,
+                     uint32_t(F.n_items / npix), st);
+    }
   }
-  launch_finalize(W.acc.as<double4>(), d_out, npix, st);
+  for (size_t i = 0; i < n; ++i)
+    if (!fs[i].black) launch_finalize(W.acc.as<double4>() + acc_off[i], d_out[i], fs[i].npix, st);
   HIP_TRY(hipGetLastError());
   if (stats) {
     stats->camera_rays = sum_cam;
@@ -423,24 +521,42 @@ int render_plain(lmn_scene* s, const lmn_render_params* p, const FrameSetup& fs,
   return LMN_OK;
 }
 
-int render_common(lmn_scene* s, const lmn_render_params* p, float* d_out, lmn_stats* stats, TraceCounters* d_tc) {
-  if (!s || !p || !d_out) return set_err(LMN_ERR_INVALID, "render: null argument");
-  FrameSetup fs;
-  int rc = frame_setup(p, fs);
-  if (rc != LMN_OK) return rc;
+// Renders n scenes; the wavefront path serves them in one batch, the plain
+// kernels one after another.  d_out[i]: device framebuffer of scene i.
+int render_common(lmn_scene* const* scenes, const lmn_render_params* const* params, float* const* d_out, size_t n,
+                  lmn_stats* stats, TraceCounters* d_tc) {
+  if (!scenes || !params || !d_out || n == 0) return set_err(LMN_ERR_INVALID, "render: null argument");
+  std::vector<FrameSetup> fs(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (!scenes[i] || !params[i] || !d_out[i]) return set_err(LMN_ERR_INVALID, "render: null argument");
+    int rc = frame_setup(params[i], fs[i]);
+    if (rc != LMN_OK) return rc;
+    if (params[i]->path != params[0]->path) return set_err(LMN_ERR_INVALID, "render: one path per batch");
+  }
   Workspace& W = workspace();
   std::lock_guard<std::mutex> lock(W.mu);
-  rc = W.init();
+  int rc = W.init();
   if (rc != LMN_OK) return rc;
-  hipStream_t st = p->stream ? static_cast<hipStream_t>(p->stream) : W.stream;
+  const lmn_render_params* p0 = params[0];
+  hipStream_t st = p0->stream ? static_cast<hipStream_t>(p0->stream) : W.stream;
   if (stats) std::memset(stats, 0, sizeof *stats);
   HIP_TRY(hipEventRecord(W.ev0, st));
-  if (fs.black) {
-    HIP_TRY(hipMemsetAsync(d_out, 0, size_t(fs.npix) * 12, st));
-  } else if (p->path == LMN_PATH_WAVEFRONT_FP32) {
-    rc = render_wavefront(s, p, fs, d_out, st, stats, d_tc);
-  } else if (p->path == LMN_PATH_PLAIN_FP64 || p->path == LMN_PATH_PLAIN_FP32) {
-    rc = render_plain(s, p, fs, d_out, st, stats, p->path == LMN_PATH_PLAIN_FP64);
+  for (size_t i = 0; i < n; ++i)
+    if (fs[i].black) HIP_TRY(hipMemsetAsync(d_out[i], 0, size_t(fs[i].npix) * 12, st));
+  if (p0->path == LMN_PATH_WAVEFRONT_FP32) {
+    rc = render_wavefront(scenes, params, fs.data(), d_out, n, st, stats, d_tc);
+  } else if (p0->path == LMN_PATH_PLAIN_FP64 || p0->path == LMN_PATH_PLAIN_FP32) {
+    for (size_t i = 0; i < n && rc == LMN_OK; ++i) {
+      if (fs[i].black) continue;
+      lmn_stats one{};
+      rc = render_plain(scenes[i], params[i], fs[i], d_out[i], st, stats ? &one : nullptr,
+                        p0->path == LMN_PATH_PLAIN_FP64);
+      if (stats) {
+        stats->camera_rays += one.camera_rays;
+        stats->extension_rays += one.extension_rays;
+        stats->shadow_rays += one.shadow_rays;
+      }
+    }
   } else {
     return set_err(LMN_ERR_INVALID, "unknown render path");
   }
@@ -450,7 +566,8 @@ int render_common(lmn_scene* s, const lmn_render_params* p, float* d_out, lmn_st
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, W.ev0, W.ev1));
   if (stats) {
-    stats->paths = uint64_t(fs.npix) * fs.F.spp_a;
+    stats->paths = 0;
+    for (size_t i = 0; i < n; ++i) stats->paths += uint64_t(fs[i].npix) * fs[i].F.spp_a;
     stats->render_ms = ms;
   }
   return LMN_OK;
@@ -461,44 +578,50 @@ int render_common(lmn_scene* s, const lmn_render_params* p, float* d_out, lmn_st
 extern "C" {
 
 int lmn_render_device(const lmn_scene* scene, const lmn_render_params* params, float* d_hdr_rgb, lmn_stats* stats) {
-  return render_common(const_cast<lmn_scene*>(scene), params, d_hdr_rgb, stats, nullptr);
+  lmn_scene* sc = const_cast<lmn_scene*>(scene);
+  return render_common(&sc, &params, &d_hdr_rgb, 1, stats, nullptr);
 }
 
 int lmn_render(const lmn_scene* scene, const lmn_render_params* params, float* hdr_rgb, lmn_stats* stats) {
-  if (!scene || !params || !hdr_rgb) return set_err(LMN_ERR_INVALID, "lmn_render: null argument");
-  FrameSetup fs;
-  int rc = frame_setup(params, fs);
-  if (rc != LMN_OK) return rc;
-  size_t bytes = size_t(fs.npix) * 12;
-  float* d = nullptr;
-  HIP_TRY(hipMalloc((void**)&d, bytes));
-  rc = render_common(const_cast<lmn_scene*>(scene), params, d, stats, nullptr);
-  if (rc == LMN_OK) {
-    hipError_t e = hipMemcpy(hdr_rgb, d, bytes, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) rc = set_err(LMN_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  const lmn_scene* const sc[1] = {scene};
+  float* const out[1] = {hdr_rgb};
+  return lmn_render_batch(sc, 1, params, out, stats);
+}
+
+int lmn_render_batch_device(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
+                            float* const* d_hdr_rgb, lmn_stats* stats) {
+  if (!scenes || !params || !d_hdr_rgb || !n_scenes) return set_err(LMN_ERR_INVALID, "lmn_render_batch: bad argument");
+  std::vector<lmn_scene*> sc(n_scenes);
+  std::vector<const lmn_render_params*> pp(n_scenes);
+  for (size_t i = 0; i < n_scenes; ++i) {
+    sc[i] = const_cast<lmn_scene*>(scenes[i]);
+    pp[i] = &params[i];
   }
-  (void)hipFree(d);
-  return rc;
+  return render_common(sc.data(), pp.data(), d_hdr_rgb, n_scenes, stats, nullptr);
 }
 
 int lmn_render_batch(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
                      float* const* hdr_rgb, lmn_stats* stats) {
-  if (!scenes || !params || !hdr_rgb) return set_err(LMN_ERR_INVALID, "lmn_render_batch: null argument");
-  if (stats) std::memset(stats, 0, sizeof *stats);
-  for (size_t i = 0; i < n_scenes; ++i) {
-    lmn_stats st{};
-    int rc = lmn_render(scenes[i], &params[i], hdr_rgb[i], &st);
-    if (rc != LMN_OK) return rc;
-    if (stats) {
-      stats->paths += st.paths;
-      stats->camera_rays += st.camera_rays;
-      stats->extension_rays += st.extension_rays;
-      stats->shadow_rays += st.shadow_rays;
-      stats->render_ms += st.render_ms;
-      stats->iterations += st.iterations;
-    }
+  if (!scenes || !params || !hdr_rgb || !n_scenes) return set_err(LMN_ERR_INVALID, "lmn_render_batch: bad argument");
+  std::vector<size_t> bytes(n_scenes);
+  std::vector<float*> d(n_scenes, nullptr);
+  int rc = LMN_OK;
+  for (size_t i = 0; i < n_scenes && rc == LMN_OK; ++i) {
+    FrameSetup fs;
+    rc = frame_setup(&params[i], fs);
+    if (rc != LMN_OK) break;
+    bytes[i] = size_t(fs.npix) * 12;
+    if (hipMalloc((void**)&d[i], bytes[i]) != hipSuccess) rc = set_err(LMN_ERR_NOMEM, "hipMalloc of a framebuffer failed");
   }
-  return LMN_OK;
+  if (rc == LMN_OK) rc = lmn_render_batch_device(scenes, n_scenes, params, d.data(), stats);
+  for (size_t i = 0; i < n_scenes; ++i) {
+    if (rc == LMN_OK && d[i]) {
+      hipError_t e = hipMemcpy(hdr_rgb[i], d[i], bytes[i], hipMemcpyDeviceToHost);
+      if (e != hipSuccess) rc = set_err(LMN_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    if (d[i]) (void)hipFree(d[i]);
+  }
+  return rc;
 }
 
 int lmn_trace_rays(const lmn_scene* scene, const lmn_ray* rays, size_t n, uint32_t path, uint32_t intersect_mode,
@@ -557,7 +680,9 @@ int lmn_profile_extend(const lmn_scene* scene, const lmn_render_params* params, 
   TraceCounters* d_tc = nullptr;
   HIP_TRY(hipMalloc((void**)&d_tc, sizeof(TraceCounters)));
   hipError_t e = hipMemset(d_tc, 0, sizeof(TraceCounters));
-  int rc = e == hipSuccess ? render_common(const_cast<lmn_scene*>(scene), &p, d_hdr_rgb, nullptr, d_tc)
+  lmn_scene* sc = const_cast<lmn_scene*>(scene);
+  const lmn_render_params* pp = &p;
+  int rc = e == hipSuccess ? render_common(&sc, &pp, &d_hdr_rgb, 1, nullptr, d_tc)
                            : set_err(LMN_ERR_DEVICE, "hipMemset failed");
   if (rc == LMN_OK) {
     TraceCounters h;

@@ -10,15 +10,15 @@ namespace lmn {
 //   ob   = origin.xyz, beta.x     db   = direction.xyz, beta.y
 //   meta = work item, remaining depth | skip-emission flag << 16,
 //          rng key lo, rng key hi
-//   aux  = rng counter, beta.z (bit pattern), 2 spare words
+//   aux  = rng counter, beta.z (bit pattern), scene index, spare
 struct PathSoA {
   float4* ob;
   float4* db;
   uint4* meta;
   uint4* aux;
 };
-// Shadow-ray queue: o = origin.xyz, distance; d = unit dir.xyz, work item;
-// c = unoccluded contribution.rgb
+// Shadow-ray queue: o = origin.xyz, distance (negated once occluded);
+// d = unit dir.xyz, work item; c = unoccluded contribution.rgb, scene index
 struct ShadowSoA {
   float4* o;
   float4* d;
@@ -38,6 +38,17 @@ struct WaveCounters {
   uint64_t iterations;
 };
 
+// Scenes of one wavefront launch: per-scene views (device array), the
+// chunk's parts (FrameDev each) and their first work items (n_parts + 1
+// prefix, chunk-relative).
+struct BatchDev {
+  const SceneView<float>* views;
+  const FrameDev* parts;
+  const uint64_t* part_begin;
+  uint32_t n_parts;
+  uint32_t pad;
+};
+
 struct WaveLaunch {
   hipStream_t stream;
   uint32_t grid;       // workgroups of 256 lanes (grid-stride kernels)
@@ -54,8 +65,8 @@ struct WaveLaunch {
 
 // One wavefront iteration: extend, shade, shadow, regenerate, bookkeeping.
 // With `ev` (5 events), ev[0..4] bracket the stages for per-stage timing.
-void launch_wave_iteration(const WaveLaunch& L, const SceneView<float>& S, const FrameDev& F, const PathSoA& cur,
-                           const PathSoA& nxt, bool count, hipEvent_t* ev = nullptr);
+void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt,
+                           hipEvent_t* ev = nullptr);
 void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s);
 void launch_resolve(const float4* samples, double4* acc, uint32_t npix, uint32_t n_samples, hipStream_t s);
 void launch_zero_acc(double4* acc, uint32_t npix, hipStream_t s);

@@ -52,76 +52,177 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, bool want) {
 }
 
 // ---------------------------------------------------------------------------
+// Wave-uniform scene selection.  Paths of one scene are issued together, so
+// a wave almost always holds one scene; the loop peels one scene per pass
+// (readfirstlane makes the index wave-uniform, the scene's view then lives
+// in scalar registers) until every lane has been served.
+template <class F>
+__device__ __forceinline__ void for_scene(uint32_t sid, F&& body) {
+  for (;;) {
+    uint32_t s0 = uint32_t(__builtin_amdgcn_readfirstlane(int(sid)));
+    if (sid == s0) {
+      body(s0);
+      return;
+    }
+  }
+}
+
+__device__ __forceinline__ void add_sample(float4* samples, uint32_t item, V3<float> c) {
+  float4 s = samples[item];
+  s.x += c.x;
+  s.y += c.y;
+  s.z += c.z;
+  samples[item] = s;
+}
+
+// ---------------------------------------------------------------------------
 // Wavefront stages
 
-template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_extend(SceneView<float> S, PathSoA cur, float2* __restrict__ hit,
-                                               const WaveCounters* __restrict__ ctr, uint32_t* spill) {
+// Extend, main pass: closest hit over the main BVH and the planes.
+template <int SS, int BS, uint32_t FEAT, bool COUNT>
+__device__ __forceinline__ void extend_body(const BatchDev& B, const PathSoA& cur, float2* hit,
+                                            const WaveCounters* ctr, uint32_t* spill, TraceCounters* tc) {
   __shared__ uint32_t lds[SS * BS];
   uint32_t n = ctr->active;
   uint32_t nthreads = gridDim.x * BS;
   uint32_t gid = blockIdx.x * BS + threadIdx.x;
   for (uint32_t i = gid; i < n; i += nthreads) {
     float4 a = cur.ob[i], b = cur.db[i];
-    LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
-    float t;
-    uint32_t idx;
-    bool h = trace<float, FEAT, false>(S, f3(a), f3(b), 0.001f, INFINITY, t, idx, st);
-    hit[i] = make_float2(t, __uint_as_float(h ? idx : 0xffffffffu));
+    uint32_t sid = cur.aux[i].z;
+    for_scene(sid, [&](uint32_t s) {
+      const SceneView<float> S = B.views[s];
+      LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
+      float t = INFINITY;
+      uint32_t idx = 0;
+      bool h = trace_pass<float, FEAT, false, PASS_MAIN>(S, f3(a), f3(b), 0.001f, t, idx, st, COUNT ? tc : nullptr);
+      hit[i] = make_float2(t, __uint_as_float(h ? idx : 0xffffffffu));
+    });
+  }
+}
+template <int SS, int BS, uint32_t FEAT>
+__global__ __launch_bounds__(BS) void k_extend(BatchDev B, PathSoA cur, float2* __restrict__ hit,
+                                               const WaveCounters* __restrict__ ctr, uint32_t* spill) {
+  extend_body<SS, BS, FEAT, false>(B, cur, hit, ctr, spill, nullptr);
+}
+// Instrumented extend (counts node fetches and primitive tests): a kernel of
+// its own name so that profiles keep it apart from the timed k_extend.
+template <int SS, int BS, uint32_t FEAT>
+__global__ __launch_bounds__(BS) void k_extend_count(BatchDev B, PathSoA cur, float2* __restrict__ hit,
+                                                     const WaveCounters* __restrict__ ctr, uint32_t* spill,
+                                                     TraceCounters* tc) {
+  extend_body<SS, BS, FEAT, true>(B, cur, hit, ctr, spill, tc);
+}
+
+// Extend, marched pass: the FP64 torus/superellipsoid/spring BVH, limited
+// to the main pass's hit distance.
+template <int SS, int BS, uint32_t FEAT>
+__global__ __launch_bounds__(BS) void k_extend_march(BatchDev B, PathSoA cur, float2* __restrict__ hit,
+                                                     const WaveCounters* __restrict__ ctr, uint32_t* spill) {
+  __shared__ uint32_t lds[SS * BS];
+  uint32_t n = ctr->active;
+  uint32_t nthreads = gridDim.x * BS;
+  uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  for (uint32_t i = gid; i < n; i += nthreads) {
+    float4 a = cur.ob[i], b = cur.db[i];
+    uint32_t sid = cur.aux[i].z;
+    for_scene(sid, [&](uint32_t s) {
+      const SceneView<float> S = B.views[s];
+      if (S.n_mnodes == 0) return;
+      float2 h0 = hit[i];
+      LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
+      float t = __float_as_uint(h0.y) == 0xffffffffu ? INFINITY : h0.x;
+      uint32_t idx = 0;
+      if (trace_pass<float, FEAT, false, PASS_MARCH>(S, f3(a), f3(b), 0.001f, t, idx, st))
+        hit[i] = make_float2(t, __uint_as_float(idx));
+    });
+  }
+}
+
+// Shadow rays: any hit over [0.001, dist - 0.001] (src/render.rs:1457-1458).
+// Without marched kinds the contribution of a visible light sample is added
+// here; with them an occluded ray's distance is negated and k_shadow_march
+// finishes the query.
+template <int SS, int BS, uint32_t FEAT>
+__global__ __launch_bounds__(BS) void k_shadow(BatchDev B, ShadowSoA sh, float4* __restrict__ samples,
+                                               const WaveCounters* __restrict__ ctr, uint32_t* spill) {
+  __shared__ uint32_t lds[SS * BS];
+  constexpr bool DEFER = (FEAT & FEAT_MARCH) != 0;
+  uint32_t n = ctr->n_shadow;
+  uint32_t nthreads = gridDim.x * BS;
+  uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  for (uint32_t i = gid; i < n; i += nthreads) {
+    float4 o = sh.o[i], d = sh.d[i], c = sh.c[i];
+    uint32_t sid = __float_as_uint(c.w);
+    for_scene(sid, [&](uint32_t s) {
+      const SceneView<float> S = B.views[s];
+      LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
+      float t = o.w - 0.001f;
+      uint32_t idx = 0;
+      bool occluded = trace_pass<float, FEAT & ~FEAT_MARCH, true, PASS_MAIN>(S, f3(o), f3(d), 0.001f, t, idx, st);
+      if (DEFER) {
+        if (occluded) sh.o[i].w = -o.w;
+      } else if (!occluded) {
+        add_sample(samples, __float_as_uint(d.w), f3(c));
+      }
+    });
   }
 }
 
 template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_shadow(SceneView<float> S, ShadowSoA sh, float4* __restrict__ samples,
-                                               const WaveCounters* __restrict__ ctr, uint32_t* spill) {
+__global__ __launch_bounds__(BS) void k_shadow_march(BatchDev B, ShadowSoA sh, float4* __restrict__ samples,
+                                                     const WaveCounters* __restrict__ ctr, uint32_t* spill) {
   __shared__ uint32_t lds[SS * BS];
   uint32_t n = ctr->n_shadow;
   uint32_t nthreads = gridDim.x * BS;
   uint32_t gid = blockIdx.x * BS + threadIdx.x;
   for (uint32_t i = gid; i < n; i += nthreads) {
-    float4 o = sh.o[i], d = sh.d[i];
-    LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
-    float t;
-    uint32_t idx;
-    // shadow query over [0.001, dist - 0.001] (src/render.rs:1457-1458)
-    bool occluded = trace<float, FEAT, true>(S, f3(o), f3(d), 0.001f, o.w - 0.001f, t, idx, st);
-    if (!occluded) {
-      float4 c = sh.c[i];
-      uint32_t item = __float_as_uint(d.w);
-      float4 s = samples[item];
-      s.x += c.x;
-      s.y += c.y;
-      s.z += c.z;
-      samples[item] = s;
-    }
+    float4 o = sh.o[i];
+    if (!(o.w > 0.f)) continue;  // occluded in the main pass
+    float4 d = sh.d[i], c = sh.c[i];
+    uint32_t sid = __float_as_uint(c.w);
+    for_scene(sid, [&](uint32_t s) {
+      const SceneView<float> S = B.views[s];
+      LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
+      float t = o.w - 0.001f;
+      uint32_t idx = 0;
+      if (S.n_mnodes == 0 || !trace_pass<float, FEAT, true, PASS_MARCH>(S, f3(o), f3(d), 0.001f, t, idx, st))
+        add_sample(samples, __float_as_uint(d.w), f3(c));
+    });
   }
 }
 
+// Shade: one recursion level of ray_color (src/render.rs:1472-1537) per
+// path -- emission, scatter, Russian roulette, NEE light sample -- then
+// wave-ballot compaction of the shadow queue and of the surviving paths.
 template <int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_shade(SceneView<float> S, FrameDev F, PathSoA cur, const float2* __restrict__ hit,
-                                              PathSoA nxt, ShadowSoA sh, float4* __restrict__ samples,
+__global__ __launch_bounds__(BS) void k_shade(BatchDev B, PathSoA cur, const float2* __restrict__ hit, PathSoA nxt,
+                                              ShadowSoA sh, float4* __restrict__ samples,
                                               WaveCounters* __restrict__ ctr) {
   uint32_t n = ctr->active;
   uint32_t nthreads = gridDim.x * BS;
   for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < n; i += nthreads) {
-    float4 A = cur.ob[i], B = cur.db[i];
+    float4 A = cur.ob[i], Bd = cur.db[i];
     uint4 M = cur.meta[i], X = cur.aux[i];
-    V3<float> o = f3(A), d = f3(B), beta = {A.w, B.w, __uint_as_float(X.y)};
+    V3<float> o = f3(A), d = f3(Bd), beta = {A.w, Bd.w, __uint_as_float(X.y)};
     Rng rng;
     rng.key = (uint64_t(M.w) << 32) | M.z;
     rng.n = X.x;
-    uint32_t item = M.x, depth = M.y & 0xffffu;
+    uint32_t item = M.x, depth = M.y & 0xffffu, sid = X.z;
     bool skip = (M.y >> 16) & 1u;
     float2 H = hit[i];
     uint32_t prim = __float_as_uint(H.y);
     V3<float> add = {0.f, 0.f, 0.f};
     bool cont = false, emit = false;
-    V3<float> sp_o, sp_d, ndir, nbeta, cb;
+    V3<float> sp_o = {0.f, 0.f, 0.f}, sp_d = {0.f, 0.f, 0.f}, ndir = {0.f, 0.f, 0.f}, nbeta = beta;
+    V3<float> cb = {0.f, 0.f, 0.f};
     float sp_dist = 0.f;
     bool use_nee = false;
-    if (prim == 0xffffffffu) {
-      add = hadamard(beta, background(S.bg, d));
-    } else {
+    for_scene(sid, [&](uint32_t sc) {
+      const SceneView<float> S = B.views[sc];
+      if (prim == 0xffffffffu) {
+        add = hadamard(beta, background(S.bg, d));
+        return;
+      }
       Surf<float> s;
       surface<float, FEAT>(S, prim, o, d, H.x, s);
       const MaterialT<float>& m = S.materials[s.mat];
@@ -129,59 +230,51 @@ __global__ __launch_bounds__(BS) void k_shade(SceneView<float> S, FrameDev F, Pa
       V3<float> att;
       if (!scatter(S, m, d, s, rng, ndir, att)) {
         add = hadamard(beta, em);
-      } else {
-        // Russian roulette on remaining depth < 47 (src/render.rs:1494-1502)
-        float surv = clamp_r(max3(att), 0.05f, 1.0f);
-        bool rr = depth < 47;
-        if (rr && rng.next<float>() > surv) {
-          add = hadamard(beta, em);
-        } else {
-          float rrw = rr ? 1.0f / surv : 1.0f;
-          use_nee = !is_specular(m) && S.n_lights > 0;
-          V3<float> c = {0.f, 0.f, 0.f};
-          bool nee_ok = false;
-          if (use_nee) nee_ok = nee_sample(S, s.p, s.n, att, rng, sp_d, sp_dist, c);
-          V3<float> attw = att * rrw;
-          // NaN guard of one recursion level (src/render.rs:1526-1531)
-          if (!(has_nan(em) || (nee_ok && has_nan(c)) || has_nan(attw))) {
-            add = hadamard(beta, em);
-            if (nee_ok) {
-              emit = true;
-              sp_o = s.p;
-              cb = hadamard(beta, c);
-            }
-            nbeta = hadamard(beta, attw);
-            cont = depth > 1;
-          }
-          sp_o = s.p;
-        }
+        return;
       }
-    }
-    if (add.x != 0.f || add.y != 0.f || add.z != 0.f) {
-      float4 sm = samples[item];
-      sm.x += add.x;
-      sm.y += add.y;
-      sm.z += add.z;
-      samples[item] = sm;
-    }
+      // Russian roulette on remaining depth < 47 (src/render.rs:1494-1502)
+      float surv = clamp_r(max3(att), 0.05f, 1.0f);
+      bool rr = depth < 47;
+      if (rr && rng.next<float>() > surv) {
+        add = hadamard(beta, em);
+        return;
+      }
+      float rrw = rr ? 1.0f / surv : 1.0f;
+      use_nee = !is_specular(m) && S.n_lights > 0;
+      V3<float> c = {0.f, 0.f, 0.f};
+      bool nee_ok = false;
+      if (use_nee) nee_ok = nee_sample(S, s.p, s.n, att, rng, sp_d, sp_dist, c);
+      V3<float> attw = att * rrw;
+      // NaN guard of one recursion level (src/render.rs:1526-1531)
+      if (has_nan(em) || (nee_ok && has_nan(c)) || has_nan(attw)) return;
+      add = hadamard(beta, em);
+      emit = nee_ok;
+      cb = hadamard(beta, c);
+      sp_o = s.p;
+      nbeta = hadamard(beta, attw);
+      cont = depth > 1;
+    });
+    if (add.x != 0.f || add.y != 0.f || add.z != 0.f) add_sample(samples, item, add);
     uint32_t ps = wave_reserve(&ctr->n_shadow, emit);
     if (emit) {
       sh.o[ps] = make_float4(sp_o.x, sp_o.y, sp_o.z, sp_dist);
       sh.d[ps] = make_float4(sp_d.x, sp_d.y, sp_d.z, __uint_as_float(item));
-      sh.c[ps] = make_float4(cb.x, cb.y, cb.z, 0.f);
+      sh.c[ps] = make_float4(cb.x, cb.y, cb.z, __uint_as_float(sid));
     }
     uint32_t pn = wave_reserve(&ctr->next_active, cont);
     if (cont) {
       nxt.ob[pn] = make_float4(sp_o.x, sp_o.y, sp_o.z, nbeta.x);
       nxt.db[pn] = make_float4(ndir.x, ndir.y, ndir.z, nbeta.y);
       nxt.meta[pn] = make_uint4(item, (depth - 1) | (use_nee ? 0x10000u : 0u), M.z, M.w);
-      nxt.aux[pn] = make_uint4(rng.n, __float_as_uint(nbeta.z), 0u, 0u);
+      nxt.aux[pn] = make_uint4(rng.n, __float_as_uint(nbeta.z), sid, 0u);
     }
   }
 }
 
+// Regenerate: new work items (camera rays, src/render.rs:1683-1690) into the
+// free slots after the compacted survivors.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_regen(SceneView<float> S, FrameDev F, PathSoA nxt, float4* __restrict__ samples,
+__global__ __launch_bounds__(BS) void k_regen(BatchDev B, PathSoA nxt, float4* __restrict__ samples,
                                               const WaveCounters* __restrict__ ctr, uint32_t pool) {
   uint32_t na = ctr->next_active;
   uint64_t left = ctr->n_items - ctr->next_item;
@@ -190,17 +283,30 @@ __global__ __launch_bounds__(BS) void k_regen(SceneView<float> S, FrameDev F, Pa
   uint32_t nthreads = gridDim.x * BS;
   for (uint32_t t = blockIdx.x * BS + threadIdx.x; t < m; t += nthreads) {
     uint64_t item = ctr->next_item + t;
-    Rng rng;
-    V3<float> o, d;
-    float w;
-    uint32_t pixel;
-    generate_item<float>(F, S.cam, F.item_base + item, rng, o, d, w, pixel);
-    samples[item] = make_float4(0.f, 0.f, 0.f, w);
-    uint32_t slot = na + t;
-    nxt.ob[slot] = make_float4(o.x, o.y, o.z, 1.f);
-    nxt.db[slot] = make_float4(d.x, d.y, d.z, 1.f);
-    nxt.meta[slot] = make_uint4(uint32_t(item), F.max_depth, uint32_t(rng.key), uint32_t(rng.key >> 32));
-    nxt.aux[slot] = make_uint4(rng.n, __float_as_uint(1.f), 0u, 0u);
+    // part holding the item: largest p with part_begin[p] <= item
+    uint32_t lo = 0, hi = B.n_parts;
+    while (hi - lo > 1) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (B.part_begin[mid] <= item)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    for_scene(lo, [&](uint32_t part) {
+      const FrameDev F = B.parts[part];
+      const CameraT<float> cam = B.views[F.scene].cam;
+      Rng rng;
+      V3<float> o, d;
+      float w;
+      uint32_t pixel;
+      generate_item<float>(F, cam, F.item_base + (item - B.part_begin[part]), rng, o, d, w, pixel);
+      samples[item] = make_float4(0.f, 0.f, 0.f, w);
+      uint32_t slot = na + t;
+      nxt.ob[slot] = make_float4(o.x, o.y, o.z, 1.f);
+      nxt.db[slot] = make_float4(d.x, d.y, d.z, 1.f);
+      nxt.meta[slot] = make_uint4(uint32_t(item), F.max_depth, uint32_t(rng.key), uint32_t(rng.key >> 32));
+      nxt.aux[slot] = make_uint4(rng.n, __float_as_uint(1.f), F.scene, 0u);
+    });
   }
 }
 
@@ -278,43 +384,33 @@ __global__ void k_finalize(const double4* __restrict__ acc, float* __restrict__ 
 // ---------------------------------------------------------------------------
 // Launchers
 
-// Instrumented extend (counts node fetches and primitive tests): a kernel of
-// its own name so that profiles keep it apart from the timed k_extend.
-template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_extend_count(SceneView<float> S, PathSoA cur, float2* __restrict__ hit,
-                                                     const WaveCounters* __restrict__ ctr, uint32_t* spill,
-                                                     TraceCounters* tc) {
-  __shared__ uint32_t lds[SS * BS];
-  uint32_t n = ctr->active;
-  uint32_t nthreads = gridDim.x * BS;
-  uint32_t gid = blockIdx.x * BS + threadIdx.x;
-  for (uint32_t i = gid; i < n; i += nthreads) {
-    float4 a = cur.ob[i], b = cur.db[i];
-    LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
-    float t;
-    uint32_t idx;
-    bool h = trace<float, FEAT, false>(S, f3(a), f3(b), 0.001f, INFINITY, t, idx, st, tc);
-    hit[i] = make_float2(t, __uint_as_float(h ? idx : 0xffffffffu));
-  }
-}
-
 template <int SS, uint32_t FEAT>
-static void launch_stage_ss(const WaveLaunch& L, const SceneView<float>& S, const PathSoA& cur, bool shadow) {
+static void launch_stage_ss(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow) {
   constexpr int BS = 256;
-  if (shadow)
-    hipLaunchKernelGGL((k_shadow<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, S, L.sh, L.samples, L.ctr, L.spill);
-  else if (L.tc)
-    hipLaunchKernelGGL((k_extend_count<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, S, cur, L.hit, L.ctr,
+  constexpr uint32_t MAIN = FEAT & ~FEAT_MARCH;
+  constexpr bool MARCH = (FEAT & FEAT_MARCH) != 0;
+  if (shadow) {
+    hipLaunchKernelGGL((k_shadow<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, L.sh, L.samples, L.ctr, L.spill);
+    if (MARCH)
+      hipLaunchKernelGGL((k_shadow_march<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, L.sh, L.samples,
+                         L.ctr, L.spill);
+    return;
+  }
+  if (L.tc)
+    hipLaunchKernelGGL((k_extend_count<SS, BS, MAIN>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.ctr,
                        L.spill, L.tc);
   else
-    hipLaunchKernelGGL((k_extend<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, S, cur, L.hit, L.ctr, L.spill);
+    hipLaunchKernelGGL((k_extend<SS, BS, MAIN>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.ctr, L.spill);
+  if (MARCH)
+    hipLaunchKernelGGL((k_extend_march<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.ctr,
+                       L.spill);
 }
 template <uint32_t FEAT>
-static void launch_stage(const WaveLaunch& L, const SceneView<float>& S, const PathSoA& cur, bool shadow) {
+static void launch_stage(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow) {
   if (L.stack_lds == 32)
-    launch_stage_ss<32, FEAT>(L, S, cur, shadow);
+    launch_stage_ss<32, FEAT>(L, B, cur, shadow);
   else
-    launch_stage_ss<16, FEAT>(L, S, cur, shadow);
+    launch_stage_ss<16, FEAT>(L, B, cur, shadow);
 }
 // The FP32 path's feature sets: marched primitives are FP64 records too, so
 // FEAT_MARCH always comes with FEAT_F64.
@@ -328,32 +424,30 @@ static void launch_stage(const WaveLaunch& L, const SceneView<float>& S, const P
     default: CALL(7); break;           \
   }
 
-static void launch_trace_stage(const WaveLaunch& L, const SceneView<float>& S, const PathSoA& cur, bool shadow) {
-#define CALL(f) launch_stage<f>(L, S, cur, shadow)
+static void launch_trace_stage(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow) {
+#define CALL(f) launch_stage<f>(L, B, cur, shadow)
   LMN_FEAT32_SWITCH(L.feat, CALL)
 #undef CALL
 }
-static void launch_shade(const WaveLaunch& L, const SceneView<float>& S, const FrameDev& F, const PathSoA& cur,
-                         const PathSoA& nxt) {
+static void launch_shade(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt) {
   constexpr int BS = 256;
 #define CALL(f) \
-  hipLaunchKernelGGL((k_shade<BS, f>), dim3(L.grid), dim3(BS), 0, L.stream, S, F, cur, L.hit, nxt, L.sh, L.samples, L.ctr)
+  hipLaunchKernelGGL((k_shade<BS, f>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, nxt, L.sh, L.samples, L.ctr)
   LMN_FEAT32_SWITCH(L.feat, CALL)
 #undef CALL
 }
 
-void launch_wave_iteration(const WaveLaunch& L, const SceneView<float>& S, const FrameDev& F, const PathSoA& cur,
-                           const PathSoA& nxt, bool count, hipEvent_t* ev) {
-  (void)count;
+void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt,
+                           hipEvent_t* ev) {
   constexpr int BS = 256;
   if (ev) (void)hipEventRecord(ev[0], L.stream);
-  launch_trace_stage(L, S, cur, false);
+  launch_trace_stage(L, B, cur, false);
   if (ev) (void)hipEventRecord(ev[1], L.stream);
-  launch_shade(L, S, F, cur, nxt);
+  launch_shade(L, B, cur, nxt);
   if (ev) (void)hipEventRecord(ev[2], L.stream);
-  launch_trace_stage(L, S, cur, true);
+  launch_trace_stage(L, B, cur, true);
   if (ev) (void)hipEventRecord(ev[3], L.stream);
-  hipLaunchKernelGGL((k_regen<BS>), dim3(L.grid), dim3(BS), 0, L.stream, S, F, nxt, L.samples, L.ctr, L.pool);
+  hipLaunchKernelGGL((k_regen<BS>), dim3(L.grid), dim3(BS), 0, L.stream, B, nxt, L.samples, L.ctr, L.pool);
   hipLaunchKernelGGL(k_book, dim3(1), dim3(64), 0, L.stream, L.ctr, L.pool);
   if (ev) (void)hipEventRecord(ev[4], L.stream);
 }

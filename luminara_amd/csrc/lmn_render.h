@@ -47,7 +47,8 @@ struct CameraT {
 
 template <class R>
 struct SceneView {
-  const typename NodeSel<R>::type* nodes;
+  const typename NodeSel<R>::type* nodes;   // main BVH (no marched kinds)
+  const typename NodeSel<R>::type* mnodes;  // BVH over the marched kinds
   const PrimT<R>* prims;          // bounded prims (leaf order) then planes
   const PrimT<double>* prims64;   // FP64 copies, same order (FP64 islands)
   const XformT<R>* xforms;
@@ -56,7 +57,7 @@ struct SceneView {
   const TextureDev* textures;
   const float* texels;
   const LightT<R>* lights;
-  uint32_t n_nodes, n_bounded, n_planes, n_lights;
+  uint32_t n_nodes, n_mnodes, n_bounded, n_planes, n_lights;
   BackgroundT<R> bg;
   CameraT<R> cam;
   uint32_t exact_marched;  // LMN_ISECT_EXACT for marched prims
@@ -181,79 +182,96 @@ LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t
   return true;
 }
 
-// Closest hit over the BVH then the unbounded list (SceneWorld::hit,
-// src/scene.rs:817-834).  ANY = stop at the first hit in range (NEE
-// visibility only asks is_some(), src/render.rs:1458).  Ordered BVH2
-// traversal: the two children of a node are stored side by side, so one
-// contiguous 64-byte fetch (FP32) brings both boxes; both are tested, the
-// nearer is visited first and the farther pushed; a popped node is
-// re-tested against the current closest hit before it is expanded.
+// One BVH walk.  Ordered BVH2 traversal: the two children of a node are
+// stored side by side, so one contiguous 64-byte fetch (FP32) brings both
+// boxes; both are tested, the nearer is visited first and the farther
+// pushed; a popped node is re-tested against the current closest hit before
+// it is expanded.
 template <class R, uint32_t FEAT, bool ANY, class Stack>
-LMN_HD bool trace(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R tmax, R& t_out, uint32_t& idx_out, Stack& st,
-                  TraceCounters* cnt = nullptr) {
+LMN_HD void bvh_walk(const SceneView<R>& S, const typename NodeSel<R>::type* nodes, V3<R> o, V3<R> d, V3<R> inv,
+                     R tmin, R& closest, uint32_t& best, bool& found, Stack& st, uint32_t& nf, uint32_t& pt,
+                     uint32_t& tt, bool counting) {
   using Node = typename NodeSel<R>::type;
-  R closest = tmax;
-  bool found = false;
-  uint32_t best = 0;
-  V3<R> inv = {R(1) / d.x, R(1) / d.y, R(1) / d.z};
-  uint32_t nf = 0, pt = 0, tt = 0;
-  if (S.n_nodes > 0) {
-    Node nd = S.nodes[0];
-    ++nf;
-    R te;
-    bool live = NodeOps<R>::test(nd, o, inv, tmin, closest, te);
-    while (live) {
-      if (nd.b & LEAF_BIT) {
-        uint32_t first = nd.a, count = nd.b & ~LEAF_BIT;
-        for (uint32_t k = 0; k < count; ++k) {
-          ++pt;
-          if (cnt && (S.prims[first + k].kind_mat & KIND_MASK) == K_TRIANGLE) ++tt;
-          if (test_prim<R, FEAT>(S, first + k, o, d, tmin, closest)) {
-            found = true;
-            best = first + k;
-            if (ANY) break;
-          }
-        }
-        if (ANY && found) break;
-      } else {
-        uint32_t c = nd.a;  // children at c, c + 1
-        Node L = S.nodes[c], Rn = S.nodes[c + 1];
-        nf += 2;
-        R tl, tr;
-        bool hl = NodeOps<R>::test(L, o, inv, tmin, closest, tl);
-        bool hr = NodeOps<R>::test(Rn, o, inv, tmin, closest, tr);
-        if (hl && hr) {
-          bool left_first = tl <= tr;
-          st.push(left_first ? c + 1 : c);
-          nd = left_first ? L : Rn;
-          continue;
-        }
-        if (hl || hr) {
-          nd = hl ? L : Rn;
-          continue;
+  Node nd = nodes[0];
+  ++nf;
+  R te;
+  bool live = NodeOps<R>::test(nd, o, inv, tmin, closest, te);
+  while (live) {
+    if (nd.b & LEAF_BIT) {
+      uint32_t first = nd.a, count = nd.b & ~LEAF_BIT;
+      for (uint32_t k = 0; k < count; ++k) {
+        ++pt;
+        if (counting && (S.prims[first + k].kind_mat & KIND_MASK) == K_TRIANGLE) ++tt;
+        if (test_prim<R, FEAT>(S, first + k, o, d, tmin, closest)) {
+          found = true;
+          best = first + k;
+          if (ANY) break;
         }
       }
-      live = false;
-      while (!st.empty()) {
-        nd = S.nodes[st.pop()];
-        ++nf;
-        if (NodeOps<R>::test(nd, o, inv, tmin, closest, te)) {
-          live = true;
-          break;
-        }
+      if (ANY && found) return;
+    } else {
+      uint32_t c = nd.a;  // children at c, c + 1
+      Node L = nodes[c], Rn = nodes[c + 1];
+      nf += 2;
+      R tl, tr;
+      bool hl = NodeOps<R>::test(L, o, inv, tmin, closest, tl);
+      bool hr = NodeOps<R>::test(Rn, o, inv, tmin, closest, tr);
+      if (hl && hr) {
+        bool left_first = tl <= tr;
+        st.push(left_first ? c + 1 : c);
+        nd = left_first ? L : Rn;
+        continue;
+      }
+      if (hl || hr) {
+        nd = hl ? L : Rn;
+        continue;
+      }
+    }
+    live = false;
+    while (!st.empty()) {
+      nd = nodes[st.pop()];
+      ++nf;
+      if (NodeOps<R>::test(nd, o, inv, tmin, closest, te)) {
+        live = true;
+        break;
       }
     }
   }
-  if (!(ANY && found)) {
+}
+
+// Which part of the scene a trace covers: everything; the main BVH plus the
+// unbounded planes; or only the marched primitives' BVH (the FP32 wavefront
+// runs that in a kernel of its own so the FP64 marching code does not set
+// the register budget of the main traversal).
+enum : uint32_t { PASS_ALL = 0, PASS_MAIN = 1, PASS_MARCH = 2 };
+
+// Closest hit over the BVH then the unbounded list (SceneWorld::hit,
+// src/scene.rs:817-834).  ANY = stop at the first hit in range (NEE
+// visibility only asks is_some(), src/render.rs:1458).  `closest` enters as
+// t_max (or a previous pass's hit) and leaves as the closest hit's t.
+template <class R, uint32_t FEAT, bool ANY, uint32_t PASS = PASS_ALL, class Stack>
+LMN_HD bool trace_pass(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R& closest, uint32_t& best, Stack& st,
+                       TraceCounters* cnt = nullptr) {
+  bool found = false;
+  V3<R> inv = {R(1) / d.x, R(1) / d.y, R(1) / d.z};
+  uint32_t nf = 0, pt = 0, tt = 0;
+  constexpr uint32_t MAIN_FEAT = FEAT & ~FEAT_MARCH;
+  if (PASS != PASS_MARCH && S.n_nodes > 0)
+    bvh_walk<R, MAIN_FEAT, ANY>(S, S.nodes, o, d, inv, tmin, closest, best, found, st, nf, pt, tt, cnt != nullptr);
+  if (PASS != PASS_MARCH && !(ANY && found)) {
     for (uint32_t k = 0; k < S.n_planes; ++k) {
       uint32_t idx = S.n_bounded + k;
       ++pt;
-      if (test_prim<R, FEAT>(S, idx, o, d, tmin, closest)) {
+      if (test_prim<R, MAIN_FEAT>(S, idx, o, d, tmin, closest)) {
         found = true;
         best = idx;
         if (ANY) break;
       }
     }
+  }
+  if constexpr ((FEAT & FEAT_MARCH) != 0) {
+    if (PASS != PASS_MAIN && S.n_mnodes > 0 && !(ANY && found))
+      bvh_walk<R, FEAT, ANY>(S, S.mnodes, o, d, inv, tmin, closest, best, found, st, nf, pt, tt, cnt != nullptr);
   }
 #if defined(__HIPCC__)
   if (cnt) {
@@ -266,6 +284,15 @@ LMN_HD bool trace(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R tmax, R& t_
   (void)cnt;
   (void)tt;
 #endif
+  return found;
+}
+
+template <class R, uint32_t FEAT, bool ANY, class Stack>
+LMN_HD bool trace(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R tmax, R& t_out, uint32_t& idx_out, Stack& st,
+                  TraceCounters* cnt = nullptr) {
+  R closest = tmax;
+  uint32_t best = 0;
+  bool found = trace_pass<R, FEAT, ANY, PASS_ALL>(S, o, d, tmin, closest, best, st, cnt);
   t_out = closest;
   idx_out = best;
   return found;
@@ -647,15 +674,19 @@ template <class R> LMN_HD R filter_weight(uint32_t f, R dx, R dy) {
   }
 }
 
-// Per-frame constants of the pixel loop (src/render.rs:1571-1596).
+// Per-frame constants of the pixel loop (src/render.rs:1571-1596).  In a
+// wavefront chunk one FrameDev describes one part: a range of whole sample
+// passes [item_base / npix, item_base / npix + n_items / npix) of `scene`.
 struct FrameDev {
   uint32_t full_w, full_h;
   uint32_t crop_x, crop_y, crop_w, crop_h;
   uint32_t sqrt_spp, spp_a;
   uint32_t max_depth, filter;
   uint64_t fkey;
-  uint64_t n_items;    // crop_w * crop_h * spp_a
-  uint64_t item_base;  // first work item of the current chunk
+  uint64_t n_items;    // work items of the part (whole frame: crop_w * crop_h * spp_a)
+  uint64_t item_base;  // first frame-local work item of the part
+  uint32_t scene;      // index into the batch's scene views
+  uint32_t pad;
 };
 
 // Generates work item k = sample * (crop_w*crop_h) + crop pixel: its camera

@@ -127,3 +127,260 @@ WORKLOADS = {
     "cornell": cornell_box,
     "random_spheres": random_spheres,
 }
+
+
+# ---------------------------------------------------------------------------
+# Meshes
+
+def perlin_table(seed):
+    """Permutation table of Perlin's improved noise, shuffled by the seed."""
+    rng = SeededRng(seed, 0x9E11)
+    perm = list(range(256))
+    for i in range(255, 0, -1):  # Fisher-Yates
+        j = rng.randint(i + 1)
+        perm[i], perm[j] = perm[j], perm[i]
+    import numpy as np
+    return np.array(perm + perm, dtype=np.int64)
+
+
+def perlin3(perm, x, y, z):
+    """Ken Perlin's improved noise (SIGGRAPH 2002) on numpy arrays."""
+    import numpy as np
+    X = np.floor(x).astype(np.int64)
+    Y = np.floor(y).astype(np.int64)
+    Z = np.floor(z).astype(np.int64)
+    x = x - X
+    y = y - Y
+    z = z - Z
+    X &= 255
+    Y &= 255
+    Z &= 255
+
+    def fade(t):
+        return t * t * t * (t * (t * 6 - 15) + 10)
+
+    def grad(h, x, y, z):
+        h = h & 15
+        u = np.where(h < 8, x, y)
+        v = np.where(h < 4, y, np.where((h == 12) | (h == 14), x, z))
+        return np.where(h & 1, -u, u) + np.where(h & 2, -v, v)
+
+    def lerp(t, a, b):
+        return a + t * (b - a)
+
+    u, v, w = fade(x), fade(y), fade(z)
+    A = perm[X] + Y
+    AA = perm[A] + Z
+    AB = perm[A + 1] + Z
+    B = perm[X + 1] + Y
+    BA = perm[B] + Z
+    BB = perm[B + 1] + Z
+    return lerp(w, lerp(v, lerp(u, grad(perm[AA], x, y, z), grad(perm[BA], x - 1, y, z)),
+                        lerp(u, grad(perm[AB], x, y - 1, z), grad(perm[BB], x - 1, y - 1, z))),
+                lerp(v, lerp(u, grad(perm[AA + 1], x, y, z - 1), grad(perm[BA + 1], x - 1, y, z - 1)),
+                     lerp(u, grad(perm[AB + 1], x, y - 1, z - 1), grad(perm[BB + 1], x - 1, y - 1, z - 1))))
+
+
+def fbm(perm, p, octaves=4, freq=1.5):
+    """Fractal sum of `octaves` noise octaves (lacunarity 2, gain 0.5),
+    normalised to about [-1, 1]."""
+    total = 0.0
+    amp = 1.0
+    norm = 0.0
+    for o in range(octaves):
+        f = freq * (2 ** o)
+        total = total + amp * perlin3(perm, p[:, 0] * f, p[:, 1] * f, p[:, 2] * f)
+        norm += amp
+        amp *= 0.5
+    return total / norm * 1.5
+
+
+_ICO = None
+
+
+def _icosahedron():
+    import numpy as np
+    global _ICO
+    if _ICO is None:
+        t = (1.0 + 5.0 ** 0.5) / 2.0
+        v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t],
+                      [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+        f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+                      [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+                      [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+        _ICO = (v, f)
+    return _ICO
+
+
+def geodesic_sphere(freq):
+    """Class-I geodesic sphere: every icosahedron face split into freq^2
+    triangles, 20 freq^2 in all (freq = 2^k gives the k-level icosphere's
+    count, e.g. freq 256 -> 1,310,720).  Shared edge points are evaluated
+    with the corners in index order, so neighbouring faces produce bitwise
+    identical vertices (watertight).  Returns unit vertices (V, 3) and
+    triangles (T, 3) with per-face vertex blocks."""
+    import numpy as np
+    V, F = _icosahedron()
+    n = freq
+    ii, jj = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    mask = ii + jj <= n
+    ii, jj = ii[mask], jj[mask]
+    kk = n - ii - jj
+    local = -np.ones((n + 1, n + 1), dtype=np.int64)
+    local[ii, jj] = np.arange(len(ii))
+    # triangles of one face in the (i, j) lattice
+    tris = []
+    for i in range(n):
+        for j in range(n - i):
+            tris.append((local[i, j], local[i + 1, j], local[i, j + 1]))
+            if j < n - i - 1:
+                tris.append((local[i + 1, j], local[i + 1, j + 1], local[i, j + 1]))
+    tris = np.array(tris, dtype=np.int64)
+    npts = len(ii)
+    verts = np.empty((20 * npts, 3))
+    faces = np.empty((20 * len(tris), 3), dtype=np.int64)
+    w = np.stack([ii, jj, kk], axis=1).astype(np.float64)
+    for fi, (a, b, c) in enumerate(F):
+        corners = [(a, 0), (b, 1), (c, 2)]
+        corners.sort()
+        p = np.zeros((npts, 3))
+        for cid, col in corners:  # canonical summation order
+            p = p + w[:, col:col + 1] * V[cid][None, :]
+        p = p / np.linalg.norm(p, axis=1, keepdims=True)
+        verts[fi * npts:(fi + 1) * npts] = p
+        faces[fi * len(tris):(fi + 1) * len(tris)] = tris + fi * npts
+    return verts, faces
+
+
+def displaced_sphere(freq, center, radius, seed, amplitude=0.05, octaves=4):
+    """Geodesic sphere displaced along its normal by seeded fBm (amplitude
+    `amplitude * radius`)."""
+    import numpy as np
+    u, f = geodesic_sphere(freq)
+    perm = perlin_table(seed)
+    h = fbm(perm, u, octaves)
+    v = np.asarray(center)[None, :] + u * (radius * (1.0 + amplitude * h))[:, None]
+    return v, f
+
+
+def write_ply(path, verts, faces):
+    with open(path, "w") as fh:
+        fh.write(f"ply\nformat ascii 1.0\nelement vertex {len(verts)}\nproperty float x\nproperty float y\n"
+                 f"property float z\nelement face {len(faces)}\nproperty list uchar int vertex_indices\nend_header\n")
+        fh.write("\n".join(f"{x!r} {y!r} {z!r}" for x, y, z in verts.tolist()))
+        fh.write("\n")
+        fh.write("\n".join(f"3 {a} {b} {c}" for a, b, c in faces.tolist()))
+        fh.write("\n")
+
+
+def _microfacet(color, roughness, metallic=0.0):
+    from . import _abi
+    m = _abi.Material()
+    m.kind = _abi.MAT_MICROFACET
+    m.texture = -1
+    for k in range(3):
+        m.color[k] = color[k]
+    m.roughness = min(max(roughness, 0.01), 1.0)
+    m.metallic = metallic
+    m.intensity = 1.0
+    m.ior = 1.5
+    return m
+
+
+def mesh_heavy(width=1920, height=1080, spp=64, depth=12, seed=42, freq=256, patch=1024, n_tori=64,
+               n_superellipsoids=16):
+    """Mesh-heavy scene: displaced geodesic icosphere (freq 256 = 1,310,720
+    triangles, 4-octave seeded Perlin fBm, amplitude 0.05 r) with one GGX
+    material per `patch` consecutive triangles (roughness U[0.05, 0.6]),
+    an infinite Lambertian ground plane, `n_tori` tori (R 0.5, r 0.15) and
+    `n_superellipsoids` superellipsoids (e1, e2 ~ U[0.3, 3]) at seeded
+    positions, three r=0.5 emissive spheres (20) sampled by NEE, sun + sky.
+    Returns (toml, add_mesh) where add_mesh(world) appends the mesh through
+    the descriptor API (the TOML part goes through the scene loader)."""
+    import numpy as np
+    rng = SeededRng(seed, 2)
+    L = _header(width, height, spp, depth, seed,
+                '{ type = "sun", direction = [0.35, 0.6, -0.45] }',
+                {"look_from": [0.0, 3.2, 10.5], "look_at": [0.0, 1.6, 0.0], "vfov": 38, "aperture": 0.0,
+                 "focus_dist": 1.0})
+    for c in ([-3.2, 5.0, 2.0], [3.4, 5.5, 1.0], [0.0, 6.0, -3.0]):
+        L += ["[[sphere]]", f"center = {_v(c)}", "radius = 0.5",
+              'material = { type = "emissive", color = [1.0, 1.0, 1.0], intensity = 20.0 }', ""]
+    L += ["[[plane]]", "point = [0.0, 0.0, 0.0]", "normal = [0.0, 1.0, 0.0]",
+          'material = { type = "lambertian", color = [0.5, 0.5, 0.5] }', ""]
+    for i in range(n_tori):
+        ang = 2 * math.pi * (i + rng.uniform()) / n_tori
+        rad = 3.6 + 2.2 * rng.uniform()
+        c = [rad * math.cos(ang), 0.15, rad * math.sin(ang) * 0.6 + 0.5]
+        col = [0.2 + 0.7 * rng.uniform() for _ in range(3)]
+        L += ["[[torus]]", f"center = {_v(c)}", "major_radius = 0.5", "minor_radius = 0.15",
+              f"material = {_mat({'type': 'lambertian', 'color': col})}", ""]
+    for i in range(n_superellipsoids):
+        ang = 2 * math.pi * (i + 0.5) / n_superellipsoids
+        c = [2.9 * math.cos(ang), 0.35, 2.9 * math.sin(ang) * 0.7 + 0.3]
+        e1, e2 = rng.uniform(0.3, 3.0), rng.uniform(0.3, 3.0)
+        col = [0.2 + 0.7 * rng.uniform() for _ in range(3)]
+        L += ["[[superellipsoid]]", f"center = {_v(c)}", "scale = [0.35, 0.35, 0.35]", f"e1 = {e1!r}",
+              f"e2 = {e2!r}", f"material = {_mat({'type': 'pbr', 'color': col, 'roughness': 0.4})}", ""]
+    toml = "\n".join(L)
+
+    def add_mesh(world):
+        v, f = displaced_sphere(freq, [0.0, 1.75, 0.0], 1.6, seed)
+        n_patch = (len(f) + patch - 1) // patch
+        mats = np.empty(len(f), dtype=np.uint32)
+        prng = SeededRng(seed, 3)
+        for p in range(n_patch):
+            col = [0.55 + 0.4 * prng.uniform() for _ in range(3)]
+            idx = world.add_material(_microfacet(col, prng.uniform(0.05, 0.6)))
+            mats[p * patch:(p + 1) * patch] = idx
+        world.add_triangles(v, f, materials=mats)
+        return len(f)
+
+    return toml, add_mesh
+
+
+def batch_scene(index, seed=42, max_res=512):
+    """Scene `index` of the 64-scene batch: a seeded displaced geodesic sphere
+    with a log-uniform triangle count in [1k, 200k] plus a few random spheres
+    on a ground quad, 1-4 emissive quad lights (NEE), depth 8; resolution in
+    {128, 256, 384, 512}^2 and spp in {16, 64, 256}.  Returns
+    (toml, add_mesh, info)."""
+    rng = SeededRng(seed, 4, index)
+    n_target = math.exp(rng.uniform(math.log(1000), math.log(200000)))
+    freq = max(7, int(round(math.sqrt(n_target / 20.0))))
+    res = [128, 256, 384, 512][rng.randint(4)]
+    res = min(res, max_res)
+    spp = [16, 64, 256][rng.randint(3)]
+    L = _header(res, res, spp, 8, seed + index, '{ type = "black" }',
+                {"look_from": [0.0, 2.2, 6.5], "look_at": [0.0, 1.0, 0.0], "vfov": 40, "aperture": 0.0,
+                 "focus_dist": 1.0})
+    for k in range(6):
+        c = [rng.uniform(-2.5, 2.5), 0.3, rng.uniform(-1.5, 2.0)]
+        choose = rng.uniform()
+        if choose < 0.6:
+            m = {"type": "lambertian", "color": [rng.uniform(0.2, 0.9) for _ in range(3)]}
+        elif choose < 0.85:
+            m = {"type": "metal", "color": [rng.uniform(0.5, 1.0) for _ in range(3)], "fuzz": rng.uniform(0, 0.4)}
+        else:
+            m = {"type": "dielectric", "refraction_index": 1.5}
+        L += ["[[sphere]]", f"center = {_v(c)}", "radius = 0.3", f"material = {_mat(m)}", ""]
+    n_lights = 1 + rng.randint(4)
+    for k in range(n_lights):
+        x = -2.5 + 5.0 * (k + 0.5) / n_lights
+        L += ["[[quad]]", f"q = {_v([x - 0.4, 4.0, -0.5])}", "u = [0.8, 0.0, 0.0]", "v = [0.0, 0.0, 1.0]",
+              f"material = {{ type = \"emissive\", color = [1.0, 0.95, 0.9], intensity = {12.0 / n_lights!r} }}", ""]
+    L += ["[[quad]]", "q = [-8.0, 0.0, -8.0]", "u = [16.0, 0.0, 0.0]", "v = [0.0, 0.0, 16.0]",
+          'material = { type = "lambertian", color = [0.6, 0.6, 0.6] }', ""]
+    toml = "\n".join(L)
+    col = [0.4 + 0.5 * rng.uniform() for _ in range(3)]
+    rough = rng.uniform(0.05, 0.6)
+    mesh_seed = seed * 1000 + index
+
+    def add_mesh(world):
+        v, f = displaced_sphere(freq, [0.0, 1.2, 0.0], 1.0, mesh_seed)
+        idx = world.add_material(_microfacet(col, rough))
+        world.add_triangles(v, f, material=idx)
+        return len(f)
+
+    info = {"res": res, "spp": spp, "tris": 20 * freq * freq, "lights": n_lights}
+    return toml, add_mesh, info
