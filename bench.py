@@ -1,13 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark of the render hot path on one MI355X.
 
-A step is one frame of the workload rendered by the FP32 wavefront path
-tracer (pixel loop of render::render, src/render.rs:1587-1747) with inputs
-(scene, BVH) resident in HBM and the framebuffer left in HBM.  Prints one
-JSON line: Mpaths/s (paths = W*H*ceil(sqrt(spp))^2), the plain FP64
-one-path-per-thread GPU kernel on the same frame, the image error between
-them, the roofline of the dominant kernel (extend: BVH traversal) and the
-CPU oracle timed on a row band of the same frame.
+A step renders the whole workload once with the FP32 batched wavefront
+path tracer (pixel loop of render::render, src/render.rs:1587-1747) with
+inputs (scenes, BVHs) resident in HBM and framebuffers left in HBM.  The
+default workload is the 64-scene batch: seeded displaced geodesic spheres
+with 1k-200k triangles plus random spheres, 128^2-512^2, 16/64/256 spp,
+depth 8, NEE on 1-4 quad lights -- all scenes in one wavefront.
+
+Prints one JSON line: Mpaths/s (paths = sum of W*H*ceil(sqrt(spp))^2), the
+plain FP64 one-path-per-thread GPU kernel on the same scenes and the
+relative RMSE between the two images, the roofline of the dominant kernel
+(extend: BVH traversal) and the CPU oracle timed on a bounded sample.
 """
 import argparse
 import json
@@ -26,13 +30,54 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload_scene(name):
+def build_workload(name, n_scenes=64):
+    """Returns (workload name, [(config, world)], description)."""
+    import luminara_amd as lm
     from luminara_amd import workloads
+    if name == "batch64":
+        scenes, tris = [], 0
+        for i in range(n_scenes):
+            toml, add_mesh, info = workloads.batch_scene(i, seed=42)
+            cfg, _, world = lm.load_scene(toml)
+            tris += add_mesh(world)
+            scenes.append((cfg, world))
+        return (f"batch{n_scenes}_displaced_spheres_1k-200k_tris_depth8_nee", scenes,
+                {"scenes": n_scenes, "triangles": tris})
     if name == "random_spheres":
-        return "random_spheres_1920x1080_spp128_d50", workloads.random_spheres(1920, 1080, 128, 50, 42)
+        cfg, _, world = lm.load_scene(workloads.random_spheres(1920, 1080, 128, 50, 42))
+        return "random_spheres_1920x1080_spp128_d50", [(cfg, world)], {"scenes": 1}
+    if name == "mesh_heavy":
+        toml, add_mesh = workloads.mesh_heavy()
+        cfg, _, world = lm.load_scene(toml)
+        tris = add_mesh(world)
+        return "mesh_heavy_1920x1080_spp64_d12_1.31M_tris", [(cfg, world)], {"scenes": 1, "triangles": tris}
     if name == "cornell":
-        return "cornell_256x256_spp64_d8", workloads.cornell_box(256, 256, 64, 8, 42)
+        cfg, _, world = lm.load_scene(workloads.cornell_box(256, 256, 64, 8, 42))
+        return "cornell_256x256_spp64_d8", [(cfg, world)], {"scenes": 1}
     raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(scenes, seconds, threads):
+    """Times the CPU FP64 oracle on row bands of every scene (the same
+    fraction of rows of each), scaled to about `seconds` of CPU time."""
+    import oracle
+    import luminara_amd as lm
+
+    def run(frac):
+        paths, secs = 0, 0.0
+        for i, (cfg, world) in enumerate(scenes):
+            h, _ = lm.output_shape(cfg)
+            rows = max(1, int(round(frac * h)))
+            _, _, st = oracle.render(world, cfg, rows=(0, rows), threads=threads, want_var=False, scene_key=i)
+            paths += st["camera_rays"]
+            secs += st["render_secs"]
+        return paths, secs
+
+    frac = 1e-3
+    paths, secs = run(frac)
+    frac = min(1.0, frac * seconds / max(secs, 1e-3))
+    paths, secs = run(frac)
+    return paths, secs, frac
 
 
 def main():
@@ -40,9 +85,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="random_spheres")
+    ap.add_argument("--workload", default="batch64")
+    ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 kernel comparison")
+    ap.add_argument("--no-count", action="store_true", help="skip the instrumented traversal count")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--stack-lds", type=int, default=0)
     ap.add_argument("--pool", type=int, default=0)
@@ -57,90 +104,97 @@ def main():
     import luminara_amd as lm
     from luminara_amd import _abi
 
-    wl_name, toml = workload_scene(args.workload)
-    cfg, cam, world = lm.load_scene(toml)
     t0 = time.time()
-    bvh = world.bvh_info()
-    log(f"[bench] {wl_name}: {len(world.prims)} prims, BVH {bvh} built+uploaded in {time.time() - t0:.2f}s")
-    h, w = lm.output_shape(cfg)
-    out = torch.empty((h, w, 3), dtype=torch.float32, device="cuda")
-    params = lm.make_params(cfg, path="wavefront", pool_paths=args.pool, stack_lds=args.stack_lds,
-                            flags=_abi.FLAG_STAGE_TIMING)
+    wl_name, scenes, desc = build_workload(args.workload, args.scenes)
+    worlds = [w for _, w in scenes]
+    nodes = 0
+    for w in worlds:
+        nodes += w.bvh_info()["nodes"]
+    log(f"[bench] {wl_name}: {desc}, {nodes} BVH nodes, scenes built and uploaded in {time.time() - t0:.1f}s")
+    outs = [torch.empty(lm.output_shape(c) + (3,), dtype=torch.float32, device="cuda") for c, _ in scenes]
+    ptrs = [o.data_ptr() for o in outs]
 
+    def params(path, flags=0):
+        return [lm.make_params(c, path=path, scene_key=i, pool_paths=args.pool, stack_lds=args.stack_lds,
+                               flags=flags) for i, (c, _) in enumerate(scenes)]
+
+    p32 = params("wavefront", _abi.FLAG_STAGE_TIMING)
     for _ in range(args.warmup):
-        lm.render_device(world, params, out.data_ptr())
+        lm.render_batch_device(worlds, p32, ptrs)
     torch.cuda.synchronize()
     stats = []
     t_start = time.perf_counter()
     for _ in range(args.steps):
-        stats.append(lm.render_device(world, params, out.data_ptr()))
+        stats.append(lm.render_batch_device(worlds, p32, ptrs))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     paths = stats[0]["paths"]
     ms_per_step = elapsed * 1e3 / args.steps
     value = paths * args.steps / elapsed / 1e6
-    img32 = out.double().cpu().numpy()
-    log(f"[bench] wavefront FP32: {ms_per_step:.1f} ms/frame, {value:.1f} Mpaths/s, stats {stats[-1]}")
+    imgs32 = [o.double().cpu().numpy() for o in outs]
+    log(f"[bench] wavefront FP32: {ms_per_step:.1f} ms/step, {value:.1f} Mpaths/s, stats {stats[-1]}")
 
-    # roofline of the extend kernel: algorithmic bytes from an instrumented
-    # run of the same traversal on the same frame, time from HIP events
-    counts = lm.profile_extend(world, lm.make_params(cfg, path="wavefront", pool_paths=args.pool,
-                                                     stack_lds=args.stack_lds), out.data_ptr())
     ext_ms = sum(s["extend_ms"] for s in stats) / args.steps
     launches = sum(s["extend_launches"] for s in stats) / args.steps
-    io_bytes = 48  # per ray: origin+beta, direction+beta (2 x 16 B) read, hit record 16 B written
-    alg_bytes = (counts["node_fetches"] * counts["node_bytes"] + counts["prim_tests"] * counts["prim_bytes"]
-                 + counts["rays"] * io_bytes)
-    bytes_per_launch = alg_bytes / launches
-    avg_launch_s = ext_ms / launches / 1e3
-    achieved = bytes_per_launch / avg_launch_s
-    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_extend",
-                "avg_launch_us": avg_launch_s * 1e6, "launches_per_step": launches,
-                "bytes_per_launch": bytes_per_launch,
-                "per_ray": {"node_fetches": counts["node_fetches"] / counts["rays"],
-                            "prim_tests": counts["prim_tests"] / counts["rays"]}}
-    log(f"[bench] extend: {counts}, {ext_ms:.2f} ms/frame over {launches:.0f} launches")
+    roofline = None
+    if not args.no_count:
+        # algorithmic bytes of the extend kernel: node and primitive fetches
+        # counted by an instrumented run of the same traversal on the same
+        # work, plus per-ray queue traffic; time from HIP events
+        counts = lm.profile_extend(worlds, params("wavefront"), ptrs)
+        io_bytes = 40  # per ray: origin+beta, direction+beta, scene id (32 + 8 B) read, hit 8 B written
+        alg_bytes = (counts["node_fetches"] * counts["node_bytes"] + counts["prim_tests"] * counts["prim_bytes"]
+                     + counts["rays"] * io_bytes)
+        bytes_per_launch = alg_bytes / launches
+        avg_launch_s = ext_ms / launches / 1e3
+        achieved = bytes_per_launch / avg_launch_s
+        roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_extend",
+                    "avg_launch_us": avg_launch_s * 1e6, "launches_per_step": launches,
+                    "bytes_per_launch": bytes_per_launch,
+                    "bytes_model": "32 B per BVH node fetched + 64 B per primitive record tested + 40 B ray/hit I/O",
+                    "per_ray": {"node_fetches": counts["node_fetches"] / counts["rays"],
+                                "prim_tests": counts["prim_tests"] / counts["rays"]}}
+        log(f"[bench] extend: {counts}, {ext_ms:.2f} ms/step over {launches:.0f} launches, "
+            f"{achieved / 1e9:.0f} GB/s algorithmic")
 
-    extra = {}
+    extra = {"stage_ms": {"extend": ext_ms, "shade": sum(s["shade_ms"] for s in stats) / args.steps,
+                          "shadow": sum(s["shadow_ms"] for s in stats) / args.steps,
+                          "regen": sum(s["regen_ms"] for s in stats) / args.steps}}
     if not args.no_fp64:
-        p64 = lm.make_params(cfg, path="plain64")
-        out64 = torch.empty_like(out)
+        p64 = params("plain64")
+        outs64 = [torch.empty_like(o) for o in outs]
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        s64 = lm.render_device(world, p64, out64.data_ptr())
+        lm.render_batch_device(worlds, p64, [o.data_ptr() for o in outs64])
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t1
-        img64 = out64.double().cpu().numpy()
-        rel = float(np.linalg.norm(img32 - img64) / max(np.linalg.norm(img64), 1e-300))
+        imgs64 = [o.double().cpu().numpy() for o in outs64]
+        num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(imgs32, imgs64))
+        den = sum(float(np.sum(b ** 2)) for b in imgs64)
+        per = [float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)) for a, b in zip(imgs32, imgs64)]
         fp64_rate = paths / t64 / 1e6
-        extra = {"plain_fp64_mpaths_per_s": fp64_rate, "speedup_vs_plain_fp64": value / fp64_rate,
-                 "relrmse_fp32_vs_fp64": rel}
-        log(f"[bench] plain FP64: {t64 * 1e3:.1f} ms, relRMSE(FP32 wavefront, FP64) = {rel:.3e}")
+        extra.update({"plain_fp64_mpaths_per_s": fp64_rate, "speedup_vs_plain_fp64": value / fp64_rate,
+                      "relrmse_fp32_vs_fp64": (num / den) ** 0.5, "relrmse_fp32_vs_fp64_max_scene": max(per)})
+        log(f"[bench] plain FP64: {t64 * 1e3:.1f} ms ({fp64_rate:.1f} Mpaths/s), relRMSE(FP32, FP64) = "
+            f"{(num / den) ** 0.5:.3e} (worst scene {max(per):.3e})")
 
     cpu = None
     if not args.no_cpu:
-        import oracle
         threads = min(16, os.cpu_count() or 1)
-        band = 2
-        _, _, st = oracle.render(world, cfg, rows=(0, band), threads=threads, want_var=False)
-        per_row = st["render_secs"] / band
-        band = int(max(1, min(h, args.cpu_seconds / max(per_row, 1e-6))))
-        _, _, st = oracle.render(world, cfg, rows=(0, band), threads=threads, want_var=False)
-        cpu_paths = st["camera_rays"]
-        cpu = {"value": cpu_paths / st["render_secs"] / 1e6, "unit": "Mpaths/s", "cores": st["threads"],
-               "kind": "port",
-               "sample": f"rows 0..{band} of the same {w}x{h} frame ({cpu_paths} paths, "
-                         f"{st['render_secs']:.1f} s), CPU FP64 oracle, counter RNG"}
+        cpaths, csecs, frac = cpu_baseline(scenes, args.cpu_seconds, threads)
+        cpu = {"value": cpaths / csecs / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+               "sample": f"first {frac * 100:.2f}% of the rows of every scene ({cpaths} paths, {csecs:.1f} s), "
+                         f"CPU FP64 oracle (restatement of the reference pixel loop), counter RNG"}
         log(f"[bench] cpu oracle: {cpu}")
 
+    cfg0 = scenes[0][0]
     line = {
         "metric": METRIC, "value": value, "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": wl_name, "width": cfg.width, "height": cfg.height,
-                   "spp": cfg.samples_per_pixel, "max_depth": cfg.max_depth, "paths_per_step": paths,
-                   "prims": len(world.prims), "bvh_nodes": bvh["nodes"]},
+        "config": dict({"workload": wl_name, "paths_per_step": paths, "bvh_nodes": nodes,
+                        "max_depth": cfg0.max_depth}, **desc),
         "roofline": roofline, "cpu_baseline": cpu,
         "mrays_per_s": (stats[-1]["camera_rays"] + stats[-1]["extension_rays"] + stats[-1]["shadow_rays"])
                        / (ms_per_step / 1e3) / 1e6,

@@ -328,8 +328,8 @@ typedef struct lmn_trace_counts {
   uint32_t node_bytes;
   uint32_t prim_bytes;
 } lmn_trace_counts;
-int lmn_profile_extend(const lmn_scene* scene, const lmn_render_params* params, float* d_hdr_rgb,
-                       lmn_trace_counts* counts);
+int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
+                       float* const* d_hdr_rgb, lmn_trace_counts* counts);
 
 #ifdef __cplusplus
 }

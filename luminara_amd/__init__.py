@@ -233,9 +233,15 @@ def render_batch_device(worlds, params_list, out_ptrs) -> dict:
     return {k: getattr(st, k) for k, _ in _abi.Stats._fields_}
 
 
-def profile_extend(world: SceneWorld, params: _abi.RenderParams, out_ptr: int) -> dict:
+def profile_extend(worlds, params_list, out_ptrs) -> dict:
+    """Instrumented wavefront render of a batch: BVH node fetches and
+    primitive tests of the extend stage."""
+    n = len(worlds)
+    scenes = (C.c_void_p * n)(*[w.device_scene().value for w in worlds])
+    params = (_abi.RenderParams * n)(*params_list)
+    ptrs = (C.c_void_p * n)(*out_ptrs)
     tc = _abi.TraceCounts()
-    check(lib().lmn_profile_extend(world.device_scene(), C.byref(params), C.c_void_p(out_ptr), C.byref(tc)))
+    check(lib().lmn_profile_extend(scenes, n, params, ptrs, C.byref(tc)))
     return {k: getattr(tc, k) for k, _ in _abi.TraceCounts._fields_}
 
 

@@ -143,8 +143,8 @@ SIGNATURES = {
     "lmn_trace_rays": (C.c_int, [C.c_void_p, C.POINTER(Ray), C.c_size_t, C.c_uint32, C.c_uint32,
                                  C.POINTER(Hit)]),
     "lmn_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
-    "lmn_profile_extend": (C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_void_p,
-                                     C.POINTER(TraceCounts)]),
+    "lmn_profile_extend": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(RenderParams),
+                                     C.POINTER(C.c_void_p), C.POINTER(TraceCounts)]),
 }
 
 _lib = None

@@ -672,17 +672,23 @@ int lmn_device_info(char* name, size_t name_len, int* n_cu, double* hbm_bytes) {
 
 // Instrumented wavefront render: counts BVH node fetches and primitive tests
 // of the extend stage (the algorithmic bytes of the roofline).
-int lmn_profile_extend(const lmn_scene* scene, const lmn_render_params* params, float* d_hdr_rgb,
-                       lmn_trace_counts* counts) {
-  if (!counts) return set_err(LMN_ERR_INVALID, "lmn_profile_extend: null counts");
-  lmn_render_params p = *params;
-  p.path = LMN_PATH_WAVEFRONT_FP32;
+int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
+                       float* const* d_hdr_rgb, lmn_trace_counts* counts) {
+  if (!counts || !scenes || !params || !d_hdr_rgb || !n_scenes)
+    return set_err(LMN_ERR_INVALID, "lmn_profile_extend: bad argument");
+  std::vector<lmn_render_params> p(params, params + n_scenes);
+  std::vector<lmn_scene*> sc(n_scenes);
+  std::vector<const lmn_render_params*> pp(n_scenes);
+  for (size_t i = 0; i < n_scenes; ++i) {
+    p[i].path = LMN_PATH_WAVEFRONT_FP32;
+    p[i].flags = 0;
+    sc[i] = const_cast<lmn_scene*>(scenes[i]);
+    pp[i] = &p[i];
+  }
   TraceCounters* d_tc = nullptr;
   HIP_TRY(hipMalloc((void**)&d_tc, sizeof(TraceCounters)));
   hipError_t e = hipMemset(d_tc, 0, sizeof(TraceCounters));
-  lmn_scene* sc = const_cast<lmn_scene*>(scene);
-  const lmn_render_params* pp = &p;
-  int rc = e == hipSuccess ? render_common(&sc, &pp, &d_hdr_rgb, 1, nullptr, d_tc)
+  int rc = e == hipSuccess ? render_common(sc.data(), pp.data(), d_hdr_rgb, n_scenes, nullptr, d_tc)
                            : set_err(LMN_ERR_DEVICE, "hipMemset failed");
   if (rc == LMN_OK) {
     TraceCounters h;
