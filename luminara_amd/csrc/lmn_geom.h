@@ -449,13 +449,16 @@ LMN_HD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double> 
                                      double& t) {
   V3<double> o = {(ro.x - p[0]) / p[3], (ro.y - p[1]) / p[4], (ro.z - p[2]) / p[5]};
   V3<double> d = {rd.x / p[3], rd.y / p[4], rd.z / p[5]};
-  // clip to the unit box |q|_inf <= 1 that contains the surface
+  // clip to the unit box |q|_inf <= 1 that contains the surface, padded so
+  // that a ray entering where the surface touches the box (the axis points)
+  // starts strictly outside and sees the sign change
+  const double B = 1.0 + 1e-9;
   double t0 = tmin, t1 = tmax;
   const double* oo = &o.x;
   const double* dd = &d.x;
   for (int k = 0; k < 3; ++k) {
     double inv = 1.0 / dd[k];
-    double a = (-1.0 - oo[k]) * inv, b = (1.0 - oo[k]) * inv;
+    double a = (-B - oo[k]) * inv, b = (B - oo[k]) * inv;
     if (a > b) { double tmp = a; a = b; b = tmp; }
     t0 = fmax(t0, a);
     t1 = fmin(t1, b);
@@ -464,10 +467,7 @@ LMN_HD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double> 
   double dl = length(d);
   int steps = 512;
   double h = (t1 - t0) / steps;
-  double fa = superellipsoid_f(o + d * t0, p[6], p[7]);
-  if (fa <= 0.0) {  // starts inside: report the exit crossing like a surface hit
-    if (m_abs(fa) < 1e-12 && t0 >= tmin) { t = t0; return true; }
-  }
+  double fa = superellipsoid_f(o + d * t0, p[6], p[7]);  // <= 0: starts inside, the exit is reported
   double ta = t0;
   for (int i = 1; i <= steps; ++i) {
     double tb = (i == steps) ? t1 : t0 + h * i;

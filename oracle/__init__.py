@@ -56,6 +56,20 @@ def lib():
                                      C.c_double, C.c_double, C.c_uint64, C.c_uint32, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
         L.oracle_last_error.restype = C.c_char_p
+        D3 = C.POINTER(C.c_double)
+        L.oracle_nee.restype = C.c_int
+        L.oracle_nee.argtypes = [C.POINTER(_abi.SceneDesc), D3, D3, D3, C.c_uint64, C.c_uint32, D3,
+                                 C.POINTER(C.c_uint64)]
+        L.oracle_camera_ray.restype = C.c_int
+        L.oracle_camera_ray.argtypes = [C.POINTER(_abi.SceneDesc), C.c_double, C.c_double, C.c_uint64, D3, D3]
+        L.oracle_emitted.restype = C.c_int
+        L.oracle_emitted.argtypes = [C.POINTER(_abi.SceneDesc), C.c_uint32, C.c_double, C.c_double, D3]
+        L.oracle_background.restype = C.c_int
+        L.oracle_background.argtypes = [C.POINTER(_abi.SceneDesc), D3, D3]
+        L.oracle_filter_weight.restype = C.c_double
+        L.oracle_filter_weight.argtypes = [C.c_uint32, C.c_double, C.c_double]
+        L.oracle_reflectance.restype = C.c_double
+        L.oracle_reflectance.argtypes = [C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -113,3 +127,40 @@ def scatter(world, material, ray_dir, point, normal, front_face, t, u, v, key, c
     if ok < 0:
         _check(ok)
     return bool(ok), np.array(od[:]), np.array(oa[:]), oc.value
+
+
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def nee(world, point, normal, albedo, key=42, n=1000):
+    """Mean of n NEE samples at a fixed surface point, and the shadow-ray count."""
+    out, sh = (C.c_double * 3)(), C.c_uint64()
+    _check(lib().oracle_nee(world.desc, _d3(point), _d3(normal), _d3(albedo), key, n, out, C.byref(sh)))
+    return np.array(out[:]), sh.value
+
+
+def camera_ray(world, s, t, key=0):
+    o, d = (C.c_double * 3)(), (C.c_double * 3)()
+    _check(lib().oracle_camera_ray(world.desc, s, t, key, o, d))
+    return np.array(o[:]), np.array(d[:])
+
+
+def emitted(world, material, u=0.5, v=0.5):
+    out = (C.c_double * 3)()
+    _check(lib().oracle_emitted(world.desc, material, u, v, out))
+    return np.array(out[:])
+
+
+def background(world, direction):
+    out = (C.c_double * 3)()
+    _check(lib().oracle_background(world.desc, _d3(direction), out))
+    return np.array(out[:])
+
+
+def filter_weight(kind, dx, dy):
+    return lib().oracle_filter_weight(kind, dx, dy)
+
+
+def reflectance(cosine, ratio):
+    return lib().oracle_reflectance(cosine, ratio)

@@ -655,10 +655,13 @@ struct Superellipsoid : Hittable {  // superellipsoid.rs:26-97
     Vec3 o((ray.o.x - c.x) / s.x, (ray.o.y - c.y) / s.y, (ray.o.z - c.z) / s.z);
     Vec3 d(ray.d.x / s.x, ray.d.y / s.y, ray.d.z / s.z);
     if (exact) {
+      // unit box, padded so that entries at the axis points (where the
+      // surface touches the box) start strictly outside
+      const double B = 1.0 + 1e-9;
       double t0 = tmin, t1 = tmax;
       for (int a = 0; a < 3; ++a) {
         double inv = 1.0 / d.axis(a);
-        double ta = (-1.0 - o.axis(a)) * inv, tb = (1.0 - o.axis(a)) * inv;
+        double ta = (-B - o.axis(a)) * inv, tb = (B - o.axis(a)) * inv;
         if (ta > tb) std::swap(ta, tb);
         t0 = std::fmax(t0, ta);
         t1 = std::fmin(t1, tb);
@@ -1609,5 +1612,79 @@ int oracle_scatter(const lmn_scene_desc* desc, uint32_t material, const double r
   if (out_counter) *out_counter = rng.n;
   return ok ? 1 : 0;
 }
+
+// ---- single-function evaluations for the known-answer tests ---------------
+
+int oracle_nee(const lmn_scene_desc* desc, const double point[3], const double normal[3], const double albedo[3],
+               uint64_t key, uint32_t n, double out_mean[3], uint64_t* out_shadow_rays) {
+  if (!desc || !point || !normal || !albedo || !out_mean || n == 0) {
+    g_err = "null argument";
+    return LMN_ERR_INVALID;
+  }
+  World w;
+  if (!build_world(*desc, true, w)) return LMN_ERR_INVALID;
+  Counters cnt;
+  Vec3 sum;
+  for (uint32_t i = 0; i < n; ++i) {
+    CounterRng rng(key + uint64_t(i) * 0x9E3779B97F4A7C15ull);
+    sum += sample_direct_light(w, v3(point), v3(normal), v3(albedo), rng, cnt);
+  }
+  out_mean[0] = sum.x / n;
+  out_mean[1] = sum.y / n;
+  out_mean[2] = sum.z / n;
+  if (out_shadow_rays) *out_shadow_rays = cnt.shadow;
+  return LMN_OK;
+}
+
+int oracle_camera_ray(const lmn_scene_desc* desc, double s, double t, uint64_t key, double origin[3],
+                      double dir[3]) {
+  if (!desc || !origin || !dir) {
+    g_err = "null argument";
+    return LMN_ERR_INVALID;
+  }
+  World w;
+  if (!build_world(*desc, true, w)) return LMN_ERR_INVALID;
+  CounterRng rng(key);
+  Ray r = w.get_ray(s, t, rng);
+  origin[0] = r.o.x;
+  origin[1] = r.o.y;
+  origin[2] = r.o.z;
+  dir[0] = r.d.x;
+  dir[1] = r.d.y;
+  dir[2] = r.d.z;
+  return LMN_OK;
+}
+
+int oracle_emitted(const lmn_scene_desc* desc, uint32_t material, double u, double v, double out[3]) {
+  if (!desc || !out || material >= desc->n_materials) {
+    g_err = "bad material";
+    return LMN_ERR_INVALID;
+  }
+  World w;
+  if (!build_world(*desc, true, w)) return LMN_ERR_INVALID;
+  Vec3 e = w.mats[material]->emitted(u, v);
+  out[0] = e.x;
+  out[1] = e.y;
+  out[2] = e.z;
+  return LMN_OK;
+}
+
+int oracle_background(const lmn_scene_desc* desc, const double dir[3], double out[3]) {
+  if (!desc || !dir || !out) {
+    g_err = "null argument";
+    return LMN_ERR_INVALID;
+  }
+  World w;
+  if (!build_world(*desc, true, w)) return LMN_ERR_INVALID;
+  Vec3 c = w.background(Ray{Vec3(), v3(dir), 0.0});
+  out[0] = c.x;
+  out[1] = c.y;
+  out[2] = c.z;
+  return LMN_OK;
+}
+
+double oracle_filter_weight(uint32_t filter, double dx, double dy) { return filter_weight(filter, dx, dy); }
+
+double oracle_reflectance(double cosine, double ratio) { return Material::reflectance(cosine, ratio); }
 
 }  // extern "C"

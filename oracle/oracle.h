@@ -70,6 +70,20 @@ int oracle_scatter(const lmn_scene_desc* desc, uint32_t material, const double r
 
 const char* oracle_last_error(void);
 
+/* Single-function evaluations for the known-answer tests: the mean of n NEE
+ * samples at a fixed surface point (src/render.rs:1346-1466; sample i uses the
+ * counter stream key + i*golden), one camera ray (src/camera.rs:82-103),
+ * emitted radiance (src/material.rs:241-243), the background colour
+ * (src/render.rs:72-172), a pixel filter weight (src/render.rs:198-237) and
+ * Schlick's reflectance (src/material.rs:138-141). */
+int oracle_nee(const lmn_scene_desc* desc, const double point[3], const double normal[3], const double albedo[3],
+               uint64_t key, uint32_t n, double out_mean[3], uint64_t* out_shadow_rays);
+int oracle_camera_ray(const lmn_scene_desc* desc, double s, double t, uint64_t key, double origin[3], double dir[3]);
+int oracle_emitted(const lmn_scene_desc* desc, uint32_t material, double u, double v, double out[3]);
+int oracle_background(const lmn_scene_desc* desc, const double dir[3], double out[3]);
+double oracle_filter_weight(uint32_t filter, double dx, double dy);
+double oracle_reflectance(double cosine, double ratio);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,14 +1,12 @@
 """GPU parity on triangle meshes, marched primitives and multi-scene batches.
 
-Meshes come in through the reference's [[mesh]] ASCII PLY path and through
-the binary triangle API; marched primitives (torus, superellipsoid) are
+Meshes come in through the binary triangle API (the PLY path builds the
+same records, a CPU test checks that); marched primitives (torus, superellipsoid) are
 checked in both intersect modes: the reference's SDF march (FP64 kernel vs
 oracle march: t within 1e-9 relative) and the exact mode (GPU closed-form
 quartic / bracketed roots vs the oracle's independent dense-bracketing
 root finder: t within 1e-7 relative).
 """
-import os
-
 import numpy as np
 import pytest
 
@@ -20,30 +18,13 @@ from test_gpu_parity import compare_hits, random_rays
 pytestmark = pytest.mark.gpu
 
 
-def mesh_world(freq=8, tmpdir=None, via_ply=False):
+def mesh_world(freq=8):
     toml, add_mesh, info = workloads.batch_scene(3, seed=5)
-    if via_ply:
-        v, f = workloads.displaced_sphere(freq, [0.0, 1.2, 0.0], 1.0, 77)
-        path = os.path.join(tmpdir, "m.ply")
-        workloads.write_ply(path, v, f)
-        toml += f'\n[[mesh]]\nfile = "{path}"\nmaterial = {{ type = "pbr", color = [0.7, 0.6, 0.5], roughness = 0.3 }}\n'
-        cfg, cam, world = lm.load_scene(toml)
-    else:
-        cfg, cam, world = lm.load_scene(toml)
-        v, f = workloads.displaced_sphere(freq, [0.0, 1.2, 0.0], 1.0, 77)
-        m = world.add_material(workloads._microfacet([0.7, 0.6, 0.5], 0.3))
-        world.add_triangles(v, f, material=m)
+    cfg, cam, world = lm.load_scene(toml)
+    v, f = workloads.displaced_sphere(freq, [0.0, 1.2, 0.0], 1.0, 77)
+    m = world.add_material(workloads._microfacet([0.7, 0.6, 0.5], 0.3))
+    world.add_triangles(v, f, material=m)
     return cfg, cam, world
-
-
-def test_ply_and_binary_meshes_agree(tmp_path):
-    _, _, a = mesh_world(via_ply=True, tmpdir=str(tmp_path))
-    _, _, b = mesh_world(via_ply=False)
-    pa, pb = a.prims, b.prims
-    assert len(pa) == len(pb)
-    tri_a = np.array([list(p.p)[:9] for p in pa if p.kind == _abi.PRIM_TRIANGLE])
-    tri_b = np.array([list(p.p)[:9] for p in pb if p.kind == _abi.PRIM_TRIANGLE])
-    np.testing.assert_allclose(tri_a, tri_b, rtol=1e-15)
 
 
 @pytest.mark.parametrize("path", ["wavefront", "plain64"])

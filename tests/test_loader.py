@@ -1,0 +1,224 @@
+"""Scene loader (host C++, no GPU): the reference's load_scene tests restated,
+its defaults, light rules and ordering, PLY meshes, and a parse sweep over
+the reference's example scenes when they are present."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import luminara_amd as lm
+from luminara_amd import _abi, workloads
+
+REF_SCENES = "/root/reference/examples/scenes"
+
+
+def test_load_simple_scene():  # src/scene.rs:2222-2256
+    toml = """
+[render]
+width = 400
+height = 200
+samples = 16
+max_depth = 10
+
+[camera]
+look_from = [0.0, 1.0, 5.0]
+look_at = [0.0, 0.0, 0.0]
+vfov = 60.0
+
+[[sphere]]
+center = [0.0, 0.0, 0.0]
+radius = 1.0
+[sphere.material]
+type = "lambertian"
+color = [0.8, 0.2, 0.1]
+
+[[sphere]]
+center = [2.0, 0.0, 0.0]
+radius = 0.5
+[sphere.material]
+type = "metal"
+color = [0.9, 0.9, 0.9]
+fuzz = 0.1
+"""
+    cfg, cam, world = lm.load_scene(toml)
+    assert (cfg.width, cfg.height, cfg.samples_per_pixel, cfg.max_depth) == (400, 200, 16, 10)
+    mats = world.materials()
+    assert [m.kind for m in mats] == [_abi.MAT_LAMBERTIAN, _abi.MAT_METAL]
+    assert abs(mats[1].fuzz - 0.1) < 1e-15
+    # the dotted [sphere.material] header addresses the last [[sphere]]
+    assert [p.material for p in world.prims] == [0, 1]
+
+
+def test_materials_created_from_toml():  # src/scene.rs:2258-2277
+    toml = """
+[[sphere]]
+center = [0.0, 0.0, 0.0]
+radius = 1.0
+[sphere.material]
+type = "dielectric"
+refraction_index = 1.5
+
+[[sphere]]
+center = [3.0, 0.0, 0.0]
+radius = 1.0
+[sphere.material]
+type = "emissive"
+color = [1.0, 1.0, 1.0]
+intensity = 5.0
+"""
+    _, _, world = lm.load_scene(toml)
+    assert len(world.prims) == 2
+    # an emissive sphere is registered as a sphere light (src/scene.rs:1182-1202)
+    lights = world.lights
+    assert len(lights) == 1 and lights[0].kind == _abi.LIGHT_SPHERE
+    assert list(lights[0].emission) == [5.0, 5.0, 5.0]
+
+
+def test_invalid_toml_returns_error():  # src/scene.rs:2279-2288
+    with pytest.raises(_abi.LuminaraError) as e:
+        lm.load_scene("this is {{{{ not valid TOML at all")
+    assert e.value.code == _abi.LMN_ERR_PARSE
+    assert "TOML parse error" in str(e.value)
+
+
+def test_missing_material_type_returns_error():  # src/scene.rs:2290-2302
+    with pytest.raises(_abi.LuminaraError):
+        lm.load_scene("[[sphere]]\ncenter = [0.0, 0.0, 0.0]\nradius = 1.0\n[sphere.material]\ncolor = [1.0, 0.0, 0.0]\n")
+
+
+def test_geometry_outside_the_path_is_rejected_by_name():  # src/scene.rs:2304-2392 (scope subset)
+    base = '[[sphere]]\ncenter = [0.0, 0.0, 0.0]\nradius = 1.0\nmaterial = { type = "lambertian", color = [0.5, 0.5, 0.5] }\n'
+    _, _, world = lm.load_scene(base)
+    assert len(world.prims) == 1
+    with pytest.raises(_abi.LuminaraError) as e:
+        lm.load_scene(base + '[[cylinder]]\ncenter = [0.0, 0.0, 0.0]\nradius = 0.5\nheight = 1.0\n'
+                             'material = { type = "metal", color = [0.5, 0.5, 0.5] }\n')
+    assert e.value.code == _abi.LMN_ERR_UNSUPPORTED and "cylinder" in str(e.value)
+
+
+def test_defaults_match_render_config_default():  # src/render.rs:422-431
+    cfg, cam, world = lm.load_scene("")
+    assert (cfg.width, cfg.height, cfg.samples_per_pixel, cfg.max_depth, cfg.seed) == (800, 450, 100, 50, 31337)
+    assert len(world.prims) == 0
+
+
+def test_fixed_type_order_and_light_rules():
+    # objects enter the world by type, not by file order (src/scene.rs:1180-1642):
+    # spheres before planes before quads before boxes
+    toml = """
+[[box]]
+min = [0.0, 0.0, 0.0]
+max = [1.0, 1.0, 1.0]
+material = { type = "lambertian", color = [0.5, 0.5, 0.5] }
+[[quad]]
+q = [0.0, 3.0, 0.0]
+u = [1.0, 0.0, 0.0]
+v = [0.0, 0.0, 1.0]
+material = { type = "emissive", color = [1.0, 0.9, 0.8], intensity = 4.0 }
+[[plane]]
+point = [0.0, 0.0, 0.0]
+normal = [0.0, 1.0, 0.0]
+material = { type = "lambertian", color = [0.5, 0.5, 0.5] }
+[[sphere]]
+center = [0.0, 5.0, 0.0]
+radius = 0.5
+material = { type = "lambertian", color = [0.5, 0.5, 0.5] }
+"""
+    _, _, world = lm.load_scene(toml)
+    kinds = [p.kind for p in world.prims]
+    assert kinds[:3] == [_abi.PRIM_SPHERE, _abi.PRIM_PLANE, _abi.PRIM_QUAD]
+    assert kinds[3:] == [_abi.PRIM_RECT_XY] * 2 + [_abi.PRIM_RECT_XZ] * 2 + [_abi.PRIM_RECT_YZ] * 2
+    lights = world.lights
+    assert len(lights) == 1 and lights[0].kind == _abi.LIGHT_RECT
+    np.testing.assert_allclose(list(lights[0].emission), [4.0, 3.6, 3.2])
+
+
+def test_camera_keeps_toml_aspect_under_overrides():
+    # -w/-H overrides change the film, not the camera (src/main.rs:205-210 vs
+    # src/scene.rs:1146-1174): the camera frame depends on the TOML size only
+    toml = "[render]\nwidth = 200\nheight = 100\n[camera]\nlook_from = [0.0, 0.0, 0.0]\nlook_at = [0.0, 0.0, -1.0]\nvfov = 90.0\n"
+    cfg, cam, _ = lm.load_scene(toml)
+    h = np.array(list(cam.horizontal))
+    v = np.array(list(cam.vertical))
+    assert abs(np.linalg.norm(h) / np.linalg.norm(v) - 2.0) < 1e-12
+
+
+def _ply(verts, faces):
+    lines = ["ply", "format ascii 1.0", f"element vertex {len(verts)}", "property float x", "property float y",
+             "property float z", f"element face {len(faces)}", "property list uchar int vertex_indices",
+             "end_header"]
+    lines += [" ".join(repr(float(x)) for x in v) for v in verts]
+    lines += [" ".join([str(len(f))] + [str(i) for i in f]) for f in faces]
+    return "\n".join(lines) + "\n"
+
+
+def test_ply_fan_triangulation_scale_and_offset(tmp_path):  # src/obj.rs:157-246
+    verts = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]]
+    path = tmp_path / "q.ply"
+    path.write_text(_ply(verts, [[0, 1, 2, 3]]))
+    toml = (f'[[mesh]]\nfile = "{path}"\nscale = 2.0\noffset = [1.0, 0.0, 0.0]\n'
+            'material = { type = "lambertian", color = [0.5, 0.5, 0.5] }\n')
+    _, _, world = lm.load_scene(toml)
+    tris = [np.array(list(p.p)[:9]).reshape(3, 3) for p in world.prims]
+    assert len(tris) == 2  # a quad face fans into (0,1,2), (0,2,3)
+    np.testing.assert_allclose(tris[0], [[1, 0, 0], [3, 0, 0], [3, 2, 0]])
+    np.testing.assert_allclose(tris[1], [[1, 0, 0], [3, 2, 0], [1, 2, 0]])
+
+
+def test_ply_errors(tmp_path):
+    bad = tmp_path / "bad.ply"
+    bad.write_text("not a ply\n")
+    toml = f'[[mesh]]\nfile = "{bad}"\nmaterial = {{ type = "lambertian", color = [0.5, 0.5, 0.5] }}\n'
+    with pytest.raises(_abi.LuminaraError):
+        lm.load_scene(toml)
+    oob = tmp_path / "oob.ply"
+    oob.write_text(_ply([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 1, 5]]))
+    with pytest.raises(_abi.LuminaraError):
+        lm.load_scene(toml.replace(str(bad), str(oob)))
+    with pytest.raises(_abi.LuminaraError) as e:
+        lm.load_scene(toml.replace(str(bad), str(tmp_path / "missing.ply")))
+    assert e.value.code == _abi.LMN_ERR_IO
+
+
+def test_ply_and_binary_meshes_agree(tmp_path):
+    toml, _, _ = workloads.batch_scene(3, seed=5)
+    v, f = workloads.displaced_sphere(8, [0.0, 1.2, 0.0], 1.0, 77)
+    path = os.path.join(str(tmp_path), "m.ply")
+    workloads.write_ply(path, v, f)
+    _, _, a = lm.load_scene(toml + f'\n[[mesh]]\nfile = "{path}"\n'
+                                   'material = { type = "pbr", color = [0.7, 0.6, 0.5], roughness = 0.3 }\n')
+    _, _, b = lm.load_scene(toml)
+    b.add_triangles(v, f, material=b.add_material(workloads._microfacet([0.7, 0.6, 0.5], 0.3)))
+    pa, pb = a.prims, b.prims
+    assert len(pa) == len(pb)
+    tri_a = np.array([list(p.p)[:9] for p in pa if p.kind == _abi.PRIM_TRIANGLE])
+    tri_b = np.array([list(p.p)[:9] for p in pb if p.kind == _abi.PRIM_TRIANGLE])
+    np.testing.assert_allclose(tri_a, tri_b, rtol=1e-15)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SCENES), reason="reference example scenes not present")
+def test_example_scene_sweep():
+    """Every example scene parses: it loads, or it is refused with
+    LMN_ERR_UNSUPPORTED naming a feature outside the render path -- never a
+    TOML or schema error."""
+    files = sorted(glob.glob(os.path.join(REF_SCENES, "*.toml")))
+    assert len(files) == 21
+    loaded = 0
+    for f in files:
+        try:
+            lm.load_scene(open(f).read(), base_dir=os.path.dirname(f))
+            loaded += 1
+        except _abi.LuminaraError as e:
+            assert e.code == _abi.LMN_ERR_UNSUPPORTED, (f, str(e))
+            assert "not on the GPU path" in str(e), (f, str(e))
+    assert loaded >= 2  # cornell.toml, prism.toml
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_SCENES), reason="reference example scenes not present")
+def test_cornell_example_geometry():
+    f = os.path.join(REF_SCENES, "cornell.toml")
+    cfg, cam, world = lm.load_scene(open(f).read(), base_dir=REF_SCENES)
+    assert (cfg.width, cfg.height, cfg.samples_per_pixel) == (600, 600, 256)
+    assert len(world.prims) == 18
+    assert len(world.lights) >= 1
