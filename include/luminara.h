@@ -133,7 +133,7 @@ typedef struct lmn_texture {
 enum { LMN_LIGHT_SPHERE = 0, LMN_LIGHT_RECT = 1, LMN_LIGHT_DISK = 2 };
 typedef struct lmn_light {
   uint32_t kind;
-  uint32_t pad;
+  uint32_t material; /* emissive material it was registered from, ~0u if none */
   double origin[3]; /* sphere/disk center, rect corner */
   double u[3];      /* rect edge                        */
   double v[3];      /* rect edge                        */
@@ -330,6 +330,45 @@ typedef struct lmn_trace_counts {
 } lmn_trace_counts;
 int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
                        float* const* d_hdr_rgb, lmn_trace_counts* counts);
+
+/* ---- path-replay adjoint ------------------------------------------------------
+ * Gradient of an image loss L with respect to scene parameters, given the
+ * loss's gradient dL/dI with respect to the linear HDR framebuffer that
+ * lmn_render produces for the same params.  Each path is replayed from its
+ * counter-based seed (nothing is stored between the forward render and the
+ * adjoint), then swept backwards to scatter gradients.  The estimator is the
+ * derivative of the expected image of the reference's ray_color
+ * (src/render.rs:1472-1537): sampled directions are held fixed (detached
+ * sampling), Russian roulette enters through its survival probability.
+ * No reference counterpart: the reference has no differentiable path.
+ * Parameters:
+ *   - the texels of one image texture (albedo of a Lambertian, or of a
+ *     metallic = 0 microfacet, material);
+ *   - the roughness of every microfacet material;
+ *   - the intensity of one emissive material (its surfaces and the NEE
+ *     lights registered from it).
+ * Supported for max_depth <= LMN_ADJOINT_MAX_DEPTH.  Gradient accumulation
+ * uses floating-point atomics: results agree run to run to rounding. */
+#define LMN_ADJOINT_MAX_DEPTH 16
+typedef struct lmn_adjoint_spec {
+  int32_t texture;           /* texture whose texels are parameters, -1 = none        */
+  int32_t emission_material; /* emissive material whose intensity is one, -1 = none   */
+  uint32_t flags;            /* reserved, 0                                           */
+  uint32_t pad;
+} lmn_adjoint_spec;
+typedef struct lmn_grads {
+  float* texels;       /* out: width*height*3 of the spec's texture (NULL = skip)  */
+  float* roughness;    /* out: n_materials; 0 for non-microfacet (NULL = skip)     */
+  double emission;     /* out: dL/d intensity                                      */
+  uint64_t paths;      /* out: paths replayed                                      */
+  double adjoint_ms;   /* out: kernel time                                         */
+} lmn_grads;
+/* Host buffers: dL/dI (crop_h*crop_w*3 floats, row 0 = top) in, grads out. */
+int lmn_adjoint(const lmn_scene* scene, const lmn_render_params* params, const lmn_adjoint_spec* spec,
+                const float* dl_dimage, lmn_grads* grads);
+/* Device buffers for dL/dI, grads->texels and grads->roughness. */
+int lmn_adjoint_device(const lmn_scene* scene, const lmn_render_params* params, const lmn_adjoint_spec* spec,
+                       const float* d_dl_dimage, lmn_grads* grads);
 
 #ifdef __cplusplus
 }

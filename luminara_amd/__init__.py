@@ -270,6 +270,52 @@ def trace_rays(world: SceneWorld, rays: np.ndarray, path="wavefront", intersect_
     return hits_to_numpy(hits, n)
 
 
+@dataclasses.dataclass
+class Gradients:
+    """Output of the path-replay adjoint."""
+    texels: Optional[np.ndarray]     # (h, w, 3) float32, d loss / d texel of the parameter texture
+    roughness: np.ndarray            # (n_materials,) float32, d loss / d roughness (microfacet materials)
+    emission: float                  # d loss / d intensity of the parameter emissive material
+    paths: int
+    adjoint_ms: float
+
+
+def adjoint(config: RenderConfig, world: SceneWorld, dl_dimage: np.ndarray, texture=-1, emission_material=-1,
+            path="wavefront", **kw) -> Gradients:
+    """Gradients of an image loss (given dL/dI for the image `render` returns
+    with the same arguments) by the path-replay adjoint (lmn_adjoint)."""
+    scene = world.device_scene()
+    params = make_params(config, path=path, **kw)
+    h, w = output_shape(config)
+    g = np.ascontiguousarray(dl_dimage, dtype=np.float32).reshape(h, w, 3)
+    spec = _abi.AdjointSpec(texture, emission_material, 0, 0)
+    n_mat = world.desc_struct.n_materials
+    rough = np.zeros(n_mat, dtype=np.float32)
+    tex = None
+    if texture >= 0:
+        if texture >= world.desc_struct.n_textures:
+            raise ValueError(f"texture {texture} out of range")
+        t = world.desc_struct.textures[texture]
+        tex = np.zeros((t.height, t.width, 3), dtype=np.float32)
+    gr = _abi.Grads()
+    gr.texels = tex.ctypes.data if tex is not None else None
+    gr.roughness = rough.ctypes.data if n_mat else None
+    check(lib().lmn_adjoint(scene, C.byref(params), C.byref(spec), g.ctypes.data_as(C.POINTER(C.c_float)),
+                            C.byref(gr)))
+    return Gradients(tex, rough, gr.emission, gr.paths, gr.adjoint_ms)
+
+
+def adjoint_device(world: SceneWorld, params: _abi.RenderParams, spec: _abi.AdjointSpec, dl_ptr: int,
+                   texels_ptr: Optional[int], rough_ptr: Optional[int]) -> dict:
+    """lmn_adjoint_device on caller-owned device buffers (e.g. torch tensors)."""
+    gr = _abi.Grads()
+    gr.texels = texels_ptr
+    gr.roughness = rough_ptr
+    check(lib().lmn_adjoint_device(world.device_scene(), C.byref(params), C.byref(spec), C.c_void_p(dl_ptr),
+                                   C.byref(gr)))
+    return {"emission": gr.emission, "paths": gr.paths, "adjoint_ms": gr.adjoint_ms}
+
+
 def device_info():
     name = C.create_string_buffer(256)
     ncu = C.c_int()

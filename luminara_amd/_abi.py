@@ -56,7 +56,7 @@ class Texture(C.Structure):
 
 
 class Light(C.Structure):
-    _fields_ = [("kind", C.c_uint32), ("pad", C.c_uint32), ("origin", D3), ("u", D3), ("v", D3),
+    _fields_ = [("kind", C.c_uint32), ("material", C.c_uint32), ("origin", D3), ("u", D3), ("v", D3),
                 ("normal", D3), ("radius", C.c_double), ("emission", D3)]
 
 
@@ -117,6 +117,19 @@ class TraceCounts(C.Structure):
                 ("rays", C.c_uint64), ("node_bytes", C.c_uint32), ("prim_bytes", C.c_uint32)]
 
 
+ADJOINT_MAX_DEPTH = 16
+
+
+class AdjointSpec(C.Structure):
+    _fields_ = [("texture", C.c_int32), ("emission_material", C.c_int32), ("flags", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+class Grads(C.Structure):
+    _fields_ = [("texels", C.c_void_p), ("roughness", C.c_void_p), ("emission", C.c_double),
+                ("paths", C.c_uint64), ("adjoint_ms", C.c_double)]
+
+
 # Every symbol include/luminara.h declares, with its ctypes signature.
 SIGNATURES = {
     "lmn_last_error": (C.c_char_p, []),
@@ -145,6 +158,10 @@ SIGNATURES = {
     "lmn_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "lmn_profile_extend": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(RenderParams),
                                      C.POINTER(C.c_void_p), C.POINTER(TraceCounts)]),
+    "lmn_adjoint": (C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.POINTER(AdjointSpec), C.POINTER(C.c_float),
+                              C.POINTER(Grads)]),
+    "lmn_adjoint_device": (C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.POINTER(AdjointSpec), C.c_void_p,
+                                     C.POINTER(Grads)]),
 }
 
 _lib = None

@@ -17,7 +17,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "liblmn.so")
 
 HOST_SOURCES = ["host/toml.cpp", "host/scene_load.cpp", "host/scene_build.cpp"]
-HIP_SOURCES = ["lmn_api.hip", "lmn_kernels.hip", "lmn_plain.hip"]
+HIP_SOURCES = ["lmn_api.hip", "lmn_kernels.hip", "lmn_plain.hip", "lmn_adjoint.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LMN_OFFLOAD_ARCH", "gfx950")

@@ -439,6 +439,7 @@ template <class R>
 void fill_light(const lmn_light& l, LightT<R>& o) {
   std::memset(&o, 0, sizeof o);
   o.kind = l.kind;
+  o.mat = l.material;
   o.origin = {R(l.origin[0]), R(l.origin[1]), R(l.origin[2])};
   o.u = {R(l.u[0]), R(l.u[1]), R(l.u[2])};
   o.v = {R(l.v[0]), R(l.v[1]), R(l.v[2])};
@@ -607,9 +608,14 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
   }
   out.matF.resize(desc.n_materials);
   out.matD.resize(desc.n_materials);
+  out.slot_mat.clear();
   for (uint32_t i = 0; i < desc.n_materials; ++i) {
     fill_material(desc.materials[i], out.matF[i]);
     fill_material(desc.materials[i], out.matD[i]);
+    if (desc.materials[i].kind == LMN_MAT_MICROFACET) {  // roughness parameters of the adjoint
+      out.matF[i].param_slot = out.matD[i].param_slot = uint32_t(out.slot_mat.size());
+      out.slot_mat.push_back(i);
+    }
   }
   out.tex.resize(desc.n_textures);
   out.texels.clear();

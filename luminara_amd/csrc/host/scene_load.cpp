@@ -446,9 +446,10 @@ class Loader {
     return o;
   }
 
-  void push_light_sphere(const double c[3], double r, const double e[3]) {
+  void push_light_sphere(const double c[3], double r, const double e[3], uint32_t mat) {
     lmn_light l{};
     l.kind = LMN_LIGHT_SPHERE;
+    l.material = mat;
     for (int k = 0; k < 3; ++k) {
       l.origin[k] = c[k];
       l.emission[k] = e[k];
@@ -471,7 +472,7 @@ class Loader {
       double r = req_num(s, "radius", ctx);
       MatInfo mi;
       uint32_t mat = material(req(s, "material", ctx), ctx + ".material", &mi);
-      if (mi.emissive) push_light_sphere(c, r, mi.emission);  // src/scene.rs:1182-1202
+      if (mi.emissive) push_light_sphere(c, r, mi.emission, mat);  // src/scene.rs:1182-1202
       if (s.get("bump_strength") || s.get("normal_map"))
         fail(LMN_ERR_UNSUPPORTED, "sphere bump/normal maps are not on the GPU path");
       std::vector<lmn_xform_op> ops;  // src/scene.rs:1216-1230
@@ -569,7 +570,7 @@ class Loader {
       for (int k = 0; k < 3; ++k) p.p[k] = pos[k];
       p.p[3] = radius;
       double e[3] = {col[0] * intensity, col[1] * intensity, col[2] * intensity};
-      push_light_sphere(pos, radius, e);
+      push_light_sphere(pos, radius, e, uint32_t(out_.materials.size() - 1));
     }
     reject_if_present(root, "hemisphere", "(hemisphere)");
     reject_if_present(root, "fog", "(constant medium)");
@@ -601,6 +602,7 @@ class Loader {
       if (mi.emissive) {
         lmn_light l{};
         l.kind = LMN_LIGHT_RECT;
+        l.material = mat;
         double n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
         double nl = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
         for (int k = 0; k < 3; ++k) {
@@ -712,6 +714,7 @@ class Loader {
       if (mi.emissive && (tag == "Emissive" || tag == "emissive")) {
         lmn_light l{};
         l.kind = LMN_LIGHT_RECT;
+        l.material = mat;
         l.origin[0] = x[0]; l.origin[1] = k; l.origin[2] = z[0];
         l.u[0] = x[1] - x[0];
         l.v[2] = z[1] - z[0];

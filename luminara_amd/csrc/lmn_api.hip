@@ -71,6 +71,7 @@ struct Workspace {
   std::mutex mu;
   DevBuf st[2][4];  // ob, db, bk, meta for the two pools
   DevBuf hit, sh_o, sh_d, sh_c, samples, acc, spill, ctr, tc, rays, out, views, parts, part_begin;
+  DevBuf adj_wsum, adj_gtex, adj_part, adj_red, adj_dldi, adj_tex_out;  // path-replay adjoint
   uint32_t pool = 0;
   WaveCounters* host_ctr = nullptr;
   hipStream_t stream = nullptr;
@@ -702,6 +703,159 @@ int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lm
     counts->prim_bytes = sizeof(PrimT<float>);
   }
   (void)hipFree(d_tc);
+  return rc;
+}
+
+// ---- path-replay adjoint ------------------------------------------------------
+
+namespace {
+
+int adjoint_common(const lmn_scene* scene, const lmn_render_params* p, const lmn_adjoint_spec* spec,
+                   const float* d_dldi, float* d_tex_out, float* rough_out, bool rough_on_device, lmn_grads* g) {
+  if (!scene || !p || !spec || !d_dldi || !g) return set_err(LMN_ERR_INVALID, "lmn_adjoint: null argument");
+  const HostScene& H = scene->host;
+  FrameSetup fs;
+  int rc = frame_setup(p, fs);
+  if (rc != LMN_OK) return rc;
+  bool fp64 = p->path == LMN_PATH_PLAIN_FP64;
+  if (!fp64 && p->path != LMN_PATH_WAVEFRONT_FP32 && p->path != LMN_PATH_PLAIN_FP32)
+    return set_err(LMN_ERR_INVALID, "lmn_adjoint: unknown path");
+  if (p->max_depth > LMN_ADJOINT_MAX_DEPTH)
+    return set_err(LMN_ERR_UNSUPPORTED, "lmn_adjoint: max_depth above LMN_ADJOINT_MAX_DEPTH (16)");
+  uint32_t n_mat = uint32_t(H.matF.size());
+  AdjDev A{};
+  A.texture = spec->texture;
+  A.emission_material = spec->emission_material;
+  A.n_slots = uint32_t(H.slot_mat.size());
+  if (A.n_slots > 256) return set_err(LMN_ERR_UNSUPPORTED, "lmn_adjoint: more than 256 microfacet materials");
+  size_t n_texels = 0;
+  if (spec->texture >= 0) {
+    if (size_t(spec->texture) >= H.tex.size()) return set_err(LMN_ERR_INVALID, "lmn_adjoint: texture out of range");
+    const TextureDev& T = H.tex[spec->texture];
+    A.texel_base = T.offset / 3;
+    n_texels = size_t(T.w) * T.h;
+    for (uint32_t i = 0; i < n_mat; ++i) {
+      const MaterialT<float>& m = H.matF[i];
+      if (m.tex != spec->texture) continue;
+      bool ok = m.kind == M_LAMBERTIAN || (m.kind == M_MICROFACET && m.metallic == 0.0f);
+      if (!ok)
+        return set_err(LMN_ERR_UNSUPPORTED,
+                       "lmn_adjoint: the parameter texture may only be the albedo of Lambertian or metallic = 0 "
+                       "microfacet materials");
+    }
+  }
+  double inv_intensity = 0.0;
+  if (spec->emission_material >= 0) {
+    if (uint32_t(spec->emission_material) >= n_mat)
+      return set_err(LMN_ERR_INVALID, "lmn_adjoint: emission material out of range");
+    const MaterialT<double>& m = H.matD[spec->emission_material];
+    if (m.kind != M_EMISSIVE || !(m.intensity > 0.0))
+      return set_err(LMN_ERR_INVALID, "lmn_adjoint: emission material must be emissive with intensity > 0");
+    inv_intensity = 1.0 / m.intensity;
+  }
+  Workspace& W = workspace();
+  std::lock_guard<std::mutex> lock(W.mu);
+  rc = W.init();
+  if (rc != LMN_OK) return rc;
+  hipStream_t st = p->stream ? static_cast<hipStream_t>(p->stream) : W.stream;
+  uint32_t npix = fs.npix;
+  const uint32_t n_blocks = 2048;
+  uint32_t width = A.n_slots + 1;
+  HIP_TRY(W.adj_wsum.ensure(size_t(npix) * 8));
+  HIP_TRY(W.adj_gtex.ensure(std::max<size_t>(n_texels * 3 * (fp64 ? 8 : 4), 16)));
+  HIP_TRY(W.adj_part.ensure(size_t(n_blocks) * width * 8));
+  HIP_TRY(W.adj_red.ensure(size_t(width) * 8));
+  HIP_TRY(hipEventRecord(W.ev0, st));
+  launch_adj_zero_f64(W.adj_part.as<double>(), size_t(n_blocks) * width, st);
+  if (fp64)
+    launch_adj_zero_f64(W.adj_gtex.as<double>(), n_texels * 3, st);
+  else
+    launch_adj_zero_f32(W.adj_gtex.as<float>(), n_texels * 3, st);
+  uint64_t n_items = fs.black ? 0 : uint64_t(npix) * fs.F.spp_a;
+  if (n_items) {
+    if (fp64)
+      launch_adj_wsum_fp64(scene->viewD.cam, fs.F, W.adj_wsum.as<double>(), st);
+    else
+      launch_adj_wsum_fp32(scene->viewF.cam, fs.F, W.adj_wsum.as<double>(), st);
+    // launches of at most 2^22 paths keep each kernel short
+    const uint64_t chunk = uint64_t(1) << 22;
+    for (uint64_t k0 = 0; k0 < n_items; k0 += chunk) {
+      uint64_t k1 = std::min(n_items, k0 + chunk);
+      if (fp64) {
+        SceneView<double> S = scene->viewD;
+        S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
+        launch_adjoint_fp64(S, H.featD, fs.F, A, d_dldi, W.adj_wsum.as<double>(), W.adj_gtex.as<double>(),
+                            W.adj_part.as<double>(), n_blocks, k0, k1, st);
+      } else {
+        SceneView<float> S = scene->viewF;
+        S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
+        launch_adjoint_fp32(S, H.featF, fs.F, A, d_dldi, W.adj_wsum.as<double>(), W.adj_gtex.as<float>(),
+                            W.adj_part.as<double>(), n_blocks, k0, k1, st);
+      }
+    }
+  }
+  launch_adj_reduce(W.adj_part.as<double>(), n_blocks, width, W.adj_red.as<double>(), st);
+  if (d_tex_out && n_texels) {
+    if (fp64)
+      launch_adj_to_float(W.adj_gtex.as<double>(), d_tex_out, n_texels * 3, st);
+    else
+      HIP_TRY(hipMemcpyAsync(d_tex_out, W.adj_gtex.p, n_texels * 3 * 4, hipMemcpyDeviceToDevice, st));
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(W.ev1, st));
+  std::vector<double> red(width);
+  HIP_TRY(hipMemcpyAsync(red.data(), W.adj_red.p, width * 8, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+  if (rough_out) {
+    std::vector<float> r(n_mat, 0.0f);
+    for (uint32_t j = 0; j < A.n_slots; ++j) r[H.slot_mat[j]] = float(red[j]);
+    if (rough_on_device)
+      HIP_TRY(hipMemcpy(rough_out, r.data(), n_mat * 4, hipMemcpyHostToDevice));
+    else
+      std::memcpy(rough_out, r.data(), n_mat * 4);
+  }
+  g->emission = red[A.n_slots] * inv_intensity;
+  g->paths = n_items;
+  g->adjoint_ms = ms;
+  return LMN_OK;
+}
+
+}  // namespace
+
+int lmn_adjoint_device(const lmn_scene* scene, const lmn_render_params* params, const lmn_adjoint_spec* spec,
+                       const float* d_dl_dimage, lmn_grads* grads) {
+  if (!grads) return set_err(LMN_ERR_INVALID, "lmn_adjoint: null argument");
+  return adjoint_common(scene, params, spec, d_dl_dimage, grads->texels, grads->roughness, true, grads);
+}
+
+int lmn_adjoint(const lmn_scene* scene, const lmn_render_params* params, const lmn_adjoint_spec* spec,
+                const float* dl_dimage, lmn_grads* grads) {
+  if (!scene || !params || !spec || !dl_dimage || !grads) return set_err(LMN_ERR_INVALID, "lmn_adjoint: null argument");
+  FrameSetup fs;
+  int rc = frame_setup(params, fs);
+  if (rc != LMN_OK) return rc;
+  size_t img = size_t(fs.npix) * 3 * 4;
+  size_t n_tex = 0;
+  if (spec->texture >= 0 && size_t(spec->texture) < scene->host.tex.size())
+    n_tex = size_t(scene->host.tex[spec->texture].w) * scene->host.tex[spec->texture].h * 3;
+  float* d_in = nullptr;
+  float* d_tex = nullptr;
+  HIP_TRY(hipMalloc(&d_in, std::max<size_t>(img, 16)));
+  hipError_t e = hipMemcpy(d_in, dl_dimage, img, hipMemcpyHostToDevice);
+  if (e == hipSuccess && grads->texels && n_tex) e = hipMalloc(&d_tex, n_tex * 4);
+  if (e != hipSuccess) {
+    (void)hipFree(d_in);
+    return set_err(LMN_ERR_DEVICE, std::string("lmn_adjoint: ") + hipGetErrorString(e));
+  }
+  rc = adjoint_common(scene, params, spec, d_in, d_tex, grads->roughness, false, grads);
+  if (rc == LMN_OK && d_tex) {
+    e = hipMemcpy(grads->texels, d_tex, n_tex * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = set_err(LMN_ERR_DEVICE, std::string("lmn_adjoint: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(d_in);
+  if (d_tex) (void)hipFree(d_tex);
   return rc;
 }
 

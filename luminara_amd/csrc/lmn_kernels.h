@@ -75,6 +75,27 @@ void launch_plain_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev
                        uint32_t s1, unsigned long long* rays, hipStream_t s);
 void launch_plain_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, double4* acc, uint32_t s0,
                        uint32_t s1, unsigned long long* rays, hipStream_t s);
+// Path-replay adjoint (lmn_adjoint.hip)
+struct AdjDev {
+  int32_t texture;            // parameter texture, -1 = none
+  int32_t emission_material;  // parameter emissive material, -1 = none
+  uint64_t texel_base;        // texel index of the parameter texture's first texel
+  uint32_t n_slots;           // roughness slots (microfacet materials)
+  uint32_t pad;
+};
+void launch_adj_wsum_fp32(const CameraT<float>& C, const FrameDev& F, double* wsum, hipStream_t s);
+void launch_adj_wsum_fp64(const CameraT<double>& C, const FrameDev& F, double* wsum, hipStream_t s);
+void launch_adjoint_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, const AdjDev& A,
+                         const float* dldi, const double* wsum, float* g_tex, double* part, uint32_t n_blocks,
+                         uint64_t k0, uint64_t k1, hipStream_t s);
+void launch_adjoint_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, const AdjDev& A,
+                         const float* dldi, const double* wsum, double* g_tex, double* part, uint32_t n_blocks,
+                         uint64_t k0, uint64_t k1, hipStream_t s);
+void launch_adj_reduce(const double* part, uint32_t n_blocks, uint32_t width, double* out, hipStream_t s);
+void launch_adj_zero_f32(float* p, size_t n, hipStream_t s);
+void launch_adj_zero_f64(double* p, size_t n, hipStream_t s);
+void launch_adj_to_float(const double* in, float* out, size_t n, hipStream_t s);
+
 void launch_trace_fp32(const SceneView<float>& S, uint32_t feat, const lmn_ray* rays, lmn_hit* hits, size_t n,
                        hipStream_t s);
 void launch_trace_fp64(const SceneView<double>& S, uint32_t feat, const lmn_ray* rays, lmn_hit* hits, size_t n,

@@ -27,7 +27,7 @@ struct TextureDev {
 
 template <class R>
 struct LightT {
-  uint32_t kind, pad;
+  uint32_t kind, mat;  // mat: emissive material it was registered from, ~0u if none
   V3<R> origin, u, v, normal, emission;
   R radius, area;
 };
@@ -447,18 +447,34 @@ template <class R> LMN_HD V3<R> sample_ggx_normal(V3<R> n, R alpha, Rng& rng) {
   return unit(tg * hl.x + bt * hl.y + n * hl.z);
 }
 
+// What the path-replay adjoint needs to know about a scatter event: the
+// lobe taken and the cosines its weight was built from.
+enum : uint32_t { LOBE_OTHER = 0, LOBE_LAMBERT = 1, LOBE_GGX_SPEC = 2, LOBE_GGX_DIFFUSE = 3 };
+template <class R>
+struct ScatterInfo {
+  uint32_t lobe;
+  R ndv, ndl, ndh;  // GGX specular lobe (ndv clamped as in the weight)
+  V3<R> datt;       // d att / d albedo per channel (textured albedo lobes)
+};
+
 // Material::scatter for the five hot-path materials (src/material.rs:59-391).
 // Returns false when the material absorbs (scatter() == None).  `ray_d` is
-// the incoming (unnormalised) direction.
+// the incoming (unnormalised) direction.  `info`, when given, receives the
+// lobe and its cosines (the forward kernels pass nothing).
 template <class R>
 LMN_HD bool scatter(const SceneView<R>& S, const MaterialT<R>& m, V3<R> ray_d, const Surf<R>& h, Rng& rng,
-                    V3<R>& out_dir, V3<R>& att) {
+                    V3<R>& out_dir, V3<R>& att, ScatterInfo<R>* info = nullptr) {
+  if (info) info->lobe = LOBE_OTHER;
   switch (m.kind) {
     case M_LAMBERTIAN: {  // src/material.rs:59-71
       V3<R> dir = h.n + random_unit_vector<R>(rng);
       if (near_zero(dir)) dir = h.n;
       out_dir = dir;
       att = texture_value(S, m, h.u, h.v);
+      if (info) {
+        info->lobe = LOBE_LAMBERT;
+        info->datt = {R(1), R(1), R(1)};
+      }
       return true;
     }
     case M_METAL: {  // src/material.rs:94-106
@@ -522,6 +538,13 @@ LMN_HD bool scatter(const SceneView<R>& S, const MaterialT<R>& m, V3<R> ray_d, c
         V3<R> w = ndh > R(0.001) ? f * (g * vdh / (ndv * ndh)) : V3<R>{R(0), R(0), R(0)};
         out_dir = l;
         att = w / sp;
+        if (info) {
+          info->lobe = LOBE_GGX_SPEC;
+          info->ndv = ndv;
+          info->ndl = ndl;
+          info->ndh = ndh;
+          info->datt = {R(0), R(0), R(0)};  // f0 does not depend on albedo at metallic = 0
+        }
         return true;
       }
       V3<R> dir = n + random_unit_vector<R>(rng);
@@ -529,6 +552,10 @@ LMN_HD bool scatter(const SceneView<R>& S, const MaterialT<R>& m, V3<R> ray_d, c
       V3<R> kd = (V3<R>{R(1), R(1), R(1)} - fw) * (R(1) - m.metallic);
       out_dir = dir;
       att = hadamard(kd, albedo) / (R(1) - sp);
+      if (info) {
+        info->lobe = LOBE_GGX_DIFFUSE;
+        info->datt = kd / (R(1) - sp);
+      }
       return true;
     }
     default: return false;  // emissive: no scatter (src/material.rs:237-239)
@@ -552,12 +579,15 @@ template <class R> LMN_HD void build_onb(V3<R> n, V3<R>& a, V3<R>& b) {
 // light, samples a point and returns the unoccluded contribution and the
 // shadow ray (origin hit_point, unit dir, range [0.001, dist - 0.001]).
 // Returns false where the reference returns zero before its shadow query.
+// `geo` receives the contribution per unit albedo and emission (the scalar
+// factor of the estimate) and `light` the light picked.
 template <class R>
 LMN_HD bool nee_sample(const SceneView<R>& S, V3<R> hp, V3<R> normal, V3<R> albedo, Rng& rng, V3<R>& dir_out,
-                       R& dist_out, V3<R>& contrib) {
+                       R& dist_out, V3<R>& contrib, R* geo = nullptr, uint32_t* light = nullptr) {
   uint32_t nl = S.n_lights;
   uint32_t li = uint32_t(rng.next<R>() * R(nl));
   li = li < nl - 1 ? li : nl - 1;
+  if (light) *light = li;
   const LightT<R>& L = S.lights[li];
   const R pi = R(3.14159265358979323846);
   V3<R> lp, ln, em = L.emission;
@@ -620,7 +650,9 @@ LMN_HD bool nee_sample(const SceneView<R>& S, V3<R> hp, V3<R> normal, V3<R> albe
   if (!(ctl > R(0.001))) return false;
   dir_out = dir;
   dist_out = dist;
-  contrib = hadamard(em, albedo) * (ct * ctl * area * R(nl) / (pi * d2));
+  R g = ct * ctl * area * R(nl) / (pi * d2);
+  contrib = hadamard(em, albedo) * g;
+  if (geo) *geo = g;
   return true;
 }
 

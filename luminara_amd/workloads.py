@@ -44,7 +44,10 @@ def _v(a):
 
 def _header(width, height, spp, depth, seed, background, camera):
     lines = ["[render]", f"width = {width}", f"height = {height}", f"samples = {spp}",
-             f"max_depth = {depth}", f"seed = {seed}", f"background = {background}", "", "[camera]"]
+             f"max_depth = {depth}", f"seed = {seed}"]
+    if background is not None:  # None: the loader's default, the sky gradient
+        lines.append(f"background = {background}")
+    lines += ["", "[camera]"]
     for k, v in camera.items():
         lines.append(f"{k} = {_v(v) if isinstance(v, (list, tuple)) else repr(float(v))}")
     lines.append("")
@@ -342,16 +345,16 @@ def mesh_heavy(width=1920, height=1080, spp=64, depth=12, seed=42, freq=256, pat
 def batch_scene(index, seed=42, max_res=512):
     """Scene `index` of the 64-scene batch: a seeded displaced geodesic sphere
     with a log-uniform triangle count in [1k, 200k] plus a few random spheres
-    on a ground quad, 1-4 emissive quad lights (NEE), depth 8; resolution in
-    {128, 256, 384, 512}^2 and spp in {16, 64, 256}.  Returns
-    (toml, add_mesh, info)."""
+    on a ground quad, 1-4 emissive quad lights (NEE), the default sky
+    background, depth 8; resolution in {128, 256, 384, 512}^2 and spp in
+    {16, 64, 256}.  Returns (toml, add_mesh, info)."""
     rng = SeededRng(seed, 4, index)
     n_target = math.exp(rng.uniform(math.log(1000), math.log(200000)))
     freq = max(7, int(round(math.sqrt(n_target / 20.0))))
     res = [128, 256, 384, 512][rng.randint(4)]
     res = min(res, max_res)
     spp = [16, 64, 256][rng.randint(3)]
-    L = _header(res, res, spp, 8, seed + index, '{ type = "black" }',
+    L = _header(res, res, spp, 8, seed + index, None,
                 {"look_from": [0.0, 2.2, 6.5], "look_at": [0.0, 1.0, 0.0], "vfov": 40, "aperture": 0.0,
                  "focus_dist": 1.0})
     for k in range(6):
@@ -384,3 +387,64 @@ def batch_scene(index, seed=42, max_res=512):
 
     info = {"res": res, "spp": spp, "tris": 20 * freq * freq, "lights": n_lights}
     return toml, add_mesh, info
+
+
+def inverse_rendering_scene(width=512, height=512, spp=32, depth=8, seed=42, tex_res=256, n_spheres=32,
+                            hidden=False, intensity=None):
+    """Differentiable inverse-rendering workload: the Cornell geometry, a
+    tex_res^2 FP32 albedo texture on the back wall (texel values k/255),
+    n_spheres seeded GGX spheres (r U[20,60]) each with its own roughness, and
+    the light's emission scalar (the intensity of its material).  hidden=True
+    draws the parameters of the target image from the seed; otherwise they
+    are the starting point (grey texture, roughness 0.3, intensity 1);
+    `intensity` overrides the emission scalar.
+    Returns (toml, setup) where setup(world) installs the texture and returns
+    {"texture", "wall_material", "emission_material", "sphere_materials"}."""
+    rng = SeededRng(seed, 5)
+    L = cornell_box(width, height, spp, depth, seed).split("\n")
+    spheres = []
+    for k in range(n_spheres):
+        r = rng.uniform(20.0, 60.0)
+        c = [rng.uniform(r, 555.0 - r), r, rng.uniform(r, 555.0 - r)]
+        col = [rng.uniform(0.3, 0.9) for _ in range(3)]
+        spheres.append((c, r, col))
+    hrng = SeededRng(seed, 6)
+    rough = [hrng.uniform(0.05, 0.6) if hidden else 0.3 for _ in range(n_spheres)]
+    hidden_intensity = hrng.uniform(0.7, 1.3)
+    if intensity is None:
+        intensity = hidden_intensity if hidden else 1.0
+    for (c, r, col), ro in zip(spheres, rough):
+        m = {"type": "pbr", "color": col, "roughness": ro, "metallic": 0.0}
+        L += ["[[sphere]]", f"center = {_v(c)}", f"radius = {r!r}", f"material = {_mat(m)}", ""]
+    toml = "\n".join(L)
+    toml = toml.replace("intensity = 1.0 }", f"intensity = {intensity!r} }}", 1)
+    import numpy as np
+    if hidden:
+        k = SeededRng(seed, 7).state
+        z = (np.uint64(k) + np.arange(1, tex_res * tex_res * 3 + 1, dtype=np.uint64)
+             * np.uint64(0x9E3779B97F4A7C15))  # SplitMix64 sequence, wrapping
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+        texels = ((z % np.uint64(256)).astype(np.float32) / np.float32(255.0)).reshape(tex_res, tex_res, 3)
+    else:
+        texels = np.full((tex_res, tex_res, 3), np.float32(186.0 / 255.0), dtype=np.float32)
+
+    def setup(world):
+        from . import _abi
+        prims = world.prims
+        wall = emit = None
+        for p in prims:
+            v = list(p.p)
+            if p.kind == _abi.PRIM_QUAD and v[:3] == [0.0, 0.0, 555.0] and v[3:6] == [555.0, 0.0, 0.0]:
+                wall = p.material
+            if p.kind == _abi.PRIM_QUAD and world.materials()[p.material].kind == _abi.MAT_EMISSIVE:
+                emit = p.material
+        tex = world.add_texture(texels)
+        m = world.materials()[wall]
+        m.texture = tex
+        world.set_material(wall, m)
+        sph = sorted({p.material for p in prims if p.kind == _abi.PRIM_SPHERE})
+        return {"texture": tex, "wall_material": wall, "emission_material": emit, "sphere_materials": sph}
+
+    return toml, setup

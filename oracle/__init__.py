@@ -26,7 +26,8 @@ class OracleParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
                 ("seed", C.c_uint64), ("filter", C.c_uint32), ("has_crop", C.c_uint32), ("crop", C.c_uint32 * 4),
                 ("rng_mode", C.c_uint32), ("scene_key", C.c_uint32), ("threads", C.c_uint32),
-                ("exact_marched", C.c_uint32), ("row_begin", C.c_uint32), ("row_end", C.c_uint32)]
+                ("exact_marched", C.c_uint32), ("row_begin", C.c_uint32), ("row_end", C.c_uint32),
+                ("row_step", C.c_uint32), ("pad", C.c_uint32)]
 
 
 class OracleStats(C.Structure):
@@ -68,6 +69,9 @@ def lib():
         L.oracle_background.argtypes = [C.POINTER(_abi.SceneDesc), D3, D3]
         L.oracle_filter_weight.restype = C.c_double
         L.oracle_filter_weight.argtypes = [C.c_uint32, C.c_double, C.c_double]
+        L.oracle_adjoint.restype = C.c_int
+        L.oracle_adjoint.argtypes = [C.POINTER(_abi.SceneDesc), C.POINTER(OracleParams), C.c_int32, C.c_int32,
+                                     D3, D3, D3, D3]
         L.oracle_reflectance.restype = C.c_double
         L.oracle_reflectance.argtypes = [C.c_double, C.c_double]
         _lib = L
@@ -89,8 +93,9 @@ def trace(world, rays: np.ndarray, exact=True):
 
 
 def render(world, config, rng_mode=RNG_COUNTER, threads=0, scene_key=0, exact=True, rows=None,
-           want_var=True):
-    """Pixel loop on the CPU.  Returns (mean HxWx3, var HxWx3 or None, stats)."""
+           want_var=True, row_step=1):
+    """Pixel loop on the CPU.  Returns (mean HxWx3, var HxWx3 or None, stats);
+    `rows` = (begin, end) restricts it to every `row_step`-th row of a band."""
     p = OracleParams()
     p.width, p.height = config.width, config.height
     p.spp, p.max_depth, p.seed = config.samples_per_pixel, config.max_depth, config.seed
@@ -104,11 +109,10 @@ def render(world, config, rng_mode=RNG_COUNTER, threads=0, scene_key=0, exact=Tr
         cw, ch = min(config.crop[2], fw), min(config.crop[3], fh)
     p.rng_mode, p.scene_key, p.threads = rng_mode, scene_key, threads
     p.exact_marched = 1 if exact else 0
-    if rows is not None:
-        p.row_begin, p.row_end = rows
-        nrows = min(rows[1], ch) - rows[0]
-    else:
-        nrows = ch
+    rb, re = rows if rows is not None else (0, ch)
+    re = min(re, ch)
+    p.row_begin, p.row_end, p.row_step = rb, re, row_step
+    nrows = max(0, (re - rb + row_step - 1) // row_step)
     mean = np.zeros((nrows, cw, 3), dtype=np.float64)
     var = np.zeros((nrows, cw, 3), dtype=np.float64) if want_var else None
     st = OracleStats()
@@ -164,3 +168,38 @@ def filter_weight(kind, dx, dy):
 
 def reflectance(cosine, ratio):
     return lib().oracle_reflectance(cosine, ratio)
+
+
+def _params(config, rng_mode=RNG_COUNTER, threads=0, scene_key=0, exact=True):
+    p = OracleParams()
+    p.width, p.height = config.width, config.height
+    p.spp, p.max_depth, p.seed = config.samples_per_pixel, config.max_depth, config.seed
+    p.filter = config.pixel_filter
+    if config.crop is not None:
+        p.has_crop = 1
+        for i, v in enumerate(config.crop):
+            p.crop[i] = v
+    p.rng_mode, p.scene_key, p.threads = rng_mode, scene_key, threads
+    p.exact_marched = 1 if exact else 0
+    return p
+
+
+def adjoint(world, config, dl_dimage, texture=-1, emission_material=-1, threads=0, scene_key=0, exact=True):
+    """FP64 path-replay adjoint on the CPU: returns (texel grads (h, w, 3) or
+    None, roughness grads per material, emission grad)."""
+    p = _params(config, threads=threads, scene_key=scene_key, exact=exact)
+    g = np.ascontiguousarray(dl_dimage, dtype=np.float64)
+    d = world.desc_struct
+    tex = None
+    if texture >= 0:
+        if texture >= d.n_textures:
+            raise ValueError(f"texture {texture} out of range")
+        t = d.textures[texture]
+        tex = np.zeros((t.height, t.width, 3))
+    rough = np.zeros(max(d.n_materials, 1))
+    em = C.c_double()
+    dp = C.POINTER(C.c_double)
+    _check(lib().oracle_adjoint(world.desc, C.byref(p), texture, emission_material, g.ctypes.data_as(dp),
+                                tex.ctypes.data_as(dp) if tex is not None else None, rough.ctypes.data_as(dp),
+                                C.byref(em)))
+    return tex, rough[:d.n_materials], em.value

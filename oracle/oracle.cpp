@@ -178,6 +178,14 @@ struct Texture {
   Vec3 color;
   const float* texels = nullptr;
   uint32_t w = 0, h = 0;
+  int32_t id = -1;  // descriptor texture index
+  int64_t texel(double u, double v) const {  // the texel a lookup reads (x + y * w)
+    u = std::min(std::max(u, 0.0), 1.0);
+    v = 1.0 - std::min(std::max(v, 0.0), 1.0);
+    uint32_t i = std::min(uint32_t(u * double(w)), w - 1);
+    uint32_t j = std::min(uint32_t(v * double(h)), h - 1);
+    return int64_t(j) * w + i;
+  }
   Vec3 value(double u, double v) const {  // texture.rs:19-23, :463-479
     if (!texels) return color;
     u = std::min(std::max(u, 0.0), 1.0);
@@ -189,8 +197,17 @@ struct Texture {
   }
 };
 
+// Lobe taken by a scatter and the cosines of its weight (for the adjoint).
+enum { LOBE_OTHER = 0, LOBE_LAMBERT = 1, LOBE_GGX_SPEC = 2, LOBE_GGX_DIFFUSE = 3 };
+struct ScatterInfo {
+  int lobe = LOBE_OTHER;
+  double ndv = 0, ndl = 0, ndh = 0;
+  Vec3 datt;  // d att / d albedo per channel
+};
+
 struct Material {
   uint32_t kind = 0;
+  uint32_t id = 0;  // descriptor material index
   Texture tex;
   Vec3 albedo;  // metal albedo / dielectric tint
   double fuzz = 0, ior = 1.5, roughness = 0, dispersion = 0, intensity = 1, metallic = 0;
@@ -232,13 +249,17 @@ struct Material {
     return (tg * hl.x + bt * hl.y + n * hl.z).unit();
   }
 
-  bool scatter(const Ray& r, const HitRecord& h, Rng& rng, Ray& out, Vec3& att) const {
+  bool scatter(const Ray& r, const HitRecord& h, Rng& rng, Ray& out, Vec3& att, ScatterInfo* info = nullptr) const {
     switch (kind) {
       case LMN_MAT_LAMBERTIAN: {  // :59-71
         Vec3 d = h.n + random_unit_vector(rng);
         if (d.near_zero()) d = h.n;
         out = Ray{h.p, d, r.time};
         att = tex.value(h.u, h.v);
+        if (info) {
+          info->lobe = LOBE_LAMBERT;
+          info->datt = Vec3(1, 1, 1);
+        }
         return true;
       }
       case LMN_MAT_METAL: {  // :94-106
@@ -302,6 +323,12 @@ struct Material {
           Vec3 w = ndh > 0.001 ? f * (g * vdh / (ndv * ndh)) : Vec3();
           out = Ray{h.p, l, r.time};
           att = w / sp;
+          if (info) {
+            info->lobe = LOBE_GGX_SPEC;
+            info->ndv = ndv;
+            info->ndl = ndl;
+            info->ndh = ndh;
+          }
           return true;
         }
         Vec3 d = n + random_unit_vector(rng);
@@ -309,6 +336,10 @@ struct Material {
         Vec3 kd = (Vec3(1, 1, 1) - fw) * (1.0 - metallic);
         out = Ray{h.p, d, r.time};
         att = kd.hadamard(alb) / (1.0 - sp);
+        if (info) {
+          info->lobe = LOBE_GGX_DIFFUSE;
+          info->datt = kd / (1.0 - sp);
+        }
         return true;
       }
       default: return false;  // Emissive (:237-239)
@@ -1073,6 +1104,7 @@ struct Light {
   uint32_t kind;
   Vec3 o, u, v, n, e;
   double radius;
+  uint32_t mat;  // emissive material it was registered from, ~0u if none
 };
 
 struct World {
@@ -1152,8 +1184,12 @@ void build_onb(const Vec3& n, Vec3& a, Vec3& b) {
 }
 
 // sample_direct_light (render.rs:1346-1466)
+// D_out (when given) receives the unoccluded contribution per unit albedo and
+// light_mat the picked light's material (both left zero / ~0u otherwise).
 Vec3 sample_direct_light(const World& w, const Vec3& hp, const Vec3& normal, const Vec3& albedo, Rng& rng,
-                         Counters& cnt) {
+                         Counters& cnt, Vec3* D_out = nullptr, uint32_t* light_mat = nullptr) {
+  if (D_out) *D_out = Vec3();
+  if (light_mat) *light_mat = ~0u;
   if (w.lights.empty()) return {};
   size_t nl = w.lights.size();
   double lf = rng.next_f64() * double(nl);
@@ -1222,7 +1258,10 @@ Vec3 sample_direct_light(const World& w, const Vec3& hp, const Vec3& normal, con
   HitRecord hh;
   cnt.shadow++;
   if (w.hit(sr, 0.001, dist - 0.001, hh)) return {};
-  return em.hadamard(albedo) * (ct * ctl * area * double(nl) / (M_PI * d2));
+  double g = ct * ctl * area * double(nl) / (M_PI * d2);
+  if (D_out) *D_out = em * g;
+  if (light_mat) *light_mat = L.mat;
+  return em.hadamard(albedo) * g;
 }
 
 // ray_color (render.rs:1472-1537)
@@ -1249,6 +1288,75 @@ Vec3 ray_color(const World& w, const Ray& r, Rng& rng, uint32_t depth, bool skip
     return em;
   }
   return w.background(r);
+}
+
+// ---- path-replay adjoint (FP64): ray_color again, with the derivative of its
+// expected value (directions held fixed, Russian roulette through its
+// survival probability q; the NEE term is att (.) D and not divided by q):
+//   d/dtheta = dlog f (att/q (.) Lnext + direct) + [q = max att] (dlog f - dlog p) direct
+// with dlog f = datt/att for albedo, dlog D + dlog G for GGX roughness (and
+// dlog p = dlog D), and emission linear in the intensity.
+struct AdjCtx {
+  int32_t tex_id = -1, em_mat = -1;
+  std::vector<double> g_tex;    // texels * 3 of the parameter texture
+  std::vector<double> g_rough;  // per material
+  double g_em = 0;              // sum of the adjoint-weighted emission terms (to divide by I)
+};
+
+void ggx_dlog(double r, double ndv, double ndl, double ndh, double& dlogd, double& dlogg) {
+  double a = r * r, c2 = ndh * ndh;
+  double den = c2 * (a * a - 1.0) + 1.0;
+  dlogd = (2.0 / a - 4.0 * c2 * a / den) * 2.0 * r;  // D = a^2 / (pi den^2)
+  double k = (r + 1.0) * (r + 1.0) / 8.0, dk = (r + 1.0) / 4.0;
+  dlogg = -((1.0 - ndv) / (ndv * (1.0 - k) + k) + (1.0 - ndl) / (ndl * (1.0 - k) + k)) * dk;
+}
+
+Vec3 ray_color_adj(const World& w, const Ray& r, Rng& rng, uint32_t depth, bool skip, Counters& cnt,
+                   const Vec3& Aw, AdjCtx& ctx) {
+  if (depth == 0) return {};
+  HitRecord h;
+  cnt.ext++;
+  if (!w.hit(r, 0.001, INFINITY, h)) return w.background(r);
+  Vec3 em = skip ? Vec3() : h.mat->emitted(h.u, h.v);
+  if (!skip && int32_t(h.mat->id) == ctx.em_mat) ctx.g_em += Aw.dot(em);
+  Ray sc;
+  Vec3 att;
+  ScatterInfo si;
+  if (!h.mat->scatter(r, h, rng, sc, att, &si)) return em;
+  double mc = std::fmax(std::fmax(att.x, att.y), att.z);
+  double surv = std::min(std::max(mc, 0.05), 1.0);
+  bool rr = depth < 47;
+  if (rr && rng.next_f64() > surv) return em;
+  double rrw = rr ? 1.0 / surv : 1.0;
+  bool use_nee = !h.mat->is_specular() && !w.lights.empty();
+  Vec3 D;
+  uint32_t lmat = ~0u;
+  Vec3 direct = use_nee ? sample_direct_light(w, h.p, h.n, att, rng, cnt, &D, &lmat) : Vec3();
+  Vec3 Lnext = ray_color_adj(w, sc, rng, depth - 1, use_nee, cnt, Aw.hadamard(att) * rrw, ctx);
+  Vec3 res = em + direct + att.hadamard(Lnext) * rrw;
+  if (std::isnan(res.x) || std::isnan(res.y) || std::isnan(res.z)) return {};
+  double Ad = Aw.dot(direct);
+  if (lmat != ~0u && int32_t(lmat) == ctx.em_mat) ctx.g_em += Ad;
+  bool rru = rr && mc > 0.05 && mc < 1.0;
+  int am = att.x >= att.y ? (att.x >= att.z ? 0 : 2) : (att.y >= att.z ? 1 : 2);
+  if ((si.lobe == LOBE_LAMBERT || si.lobe == LOBE_GGX_DIFFUSE) && h.mat->tex.texels && h.mat->tex.id == ctx.tex_id) {
+    int64_t t = h.mat->tex.texel(h.u, h.v);
+    double aw[3] = {Aw.x, Aw.y, Aw.z}, da[3] = {si.datt.x, si.datt.y, si.datt.z};
+    double ln[3] = {Lnext.x, Lnext.y, Lnext.z}, dd[3] = {D.x, D.y, D.z};
+    for (int c = 0; c < 3; ++c) {
+      double g = aw[c] * da[c] * (rrw * ln[c] + dd[c]);
+      if (rru && c == am) g += da[c] * Ad / surv;
+      ctx.g_tex[size_t(t) * 3 + c] += g;
+    }
+  }
+  if (si.lobe == LOBE_GGX_SPEC && h.mat->kind == LMN_MAT_MICROFACET && mc > 0.0) {
+    double dd, dg;
+    ggx_dlog(h.mat->roughness, si.ndv, si.ndl, si.ndh, dd, dg);
+    double g = (dd + dg) * (Aw.dot(att.hadamard(Lnext) * rrw) + Ad);
+    if (rru) g += dg * Ad;
+    ctx.g_rough[h.mat->id] += g;
+  }
+  return res;
 }
 
 // PixelFilter::weight (render.rs:198-237)
@@ -1391,6 +1499,7 @@ bool build_world(const lmn_scene_desc& d, bool exact, World& w) {
     const lmn_material& m = d.materials[i];
     auto mm = std::make_unique<Material>();
     mm->kind = m.kind;
+    mm->id = i;
     mm->tex.color = v3(m.color);
     if (m.texture >= 0) {
       if (uint32_t(m.texture) >= d.n_textures) {
@@ -1401,6 +1510,7 @@ bool build_world(const lmn_scene_desc& d, bool exact, World& w) {
       mm->tex.texels = t.rgb;
       mm->tex.w = t.width;
       mm->tex.h = t.height;
+      mm->tex.id = m.texture;
     }
     mm->albedo = v3(m.color);
     mm->fuzz = m.fuzz;
@@ -1440,7 +1550,8 @@ bool build_world(const lmn_scene_desc& d, bool exact, World& w) {
   if (!bounded.empty()) w.bvh = bvh_build(std::move(bounded));  // SceneWorld::from_list (scene.rs:786-808)
   for (uint32_t k = 0; k < d.n_lights; ++k) {
     const lmn_light& l = d.lights[k];
-    w.lights.push_back(Light{l.kind, v3(l.origin), v3(l.u), v3(l.v), v3(l.normal), v3(l.emission), l.radius});
+    w.lights.push_back(
+        Light{l.kind, v3(l.origin), v3(l.u), v3(l.v), v3(l.normal), v3(l.emission), l.radius, l.material});
   }
   const lmn_background& b = d.config.background;
   w.bg_kind = b.kind;
@@ -1511,17 +1622,20 @@ int oracle_render(const lmn_scene_desc* desc, const oracle_params* p, double* me
     ch = std::min(p->crop[3], fh);
   }
   uint32_t rb = p->row_begin, re = p->row_end ? std::min(p->row_end, ch) : ch;
+  uint32_t rs = p->row_step ? p->row_step : 1;
+  uint32_t n_rows = re > rb ? (re - rb + rs - 1) / rs : 0;
   uint32_t sq = uint32_t(std::ceil(std::sqrt(double(p->spp))));
   uint64_t fkey = frame_key(p->seed, p->scene_key);
-  std::atomic<uint32_t> next_row{rb};
+  std::atomic<uint32_t> next_row{0};
   std::atomic<uint64_t> n_ext{0}, n_sh{0};
   unsigned nt = p->threads ? p->threads : std::max(1u, std::thread::hardware_concurrency());
   auto t0 = std::chrono::steady_clock::now();
   auto worker = [&]() {
     Counters cnt;
     for (;;) {
-      uint32_t j = next_row.fetch_add(1);
-      if (j >= re) break;
+      uint32_t r = next_row.fetch_add(1);
+      if (r >= n_rows) break;
+      uint32_t j = rb + r * rs;
       uint32_t gj = j + cy;
       double y = double(fh - 1 - gj);
       std::unique_ptr<XoshiroRng> row_rng;
@@ -1556,7 +1670,7 @@ int oracle_render(const lmn_scene_desc* desc, const oracle_params* p, double* me
             }
           }
         Vec3 out = tw > 0.0 ? color / tw : color;
-        size_t idx = (size_t(j - rb) * cw + i) * 3;
+        size_t idx = (size_t(r) * cw + i) * 3;
         mean[idx] = out.x;
         mean[idx + 1] = out.y;
         mean[idx + 2] = out.z;
@@ -1572,7 +1686,7 @@ int oracle_render(const lmn_scene_desc* desc, const oracle_params* p, double* me
   for (auto& t : th) t.join();
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (st) {
-    st->camera_rays = uint64_t(re - rb) * cw * sq * sq;
+    st->camera_rays = uint64_t(n_rows) * cw * sq * sq;
     st->extension_rays = n_ext.load() - (sq ? st->camera_rays : 0);
     st->shadow_rays = n_sh.load();
     st->render_secs = secs;
@@ -1686,5 +1800,111 @@ int oracle_background(const lmn_scene_desc* desc, const double dir[3], double ou
 double oracle_filter_weight(uint32_t filter, double dx, double dy) { return filter_weight(filter, dx, dy); }
 
 double oracle_reflectance(double cosine, double ratio) { return Material::reflectance(cosine, ratio); }
+
+// Path-replay adjoint on the CPU in FP64 (counter RNG): each sample is traced
+// once for its radiance (the luminance clamp's Jacobian needs it) and replayed
+// from the same seed by ray_color_adj.  Rows are split statically over the
+// threads and their sums merged in thread order, so the result is
+// deterministic.
+int oracle_adjoint(const lmn_scene_desc* desc, const oracle_params* p, int32_t texture, int32_t emission_material,
+                   const double* dldi, double* g_tex, double* g_rough, double* g_em) {
+  if (!desc || !p || !dldi) {
+    g_err = "null argument";
+    return LMN_ERR_INVALID;
+  }
+  World w;
+  if (!build_world(*desc, p->exact_marched != 0, w)) return LMN_ERR_INVALID;
+  uint32_t fw = std::max<uint32_t>(p->width, 1), fh = std::max<uint32_t>(p->height, 1);
+  uint32_t cx = 0, cy = 0, cw = fw, ch = fh;
+  if (p->has_crop) {
+    cx = std::min(p->crop[0], fw);
+    cy = std::min(p->crop[1], fh);
+    cw = std::min(p->crop[2], fw);
+    ch = std::min(p->crop[3], fh);
+  }
+  uint32_t sq = uint32_t(std::ceil(std::sqrt(double(p->spp))));
+  uint64_t fkey = frame_key(p->seed, p->scene_key);
+  size_t n_tex = 0;
+  if (texture >= 0) {
+    if (uint32_t(texture) >= desc->n_textures) {
+      g_err = "texture out of range";
+      return LMN_ERR_INVALID;
+    }
+    n_tex = size_t(desc->textures[texture].width) * desc->textures[texture].height * 3;
+  }
+  unsigned nt = p->threads ? p->threads : std::max(1u, std::thread::hardware_concurrency());
+  std::vector<AdjCtx> ctx(nt);
+  for (auto& c : ctx) {
+    c.tex_id = texture;
+    c.em_mat = emission_material;
+    c.g_tex.assign(n_tex, 0.0);
+    c.g_rough.assign(desc->n_materials, 0.0);
+  }
+  auto worker = [&](unsigned ti) {
+    Counters cnt;
+    AdjCtx& C = ctx[ti];
+    for (uint32_t j = ti; j < ch && p->max_depth > 0; j += nt) {
+      uint32_t gj = j + cy;
+      double y = double(fh - 1 - gj);
+      for (uint32_t i = 0; i < cw; ++i) {
+        uint32_t gi = i + cx;
+        uint64_t gpix = uint64_t(gj) * fw + gi;
+        double tw = 0.0;
+        for (uint32_t s = 0; s < sq * sq; ++s) {
+          CounterRng rng(path_key(fkey, gpix, s));
+          uint32_t sy = s / sq, sx = s % sq;
+          double jx = (double(sx) + rng.next_f64()) / double(sq);
+          double jy = (double(sy) + rng.next_f64()) / double(sq);
+          tw += filter_weight(p->filter, jx - 0.5, jy - 0.5);
+        }
+        const double* g = dldi + (size_t(j) * cw + i) * 3;
+        for (uint32_t s = 0; s < sq * sq; ++s) {
+          uint32_t sy = s / sq, sx = s % sq;
+          Vec3 L;
+          double wt = 0;
+          for (int pass = 0; pass < 2; ++pass) {
+            CounterRng rng(path_key(fkey, gpix, s));
+            double jx = (double(sx) + rng.next_f64()) / double(sq);
+            double jy = (double(sy) + rng.next_f64()) / double(sq);
+            double u = (double(gi) + jx) / double(fw - 1);
+            double v = (y + jy) / double(fh - 1);
+            Ray r = w.get_ray(u, v, rng);
+            if (pass == 0) {
+              L = ray_color(w, r, rng, p->max_depth, false, cnt);
+              wt = filter_weight(p->filter, jx - 0.5, jy - 0.5);
+              continue;
+            }
+            double gx = g[0], gy = g[1], gz = g[2];
+            double lum = 0.2126 * L.x + 0.7152 * L.y + 0.0722 * L.z;
+            if (lum > 100.0) {
+              double f = 100.0 / lum, sg = (L.x * gx + L.y * gy + L.z * gz) / lum;
+              gx = f * (gx - 0.2126 * sg);
+              gy = f * (gy - 0.7152 * sg);
+              gz = f * (gz - 0.0722 * sg);
+            }
+            double sc = tw > 0.0 ? wt / tw : 1.0;
+            ray_color_adj(w, r, rng, p->max_depth, false, cnt, Vec3(gx * sc, gy * sc, gz * sc), C);
+          }
+        }
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned k = 0; k < nt; ++k) th.emplace_back(worker, k);
+  for (auto& t : th) t.join();
+  double em = 0.0;
+  for (unsigned k = 0; k < nt; ++k) {
+    for (size_t q = 0; q < n_tex; ++q)
+      if (g_tex) g_tex[q] = (k ? g_tex[q] : 0.0) + ctx[k].g_tex[q];
+    for (uint32_t m = 0; m < desc->n_materials; ++m)
+      if (g_rough) g_rough[m] = (k ? g_rough[m] : 0.0) + ctx[k].g_rough[m];
+    em += ctx[k].g_em;
+  }
+  if (g_em) {
+    double inten = emission_material >= 0 ? desc->materials[emission_material].intensity : 0.0;
+    *g_em = inten > 0.0 ? em / inten : 0.0;
+  }
+  return LMN_OK;
+}
 
 }  // extern "C"

@@ -42,8 +42,10 @@ typedef struct oracle_params {
   uint32_t scene_key;
   uint32_t threads;        /* 0 = hardware concurrency */
   uint32_t exact_marched;  /* 1 = exact roots for torus/superellipsoid/spring */
-  uint32_t row_begin;      /* render only rows [row_begin, row_end) of the crop; */
-  uint32_t row_end;        /* row_end = 0 means all rows                        */
+  uint32_t row_begin;      /* render only rows [row_begin, row_end) of the crop, */
+  uint32_t row_end;        /* every row_step-th one; row_end = 0 means all rows, */
+  uint32_t row_step;       /* row_step 0 means 1                                 */
+  uint32_t pad;
 } oracle_params;
 
 typedef struct oracle_stats {
@@ -82,6 +84,13 @@ int oracle_camera_ray(const lmn_scene_desc* desc, double s, double t, uint64_t k
 int oracle_emitted(const lmn_scene_desc* desc, uint32_t material, double u, double v, double out[3]);
 int oracle_background(const lmn_scene_desc* desc, const double dir[3], double out[3]);
 double oracle_filter_weight(uint32_t filter, double dx, double dy);
+
+/* Path-replay adjoint in FP64 (counter RNG): dL/dI (crop_h*crop_w*3) in;
+ * gradients with respect to the texels of `texture` (width*height*3), every
+ * material's roughness (n_materials; microfacet only) and the intensity of
+ * `emission_material` out.  Either index may be -1. */
+int oracle_adjoint(const lmn_scene_desc* desc, const oracle_params* p, int32_t texture, int32_t emission_material,
+                   const double* dldi, double* g_tex, double* g_rough, double* g_em);
 double oracle_reflectance(double cosine, double ratio);
 
 #ifdef __cplusplus

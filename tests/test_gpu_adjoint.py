@@ -1,0 +1,139 @@
+"""GPU path-replay adjoint (lmn_adjoint) on the inverse-rendering workload:
+Cornell geometry, an albedo texture on the back wall, GGX spheres with
+per-sphere roughness and the light's emission scalar.
+
+* FP64 kernel vs the CPU FP64 adjoint (oracle): the same estimator on the
+  same paths, so they agree to accumulation rounding (texel gradients come
+  back in FP32).
+* FP32 kernel vs FP64 kernel: within FP32 accuracy.
+* Against central finite differences (h = 1e-3, common random numbers) of
+  the GPU FP64 forward render, on a linear functional sum(G * I) with fixed
+  weights: the emission gradient exactly (the image is linear in it); texels
+  and roughness in expectation over seeds, within 1 % and 3 standard errors.
+"""
+import numpy as np
+import pytest
+
+import luminara_amd as lm
+from luminara_amd import workloads
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def scene(w=32, h=32, spp=16, tex_res=8, n_spheres=4, intensity=None, hidden=False):
+    toml, setup = workloads.inverse_rendering_scene(w, h, spp, 6, 42, tex_res=tex_res, n_spheres=n_spheres,
+                                                    hidden=hidden, intensity=intensity)
+    cfg, _, world = lm.load_scene(toml)
+    return cfg, world, setup(world)
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def test_fp64_adjoint_matches_oracle(gpu):
+    cfg, w, info = scene()
+    # dL/dI crosses the C ABI in FP32: give the oracle the same values
+    G = np.random.default_rng(1).normal(size=(32, 32, 3)).astype(np.float32).astype(np.float64)
+    ot, orr, oe = oracle.adjoint(w, cfg, G, texture=info["texture"], emission_material=info["emission_material"])
+    g = lm.adjoint(cfg, w, G, texture=info["texture"], emission_material=info["emission_material"], path="plain64")
+    sph = info["sphere_materials"]
+    print("texel", rel_err(g.texels, ot), "rough", g.roughness[sph], orr[sph], "emission", g.emission, oe)
+    assert g.paths == 32 * 32 * 16
+    assert rel_err(g.texels, ot) < 1e-5
+    np.testing.assert_allclose(g.roughness[sph], orr[sph], rtol=1e-5, atol=1e-6 * np.abs(orr[sph]).max())
+    assert g.emission == pytest.approx(oe, rel=1e-9)
+    others = np.setdiff1d(np.arange(len(g.roughness)), sph)
+    assert np.all(g.roughness[others] == 0.0)
+
+
+def test_fp32_adjoint_close_to_fp64(gpu):
+    # the FP32 kernel's deviation from the FP64 kernel (same seed: rounding,
+    # and the rare path that rounding sends elsewhere) is small next to the
+    # Monte Carlo noise of the gradient estimate (FP64, another seed)
+    cfg, w, info = scene(64, 64, 64, tex_res=16, n_spheres=8)
+    G = np.random.default_rng(2).normal(size=(64, 64, 3))
+    kw = dict(texture=info["texture"], emission_material=info["emission_material"])
+    g64 = lm.adjoint(cfg, w, G, path="plain64", **kw)
+    g32 = lm.adjoint(cfg, w, G, path="wavefront", **kw)
+    cfg.seed += 1
+    g64b = lm.adjoint(cfg, w, G, path="plain64", **kw)
+    sph = info["sphere_materials"]
+    for name, a, b, c in (("texels", g32.texels, g64.texels, g64b.texels),
+                          ("roughness", g32.roughness[sph], g64.roughness[sph], g64b.roughness[sph]),
+                          ("emission", g32.emission, g64.emission, g64b.emission)):
+        dev, noise = rel_err(a, b), rel_err(c, b)
+        print(name, "fp32 vs fp64 %.3g, seed to seed %.3g" % (dev, noise))
+        assert dev < 0.25 * noise, name
+
+
+def test_adjoint_is_deterministic_to_rounding(gpu):
+    cfg, w, info = scene()
+    G = np.random.default_rng(3).normal(size=(32, 32, 3))
+    kw = dict(texture=info["texture"], emission_material=info["emission_material"])
+    a = lm.adjoint(cfg, w, G, **kw)
+    b = lm.adjoint(cfg, w, G, **kw)
+    assert rel_err(a.texels, b.texels) < 1e-5
+    assert a.emission == pytest.approx(b.emission, rel=1e-6)
+
+
+def _functional(cfg, world, G):
+    return float(np.sum(G * lm.render(cfg, None, world, path="plain64").rows.astype(np.float64)))
+
+
+def test_emission_gradient_matches_finite_difference_exactly(gpu):
+    cfg, w, info = scene(64, 64, 64, tex_res=16, n_spheres=8)
+    G = np.random.default_rng(4).normal(size=(64, 64, 3))
+    g = lm.adjoint(cfg, w, G, emission_material=info["emission_material"], path="plain64")
+    h = 1e-3
+    fp = _functional(*scene(64, 64, 64, tex_res=16, n_spheres=8, intensity=1.0 + h)[:2], G)
+    fm = _functional(*scene(64, 64, 64, tex_res=16, n_spheres=8, intensity=1.0 - h)[:2], G)
+    # the framebuffer is FP32: the difference carries ~1e-7 of the functional
+    assert g.emission == pytest.approx((fp - fm) / (2 * h), rel=1e-3)
+
+
+def test_texel_and_roughness_gradients_match_finite_differences(gpu):
+    # 128x128 at 4096 spp per seed (67 M paths), 8 seeds; the functional's
+    # weights G come from an independent render, as a loss gradient would
+    W = H = 128
+    cfg, w, info = scene(W, H, 4096, tex_res=16, n_spheres=8)
+    tcfg, tw, _ = scene(W, H, 256, tex_res=16, n_spheres=8, hidden=True)
+    T = lm.render(tcfg, None, tw, path="plain64").rows.astype(np.float64)
+    cfg0, w0, _ = scene(W, H, 256, tex_res=16, n_spheres=8)
+    cfg0.seed = 1234
+    G = 2.0 * (lm.render(cfg0, None, w0, path="plain64").rows.astype(np.float64) - T)
+    tex, sph = info["texture"], info["sphere_materials"]
+    base = np.full((16, 16, 3), np.float32(186.0 / 255.0), dtype=np.float32)
+    h_t = 2.0 ** -10
+    fd_t, ad_t, fd_r, ad_r = [], [], [], []
+    for seed in range(1, 9):
+        cfg.seed = seed
+        g = lm.adjoint(cfg, w, G, texture=tex, path="plain64")
+        vals = []
+        for s in (1, -1):
+            w.set_texture(tex, base + np.float32(s * h_t))
+            vals.append(_functional(cfg, w, G))
+        w.set_texture(tex, base)
+        fd_t.append((vals[0] - vals[1]) / (2 * h_t))
+        ad_t.append(float(g.texels.astype(np.float64).sum()))
+        vals = []
+        for s in (1, -1):
+            for m in sph:
+                mm = w.materials()[m]
+                mm.roughness = 0.3 + s * 1e-3
+                w.set_material(m, mm)
+            vals.append(_functional(cfg, w, G))
+        for m in sph:
+            mm = w.materials()[m]
+            mm.roughness = 0.3
+            w.set_material(m, mm)
+        fd_r.append((vals[0] - vals[1]) / 2e-3)
+        ad_r.append(float(g.roughness[sph].astype(np.float64).sum()))
+    for name, fd, ad in (("texels", fd_t, ad_t), ("roughness", fd_r, ad_r)):
+        fd, ad = np.array(fd), np.array(ad)
+        se = np.sqrt(fd.var(ddof=1) / len(fd) + ad.var(ddof=1) / len(ad))
+        print(name, "fd %.6g +- %.2g, adjoint %.6g +- %.2g" % (fd.mean(), fd.std(ddof=1) / np.sqrt(len(fd)),
+                                                            ad.mean(), ad.std(ddof=1) / np.sqrt(len(ad))))
+        diff = abs(fd.mean() - ad.mean())
+        assert diff <= max(3.0 * se, 0.01 * abs(fd.mean())), name
