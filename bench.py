@@ -11,7 +11,9 @@ depth 8, NEE on 1-4 quad lights -- all scenes in one wavefront.
 Prints one JSON line: Mpaths/s (paths = sum of W*H*ceil(sqrt(spp))^2), the
 plain FP64 one-path-per-thread GPU kernel on the same scenes and the
 relative RMSE between the two images, the roofline of the dominant kernel
-(extend: BVH traversal) and the CPU oracle timed on a bounded sample.
+(extend: BVH traversal), the CPU oracle timed on a bounded sample, and the
+inverse-rendering leg: adjoint gradient evaluations per second (a 32-spp
+forward render plus a 32-spp path-replay adjoint at 512^2).
 """
 import argparse
 import json
@@ -89,6 +91,61 @@ def cpu_baseline(scenes, seconds, threads):
     return paths, secs, 1.0 / step
 
 
+def adjoint_leg(steps, warmup, res=512, spp=32, tex_res=256, n_spheres=32):
+    """Inverse-rendering step on the Cornell geometry: a 32-spp FP32 forward
+    render, dL/dI of the L2 loss against a target rendered with hidden
+    parameters, and a 32-spp FP32 path-replay adjoint for the back wall's
+    albedo texels, the spheres' GGX roughness and the light's emission."""
+    import torch
+    import luminara_amd as lm
+    from luminara_amd import _abi, workloads
+    toml, setup = workloads.inverse_rendering_scene(res, res, spp, 8, 42, tex_res=tex_res, n_spheres=n_spheres)
+    cfg, _, w = lm.load_scene(toml)
+    info = setup(w)
+    ttoml, tsetup = workloads.inverse_rendering_scene(res, res, spp, 8, 42, tex_res=tex_res, n_spheres=n_spheres,
+                                                      hidden=True)
+    tcfg, _, tw = lm.load_scene(ttoml)
+    tsetup(tw)
+    shape = lm.output_shape(cfg) + (3,)
+    target = torch.empty(shape, dtype=torch.float32, device="cuda")
+    tp = lm.make_params(tcfg, path="plain64")
+    lm.render_device(tw, tp, target.data_ptr())
+    img = torch.empty_like(target)
+    dl = torch.empty_like(target)
+    n_mat = w.desc_struct.n_materials
+    gtex = torch.zeros(tex_res * tex_res * 3, dtype=torch.float32, device="cuda")
+    grough = torch.zeros(n_mat, dtype=torch.float32, device="cuda")
+    p = lm.make_params(cfg, path="wavefront")
+    spec = _abi.AdjointSpec(info["texture"], info["emission_material"], 0, 0)
+    n_params = tex_res * tex_res * 3 + len(info["sphere_materials"]) + 1
+
+    def step():
+        st = lm.render_device(w, p, img.data_ptr())
+        torch.sub(img, target, out=dl)
+        dl.mul_(2.0)
+        ad = lm.adjoint_device(w, p, spec, dl.data_ptr(), gtex.data_ptr(), grough.data_ptr())
+        return st, ad
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    fwd_ms = adj_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st, ad = step()
+        fwd_ms += st["render_ms"]
+        adj_ms += ad["adjoint_ms"]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    loss = float(((img - target) ** 2).sum())
+    paths = ad["paths"]
+    return {"workload": f"cornell_inverse_{res}x{res}_spp{spp}_tex{tex_res}_spheres{n_spheres}_depth8",
+            "grads_per_s": steps / el, "unit": "full gradient evaluations (forward + adjoint) per second",
+            "parameters": n_params, "parameter_grads_per_s": n_params * steps / el,
+            "ms_per_step": el * 1e3 / steps, "forward_ms": fwd_ms / steps, "adjoint_ms": adj_ms / steps,
+            "adjoint_mpaths_per_s": paths / (adj_ms / steps / 1e3) / 1e6, "loss": loss, "dtype": "f32"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +159,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--stack-lds", type=int, default=0)
     ap.add_argument("--pool", type=int, default=0)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering adjoint leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: every GPU box here has a single "
@@ -198,6 +256,11 @@ def main():
                          f"CPU FP64 oracle (restatement of the reference pixel loop), counter RNG"}
         log(f"[bench] cpu oracle: {cpu}")
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_leg(max(args.steps, 3), 1)
+        log(f"[bench] adjoint: {adj}")
+
     cfg0 = scenes[0][0]
     line = {
         "metric": METRIC, "value": value, "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
@@ -205,7 +268,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": dict({"workload": wl_name, "paths_per_step": paths, "bvh_nodes": nodes,
                         "max_depth": cfg0.max_depth}, **desc),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "adjoint": adj,
         "mrays_per_s": (stats[-1]["camera_rays"] + stats[-1]["extension_rays"] + stats[-1]["shadow_rays"])
                        / (ms_per_step / 1e3) / 1e6,
     }
