@@ -160,7 +160,20 @@ def main():
     ap.add_argument("--stack-lds", type=int, default=0)
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering adjoint leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (5 times): the reference kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 18  # float32 elements
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only: every GPU box here has a single "
                                    "MI355X; run with --gpus 1"}))

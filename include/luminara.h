@@ -218,10 +218,11 @@ typedef struct lmn_render_params {
   uint32_t block_size;     /* threads per workgroup, 0 = library pick     */
   uint32_t stack_lds;      /* LDS short-stack entries, 0 = library pick   */
   uint32_t flags;          /* LMN_FLAG_*                                  */
-  uint32_t pad;
+  uint32_t grid_blocks;    /* workgroups of the wavefront kernels, 0 = library pick */
   void* stream;            /* hipStream_t, NULL = library stream          */
 } lmn_render_params;
 #define LMN_FLAG_STAGE_TIMING 1u /* time each wavefront stage with HIP events */
+#define LMN_FLAG_NO_GRAPH 2u     /* launch wavefront iterations directly, not as a captured hipGraph */
 
 typedef struct lmn_stats {
   uint64_t paths;          /* W*H*ceil(sqrt(spp))^2 (src/render.rs:1589)  */
@@ -330,6 +331,27 @@ typedef struct lmn_trace_counts {
 } lmn_trace_counts;
 int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
                        float* const* d_hdr_rgb, lmn_trace_counts* counts);
+
+/* ---- autotuner of the wavefront launch -------------------------------------
+ * No reference counterpart (the reference's only knob is its thread count,
+ * src/main.rs:562-567).  Times short probe renders of the batch -- the same
+ * scenes at reduced spp, written into d_hdr_rgb -- over the path-pool size,
+ * workgroup size, LDS short-stack depth and grid size (coordinate descent
+ * from the defaults), stores the best under the workload's signature in the
+ * tuning cache (file $LMN_TUNE_CACHE, else ~/.cache/luminara_amd/tuning.txt)
+ * and returns it.  Wavefront renders that leave a knob at 0 in
+ * lmn_render_params take it from the cache.  Results do not depend on these
+ * knobs: images are bitwise identical under every configuration. */
+typedef struct lmn_tuning {
+  uint32_t pool_paths, block_size, stack_lds, grid_blocks;
+  double probe_ms;    /* probe render time of the chosen configuration  */
+  double default_ms;  /* probe render time of the default configuration */
+  uint64_t probe_paths;
+  uint32_t candidates; /* configurations timed */
+  uint32_t pad;
+} lmn_tuning;
+int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
+                 float* const* d_hdr_rgb, lmn_tuning* out);
 
 /* ---- path-replay adjoint ------------------------------------------------------
  * Gradient of an image loss L with respect to scene parameters, given the

@@ -151,7 +151,7 @@ def load_scene(toml_text: str, base_dir: Optional[str] = None):
 
 
 def make_params(config: RenderConfig, path="wavefront", scene_key=0, intersect_mode=_abi.ISECT_EXACT,
-                pool_paths=0, stack_lds=0, stream=None, flags=0) -> _abi.RenderParams:
+                pool_paths=0, stack_lds=0, stream=None, flags=0, block_size=0, grid_blocks=0) -> _abi.RenderParams:
     p = _abi.RenderParams()
     p.width, p.height = config.width, config.height
     p.spp, p.max_depth = config.samples_per_pixel, config.max_depth
@@ -166,6 +166,8 @@ def make_params(config: RenderConfig, path="wavefront", scene_key=0, intersect_m
     p.scene_key = scene_key
     p.pool_paths = pool_paths
     p.stack_lds = stack_lds
+    p.block_size = block_size
+    p.grid_blocks = grid_blocks
     p.flags = flags
     p.stream = stream
     return p
@@ -243,6 +245,18 @@ def profile_extend(worlds, params_list, out_ptrs) -> dict:
     tc = _abi.TraceCounts()
     check(lib().lmn_profile_extend(scenes, n, params, ptrs, C.byref(tc)))
     return {k: getattr(tc, k) for k, _ in _abi.TraceCounts._fields_}
+
+
+def autotune(worlds, params_list, out_ptrs) -> dict:
+    """lmn_autotune: picks the wavefront launch configuration for this batch
+    (probe renders into out_ptrs), caches it, and returns it."""
+    n = len(worlds)
+    scenes = (C.c_void_p * n)(*[w.device_scene().value for w in worlds])
+    params = (_abi.RenderParams * n)(*params_list)
+    ptrs = (C.c_void_p * n)(*out_ptrs)
+    t = _abi.Tuning()
+    check(lib().lmn_autotune(scenes, n, params, ptrs, C.byref(t)))
+    return {k: getattr(t, k) for k, _ in _abi.Tuning._fields_ if k != "pad"}
 
 
 def rays_to_struct(rays: np.ndarray):

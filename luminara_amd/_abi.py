@@ -91,7 +91,7 @@ class RenderParams(C.Structure):
                 ("has_crop", C.c_uint32), ("crop", C.c_uint32 * 4), ("path", C.c_uint32),
                 ("intersect_mode", C.c_uint32), ("scene_key", C.c_uint32), ("pool_paths", C.c_uint32),
                 ("block_size", C.c_uint32), ("stack_lds", C.c_uint32), ("flags", C.c_uint32),
-                ("pad", C.c_uint32), ("stream", C.c_void_p)]
+                ("grid_blocks", C.c_uint32), ("stream", C.c_void_p)]
 
 FLAG_STAGE_TIMING = 1
 
@@ -115,6 +115,15 @@ class Hit(C.Structure):
 class TraceCounts(C.Structure):
     _fields_ = [("node_fetches", C.c_uint64), ("prim_tests", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("rays", C.c_uint64), ("node_bytes", C.c_uint32), ("prim_bytes", C.c_uint32)]
+
+
+FLAG_NO_GRAPH = 2
+
+
+class Tuning(C.Structure):
+    _fields_ = [("pool_paths", C.c_uint32), ("block_size", C.c_uint32), ("stack_lds", C.c_uint32),
+                ("grid_blocks", C.c_uint32), ("probe_ms", C.c_double), ("default_ms", C.c_double),
+                ("probe_paths", C.c_uint64), ("candidates", C.c_uint32), ("pad", C.c_uint32)]
 
 
 ADJOINT_MAX_DEPTH = 16
@@ -158,6 +167,8 @@ SIGNATURES = {
     "lmn_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "lmn_profile_extend": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(RenderParams),
                                      C.POINTER(C.c_void_p), C.POINTER(TraceCounts)]),
+    "lmn_autotune": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t, C.POINTER(RenderParams), C.POINTER(C.c_void_p),
+                               C.POINTER(Tuning)]),
     "lmn_adjoint": (C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.POINTER(AdjointSpec), C.POINTER(C.c_float),
                               C.POINTER(Grads)]),
     "lmn_adjoint_device": (C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.POINTER(AdjointSpec), C.c_void_p,

@@ -2,9 +2,11 @@
 // upload, render entry points (wavefront FP32 / plain FP64 / plain FP32),
 // the ray-batch parity harness, device information.
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -73,6 +75,7 @@ struct Workspace {
   DevBuf hit, sh_o, sh_d, sh_c, samples, acc, spill, ctr, tc, rays, out, views, parts, part_begin;
   DevBuf adj_wsum, adj_gtex, adj_part, adj_red, adj_dldi, adj_tex_out;  // path-replay adjoint
   uint32_t pool = 0;
+  bool graph_failed[2] = {false, false};  // capture refused once [without, with stage events]: launch directly
   WaveCounters* host_ctr = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -331,6 +334,130 @@ int frame_setup(const lmn_render_params* p, FrameSetup& fs) {
 
 constexpr uint32_t kMaxTreeDepth = 48;
 
+// ---- wavefront launch configuration and its tuning cache ---------------------
+// The knobs: path-pool size (paths in flight per iteration), workgroup size,
+// LDS short-stack depth and grid size of the grid-stride kernels.  Explicit
+// values in lmn_render_params win; otherwise an entry that lmn_autotune stored
+// for the workload's signature; otherwise the defaults.
+
+struct WaveConfig {
+  uint32_t pool, block, stack, grid;
+};
+
+struct TuneCache {
+  std::mutex mu;
+  bool loaded = false;
+  std::vector<std::pair<std::string, WaveConfig>> entries;
+  std::string path() const {
+    if (const char* e = std::getenv("LMN_TUNE_CACHE")) return e;
+    const char* home = std::getenv("HOME");
+    return std::string(home ? home : "/tmp") + "/.cache/luminara_amd/tuning.txt";
+  }
+  void load() {
+    if (loaded) return;
+    loaded = true;
+    std::ifstream f(path());
+    std::string line;
+    while (std::getline(f, line)) {
+      size_t tab = line.find('\t');
+      if (tab == std::string::npos) continue;
+      WaveConfig c{};
+      std::istringstream v(line.substr(tab + 1));
+      if (v >> c.pool >> c.block >> c.stack >> c.grid) entries.push_back({line.substr(0, tab), c});
+    }
+  }
+  bool find(const std::string& key, WaveConfig& c) {
+    std::lock_guard<std::mutex> g(mu);
+    load();
+    for (auto it = entries.rbegin(); it != entries.rend(); ++it)
+      if (it->first == key) {
+        c = it->second;
+        return true;
+      }
+    return false;
+  }
+  void store(const std::string& key, const WaveConfig& c) {
+    std::lock_guard<std::mutex> g(mu);
+    load();
+    entries.push_back({key, c});
+    std::string pth = path();
+    size_t slash = pth.rfind('/');
+    if (slash != std::string::npos) {  // mkdir -p of the cache directory
+      std::string dir = pth.substr(0, slash);
+      for (size_t k = 1; k <= dir.size(); ++k)
+        if (k == dir.size() || dir[k] == '/') (void)mkdir(dir.substr(0, k).c_str(), 0755);
+    }
+    std::ofstream f(pth, std::ios::app);
+    if (f) f << key << '\t' << c.pool << ' ' << c.block << ' ' << c.stack << ' ' << c.grid << '\n';
+  }
+};
+TuneCache& tune_cache() {
+  static TuneCache* c = new TuneCache();
+  return *c;
+}
+
+int log2_bucket(uint64_t v) {
+  int b = 0;
+  while (v > 1) {
+    v >>= 1;
+    ++b;
+  }
+  return b;
+}
+
+// Signature of a wavefront workload: device, feature set, scene count, and
+// power-of-two buckets of the path count, BVH size and depth.
+std::string tune_key(lmn_scene* const* scenes, const lmn_render_params* const* params, const FrameSetup* fs,
+                     size_t n) {
+  hipDeviceProp_t prop{};
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipGetDeviceProperties(&prop, dev);
+  uint64_t total = 0, nodes = 0;
+  uint32_t feat = 0, depth = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (fs[i].black) continue;
+    total += fs[i].F.n_items;
+    nodes += scenes[i]->host.nodesF.size();
+    feat |= scenes[i]->host.featF;
+    depth = std::max(depth, params[i]->max_depth);
+  }
+  std::ostringstream k;
+  k << prop.gcnArchName << "/cu" << prop.multiProcessorCount << "|feat" << feat << "|scenes" << log2_bucket(n)
+    << "|paths" << log2_bucket(total) << "|nodes" << log2_bucket(nodes) << "|depth" << log2_bucket(depth);
+  return k.str();
+}
+
+WaveConfig default_wave_config(uint64_t total) {
+  WaveConfig c;
+  c.pool = uint32_t(std::min<uint64_t>(std::max<uint64_t>(total, 64), 1u << 23));
+  c.block = 256;
+  c.stack = 16;
+  c.grid = 0;
+  return c;
+}
+
+int resolve_wave_config(lmn_scene* const* scenes, const lmn_render_params* const* params, const FrameSetup* fs,
+                        size_t n, uint64_t total, WaveConfig& c) {
+  const lmn_render_params* p0 = params[0];
+  c = default_wave_config(total);
+  if (!p0->pool_paths || !p0->block_size || !p0->stack_lds || !p0->grid_blocks) {
+    WaveConfig t;
+    if (tune_cache().find(tune_key(scenes, params, fs, n), t)) c = t;
+  }
+  if (p0->pool_paths) c.pool = p0->pool_paths;
+  if (p0->block_size) c.block = p0->block_size;
+  if (p0->stack_lds) c.stack = p0->stack_lds;
+  if (p0->grid_blocks) c.grid = p0->grid_blocks;
+  c.pool = std::max<uint32_t>(c.pool, 64);
+  if (c.block != 128 && c.block != 256) return set_err(LMN_ERR_INVALID, "block_size must be 128 or 256");
+  if (c.stack != 16 && c.stack != 32) return set_err(LMN_ERR_INVALID, "stack_lds must be 16 or 32");
+  uint32_t need = (c.pool + c.block - 1) / c.block;
+  if (!c.grid) c.grid = std::min<uint32_t>(need, (1u << 19) / c.block);  // 2048 workgroups of 256
+  c.grid = std::max<uint32_t>(1, std::min(c.grid, need));
+  return LMN_OK;
+}
+
 // Wavefront over a batch of scenes: every scene's work items are issued from
 // one list (scene by scene, sample pass by sample pass), so one set of
 // kernel launches per iteration serves the whole batch.  Items are grouped in
@@ -348,10 +475,10 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
     feat |= scenes[i]->host.featF;
   }
   if (feat & FEAT_MARCH) feat |= FEAT_F64;
-  uint32_t pool = p0->pool_paths ? p0->pool_paths : uint32_t(std::min<uint64_t>(std::max<uint64_t>(total, 64), 1u << 22));
-  pool = std::max<uint32_t>(pool, 64);
-  uint32_t ss = p0->stack_lds ? p0->stack_lds : 16;
-  if (ss != 16 && ss != 32) return set_err(LMN_ERR_INVALID, "stack_lds must be 16 or 32");
+  WaveConfig wc;
+  int rc = resolve_wave_config(scenes, params, fs, n, total, wc);
+  if (rc != LMN_OK) return rc;
+  const uint32_t pool = wc.pool, ss = wc.stack;
   const uint64_t kMaxChunkItems = 1ull << 27;  // 2 GiB of float4 samples
   // parts: whole sample passes of one scene
   std::vector<FrameDev> parts;
@@ -380,8 +507,8 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
     chunk_max = std::max(chunk_max, items);
     b = e;
   }
-  uint32_t grid = std::min<uint32_t>((pool + 255) / 256, 256 * 8);
-  uint32_t nthreads = grid * 256;
+  const uint32_t grid = wc.grid;
+  const uint32_t nthreads = grid * wc.block;
   for (int k = 0; k < 2; ++k)
     for (int f = 0; f < 4; ++f) HIP_TRY(W.st[k][f].ensure(size_t(pool) * 16));
   HIP_TRY(W.hit.ensure(size_t(pool) * 8));
@@ -410,6 +537,7 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
   WaveLaunch L;
   L.stream = st;
   L.grid = grid;
+  L.block = wc.block;
   L.pool = pool;
   L.stack_lds = ss;
   L.feat = feat;
@@ -424,6 +552,7 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
   uint64_t sum_ext = 0, sum_sh = 0, sum_cam = 0, iters = 0;
   const int block = Workspace::kBlock;
   const bool timing = (p0->flags & LMN_FLAG_STAGE_TIMING) != 0;
+  bool use_graph = !(p0->flags & LMN_FLAG_NO_GRAPH) && !W.graph_failed[timing ? 1 : 0];
   double t_stage[4] = {0, 0, 0, 0};
   uint32_t n_launch = 0;
   // parts of a chunk: uploaded in one copy, chunk after chunk
@@ -435,12 +564,43 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
     HIP_TRY(hipMemcpyAsync(W.part_begin.p, begins.data(), begins.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
     BatchDev B{W.views.as<SceneView<float>>(), W.parts.as<FrameDev>(), W.part_begin.as<uint64_t>(), uint32_t(pe - pb), 0};
     launch_counters_init(L.ctr, begins.back(), st);
+    // The block of iterations is captured once per chunk as a hipGraph and
+    // replayed until the pool drains: every kernel reads its work counts from
+    // device memory, so the same graph serves every block (an even number of
+    // iterations brings the double-buffered pools back to the same roles).
+    hipGraphExec_t gexec = nullptr;
+    if (use_graph) {
+      hipGraph_t graph = nullptr;
+      bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess;
+      if (ok) {
+        for (int k = 0; k < block; ++k)
+          launch_wave_iteration(L, B, P[k & 1], P[(k & 1) ^ 1], timing ? W.stage_ev[k] : nullptr);
+        ok = hipStreamEndCapture(st, &graph) == hipSuccess && graph;
+      }
+      if (ok) ok = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) == hipSuccess;
+      if (graph) (void)hipGraphDestroy(graph);
+      if (!ok) {  // capture refused (e.g. event nodes): launch directly from now on
+        (void)hipGetLastError();
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        use_graph = false;
+        W.graph_failed[timing ? 1 : 0] = true;
+      }
+    }
     int cur = 0;
     auto t_start = std::chrono::steady_clock::now();
     for (;;) {
-      for (int k = 0; k < block; ++k) {
-        launch_wave_iteration(L, B, P[cur], P[cur ^ 1], timing ? W.stage_ev[k] : nullptr);
-        cur ^= 1;
+      if (gexec) {
+        hipError_t e = hipGraphLaunch(gexec, st);
+        if (e != hipSuccess) {
+          (void)hipGraphExecDestroy(gexec);
+          return set_err(LMN_ERR_DEVICE, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+        }
+      } else {
+        for (int k = 0; k < block; ++k) {
+          launch_wave_iteration(L, B, P[cur], P[cur ^ 1], timing ? W.stage_ev[k] : nullptr);
+          cur ^= 1;
+        }
       }
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync(W.host_ctr, L.ctr, sizeof(WaveCounters), hipMemcpyDeviceToHost, st));
@@ -462,9 +622,12 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
         iters += c.iterations;
         break;
       }
-      if (std::chrono::steady_clock::now() - t_start > std::chrono::seconds(600))
+      if (std::chrono::steady_clock::now() - t_start > std::chrono::seconds(600)) {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         return set_err(LMN_ERR_DEVICE, "wavefront did not drain within 600 s");
+      }
     }
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     for (size_t q = pb; q < pe; ++q) {
       const FrameDev& F = parts[q];
       uint32_t npix = fs[F.scene].npix;
@@ -704,6 +867,116 @@ int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lm
   }
   (void)hipFree(d_tc);
   return rc;
+}
+
+// ---- autotuner ------------------------------------------------------------------
+
+int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
+                 float* const* d_hdr_rgb, lmn_tuning* out) {
+  if (!scenes || !params || !d_hdr_rgb || !n_scenes || !out) return set_err(LMN_ERR_INVALID, "lmn_autotune: bad argument");
+  std::vector<lmn_scene*> sc(n_scenes);
+  std::vector<FrameSetup> fs(n_scenes);
+  std::vector<const lmn_render_params*> orig(n_scenes);
+  uint64_t total = 0;
+  for (size_t i = 0; i < n_scenes; ++i) {
+    if (!scenes[i]) return set_err(LMN_ERR_INVALID, "lmn_autotune: null scene");
+    if (params[i].path != LMN_PATH_WAVEFRONT_FP32)
+      return set_err(LMN_ERR_INVALID, "lmn_autotune: tunes the wavefront path only");
+    sc[i] = const_cast<lmn_scene*>(scenes[i]);
+    orig[i] = &params[i];
+    int rc = frame_setup(&params[i], fs[i]);
+    if (rc != LMN_OK) return rc;
+    if (!fs[i].black) total += fs[i].F.n_items;
+  }
+  if (total == 0) return set_err(LMN_ERR_INVALID, "lmn_autotune: nothing to render");
+  const std::string key = tune_key(sc.data(), orig.data(), fs.data(), n_scenes);
+  // probe: the same frames at reduced spp, about 2^26 paths in all
+  const double scale = std::min(1.0, double(1ull << 26) / double(total));
+  std::vector<lmn_render_params> pp(params, params + n_scenes);
+  uint64_t probe_paths = 0;
+  for (size_t i = 0; i < n_scenes; ++i) {
+    pp[i].flags &= ~LMN_FLAG_STAGE_TIMING;
+    if (fs[i].black) continue;
+    uint32_t sq = fs[i].F.sqrt_spp;
+    uint32_t k = std::max<uint32_t>(1, uint32_t(std::floor(double(sq) * std::sqrt(scale))));
+    pp[i].spp = k * k;
+    probe_paths += uint64_t(fs[i].npix) * k * k;
+  }
+  std::vector<const lmn_render_params*> ppp(n_scenes);
+  for (size_t i = 0; i < n_scenes; ++i) ppp[i] = &pp[i];
+  uint32_t n_timed = 0;
+  auto timed = [&](const WaveConfig& c, double& ms) -> int {
+    for (auto& q : pp) {
+      q.pool_paths = c.pool;
+      q.block_size = c.block;
+      q.stack_lds = c.stack;
+      q.grid_blocks = c.grid;
+    }
+    ms = 1e30;
+    for (int rep = 0; rep < 2; ++rep) {  // the first run warms caches and allocations
+      lmn_stats st{};
+      int rc = render_common(sc.data(), ppp.data(), d_hdr_rgb, n_scenes, &st, nullptr);
+      if (rc != LMN_OK) return rc;
+      if (rep) ms = st.render_ms;
+    }
+    ++n_timed;
+    return LMN_OK;
+  };
+  auto with_grid = [](WaveConfig c, double f) {
+    uint32_t need = (c.pool + c.block - 1) / c.block;
+    uint32_t g = std::min<uint32_t>(need, (1u << 19) / c.block);
+    c.grid = std::max<uint32_t>(1, std::min<uint32_t>(need, uint32_t(double(g) * f)));
+    return c;
+  };
+  WaveConfig best = with_grid(default_wave_config(total), 1.0);
+  double best_ms = 0.0;
+  int rc = timed(best, best_ms);
+  if (rc != LMN_OK) return rc;
+  const double default_ms = best_ms;
+  auto consider = [&](const WaveConfig& c) -> int {
+    double ms;
+    int r = timed(c, ms);
+    if (r != LMN_OK) return r;
+    if (ms < best_ms * 0.99) {  // keep the incumbent unless clearly faster
+      best = c;
+      best_ms = ms;
+    }
+    return LMN_OK;
+  };
+  // coordinate descent: pool size, workgroup size, stack depth, grid size
+  for (uint32_t pool : {1u << 21, 1u << 22, 1u << 24}) {
+    WaveConfig c = best;
+    c.pool = uint32_t(std::min<uint64_t>(pool, std::max<uint64_t>(total, 64)));
+    if (c.pool == best.pool) continue;
+    if ((rc = consider(with_grid(c, 1.0))) != LMN_OK) return rc;
+  }
+  {
+    WaveConfig c = best;
+    c.block = best.block == 256 ? 128 : 256;
+    if ((rc = consider(with_grid(c, 1.0))) != LMN_OK) return rc;
+  }
+  {
+    WaveConfig c = best;
+    c.stack = best.stack == 16 ? 32 : 16;
+    if ((rc = consider(c)) != LMN_OK) return rc;
+  }
+  for (double f : {0.5, 2.0}) {
+    WaveConfig c = with_grid(best, 1.0);
+    c = with_grid(c, f);
+    if (c.grid == best.grid) continue;
+    if ((rc = consider(c)) != LMN_OK) return rc;
+  }
+  tune_cache().store(key, best);
+  out->pool_paths = best.pool;
+  out->block_size = best.block;
+  out->stack_lds = best.stack;
+  out->grid_blocks = best.grid;
+  out->probe_ms = best_ms;
+  out->default_ms = default_ms;
+  out->probe_paths = probe_paths;
+  out->candidates = n_timed;
+  out->pad = 0;
+  return LMN_OK;
 }
 
 // ---- path-replay adjoint ------------------------------------------------------

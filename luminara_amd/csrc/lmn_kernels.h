@@ -51,7 +51,8 @@ struct BatchDev {
 
 struct WaveLaunch {
   hipStream_t stream;
-  uint32_t grid;       // workgroups of 256 lanes (grid-stride kernels)
+  uint32_t grid;       // workgroups of the grid-stride kernels
+  uint32_t block;      // lanes per workgroup: 128 or 256
   uint32_t pool;       // path pool size
   uint32_t stack_lds;  // LDS short-stack entries per lane
   uint32_t feat;       // FEAT_* specialisation of the scene

@@ -384,9 +384,8 @@ __global__ void k_finalize(const double4* __restrict__ acc, float* __restrict__ 
 // ---------------------------------------------------------------------------
 // Launchers
 
-template <int SS, uint32_t FEAT>
+template <int SS, int BS, uint32_t FEAT>
 static void launch_stage_ss(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow) {
-  constexpr int BS = 256;
   constexpr uint32_t MAIN = FEAT & ~FEAT_MARCH;
   constexpr bool MARCH = (FEAT & FEAT_MARCH) != 0;
   if (shadow) {
@@ -407,10 +406,17 @@ static void launch_stage_ss(const WaveLaunch& L, const BatchDev& B, const PathSo
 }
 template <uint32_t FEAT>
 static void launch_stage(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow) {
-  if (L.stack_lds == 32)
-    launch_stage_ss<32, FEAT>(L, B, cur, shadow);
-  else
-    launch_stage_ss<16, FEAT>(L, B, cur, shadow);
+  if (L.block == 128) {
+    if (L.stack_lds == 32)
+      launch_stage_ss<32, 128, FEAT>(L, B, cur, shadow);
+    else
+      launch_stage_ss<16, 128, FEAT>(L, B, cur, shadow);
+  } else {
+    if (L.stack_lds == 32)
+      launch_stage_ss<32, 256, FEAT>(L, B, cur, shadow);
+    else
+      launch_stage_ss<16, 256, FEAT>(L, B, cur, shadow);
+  }
 }
 // The FP32 path's feature sets: marched primitives are FP64 records too, so
 // FEAT_MARCH always comes with FEAT_F64.
@@ -429,17 +435,22 @@ static void launch_trace_stage(const WaveLaunch& L, const BatchDev& B, const Pat
   LMN_FEAT32_SWITCH(L.feat, CALL)
 #undef CALL
 }
-static void launch_shade(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt) {
-  constexpr int BS = 256;
+template <int BS>
+static void launch_shade_bs(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt) {
 #define CALL(f) \
   hipLaunchKernelGGL((k_shade<BS, f>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, nxt, L.sh, L.samples, L.ctr)
   LMN_FEAT32_SWITCH(L.feat, CALL)
 #undef CALL
 }
+static void launch_shade(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt) {
+  if (L.block == 128)
+    launch_shade_bs<128>(L, B, cur, nxt);
+  else
+    launch_shade_bs<256>(L, B, cur, nxt);
+}
 
 void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt,
                            hipEvent_t* ev) {
-  constexpr int BS = 256;
   if (ev) (void)hipEventRecord(ev[0], L.stream);
   launch_trace_stage(L, B, cur, false);
   if (ev) (void)hipEventRecord(ev[1], L.stream);
@@ -447,7 +458,10 @@ void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA
   if (ev) (void)hipEventRecord(ev[2], L.stream);
   launch_trace_stage(L, B, cur, true);
   if (ev) (void)hipEventRecord(ev[3], L.stream);
-  hipLaunchKernelGGL((k_regen<BS>), dim3(L.grid), dim3(BS), 0, L.stream, B, nxt, L.samples, L.ctr, L.pool);
+  if (L.block == 128)
+    hipLaunchKernelGGL((k_regen<128>), dim3(L.grid), dim3(128), 0, L.stream, B, nxt, L.samples, L.ctr, L.pool);
+  else
+    hipLaunchKernelGGL((k_regen<256>), dim3(L.grid), dim3(256), 0, L.stream, B, nxt, L.samples, L.ctr, L.pool);
   hipLaunchKernelGGL(k_book, dim3(1), dim3(64), 0, L.stream, L.ctr, L.pool);
   if (ev) (void)hipEventRecord(ev[4], L.stream);
 }

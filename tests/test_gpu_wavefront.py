@@ -1,0 +1,88 @@
+"""Launch machinery of the batched wavefront: hipGraph-captured iteration
+blocks, the launch-configuration knobs (path pool, workgroup size, LDS stack
+depth, grid) and the autotuner that picks them.  None of these may change a
+single bit of the image."""
+import os
+
+import numpy as np
+import pytest
+
+import luminara_amd as lm
+from luminara_amd import _abi, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch(n=4, spp=4):
+    cfgs, worlds = [], []
+    for i in range(n):
+        toml, add_mesh, info = workloads.batch_scene(i, seed=11, max_res=64)
+        cfg, _, world = lm.load_scene(toml)
+        v, f = workloads.displaced_sphere(6 + i, [0.0, 1.2, 0.0], 1.0, 200 + i)
+        world.add_triangles(v, f, material=world.add_material(workloads._microfacet([0.6, 0.5, 0.4], 0.3)))
+        cfg.samples_per_pixel = spp
+        cfgs.append(cfg)
+        worlds.append(world)
+    return cfgs, worlds
+
+
+def render_with(cfgs, worlds, **kw):
+    outs, st = lm.render_batch(cfgs, worlds, **kw)
+    return outs, st
+
+
+def test_graph_and_direct_launches_agree(gpu):
+    cfgs, worlds = small_batch()
+    a, sa = render_with(cfgs, worlds)
+    b, sb = render_with(cfgs, worlds, flags=_abi.FLAG_NO_GRAPH)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert sa["shadow_rays"] == sb["shadow_rays"] and sa["iterations"] == sb["iterations"]
+
+
+@pytest.mark.parametrize("knobs", [
+    dict(pool_paths=4096, block_size=128, stack_lds=16),
+    dict(pool_paths=65536, block_size=256, stack_lds=32),
+    dict(pool_paths=20000, block_size=128, stack_lds=32, grid_blocks=7),
+    dict(pool_paths=1 << 20, block_size=256, stack_lds=16, grid_blocks=4096),
+])
+def test_launch_configuration_does_not_change_the_image(gpu, knobs):
+    cfgs, worlds = small_batch()
+    ref, sref = render_with(cfgs, worlds)
+    out, st = render_with(cfgs, worlds, **knobs)
+    for x, y in zip(ref, out):
+        assert np.array_equal(x, y)
+    assert st["extension_rays"] == sref["extension_rays"]
+
+
+def test_stage_timing_with_graphs(gpu):
+    cfgs, worlds = small_batch()
+    _, st = render_with(cfgs, worlds, flags=_abi.FLAG_STAGE_TIMING)
+    assert st["extend_launches"] > 0
+    assert st["extend_ms"] > 0 and st["shade_ms"] > 0
+    assert st["extend_ms"] + st["shade_ms"] + st["shadow_ms"] + st["regen_ms"] <= st["render_ms"] * 1.05
+
+
+def test_autotune_picks_and_caches_a_configuration(gpu, tmp_path):
+    os.environ["LMN_TUNE_CACHE"] = str(tmp_path / "tuning.txt")
+    try:
+        import torch
+        cfgs, worlds = small_batch(spp=16)
+        outs = [torch.empty(lm.output_shape(c) + (3,), dtype=torch.float32, device="cuda") for c in cfgs]
+        params = [lm.make_params(c, path="wavefront", scene_key=i) for i, c in enumerate(cfgs)]
+        t = lm.autotune(worlds, params, [o.data_ptr() for o in outs])
+        assert t["block_size"] in (128, 256) and t["stack_lds"] in (16, 32)
+        assert t["pool_paths"] >= 64 and t["grid_blocks"] >= 1 and t["candidates"] >= 4
+        assert t["probe_ms"] <= t["default_ms"]
+        line = (tmp_path / "tuning.txt").read_text().strip().splitlines()[-1]
+        key, vals = line.split("\t")
+        assert [int(v) for v in vals.split()] == [t["pool_paths"], t["block_size"], t["stack_lds"],
+                                                 t["grid_blocks"]]
+        # renders that leave the knobs at 0 now use the cached configuration,
+        # which must give the same image as the defaults
+        a, _ = render_with(cfgs, worlds)
+        b, _ = render_with(cfgs, worlds, pool_paths=1 << 16, block_size=256, stack_lds=16)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    finally:
+        del os.environ["LMN_TUNE_CACHE"]
