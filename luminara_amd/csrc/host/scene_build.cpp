@@ -2,6 +2,7 @@
 #include "scene_build.h"
 
 #include <algorithm>
+#include <functional>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -341,6 +342,152 @@ struct Builder {
   }
 };
 
+// Collapses the FP32 binary BVH (sibling-pair layout) into a 4-wide one: a
+// 4-wide node starts from a binary node's two children and keeps opening the
+// child of largest surface area until it holds four, the usual top-down
+// collapse of a binary SAH tree.  Leaves keep their primitive ranges; a leaf
+// with more than LEAF4_MAX primitives (possible only at the depth cap) is
+// split into entries of LEAF4_MAX under extra nodes with the leaf's box.
+// Returns the worst-case traversal stack in 32-bit words: along any
+// root-to-leaf path, every node can push all but one of its children, two
+// words each.
+uint32_t collapse_bvh4(const std::vector<NodeF>& n2, std::vector<Node4F>& n4) {
+  n4.clear();
+  if (n2.empty()) return 0;
+  auto area = [&](uint32_t i) {
+    const NodeF& n = n2[i];
+    double dx = double(n.hi[0]) - n.lo[0], dy = double(n.hi[1]) - n.lo[1], dz = double(n.hi[2]) - n.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+  };
+  struct Entry {
+    float lo[3], hi[3];
+    uint32_t node2;   // binary node, or ~0u for a leaf chunk
+    uint32_t first, count;
+  };
+  uint32_t worst = 0;
+  // returns the index of the new 4-wide node; `pushed` = stack words above it
+  std::function<uint32_t(std::vector<Entry>, uint32_t)> make = [&](std::vector<Entry> cand, uint32_t pushed) {
+    uint32_t id = uint32_t(n4.size());
+    n4.emplace_back();
+    Node4F node{};
+    node.n = uint32_t(cand.size());
+    uint32_t here = pushed + 2 * (uint32_t(cand.size()) - 1);
+    worst = std::max(worst, here);
+    for (size_t k = 0; k < cand.size(); ++k) {
+      const Entry& e = cand[k];
+      node.lox[k] = e.lo[0];
+      node.loy[k] = e.lo[1];
+      node.loz[k] = e.lo[2];
+      node.hix[k] = e.hi[0];
+      node.hiy[k] = e.hi[1];
+      node.hiz[k] = e.hi[2];
+    }
+    for (size_t k = 0; k < cand.size(); ++k) {
+      const Entry& e = cand[k];
+      if (e.node2 == ~0u || (n2[e.node2].b & LEAF_BIT)) {
+        uint32_t first = e.node2 == ~0u ? e.first : n2[e.node2].a;
+        uint32_t count = e.node2 == ~0u ? e.count : (n2[e.node2].b & ~LEAF_BIT);
+        if (count <= LEAF4_MAX) {
+          node.child[k] = LEAF4 | ((count - 1) << 28) | first;
+          continue;
+        }
+        std::vector<Entry> parts;  // oversized leaf: chunks under extra nodes
+        uint32_t per = (count + 3) / 4;
+        for (uint32_t c = 0; c < count; c += per) {
+          Entry p = e;
+          p.node2 = ~0u;
+          p.first = first + c;
+          p.count = std::min(per, count - c);
+          parts.push_back(p);
+        }
+        node.child[k] = make(parts, here);
+        continue;
+      }
+      // internal: open the largest-area internal entry until four
+      std::vector<Entry> sub;
+      uint32_t c = n2[e.node2].a;
+      for (uint32_t j : {c, c + 1}) {
+        Entry x{};
+        for (int a = 0; a < 3; ++a) {
+          x.lo[a] = n2[j].lo[a];
+          x.hi[a] = n2[j].hi[a];
+        }
+        x.node2 = j;
+        sub.push_back(x);
+      }
+      while (sub.size() < 4) {
+        int bi = -1;
+        double ba = -1.0;
+        for (size_t q = 0; q < sub.size(); ++q)
+          if (sub[q].node2 != ~0u && !(n2[sub[q].node2].b & LEAF_BIT) && area(sub[q].node2) > ba) {
+            ba = area(sub[q].node2);
+            bi = int(q);
+          }
+        if (bi < 0) break;
+        uint32_t open = sub[bi].node2, oc = n2[open].a;
+        sub.erase(sub.begin() + bi);
+        for (uint32_t j : {oc, oc + 1}) {
+          Entry x{};
+          for (int a = 0; a < 3; ++a) {
+            x.lo[a] = n2[j].lo[a];
+            x.hi[a] = n2[j].hi[a];
+          }
+          x.node2 = j;
+          sub.push_back(x);
+        }
+      }
+      node.child[k] = make(sub, here);
+    }
+    n4[id] = node;
+    return id;
+  };
+  Entry root{};
+  for (int a = 0; a < 3; ++a) {
+    root.lo[a] = n2[0].lo[a];
+    root.hi[a] = n2[0].hi[a];
+  }
+  root.node2 = 0;
+  if (n2[0].b & LEAF_BIT) {
+    make({root}, 0);
+  } else {
+    // the root's own box is not stored: start from its opened children
+    std::vector<Entry> top;
+    uint32_t c = n2[0].a;
+    for (uint32_t j : {c, c + 1}) {
+      Entry x{};
+      for (int a = 0; a < 3; ++a) {
+        x.lo[a] = n2[j].lo[a];
+        x.hi[a] = n2[j].hi[a];
+      }
+      x.node2 = j;
+      top.push_back(x);
+    }
+    while (top.size() < 4) {
+      int bi = -1;
+      double ba = -1.0;
+      for (size_t q = 0; q < top.size(); ++q)
+        if (!(n2[top[q].node2].b & LEAF_BIT) && area(top[q].node2) > ba) {
+          ba = area(top[q].node2);
+          bi = int(q);
+        }
+      if (bi < 0) break;
+      uint32_t open = top[bi].node2, oc = n2[open].a;
+      top.erase(top.begin() + bi);
+      for (uint32_t j : {oc, oc + 1}) {
+        Entry x{};
+        for (int a = 0; a < 3; ++a) {
+          x.lo[a] = n2[j].lo[a];
+          x.hi[a] = n2[j].hi[a];
+        }
+        x.node2 = j;
+        top.push_back(x);
+      }
+    }
+    make(top, 0);
+  }
+  return worst;
+}
+
 float round_down(double x) {
   float f = float(x);
   if (double(f) > x) f = std::nextafter(f, -INFINITY);
@@ -578,6 +725,11 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
   out.n_leaves = 0;
   build_tree(bounded, 0, out.nodesD, out.nodesF);
   build_tree(marched, uint32_t(bounded.size()), out.mnodesD, out.mnodesF);
+  if (bounded.size() + marched.size() > LEAF4_FIRST) {
+    err = "too many primitives for the 4-wide BVH";
+    return LMN_ERR_INVALID;
+  }
+  out.stack4_words = collapse_bvh4(out.nodesF, out.nodes4F);
   out.n_bounded = uint32_t(bounded.size() + marched.size());
   out.n_planes = uint32_t(planes.size());
   for (uint32_t p : planes) out.order.push_back(p);

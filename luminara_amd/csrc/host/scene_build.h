@@ -19,6 +19,8 @@ struct BvhOptions {
 };
 
 struct HostScene {
+  std::vector<Node4F> nodes4F;  // main BVH collapsed to 4-wide (FP32 path)
+  uint32_t stack4_words = 0;      // worst-case traversal stack of nodes4F, 32-bit words
   std::vector<NodeF> nodesF;   // main BVH
   std::vector<NodeD> nodesD;
   std::vector<NodeF> mnodesF;  // BVH over the marched kinds

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void k_adjoint(SceneView<R> S, FrameDev F, Adj
     bool skip = false;
     // ---- replay (the plain kernels' loop, recording vertices)
     for (uint32_t depth = F.max_depth; depth > 0; --depth) {
-      PrivateStack st;
+      PrivateStack<R> st;
       R t;
       uint32_t idx;
       if (!trace<R, FEAT, false>(S, o, d, R(0.001), R(INFINITY), t, idx, st)) {
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_adjoint(SceneView<R> S, FrameDev F, Adj
         uint32_t li;
         if (nee_sample(S, sf.p, sf.n, att, rng, ld, dist, c, &geo, &li)) {
           nee_nan = has_nan(c);
-          PrivateStack st2;
+          PrivateStack<R> st2;
           R t2;
           uint32_t i2;
           if (!trace<R, FEAT, true>(S, sf.p, ld, R(0.001), dist - R(0.001), t2, i2, st2)) {

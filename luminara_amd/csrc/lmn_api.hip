@@ -102,7 +102,7 @@ Workspace& workspace() {
 
 struct lmn_scene {
   HostScene host;
-  DevBuf nodesF, nodesD, mnodesF, mnodesD, primsF, primsD, xfF, xfD, matF, matD, tex, texels, lightF, lightD;
+  DevBuf nodes4F, nodesF, nodesD, mnodesF, mnodesD, primsF, primsD, xfF, xfD, matF, matD, tex, texels, lightF, lightD;
   SceneView<float> viewF{};
   SceneView<double> viewD{};
   lmn_camera camera{};
@@ -216,6 +216,9 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   if (rc != LMN_OK) return set_err(rc, err);
   HIP_TRY(hipGetDevice(&s->device));
   HostScene& h = s->host;
+  if (h.stack4_words > uint32_t(kPrivateStackWords32))
+    return set_err(LMN_ERR_UNSUPPORTED, "BVH too deep for the FP32 traversal stack");
+  HIP_TRY(upload(s->nodes4F, h.nodes4F));
   HIP_TRY(upload(s->nodesF, h.nodesF));
   HIP_TRY(upload(s->nodesD, h.nodesD));
   HIP_TRY(upload(s->mnodesF, h.mnodesF));
@@ -230,10 +233,12 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   HIP_TRY(upload(s->texels, h.texels));
   HIP_TRY(upload(s->lightF, h.lightF));
   HIP_TRY(upload(s->lightD, h.lightD));
-  s->device_bytes = s->nodesF.bytes + s->nodesD.bytes + s->mnodesF.bytes + s->mnodesD.bytes + s->primsF.bytes + s->primsD.bytes + s->xfF.bytes +
+  s->device_bytes = s->nodes4F.bytes + s->nodesF.bytes + s->nodesD.bytes + s->mnodesF.bytes + s->mnodesD.bytes + s->primsF.bytes + s->primsD.bytes + s->xfF.bytes +
                     s->xfD.bytes + s->matF.bytes + s->matD.bytes + s->tex.bytes + s->texels.bytes +
                     s->lightF.bytes + s->lightD.bytes;
   SceneView<float>& F = s->viewF;
+  F.nodes4 = s->nodes4F.as<Node4F>();
+  F.n_nodes4 = uint32_t(h.nodes4F.size());
   F.nodes = s->nodesF.as<NodeF>();
   F.mnodes = s->mnodesF.as<NodeF>();
   F.n_mnodes = uint32_t(h.mnodesF.size());
@@ -252,6 +257,8 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   F.bg = h.bgF;
   F.cam = h.camF;
   SceneView<double>& D = s->viewD;
+  D.nodes4 = nullptr;
+  D.n_nodes4 = 0;
   D.nodes = s->nodesD.as<NodeD>();
   D.mnodes = s->mnodesD.as<NodeD>();
   D.n_mnodes = F.n_mnodes;
@@ -516,7 +523,11 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
   HIP_TRY(W.sh_d.ensure(size_t(pool) * 16));
   HIP_TRY(W.sh_c.ensure(size_t(pool) * 16));
   HIP_TRY(W.samples.ensure(size_t(std::max<uint64_t>(chunk_max, 1)) * 16));
-  HIP_TRY(W.spill.ensure(size_t(kMaxTreeDepth) * nthreads * 4));
+  // traversal stack words beyond the LDS part: the 4-wide walk's worst case,
+  // or the binary marched walk's depth cap
+  uint32_t stack_words = kMaxTreeDepth;
+  for (size_t i = 0; i < n; ++i) stack_words = std::max(stack_words, scenes[i]->host.stack4_words);
+  HIP_TRY(W.spill.ensure(size_t(stack_words) * nthreads * 4));
   HIP_TRY(W.ctr.ensure(sizeof(WaveCounters)));
   // per-scene accumulators, scene views, parts
   std::vector<size_t> acc_off(n + 1, 0);
@@ -862,7 +873,7 @@ int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lm
     counts->prim_tests = h.prim_tests;
     counts->tri_tests = h.tri_tests;
     counts->rays = h.rays;
-    counts->node_bytes = sizeof(NodeF);
+    counts->node_bytes = sizeof(Node4F);
     counts->prim_bytes = sizeof(PrimT<float>);
   }
   (void)hipFree(d_tc);

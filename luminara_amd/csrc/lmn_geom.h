@@ -1,10 +1,14 @@
 // Device scene layout and the primitive intersection library.
 //
 // Layout in HBM (one scene):
-//   nodes   : BVH2 in depth-first order, left child = node + 1.  FP32 nodes are
-//             32 bytes {lo.xyz, a, hi.xyz, b}; FP64 nodes 64 bytes.
-//             internal: a = right child, b = split axis; leaf: a = first prim,
-//             b = LEAF_BIT | prim count.
+//   nodes   : binary SAH BVH with sibling pairs: root at 0, index 1 unused,
+//             the two children of a node side by side at an even index.
+//             FP32 nodes are 32 bytes {lo.xyz, a, hi.xyz, b}, FP64 nodes 64.
+//             internal: a = first child, b = split axis; leaf: a = first
+//             prim, b = LEAF_BIT | prim count.  The FP64 path traverses it.
+//   nodes4  : the FP32 path's 4-wide BVH, collapsed from the binary one:
+//             128-byte nodes holding the boxes of up to four children.
+//             Two nodes map onto the same primitive records.
 //   prims   : primitive records reordered into leaf order (64 B FP32 /
 //             128 B FP64), so a leaf's primitives are one contiguous fetch.
 //   planes  : unbounded primitives (infinite planes), tested after the BVH
@@ -54,6 +58,21 @@ struct alignas(16) NodeD {
   uint32_t a, b;
 };
 static_assert(sizeof(NodeD) == 64, "FP64 BVH node must be 64 bytes");
+// FP32 4-wide BVH node: the boxes of up to four children in SoA order (one
+// node is eight 16-byte loads, two 64-byte lines) and the child entries: a
+// node index, or a leaf LEAF4 | (count - 1) << 28 | first primitive record.
+struct alignas(128) Node4F {
+  float lox[4], loy[4], loz[4];
+  float hix[4], hiy[4], hiz[4];
+  uint32_t child[4];
+  uint32_t n;  // children in use, 1..4
+  uint32_t pad[3];
+};
+static_assert(sizeof(Node4F) == 128, "FP32 BVH4 node must be 128 bytes");
+constexpr uint32_t LEAF4 = 0x80000000u;
+constexpr uint32_t LEAF4_FIRST = 0x0fffffffu;  // records addressable by a leaf entry
+constexpr uint32_t LEAF4_MAX = 8;              // primitives per leaf entry
+
 template <class R> struct NodeSel;
 template <> struct NodeSel<float> { using type = NodeF; };
 template <> struct NodeSel<double> { using type = NodeD; };

@@ -24,6 +24,18 @@ using std::pow; using std::sin; using std::sqrt; using std::trunc;
 #endif
 
 
+// Bit casts usable on host and device.
+LMN_HD uint32_t bits_of(float x) {
+  uint32_t u;
+  __builtin_memcpy(&u, &x, 4);
+  return u;
+}
+LMN_HD float float_of(uint32_t u) {
+  float x;
+  __builtin_memcpy(&x, &u, 4);
+  return x;
+}
+
 // Precision-generic math: float and double overloads that map to the
 // single- and double-precision device/libm routines.
 LMN_HD float m_abs(float x) { return fabsf(x); }

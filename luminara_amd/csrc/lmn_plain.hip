@@ -24,7 +24,7 @@ __global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, doubl
     V3<R> L = {R(0), R(0), R(0)}, beta = {R(1), R(1), R(1)};
     bool skip = false;
     for (uint32_t depth = F.max_depth; depth > 0; --depth) {
-      PrivateStack st;
+      PrivateStack<R> st;
       R t;
       uint32_t idx;
       ++n_ext;
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, doubl
         R dist;
         if (nee_sample(S, sf.p, sf.n, att, rng, ld, dist, c)) {
           nee_nan = has_nan(c);
-          PrivateStack st2;
+          PrivateStack<R> st2;
           R t2;
           uint32_t i2;
           ++n_sh;
@@ -101,7 +101,7 @@ __global__ void k_trace(SceneView<R> S, const lmn_ray* __restrict__ rays, lmn_hi
   const lmn_ray& r = rays[i];
   V3<R> o = {R(r.origin[0]), R(r.origin[1]), R(r.origin[2])};
   V3<R> d = {R(r.direction[0]), R(r.direction[1]), R(r.direction[2])};
-  PrivateStack st;
+  PrivateStack<R> st;
   R t;
   uint32_t idx;
   lmn_hit h;

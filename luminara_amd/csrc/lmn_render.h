@@ -47,6 +47,7 @@ struct CameraT {
 
 template <class R>
 struct SceneView {
+  const Node4F* nodes4;                     // main BVH, 4-wide (FP32 path)
   const typename NodeSel<R>::type* nodes;   // main BVH (no marched kinds)
   const typename NodeSel<R>::type* mnodes;  // BVH over the marched kinds
   const PrimT<R>* prims;          // bounded prims (leaf order) then planes
@@ -58,6 +59,7 @@ struct SceneView {
   const float* texels;
   const LightT<R>* lights;
   uint32_t n_nodes, n_mnodes, n_bounded, n_planes, n_lights;
+  uint32_t n_nodes4;
   BackgroundT<R> bg;
   CameraT<R> cam;
   uint32_t exact_marched;  // LMN_ISECT_EXACT for marched prims
@@ -72,14 +74,23 @@ struct TraceCounters {
   unsigned long long rays;
 };
 
-// Stack of node indices in per-thread private memory (plain kernels).
-struct PrivateStack {
-  uint32_t s[64];
+// Traversal stack in per-thread private memory (plain kernels).  The FP64
+// path walks the binary BVH (one word per entry, depth <= 48); the FP32 path
+// walks the 4-wide BVH (a node index and its entry distance per entry, up to
+// three entries per level).  Scene creation checks a scene's worst case
+// against these capacities.
+constexpr int kPrivateStackWords64 = 64;
+constexpr int kPrivateStackWords32 = 192;
+template <int N>
+struct PrivateStackN {
+  uint32_t s[N];
   int sp = 0;
   LMN_HD void push(uint32_t v) { s[sp++] = v; }
   LMN_HD uint32_t pop() { return s[--sp]; }
   LMN_HD bool empty() const { return sp == 0; }
 };
+template <class R>
+using PrivateStack = PrivateStackN<sizeof(R) == 4 ? kPrivateStackWords32 : kPrivateStackWords64>;
 
 template <class R>
 LMN_HD bool slab(const float* lo, const float* hi, V3<R> o, V3<R> inv, R tmin, R tmax, R& tenter);
@@ -239,6 +250,98 @@ LMN_HD void bvh_walk(const SceneView<R>& S, const typename NodeSel<R>::type* nod
   }
 }
 
+// Ordered walk of the FP32 4-wide BVH.  A node's (up to) four child boxes
+// are tested together; the hits are sorted by entry distance, the nearest is
+// visited next and the others pushed farthest first, each with its entry
+// distance so that a pop can drop it once a closer hit is known.
+template <uint32_t FEAT, bool ANY, class Stack>
+LMN_HD void bvh4_walk(const SceneView<float>& S, V3<float> o, V3<float> d, V3<float> inv, float tmin,
+                      float& closest, uint32_t& best, bool& found, Stack& st, uint32_t& nf, uint32_t& pt,
+                      uint32_t& tt, bool counting) {
+  uint32_t ref = 0;
+  for (;;) {
+    if (!(ref & LEAF4)) {
+      const Node4F& nd = S.nodes4[ref];
+      ++nf;
+      float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
+      uint32_t ch[4];
+      for (int k = 0; k < 4; ++k) {
+        lox[k] = nd.lox[k];
+        loy[k] = nd.loy[k];
+        loz[k] = nd.loz[k];
+        hix[k] = nd.hix[k];
+        hiy[k] = nd.hiy[k];
+        hiz[k] = nd.hiz[k];
+        ch[k] = nd.child[k];
+      }
+      uint32_t nc = nd.n;
+      float tk[4];
+      for (int k = 0; k < 4; ++k) {
+        float tx0 = (lox[k] - o.x) * inv.x, tx1 = (hix[k] - o.x) * inv.x;
+        float ty0 = (loy[k] - o.y) * inv.y, ty1 = (hiy[k] - o.y) * inv.y;
+        float tz0 = (loz[k] - o.z) * inv.z, tz1 = (hiz[k] - o.z) * inv.z;
+        float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+        float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), closest));
+        tk[k] = (uint32_t(k) < nc && tn <= tf) ? tn : INFINITY;
+      }
+      // sorting network on (tk, ch): misses (INFINITY) sink to the end
+#define LMN_CSWAP(a, b)                    \
+  if (tk[b] < tk[a]) {                     \
+    float tt_ = tk[a];                     \
+    tk[a] = tk[b];                         \
+    tk[b] = tt_;                           \
+    uint32_t cc_ = ch[a];                  \
+    ch[a] = ch[b];                         \
+    ch[b] = cc_;                           \
+  }
+      LMN_CSWAP(0, 1)
+      LMN_CSWAP(2, 3)
+      LMN_CSWAP(0, 2)
+      LMN_CSWAP(1, 3)
+      LMN_CSWAP(1, 2)
+#undef LMN_CSWAP
+      if (tk[0] != INFINITY) {
+        if (tk[3] != INFINITY) {
+          st.push(ch[3]);
+          st.push(bits_of(tk[3]));
+        }
+        if (tk[2] != INFINITY) {
+          st.push(ch[2]);
+          st.push(bits_of(tk[2]));
+        }
+        if (tk[1] != INFINITY) {
+          st.push(ch[1]);
+          st.push(bits_of(tk[1]));
+        }
+        ref = ch[0];
+        continue;
+      }
+    } else {
+      uint32_t first = ref & LEAF4_FIRST, count = ((ref >> 28) & 7u) + 1u;
+      for (uint32_t k = 0; k < count; ++k) {
+        ++pt;
+        if (counting && (S.prims[first + k].kind_mat & KIND_MASK) == K_TRIANGLE) ++tt;
+        if (test_prim<float, FEAT>(S, first + k, o, d, tmin, closest)) {
+          found = true;
+          best = first + k;
+          if (ANY) return;
+        }
+      }
+    }
+    bool next = false;
+    while (!st.empty()) {
+      float te = float_of(st.pop());
+      uint32_t r = st.pop();
+      if (te <= closest) {
+        ref = r;
+        next = true;
+        break;
+      }
+    }
+    if (!next) return;
+  }
+}
+
 // Which part of the scene a trace covers: everything; the main BVH plus the
 // unbounded planes; or only the marched primitives' BVH (the FP32 wavefront
 // runs that in a kernel of its own so the FP64 marching code does not set
@@ -256,8 +359,13 @@ LMN_HD bool trace_pass(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R& close
   V3<R> inv = {R(1) / d.x, R(1) / d.y, R(1) / d.z};
   uint32_t nf = 0, pt = 0, tt = 0;
   constexpr uint32_t MAIN_FEAT = FEAT & ~FEAT_MARCH;
-  if (PASS != PASS_MARCH && S.n_nodes > 0)
-    bvh_walk<R, MAIN_FEAT, ANY>(S, S.nodes, o, d, inv, tmin, closest, best, found, st, nf, pt, tt, cnt != nullptr);
+  if constexpr (sizeof(R) == 4) {
+    if (PASS != PASS_MARCH && S.n_nodes4 > 0)
+      bvh4_walk<MAIN_FEAT, ANY>(S, o, d, inv, tmin, closest, best, found, st, nf, pt, tt, cnt != nullptr);
+  } else {
+    if (PASS != PASS_MARCH && S.n_nodes > 0)
+      bvh_walk<R, MAIN_FEAT, ANY>(S, S.nodes, o, d, inv, tmin, closest, best, found, st, nf, pt, tt, cnt != nullptr);
+  }
   if (PASS != PASS_MARCH && !(ANY && found)) {
     for (uint32_t k = 0; k < S.n_planes; ++k) {
       uint32_t idx = S.n_bounded + k;

@@ -1,0 +1,91 @@
+"""The FP32 path's traversal code (4-wide BVH, FP64 islands, transforms,
+marched kinds) compiled for the host CPU by tests/native/Makefile and checked
+against the oracle on ray batches with the GPU tests' FP32 tolerances: a CPU
+check of the device traversal logic (the GPU tests repeat it on the MI355X)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import luminara_amd as lm
+from luminara_amd import _abi, workloads
+import oracle
+from test_gpu_parity import compare_hits, random_rays
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NATIVE = os.path.join(HERE, "native")
+
+
+@pytest.fixture(scope="module")
+def harness():
+    subprocess.run(["make", "-s", "-C", NATIVE], check=True)
+    L = C.CDLL(os.path.join(NATIVE, "_build", "libhosttrace.so"))
+    L.host_trace_fp32.restype = C.c_int
+    L.host_trace_fp32.argtypes = [C.POINTER(_abi.SceneDesc), C.POINTER(_abi.Ray), C.c_size_t, C.c_uint32,
+                                  C.POINTER(_abi.Hit), C.POINTER(C.c_uint32)]
+    L.host_trace_error.restype = C.c_char_p
+    return L
+
+
+def host_trace(L, world, rays, exact=True):
+    n = len(rays)
+    r = lm.rays_to_struct(rays)
+    hits = (_abi.Hit * max(n, 1))()
+    sw = C.c_uint32()
+    rc = L.host_trace_fp32(world.desc, r, n, 1 if exact else 0, hits, C.byref(sw))
+    assert rc == 0, L.host_trace_error().decode()
+    return lm.hits_to_numpy(hits, n), sw.value
+
+
+SCENES = {
+    "cornell": (lambda: workloads.cornell_box(32, 32, 4, 8), None, [5, 5, -600], [550, 550, 550]),
+    "spheres": (lambda: workloads.random_spheres(32, 18, 4, 50, grid=11), None, [-12, 0.01, -12], [12, 3, 12]),
+    "mesh": (lambda: workloads.batch_scene(3, seed=5)[0], 12, [-2.5, 0.05, -2.5], [2.5, 3.0, 3.0]),
+}
+
+
+@pytest.mark.parametrize("scene", list(SCENES))
+def test_fp32_traversal_matches_oracle(harness, scene):
+    make, freq, lo, hi = SCENES[scene]
+    cfg, cam, world = lm.load_scene(make())
+    if freq:
+        v, f = workloads.displaced_sphere(freq, [0.0, 1.2, 0.0], 1.0, 77)
+        world.add_triangles(v, f, material=world.add_material(workloads._microfacet([0.7, 0.6, 0.5], 0.3)))
+    rays = random_rays(20000, lo, hi, seed=7)
+    ho = oracle.trace(world, rays)
+    hh, words = host_trace(harness, world, rays)
+    r = compare_hits(rays, ho, hh, fp32=True)
+    print(scene, r, "stack words", words)
+    assert r["hits"] > 2000
+    assert r["bad_t"] == 0
+    assert r["flips"] <= int(5e-4 * r["n"])
+    assert 0 < words <= 192
+
+
+def test_marched_kinds_exact_mode(harness):
+    from test_gpu_mesh_batch import MARCHED
+    cfg, cam, world = lm.load_scene(MARCHED)
+    rng = np.random.default_rng(3)
+    n = 20000
+    o = rng.uniform([-3, 0.2, 3], [3, 1.5, 5], size=(n, 3))
+    tgt = rng.uniform([-1.8, 0.0, -1.0], [1.8, 0.9, 1.0], size=(n, 3))
+    rays = np.concatenate([o, tgt - o, np.full((n, 1), 0.001), np.full((n, 1), np.inf)], axis=1)
+    ho = oracle.trace(world, rays, exact=True)
+    hh, _ = host_trace(harness, world, rays, exact=True)
+    r = compare_hits(rays, ho, hh, fp32=True)
+    print(r)
+    assert r["hits"] > 5000 and r["bad_t"] <= int(2e-4 * n) and r["flips"] <= int(5e-4 * n)
+
+
+def test_coincident_primitives(harness):
+    # 40 identical triangles: coincident centroids force median splits
+    tri = 'material = { type = "lambertian", color = [0.5, 0.5, 0.5] }\n'
+    body = "".join('[[triangle]]\nv0 = [0.0, 0.0, 0.0]\nv1 = [1.0, 0.0, 0.0]\nv2 = [0.0, 1.0, 0.0]\n' + tri
+                   for _ in range(40))
+    _, _, world = lm.load_scene(body)
+    rays = np.array([[0.2, 0.2, -1.0, 0.0, 0.0, 1.0, 0.001, np.inf], [2.0, 2.0, -1.0, 0.0, 0.0, 1.0, 0.001, np.inf]])
+    hh, _ = host_trace(harness, world, rays)
+    assert hh["prim"][0] >= 0 and abs(hh["t"][0] - 1.0) < 1e-6
+    assert hh["prim"][1] == -1
