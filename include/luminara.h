@@ -335,7 +335,8 @@ int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lm
 /* ---- autotuner of the wavefront launch -------------------------------------
  * No reference counterpart (the reference's only knob is its thread count,
  * src/main.rs:562-567).  Times short probe renders of the batch -- the same
- * scenes at reduced spp, written into d_hdr_rgb -- over the path-pool size,
+ * scenes at reduced spp, written into d_hdr_rgb, or into buffers of the
+ * library's own when d_hdr_rgb is NULL -- over the path-pool size,
  * workgroup size, LDS short-stack depth and grid size (coordinate descent
  * from the defaults), stores the best under the workload's signature in the
  * tuning cache (file $LMN_TUNE_CACHE, else ~/.cache/luminara_amd/tuning.txt)

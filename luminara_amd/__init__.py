@@ -247,13 +247,14 @@ def profile_extend(worlds, params_list, out_ptrs) -> dict:
     return {k: getattr(tc, k) for k, _ in _abi.TraceCounts._fields_}
 
 
-def autotune(worlds, params_list, out_ptrs) -> dict:
+def autotune(worlds, params_list, out_ptrs=None) -> dict:
     """lmn_autotune: picks the wavefront launch configuration for this batch
-    (probe renders into out_ptrs), caches it, and returns it."""
+    (probe renders into out_ptrs, or into library buffers when None), caches
+    it, and returns it."""
     n = len(worlds)
     scenes = (C.c_void_p * n)(*[w.device_scene().value for w in worlds])
     params = (_abi.RenderParams * n)(*params_list)
-    ptrs = (C.c_void_p * n)(*out_ptrs)
+    ptrs = (C.c_void_p * n)(*out_ptrs) if out_ptrs is not None else None
     t = _abi.Tuning()
     check(lib().lmn_autotune(scenes, n, params, ptrs, C.byref(t)))
     return {k: getattr(t, k) for k, _ in _abi.Tuning._fields_ if k != "pad"}

@@ -563,7 +563,9 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
   uint64_t sum_ext = 0, sum_sh = 0, sum_cam = 0, iters = 0;
   const int block = Workspace::kBlock;
   const bool timing = (p0->flags & LMN_FLAG_STAGE_TIMING) != 0;
-  bool use_graph = !(p0->flags & LMN_FLAG_NO_GRAPH) && !W.graph_failed[timing ? 1 : 0];
+  // Stage timing records events between the stages; events recorded by graph
+  // nodes cannot be timed, so timed renders launch directly.
+  bool use_graph = !(p0->flags & LMN_FLAG_NO_GRAPH) && !timing && !W.graph_failed[0];
   double t_stage[4] = {0, 0, 0, 0};
   uint32_t n_launch = 0;
   // parts of a chunk: uploaded in one copy, chunk after chunk
@@ -884,7 +886,22 @@ int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lm
 
 int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
                  float* const* d_hdr_rgb, lmn_tuning* out) {
-  if (!scenes || !params || !d_hdr_rgb || !n_scenes || !out) return set_err(LMN_ERR_INVALID, "lmn_autotune: bad argument");
+  if (!scenes || !params || !n_scenes || !out) return set_err(LMN_ERR_INVALID, "lmn_autotune: bad argument");
+  // without caller buffers the probes render into buffers of their own
+  std::vector<DevBuf> own;
+  std::vector<float*> own_ptr;
+  if (!d_hdr_rgb) {
+    own.resize(n_scenes);
+    own_ptr.resize(n_scenes);
+    for (size_t i = 0; i < n_scenes; ++i) {
+      FrameSetup f;
+      int rc = frame_setup(&params[i], f);
+      if (rc != LMN_OK) return rc;
+      HIP_TRY(own[i].ensure(size_t(f.npix) * 12));
+      own_ptr[i] = own[i].as<float>();
+    }
+    d_hdr_rgb = own_ptr.data();
+  }
   std::vector<lmn_scene*> sc(n_scenes);
   std::vector<FrameSetup> fs(n_scenes);
   std::vector<const lmn_render_params*> orig(n_scenes);
@@ -955,7 +972,7 @@ int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_rend
     return LMN_OK;
   };
   // coordinate descent: pool size, workgroup size, stack depth, grid size
-  for (uint32_t pool : {1u << 21, 1u << 22, 1u << 24}) {
+  for (uint32_t pool : {1u << 21, 1u << 22, 1u << 24, 1u << 25}) {
     WaveConfig c = best;
     c.pool = uint32_t(std::min<uint64_t>(pool, std::max<uint64_t>(total, 64)));
     if (c.pool == best.pool) continue;
@@ -971,9 +988,8 @@ int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_rend
     c.stack = best.stack == 16 ? 32 : 16;
     if ((rc = consider(c)) != LMN_OK) return rc;
   }
-  for (double f : {0.5, 2.0}) {
-    WaveConfig c = with_grid(best, 1.0);
-    c = with_grid(c, f);
+  for (double f : {0.5, 2.0, 4.0, 8.0}) {
+    WaveConfig c = with_grid(best, f);
     if (c.grid == best.grid) continue;
     if ((rc = consider(c)) != LMN_OK) return rc;
   }
