@@ -160,6 +160,7 @@ def main():
     ap.add_argument("--stack-lds", type=int, default=0)
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering adjoint leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults or cache)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -198,6 +199,13 @@ def main():
         return [lm.make_params(c, path=path, scene_key=i, pool_paths=args.pool, stack_lds=args.stack_lds,
                                flags=flags) for i, (c, _) in enumerate(scenes)]
 
+    # launch configuration (path pool, workgroup size, LDS stack depth, grid)
+    # chosen by the library's autotuner on probe renders of this batch; the
+    # timed renders below leave those knobs at 0 and take the tuned values
+    tuning = None
+    if not args.pool and not args.stack_lds and not args.no_tune:
+        tuning = lm.autotune(worlds, params("wavefront"), ptrs)
+        log(f"[bench] autotune: {tuning}")
     p32 = params("wavefront", _abi.FLAG_STAGE_TIMING)
     for _ in range(args.warmup):
         lm.render_batch_device(worlds, p32, ptrs)
@@ -222,7 +230,7 @@ def main():
         # counted by an instrumented run of the same traversal on the same
         # work, plus per-ray queue traffic; time from HIP events
         counts = lm.profile_extend(worlds, params("wavefront"), ptrs)
-        io_bytes = 40  # per ray: origin+beta, direction+beta, scene id (32 + 8 B) read, hit 8 B written
+        io_bytes = 40  # per ray: origin+beta, direction+beta (32 B) and scene id read, hit (8 B) written
         alg_bytes = (counts["node_fetches"] * counts["node_bytes"] + counts["prim_tests"] * counts["prim_bytes"]
                      + counts["rays"] * io_bytes)
         bytes_per_launch = alg_bytes / launches
@@ -232,7 +240,8 @@ def main():
                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_extend",
                     "avg_launch_us": avg_launch_s * 1e6, "launches_per_step": launches,
                     "bytes_per_launch": bytes_per_launch,
-                    "bytes_model": "32 B per BVH node fetched + 64 B per primitive record tested + 40 B ray/hit I/O",
+                    "bytes_model": "128 B per 4-wide BVH node fetched + 64 B per primitive record tested "
+                                   "+ 40 B ray/hit I/O",
                     "per_ray": {"node_fetches": counts["node_fetches"] / counts["rays"],
                                 "prim_tests": counts["prim_tests"] / counts["rays"]}}
         log(f"[bench] extend: {counts}, {ext_ms:.2f} ms/step over {launches:.0f} launches, "
@@ -280,7 +289,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": dict({"workload": wl_name, "paths_per_step": paths, "bvh_nodes": nodes,
-                        "max_depth": cfg0.max_depth}, **desc),
+                        "max_depth": cfg0.max_depth, "launch": tuning}, **desc),
         "roofline": roofline, "cpu_baseline": cpu, "adjoint": adj,
         "mrays_per_s": (stats[-1]["camera_rays"] + stats[-1]["extension_rays"] + stats[-1]["shadow_rays"])
                        / (ms_per_step / 1e3) / 1e6,

@@ -49,6 +49,15 @@ struct BatchDev {
   uint32_t pad;
 };
 
+// Closest hit of a path, reconstructed by the extend stage while the hit
+// primitive's record is still in cache: a = (unit normal facing the ray,
+// true t), b = (u, v, material | front << 31, reported t).  b.z = ~0u: miss.
+struct HitSoA {
+  float4* a;
+  float4* b;
+};
+constexpr uint32_t HIT_MISS = 0xffffffffu;
+
 struct WaveLaunch {
   hipStream_t stream;
   uint32_t grid;       // workgroups of the grid-stride kernels
@@ -56,7 +65,7 @@ struct WaveLaunch {
   uint32_t pool;       // path pool size
   uint32_t stack_lds;  // LDS short-stack entries per lane
   uint32_t feat;       // FEAT_* specialisation of the scene
-  float2* hit;         // per path: t, primitive record index (bits)
+  HitSoA hit;          // per path: the closest hit's surface (extend -> shade)
   ShadowSoA sh;
   float4* samples;
   WaveCounters* ctr;

@@ -67,6 +67,19 @@ __device__ __forceinline__ void for_scene(uint32_t sid, F&& body) {
   }
 }
 
+// Surface record of a closest hit (see HitSoA).
+template <uint32_t FEAT>
+__device__ __forceinline__ void store_hit(const SceneView<float>& S, HitSoA hit, uint32_t i, V3<float> o, V3<float> d,
+                                          float t, uint32_t idx) {
+  Surf<float> s;
+  surface<float, FEAT>(S, idx, o, d, t, s);
+  hit.a[i] = make_float4(s.n.x, s.n.y, s.n.z, s.t_true);
+  hit.b[i] = make_float4(s.u, s.v, __uint_as_float(s.mat | (s.front ? 0x80000000u : 0u)), s.t);
+}
+__device__ __forceinline__ void store_miss(HitSoA hit, uint32_t i) {
+  hit.b[i] = make_float4(0.f, 0.f, __uint_as_float(HIT_MISS), INFINITY);
+}
+
 __device__ __forceinline__ void add_sample(float4* samples, uint32_t item, V3<float> c) {
   float4 s = samples[item];
   s.x += c.x;
@@ -80,7 +93,7 @@ __device__ __forceinline__ void add_sample(float4* samples, uint32_t item, V3<fl
 
 // Extend, main pass: closest hit over the main BVH and the planes.
 template <int SS, int BS, uint32_t FEAT, bool COUNT>
-__device__ __forceinline__ void extend_body(const BatchDev& B, const PathSoA& cur, float2* hit,
+__device__ __forceinline__ void extend_body(const BatchDev& B, const PathSoA& cur, HitSoA hit,
                                             const WaveCounters* ctr, uint32_t* spill, TraceCounters* tc) {
   __shared__ uint32_t lds[SS * BS];
   uint32_t n = ctr->active;
@@ -95,19 +108,22 @@ __device__ __forceinline__ void extend_body(const BatchDev& B, const PathSoA& cu
       float t = INFINITY;
       uint32_t idx = 0;
       bool h = trace_pass<float, FEAT, false, PASS_MAIN>(S, f3(a), f3(b), 0.001f, t, idx, st, COUNT ? tc : nullptr);
-      hit[i] = make_float2(t, __uint_as_float(h ? idx : 0xffffffffu));
+      if (h)
+        store_hit<FEAT>(S, hit, i, f3(a), f3(b), t, idx);
+      else
+        store_miss(hit, i);
     });
   }
 }
 template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_extend(BatchDev B, PathSoA cur, float2* __restrict__ hit,
+__global__ __launch_bounds__(BS) void k_extend(BatchDev B, PathSoA cur, HitSoA hit,
                                                const WaveCounters* __restrict__ ctr, uint32_t* spill) {
   extend_body<SS, BS, FEAT, false>(B, cur, hit, ctr, spill, nullptr);
 }
 // Instrumented extend (counts node fetches and primitive tests): a kernel of
 // its own name so that profiles keep it apart from the timed k_extend.
 template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_extend_count(BatchDev B, PathSoA cur, float2* __restrict__ hit,
+__global__ __launch_bounds__(BS) void k_extend_count(BatchDev B, PathSoA cur, HitSoA hit,
                                                      const WaveCounters* __restrict__ ctr, uint32_t* spill,
                                                      TraceCounters* tc) {
   extend_body<SS, BS, FEAT, true>(B, cur, hit, ctr, spill, tc);
@@ -116,7 +132,7 @@ __global__ __launch_bounds__(BS) void k_extend_count(BatchDev B, PathSoA cur, fl
 // Extend, marched pass: the FP64 torus/superellipsoid/spring BVH, limited
 // to the main pass's hit distance.
 template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_extend_march(BatchDev B, PathSoA cur, float2* __restrict__ hit,
+__global__ __launch_bounds__(BS) void k_extend_march(BatchDev B, PathSoA cur, HitSoA hit,
                                                      const WaveCounters* __restrict__ ctr, uint32_t* spill) {
   __shared__ uint32_t lds[SS * BS];
   uint32_t n = ctr->active;
@@ -128,12 +144,12 @@ __global__ __launch_bounds__(BS) void k_extend_march(BatchDev B, PathSoA cur, fl
     for_scene(sid, [&](uint32_t s) {
       const SceneView<float> S = B.views[s];
       if (S.n_mnodes == 0) return;
-      float2 h0 = hit[i];
+      float4 h0 = hit.b[i];
       LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
-      float t = __float_as_uint(h0.y) == 0xffffffffu ? INFINITY : h0.x;
+      float t = __float_as_uint(h0.z) == HIT_MISS ? INFINITY : h0.w;
       uint32_t idx = 0;
       if (trace_pass<float, FEAT, false, PASS_MARCH>(S, f3(a), f3(b), 0.001f, t, idx, st))
-        hit[i] = make_float2(t, __uint_as_float(idx));
+        store_hit<FEAT>(S, hit, i, f3(a), f3(b), t, idx);
     });
   }
 }
@@ -195,7 +211,7 @@ __global__ __launch_bounds__(BS) void k_shadow_march(BatchDev B, ShadowSoA sh, f
 // path -- emission, scatter, Russian roulette, NEE light sample -- then
 // wave-ballot compaction of the shadow queue and of the surviving paths.
 template <int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_shade(BatchDev B, PathSoA cur, const float2* __restrict__ hit, PathSoA nxt,
+__global__ __launch_bounds__(BS) void k_shade(BatchDev B, PathSoA cur, HitSoA hit, PathSoA nxt,
                                               ShadowSoA sh, float4* __restrict__ samples,
                                               WaveCounters* __restrict__ ctr) {
   uint32_t n = ctr->active;
@@ -209,8 +225,8 @@ __global__ __launch_bounds__(BS) void k_shade(BatchDev B, PathSoA cur, const flo
     rng.n = X.x;
     uint32_t item = M.x, depth = M.y & 0xffffu, sid = X.z;
     bool skip = (M.y >> 16) & 1u;
-    float2 H = hit[i];
-    uint32_t prim = __float_as_uint(H.y);
+    float4 HA = hit.a[i], HB = hit.b[i];
+    uint32_t mword = __float_as_uint(HB.z);
     V3<float> add = {0.f, 0.f, 0.f};
     bool cont = false, emit = false;
     V3<float> sp_o = {0.f, 0.f, 0.f}, sp_d = {0.f, 0.f, 0.f}, ndir = {0.f, 0.f, 0.f}, nbeta = beta;
@@ -219,12 +235,19 @@ __global__ __launch_bounds__(BS) void k_shade(BatchDev B, PathSoA cur, const flo
     bool use_nee = false;
     for_scene(sid, [&](uint32_t sc) {
       const SceneView<float> S = B.views[sc];
-      if (prim == 0xffffffffu) {
+      if (mword == HIT_MISS) {
         add = hadamard(beta, background(S.bg, d));
         return;
       }
       Surf<float> s;
-      surface<float, FEAT>(S, prim, o, d, H.x, s);
+      s.p = surface_point(o, d, HA.w);
+      s.n = {HA.x, HA.y, HA.z};
+      s.t = HB.w;
+      s.u = HB.x;
+      s.v = HB.y;
+      s.front = (mword >> 31) != 0;
+      s.mat = mword & 0x7fffffffu;
+      s.prim_id = 0;
       const MaterialT<float>& m = S.materials[s.mat];
       V3<float> em = skip ? V3<float>{0.f, 0.f, 0.f} : emitted(S, m, s.u, s.v);
       V3<float> att;

@@ -416,11 +416,16 @@ struct Surf {
   V3<R> p;       // world point
   V3<R> n;       // unit normal facing the incoming ray
   R t;           // reference-reported t
+  R t_true;      // ray parameter of the point (differs from t under Scale wrappers)
   R u, v;
   bool front;
   uint32_t mat;
   uint32_t prim_id;
 };
+
+// The hit point of ray (o, d) at parameter t (one definition for every kernel
+// that forms it, so that they round alike).
+template <class R> LMN_HD V3<R> surface_point(V3<R> o, V3<R> d, R t) { return o + d * t; }
 
 template <class R, uint32_t FEAT>
 LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_rep, Surf<R>& s) {
@@ -469,9 +474,10 @@ LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_r
     n = xf_nrm(X.nrm, n);
     if (X.renorm) n = unit(n);
   }
-  s.p = o + d * t_true;
+  s.p = surface_point(o, d, t_true);
   s.n = n;
   s.t = t_rep;
+  s.t_true = t_true;
   s.front = front;
   s.mat = km >> 8;
   s.prim_id = pr.prim_id;
