@@ -7,7 +7,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "liblmn.so")
+# LMN_LIB names another build of the same library (e.g. a tuning experiment)
+LIB_PATH = os.environ.get("LMN_LIB") or os.path.join(HERE, "_lib", "liblmn.so")
 
 LMN_OK = 0
 LMN_ERR_INVALID = 1

@@ -40,15 +40,40 @@ struct LdsStack {
 
 __device__ __forceinline__ V3<float> f3(float4 a) { return {a.x, a.y, a.z}; }
 
-// wave-aggregated slot reservation: one atomic per wave
-__device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, bool want) {
-  unsigned long long active = __ballot(1);
-  unsigned long long m = __ballot(want);
-  uint32_t leader = uint32_t(__ffsll((long long)active) - 1);
-  uint32_t base = 0;
-  if (__lane_id() == leader && m) base = atomicAdd(counter, uint32_t(__popcll(m)));
-  base = __shfl(base, int(leader));
-  return base + uint32_t(__popcll(m & __lanemask_lt()));
+// Block-aggregated slot reservation in two queues at once: each wave's
+// ballot counts go to LDS, one thread takes the block's total from each
+// global counter (one atomic per queue per block and round instead of one
+// per wave: the counters are device-wide, so their atomics are the
+// contended resource), and every lane gets its slot.  All lanes of the
+// block must call it; `lds` holds 2 * (BS / 64) + 2 words and is reused
+// after the last barrier of the next call, so a caller alternates two.
+template <int BS>
+__device__ __forceinline__ void block_reserve2(uint32_t* c0, uint32_t* c1, bool w0, bool w1, uint32_t& s0,
+                                               uint32_t& s1, uint32_t* lds) {
+  constexpr int NW = BS / 64;
+  uint32_t wave = threadIdx.x / 64u;
+  unsigned long long m0 = __ballot(w0), m1 = __ballot(w1);
+  if (__lane_id() == 0) {
+    lds[wave] = uint32_t(__popcll(m0));
+    lds[NW + wave] = uint32_t(__popcll(m1));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t0 = 0, t1 = 0;
+    for (int k = 0; k < NW; ++k) {
+      uint32_t a = lds[k], b = lds[NW + k];
+      lds[k] = t0;
+      lds[NW + k] = t1;
+      t0 += a;
+      t1 += b;
+    }
+    lds[2 * NW] = t0 ? atomicAdd(c0, t0) : 0u;
+    lds[2 * NW + 1] = t1 ? atomicAdd(c1, t1) : 0u;
+  }
+  __syncthreads();
+  uint64_t lt = __lanemask_lt();
+  s0 = lds[2 * NW] + lds[wave] + uint32_t(__popcll(m0 & lt));
+  s1 = lds[2 * NW + 1] + lds[NW + wave] + uint32_t(__popcll(m1 & lt));
 }
 
 // ---------------------------------------------------------------------------
@@ -115,15 +140,19 @@ __device__ __forceinline__ void extend_body(const BatchDev& B, const PathSoA& cu
     });
   }
 }
+// The traversal kernels are latency-bound: the register budget is capped at
+// 8 waves per SIMD (64 VGPRs), which outweighs the few spills it costs.
+#define LMN_TRAVERSAL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8, 8)))
+
 template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_extend(BatchDev B, PathSoA cur, HitSoA hit,
+__global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_extend(BatchDev B, PathSoA cur, HitSoA hit,
                                                const WaveCounters* __restrict__ ctr, uint32_t* spill) {
   extend_body<SS, BS, FEAT, false>(B, cur, hit, ctr, spill, nullptr);
 }
 // Instrumented extend (counts node fetches and primitive tests): a kernel of
 // its own name so that profiles keep it apart from the timed k_extend.
 template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_extend_count(BatchDev B, PathSoA cur, HitSoA hit,
+__global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_extend_count(BatchDev B, PathSoA cur, HitSoA hit,
                                                      const WaveCounters* __restrict__ ctr, uint32_t* spill,
                                                      TraceCounters* tc) {
   extend_body<SS, BS, FEAT, true>(B, cur, hit, ctr, spill, tc);
@@ -159,7 +188,7 @@ __global__ __launch_bounds__(BS) void k_extend_march(BatchDev B, PathSoA cur, Hi
 // here; with them an occluded ray's distance is negated and k_shadow_march
 // finishes the query.
 template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_shadow(BatchDev B, ShadowSoA sh, float4* __restrict__ samples,
+__global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_shadow(BatchDev B, ShadowSoA sh, float4* __restrict__ samples,
                                                const WaveCounters* __restrict__ ctr, uint32_t* spill) {
   __shared__ uint32_t lds[SS * BS];
   constexpr bool DEFER = (FEAT & FEAT_MARCH) != 0;
@@ -214,18 +243,24 @@ template <int BS, uint32_t FEAT>
 __global__ __launch_bounds__(BS) void k_shade(BatchDev B, PathSoA cur, HitSoA hit, PathSoA nxt,
                                               ShadowSoA sh, float4* __restrict__ samples,
                                               WaveCounters* __restrict__ ctr) {
+  __shared__ uint32_t res_lds[2][2 * (BS / 64) + 2];
   uint32_t n = ctr->active;
   uint32_t nthreads = gridDim.x * BS;
-  for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < n; i += nthreads) {
-    float4 A = cur.ob[i], Bd = cur.db[i];
-    uint4 M = cur.meta[i], X = cur.aux[i];
+  uint32_t round = 0;
+  // block-uniform trip count: every lane reaches the reservation barriers
+  for (uint32_t base = blockIdx.x * BS; base < n; base += nthreads, ++round) {
+    uint32_t i = base + threadIdx.x;
+    bool valid = i < n;
+    uint32_t j = valid ? i : base;  // invalid lanes read a valid slot and do nothing with it
+    float4 A = cur.ob[j], Bd = cur.db[j];
+    uint4 M = cur.meta[j], X = cur.aux[j];
     V3<float> o = f3(A), d = f3(Bd), beta = {A.w, Bd.w, __uint_as_float(X.y)};
     Rng rng;
     rng.key = (uint64_t(M.w) << 32) | M.z;
     rng.n = X.x;
     uint32_t item = M.x, depth = M.y & 0xffffu, sid = X.z;
     bool skip = (M.y >> 16) & 1u;
-    float4 HA = hit.a[i], HB = hit.b[i];
+    float4 HA = hit.a[j], HB = hit.b[j];
     uint32_t mword = __float_as_uint(HB.z);
     V3<float> add = {0.f, 0.f, 0.f};
     bool cont = false, emit = false;
@@ -233,7 +268,7 @@ __global__ __launch_bounds__(BS) void k_shade(BatchDev B, PathSoA cur, HitSoA hi
     V3<float> cb = {0.f, 0.f, 0.f};
     float sp_dist = 0.f;
     bool use_nee = false;
-    for_scene(sid, [&](uint32_t sc) {
+    if (valid) for_scene(sid, [&](uint32_t sc) {
       const SceneView<float> S = B.views[sc];
       if (mword == HIT_MISS) {
         add = hadamard(beta, background(S.bg, d));
@@ -278,13 +313,13 @@ __global__ __launch_bounds__(BS) void k_shade(BatchDev B, PathSoA cur, HitSoA hi
       cont = depth > 1;
     });
     if (add.x != 0.f || add.y != 0.f || add.z != 0.f) add_sample(samples, item, add);
-    uint32_t ps = wave_reserve(&ctr->n_shadow, emit);
+    uint32_t ps, pn;
+    block_reserve2<BS>(&ctr->n_shadow, &ctr->next_active, emit, cont, ps, pn, res_lds[round & 1u]);
     if (emit) {
       sh.o[ps] = make_float4(sp_o.x, sp_o.y, sp_o.z, sp_dist);
       sh.d[ps] = make_float4(sp_d.x, sp_d.y, sp_d.z, __uint_as_float(item));
       sh.c[ps] = make_float4(cb.x, cb.y, cb.z, __uint_as_float(sid));
     }
-    uint32_t pn = wave_reserve(&ctr->next_active, cont);
     if (cont) {
       nxt.ob[pn] = make_float4(sp_o.x, sp_o.y, sp_o.z, nbeta.x);
       nxt.db[pn] = make_float4(ndir.x, ndir.y, ndir.z, nbeta.y);
