@@ -249,7 +249,8 @@ def main():
 
     extra = {"stage_ms": {"extend": ext_ms, "shade": sum(s["shade_ms"] for s in stats) / args.steps,
                           "shadow": sum(s["shadow_ms"] for s in stats) / args.steps,
-                          "regen": sum(s["regen_ms"] for s in stats) / args.steps}}
+                          "regen": sum(s["regen_ms"] for s in stats) / args.steps,
+                          "marched_kinds": sum(s["march_ms"] for s in stats) / args.steps}}
     if not args.no_fp64:
         p64 = params("plain64")
         outs64 = [torch.empty_like(o) for o in outs]

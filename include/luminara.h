@@ -232,10 +232,11 @@ typedef struct lmn_stats {
   double render_ms;        /* device time of the pixel loop               */
   uint32_t iterations;     /* wavefront iterations executed               */
   uint32_t extend_launches;
-  double extend_ms;        /* summed stage times (LMN_FLAG_STAGE_TIMING)  */
-  double shade_ms;
+  double extend_ms;        /* summed stage times (LMN_FLAG_STAGE_TIMING):  */
+  double shade_ms;         /* extend and shadow are the main BVH's kernels */
   double shadow_ms;
   double regen_ms;
+  double march_ms;         /* the marched kinds' extend and shadow kernels */
 } lmn_stats;
 
 /* ---- ray-batch parity harness -------------------------------------------- */

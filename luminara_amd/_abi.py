@@ -101,7 +101,7 @@ class Stats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("render_ms", C.c_double), ("iterations", C.c_uint32),
                 ("extend_launches", C.c_uint32), ("extend_ms", C.c_double), ("shade_ms", C.c_double),
-                ("shadow_ms", C.c_double), ("regen_ms", C.c_double)]
+                ("shadow_ms", C.c_double), ("regen_ms", C.c_double), ("march_ms", C.c_double)]
 
 
 class Ray(C.Structure):

@@ -80,7 +80,7 @@ struct Workspace {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   static constexpr int kBlock = 8;  // wavefront iterations between host checks
-  hipEvent_t stage_ev[kBlock][5] = {};
+  hipEvent_t stage_ev[kBlock][7] = {};
   int init() {
     if (!stream) {
       HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -567,7 +567,7 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
   // Stage timing records events between the stages; events recorded by graph
   // nodes cannot be timed, so timed renders launch directly.
   bool use_graph = !(p0->flags & LMN_FLAG_NO_GRAPH) && !timing && !W.graph_failed[0];
-  double t_stage[4] = {0, 0, 0, 0};
+  double t_stage[6] = {0, 0, 0, 0, 0, 0};
   uint32_t n_launch = 0;
   // parts of a chunk: uploaded in one copy, chunk after chunk
   std::vector<uint64_t> begins;
@@ -621,7 +621,7 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
       HIP_TRY(hipStreamSynchronize(st));
       if (timing) {
         for (int k = 0; k < block; ++k)
-          for (int g = 0; g < 4; ++g) {
+          for (int g = 0; g < 6; ++g) {
             float ms = 0.f;
             HIP_TRY(hipEventElapsedTime(&ms, W.stage_ev[k][g], W.stage_ev[k][g + 1]));
             t_stage[g] += ms;
@@ -659,9 +659,10 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
     stats->iterations = uint32_t(iters);
     stats->extend_launches = n_launch;
     stats->extend_ms = t_stage[0];
-    stats->shade_ms = t_stage[1];
-    stats->shadow_ms = t_stage[2];
-    stats->regen_ms = t_stage[3];
+    stats->march_ms = t_stage[1] + t_stage[4];
+    stats->shade_ms = t_stage[2];
+    stats->shadow_ms = t_stage[3];
+    stats->regen_ms = t_stage[5];
   }
   return LMN_OK;
 }

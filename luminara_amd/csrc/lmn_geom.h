@@ -461,11 +461,37 @@ LMN_HD bool hit_superellipsoid_march(const double* p, V3<double> ro, V3<double> 
   return false;
 }
 
-// Superellipsoid, exact mode: the same implicit surface f = 0 bracketed by a
-// sign change of f along the ray (sampled at the reference's step rule) and
-// refined by bisection to 1e-12 relative, FP64.
-LMN_HD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax,
-                                     double& t) {
+// Superellipsoid, exact mode: the first crossing of the implicit surface
+// val(q) = (|x|^(2/e2) + |z|^(2/e2))^(e2/e1) + |y|^(2/e1) = 1 along the ray
+// (val is the reference's f + 1 raised to 2/e1, a monotone map, so the
+// surface and the inside are the same), FP64.  Roots are isolated by
+// interval arithmetic on a binary subdivision of the chord through the unit
+// box (Mitchell, "Robust ray intersection with interval arithmetic", 1990):
+// every term is monotone in |x|, |y|, |z| or in the partial sums, so the
+// range of val over a sub-interval of t follows from the ranges of the
+// coordinates.  Sub-intervals whose range excludes 1 hold no crossing and are
+// skipped whole; the first interval of at most 1/4096 of the chord with a
+// sign change across it is refined by the Illinois variant of regula falsi.
+// Intervals that may hold a crossing but show no sign change keep splitting
+// down to 2^-24 of the chord, so slivers thinner than any fixed sampling are
+// found; a touch without a sign change at that scale is no hit.
+LMN_HD double se_val(V3<double> q, double px, double qa, double py) {
+  double a = pow(m_abs(q.x), px) + pow(m_abs(q.z), px);
+  return pow(a, qa) + pow(m_abs(q.y), py);
+}
+LMN_HD void se_range1(double o, double d, double ta, double tb, double& lo, double& hi) {
+  double a = o + d * ta, b = o + d * tb;
+  double amin = a < b ? a : b, amax = a < b ? b : a;
+  if (amin <= 0.0 && amax >= 0.0) {
+    lo = 0.0;
+    hi = m_abs(amin) > m_abs(amax) ? m_abs(amin) : m_abs(amax);
+  } else {
+    lo = m_abs(amin) < m_abs(amax) ? m_abs(amin) : m_abs(amax);
+    hi = m_abs(amin) < m_abs(amax) ? m_abs(amax) : m_abs(amin);
+  }
+}
+LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax,
+                                       double& t) {
   V3<double> o = {(ro.x - p[0]) / p[3], (ro.y - p[1]) / p[4], (ro.z - p[2]) / p[5]};
   V3<double> d = {rd.x / p[3], rd.y / p[4], rd.z / p[5]};
   // clip to the unit box |q|_inf <= 1 that contains the surface, padded so
@@ -483,28 +509,78 @@ LMN_HD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double> 
     t1 = fmin(t1, b);
   }
   if (!(t0 <= t1)) return false;
-  double dl = length(d);
-  int steps = 512;
-  double h = (t1 - t0) / steps;
-  double fa = superellipsoid_f(o + d * t0, p[6], p[7]);  // <= 0: starts inside, the exit is reported
-  double ta = t0;
-  for (int i = 1; i <= steps; ++i) {
-    double tb = (i == steps) ? t1 : t0 + h * i;
-    double fb = superellipsoid_f(o + d * tb, p[6], p[7]);
-    if ((fa > 0.0) != (fb > 0.0)) {
-      double lo = ta, hi = tb, flo = fa;
-      for (int k = 0; k < 80 && (hi - lo) * dl > 1e-13 * (1.0 + m_abs(hi) * dl); ++k) {
-        double mid = 0.5 * (lo + hi);
-        double fm = superellipsoid_f(o + d * mid, p[6], p[7]);
-        if ((fm > 0.0) == (flo > 0.0)) { lo = mid; flo = fm; } else { hi = mid; }
-      }
-      double th = 0.5 * (lo + hi);
-      if (th >= tmin && th <= tmax) { t = th; return true; }
+  const double px = 2.0 / p[7], qa = p[7] / p[6], py = 2.0 / p[6];
+  const int kSolveLevel = 12, kMaxLevel = 24;
+  const double w = t1 - t0;
+  double fa = se_val(o + d * t0, px, qa, py) - 1.0;  // sign at the left end of the current interval
+  // stackless depth-first walk of the subdivision tree: interval (level, idx)
+  // spans [t0 + idx * w / 2^level, t0 + (idx + 1) * w / 2^level]
+  int level = 0;
+  uint32_t idx = 0;
+  for (;;) {
+    double scale = w / double(1u << level);
+    double a = t0 + double(idx) * scale, b = (idx + 1u == (1u << level)) ? t1 : a + scale;
+    double xl, xh, yl, yh, zl, zh;
+    se_range1(o.x, d.x, a, b, xl, xh);
+    se_range1(o.y, d.y, a, b, yl, yh);
+    se_range1(o.z, d.z, a, b, zl, zh);
+    double vlo = pow(pow(xl, px) + pow(zl, px), qa) + pow(yl, py);
+    double vhi = pow(pow(xh, px) + pow(zh, px), qa) + pow(yh, py);
+    bool constant = vlo > 1.0 + 1e-12 || vhi < 1.0 - 1e-12;  // no crossing inside
+    if (!constant && level < kSolveLevel) {  // subdivide: left child first
+      ++level;
+      idx *= 2u;
+      continue;
     }
-    ta = tb;
-    fa = fb;
+    if (!constant) {
+      double fb = se_val(o + d * b, px, qa, py) - 1.0;
+      if ((fa > 0.0) == (fb > 0.0) && level < kMaxLevel) {
+        // no sign change across the leaf: a sliver entered and left within
+        // it, or a grazing touch; keep splitting (down to 2^-24 of the chord)
+        ++level;
+        idx *= 2u;
+        continue;
+      }
+      if ((fa > 0.0) != (fb > 0.0)) {
+        // Illinois regula falsi on [a, b] (the bracket always holds the root)
+        double lo = a, hi = b, flo = fa, fhi = fb;
+        int side = 0;
+        for (int it = 0; it < 100 && (hi - lo) > 1e-13 * (m_abs(hi) + 1e-300); ++it) {
+          double m = (lo * fhi - hi * flo) / (fhi - flo);
+          if (!(m > lo && m < hi)) m = 0.5 * (lo + hi);
+          double fm = se_val(o + d * m, px, qa, py) - 1.0;
+          if ((fm > 0.0) == (flo > 0.0)) {
+            lo = m;
+            flo = fm;
+            if (side == -1) fhi *= 0.5;
+            side = -1;
+          } else {
+            hi = m;
+            fhi = fm;
+            if (side == 1) flo *= 0.5;
+            side = 1;
+          }
+        }
+        double th = 0.5 * (lo + hi);
+        if (th >= tmin && th <= tmax) {
+          t = th;
+          return true;
+        }
+      }
+      fa = fb;
+    } else {
+      // constant sign over [a, b]: the sign at b is the one at a unless the
+      // range touches 1 exactly; evaluate it only where that can matter
+      fa = (vlo > 1.0 + 1e-12) ? 1.0 : -1.0;
+    }
+    // next interval: climb while this is a right child, then step right
+    while (idx & 1u) {
+      idx >>= 1;
+      --level;
+    }
+    if (level == 0) return false;
+    ++idx;
   }
-  return false;
 }
 
 LMN_HD double spring_sdf(V3<double> q, const double* p) {

@@ -74,7 +74,8 @@ struct WaveLaunch {
 };
 
 // One wavefront iteration: extend, shade, shadow, regenerate, bookkeeping.
-// With `ev` (5 events), ev[0..4] bracket the stages for per-stage timing.
+// With `ev` (7 events), ev[0..6] bracket extend, its marched pass, shade,
+// shadow, its marched pass and regenerate+bookkeeping for per-stage timing.
 void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt,
                            hipEvent_t* ev = nullptr);
 void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s);

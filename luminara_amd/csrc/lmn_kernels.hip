@@ -443,11 +443,13 @@ __global__ void k_finalize(const double4* __restrict__ acc, float* __restrict__ 
 // Launchers
 
 template <int SS, int BS, uint32_t FEAT>
-static void launch_stage_ss(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow) {
+static void launch_stage_ss(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow,
+                            hipEvent_t mid) {
   constexpr uint32_t MAIN = FEAT & ~FEAT_MARCH;
   constexpr bool MARCH = (FEAT & FEAT_MARCH) != 0;
   if (shadow) {
     hipLaunchKernelGGL((k_shadow<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, L.sh, L.samples, L.ctr, L.spill);
+    if (mid) (void)hipEventRecord(mid, L.stream);
     if (MARCH)
       hipLaunchKernelGGL((k_shadow_march<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, L.sh, L.samples,
                          L.ctr, L.spill);
@@ -458,22 +460,23 @@ static void launch_stage_ss(const WaveLaunch& L, const BatchDev& B, const PathSo
                        L.spill, L.tc);
   else
     hipLaunchKernelGGL((k_extend<SS, BS, MAIN>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.ctr, L.spill);
+  if (mid) (void)hipEventRecord(mid, L.stream);
   if (MARCH)
     hipLaunchKernelGGL((k_extend_march<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.ctr,
                        L.spill);
 }
 template <uint32_t FEAT>
-static void launch_stage(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow) {
+static void launch_stage(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow, hipEvent_t mid) {
   if (L.block == 128) {
     if (L.stack_lds == 32)
-      launch_stage_ss<32, 128, FEAT>(L, B, cur, shadow);
+      launch_stage_ss<32, 128, FEAT>(L, B, cur, shadow, mid);
     else
-      launch_stage_ss<16, 128, FEAT>(L, B, cur, shadow);
+      launch_stage_ss<16, 128, FEAT>(L, B, cur, shadow, mid);
   } else {
     if (L.stack_lds == 32)
-      launch_stage_ss<32, 256, FEAT>(L, B, cur, shadow);
+      launch_stage_ss<32, 256, FEAT>(L, B, cur, shadow, mid);
     else
-      launch_stage_ss<16, 256, FEAT>(L, B, cur, shadow);
+      launch_stage_ss<16, 256, FEAT>(L, B, cur, shadow, mid);
   }
 }
 // The FP32 path's feature sets: marched primitives are FP64 records too, so
@@ -488,8 +491,9 @@ static void launch_stage(const WaveLaunch& L, const BatchDev& B, const PathSoA& 
     default: CALL(7); break;           \
   }
 
-static void launch_trace_stage(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow) {
-#define CALL(f) launch_stage<f>(L, B, cur, shadow)
+static void launch_trace_stage(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow,
+                               hipEvent_t mid) {
+#define CALL(f) launch_stage<f>(L, B, cur, shadow, mid)
   LMN_FEAT32_SWITCH(L.feat, CALL)
 #undef CALL
 }
@@ -509,19 +513,20 @@ static void launch_shade(const WaveLaunch& L, const BatchDev& B, const PathSoA& 
 
 void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt,
                            hipEvent_t* ev) {
+  // events: 0 extend 1 extend (marched) 2 shade 3 shadow 4 shadow (marched) 5 regen 6
   if (ev) (void)hipEventRecord(ev[0], L.stream);
-  launch_trace_stage(L, B, cur, false);
-  if (ev) (void)hipEventRecord(ev[1], L.stream);
-  launch_shade(L, B, cur, nxt);
+  launch_trace_stage(L, B, cur, false, ev ? ev[1] : nullptr);
   if (ev) (void)hipEventRecord(ev[2], L.stream);
-  launch_trace_stage(L, B, cur, true);
+  launch_shade(L, B, cur, nxt);
   if (ev) (void)hipEventRecord(ev[3], L.stream);
+  launch_trace_stage(L, B, cur, true, ev ? ev[4] : nullptr);
+  if (ev) (void)hipEventRecord(ev[5], L.stream);
   if (L.block == 128)
     hipLaunchKernelGGL((k_regen<128>), dim3(L.grid), dim3(128), 0, L.stream, B, nxt, L.samples, L.ctr, L.pool);
   else
     hipLaunchKernelGGL((k_regen<256>), dim3(L.grid), dim3(256), 0, L.stream, B, nxt, L.samples, L.ctr, L.pool);
   hipLaunchKernelGGL(k_book, dim3(1), dim3(64), 0, L.stream, L.ctr, L.pool);
-  if (ev) (void)hipEventRecord(ev[4], L.stream);
+  if (ev) (void)hipEventRecord(ev[6], L.stream);
 }
 
 void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s) {
