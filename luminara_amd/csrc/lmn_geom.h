@@ -471,14 +471,36 @@ LMN_HD bool hit_superellipsoid_march(const double* p, V3<double> ro, V3<double> 
 // range of val over a sub-interval of t follows from the ranges of the
 // coordinates.  Sub-intervals whose range excludes 1 hold no crossing and are
 // skipped whole; the first interval of at most 1/4096 of the chord with a
-// sign change across it is refined by the Illinois variant of regula falsi.
+// sign change across it is refined by bracketed Newton iteration.
 // Intervals that may hold a crossing but show no sign change keep splitting
 // down to 2^-24 of the chord, so slivers thinner than any fixed sampling are
-// found; a touch without a sign change at that scale is no hit.
+// found; a touch without a sign change at that scale is no hit.  The
+// interval bounds are screened first in FP32 with the hardware log2/exp2:
+// their error near val = 1 is below 1e-5 for exponents up to 10 and
+// coordinates inside the padded box, and an interval counts as ruled out
+// only with a margin of 5e-4 on top; where a screened bound falls inside
+// that band the FP64 bounds decide.  Below the first levels, intervals the
+// plain bounds cannot rule out get the mean-value form as well (rays that
+// graze the surface).  All sign evaluations are FP64.
 LMN_HD double se_val(V3<double> q, double px, double qa, double py) {
-  double a = pow(m_abs(q.x), px) + pow(m_abs(q.z), px);
-  return pow(a, qa) + pow(m_abs(q.y), py);
+  double a = pow_pos(m_abs(q.x), px) + pow_pos(m_abs(q.z), px);
+  return pow_pos(a, qa) + pow_pos(m_abs(q.y), py);
 }
+// val at o + d t and its derivative in t (the terms' derivatives follow from
+// their values: d|x|^p/dt = p |x|^p / x * dx/dt)
+LMN_HD double se_val_d(V3<double> o, V3<double> d, double t, double px, double qa, double py, double& dv) {
+  V3<double> q = o + d * t;
+  double tx = pow_pos(m_abs(q.x), px), tz = pow_pos(m_abs(q.z), px), ty = pow_pos(m_abs(q.y), py);
+  double A = tx + tz, tA = pow_pos(A, qa);
+  double dtx = q.x != 0.0 ? px * tx / q.x * d.x : 0.0;
+  double dtz = q.z != 0.0 ? px * tz / q.z * d.z : 0.0;
+  double dty = q.y != 0.0 ? py * ty / q.y * d.y : 0.0;
+  dv = (A > 0.0 ? qa * tA / A * (dtx + dtz) : 0.0) + dty;
+  return tA + ty;
+}
+LMN_HD float pow_screen(float x, float y) { return exp2f(y * log2f(x)); }
+
+// Range of |o + d t| over [ta, tb].
 LMN_HD void se_range1(double o, double d, double ta, double tb, double& lo, double& hi) {
   double a = o + d * ta, b = o + d * tb;
   double amin = a < b ? a : b, amax = a < b ? b : a;
@@ -490,6 +512,76 @@ LMN_HD void se_range1(double o, double d, double ta, double tb, double& lo, doub
     hi = m_abs(amin) < m_abs(amax) ? m_abs(amax) : m_abs(amin);
   }
 }
+
+// FP32 terms of the interval bound over a sub-chord: the coordinate
+// magnitudes' ranges [ul, uh], the terms |c|^p at both ends of each range,
+// and A = X + Z, B = A^qa at the ends of A's range.
+struct SeScreen {
+  float xl, xh, yl, yh, zl, zh;
+  float txl, txh, tyl, tyh, tzl, tzh;
+  float Al, Ah, bl, bh;
+};
+
+// Sign of val - 1 over [a, b] by the mean-value form val([a, b]) in
+// val(m) +- (b - a)/2 * max|val'([a, b])|, for sub-chords on which no
+// coordinate changes sign: +1 / -1 where val stays above / below 1, else 0.
+// The plain interval bound adds up the variation of every term and cannot
+// rule out sub-chords of rays that graze the surface (terms that cancel
+// along the ray, as on rays leaving the surface at a shallow angle); this
+// form's width shrinks like (b - a)^2 plus the true slope.  The slope bound
+// comes from the FP32 terms: val' = qa A^(qa-1) A' + Y', A' = X' + Z', each
+// term's derivative p |c|^(p-1) d|c|/dt ranging between its values at the
+// sub-chord's ends (|c|^(p-1) = |c|^p / |c|), inflated by 1 % for the FP32
+// error; val(m) is screened in FP32 like the plain bounds and evaluated in
+// FP64 where that cannot decide.
+LMN_HD int se_mean_value_sign(V3<double> o, V3<double> d, double a, double b, float pxf, float qaf, float pyf,
+                              double px, double qa, double py, const SeScreen& S, float margin) {
+  const float kTiny = 1e-30f;  // terms of smaller magnitudes may have flushed to zero
+  if (!(S.xl > kTiny && S.yl > kTiny && S.zl > kTiny && S.Al > kTiny)) return 0;
+  const double m = 0.5 * (a + b);
+  const V3<double> q = o + d * m;
+  auto dterm = [](float p, float tl, float th, float ul, float uh, float k, float& lo, float& hi) {
+    float g1 = p * (tl / ul), g2 = p * (th / uh);
+    float gl = g1 < g2 ? g1 : g2, gh = g1 < g2 ? g2 : g1;
+    lo = k >= 0.0f ? gl * k : gh * k;
+    hi = k >= 0.0f ? gh * k : gl * k;
+  };
+  float xdl, xdh, zdl, zdh, ydl, ydh;
+  dterm(pxf, S.txl, S.txh, S.xl, S.xh, float(q.x > 0.0 ? d.x : -d.x), xdl, xdh);
+  dterm(pxf, S.tzl, S.tzh, S.zl, S.zh, float(q.z > 0.0 ? d.z : -d.z), zdl, zdh);
+  dterm(pyf, S.tyl, S.tyh, S.yl, S.yh, float(q.y > 0.0 ? d.y : -d.y), ydl, ydh);
+  const float adl = xdl + zdl, adh = xdh + zdh;
+  const float P1 = S.bl / S.Al, P2 = S.bh / S.Ah;  // A^(qa-1) at the ends of A's range
+  const float Pl = P1 < P2 ? P1 : P2, Ph = P1 < P2 ? P2 : P1;
+  float bdl, bdh;  // A^(qa-1) A', with A^(qa-1) >= 0
+  if (adl >= 0.0f) {
+    bdl = Pl * adl;
+    bdh = Ph * adh;
+  } else if (adh <= 0.0f) {
+    bdl = Ph * adl;
+    bdh = Pl * adh;
+  } else {
+    bdl = Ph * adl;
+    bdh = Ph * adh;
+  }
+  const float vdl = qaf * bdl + ydl, vdh = qaf * bdh + ydh;
+  const float slope = fabsf(vdl) > fabsf(vdh) ? fabsf(vdl) : fabsf(vdh);
+  if (!(slope < 1e30f)) return 0;  // also NaN
+  // the centre value in FP32 first, with the screen's margin: clear of the
+  // band on both sides, the FP64 form would straddle 1 as well
+  const float r32 = 0.5f * float(b - a) * slope * 1.01f;
+  const float vm32 = pow_screen(pow_screen(fabsf(float(q.x)), pxf) + pow_screen(fabsf(float(q.z)), pxf), qaf) +
+                     pow_screen(fabsf(float(q.y)), pyf);
+  if (vm32 - r32 > 1.0f + margin) return 1;
+  if (vm32 + r32 < 1.0f - margin) return -1;
+  if (vm32 - r32 < 1.0f - margin && vm32 + r32 > 1.0f + margin) return 0;
+  const double vm = se_val(q, px, qa, py);
+  const double r = 0.5 * (b - a) * double(slope) * 1.01 + 1e-12 * (1.0 + vm);
+  if (vm - r > 1.0 + 1e-12) return 1;
+  if (vm + r < 1.0 - 1e-12) return -1;
+  return 0;
+}
+
 LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax,
                                        double& t) {
   V3<double> o = {(ro.x - p[0]) / p[3], (ro.y - p[1]) / p[4], (ro.z - p[2]) / p[5]};
@@ -510,7 +602,9 @@ LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double
   }
   if (!(t0 <= t1)) return false;
   const double px = 2.0 / p[7], qa = p[7] / p[6], py = 2.0 / p[6];
-  const int kSolveLevel = 12, kMaxLevel = 24;
+  const float pxf = float(px), qaf = float(qa), pyf = float(py);
+  const float kScreenMargin = 5e-4f;
+  const int kSolveLevel = 12, kMaxLevel = 24, kMeanValueLevel = 4;
   const double w = t1 - t0;
   double fa = se_val(o + d * t0, px, qa, py) - 1.0;  // sign at the left end of the current interval
   // stackless depth-first walk of the subdivision tree: interval (level, idx)
@@ -524,9 +618,25 @@ LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double
     se_range1(o.x, d.x, a, b, xl, xh);
     se_range1(o.y, d.y, a, b, yl, yh);
     se_range1(o.z, d.z, a, b, zl, zh);
-    double vlo = pow(pow(xl, px) + pow(zl, px), qa) + pow(yl, py);
-    double vhi = pow(pow(xh, px) + pow(zh, px), qa) + pow(yh, py);
-    bool constant = vlo > 1.0 + 1e-12 || vhi < 1.0 - 1e-12;  // no crossing inside
+    // sign of val - 1 over the interval where it is constant, else 0
+    SeScreen S;
+    S.xl = float(xl), S.xh = float(xh), S.yl = float(yl), S.yh = float(yh), S.zl = float(zl), S.zh = float(zh);
+    S.txl = pow_screen(S.xl, pxf), S.txh = pow_screen(S.xh, pxf), S.tzl = pow_screen(S.zl, pxf);
+    S.tzh = pow_screen(S.zh, pxf), S.tyl = pow_screen(S.yl, pyf), S.tyh = pow_screen(S.yh, pyf);
+    S.Al = S.txl + S.tzl, S.Ah = S.txh + S.tzh;
+    S.bl = pow_screen(S.Al, qaf), S.bh = pow_screen(S.Ah, qaf);
+    const float slo = S.bl + S.tyl, shi = S.bh + S.tyh;
+    int sign = slo > 1.0f + kScreenMargin ? 1 : (shi < 1.0f - kScreenMargin ? -1 : 0);
+    if (sign == 0 && !(slo < 1.0f - kScreenMargin && shi > 1.0f + kScreenMargin)) {
+      // a bound inside the band: the FP64 bounds decide (with both bounds
+      // clear of the band they would straddle 1 as well)
+      double vlo = pow_pos(pow_pos(xl, px) + pow_pos(zl, px), qa) + pow_pos(yl, py);
+      double vhi = pow_pos(pow_pos(xh, px) + pow_pos(zh, px), qa) + pow_pos(yh, py);
+      sign = vlo > 1.0 + 1e-12 ? 1 : (vhi < 1.0 - 1e-12 ? -1 : 0);
+    }
+    if (sign == 0 && level >= kMeanValueLevel)
+      sign = se_mean_value_sign(o, d, a, b, pxf, qaf, pyf, px, qa, py, S, kScreenMargin);
+    bool constant = sign != 0;  // no crossing inside
     if (!constant && level < kSolveLevel) {  // subdivide: left child first
       ++level;
       idx *= 2u;
@@ -542,26 +652,27 @@ LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double
         continue;
       }
       if ((fa > 0.0) != (fb > 0.0)) {
-        // Illinois regula falsi on [a, b] (the bracket always holds the root)
-        double lo = a, hi = b, flo = fa, fhi = fb;
-        int side = 0;
-        for (int it = 0; it < 100 && (hi - lo) > 1e-13 * (m_abs(hi) + 1e-300); ++it) {
-          double m = (lo * fhi - hi * flo) / (fhi - flo);
-          if (!(m > lo && m < hi)) m = 0.5 * (lo + hi);
-          double fm = se_val(o + d * m, px, qa, py) - 1.0;
-          if ((fm > 0.0) == (flo > 0.0)) {
-            lo = m;
-            flo = fm;
-            if (side == -1) fhi *= 0.5;
-            side = -1;
+        // Newton's method from the secant point, kept inside the bracket
+        // [lo, hi] (which always holds the root) by bisection
+        double lo = a, hi = b, flo = fa;
+        double th = (a * fb - b * fa) / (fb - fa);
+        if (!(th > lo && th < hi)) th = 0.5 * (lo + hi);
+        for (int it = 0; it < 64; ++it) {
+          double g;
+          double fx = se_val_d(o, d, th, px, qa, py, g) - 1.0;
+          if (fx == 0.0) break;
+          if ((fx > 0.0) == (flo > 0.0)) {
+            lo = th;
+            flo = fx;
           } else {
-            hi = m;
-            fhi = fm;
-            if (side == 1) flo *= 0.5;
-            side = 1;
+            hi = th;
           }
+          double tn = th - fx / g;
+          if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+          bool done = m_abs(tn - th) <= 1e-13 * m_abs(tn) || hi - lo <= 1e-13 * m_abs(hi);
+          th = tn;
+          if (done) break;
         }
-        double th = 0.5 * (lo + hi);
         if (th >= tmin && th <= tmax) {
           t = th;
           return true;
@@ -569,9 +680,8 @@ LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double
       }
       fa = fb;
     } else {
-      // constant sign over [a, b]: the sign at b is the one at a unless the
-      // range touches 1 exactly; evaluate it only where that can matter
-      fa = (vlo > 1.0 + 1e-12) ? 1.0 : -1.0;
+      // constant sign over [a, b]: the sign at b
+      fa = double(sign);
     }
     // next interval: climb while this is a right child, then step right
     while (idx & 1u) {

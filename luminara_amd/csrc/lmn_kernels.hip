@@ -158,10 +158,15 @@ __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_extend_count(Bat
   extend_body<SS, BS, FEAT, true>(B, cur, hit, ctr, spill, tc);
 }
 
+// The marched kernels' FP64 root finders would take every register a wave
+// can have (one wave per SIMD); capped at 4 waves per SIMD they spill some
+// and run about a third faster.
+#define LMN_MARCH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 8)))
+
 // Extend, marched pass: the FP64 torus/superellipsoid/spring BVH, limited
 // to the main pass's hit distance.
 template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_extend_march(BatchDev B, PathSoA cur, HitSoA hit,
+__global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_extend_march(BatchDev B, PathSoA cur, HitSoA hit,
                                                      const WaveCounters* __restrict__ ctr, uint32_t* spill) {
   __shared__ uint32_t lds[SS * BS];
   uint32_t n = ctr->active;
@@ -214,7 +219,7 @@ __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_shadow(BatchDev 
 }
 
 template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) void k_shadow_march(BatchDev B, ShadowSoA sh, float4* __restrict__ samples,
+__global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_shadow_march(BatchDev B, ShadowSoA sh, float4* __restrict__ samples,
                                                      const WaveCounters* __restrict__ ctr, uint32_t* spill) {
   __shared__ uint32_t lds[SS * BS];
   uint32_t n = ctr->n_shadow;

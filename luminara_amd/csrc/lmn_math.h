@@ -19,7 +19,8 @@ namespace lmn {
 
 #if !defined(__HIPCC__)
 using std::acos; using std::asin; using std::atan; using std::atan2; using std::cbrt; using std::cos;
-using std::exp; using std::fabs; using std::floor; using std::fmax; using std::fmin; using std::log;
+using std::exp; using std::fabs; using std::floor; using std::fma; using std::fmax; using std::fmin; using std::frexp;
+using std::ldexp; using std::log; using std::rint;
 using std::pow; using std::sin; using std::sqrt; using std::trunc;
 #endif
 
@@ -60,6 +61,59 @@ LMN_HD float m_fmax(float a, float b) { return fmaxf(a, b); }
 LMN_HD double m_fmax(double a, double b) { return fmax(a, b); }
 LMN_HD float m_fmin(float a, float b) { return fminf(a, b); }
 LMN_HD double m_fmin(double a, double b) { return fmin(a, b); }
+
+// x^y for x >= 0 and finite y > 0, FP64, for the exact-mode root finders
+// (the library pow spends several hundred operations on range cases these
+// never reach).  x = m 2^e with m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s),
+// s = (m - 1) / (m + 1), by the odd Taylor series of atanh through s^21
+// (|s| <= 0.1716: truncation below 2^-54); exp(L), L = y ln x, by Cody-Waite
+// reduction r = L - n ln2 with ln2 split so that n * LN2_HI is exact, and the
+// Taylor series of exp(r), |r| <= ln2 / 2, through degree 13 (truncation
+// below 2^-57).  Relative error: a few ulp times max(1, |L|).
+LMN_HD double pow_pos(double x, double y) {
+  if (x == 0.0) return 0.0;
+  if (!(x < 1.0 / 0.0)) return x;  // NaN, +inf
+  int e;
+  double m = frexp(x, &e);  // [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m *= 2.0;
+    --e;
+  }
+  const double s = (m - 1.0) / (m + 1.0), z = s * s;
+  double a = 1.0 / 21.0;
+  a = fma(a, z, 1.0 / 19.0);
+  a = fma(a, z, 1.0 / 17.0);
+  a = fma(a, z, 1.0 / 15.0);
+  a = fma(a, z, 1.0 / 13.0);
+  a = fma(a, z, 1.0 / 11.0);
+  a = fma(a, z, 1.0 / 9.0);
+  a = fma(a, z, 1.0 / 7.0);
+  a = fma(a, z, 1.0 / 5.0);
+  a = fma(a, z, 1.0 / 3.0);
+  a = fma(a, z, 1.0);
+  const double LN2_HI = 0x1.62e42fee00000p-1, LN2_LO = 0x1.a39ef35793c76p-33;
+  const double de = double(e);
+  const double L = y * fma(de, LN2_HI, fma(de, LN2_LO, 2.0 * s * a));
+  if (L < -745.2) return 0.0;
+  if (L > 709.8) return 1.0 / 0.0;
+  const double n = rint(L * 1.4426950408889634);
+  const double r = fma(-n, LN2_LO, fma(-n, LN2_HI, L));
+  double q = 1.0 / 6227020800.0;  // 1/13!
+  q = fma(q, r, 1.0 / 479001600.0);
+  q = fma(q, r, 1.0 / 39916800.0);
+  q = fma(q, r, 1.0 / 3628800.0);
+  q = fma(q, r, 1.0 / 362880.0);
+  q = fma(q, r, 1.0 / 40320.0);
+  q = fma(q, r, 1.0 / 5040.0);
+  q = fma(q, r, 1.0 / 720.0);
+  q = fma(q, r, 1.0 / 120.0);
+  q = fma(q, r, 1.0 / 24.0);
+  q = fma(q, r, 1.0 / 6.0);
+  q = fma(q, r, 0.5);
+  q = fma(q, r, 1.0);
+  q = fma(q, r, 1.0);
+  return ldexp(q, int(n));
+}
 
 template <class R>
 struct V3 {

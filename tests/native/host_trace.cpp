@@ -97,4 +97,9 @@ int host_trace_fp32(const lmn_scene_desc* desc, const lmn_ray* rays, size_t n, u
   return LMN_OK;
 }
 
+// The exact-mode FP64 power used by the superellipsoid root finder.
+void host_pow_pos(const double* x, const double* y, double* out, size_t n) {
+  for (size_t i = 0; i < n; ++i) out[i] = pow_pos(x[i], y[i]);
+}
+
 }  // extern "C"

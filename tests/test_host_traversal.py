@@ -112,3 +112,68 @@ def test_superellipsoid_exponent_range(harness):
     # the oracle samples 8192 points per chord; the interval walk finds every
     # crossing, so the two differ only on features thinner than that
     assert r["bad_t"] <= int(5e-4 * r["n"]) and r["flips"] <= int(5e-4 * r["n"])
+
+
+def test_superellipsoid_grazing_rays(harness):
+    # rays leaving a superellipsoid's surface at shallow angles (the shadow
+    # and bounce rays of its own shading): the sub-chords near the origin lie
+    # within 1e-6..1e-3 of the surface, where only the mean-value bound can
+    # rule intervals out; outward and inward tilts, convex and star-shaped
+    body = ""
+    shapes = [(0.36167, 2.18859), (0.3, 0.3), (2.5, 0.4), (1.0, 1.0)]
+    for k, (e1, e2) in enumerate(shapes):
+        body += (f"[[superellipsoid]]\ncenter = [{3.0 * k}, 0.35, 0.0]\nscale = [0.35, 0.35, 0.35]\n"
+                 f"e1 = {e1}\ne2 = {e2}\n" 'material = { type = "lambertian", color = [0.5, 0.5, 0.5] }\n')
+    _, _, world = lm.load_scene(body)
+    rng = np.random.default_rng(11)
+    n = 4000
+    k = rng.integers(0, len(shapes), n)
+    c = np.stack([3.0 * k, np.full(n, 0.35), np.zeros(n)], axis=1)
+    o = c + rng.normal(size=(n, 3)) * 2.0
+    aim = np.concatenate([o, c + rng.uniform(-0.1, 0.1, (n, 3)) - o, np.full((n, 1), 0.001),
+                          np.full((n, 1), np.inf)], axis=1)
+    h0 = oracle.trace(world, aim, exact=True)
+    ok = h0["prim"] >= 0
+    p = aim[ok, :3] + aim[ok, 3:6] * h0["t"][ok, None]
+    nrm = h0["normal"][ok] * np.where(h0["front_face"][ok], 1.0, -1.0)[:, None]  # outward
+    tang = np.cross(nrm, rng.normal(size=nrm.shape))
+    tang /= np.linalg.norm(tang, axis=1, keepdims=True)
+    theta = 10.0 ** rng.uniform(-4, -1.3, len(p)) * rng.choice([-1.0, 1.0], len(p))
+    d = np.cos(theta)[:, None] * tang + np.sin(theta)[:, None] * nrm
+    rays = np.concatenate([p, d, np.full((len(p), 1), 0.001), np.full((len(p), 1), np.inf)], axis=1)
+    rays = rays.astype(np.float32).astype(np.float64)
+    rays[:, 7] = np.inf
+    ho = oracle.trace(world, rays, exact=True)
+    hh, _ = host_trace(harness, world, rays, exact=True)
+    r = compare_hits(rays, ho, hh, fp32=True)
+    print(len(rays), r)
+    assert len(rays) > 2500 and r["hits"] > 500
+    # a grazing chord may touch the surface between the oracle's 8192 samples
+    assert r["bad_t"] <= int(2e-3 * r["n"]) and r["flips"] <= int(5e-3 * r["n"])
+
+
+def test_pow_pos_accuracy(harness):
+    # x^y for x >= 0, y > 0 (the exact-mode superellipsoid's power) against
+    # the C library: a few ulp times max(1, |y ln x|)
+    L = harness
+    L.host_pow_pos.argtypes = [C.POINTER(C.c_double)] * 3 + [C.c_size_t]
+    rng = np.random.default_rng(4)
+    n = 400000
+    x = np.concatenate([rng.uniform(0.0, 2.2, n // 2), 2.2 * 10.0 ** rng.uniform(-300, 0, n // 2)])
+    y = rng.uniform(0.1, 10.0, n)
+    x = np.concatenate([x, [0.0, 1.0, 0.5, 2.0, np.inf, np.nan, 1e-310, 5e-324]])
+    y = np.concatenate([y, [2.5, 7.0, 2.0, 3.0, 2.0, 2.0, 0.5, 0.3]])
+    out = np.empty_like(x)
+    ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    L.host_pow_pos(ptr(x), ptr(y), ptr(out), len(x))
+    ref = np.power(x, y)
+    assert out[-8] == 0.0 and out[-7] == 1.0
+    assert out[-6] == pytest.approx(0.25, rel=4 * 2.0 ** -52) and out[-5] == pytest.approx(8.0, rel=4 * 2.0 ** -52)
+    assert np.isinf(out[-4]) and np.isnan(out[-3])
+    fin = np.isfinite(ref) & (ref > 1e-300)
+    lnx = np.log(np.where(x > 0, x, 1.0))
+    err = np.abs(out[fin] - ref[fin]) / ref[fin] / (np.maximum(1.0, np.abs(y * lnx)[fin]) * 2.0 ** -52)
+    print("max error (ulp * max(1, |y ln x|))", err.max())
+    assert err.max() < 4.0
+    tiny = ~fin & np.isfinite(ref)
+    assert np.all(np.abs(out[tiny] - ref[tiny]) <= 1e-300)
