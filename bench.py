@@ -33,6 +33,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(workload, kernel="k_extend"):
+    """HBM-side bytes per launch of `kernel` on `workload`, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
+    calibration recorded there), or None when no summary covers it: PMC
+    counters need a profiler run of their own and cannot be read here."""
+    import glob
+    import re
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"{kernel}_*.md"))):
+        m = re.search(r"<!-- traffic (\{.*\}) -->", open(path).read())
+        if not m:
+            continue
+        rec = json.loads(m.group(1))
+        if rec.get("workload") == workload and rec.get("kernel") == kernel:
+            return rec["bytes_per_launch"], os.path.relpath(path, ROOT), rec.get("l2_hit_rate")
+    return None, None, None
+
+
 def build_workload(name, n_scenes=64):
     """Returns (workload name, [(config, world)], description)."""
     import luminara_amd as lm
@@ -236,8 +253,10 @@ def main():
         bytes_per_launch = alg_bytes / launches
         avg_launch_s = ext_ms / launches / 1e3
         achieved = bytes_per_launch / avg_launch_s
+        traffic, traffic_src, l2_hit = measured_traffic(wl_name)
         roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_extend",
+                    "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_unit": "bytes per launch",
+                    "traffic_source": traffic_src, "l2_hit_rate": l2_hit, "kernel": "k_extend",
                     "avg_launch_us": avg_launch_s * 1e6, "launches_per_step": launches,
                     "bytes_per_launch": bytes_per_launch,
                     "bytes_model": "128 B per 4-wide BVH node fetched + 64 B per primitive record tested "
