@@ -223,21 +223,35 @@ def main():
     if not args.pool and not args.stack_lds and not args.no_tune:
         tuning = lm.autotune(worlds, params("wavefront"), ptrs)
         log(f"[bench] autotune: {tuning}")
+    # Timed region 1 (`value`): the product path, blocks of iterations
+    # replayed as hipGraphs.  Timed region 2: the same renders launched
+    # directly with HIP events between the stages (events recorded by graph
+    # nodes cannot be timed), for the per-stage times and k_extend's average
+    # launch duration on its stream.
+    p_graph = params("wavefront")
     p32 = params("wavefront", _abi.FLAG_STAGE_TIMING)
     for _ in range(args.warmup):
+        lm.render_batch_device(worlds, p_graph, ptrs)
         lm.render_batch_device(worlds, p32, ptrs)
     torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        lm.render_batch_device(worlds, p_graph, ptrs)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
     stats = []
     t_start = time.perf_counter()
     for _ in range(args.steps):
         stats.append(lm.render_batch_device(worlds, p32, ptrs))
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
+    elapsed_direct = time.perf_counter() - t_start
     paths = stats[0]["paths"]
     ms_per_step = elapsed * 1e3 / args.steps
     value = paths * args.steps / elapsed / 1e6
+    value_direct = paths * args.steps / elapsed_direct / 1e6
     imgs32 = [o.double().cpu().numpy() for o in outs]
-    log(f"[bench] wavefront FP32: {ms_per_step:.1f} ms/step, {value:.1f} Mpaths/s, stats {stats[-1]}")
+    log(f"[bench] wavefront FP32: {ms_per_step:.1f} ms/step, {value:.1f} Mpaths/s (graph launches); "
+        f"{value_direct:.1f} Mpaths/s launched directly with stage events; stats {stats[-1]}")
 
     ext_ms = sum(s["extend_ms"] for s in stats) / args.steps
     launches = sum(s["extend_launches"] for s in stats) / args.steps
@@ -266,7 +280,8 @@ def main():
         log(f"[bench] extend: {counts}, {ext_ms:.2f} ms/step over {launches:.0f} launches, "
             f"{achieved / 1e9:.0f} GB/s algorithmic")
 
-    extra = {"stage_ms": {"extend": ext_ms, "shade": sum(s["shade_ms"] for s in stats) / args.steps,
+    extra = {"value_direct_launch": value_direct,
+             "stage_ms": {"extend": ext_ms, "shade": sum(s["shade_ms"] for s in stats) / args.steps,
                           "shadow": sum(s["shadow_ms"] for s in stats) / args.steps,
                           "regen": sum(s["regen_ms"] for s in stats) / args.steps,
                           "marched_kinds": sum(s["march_ms"] for s in stats) / args.steps}}

@@ -684,10 +684,11 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
     return LMN_ERR_INVALID;
   }
   auto build_tree = [&](const std::vector<uint32_t>& ids, uint32_t rec_base, std::vector<NodeD>& nodesD,
-                        std::vector<NodeF>& nodesF) {
+                        std::vector<NodeF>& nodesF, int max_leaf) {
     std::vector<double> blo(ids.size() * 3), bhi(ids.size() * 3);
     Builder B(blo, bhi);
     B.opt = opt;
+    B.opt.max_leaf = std::min(opt.max_leaf, max_leaf);
     B.refs.resize(ids.size());
     for (size_t i = 0; i < ids.size(); ++i) {
       prim_world_bounds(desc, desc.prims[ids[i]], &blo[i * 3], &bhi[i * 3]);
@@ -723,8 +724,10 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
   out.order.clear();
   out.tree_depth = 0;
   out.n_leaves = 0;
-  build_tree(bounded, 0, out.nodesD, out.nodesF);
-  build_tree(marched, uint32_t(bounded.size()), out.mnodesD, out.mnodesF);
+  build_tree(bounded, 0, out.nodesD, out.nodesF, opt.max_leaf);
+  // the marched kinds' root finders cost far more than a box test: one
+  // primitive per leaf, so a ray reaches only those whose own box it enters
+  build_tree(marched, uint32_t(bounded.size()), out.mnodesD, out.mnodesF, 1);
   if (bounded.size() + marched.size() > LEAF4_FIRST) {
     err = "too many primitives for the 4-wide BVH";
     return LMN_ERR_INVALID;
@@ -737,6 +740,7 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
   out.primsF.resize(nrec);
   out.primsD.resize(nrec);
   out.featF = out.featD = 0;
+  out.march_kinds = 0;
   for (size_t i = 0; i < nrec; ++i) {
     const lmn_prim& p = desc.prims[out.order[i]];
     bool isl = fp64_island(p);
@@ -745,6 +749,9 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
     if (p.transform >= 0) out.featF |= FEAT_XF, out.featD |= FEAT_XF;
     if (isl) out.featF |= FEAT_F64;
     if (p.kind >= LMN_PRIM_TORUS) out.featF |= FEAT_MARCH, out.featD |= FEAT_MARCH;
+    if (p.kind == LMN_PRIM_TORUS) out.march_kinds |= 1u;
+    if (p.kind == LMN_PRIM_SUPERELLIPSOID) out.march_kinds |= 2u;
+    if (p.kind == LMN_PRIM_SPRING) out.march_kinds |= 4u;
   }
   out.xfD.resize(desc.n_transforms);
   out.xfF.resize(desc.n_transforms);

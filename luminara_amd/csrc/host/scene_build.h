@@ -41,6 +41,7 @@ struct HostScene {
   CameraT<double> camD;
   uint32_t n_bounded = 0, n_planes = 0;
   uint32_t featF = 0, featD = 0;  // FEAT_* of the FP32 and FP64 paths
+  uint32_t march_kinds = 0;        // bit k: marched kind K_TORUS + k present
   uint32_t tree_depth = 0, n_leaves = 0;
   std::vector<uint32_t> order;  // order[i] = descriptor prim index of record i
   std::vector<uint32_t> slot_mat;  // slot_mat[j] = material of roughness slot j (microfacet materials)

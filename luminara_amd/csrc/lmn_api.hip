@@ -74,6 +74,7 @@ struct Workspace {
   DevBuf st[2][4];  // ob, db, bk, meta for the two pools
   DevBuf hit, hit_b, sh_o, sh_d, sh_c, samples, acc, spill, ctr, tc, rays, out, views, parts, part_begin;
   DevBuf adj_wsum, adj_gtex, adj_part, adj_red, adj_dldi, adj_tex_out;  // path-replay adjoint
+  DevBuf mq_cand, mq_key, mq_ovf;  // marched-pass candidate queues
   uint32_t pool = 0;
   bool graph_failed[2] = {false, false};  // capture refused once [without, with stage events]: launch directly
   WaveCounters* host_ctr = nullptr;
@@ -530,6 +531,14 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
   for (size_t i = 0; i < n; ++i) stack_words = std::max(stack_words, scenes[i]->host.stack4_words);
   HIP_TRY(W.spill.ensure(size_t(stack_words) * nthreads * 4));
   HIP_TRY(W.ctr.ensure(sizeof(WaveCounters)));
+  // marched-pass queues: per kind one candidate slot per pool path
+  uint32_t march_kinds = 0;
+  for (size_t i = 0; i < n; ++i) march_kinds |= scenes[i]->host.march_kinds;
+  if (feat & FEAT_MARCH) {
+    HIP_TRY(W.mq_cand.ensure(size_t(pool) * 4 * sizeof(uint2)));
+    HIP_TRY(W.mq_key.ensure(size_t(pool) * 8));
+    HIP_TRY(W.mq_ovf.ensure(size_t(pool) * 4));
+  }
   // per-scene accumulators, scene views, parts
   std::vector<size_t> acc_off(n + 1, 0);
   for (size_t i = 0; i < n; ++i) acc_off[i + 1] = acc_off[i] + (fs[i].black ? 0 : fs[i].npix);
@@ -558,6 +567,9 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
   L.samples = W.samples.as<float4>();
   L.ctr = W.ctr.as<WaveCounters>();
   L.spill = W.spill.as<uint32_t>();
+  L.mq = MarchQ{};
+  if (feat & FEAT_MARCH)
+    L.mq = MarchQ{W.mq_cand.as<uint2>(), W.mq_key.as<unsigned long long>(), W.mq_ovf.as<uint32_t>(), pool, march_kinds};
   L.tc = d_tc;
   for (size_t i = 0; i < n; ++i)
     if (!fs[i].black) launch_zero_acc(W.acc.as<double4>() + acc_off[i], fs[i].npix, st);

@@ -582,15 +582,17 @@ LMN_HD int se_mean_value_sign(V3<double> o, V3<double> d, double a, double b, fl
   return 0;
 }
 
-LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax,
-                                       double& t) {
-  V3<double> o = {(ro.x - p[0]) / p[3], (ro.y - p[1]) / p[4], (ro.z - p[2]) / p[5]};
-  V3<double> d = {rd.x / p[3], rd.y / p[4], rd.z / p[5]};
-  // clip to the unit box |q|_inf <= 1 that contains the surface, padded so
-  // that a ray entering where the surface touches the box (the axis points)
-  // starts strictly outside and sees the sign change
+// Object-space ray of a superellipsoid and its chord [t0, t1] through the
+// unit box |q|_inf <= 1 that contains the surface, padded so that a ray
+// entering where the surface touches the box (the axis points) starts
+// strictly outside and sees the sign change; false if the chord is empty.
+LMN_HD bool se_chord(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax, V3<double>& o,
+                     V3<double>& d, double& t0, double& t1) {
+  o = {(ro.x - p[0]) / p[3], (ro.y - p[1]) / p[4], (ro.z - p[2]) / p[5]};
+  d = {rd.x / p[3], rd.y / p[4], rd.z / p[5]};
   const double B = 1.0 + 1e-9;
-  double t0 = tmin, t1 = tmax;
+  t0 = tmin;
+  t1 = tmax;
   const double* oo = &o.x;
   const double* dd = &d.x;
   for (int k = 0; k < 3; ++k) {
@@ -600,10 +602,60 @@ LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double
     t0 = fmax(t0, a);
     t1 = fmin(t1, b);
   }
-  if (!(t0 <= t1)) return false;
+  return t0 <= t1;
+}
+
+constexpr float kSeScreenMargin = 5e-4f;
+
+// FP32 screen of the interval walk below, on its own: false where the
+// screened bounds rule out every sub-chord down to `max_level`, i.e. where
+// hit_superellipsoid_exact finds no crossing either (the screen only rules
+// out sub-chords whose range excludes 1 with the margin).  The marched
+// pass runs it on every candidate and the exact walk only on the rest,
+// whose costs are closer to each other.
+LMN_HD bool se_may_cross(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax, int max_level) {
+  V3<double> o, d;
+  double t0, t1;
+  if (!se_chord(p, ro, rd, tmin, tmax, o, d, t0, t1)) return false;
+  const float pxf = float(2.0 / p[7]), qaf = float(p[7] / p[6]), pyf = float(2.0 / p[6]);
+  const double w = t1 - t0;
+  int level = 0;
+  uint32_t idx = 0;
+  for (;;) {
+    double scale = w / double(1u << level);
+    double a = t0 + double(idx) * scale, b = (idx + 1u == (1u << level)) ? t1 : a + scale;
+    double xl, xh, yl, yh, zl, zh;
+    se_range1(o.x, d.x, a, b, xl, xh);
+    se_range1(o.y, d.y, a, b, yl, yh);
+    se_range1(o.z, d.z, a, b, zl, zh);
+    float slo = pow_screen(pow_screen(float(xl), pxf) + pow_screen(float(zl), pxf), qaf) + pow_screen(float(yl), pyf);
+    float shi = pow_screen(pow_screen(float(xh), pxf) + pow_screen(float(zh), pxf), qaf) + pow_screen(float(yh), pyf);
+    if (!(slo > 1.0f + kSeScreenMargin || shi < 1.0f - kSeScreenMargin)) {
+      if (level == max_level) return true;
+      ++level;
+      idx *= 2u;
+      continue;
+    }
+    while (idx & 1u) {
+      idx >>= 1;
+      --level;
+    }
+    if (level == 0) return false;
+    ++idx;
+  }
+}
+
+// ANY: visibility only.  The root of a sign-change leaf lies inside the
+// leaf, which lies inside [tmin, tmax], so the bracket answers the query
+// and the refinement is skipped.
+template <bool ANY>
+LMN_HD bool se_exact_walk(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax, double& t) {
+  V3<double> o, d;
+  double t0, t1;
+  if (!se_chord(p, ro, rd, tmin, tmax, o, d, t0, t1)) return false;
   const double px = 2.0 / p[7], qa = p[7] / p[6], py = 2.0 / p[6];
   const float pxf = float(px), qaf = float(qa), pyf = float(py);
-  const float kScreenMargin = 5e-4f;
+  const float kScreenMargin = kSeScreenMargin;
   const int kSolveLevel = 12, kMaxLevel = 24, kMeanValueLevel = 4;
   const double w = t1 - t0;
   double fa = se_val(o + d * t0, px, qa, py) - 1.0;  // sign at the left end of the current interval
@@ -652,6 +704,10 @@ LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double
         continue;
       }
       if ((fa > 0.0) != (fb > 0.0)) {
+        if (ANY) {
+          t = 0.5 * (a + b);
+          return true;
+        }
         // Newton's method from the secant point, kept inside the bracket
         // [lo, hi] (which always holds the root) by bisection
         double lo = a, hi = b, flo = fa;
@@ -691,6 +747,15 @@ LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double
     if (level == 0) return false;
     ++idx;
   }
+}
+LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax,
+                                       double& t) {
+  return se_exact_walk<false>(p, ro, rd, tmin, tmax, t);
+}
+LMN_COLD bool hit_superellipsoid_exact_any(const double* p, V3<double> ro, V3<double> rd, double tmin,
+                                           double tmax) {
+  double t;
+  return se_exact_walk<true>(p, ro, rd, tmin, tmax, t);
 }
 
 LMN_HD double spring_sdf(V3<double> q, const double* p) {

@@ -36,7 +36,30 @@ struct WaveCounters {
   uint64_t stat_shadow;  // any-hit queries issued
   uint64_t stat_camera;  // camera rays generated
   uint64_t iterations;
+  uint32_t n_cand[2][4];  // marched candidates per kind + screened superellipsoids, [extend, shadow]
+  uint32_t n_ovf[2];      // rays left to the marched fallback, [extend, shadow]
 };
+
+// Candidate queues of the marched pass.  A ray lists the marched
+// primitives whose boxes it enters (within the main pass's hit distance);
+// each kind (torus, superellipsoid, spring) has `cap` entries (ray index,
+// primitive record) and is tested by a kernel of its own with one candidate
+// per lane; superellipsoid candidates pass an FP32 screen first, and the
+// survivors (region 3) get the exact walk.  `key` holds per ray the closest
+// marched hit (t bits << 32 |
+// record, MARCH_EMPTY: none) or, for shadow rays, a nonzero occluded flag.
+// Rays with more than MARCH_MAX_CAND candidates, or whose candidates do not
+// fit, are listed in `ovf` and traced whole by the fallback kernel.
+struct MarchQ {
+  uint2* cand;               // 4 * cap entries; region r at cand + r * cap
+  unsigned long long* key;   // per path / shadow ray
+  uint32_t* ovf;             // per path / shadow ray
+  uint32_t cap;
+  uint32_t kinds;            // bit k: kind K_TORUS + k present in the batch
+};
+constexpr unsigned long long MARCH_EMPTY = ~0ull;
+constexpr uint32_t MARCH_SKIP = 0xffffffffu;  // reserved slot left unused
+constexpr uint32_t MARCH_MAX_CAND = 16;
 
 // Scenes of one wavefront launch: per-scene views (device array), the
 // chunk's parts (FrameDev each) and their first work items (n_parts + 1
@@ -71,6 +94,7 @@ struct WaveLaunch {
   WaveCounters* ctr;
   uint32_t* spill;
   TraceCounters* tc;
+  MarchQ mq;           // marched pass (scenes with torus/superellipsoid/spring)
 };
 
 // One wavefront iteration: extend, shade, shadow, regenerate, bookkeeping.

@@ -158,40 +158,10 @@ __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_extend_count(Bat
   extend_body<SS, BS, FEAT, true>(B, cur, hit, ctr, spill, tc);
 }
 
-// The marched kernels' FP64 root finders would take every register a wave
-// can have (one wave per SIMD); capped at 4 waves per SIMD they spill some
-// and run about a third faster.
-#define LMN_MARCH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 8)))
-
-// Extend, marched pass: the FP64 torus/superellipsoid/spring BVH, limited
-// to the main pass's hit distance.
-template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_extend_march(BatchDev B, PathSoA cur, HitSoA hit,
-                                                     const WaveCounters* __restrict__ ctr, uint32_t* spill) {
-  __shared__ uint32_t lds[SS * BS];
-  uint32_t n = ctr->active;
-  uint32_t nthreads = gridDim.x * BS;
-  uint32_t gid = blockIdx.x * BS + threadIdx.x;
-  for (uint32_t i = gid; i < n; i += nthreads) {
-    float4 a = cur.ob[i], b = cur.db[i];
-    uint32_t sid = cur.aux[i].z;
-    for_scene(sid, [&](uint32_t s) {
-      const SceneView<float> S = B.views[s];
-      if (S.n_mnodes == 0) return;
-      float4 h0 = hit.b[i];
-      LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
-      float t = __float_as_uint(h0.z) == HIT_MISS ? INFINITY : h0.w;
-      uint32_t idx = 0;
-      if (trace_pass<float, FEAT, false, PASS_MARCH>(S, f3(a), f3(b), 0.001f, t, idx, st))
-        store_hit<FEAT>(S, hit, i, f3(a), f3(b), t, idx);
-    });
-  }
-}
-
 // Shadow rays: any hit over [0.001, dist - 0.001] (src/render.rs:1457-1458).
 // Without marched kinds the contribution of a visible light sample is added
-// here; with them an occluded ray's distance is negated and k_shadow_march
-// finishes the query.
+// here; with them an occluded ray's distance is negated and the marched
+// pass finishes the query.
 template <int SS, int BS, uint32_t FEAT>
 __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_shadow(BatchDev B, ShadowSoA sh, float4* __restrict__ samples,
                                                const WaveCounters* __restrict__ ctr, uint32_t* spill) {
@@ -218,25 +188,237 @@ __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_shadow(BatchDev 
   }
 }
 
-template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_shadow_march(BatchDev B, ShadowSoA sh, float4* __restrict__ samples,
-                                                     const WaveCounters* __restrict__ ctr, uint32_t* spill) {
+// ---------------------------------------------------------------------------
+// Marched pass (scenes with torus / superellipsoid / spring records): the
+// main pass leaves each ray's hit distance (extend) or visibility (shadow);
+// the marched kinds' FP64 root finders run after it, compacted:
+//   collect  - every ray walks the marched BVH (FP32 boxes up to that
+//              distance) and queues (ray, record) per kind;
+//   test     - one kernel per kind, one candidate per lane: closest hit per
+//              ray by a 64-bit atomic min on (t bits, record), or the
+//              occluded flag;
+//   fallback - rays with too many candidates, traced whole as before;
+//   finish   - surface record of the marched hit / the shadow ray's sample.
+// Tested in place, the root finders ran on the few lanes of a wave whose
+// rays had entered a marched box, with the rest idle (8 % lane use).
+
+// Ray `i` of the extend (path pool) or shadow (queue) stage: origin,
+// direction, scene, range end; `live` false for shadow rays the main pass
+// found occluded.
+template <bool SHADOW>
+__device__ __forceinline__ bool march_ray(const PathSoA& cur, const HitSoA& hit, const ShadowSoA& sh, uint32_t i,
+                                          V3<float>& o, V3<float>& d, uint32_t& sid, float& tmax) {
+  if (SHADOW) {
+    float4 o4 = sh.o[i], d4 = sh.d[i];
+    o = f3(o4);
+    d = f3(d4);
+    sid = __float_as_uint(sh.c[i].w);
+    tmax = o4.w - 0.001f;
+    return o4.w > 0.f;
+  }
+  float4 o4 = cur.ob[i], d4 = cur.db[i], h = hit.b[i];
+  o = f3(o4);
+  d = f3(d4);
+  sid = cur.aux[i].z;
+  tmax = __float_as_uint(h.z) == HIT_MISS ? INFINITY : h.w;
+  return true;
+}
+
+template <int SS, int BS, uint32_t FEAT, bool SHADOW>
+__global__ __launch_bounds__(BS) void k_march_collect(BatchDev B, PathSoA cur, HitSoA hit, ShadowSoA sh, MarchQ q,
+                                                      WaveCounters* __restrict__ ctr, uint32_t* spill) {
   __shared__ uint32_t lds[SS * BS];
-  uint32_t n = ctr->n_shadow;
-  uint32_t nthreads = gridDim.x * BS;
-  uint32_t gid = blockIdx.x * BS + threadIdx.x;
-  for (uint32_t i = gid; i < n; i += nthreads) {
-    float4 o = sh.o[i];
-    if (!(o.w > 0.f)) continue;  // occluded in the main pass
-    float4 d = sh.d[i], c = sh.c[i];
-    uint32_t sid = __float_as_uint(c.w);
+  __shared__ uint32_t cnt[3], base[3];
+  const uint32_t n = SHADOW ? ctr->n_shadow : ctr->active;
+  const uint32_t nthreads = gridDim.x * BS;
+  const uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  uint32_t* qn = ctr->n_cand[SHADOW ? 1 : 0];
+  // block-uniform rounds: every lane reaches the barriers
+  for (uint32_t r0 = blockIdx.x * BS; r0 < n; r0 += nthreads) {
+    const uint32_t i = r0 + threadIdx.x;
+    uint32_t c[MARCH_MAX_CAND];  // record | kind << 30
+    uint32_t nc = 0, nk0 = 0, nk1 = 0, nk2 = 0;
+    bool over = false;
+    V3<float> o, d;
+    uint32_t sid;
+    float tmax;
+    if (i < n && march_ray<SHADOW>(cur, hit, sh, i, o, d, sid, tmax)) {
+      q.key[i] = SHADOW ? 0ull : MARCH_EMPTY;
+      for_scene(sid, [&](uint32_t s) {
+        const SceneView<float> S = B.views[s];
+        if (S.n_mnodes == 0) return;
+        LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
+        V3<float> inv = {1.f / d.x, 1.f / d.y, 1.f / d.z};
+        bvh_leaves(S.mnodes, o, inv, 0.001f, tmax, st, [&](uint32_t rec) {
+          if (nc == MARCH_MAX_CAND) {
+            over = true;
+            return;
+          }
+          uint32_t k = (S.prims[rec].kind_mat & KIND_MASK) - K_TORUS;
+          c[nc++] = rec | (k << 30);
+          nk0 += k == 0u;
+          nk1 += k == 1u;
+          nk2 += k == 2u;
+        });
+      });
+      if (over) nc = nk0 = nk1 = nk2 = 0;
+    }
+    // slots: LDS counts per kind, then one device atomic per kind per block
+    if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t off0 = nk0 ? atomicAdd(&cnt[0], nk0) : 0u;
+    uint32_t off1 = nk1 ? atomicAdd(&cnt[1], nk1) : 0u;
+    uint32_t off2 = nk2 ? atomicAdd(&cnt[2], nk2) : 0u;
+    __syncthreads();
+    if (threadIdx.x < 3) base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&qn[threadIdx.x], cnt[threadIdx.x]) : 0u;
+    __syncthreads();
+    off0 += base[0];
+    off1 += base[1];
+    off2 += base[2];
+    const bool fits = off0 + nk0 <= q.cap && off1 + nk1 <= q.cap && off2 + nk2 <= q.cap;
+    for (uint32_t j = 0; j < nc; ++j) {
+      uint32_t k = c[j] >> 30;
+      uint32_t slot = k == 0u ? off0++ : (k == 1u ? off1++ : off2++);
+      if (slot < q.cap) q.cand[size_t(k) * q.cap + slot] = fits ? make_uint2(i, c[j] & 0x3fffffffu) : make_uint2(MARCH_SKIP, 0u);
+    }
+    if (over || !fits) q.ovf[atomicAdd(&ctr->n_ovf[SHADOW ? 1 : 0], 1u)] = i;
+    __syncthreads();  // cnt / base are rewritten next round
+  }
+}
+
+// The root finders would take every register a wave can have (one wave per
+// SIMD); capped at 4 waves per SIMD they spill some and run faster.
+#define LMN_MARCH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4, 8)))
+
+// Superellipsoid candidates: the exact walk's FP32 screen alone (down to
+// level 8); candidates it cannot rule out move to region 3.  The walk's
+// cost spreads from one node (a chord missing the body) to hundreds (rays
+// grazing it), and a wave takes as long as its slowest lane: screening
+// first leaves the exact kernel the candidates of similar, higher cost.
+template <int BS, uint32_t FEAT, bool SHADOW>
+__global__ __launch_bounds__(BS) void k_march_screen(BatchDev B, PathSoA cur, HitSoA hit, ShadowSoA sh, MarchQ q,
+                                                     WaveCounters* __restrict__ ctr) {
+  __shared__ uint32_t cnt, base;
+  uint32_t* qn = ctr->n_cand[SHADOW ? 1 : 0];
+  const uint32_t nq = qn[1];
+  const uint32_t n = nq < q.cap ? nq : q.cap;
+  const uint2* in = q.cand + size_t(1) * q.cap;
+  uint2* out = q.cand + size_t(3) * q.cap;
+  const uint32_t nthreads = gridDim.x * BS;
+  for (uint32_t r0 = blockIdx.x * BS; r0 < n; r0 += nthreads) {  // block-uniform rounds
+    const uint32_t j = r0 + threadIdx.x;
+    uint2 e = make_uint2(MARCH_SKIP, 0u);
+    bool keep = false;
+    if (j < n) e = in[j];
+    if (e.x != MARCH_SKIP) {
+      V3<float> o, d;
+      uint32_t sid;
+      float tmax;
+      march_ray<SHADOW>(cur, hit, sh, e.x, o, d, sid, tmax);
+      for_scene(sid, [&](uint32_t s) {
+        const SceneView<float> S = B.views[s];
+        const PrimT<float>& pr = S.prims[e.y];
+        V3<double> o64 = cvt<double>(o), d64 = cvt<double>(d);
+        if ((FEAT & FEAT_XF) && pr.xform != XF_NONE) {
+          const XformT<double>& X64 = S.xforms64[pr.xform];
+          o64 = xf_point(X64.inv, o64);
+          d64 = xf_dir(X64.inv, d64);
+        }
+        keep = S.exact_marched == 0 || se_may_cross(S.prims64[e.y].p, o64, d64, 0.001, double(tmax), 8);
+      });
+    }
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    uint32_t off = keep ? atomicAdd(&cnt, 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) base = cnt ? atomicAdd(&qn[3], cnt) : 0u;
+    __syncthreads();
+    if (keep) out[base + off] = e;
+    __syncthreads();  // cnt / base are rewritten next round
+  }
+}
+
+template <int BS, uint32_t FEAT, uint32_t KIND, bool SHADOW>
+__global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_test(BatchDev B, PathSoA cur, HitSoA hit,
+                                                                      ShadowSoA sh, MarchQ q,
+                                                                      const WaveCounters* __restrict__ ctr) {
+  // superellipsoids: the screened region
+  constexpr uint32_t region = KIND == K_SUPERELLIPSOID ? 3u : KIND - K_TORUS;
+  const uint32_t nq = ctr->n_cand[SHADOW ? 1 : 0][region];
+  const uint32_t n = nq < q.cap ? nq : q.cap;
+  const uint2* cand = q.cand + size_t(region) * q.cap;
+  const uint32_t nthreads = gridDim.x * BS;
+  for (uint32_t j = blockIdx.x * BS + threadIdx.x; j < n; j += nthreads) {
+    const uint2 e = cand[j];
+    if (e.x == MARCH_SKIP) continue;
+    V3<float> o, d;
+    uint32_t sid;
+    float tmax;
+    march_ray<SHADOW>(cur, hit, sh, e.x, o, d, sid, tmax);
+    for_scene(sid, [&](uint32_t s) {
+      const SceneView<float> S = B.views[s];
+      float t = tmax;
+      if (test_marched<KIND, FEAT, SHADOW>(S, e.y, o, d, 0.001f, t)) {
+        if (SHADOW)
+          q.key[e.x] = 1ull;
+        else
+          atomicMin(&q.key[e.x], (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | e.y);
+      }
+    });
+  }
+}
+
+template <int SS, int BS, uint32_t FEAT, bool SHADOW>
+__global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_fallback(BatchDev B, PathSoA cur, HitSoA hit,
+                                                                          ShadowSoA sh, MarchQ q,
+                                                                          const WaveCounters* __restrict__ ctr,
+                                                                          uint32_t* spill) {
+  __shared__ uint32_t lds[SS * BS];
+  const uint32_t n = ctr->n_ovf[SHADOW ? 1 : 0];
+  const uint32_t nthreads = gridDim.x * BS;
+  const uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  for (uint32_t j = gid; j < n; j += nthreads) {
+    const uint32_t i = q.ovf[j];
+    V3<float> o, d;
+    uint32_t sid;
+    float tmax;
+    march_ray<SHADOW>(cur, hit, sh, i, o, d, sid, tmax);
     for_scene(sid, [&](uint32_t s) {
       const SceneView<float> S = B.views[s];
       LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
-      float t = o.w - 0.001f;
+      float t = tmax;
       uint32_t idx = 0;
-      if (S.n_mnodes == 0 || !trace_pass<float, FEAT, true, PASS_MARCH>(S, f3(o), f3(d), 0.001f, t, idx, st))
-        add_sample(samples, __float_as_uint(d.w), f3(c));
+      if (trace_pass<float, FEAT, SHADOW, PASS_MARCH>(S, o, d, 0.001f, t, idx, st)) {
+        if (SHADOW)
+          q.key[i] = 1ull;
+        else
+          atomicMin(&q.key[i], (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | idx);
+      }
+    });
+  }
+}
+
+template <int BS, uint32_t FEAT, bool SHADOW>
+__global__ __launch_bounds__(BS) void k_march_finish(BatchDev B, PathSoA cur, HitSoA hit, ShadowSoA sh,
+                                                     float4* __restrict__ samples, MarchQ q,
+                                                     const WaveCounters* __restrict__ ctr) {
+  const uint32_t n = SHADOW ? ctr->n_shadow : ctr->active;
+  const uint32_t nthreads = gridDim.x * BS;
+  for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < n; i += nthreads) {
+    if (SHADOW) {
+      if (sh.o[i].w > 0.f && q.key[i] == 0ull) {
+        float4 c = sh.c[i];
+        add_sample(samples, __float_as_uint(sh.d[i].w), f3(c));
+      }
+      continue;
+    }
+    const unsigned long long key = q.key[i];
+    if (key == MARCH_EMPTY) continue;
+    const float4 o4 = cur.ob[i], d4 = cur.db[i];
+    const uint32_t sid = cur.aux[i].z;
+    for_scene(sid, [&](uint32_t s) {
+      const SceneView<float> S = B.views[s];
+      store_hit<FEAT>(S, hit, i, f3(o4), f3(d4), __uint_as_float(uint32_t(key >> 32)), uint32_t(key));
     });
   }
 }
@@ -387,6 +569,8 @@ __global__ void k_book(WaveCounters* ctr, uint32_t pool) {
   ctr->next_active = 0;
   ctr->n_shadow = 0;
   ctr->iterations += 1;
+  for (int k = 0; k < 4; ++k) ctr->n_cand[0][k] = ctr->n_cand[1][k] = 0;
+  ctr->n_ovf[0] = ctr->n_ovf[1] = 0;
 }
 
 __global__ void k_counters_init(WaveCounters* ctr, uint64_t n_items) {
@@ -401,6 +585,8 @@ __global__ void k_counters_init(WaveCounters* ctr, uint64_t n_items) {
   ctr->stat_shadow = 0;
   ctr->stat_camera = 0;
   ctr->iterations = 0;
+  for (int k = 0; k < 4; ++k) ctr->n_cand[0][k] = ctr->n_cand[1][k] = 0;
+  ctr->n_ovf[0] = ctr->n_ovf[1] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -447,6 +633,33 @@ __global__ void k_finalize(const double4* __restrict__ acc, float* __restrict__ 
 // ---------------------------------------------------------------------------
 // Launchers
 
+template <int BS, uint32_t FEAT, bool SHADOW>
+static void launch_march_tests(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur) {
+  if (L.mq.kinds & 1u)
+    hipLaunchKernelGGL((k_march_test<BS, FEAT, K_TORUS, SHADOW>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit,
+                       L.sh, L.mq, L.ctr);
+  if (L.mq.kinds & 2u) {
+    hipLaunchKernelGGL((k_march_screen<BS, FEAT, SHADOW>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.sh,
+                       L.mq, L.ctr);
+    hipLaunchKernelGGL((k_march_test<BS, FEAT, K_SUPERELLIPSOID, SHADOW>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur,
+                       L.hit, L.sh, L.mq, L.ctr);
+  }
+  if (L.mq.kinds & 4u)
+    hipLaunchKernelGGL((k_march_test<BS, FEAT, K_SPRING, SHADOW>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit,
+                       L.sh, L.mq, L.ctr);
+}
+
+template <int SS, int BS, uint32_t FEAT, bool SHADOW>
+static void launch_march(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur) {
+  hipLaunchKernelGGL((k_march_collect<SS, BS, FEAT, SHADOW>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.sh,
+                     L.mq, L.ctr, L.spill);
+  launch_march_tests<BS, FEAT, SHADOW>(L, B, cur);
+  hipLaunchKernelGGL((k_march_fallback<SS, BS, FEAT, SHADOW>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit,
+                     L.sh, L.mq, L.ctr, L.spill);
+  hipLaunchKernelGGL((k_march_finish<BS, FEAT, SHADOW>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.sh,
+                     L.samples, L.mq, L.ctr);
+}
+
 template <int SS, int BS, uint32_t FEAT>
 static void launch_stage_ss(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow,
                             hipEvent_t mid) {
@@ -455,9 +668,7 @@ static void launch_stage_ss(const WaveLaunch& L, const BatchDev& B, const PathSo
   if (shadow) {
     hipLaunchKernelGGL((k_shadow<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, L.sh, L.samples, L.ctr, L.spill);
     if (mid) (void)hipEventRecord(mid, L.stream);
-    if (MARCH)
-      hipLaunchKernelGGL((k_shadow_march<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, L.sh, L.samples,
-                         L.ctr, L.spill);
+    if constexpr (MARCH) launch_march<SS, BS, FEAT, true>(L, B, cur);
     return;
   }
   if (L.tc)
@@ -466,9 +677,7 @@ static void launch_stage_ss(const WaveLaunch& L, const BatchDev& B, const PathSo
   else
     hipLaunchKernelGGL((k_extend<SS, BS, MAIN>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.ctr, L.spill);
   if (mid) (void)hipEventRecord(mid, L.stream);
-  if (MARCH)
-    hipLaunchKernelGGL((k_extend_march<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.ctr,
-                       L.spill);
+  if constexpr (MARCH) launch_march<SS, BS, FEAT, false>(L, B, cur);
 }
 template <uint32_t FEAT>
 static void launch_stage(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow, hipEvent_t mid) {

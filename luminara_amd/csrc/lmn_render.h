@@ -193,6 +193,86 @@ LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t
   return true;
 }
 
+// test_prim for an FP32-path record of the marched kind KIND, for kernels
+// that test one kind per launch: the same FP64 evaluation, transform and
+// reported-t rules, with only that kind's root finder compiled in.  ANY:
+// visibility only (closest is then left unchanged on a superellipsoid hit
+// in exact mode, whose any-hit query stops at the bracketing leaf).
+template <uint32_t KIND, uint32_t FEAT, bool ANY = false>
+LMN_HD bool test_marched(const SceneView<float>& S, uint32_t idx, V3<float> o, V3<float> d, float tmin,
+                         float& closest) {
+  const PrimT<float>& pr = S.prims[idx];
+  const bool has_xf = (FEAT & FEAT_XF) && pr.xform != XF_NONE;
+  V3<double> o64 = cvt<double>(o), d64 = cvt<double>(d);
+  V3<float> dd = d;
+  if (has_xf) {
+    const XformT<double>& X64 = S.xforms64[pr.xform];
+    o64 = xf_point(X64.inv, o64);
+    d64 = xf_dir(X64.inv, d64);
+    dd = xf_dir(S.xforms[pr.xform].inv, d);
+  }
+  const double* p = S.prims64[idx].p;
+  const bool exact = S.exact_marched != 0;
+  double t64;
+  bool hit;
+  if constexpr (KIND == K_TORUS)
+    hit = exact ? hit_torus_exact(p, o64, d64, double(tmin), double(closest), t64)
+                : hit_torus_march(p, o64, d64, double(tmin), double(closest), t64);
+  else if constexpr (KIND == K_SUPERELLIPSOID && ANY) {
+    if (exact) return hit_superellipsoid_exact_any(p, o64, d64, double(tmin), double(closest));
+    hit = hit_superellipsoid_march(p, o64, d64, double(tmin), double(closest), t64);
+  } else if constexpr (KIND == K_SUPERELLIPSOID)
+    hit = exact ? hit_superellipsoid_exact(p, o64, d64, double(tmin), double(closest), t64)
+                : hit_superellipsoid_march(p, o64, d64, double(tmin), double(closest), t64);
+  else
+    hit = exact ? hit_spring_exact(p, o64, d64, double(tmin), double(closest), t64)
+                : hit_spring_march(p, o64, d64, double(tmin), double(closest), t64);
+  if (!hit) return false;
+  float t = float(t64);
+  if (!(t <= closest)) return false;  // rounding back to FP32 must not leave the range
+  if (has_xf) {
+    const XformT<float>& X = S.xforms[pr.xform];
+    if (X.tmode != 0) {
+      float tr = t * t_report_factor(X, dd, d);
+      if (!(tr <= closest)) return false;
+      t = tr;
+    }
+  }
+  closest = t;
+  return true;
+}
+
+// Every primitive record in the leaves of a binary BVH whose boxes the ray
+// enters within [tmin, tmax], in no particular order: `f(record)` per record.
+template <class Stack, class F>
+LMN_HD void bvh_leaves(const NodeF* nodes, V3<float> o, V3<float> inv, float tmin, float tmax, Stack& st, F&& f) {
+  NodeF nd = nodes[0];
+  float te;
+  bool live = NodeOps<float>::test(nd, o, inv, tmin, tmax, te);
+  while (live) {
+    if (nd.b & LEAF_BIT) {
+      for (uint32_t k = 0; k < (nd.b & ~LEAF_BIT); ++k) f(nd.a + k);
+    } else {
+      const uint32_t c = nd.a;  // children at c, c + 1
+      NodeF L = nodes[c], Rn = nodes[c + 1];
+      float tl, tr;
+      bool hl = NodeOps<float>::test(L, o, inv, tmin, tmax, tl);
+      bool hr = NodeOps<float>::test(Rn, o, inv, tmin, tmax, tr);
+      if (hl && hr) {
+        st.push(c + 1);
+        nd = L;
+        continue;
+      }
+      if (hl || hr) {
+        nd = hl ? L : Rn;
+        continue;
+      }
+    }
+    live = !st.empty();
+    if (live) nd = nodes[st.pop()];
+  }
+}
+
 // One BVH walk.  Ordered BVH2 traversal: the two children of a node are
 // stored side by side, so one contiguous 64-byte fetch (FP32) brings both
 // boxes; both are tested, the nearer is visited first and the farther

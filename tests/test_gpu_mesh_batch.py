@@ -139,3 +139,40 @@ def test_batch_matches_individual_renders(gpu):
         single = lm.render(c, None, w, scene_key=i)
         assert np.array_equal(outs[i], single.rows), i
     assert st["paths"] == sum(c.width * c.height * 4 for c in cfgs)
+
+
+def test_marched_wavefront_equals_plain_fp32(gpu):
+    # the wavefront's compacted marched pass (candidate queues, one kind per
+    # launch, closest hit by atomic min) against the plain FP32 kernel's
+    # in-place marched traversal: the same FP32 hits per path
+    cfg, cam, world = lm.load_scene(MARCHED)
+    cfg.samples_per_pixel = 16
+    a = lm.render(cfg, cam, world, path="wavefront", intersect_mode=_abi.ISECT_EXACT)
+    b = lm.render(cfg, cam, world, path="plain32", intersect_mode=_abi.ISECT_EXACT)
+    np.testing.assert_allclose(a.rows, b.rows, rtol=1e-6, atol=1e-7)
+    assert a.stats["extension_rays"] == b.stats["extension_rays"]
+    assert a.stats["shadow_rays"] == b.stats["shadow_rays"]
+
+
+def _torus_row(n):
+    # n tori one behind the other along the camera's view: camera rays cross
+    # up to n torus boxes, more than a ray may queue
+    body = MARCHED.split("[[torus]]")[0]
+    for k in range(n):
+        body += ("[[torus]]\n" f"center = [0.0, 0.3, {1.0 - 0.15 * k}]\n" "major_radius = 0.5\nminor_radius = 0.15\n"
+                 'material = { type = "lambertian", color = [0.7, 0.5, 0.3] }\n')
+    body += '[[light]]\nposition = [0.0, 3.0, 2.0]\nradius = 0.3\nintensity = 30.0\n'
+    return body
+
+
+@pytest.mark.parametrize("pool", [0, 4096])
+def test_marched_overflow_fallback(gpu, pool):
+    # rays with more candidates than the per-ray limit, and (small pool)
+    # candidates beyond the queue capacity, go to the fallback kernel; the
+    # image must still be the plain FP32 kernel's
+    cfg, cam, world = lm.load_scene(_torus_row(16))
+    cfg.samples_per_pixel = 4
+    a = lm.render(cfg, cam, world, path="wavefront", intersect_mode=_abi.ISECT_EXACT, pool_paths=pool)
+    b = lm.render(cfg, cam, world, path="plain32", intersect_mode=_abi.ISECT_EXACT)
+    np.testing.assert_allclose(a.rows, b.rows, rtol=1e-6, atol=1e-7)
+    assert a.stats["extension_rays"] == b.stats["extension_rays"]
