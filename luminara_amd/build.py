@@ -2,7 +2,8 @@
 
 Host sources are compiled with g++, HIP sources with hipcc
 (--offload-arch=gfx950), in parallel, then linked into
-luminara_amd/_lib/liblmn.so.  Always rebuilds.
+luminara_amd/_lib/liblmn.so; the command line front end
+(luminara_amd/_lib/luminara-gpu) links against it.  Always rebuilds.
 """
 import concurrent.futures
 import os
@@ -15,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "liblmn.so")
+CLI = os.path.join(OUT_DIR, "luminara-gpu")
 
 HOST_SOURCES = ["host/toml.cpp", "host/scene_load.cpp", "host/scene_build.cpp"]
 HIP_SOURCES = ["lmn_api.hip", "lmn_kernels.hip", "lmn_plain.hip", "lmn_adjoint.hip"]
@@ -58,6 +60,8 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "cli", "luminara_gpu.cpp"), "-o", CLI,
+          f"-L{OUT_DIR}", "-llmn", "-lz", "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 

@@ -1,0 +1,379 @@
+// luminara-gpu: the reference's command line (src/main.rs:154-945) for the
+// flags that reach the render hot path, rendering on the MI355X through the
+// library's C ABI.  Scene loading follows scene::load_scene
+// (lmn_load_scene); -w/-H/-s/-d/--seed override the [render] table like
+// src/main.rs:204-230 and -p/--preview like :231-237 (the camera keeps the
+// TOML aspect, :205-210).  The framebuffer goes through the reference's
+// default post chain -- exposure, tone map, sRGB, 8-bit
+// (src/render.rs:2594-2632, :3099-3202, :3606-3658) -- into PNG or PPM
+// (src/main.rs:774-810); --save-hdr writes Radiance RGBE after exposure
+// (:915-944, :948-989); --save-json the render statistics (:840-857).
+// Effects beyond the default chain are not part of this path.
+#include <zlib.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../../include/luminara.h"
+
+namespace {
+
+struct Cli {
+  std::string scene, output = "output.png", save_hdr, save_json;
+  long width = -1, height = -1, samples = -1, depth = -1, threads = -1;
+  long long seed = -1;
+  double exposure = -1.0;
+  bool quiet = false, preview = false, benchmark = false, help = false;
+  std::vector<double> post_check;
+};
+
+[[noreturn]] void die(const std::string& msg) {
+  std::fprintf(stderr, "Error: %s\n", msg.c_str());
+  std::exit(1);
+}
+
+// Numeric option values: a value that does not parse leaves the setting
+// unchanged, as `args[i].parse().ok()` does.
+template <class T>
+void parse_num(const char* s, T& out) {
+  char* end = nullptr;
+  errno = 0;
+  if constexpr (std::is_floating_point<T>::value) {
+    double v = std::strtod(s, &end);
+    if (end != s && *end == '\0' && errno == 0) out = T(v);
+  } else {
+    long long v = std::strtoll(s, &end, 10);
+    if (end != s && *end == '\0' && errno == 0 && v >= 0) out = T(v);
+  }
+}
+
+Cli parse(int argc, char** argv) {
+  Cli c;
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    auto next = [&]() -> const char* { return (i + 1 < argc) ? argv[++i] : nullptr; };
+    if (a == "-o" || a == "--output") {
+      if (const char* v = next()) c.output = v;
+    } else if (a == "-w" || a == "--width") {
+      if (const char* v = next()) parse_num(v, c.width);
+    } else if (a == "-H" || a == "--height") {
+      if (const char* v = next()) parse_num(v, c.height);
+    } else if (a == "-s" || a == "--samples") {
+      if (const char* v = next()) parse_num(v, c.samples);
+    } else if (a == "-d" || a == "--depth") {
+      if (const char* v = next()) parse_num(v, c.depth);
+    } else if (a == "--seed") {
+      if (const char* v = next()) parse_num(v, c.seed);
+    } else if (a == "-t" || a == "--threads") {
+      if (const char* v = next()) parse_num(v, c.threads);
+    } else if (a == "-e" || a == "--exposure") {
+      if (const char* v = next()) parse_num(v, c.exposure);
+    } else if (a == "-q" || a == "--quiet") {
+      c.quiet = true;
+    } else if (a == "-p" || a == "--preview") {
+      c.preview = true;
+    } else if (a == "--save-hdr") {
+      if (const char* v = next()) c.save_hdr = v;
+    } else if (a == "--save-json") {
+      if (const char* v = next()) c.save_json = v;
+    } else if (a == "--benchmark" || a == "--bench") {
+      c.benchmark = true;
+    } else if (a == "-h" || a == "--help") {
+      c.help = true;
+    } else if (a == "--post-check") {  // post chain on the given values (no render)
+      while (i + 1 < argc) {
+        double v;
+        char* end = nullptr;
+        v = std::strtod(argv[i + 1], &end);
+        if (end == argv[i + 1] || *end != '\0') break;
+        c.post_check.push_back(v);
+        ++i;
+      }
+    } else if (!a.empty() && a[0] != '-' && c.scene.empty()) {
+      c.scene = a;
+    } else {
+      std::fprintf(stderr, "Warning: option '%s' is not supported by luminara-gpu and is ignored\n", a.c_str());
+    }
+  }
+  return c;
+}
+
+// ---- default post chain ---------------------------------------------------
+// Tone-map curves: ACES filmic fit (Narkowicz 2015), Reinhard x/(1+x),
+// Uncharted 2 filmic (Hable 2010) normalised at white point 11.2; then the
+// sRGB transfer function and the reference's 8-bit quantisation
+// floor(min(max(s, 0), 0.999) * 256) (src/render.rs:3606-3658).
+double tone_aces(double x) {
+  x = std::max(x, 0.0);
+  const double a = 2.51, b = 0.03, c = 2.43, d = 0.59, e = 0.14;
+  return std::min(std::max((x * (a * x + b)) / (x * (c * x + d) + e), 0.0), 1.0);
+}
+double tone_reinhard(double x) {
+  x = std::max(x, 0.0);
+  return x / (1.0 + x);
+}
+double hable(double v) {
+  const double A = 0.15, B = 0.50, C = 0.10, D = 0.20, E = 0.02, F = 0.30;
+  return ((v * (A * v + C * B) + D * E) / (v * (A * v + B) + D * F)) - E / F;
+}
+double tone_filmic(double x) {
+  x = std::max(x, 0.0);
+  return std::min(std::max(hable(x) / hable(11.2), 0.0), 1.0);
+}
+double tone(uint32_t mode, double x) {
+  switch (mode) {
+    case 0: return tone_aces(x);
+    case 1: return tone_reinhard(x);
+    case 2: return tone_filmic(x);
+    default: return std::min(std::max(x, 0.0), 1.0);
+  }
+}
+uint8_t srgb8(double x) {
+  x = std::min(std::max(x, 0.0), 1.0);
+  double s = x <= 0.0031308 ? 12.92 * x : 1.055 * std::pow(x, 1.0 / 2.4) - 0.055;
+  return uint8_t(std::min(std::max(s, 0.0), 0.999) * 256.0);
+}
+
+// ---- writers --------------------------------------------------------------
+void put_be32(std::vector<uint8_t>& b, uint32_t v) {
+  for (int k = 3; k >= 0; --k) b.push_back(uint8_t(v >> (8 * k)));
+}
+void png_chunk(std::vector<uint8_t>& out, const char* type, const std::vector<uint8_t>& data) {
+  put_be32(out, uint32_t(data.size()));
+  size_t start = out.size();
+  out.insert(out.end(), type, type + 4);
+  out.insert(out.end(), data.begin(), data.end());
+  uLong crc = crc32(0L, out.data() + start, uInt(out.size() - start));
+  put_be32(out, uint32_t(crc));
+}
+bool write_png(const std::string& path, uint32_t w, uint32_t h, const std::vector<uint8_t>& rgba) {
+  std::vector<uint8_t> raw;
+  raw.reserve(size_t(h) * (size_t(w) * 4 + 1));
+  for (uint32_t y = 0; y < h; ++y) {
+    raw.push_back(0);  // filter: none
+    raw.insert(raw.end(), rgba.begin() + size_t(y) * w * 4, rgba.begin() + size_t(y + 1) * w * 4);
+  }
+  uLongf zlen = compressBound(uLong(raw.size()));
+  std::vector<uint8_t> z(zlen);
+  if (compress2(z.data(), &zlen, raw.data(), uLong(raw.size()), 6) != Z_OK) return false;
+  z.resize(zlen);
+  std::vector<uint8_t> out = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+  std::vector<uint8_t> ihdr;
+  put_be32(ihdr, w);
+  put_be32(ihdr, h);
+  ihdr.insert(ihdr.end(), {8, 6, 0, 0, 0});  // 8-bit RGBA
+  png_chunk(out, "IHDR", ihdr);
+  png_chunk(out, "IDAT", z);
+  png_chunk(out, "IEND", {});
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(out.data()), std::streamsize(out.size()));
+  return bool(f);
+}
+std::string ppm_text(uint32_t w, uint32_t h, const std::vector<uint8_t>& rgba) {
+  std::ostringstream s;
+  s << "P3\n" << w << " " << h << "\n255\n";
+  for (size_t i = 0; i < size_t(w) * h; ++i)
+    s << int(rgba[i * 4]) << " " << int(rgba[i * 4 + 1]) << " " << int(rgba[i * 4 + 2]) << "\n";
+  return s.str();
+}
+// Radiance RGBE, flat (no run-length), as src/main.rs:948-989 writes it.
+bool write_rgbe(const std::string& path, uint32_t w, uint32_t h, const std::vector<float>& hdr) {
+  std::ostringstream head;
+  head << "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y " << h << " +X " << w << "\n";
+  std::string out = head.str();
+  for (size_t i = 0; i < size_t(w) * h; ++i) {
+    float r = std::max(hdr[i * 3], 0.f), g = std::max(hdr[i * 3 + 1], 0.f), b = std::max(hdr[i * 3 + 2], 0.f);
+    float m = std::max(r, std::max(g, b));
+    if (m < 1e-32f) {
+      out.append(4, '\0');
+      continue;
+    }
+    int e = int(std::ceil(std::log2(m)));
+    float scale = 256.0f / std::pow(2.0f, float(e));
+    out.push_back(char(uint8_t(std::min(r * scale, 255.0f))));
+    out.push_back(char(uint8_t(std::min(g * scale, 255.0f))));
+    out.push_back(char(uint8_t(std::min(b * scale, 255.0f))));
+    out.push_back(char(uint8_t(e + 128)));
+  }
+  std::ofstream f(path, std::ios::binary);
+  f << out;
+  return bool(f);
+}
+
+std::string read_file(const std::string& path, bool& ok) {
+  std::ifstream f(path, std::ios::binary);
+  ok = bool(f);
+  std::ostringstream s;
+  s << f.rdbuf();
+  return s.str();
+}
+
+void usage() {
+  std::fprintf(stderr,
+               "Usage: luminara-gpu scene.toml [options]\n\n"
+               "  -o, --output FILE     PNG output, or PPM for '.ppm' / '-' (stdout) [output.png]\n"
+               "  -w, --width N         image width (the camera keeps the scene's aspect)\n"
+               "  -H, --height N        image height\n"
+               "  -s, --samples N       samples per pixel (rounded up to a square)\n"
+               "  -d, --depth N         maximum bounce depth\n"
+               "      --seed N          random seed\n"
+               "  -e, --exposure X      exposure multiplier\n"
+               "  -p, --preview         1/4 resolution, <= 4 spp, depth <= 10\n"
+               "  -q, --quiet           no progress output\n"
+               "  -t, --threads N       accepted; the render runs on the GPU\n"
+               "      --save-hdr FILE   Radiance HDR of the exposed framebuffer\n"
+               "      --save-json FILE  render statistics\n"
+               "      --benchmark       the scene at 400x225, 64 spp; prints Mrays/s\n");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Cli cli = parse(argc, argv);
+  if (cli.help) {
+    usage();
+    return 0;
+  }
+  if (!cli.post_check.empty()) {
+    for (double v : cli.post_check)
+      std::printf("%.17g %.17g %.17g %.17g %d\n", v, tone_aces(v), tone_reinhard(v), tone_filmic(v), int(srgb8(v)));
+    return 0;
+  }
+  const auto t_start = std::chrono::steady_clock::now();
+  if (cli.scene.empty()) die("no scene file given (luminara-gpu scene.toml [options])");
+  bool ok = false;
+  std::string text = read_file(cli.scene, ok);
+  if (!ok) die("cannot read scene file '" + cli.scene + "'");
+  std::string base = cli.scene.find('/') == std::string::npos ? "." : cli.scene.substr(0, cli.scene.rfind('/'));
+  lmn_scene_desc* desc = nullptr;
+  if (lmn_load_scene(text.c_str(), base.c_str(), &desc) != 0)
+    die("invalid scene '" + cli.scene + "': " + lmn_last_error());
+  lmn_render_config rc = desc->config;
+  lmn_scene* scene = nullptr;
+  int rcode = lmn_scene_create(desc, &scene);
+  lmn_desc_free(desc);
+  if (rcode != 0) die(std::string("scene upload failed: ") + lmn_last_error());
+
+  if (cli.width >= 0) rc.width = uint32_t(cli.width);
+  if (cli.height >= 0) rc.height = uint32_t(cli.height);
+  if (cli.samples >= 0) rc.samples = uint32_t(cli.samples);
+  if (cli.depth >= 0) rc.max_depth = uint32_t(cli.depth);
+  if (cli.seed >= 0) rc.seed = uint64_t(cli.seed);
+  if (cli.exposure >= 0.0) rc.exposure = cli.exposure;
+  if (cli.preview) {
+    rc.width = std::max<uint32_t>(rc.width / 4, 1);
+    rc.height = std::max<uint32_t>(rc.height / 4, 1);
+    rc.samples = std::min<uint32_t>(rc.samples, 4);
+    rc.max_depth = std::min<uint32_t>(rc.max_depth, 10);
+  }
+  char dev[256] = {0};
+  int n_cu = 0;
+  double hbm = 0;
+  lmn_device_info(dev, sizeof dev, &n_cu, &hbm);
+
+  lmn_render_params p;
+  std::memset(&p, 0, sizeof p);
+  if (cli.benchmark) {
+    rc.width = 400;
+    rc.height = 225;
+    rc.samples = 64;
+    rc.has_crop = 0;
+  }
+  if (rc.width == 0 || rc.height == 0) die("width and height must be > 0");
+  if (rc.samples == 0) die("samples must be > 0");
+  p.width = rc.width;
+  p.height = rc.height;
+  p.spp = rc.samples;
+  p.max_depth = rc.max_depth;
+  p.seed = rc.seed;
+  p.filter = rc.filter;
+  p.has_crop = rc.has_crop;
+  std::memcpy(p.crop, rc.crop, sizeof p.crop);
+  p.path = LMN_PATH_WAVEFRONT_FP32;
+  p.intersect_mode = LMN_ISECT_EXACT;
+  uint32_t out_w = rc.width, out_h = rc.height;
+  if (rc.has_crop) {
+    out_w = std::min(rc.crop[2], rc.width > rc.crop[0] ? rc.width - rc.crop[0] : 0u);
+    out_h = std::min(rc.crop[3], rc.height > rc.crop[1] ? rc.height - rc.crop[1] : 0u);
+  }
+  if (!cli.quiet && !cli.benchmark)
+    std::fprintf(stderr, "Rendering %ux%u (%.1fMP) @ %u spp, max depth %u, on %s (%d CUs)\n", rc.width, rc.height,
+                 double(out_w) * out_h / 1e6, rc.samples, rc.max_depth, dev, n_cu);
+  std::vector<float> hdr(size_t(std::max(out_w, 1u)) * std::max(out_h, 1u) * 3);
+  lmn_stats st;
+  std::memset(&st, 0, sizeof st);
+  if (lmn_render(scene, &p, hdr.data(), &st) != 0) die(std::string("render failed: ") + lmn_last_error());
+  lmn_scene_destroy(scene);
+  const double trace_s = st.render_ms / 1e3;
+  if (cli.benchmark) {
+    double mrays = double(st.paths) / trace_s / 1e6;
+    uint32_t sq = uint32_t(std::ceil(std::sqrt(double(rc.samples))));
+    std::fprintf(stderr, "Benchmark result: %.1f Mrays/s (%.3fs, %s)\n", mrays, trace_s, dev);
+    std::fprintf(stderr, "  %llu primary rays, %u spp (stratified %ux%u)\n", (unsigned long long)st.paths, sq * sq,
+                 sq, sq);
+    return 0;
+  }
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  const double mrays = double(st.paths) / trace_s / 1e6;
+  if (!cli.quiet)
+    std::fprintf(stderr, "Rendered in %.2fs (trace: %.2fs, %.1f Mrays/s, %llu total rays)\n", secs, trace_s, mrays,
+                 (unsigned long long)st.paths);
+
+  // default post chain: exposure, tone map, sRGB, 8-bit RGBA
+  std::vector<uint8_t> rgba(size_t(out_w) * out_h * 4);
+  for (size_t i = 0; i < size_t(out_w) * out_h; ++i) {
+    for (int k = 0; k < 3; ++k) rgba[i * 4 + k] = srgb8(tone(rc.tone_map, double(hdr[i * 3 + k]) * rc.exposure));
+    rgba[i * 4 + 3] = 255;
+  }
+  const std::string& out = cli.output;
+  bool ppm = out == "-" || (out.size() >= 4 && out.compare(out.size() - 4, 4, ".ppm") == 0);
+  if (ppm) {
+    std::string s = ppm_text(out_w, out_h, rgba);
+    if (out == "-") {
+      std::fwrite(s.data(), 1, s.size(), stdout);
+      if (!cli.quiet) std::fprintf(stderr, "Written to stdout (PPM)\n");
+    } else {
+      std::ofstream f(out, std::ios::binary);
+      f << s;
+      if (!f) die("failed to write '" + out + "'");
+      if (!cli.quiet) std::fprintf(stderr, "Saved to %s\n", out.c_str());
+    }
+  } else {
+    if (!write_png(out, out_w, out_h, rgba)) die("failed to save image '" + out + "'");
+    if (!cli.quiet) std::fprintf(stderr, "Saved to %s\n", out.c_str());
+  }
+  if (!cli.save_json.empty()) {
+    std::ostringstream j;
+    j.setf(std::ios::fixed);
+    j << "{\n  \"width\": " << out_w << ",\n  \"height\": " << out_h << ",\n  \"samples\": " << rc.samples
+      << ",\n  \"max_depth\": " << rc.max_depth << ",\n  \"total_rays\": " << st.paths;
+    j.precision(4);
+    j << ",\n  \"render_time_secs\": " << trace_s << ",\n  \"total_time_secs\": " << secs;
+    j.precision(2);
+    j << ",\n  \"mrays_per_sec\": " << mrays << ",\n  \"device\": \"" << dev << "\",\n  \"output\": \"" << out
+      << "\"\n}";
+    std::ofstream f(cli.save_json);
+    f << j.str();
+    if (!f)
+      std::fprintf(stderr, "Warning: failed to write JSON stats '%s'\n", cli.save_json.c_str());
+    else if (!cli.quiet)
+      std::fprintf(stderr, "Stats saved to %s\n", cli.save_json.c_str());
+  }
+  if (!cli.save_hdr.empty()) {
+    std::vector<float> exposed(hdr.size());
+    for (size_t i = 0; i < hdr.size(); ++i) exposed[i] = float(double(hdr[i]) * rc.exposure);
+    if (!write_rgbe(cli.save_hdr, out_w, out_h, exposed)) die("failed to save HDR '" + cli.save_hdr + "'");
+    if (!cli.quiet) std::fprintf(stderr, "Saved HDR to %s\n", cli.save_hdr.c_str());
+  }
+  return 0;
+}

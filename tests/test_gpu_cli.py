@@ -1,0 +1,99 @@
+"""luminara-gpu renders through the library and writes the reference's
+outputs: the PNG/PPM of the default post chain (exposure, ACES, sRGB,
+8-bit) of the same framebuffer `render` returns, the Radiance HDR of the
+exposed framebuffer, and the statistics JSON (src/main.rs:774-989)."""
+import json
+import os
+import struct
+import subprocess
+import zlib
+
+import numpy as np
+import pytest
+
+import luminara_amd as lm
+from luminara_amd import build, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def read_png(path):
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, idat, w, h = 8, b"", 0, 0
+    while pos < len(data):
+        n, typ = struct.unpack(">I4s", data[pos:pos + 8])
+        body = data[pos + 8:pos + 8 + n]
+        if typ == b"IHDR":
+            w, h, depth, ctype = struct.unpack(">IIBB", body[:10])
+            assert depth == 8 and ctype == 6
+        elif typ == b"IDAT":
+            idat += body
+        pos += 12 + n
+    raw = np.frombuffer(zlib.decompress(idat), dtype=np.uint8).reshape(h, w * 4 + 1)
+    assert np.all(raw[:, 0] == 0)
+    return raw[:, 1:].reshape(h, w, 4)
+
+
+def read_ppm(path):
+    tok = open(path).read().split()
+    assert tok[0] == "P3"
+    w, h = int(tok[1]), int(tok[2])
+    return np.array(tok[4:], dtype=np.int64).reshape(h, w, 3)
+
+
+def read_rgbe(path):
+    data = open(path, "rb").read()
+    head, body = data.split(b"\n\n", 1)
+    line, pix = body.split(b"\n", 1)
+    _, h, _, w = line.split()
+    e = np.frombuffer(pix, dtype=np.uint8).reshape(int(h), int(w), 4).astype(np.float64)
+    scale = np.where(e[..., 3:] > 0, np.ldexp(1.0, (e[..., 3:] - 128).astype(int)) / 256.0, 0.0)
+    return e[..., :3] * scale
+
+
+def srgb8(x):
+    x = np.clip(x, 0.0, 1.0)
+    s = np.where(x <= 0.0031308, 12.92 * x, 1.055 * np.power(x, 1.0 / 2.4) - 0.055)
+    return (np.clip(s, 0.0, 0.999) * 256.0).astype(np.int64)
+
+
+def aces(x):
+    x = np.maximum(x, 0.0)
+    return np.clip((x * (2.51 * x + 0.03)) / (x * (2.43 * x + 0.59) + 0.14), 0.0, 1.0)
+
+
+def test_cli_outputs_match_the_library_render(gpu, tmp_path):
+    scene = tmp_path / "cornell.toml"
+    scene.write_text(workloads.cornell_box(64, 64, 16, 8))
+    args = [build.CLI, str(scene), "-w", "48", "-H", "40", "-s", "9", "-d", "6", "--seed", "7", "-q"]
+    png, ppm, hdr, js = (tmp_path / n for n in ("o.png", "o.ppm", "o.hdr", "o.json"))
+    subprocess.run(args + ["-o", str(png), "--save-hdr", str(hdr), "--save-json", str(js)], check=True, timeout=120)
+    subprocess.run(args + ["-o", str(ppm)], check=True, timeout=120)
+    cfg, cam, world = lm.load_scene(scene.read_text())
+    cfg.width, cfg.height, cfg.samples_per_pixel, cfg.max_depth, cfg.seed = 48, 40, 9, 6, 7
+    rows = lm.render(cfg, cam, world).rows.astype(np.float64)
+    img = read_png(png)
+    assert img.shape == (40, 48, 4) and np.all(img[..., 3] == 255)
+    expect = srgb8(aces(rows * cfg.exposure))
+    # the C++ and numpy powers may round the last bit differently
+    assert np.abs(img[..., :3].astype(np.int64) - expect).max() <= 1
+    assert (img[..., :3] == expect).mean() > 0.999
+    assert np.array_equal(read_ppm(ppm), img[..., :3].astype(np.int64))
+    h = read_rgbe(hdr)
+    ref = rows * cfg.exposure
+    m = ref.max(axis=2, keepdims=True)
+    assert np.all(np.abs(h - ref) <= m / 128.0 + 1e-30)
+    st = json.loads(js.read_text())
+    assert st["width"] == 48 and st["height"] == 40 and st["samples"] == 9 and st["max_depth"] == 6
+    assert st["total_rays"] == 48 * 40 * 9
+
+
+def test_cli_preview_and_benchmark(gpu, tmp_path):
+    scene = tmp_path / "cornell.toml"
+    scene.write_text(workloads.cornell_box(64, 64, 16, 8))
+    png = tmp_path / "p.png"
+    subprocess.run([build.CLI, str(scene), "-p", "-q", "-o", str(png)], check=True, timeout=120)
+    assert read_png(png).shape == (16, 16, 4)
+    r = subprocess.run([build.CLI, str(scene), "--benchmark"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "Benchmark result:" in r.stderr and "Mrays/s" in r.stderr
