@@ -101,7 +101,7 @@ def cpu_baseline(scenes, seconds, threads):
     step = 256
     for _ in range(4):  # calibrate: thread start-up dominates the smallest samples
         paths, secs = run(step)
-        if secs >= 0.6 * seconds or step == 1:
+        if secs >= 0.7 * seconds or step == 1:
             break
         frac = min(1.0, seconds * paths / max(secs, 1e-6) / total)
         step = max(1, min(step - 1, int(round(1.0 / max(frac, 1e-9)))))
@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 kernel comparison")
     ap.add_argument("--no-count", action="store_true", help="skip the instrumented traversal count")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--stack-lds", type=int, default=0)
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering adjoint leg")
