@@ -436,6 +436,8 @@ std::string tune_key(lmn_scene* const* scenes, const lmn_render_params* const* p
   return k.str();
 }
 
+constexpr uint32_t kDefaultGridThreads = 1u << 22;
+
 WaveConfig default_wave_config(uint64_t total) {
   WaveConfig c;
   c.pool = uint32_t(std::min<uint64_t>(std::max<uint64_t>(total, 64), 1u << 23));
@@ -461,7 +463,9 @@ int resolve_wave_config(lmn_scene* const* scenes, const lmn_render_params* const
   if (c.block != 128 && c.block != 256) return set_err(LMN_ERR_INVALID, "block_size must be 128 or 256");
   if (c.stack != 16 && c.stack != 32) return set_err(LMN_ERR_INVALID, "stack_lds must be 16 or 32");
   uint32_t need = (c.pool + c.block - 1) / c.block;
-  if (!c.grid) c.grid = std::min<uint32_t>(need, (1u << 19) / c.block);  // 2048 workgroups of 256
+  // 16384 workgroups of 256 (2.5 rays per lane at 10 M rays): the hardware
+  // balances many short waves better than a persistent grid of long ones
+  if (!c.grid) c.grid = std::min<uint32_t>(need, kDefaultGridThreads / c.block);
   c.grid = std::max<uint32_t>(1, std::min(c.grid, need));
   return LMN_OK;
 }
@@ -932,8 +936,10 @@ int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_rend
   }
   if (total == 0) return set_err(LMN_ERR_INVALID, "lmn_autotune: nothing to render");
   const std::string key = tune_key(sc.data(), orig.data(), fs.data(), n_scenes);
-  // probe: the same frames at reduced spp, about 2^26 paths in all
-  const double scale = std::min(1.0, double(1ull << 26) / double(total));
+  // probe: the same frames at reduced spp, about 2^28 paths in all (several
+  // fills of the largest pool, so that the drain at the end of a render
+  // weighs about as little as in the full one)
+  const double scale = std::min(1.0, double(1ull << 28) / double(total));
   std::vector<lmn_render_params> pp(params, params + n_scenes);
   uint64_t probe_paths = 0;
   for (size_t i = 0; i < n_scenes; ++i) {
@@ -955,18 +961,18 @@ int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_rend
       q.grid_blocks = c.grid;
     }
     ms = 1e30;
-    for (int rep = 0; rep < 2; ++rep) {  // the first run warms caches and allocations
+    for (int rep = 0; rep < 3; ++rep) {  // the first run warms caches and allocations
       lmn_stats st{};
       int rc = render_common(sc.data(), ppp.data(), d_hdr_rgb, n_scenes, &st, nullptr);
       if (rc != LMN_OK) return rc;
-      if (rep) ms = st.render_ms;
+      if (rep) ms = std::min(ms, st.render_ms);
     }
     ++n_timed;
     return LMN_OK;
   };
   auto with_grid = [](WaveConfig c, double f) {
     uint32_t need = (c.pool + c.block - 1) / c.block;
-    uint32_t g = std::min<uint32_t>(need, (1u << 19) / c.block);
+    uint32_t g = std::min<uint32_t>(need, kDefaultGridThreads / c.block);
     c.grid = std::max<uint32_t>(1, std::min<uint32_t>(need, uint32_t(double(g) * f)));
     return c;
   };
@@ -1002,7 +1008,7 @@ int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_rend
     c.stack = best.stack == 16 ? 32 : 16;
     if ((rc = consider(c)) != LMN_OK) return rc;
   }
-  for (double f : {0.5, 2.0, 4.0, 8.0}) {
+  for (double f : {0.125, 0.25, 0.5, 2.0, 4.0, 8.0}) {
     WaveConfig c = with_grid(best, f);
     if (c.grid == best.grid) continue;
     if ((rc = consider(c)) != LMN_OK) return rc;

@@ -176,3 +176,13 @@ def test_marched_overflow_fallback(gpu, pool):
     b = lm.render(cfg, cam, world, path="plain32", intersect_mode=_abi.ISECT_EXACT)
     np.testing.assert_allclose(a.rows, b.rows, rtol=1e-6, atol=1e-7)
     assert a.stats["extension_rays"] == b.stats["extension_rays"]
+
+
+def test_marched_wavefront_equals_plain_fp32_reference_mode(gpu):
+    # the same in the reference's march semantics (SDF marches per kind)
+    cfg, cam, world = lm.load_scene(MARCHED)
+    cfg.samples_per_pixel = 16
+    a = lm.render(cfg, cam, world, path="wavefront", intersect_mode=_abi.ISECT_REFERENCE)
+    b = lm.render(cfg, cam, world, path="plain32", intersect_mode=_abi.ISECT_REFERENCE)
+    np.testing.assert_allclose(a.rows, b.rows, rtol=1e-6, atol=1e-7)
+    assert a.stats["extension_rays"] == b.stats["extension_rays"]
