@@ -309,7 +309,7 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         cpaths, csecs, frac = cpu_baseline(scenes, args.cpu_seconds, threads)
         cpu = {"value": cpaths / csecs / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-               "sample": f"every {round(1 / frac)}th row of every scene ({cpaths} paths, {csecs:.1f} s), "
+               "sample": f"every k-th row of every scene, k = {round(1 / frac)} ({cpaths} paths, {csecs:.1f} s), "
                          f"CPU FP64 oracle (restatement of the reference pixel loop), counter RNG"}
         log(f"[bench] cpu oracle: {cpu}")
 
