@@ -74,7 +74,7 @@ struct Workspace {
   DevBuf st[2][4];  // ob, db, bk, meta for the two pools
   DevBuf hit, hit_b, sh_o, sh_d, sh_c, samples, acc, spill, ctr, tc, rays, out, views, parts, part_begin;
   DevBuf adj_wsum, adj_gtex, adj_part, adj_red, adj_dldi, adj_tex_out;  // path-replay adjoint
-  DevBuf mq_cand, mq_key, mq_ovf;  // marched-pass candidate queues
+  DevBuf mq_cand, mq_key, mq_ovf, mq_brk;  // marched-pass candidate queues
   uint32_t pool = 0;
   bool graph_failed[2] = {false, false};  // capture refused once [without, with stage events]: launch directly
   WaveCounters* host_ctr = nullptr;
@@ -542,6 +542,7 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
     HIP_TRY(W.mq_cand.ensure(size_t(pool) * 4 * sizeof(uint2)));
     HIP_TRY(W.mq_key.ensure(size_t(pool) * 8));
     HIP_TRY(W.mq_ovf.ensure(size_t(pool) * 4));
+    if (march_kinds & 2u) HIP_TRY(W.mq_brk.ensure(size_t(pool / 2 + 1) * sizeof(MarchBracket)));
   }
   // per-scene accumulators, scene views, parts
   std::vector<size_t> acc_off(n + 1, 0);
@@ -573,7 +574,14 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
   L.spill = W.spill.as<uint32_t>();
   L.mq = MarchQ{};
   if (feat & FEAT_MARCH)
-    L.mq = MarchQ{W.mq_cand.as<uint2>(), W.mq_key.as<unsigned long long>(), W.mq_ovf.as<uint32_t>(), pool, march_kinds};
+    L.mq = MarchQ{W.mq_cand.as<uint2>(),
+                  W.mq_key.as<unsigned long long>(),
+                  W.mq_ovf.as<uint32_t>(),
+                  (march_kinds & 2u) ? W.mq_brk.as<MarchBracket>() : nullptr,
+                  pool,
+                  (march_kinds & 2u) ? pool / 2 + 1 : 0u,
+                  march_kinds,
+                  0u};
   L.tc = d_tc;
   for (size_t i = 0; i < n; ++i)
     if (!fs[i].black) launch_zero_acc(W.acc.as<double4>() + acc_off[i], fs[i].npix, st);

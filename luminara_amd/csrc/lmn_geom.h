@@ -645,11 +645,15 @@ LMN_HD bool se_may_cross(const double* p, V3<double> ro, V3<double> rd, double t
   }
 }
 
-// ANY: visibility only.  The root of a sign-change leaf lies inside the
-// leaf, which lies inside [tmin, tmax], so the bracket answers the query
-// and the refinement is skipped.
-template <bool ANY>
-LMN_HD bool se_exact_walk(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax, double& t) {
+// The first sign-change leaf [a, b] of the walk (fa, fb: val - 1 at its
+// ends), found by se_exact_walk and refined by se_refine.  The root lies
+// inside the leaf and the leaf inside [tmin, tmax], so the bracket alone
+// answers a visibility query.
+struct SeBracket {
+  double a, b, fa, fb;
+};
+
+LMN_HD bool se_exact_walk(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax, SeBracket& br) {
   V3<double> o, d;
   double t0, t1;
   if (!se_chord(p, ro, rd, tmin, tmax, o, d, t0, t1)) return false;
@@ -704,35 +708,8 @@ LMN_HD bool se_exact_walk(const double* p, V3<double> ro, V3<double> rd, double 
         continue;
       }
       if ((fa > 0.0) != (fb > 0.0)) {
-        if (ANY) {
-          t = 0.5 * (a + b);
-          return true;
-        }
-        // Newton's method from the secant point, kept inside the bracket
-        // [lo, hi] (which always holds the root) by bisection
-        double lo = a, hi = b, flo = fa;
-        double th = (a * fb - b * fa) / (fb - fa);
-        if (!(th > lo && th < hi)) th = 0.5 * (lo + hi);
-        for (int it = 0; it < 64; ++it) {
-          double g;
-          double fx = se_val_d(o, d, th, px, qa, py, g) - 1.0;
-          if (fx == 0.0) break;
-          if ((fx > 0.0) == (flo > 0.0)) {
-            lo = th;
-            flo = fx;
-          } else {
-            hi = th;
-          }
-          double tn = th - fx / g;
-          if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
-          bool done = m_abs(tn - th) <= 1e-13 * m_abs(tn) || hi - lo <= 1e-13 * m_abs(hi);
-          th = tn;
-          if (done) break;
-        }
-        if (th >= tmin && th <= tmax) {
-          t = th;
-          return true;
-        }
+        br = SeBracket{a, b, fa, fb};
+        return true;
       }
       fa = fb;
     } else {
@@ -748,14 +725,50 @@ LMN_HD bool se_exact_walk(const double* p, V3<double> ro, V3<double> rd, double 
     ++idx;
   }
 }
+// Newton's method from the secant point of the walk's bracket, kept inside
+// the bracket (which always holds the root) by bisection.
+LMN_HD bool se_refine(const double* p, V3<double> ro, V3<double> rd, const SeBracket& br, double tmin, double tmax,
+                      double& t) {
+  const V3<double> o = {(ro.x - p[0]) / p[3], (ro.y - p[1]) / p[4], (ro.z - p[2]) / p[5]};
+  const V3<double> d = {rd.x / p[3], rd.y / p[4], rd.z / p[5]};
+  const double px = 2.0 / p[7], qa = p[7] / p[6], py = 2.0 / p[6];
+  double lo = br.a, hi = br.b, flo = br.fa;
+  double th = (br.a * br.fb - br.b * br.fa) / (br.fb - br.fa);
+  if (!(th > lo && th < hi)) th = 0.5 * (lo + hi);
+  for (int it = 0; it < 64; ++it) {
+    double g;
+    double fx = se_val_d(o, d, th, px, qa, py, g) - 1.0;
+    if (fx == 0.0) break;
+    if ((fx > 0.0) == (flo > 0.0)) {
+      lo = th;
+      flo = fx;
+    } else {
+      hi = th;
+    }
+    double tn = th - fx / g;
+    if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+    bool done = m_abs(tn - th) <= 1e-13 * m_abs(tn) || hi - lo <= 1e-13 * m_abs(hi);
+    th = tn;
+    if (done) break;
+  }
+  if (!(th >= tmin && th <= tmax)) return false;
+  t = th;
+  return true;
+}
+
 LMN_COLD bool hit_superellipsoid_exact(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax,
                                        double& t) {
-  return se_exact_walk<false>(p, ro, rd, tmin, tmax, t);
+  SeBracket br;
+  return se_exact_walk(p, ro, rd, tmin, tmax, br) && se_refine(p, ro, rd, br, tmin, tmax, t);
 }
 LMN_COLD bool hit_superellipsoid_exact_any(const double* p, V3<double> ro, V3<double> rd, double tmin,
                                            double tmax) {
-  double t;
-  return se_exact_walk<true>(p, ro, rd, tmin, tmax, t);
+  SeBracket br;
+  return se_exact_walk(p, ro, rd, tmin, tmax, br);
+}
+LMN_COLD bool se_exact_bracket(const double* p, V3<double> ro, V3<double> rd, double tmin, double tmax,
+                               SeBracket& br) {
+  return se_exact_walk(p, ro, rd, tmin, tmax, br);
 }
 
 LMN_HD double spring_sdf(V3<double> q, const double* p) {

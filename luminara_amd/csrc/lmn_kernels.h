@@ -38,24 +38,35 @@ struct WaveCounters {
   uint64_t iterations;
   uint32_t n_cand[2][4];  // marched candidates per kind + screened superellipsoids, [extend, shadow]
   uint32_t n_ovf[2];      // rays left to the marched fallback, [extend, shadow]
+  uint32_t n_brk;         // superellipsoid brackets queued for refinement (extend)
+  uint32_t pad2;
 };
 
 // Candidate queues of the marched pass.  A ray lists the marched
 // primitives whose boxes it enters (within the main pass's hit distance);
 // each kind (torus, superellipsoid, spring) has `cap` entries (ray index,
 // primitive record) and is tested by a kernel of its own with one candidate
-// per lane; superellipsoid candidates pass an FP32 screen first, and the
-// survivors (region 3) get the exact walk.  `key` holds per ray the closest
+// per lane; superellipsoid candidates pass an FP32 screen first, the
+// survivors (region 3) get the exact walk, and the extend stage refines the
+// brackets it finds in a kernel of their own.  `key` holds per ray the closest
 // marched hit (t bits << 32 |
 // record, MARCH_EMPTY: none) or, for shadow rays, a nonzero occluded flag.
 // Rays with more than MARCH_MAX_CAND candidates, or whose candidates do not
 // fit, are listed in `ovf` and traced whole by the fallback kernel.
+// A superellipsoid candidate's bracketing leaf, queued between the exact
+// walk and its refinement (extend stage).
+struct MarchBracket {
+  uint32_t ray, rec, pad0, pad1;
+  double a, b, fa, fb;
+};
 struct MarchQ {
   uint2* cand;               // 4 * cap entries; region r at cand + r * cap
   unsigned long long* key;   // per path / shadow ray
   uint32_t* ovf;             // per path / shadow ray
-  uint32_t cap;
+  MarchBracket* brk;         // cap_brk entries
+  uint32_t cap, cap_brk;
   uint32_t kinds;            // bit k: kind K_TORUS + k present in the batch
+  uint32_t pad;
 };
 constexpr unsigned long long MARCH_EMPTY = ~0ull;
 constexpr uint32_t MARCH_SKIP = 0xffffffffu;  // reserved slot left unused

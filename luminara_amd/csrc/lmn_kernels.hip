@@ -317,14 +317,8 @@ __global__ __launch_bounds__(BS) void k_march_screen(BatchDev B, PathSoA cur, Hi
       march_ray<SHADOW>(cur, hit, sh, e.x, o, d, sid, tmax);
       for_scene(sid, [&](uint32_t s) {
         const SceneView<float> S = B.views[s];
-        const PrimT<float>& pr = S.prims[e.y];
-        V3<double> o64 = cvt<double>(o), d64 = cvt<double>(d);
-        if ((FEAT & FEAT_XF) && pr.xform != XF_NONE) {
-          const XformT<double>& X64 = S.xforms64[pr.xform];
-          o64 = xf_point(X64.inv, o64);
-          d64 = xf_dir(X64.inv, d64);
-        }
-        keep = S.exact_marched == 0 || se_may_cross(S.prims64[e.y].p, o64, d64, 0.001, double(tmax), 8);
+        const MarchedRay r = marched_ray<FEAT>(S, e.y, o, d);
+        keep = S.exact_marched == 0 || se_may_cross(S.prims64[e.y].p, r.o64, r.d64, 0.001, double(tmax), 8);
       });
     }
     if (threadIdx.x == 0) cnt = 0;
@@ -364,6 +358,93 @@ __global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_test(BatchDev 
         else
           atomicMin(&q.key[e.x], (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | e.y);
       }
+    });
+  }
+}
+
+// Slot in a device-wide queue for every lane of the wave that wants one:
+// one atomic per wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool want) {
+  const unsigned long long m = __ballot(want);
+  if (m == 0ull) return 0u;
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0u;
+  if (int(__lane_id()) == leader) base = atomicAdd(ctr, uint32_t(__popcll(m)));
+  base = uint32_t(__shfl(int(base), leader));
+  return base + uint32_t(__popcll(m & __lanemask_lt()));
+}
+
+// Superellipsoid candidates of the extend stage, exact mode: the interval
+// walk up to the bracketing leaf; brackets are queued for k_march_refine, so
+// that the walk's lanes and the refinement's Newton iterations each run
+// with like work across the wave.  Reference mode, and brackets beyond the
+// queue, are finished in place.
+template <int BS, uint32_t FEAT>
+__global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_bracket(BatchDev B, PathSoA cur, HitSoA hit,
+                                                                         ShadowSoA sh, MarchQ q,
+                                                                         WaveCounters* __restrict__ ctr) {
+  const uint32_t nq = ctr->n_cand[0][3];
+  const uint32_t n = nq < q.cap ? nq : q.cap;
+  const uint2* cand = q.cand + size_t(3) * q.cap;
+  const uint32_t nthreads = gridDim.x * BS;
+  for (uint32_t j = blockIdx.x * BS + threadIdx.x; j < n; j += nthreads) {
+    const uint2 e = cand[j];
+    if (e.x == MARCH_SKIP) continue;
+    V3<float> o, d;
+    uint32_t sid;
+    float tmax;
+    march_ray<false>(cur, hit, sh, e.x, o, d, sid, tmax);
+    SeBracket br{};
+    bool queue = false;
+    for_scene(sid, [&](uint32_t s) {
+      const SceneView<float> S = B.views[s];
+      float t = tmax;
+      if (S.exact_marched == 0) {
+        if (test_marched<K_SUPERELLIPSOID, FEAT>(S, e.y, o, d, 0.001f, t))
+          atomicMin(&q.key[e.x], (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | e.y);
+        return;
+      }
+      const MarchedRay r = marched_ray<FEAT>(S, e.y, o, d);
+      queue = se_exact_bracket(S.prims64[e.y].p, r.o64, r.d64, 0.001, double(tmax), br);
+    });
+    const uint32_t slot = wave_append(&ctr->n_brk, queue);
+    if (!queue) continue;
+    if (slot < q.cap_brk) {
+      q.brk[slot] = MarchBracket{e.x, e.y, 0u, 0u, br.a, br.b, br.fa, br.fb};
+      continue;
+    }
+    for_scene(sid, [&](uint32_t s) {  // queue full: refine here
+      const SceneView<float> S = B.views[s];
+      const MarchedRay r = marched_ray<FEAT>(S, e.y, o, d);
+      double t64;
+      float t = tmax;
+      if (se_refine(S.prims64[e.y].p, r.o64, r.d64, br, 0.001, double(tmax), t64) &&
+          marched_accept<FEAT>(S, e.y, r, d, t64, t))
+        atomicMin(&q.key[e.x], (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | e.y);
+    });
+  }
+}
+
+template <int BS, uint32_t FEAT>
+__global__ __launch_bounds__(BS) void k_march_refine(BatchDev B, PathSoA cur, HitSoA hit, ShadowSoA sh, MarchQ q,
+                                                     const WaveCounters* __restrict__ ctr) {
+  const uint32_t nb = ctr->n_brk;
+  const uint32_t n = nb < q.cap_brk ? nb : q.cap_brk;
+  const uint32_t nthreads = gridDim.x * BS;
+  for (uint32_t j = blockIdx.x * BS + threadIdx.x; j < n; j += nthreads) {
+    const MarchBracket m = q.brk[j];
+    V3<float> o, d;
+    uint32_t sid;
+    float tmax;
+    march_ray<false>(cur, hit, sh, m.ray, o, d, sid, tmax);
+    for_scene(sid, [&](uint32_t s) {
+      const SceneView<float> S = B.views[s];
+      const MarchedRay r = marched_ray<FEAT>(S, m.rec, o, d);
+      double t64;
+      float t = tmax;
+      if (se_refine(S.prims64[m.rec].p, r.o64, r.d64, SeBracket{m.a, m.b, m.fa, m.fb}, 0.001, double(tmax), t64) &&
+          marched_accept<FEAT>(S, m.rec, r, d, t64, t))
+        atomicMin(&q.key[m.ray], (static_cast<unsigned long long>(__float_as_uint(t)) << 32) | m.rec);
     });
   }
 }
@@ -571,6 +652,7 @@ __global__ void k_book(WaveCounters* ctr, uint32_t pool) {
   ctr->iterations += 1;
   for (int k = 0; k < 4; ++k) ctr->n_cand[0][k] = ctr->n_cand[1][k] = 0;
   ctr->n_ovf[0] = ctr->n_ovf[1] = 0;
+  ctr->n_brk = 0;
 }
 
 __global__ void k_counters_init(WaveCounters* ctr, uint64_t n_items) {
@@ -587,6 +669,7 @@ __global__ void k_counters_init(WaveCounters* ctr, uint64_t n_items) {
   ctr->iterations = 0;
   for (int k = 0; k < 4; ++k) ctr->n_cand[0][k] = ctr->n_cand[1][k] = 0;
   ctr->n_ovf[0] = ctr->n_ovf[1] = 0;
+  ctr->n_brk = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -641,8 +724,15 @@ static void launch_march_tests(const WaveLaunch& L, const BatchDev& B, const Pat
   if (L.mq.kinds & 2u) {
     hipLaunchKernelGGL((k_march_screen<BS, FEAT, SHADOW>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.sh,
                        L.mq, L.ctr);
-    hipLaunchKernelGGL((k_march_test<BS, FEAT, K_SUPERELLIPSOID, SHADOW>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur,
-                       L.hit, L.sh, L.mq, L.ctr);
+    if constexpr (SHADOW) {
+      hipLaunchKernelGGL((k_march_test<BS, FEAT, K_SUPERELLIPSOID, SHADOW>), dim3(L.grid), dim3(BS), 0, L.stream, B,
+                         cur, L.hit, L.sh, L.mq, L.ctr);
+    } else {
+      hipLaunchKernelGGL((k_march_bracket<BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.sh, L.mq,
+                         L.ctr);
+      hipLaunchKernelGGL((k_march_refine<BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.sh, L.mq,
+                         L.ctr);
+    }
   }
   if (L.mq.kinds & 4u)
     hipLaunchKernelGGL((k_march_test<BS, FEAT, K_SPRING, SHADOW>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit,

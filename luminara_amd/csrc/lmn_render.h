@@ -193,53 +193,67 @@ LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t
   return true;
 }
 
-// test_prim for an FP32-path record of the marched kind KIND, for kernels
-// that test one kind per launch: the same FP64 evaluation, transform and
-// reported-t rules, with only that kind's root finder compiled in.  ANY:
-// visibility only (closest is then left unchanged on a superellipsoid hit
-// in exact mode, whose any-hit query stops at the bracketing leaf).
-template <uint32_t KIND, uint32_t FEAT, bool ANY = false>
-LMN_HD bool test_marched(const SceneView<float>& S, uint32_t idx, V3<float> o, V3<float> d, float tmin,
-                         float& closest) {
+// test_prim for an FP32-path record of the marched kinds, in pieces for
+// kernels that test one kind per launch: the object-space FP64 ray, and the
+// acceptance of an FP64 root (rounding to FP32, reported-t rule), exactly as
+// test_prim does them.
+struct MarchedRay {
+  V3<double> o64, d64;
+  V3<float> dd;  // object-space FP32 direction (reported-t factor)
+  bool has_xf;
+};
+template <uint32_t FEAT>
+LMN_HD MarchedRay marched_ray(const SceneView<float>& S, uint32_t idx, V3<float> o, V3<float> d) {
   const PrimT<float>& pr = S.prims[idx];
-  const bool has_xf = (FEAT & FEAT_XF) && pr.xform != XF_NONE;
-  V3<double> o64 = cvt<double>(o), d64 = cvt<double>(d);
-  V3<float> dd = d;
-  if (has_xf) {
+  MarchedRay r{cvt<double>(o), cvt<double>(d), d, (FEAT & FEAT_XF) && pr.xform != XF_NONE};
+  if (r.has_xf) {
     const XformT<double>& X64 = S.xforms64[pr.xform];
-    o64 = xf_point(X64.inv, o64);
-    d64 = xf_dir(X64.inv, d64);
-    dd = xf_dir(S.xforms[pr.xform].inv, d);
+    r.o64 = xf_point(X64.inv, r.o64);
+    r.d64 = xf_dir(X64.inv, r.d64);
+    r.dd = xf_dir(S.xforms[pr.xform].inv, d);
   }
-  const double* p = S.prims64[idx].p;
-  const bool exact = S.exact_marched != 0;
-  double t64;
-  bool hit;
-  if constexpr (KIND == K_TORUS)
-    hit = exact ? hit_torus_exact(p, o64, d64, double(tmin), double(closest), t64)
-                : hit_torus_march(p, o64, d64, double(tmin), double(closest), t64);
-  else if constexpr (KIND == K_SUPERELLIPSOID && ANY) {
-    if (exact) return hit_superellipsoid_exact_any(p, o64, d64, double(tmin), double(closest));
-    hit = hit_superellipsoid_march(p, o64, d64, double(tmin), double(closest), t64);
-  } else if constexpr (KIND == K_SUPERELLIPSOID)
-    hit = exact ? hit_superellipsoid_exact(p, o64, d64, double(tmin), double(closest), t64)
-                : hit_superellipsoid_march(p, o64, d64, double(tmin), double(closest), t64);
-  else
-    hit = exact ? hit_spring_exact(p, o64, d64, double(tmin), double(closest), t64)
-                : hit_spring_march(p, o64, d64, double(tmin), double(closest), t64);
-  if (!hit) return false;
+  return r;
+}
+template <uint32_t FEAT>
+LMN_HD bool marched_accept(const SceneView<float>& S, uint32_t idx, const MarchedRay& r, V3<float> d, double t64,
+                           float& closest) {
   float t = float(t64);
   if (!(t <= closest)) return false;  // rounding back to FP32 must not leave the range
-  if (has_xf) {
-    const XformT<float>& X = S.xforms[pr.xform];
+  if (r.has_xf) {
+    const XformT<float>& X = S.xforms[S.prims[idx].xform];
     if (X.tmode != 0) {
-      float tr = t * t_report_factor(X, dd, d);
+      float tr = t * t_report_factor(X, r.dd, d);
       if (!(tr <= closest)) return false;
       t = tr;
     }
   }
   closest = t;
   return true;
+}
+
+// The whole test for kind KIND.  ANY: visibility only (closest is then left
+// unchanged on a superellipsoid hit in exact mode, whose visibility query
+// stops at the bracketing leaf).
+template <uint32_t KIND, uint32_t FEAT, bool ANY = false>
+LMN_HD bool test_marched(const SceneView<float>& S, uint32_t idx, V3<float> o, V3<float> d, float tmin,
+                         float& closest) {
+  const MarchedRay r = marched_ray<FEAT>(S, idx, o, d);
+  const double* p = S.prims64[idx].p;
+  const bool exact = S.exact_marched != 0;
+  const double t0 = double(tmin), t1 = double(closest);
+  double t64;
+  bool hit;
+  if constexpr (KIND == K_TORUS)
+    hit = exact ? hit_torus_exact(p, r.o64, r.d64, t0, t1, t64) : hit_torus_march(p, r.o64, r.d64, t0, t1, t64);
+  else if constexpr (KIND == K_SUPERELLIPSOID && ANY) {
+    if (exact) return hit_superellipsoid_exact_any(p, r.o64, r.d64, t0, t1);
+    hit = hit_superellipsoid_march(p, r.o64, r.d64, t0, t1, t64);
+  } else if constexpr (KIND == K_SUPERELLIPSOID)
+    hit = exact ? hit_superellipsoid_exact(p, r.o64, r.d64, t0, t1, t64)
+                : hit_superellipsoid_march(p, r.o64, r.d64, t0, t1, t64);
+  else
+    hit = exact ? hit_spring_exact(p, r.o64, r.d64, t0, t1, t64) : hit_spring_march(p, r.o64, r.d64, t0, t1, t64);
+  return hit && marched_accept<FEAT>(S, idx, r, d, t64, closest);
 }
 
 // Every primitive record in the leaves of a binary BVH whose boxes the ray
