@@ -532,13 +532,13 @@ struct SeScreen {
 // comes from the FP32 terms: val' = qa A^(qa-1) A' + Y', A' = X' + Z', each
 // term's derivative p |c|^(p-1) d|c|/dt ranging between its values at the
 // sub-chord's ends (|c|^(p-1) = |c|^p / |c|), inflated by 1 % for the FP32
-// error; val(m) is screened in FP32 like the plain bounds and evaluated in
-// FP64 where that cannot decide.
-LMN_HD int se_mean_value_sign(V3<double> o, V3<double> d, double a, double b, float pxf, float qaf, float pyf,
-                              double px, double qa, double py, const SeScreen& S, float margin) {
+// error; val(m) is screened in FP32 like the plain bounds, and where that
+// cannot decide the walk evaluates it in FP64 (se_mean_value_decide).
+LMN_HD int se_mean_value_screen(V3<double> o, V3<double> d, double a, double b, float pxf, float qaf, float pyf,
+                                const SeScreen& S, float margin, double& m, double& r) {
   const float kTiny = 1e-30f;  // terms of smaller magnitudes may have flushed to zero
   if (!(S.xl > kTiny && S.yl > kTiny && S.zl > kTiny && S.Al > kTiny)) return 0;
-  const double m = 0.5 * (a + b);
+  m = 0.5 * (a + b);
   const V3<double> q = o + d * m;
   auto dterm = [](float p, float tl, float th, float ul, float uh, float k, float& lo, float& hi) {
     float g1 = p * (tl / ul), g2 = p * (th / uh);
@@ -575,8 +575,11 @@ LMN_HD int se_mean_value_sign(V3<double> o, V3<double> d, double a, double b, fl
   if (vm32 - r32 > 1.0f + margin) return 1;
   if (vm32 + r32 < 1.0f - margin) return -1;
   if (vm32 - r32 < 1.0f - margin && vm32 + r32 > 1.0f + margin) return 0;
-  const double vm = se_val(q, px, qa, py);
-  const double r = 0.5 * (b - a) * double(slope) * 1.01 + 1e-12 * (1.0 + vm);
+  r = 0.5 * (b - a) * double(slope) * 1.01;
+  return 2;  // the FP64 centre value decides: se_mean_value_decide
+}
+LMN_HD int se_mean_value_decide(double vm, double r) {
+  r += 1e-12 * (1.0 + vm);
   if (vm - r > 1.0 + 1e-12) return 1;
   if (vm + r < 1.0 - 1e-12) return -1;
   return 0;
@@ -662,60 +665,96 @@ LMN_HD bool se_exact_walk(const double* p, V3<double> ro, V3<double> rd, double 
   const float kScreenMargin = kSeScreenMargin;
   const int kSolveLevel = 12, kMaxLevel = 24, kMeanValueLevel = 4;
   const double w = t1 - t0;
-  double fa = se_val(o + d * t0, px, qa, py) - 1.0;  // sign at the left end of the current interval
-  // stackless depth-first walk of the subdivision tree: interval (level, idx)
-  // spans [t0 + idx * w / 2^level, t0 + (idx + 1) * w / 2^level]
+  // Every FP64 evaluation of the walk is val at one point of the chord (the
+  // chord's start, a sub-chord's centre for the mean-value form, a leaf's
+  // end), made at one place per step: lanes of a wave that need one at
+  // different stages of their walks share it.
+  enum : int { NEED_NONE, NEED_START, NEED_CENTRE, NEED_LEAF };
+  int need = NEED_START;
+  double fa = 0.0;  // val - 1 at the left end of the current interval (a sign once ruled out)
+  double r_mv = 0.0;
+  bool leaf = false;  // the current interval is past its screens: its right end decides
   int level = 0;
   uint32_t idx = 0;
   for (;;) {
     double scale = w / double(1u << level);
     double a = t0 + double(idx) * scale, b = (idx + 1u == (1u << level)) ? t1 : a + scale;
-    double xl, xh, yl, yh, zl, zh;
-    se_range1(o.x, d.x, a, b, xl, xh);
-    se_range1(o.y, d.y, a, b, yl, yh);
-    se_range1(o.z, d.z, a, b, zl, zh);
-    // sign of val - 1 over the interval where it is constant, else 0
-    SeScreen S;
-    S.xl = float(xl), S.xh = float(xh), S.yl = float(yl), S.yh = float(yh), S.zl = float(zl), S.zh = float(zh);
-    S.txl = pow_screen(S.xl, pxf), S.txh = pow_screen(S.xh, pxf), S.tzl = pow_screen(S.zl, pxf);
-    S.tzh = pow_screen(S.zh, pxf), S.tyl = pow_screen(S.yl, pyf), S.tyh = pow_screen(S.yh, pyf);
-    S.Al = S.txl + S.tzl, S.Ah = S.txh + S.tzh;
-    S.bl = pow_screen(S.Al, qaf), S.bh = pow_screen(S.Ah, qaf);
-    const float slo = S.bl + S.tyl, shi = S.bh + S.tyh;
-    int sign = slo > 1.0f + kScreenMargin ? 1 : (shi < 1.0f - kScreenMargin ? -1 : 0);
-    if (sign == 0 && !(slo < 1.0f - kScreenMargin && shi > 1.0f + kScreenMargin)) {
-      // a bound inside the band: the FP64 bounds decide (with both bounds
-      // clear of the band they would straddle 1 as well)
-      double vlo = pow_pos(pow_pos(xl, px) + pow_pos(zl, px), qa) + pow_pos(yl, py);
-      double vhi = pow_pos(pow_pos(xh, px) + pow_pos(zh, px), qa) + pow_pos(yh, py);
-      sign = vlo > 1.0 + 1e-12 ? 1 : (vhi < 1.0 - 1e-12 ? -1 : 0);
+    int sign = 0;
+    double t_eval = t0;
+    if (need == NEED_NONE && !leaf) {
+      double xl, xh, yl, yh, zl, zh;
+      se_range1(o.x, d.x, a, b, xl, xh);
+      se_range1(o.y, d.y, a, b, yl, yh);
+      se_range1(o.z, d.z, a, b, zl, zh);
+      SeScreen S;
+      S.xl = float(xl), S.xh = float(xh), S.yl = float(yl), S.yh = float(yh), S.zl = float(zl), S.zh = float(zh);
+      S.txl = pow_screen(S.xl, pxf), S.txh = pow_screen(S.xh, pxf), S.tzl = pow_screen(S.zl, pxf);
+      S.tzh = pow_screen(S.zh, pxf), S.tyl = pow_screen(S.yl, pyf), S.tyh = pow_screen(S.yh, pyf);
+      S.Al = S.txl + S.tzl, S.Ah = S.txh + S.tzh;
+      S.bl = pow_screen(S.Al, qaf), S.bh = pow_screen(S.Ah, qaf);
+      const float slo = S.bl + S.tyl, shi = S.bh + S.tyh;
+      sign = slo > 1.0f + kScreenMargin ? 1 : (shi < 1.0f - kScreenMargin ? -1 : 0);
+      const bool in_band = sign == 0 && !(slo < 1.0f - kScreenMargin && shi > 1.0f + kScreenMargin);
+      if (sign == 0 && (in_band || level >= kMeanValueLevel)) {
+        double m;
+        int mv = se_mean_value_screen(o, d, a, b, pxf, qaf, pyf, S, kScreenMargin, m, r_mv);
+        if (mv == 2) {
+          need = NEED_CENTRE;
+          t_eval = m;
+        } else {
+          sign = mv;
+        }
+      }
+      if (sign == 0 && need == NEED_NONE) {
+        if (level < kSolveLevel) {  // subdivide: left child first
+          ++level;
+          idx *= 2u;
+          continue;
+        }
+        leaf = true;
+      }
     }
-    if (sign == 0 && level >= kMeanValueLevel)
-      sign = se_mean_value_sign(o, d, a, b, pxf, qaf, pyf, px, qa, py, S, kScreenMargin);
-    bool constant = sign != 0;  // no crossing inside
-    if (!constant && level < kSolveLevel) {  // subdivide: left child first
-      ++level;
-      idx *= 2u;
-      continue;
+    if (need == NEED_NONE && leaf) {
+      need = NEED_LEAF;
+      t_eval = b;
     }
-    if (!constant) {
-      double fb = se_val(o + d * b, px, qa, py) - 1.0;
-      if ((fa > 0.0) == (fb > 0.0) && level < kMaxLevel) {
-        // no sign change across the leaf: a sliver entered and left within
-        // it, or a grazing touch; keep splitting (down to 2^-24 of the chord)
-        ++level;
-        idx *= 2u;
+    if (need != NEED_NONE) {
+      const double v = se_val(o + d * t_eval, px, qa, py);
+      const int what = need;
+      need = NEED_NONE;
+      if (what == NEED_START) {
+        fa = v - 1.0;
         continue;
       }
-      if ((fa > 0.0) != (fb > 0.0)) {
-        br = SeBracket{a, b, fa, fb};
-        return true;
+      if (what == NEED_CENTRE) {
+        sign = se_mean_value_decide(v, r_mv);
+        if (sign == 0) {
+          if (level < kSolveLevel) {
+            ++level;
+            idx *= 2u;
+          } else {
+            leaf = true;  // its right end next step
+          }
+          continue;
+        }
+      } else {  // NEED_LEAF
+        leaf = false;
+        const double fb = v - 1.0;
+        if ((fa > 0.0) == (fb > 0.0) && level < kMaxLevel) {
+          // no sign change across the leaf: a sliver entered and left within
+          // it, or a grazing touch; keep splitting (down to 2^-24 of the chord)
+          ++level;
+          idx *= 2u;
+          continue;
+        }
+        if ((fa > 0.0) != (fb > 0.0)) {
+          br = SeBracket{a, b, fa, fb};
+          return true;
+        }
+        fa = fb;
       }
-      fa = fb;
-    } else {
-      // constant sign over [a, b]: the sign at b
-      fa = double(sign);
     }
+    if (sign != 0) fa = double(sign);  // constant sign over [a, b]: the sign at b
     // next interval: climb while this is a right child, then step right
     while (idx & 1u) {
       idx >>= 1;
