@@ -163,6 +163,57 @@ def adjoint_leg(steps, warmup, res=512, spp=32, tex_res=256, n_spheres=32):
             "adjoint_mpaths_per_s": paths / (adj_ms / steps / 1e3) / 1e6, "loss": loss, "dtype": "f32"}
 
 
+def image_leg(scenes, worlds, frame, n_pix, reps):
+    """The image outputs of the batch, timed with HIP events on the stream
+    their kernels run on: the default post chain (exposure, ACES, sRGB, 8-bit;
+    12 B read + 4 B written per pixel) over every framebuffer of the batch in
+    one launch, the Radiance RGBE pixels of the exposed HDR (12 B read + 4 B
+    written), and the AOV passes (one pixel-centre ray per pixel, per scene)."""
+    import torch
+    import luminara_amd as lm
+    s = torch.cuda.current_stream()
+    rgba = torch.empty(n_pix * 4, dtype=torch.uint8, device="cuda")
+    hx = torch.empty(n_pix * 3, dtype=torch.float32, device="cuda")
+    rgbe = torch.empty(n_pix * 4, dtype=torch.uint8, device="cuda")
+    cfg = lm.RenderConfig()
+
+    def timed(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    post_ms = timed(lambda: lm.post_device(frame.data_ptr(), n_pix, 1, cfg, 0, rgba.data_ptr(), 0, s.cuda_stream))
+    lm.post_device(frame.data_ptr(), n_pix, 1, cfg, 0, rgba.data_ptr(), hx.data_ptr(), s.cuda_stream)
+    rgbe_ms = timed(lambda: lm.encode_rgbe_device(hx.data_ptr(), n_pix, rgbe.data_ptr(), s.cuda_stream))
+    bufs = []
+    for c, _ in scenes:
+        h, w = lm.output_shape(c)
+        bufs.append((torch.empty((h, w), dtype=torch.float32, device="cuda"),
+                     torch.empty((h, w, 3), dtype=torch.uint8, device="cuda"),
+                     torch.empty((h, w, 3), dtype=torch.uint8, device="cuda")))
+    aparams = [lm.make_params(c, scene_key=i, stream=s.cuda_stream) for i, (c, _) in enumerate(scenes)]
+
+    def aov_all():
+        for w, p, (d, n, a) in zip(worlds, aparams, bufs):
+            lm.aov_device(w, p, d.data_ptr(), n.data_ptr(), a.data_ptr())
+    aov_ms = timed(aov_all)
+    hbm = 8.0e12
+    return {"pixels": n_pix,
+            "post_chain": {"kernel": "k_post", "ms": post_ms, "bytes_per_pixel": 16,
+                           "achieved_gb_s": n_pix * 16 / (post_ms / 1e3) / 1e9,
+                           "frac_of_hbm_peak": n_pix * 16 / (post_ms / 1e3) / hbm},
+            "rgbe": {"kernel": "k_rgbe", "ms": rgbe_ms, "bytes_per_pixel": 16,
+                     "achieved_gb_s": n_pix * 16 / (rgbe_ms / 1e3) / 1e9,
+                     "frac_of_hbm_peak": n_pix * 16 / (rgbe_ms / 1e3) / hbm},
+            "aov": {"kernel": "k_aov", "ms": aov_ms, "launches": len(scenes),
+                    "mrays_per_s": n_pix / (aov_ms / 1e3) / 1e6}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +229,7 @@ def main():
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering adjoint leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults or cache)")
+    ap.add_argument("--no-images", action="store_true", help="skip the post chain / RGBE / AOV timings")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -209,7 +261,15 @@ def main():
     for w in worlds:
         nodes += w.bvh_info()["nodes"]
     log(f"[bench] {wl_name}: {desc}, {nodes} BVH nodes, scenes built and uploaded in {time.time() - t0:.1f}s")
-    outs = [torch.empty(lm.output_shape(c) + (3,), dtype=torch.float32, device="cuda") for c, _ in scenes]
+    # every scene's framebuffer is a view into one device buffer, so the post
+    # chain can run over the whole batch's pixels in one launch
+    shapes = [lm.output_shape(c) for c, _ in scenes]
+    n_pix = sum(h * w for h, w in shapes)
+    frame = torch.empty(n_pix * 3, dtype=torch.float32, device="cuda")
+    outs, off = [], 0
+    for h, w in shapes:
+        outs.append(frame[off:off + h * w * 3].view(h, w, 3))
+        off += h * w * 3
     ptrs = [o.data_ptr() for o in outs]
 
     def params(path, flags=0):
@@ -303,6 +363,12 @@ def main():
                       "relrmse_per_scene": [float(f"{x:.3g}") for x in per]})
         log(f"[bench] plain FP64: {t64 * 1e3:.1f} ms ({fp64_rate:.1f} Mpaths/s), relRMSE(FP32, FP64) = "
             f"{(num / den) ** 0.5:.3e} (worst scene {max(per):.3e})")
+
+    images = None
+    if not args.no_images:
+        images = image_leg(scenes, worlds, frame, n_pix, max(args.steps, 3) * 10)
+        log(f"[bench] image outputs: {images}")
+        extra["image_outputs"] = images
 
     cpu = None
     if not args.no_cpu:

@@ -173,8 +173,8 @@ typedef struct lmn_render_config {
   uint32_t has_crop;
   uint32_t crop[4]; /* x, y, w, h */
   double exposure;
-  uint32_t tone_map; /* 0 ACES, 1 Reinhard, 2 Filmic, 3 none (post chain, CPU) */
-  uint32_t pad;
+  uint32_t tone_map;      /* LMN_TONE_*: 0 ACES, 1 Reinhard, 2 Filmic, 3 none */
+  uint32_t auto_exposure; /* [render] auto_exposure (src/scene.rs:870-872)      */
   lmn_background background;
 } lmn_render_config;
 
@@ -393,6 +393,86 @@ int lmn_adjoint(const lmn_scene* scene, const lmn_render_params* params, const l
 /* Device buffers for dL/dI, grads->texels and grads->roughness. */
 int lmn_adjoint_device(const lmn_scene* scene, const lmn_render_params* params, const lmn_adjoint_spec* spec,
                        const float* d_dl_dimage, lmn_grads* grads);
+
+/* ---- image outputs: post chain, HDR encoding, AOV passes -------------------
+ * What render() does with the pixel loop's rows before returning its
+ * RenderResult (src/render.rs:1540-1554), restricted to the default chain:
+ *   exposure, optionally automatic          src/render.rs:2594-2608, :525-541
+ *   hdr_data = rows * exposure as f32        src/render.rs:2610-2624
+ *   depth fog from a pixel-centre ray        src/render.rs:2634-2665, :3117-3124
+ *   tone curve, sRGB, 8-bit quantisation     src/render.rs:3099-3202, :3606-3658
+ *   depth / normal / albedo passes           src/render.rs:3508-3584
+ * and the Radiance RGBE pixels of the HDR writer (src/main.rs:948-989).
+ * Per-pixel arithmetic is FP64 in the reference's order (contraction off).
+ * The options that change pixels otherwise (denoise, bloom, vignette, grading,
+ * dither, gamma, brightness, ...) are not on this path.  The host-buffer entry
+ * points return when their outputs are written.  The _device entry points are
+ * ordered on the caller's stream (params->stream or `stream`): they return once
+ * the work is enqueued there, or once it is done when the stream is NULL (the
+ * library's stream) or the automatic exposure has to be read back. */
+enum { LMN_TONE_ACES = 0, LMN_TONE_REINHARD = 1, LMN_TONE_FILMIC = 2, LMN_TONE_NONE = 3 };
+typedef struct lmn_post_params {
+  double exposure;        /* RenderConfig.exposure                                 */
+  uint32_t tone_map;      /* LMN_TONE_* (RenderConfig.tone_map)                    */
+  uint32_t auto_exposure; /* 1: exposure *= 0.18 / log-average luminance, or 1 when
+                             that average is <= 1e-6 (RenderConfig.auto_exposure) */
+  double depth_fog;       /* fog density, 0 = off (RenderConfig.depth_fog)         */
+  double fog_color[3];    /* RenderConfig.depth_fog_color, default 0.8, 0.85, 0.9  */
+} lmn_post_params;
+
+/* AOV passes and the depth-fog buffer of one frame (same scene and params as
+ * lmn_render; spp and max_depth are not used).  One ray per crop pixel through
+ * the pixel centre, u = (i + 0.5) / (W - 1), v = (y + 0.5) / (H - 1), with the
+ * camera's lens sample from the pixel's own stream, closest hit over
+ * [0.001, inf).  Outputs (crop_h*crop_w pixels, row 0 = top; NULL = skip):
+ *   depth   FP32 hit t in units of the unnormalised camera direction, +inf where
+ *           the ray escapes (the fog buffer of :2634-2665; the depth pass of
+ *           :3532 is this with +inf replaced by 0);
+ *   normal  3 bytes: (n * 0.5 + 0.5) * 255 of the facing normal, truncated;
+ *   albedo  3 bytes: one scatter's attenuation clamped to [0, 1] * 255, or for a
+ *           material that does not scatter, emission / max(max(e), 1) * 255.
+ * Normal and albedo bytes are 0 where the ray escapes.  params->path
+ * LMN_PATH_PLAIN_FP64 traces in FP64, any other path in FP32. */
+int lmn_aov(const lmn_scene* scene, const lmn_render_params* params, float* depth, uint8_t* normal,
+            uint8_t* albedo);
+int lmn_aov_device(const lmn_scene* scene, const lmn_render_params* params, float* d_depth, uint8_t* d_normal,
+                   uint8_t* d_albedo);
+
+/* Post chain of one framebuffer: hdr (height*width*3 floats, row 0 = top, linear
+ * HDR as lmn_render writes it) -> rgba (height*width*4 bytes, alpha 255).
+ * fog_depth: the depth output of lmn_aov for the same frame, required when
+ * post->depth_fog > 0, else ignored.  hdr_exposed (optional): hdr * exposure
+ * rounded to FP32 (RenderResult.hdr_data).  exposure_used (optional): the
+ * exposure applied.  stream: hipStream_t, NULL = the library's stream. */
+int lmn_post(const float* hdr, uint32_t width, uint32_t height, const lmn_post_params* post,
+             const float* fog_depth, uint8_t* rgba, float* hdr_exposed, double* exposure_used);
+int lmn_post_device(const float* d_hdr, uint32_t width, uint32_t height, const lmn_post_params* post,
+                    const float* d_fog_depth, uint8_t* d_rgba, float* d_hdr_exposed, double* exposure_used,
+                    void* stream);
+
+/* Radiance RGBE pixels (src/main.rs:964-986): n_pixels*3 floats in, n_pixels*4
+ * bytes out (the writer prepends the header "#?RADIANCE\nFORMAT=32-bit_rle_rgbe
+ * \n\n-Y H +X W\n"). */
+int lmn_encode_rgbe(const float* rgb, uint64_t n_pixels, uint8_t* rgbe);
+int lmn_encode_rgbe_device(const float* d_rgb, uint64_t n_pixels, uint8_t* d_rgbe, void* stream);
+
+/* A whole RenderResult in one call: lmn_render, then on the device the AOV
+ * ray (when fog or a pass is asked for), the post chain and the RGBE encoding;
+ * only the requested outputs cross PCIe.  Host buffers, crop_h*crop_w pixels,
+ * NULL = not wanted. */
+typedef struct lmn_image {
+  uint8_t* rgba;   /* *4  RenderResult.pixels                              */
+  float* hdr;      /* *3  RenderResult.hdr_data (after exposure)           */
+  uint8_t* rgbe;   /* *4  Radiance pixels of hdr_data                      */
+  float* depth;    /* *1  RenderResult.depth_pass (0 where the ray escapes) */
+  uint8_t* normal; /* *3  RenderResult.normal_pass                         */
+  uint8_t* albedo; /* *3  RenderResult.albedo_pass                         */
+  double exposure;    /* out: exposure applied                             */
+  double log_average; /* out: log-average luminance (auto-exposure), else 0 */
+  double post_ms;     /* out: device time of AOV ray + post chain + encoding */
+} lmn_image;
+int lmn_render_image(const lmn_scene* scene, const lmn_render_params* params, const lmn_post_params* post,
+                     lmn_image* image, lmn_stats* stats);
 
 #ifdef __cplusplus
 }

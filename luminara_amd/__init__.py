@@ -38,6 +38,9 @@ class RenderConfig:
     crop: Optional[Tuple[int, int, int, int]] = None
     exposure: float = 1.0
     tone_map: int = 0
+    auto_exposure: bool = False
+    depth_fog: float = 0.0
+    depth_fog_color: Tuple[float, float, float] = (0.8, 0.85, 0.9)
 
 
 class SceneWorld:
@@ -146,7 +149,8 @@ def load_scene(toml_text: str, base_dir: Optional[str] = None):
     world = SceneWorld(out)
     c = out.contents.config
     cfg = RenderConfig(width=c.width, height=c.height, samples_per_pixel=c.samples, max_depth=c.max_depth,
-                       seed=c.seed, pixel_filter=c.filter, exposure=c.exposure, tone_map=c.tone_map)
+                       seed=c.seed, pixel_filter=c.filter, exposure=c.exposure, tone_map=c.tone_map,
+                       auto_exposure=bool(c.auto_exposure))
     return cfg, out.contents.camera, world
 
 
@@ -331,6 +335,120 @@ def adjoint_device(world: SceneWorld, params: _abi.RenderParams, spec: _abi.Adjo
     return {"emission": gr.emission, "paths": gr.paths, "adjoint_ms": gr.adjoint_ms}
 
 
+def post_params(config: RenderConfig) -> _abi.PostParams:
+    pp = _abi.PostParams()
+    pp.exposure = config.exposure
+    pp.tone_map = config.tone_map
+    pp.auto_exposure = 1 if config.auto_exposure else 0
+    pp.depth_fog = config.depth_fog
+    for k in range(3):
+        pp.fog_color[k] = config.depth_fog_color[k]
+    return pp
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def aov(config: RenderConfig, world: SceneWorld, depth=True, normal=True, albedo=True, path="wavefront",
+        **kw):
+    """AOV passes of one frame (src/render.rs:3508-3584): (depth, normal,
+    albedo) arrays of the crop, None for passes not asked for.  depth is +inf
+    where the pixel's ray escapes."""
+    h, w = output_shape(config)
+    d = np.empty((h, w), np.float32) if depth else None
+    n = np.empty((h, w, 3), np.uint8) if normal else None
+    a = np.empty((h, w, 3), np.uint8) if albedo else None
+    params = make_params(config, path=path, **kw)
+    check(lib().lmn_aov(world.device_scene(), C.byref(params), _ptr(d), _ptr(n), _ptr(a)))
+    return d, n, a
+
+
+def post(hdr: np.ndarray, config: RenderConfig, fog_depth=None, want_hdr=False):
+    """Default post chain of a (H, W, 3) float32 framebuffer on the GPU:
+    returns (rgba (H, W, 4) uint8, exposed HDR or None, exposure used)."""
+    hdr = np.ascontiguousarray(hdr, dtype=np.float32)
+    h, w = hdr.shape[:2]
+    rgba = np.empty((h, w, 4), np.uint8)
+    hx = np.empty((h, w, 3), np.float32) if want_hdr else None
+    fog = None if fog_depth is None else np.ascontiguousarray(fog_depth, dtype=np.float32)
+    ex = C.c_double()
+    pp = post_params(config)
+    check(lib().lmn_post(_ptr(hdr), w, h, C.byref(pp), _ptr(fog), _ptr(rgba), _ptr(hx), C.byref(ex)))
+    return rgba, hx, ex.value
+
+
+def encode_rgbe(rgb: np.ndarray) -> np.ndarray:
+    """Radiance RGBE pixels of (..., 3) float32 data on the GPU."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    n = rgb.size // 3
+    out = np.empty(rgb.shape[:-1] + (4,), np.uint8)
+    check(lib().lmn_encode_rgbe(_ptr(rgb), n, _ptr(out)))
+    return out
+
+
+def aov_device(world: SceneWorld, params: _abi.RenderParams, depth_ptr=0, normal_ptr=0, albedo_ptr=0):
+    """lmn_aov_device into caller-owned device buffers (0 = skip), ordered on
+    params.stream."""
+    check(lib().lmn_aov_device(world.device_scene(), C.byref(params), C.c_void_p(depth_ptr or None),
+                               C.c_void_p(normal_ptr or None), C.c_void_p(albedo_ptr or None)))
+
+
+def post_device(hdr_ptr: int, width: int, height: int, config: RenderConfig, fog_ptr=0, rgba_ptr=0, hdr_out_ptr=0,
+                stream=None) -> float:
+    """lmn_post_device on device buffers, ordered on `stream` (a HIP stream
+    handle, None = the library's); returns the exposure applied."""
+    ex = C.c_double()
+    pp = post_params(config)
+    check(lib().lmn_post_device(C.c_void_p(hdr_ptr), width, height, C.byref(pp), C.c_void_p(fog_ptr or None),
+                                C.c_void_p(rgba_ptr or None), C.c_void_p(hdr_out_ptr or None), C.byref(ex),
+                                C.c_void_p(stream) if stream else None))
+    return ex.value
+
+
+def encode_rgbe_device(rgb_ptr: int, n_pixels: int, out_ptr: int, stream=None):
+    check(lib().lmn_encode_rgbe_device(C.c_void_p(rgb_ptr), n_pixels, C.c_void_p(out_ptr),
+                                       C.c_void_p(stream) if stream else None))
+
+
+@dataclasses.dataclass
+class ImageResult:
+    """RenderResult (src/render.rs:1540-1554) with the default post chain."""
+    pixels: np.ndarray
+    hdr_data: Optional[np.ndarray]
+    rgbe: Optional[np.ndarray]
+    depth_pass: Optional[np.ndarray]
+    normal_pass: Optional[np.ndarray]
+    albedo_pass: Optional[np.ndarray]
+    exposure: float
+    total_rays: int
+    render_time_secs: float
+    post_ms: float
+
+
+def render_image(config: RenderConfig, world: SceneWorld, save_hdr=False, save_depth=False, save_normals=False,
+                 save_albedo=False, path="wavefront", **kw) -> ImageResult:
+    """lmn_render_image: the frame, its AOV ray, post chain and RGBE pixels on
+    the GPU; only the outputs asked for come back."""
+    h, w = output_shape(config)
+    rgba = np.empty((h, w, 4), np.uint8)
+    hx = np.empty((h, w, 3), np.float32) if save_hdr else None
+    rgbe = np.empty((h, w, 4), np.uint8) if save_hdr else None
+    d = np.empty((h, w), np.float32) if save_depth else None
+    n = np.empty((h, w, 3), np.uint8) if save_normals else None
+    a = np.empty((h, w, 3), np.uint8) if save_albedo else None
+    img = _abi.Image(rgba=rgba.ctypes.data, hdr=None if hx is None else hx.ctypes.data,
+                     rgbe=None if rgbe is None else rgbe.ctypes.data, depth=None if d is None else d.ctypes.data,
+                     normal=None if n is None else n.ctypes.data, albedo=None if a is None else a.ctypes.data)
+    params = make_params(config, path=path, **kw)
+    pp = post_params(config)
+    st = _abi.Stats()
+    check(lib().lmn_render_image(world.device_scene(), C.byref(params), C.byref(pp), C.byref(img), C.byref(st)))
+    return ImageResult(pixels=rgba, hdr_data=hx, rgbe=rgbe, depth_pass=d, normal_pass=n, albedo_pass=a,
+                       exposure=img.exposure, total_rays=st.paths, render_time_secs=st.render_ms / 1000.0,
+                       post_ms=img.post_ms)
+
+
 def device_info():
     name = C.create_string_buffer(256)
     ncu = C.c_int()
@@ -339,5 +457,6 @@ def device_info():
     return {"name": name.value.decode(), "cu": ncu.value, "hbm_bytes": hbm.value}
 
 
-__all__ = ["RenderConfig", "SceneWorld", "RenderResult", "LuminaraError", "load_scene", "render",
-           "render_device", "trace_rays", "device_info", "make_params", "profile_extend"]
+__all__ = ["RenderConfig", "SceneWorld", "RenderResult", "ImageResult", "LuminaraError", "load_scene", "render",
+           "render_device", "render_image", "aov", "post", "encode_rgbe", "trace_rays", "device_info",
+           "make_params", "profile_extend"]

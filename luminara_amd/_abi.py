@@ -75,7 +75,7 @@ class RenderConfig(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("samples", C.c_uint32),
                 ("max_depth", C.c_uint32), ("seed", C.c_uint64), ("filter", C.c_uint32),
                 ("has_crop", C.c_uint32), ("crop", C.c_uint32 * 4), ("exposure", C.c_double),
-                ("tone_map", C.c_uint32), ("pad", C.c_uint32), ("background", Background)]
+                ("tone_map", C.c_uint32), ("auto_exposure", C.c_uint32), ("background", Background)]
 
 
 class SceneDesc(C.Structure):
@@ -140,6 +140,20 @@ class Grads(C.Structure):
                 ("paths", C.c_uint64), ("adjoint_ms", C.c_double)]
 
 
+TONE_ACES, TONE_REINHARD, TONE_FILMIC, TONE_NONE = 0, 1, 2, 3
+
+
+class PostParams(C.Structure):
+    _fields_ = [("exposure", C.c_double), ("tone_map", C.c_uint32), ("auto_exposure", C.c_uint32),
+                ("depth_fog", C.c_double), ("fog_color", D3)]
+
+
+class Image(C.Structure):
+    _fields_ = [("rgba", C.c_void_p), ("hdr", C.c_void_p), ("rgbe", C.c_void_p), ("depth", C.c_void_p),
+                ("normal", C.c_void_p), ("albedo", C.c_void_p), ("exposure", C.c_double),
+                ("log_average", C.c_double), ("post_ms", C.c_double)]
+
+
 # Every symbol include/luminara.h declares, with its ctypes signature.
 SIGNATURES = {
     "lmn_last_error": (C.c_char_p, []),
@@ -174,6 +188,16 @@ SIGNATURES = {
                               C.POINTER(Grads)]),
     "lmn_adjoint_device": (C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.POINTER(AdjointSpec), C.c_void_p,
                                      C.POINTER(Grads)]),
+    "lmn_aov": (C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lmn_aov_device": (C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lmn_post": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(PostParams), C.c_void_p, C.c_void_p,
+                           C.c_void_p, C.POINTER(C.c_double)]),
+    "lmn_post_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(PostParams), C.c_void_p,
+                                  C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
+    "lmn_encode_rgbe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "lmn_encode_rgbe_device": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "lmn_render_image": (C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.POINTER(PostParams), C.POINTER(Image),
+                                   C.POINTER(Stats)]),
 }
 
 _lib = None

@@ -19,7 +19,7 @@ LIB = os.path.join(OUT_DIR, "liblmn.so")
 CLI = os.path.join(OUT_DIR, "luminara-gpu")
 
 HOST_SOURCES = ["host/toml.cpp", "host/scene_load.cpp", "host/scene_build.cpp"]
-HIP_SOURCES = ["lmn_api.hip", "lmn_kernels.hip", "lmn_plain.hip", "lmn_adjoint.hip"]
+HIP_SOURCES = ["lmn_api.hip", "lmn_kernels.hip", "lmn_plain.hip", "lmn_adjoint.hip", "lmn_post.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LMN_OFFLOAD_ARCH", "gfx950")

@@ -3,12 +3,14 @@
 // library's C ABI.  Scene loading follows scene::load_scene
 // (lmn_load_scene); -w/-H/-s/-d/--seed override the [render] table like
 // src/main.rs:204-230 and -p/--preview like :231-237 (the camera keeps the
-// TOML aspect, :205-210).  The framebuffer goes through the reference's
-// default post chain -- exposure, tone map, sRGB, 8-bit
-// (src/render.rs:2594-2632, :3099-3202, :3606-3658) -- into PNG or PPM
-// (src/main.rs:774-810); --save-hdr writes Radiance RGBE after exposure
-// (:915-944, :948-989); --save-json the render statistics (:840-857).
-// Effects beyond the default chain are not part of this path.
+// TOML aspect, :205-210).  lmn_render_image runs the frame and, on the
+// device, the reference's default post chain -- exposure (--auto-exposure),
+// depth fog, tone map, sRGB, 8-bit (src/render.rs:2594-2665, :3099-3202,
+// :3606-3658) -- the AOV passes (:3508-3584) and the Radiance RGBE pixels;
+// this front end writes PNG or PPM (src/main.rs:774-810), --save-hdr
+// (:915-944, :948-989), --save-depth/--save-normals/--save-albedo
+// (:855-913) and --save-json (:840-857).  Effects beyond the default chain
+// are not part of this path.
 #include <zlib.h>
 
 #include <algorithm>
@@ -29,12 +31,12 @@
 namespace {
 
 struct Cli {
-  std::string scene, output = "output.png", save_hdr, save_json;
+  std::string scene, output = "output.png", save_hdr, save_json, save_depth, save_normals, save_albedo, tone_map;
   long width = -1, height = -1, samples = -1, depth = -1, threads = -1;
   long long seed = -1;
-  double exposure = -1.0;
-  bool quiet = false, preview = false, benchmark = false, help = false;
-  std::vector<double> post_check;
+  double exposure = -1.0, depth_fog = -1.0;
+  double fog_color[3] = {-1.0, -1.0, -1.0};
+  bool quiet = false, preview = false, benchmark = false, help = false, auto_exposure = false;
 };
 
 [[noreturn]] void die(const std::string& msg) {
@@ -55,6 +57,30 @@ void parse_num(const char* s, T& out) {
     long long v = std::strtoll(s, &end, 10);
     if (end != s && *end == '\0' && errno == 0 && v >= 0) out = T(v);
   }
+}
+
+// --fog-color R,G,B: three comma-separated numbers, else ignored (:1396-1404).
+void parse_rgb(const char* s, double out[3]) {
+  double v[3];
+  int n = 0;
+  std::string str(s);
+  size_t pos = 0;
+  while (n < 4) {
+    size_t comma = str.find(',', pos);
+    std::string tok = str.substr(pos, comma == std::string::npos ? std::string::npos : comma - pos);
+    size_t a = tok.find_first_not_of(" \t"), b = tok.find_last_not_of(" \t");
+    tok = a == std::string::npos ? "" : tok.substr(a, b - a + 1);
+    char* end = nullptr;
+    double x = std::strtod(tok.c_str(), &end);
+    if (!tok.empty() && *end == '\0') {  // filter_map keeps the values that parse
+      if (n < 3) v[n] = x;
+      ++n;
+    }
+    if (comma == std::string::npos) break;
+    pos = comma + 1;
+  }
+  if (n == 3)
+    for (int k = 0; k < 3; ++k) out[k] = v[k];
 }
 
 Cli parse(int argc, char** argv) {
@@ -78,6 +104,14 @@ Cli parse(int argc, char** argv) {
       if (const char* v = next()) parse_num(v, c.threads);
     } else if (a == "-e" || a == "--exposure") {
       if (const char* v = next()) parse_num(v, c.exposure);
+    } else if (a == "--auto-exposure") {
+      c.auto_exposure = true;
+    } else if (a == "--tone-map") {
+      if (const char* v = next()) c.tone_map = v;
+    } else if (a == "--depth-fog") {
+      if (const char* v = next()) parse_num(v, c.depth_fog);
+    } else if (a == "--fog-color") {
+      if (const char* v = next()) parse_rgb(v, c.fog_color);
     } else if (a == "-q" || a == "--quiet") {
       c.quiet = true;
     } else if (a == "-p" || a == "--preview") {
@@ -86,19 +120,16 @@ Cli parse(int argc, char** argv) {
       if (const char* v = next()) c.save_hdr = v;
     } else if (a == "--save-json") {
       if (const char* v = next()) c.save_json = v;
+    } else if (a == "--save-depth") {
+      if (const char* v = next()) c.save_depth = v;
+    } else if (a == "--save-normals") {
+      if (const char* v = next()) c.save_normals = v;
+    } else if (a == "--save-albedo") {
+      if (const char* v = next()) c.save_albedo = v;
     } else if (a == "--benchmark" || a == "--bench") {
       c.benchmark = true;
     } else if (a == "-h" || a == "--help") {
       c.help = true;
-    } else if (a == "--post-check") {  // post chain on the given values (no render)
-      while (i + 1 < argc) {
-        double v;
-        char* end = nullptr;
-        v = std::strtod(argv[i + 1], &end);
-        if (end == argv[i + 1] || *end != '\0') break;
-        c.post_check.push_back(v);
-        ++i;
-      }
     } else if (!a.empty() && a[0] != '-' && c.scene.empty()) {
       c.scene = a;
     } else {
@@ -106,42 +137,6 @@ Cli parse(int argc, char** argv) {
     }
   }
   return c;
-}
-
-// ---- default post chain ---------------------------------------------------
-// Tone-map curves: ACES filmic fit (Narkowicz 2015), Reinhard x/(1+x),
-// Uncharted 2 filmic (Hable 2010) normalised at white point 11.2; then the
-// sRGB transfer function and the reference's 8-bit quantisation
-// floor(min(max(s, 0), 0.999) * 256) (src/render.rs:3606-3658).
-double tone_aces(double x) {
-  x = std::max(x, 0.0);
-  const double a = 2.51, b = 0.03, c = 2.43, d = 0.59, e = 0.14;
-  return std::min(std::max((x * (a * x + b)) / (x * (c * x + d) + e), 0.0), 1.0);
-}
-double tone_reinhard(double x) {
-  x = std::max(x, 0.0);
-  return x / (1.0 + x);
-}
-double hable(double v) {
-  const double A = 0.15, B = 0.50, C = 0.10, D = 0.20, E = 0.02, F = 0.30;
-  return ((v * (A * v + C * B) + D * E) / (v * (A * v + B) + D * F)) - E / F;
-}
-double tone_filmic(double x) {
-  x = std::max(x, 0.0);
-  return std::min(std::max(hable(x) / hable(11.2), 0.0), 1.0);
-}
-double tone(uint32_t mode, double x) {
-  switch (mode) {
-    case 0: return tone_aces(x);
-    case 1: return tone_reinhard(x);
-    case 2: return tone_filmic(x);
-    default: return std::min(std::max(x, 0.0), 1.0);
-  }
-}
-uint8_t srgb8(double x) {
-  x = std::min(std::max(x, 0.0), 1.0);
-  double s = x <= 0.0031308 ? 12.92 * x : 1.055 * std::pow(x, 1.0 / 2.4) - 0.055;
-  return uint8_t(std::min(std::max(s, 0.0), 0.999) * 256.0);
 }
 
 // ---- writers --------------------------------------------------------------
@@ -186,28 +181,46 @@ std::string ppm_text(uint32_t w, uint32_t h, const std::vector<uint8_t>& rgba) {
     s << int(rgba[i * 4]) << " " << int(rgba[i * 4 + 1]) << " " << int(rgba[i * 4 + 2]) << "\n";
   return s.str();
 }
-// Radiance RGBE, flat (no run-length), as src/main.rs:948-989 writes it.
-bool write_rgbe(const std::string& path, uint32_t w, uint32_t h, const std::vector<float>& hdr) {
+// Radiance RGBE, flat (no run-length), as src/main.rs:948-989 writes it; the
+// pixels come encoded from the device.
+bool write_rgbe(const std::string& path, uint32_t w, uint32_t h, const std::vector<uint8_t>& rgbe) {
   std::ostringstream head;
   head << "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y " << h << " +X " << w << "\n";
   std::string out = head.str();
-  for (size_t i = 0; i < size_t(w) * h; ++i) {
-    float r = std::max(hdr[i * 3], 0.f), g = std::max(hdr[i * 3 + 1], 0.f), b = std::max(hdr[i * 3 + 2], 0.f);
-    float m = std::max(r, std::max(g, b));
-    if (m < 1e-32f) {
-      out.append(4, '\0');
-      continue;
-    }
-    int e = int(std::ceil(std::log2(m)));
-    float scale = 256.0f / std::pow(2.0f, float(e));
-    out.push_back(char(uint8_t(std::min(r * scale, 255.0f))));
-    out.push_back(char(uint8_t(std::min(g * scale, 255.0f))));
-    out.push_back(char(uint8_t(std::min(b * scale, 255.0f))));
-    out.push_back(char(uint8_t(e + 128)));
-  }
+  out.append(reinterpret_cast<const char*>(rgbe.data()), rgbe.size());
   std::ofstream f(path, std::ios::binary);
   f << out;
   return bool(f);
+}
+// Depth pass image (src/main.rs:856-876): grey = depth * 255 / max depth.
+std::vector<uint8_t> depth_rgba(const std::vector<float>& d) {
+  float mx = 0.0f;
+  for (float v : d) mx = std::fmax(mx, v);
+  float norm = mx > 0.0f ? 255.0f / mx : 1.0f;
+  std::vector<uint8_t> px(d.size() * 4);
+  for (size_t i = 0; i < d.size(); ++i) {
+    float x = d[i] * norm;
+    x = x < 0.0f ? 0.0f : (x > 255.0f ? 255.0f : x);
+    uint8_t v = x > 0.0f ? uint8_t(x) : 0;  // NaN -> 0
+    px[4 * i] = px[4 * i + 1] = px[4 * i + 2] = v;
+    px[4 * i + 3] = 255;
+  }
+  return px;
+}
+std::vector<uint8_t> rgb_to_rgba(const std::vector<uint8_t>& rgb) {
+  std::vector<uint8_t> px(rgb.size() / 3 * 4);
+  for (size_t i = 0; i < rgb.size() / 3; ++i) {
+    for (int k = 0; k < 3; ++k) px[4 * i + k] = rgb[3 * i + k];
+    px[4 * i + 3] = 255;
+  }
+  return px;
+}
+void save_pass(const std::string& path, const char* what, uint32_t w, uint32_t h, const std::vector<uint8_t>& px) {
+  if (path.empty()) return;
+  if (!write_png(path, w, h, px))
+    std::fprintf(stderr, "Error: failed to save %s pass '%s'\n", what, path.c_str());
+  else
+    std::fprintf(stderr, "Saved %s pass to %s\n", what, path.c_str());
 }
 
 std::string read_file(const std::string& path, bool& ok) {
@@ -221,19 +234,26 @@ std::string read_file(const std::string& path, bool& ok) {
 void usage() {
   std::fprintf(stderr,
                "Usage: luminara-gpu scene.toml [options]\n\n"
-               "  -o, --output FILE     PNG output, or PPM for '.ppm' / '-' (stdout) [output.png]\n"
-               "  -w, --width N         image width (the camera keeps the scene's aspect)\n"
-               "  -H, --height N        image height\n"
-               "  -s, --samples N       samples per pixel (rounded up to a square)\n"
-               "  -d, --depth N         maximum bounce depth\n"
-               "      --seed N          random seed\n"
-               "  -e, --exposure X      exposure multiplier\n"
-               "  -p, --preview         1/4 resolution, <= 4 spp, depth <= 10\n"
-               "  -q, --quiet           no progress output\n"
-               "  -t, --threads N       accepted; the render runs on the GPU\n"
-               "      --save-hdr FILE   Radiance HDR of the exposed framebuffer\n"
-               "      --save-json FILE  render statistics\n"
-               "      --benchmark       the scene at 400x225, 64 spp; prints Mrays/s\n");
+               "  -o, --output FILE       PNG output, or PPM for '.ppm' / '-' (stdout) [output.png]\n"
+               "  -w, --width N           image width (the camera keeps the scene's aspect)\n"
+               "  -H, --height N          image height\n"
+               "  -s, --samples N         samples per pixel (rounded up to a square)\n"
+               "  -d, --depth N           maximum bounce depth\n"
+               "      --seed N            random seed\n"
+               "  -e, --exposure X        exposure multiplier\n"
+               "      --auto-exposure     exposure from the log-average luminance\n"
+               "      --tone-map NAME     aces, reinhard, filmic (uncharted2) or none\n"
+               "      --depth-fog N       atmospheric depth fog density (e.g. 0.1)\n"
+               "      --fog-color R,G,B   fog colour (0-1 each, default: 0.8,0.85,0.9)\n"
+               "  -p, --preview           1/4 resolution, <= 4 spp, depth <= 10\n"
+               "  -q, --quiet             no progress output\n"
+               "  -t, --threads N         accepted; the render runs on the GPU\n"
+               "      --save-hdr FILE     Radiance HDR of the exposed framebuffer\n"
+               "      --save-depth FILE   depth pass image\n"
+               "      --save-normals FILE normal pass image\n"
+               "      --save-albedo FILE  albedo pass image\n"
+               "      --save-json FILE    render statistics\n"
+               "      --benchmark         the scene at 400x225, 64 spp; prints Mrays/s\n");
 }
 
 }  // namespace
@@ -242,11 +262,6 @@ int main(int argc, char** argv) {
   Cli cli = parse(argc, argv);
   if (cli.help) {
     usage();
-    return 0;
-  }
-  if (!cli.post_check.empty()) {
-    for (double v : cli.post_check)
-      std::printf("%.17g %.17g %.17g %.17g %d\n", v, tone_aces(v), tone_reinhard(v), tone_filmic(v), int(srgb8(v)));
     return 0;
   }
   const auto t_start = std::chrono::steady_clock::now();
@@ -275,6 +290,33 @@ int main(int argc, char** argv) {
     rc.height = std::max<uint32_t>(rc.height / 4, 1);
     rc.samples = std::min<uint32_t>(rc.samples, 4);
     rc.max_depth = std::min<uint32_t>(rc.max_depth, 10);
+  }
+  if (cli.auto_exposure) rc.auto_exposure = 1;
+  lmn_post_params post;
+  std::memset(&post, 0, sizeof post);
+  post.exposure = rc.exposure;
+  post.tone_map = rc.tone_map;
+  post.auto_exposure = rc.auto_exposure;
+  post.depth_fog = cli.depth_fog >= 0.0 ? cli.depth_fog : 0.0;
+  const double fog_default[3] = {0.8, 0.85, 0.9};
+  for (int k = 0; k < 3; ++k) post.fog_color[k] = cli.fog_color[0] >= 0.0 || cli.fog_color[1] >= 0.0 ||
+                                                          cli.fog_color[2] >= 0.0
+                                                      ? cli.fog_color[k]
+                                                      : fog_default[k];
+  if (!cli.tone_map.empty()) {  // src/main.rs:550-561
+    const std::string& t = cli.tone_map;
+    if (t == "aces")
+      post.tone_map = LMN_TONE_ACES;
+    else if (t == "reinhard")
+      post.tone_map = LMN_TONE_REINHARD;
+    else if (t == "filmic" || t == "uncharted2")
+      post.tone_map = LMN_TONE_FILMIC;
+    else if (t == "none")
+      post.tone_map = LMN_TONE_NONE;
+    else {
+      std::fprintf(stderr, "Warning: unknown tone map '%s', using ACES\n", t.c_str());
+      post.tone_map = LMN_TONE_ACES;
+    }
   }
   char dev[256] = {0};
   int n_cu = 0;
@@ -309,10 +351,33 @@ int main(int argc, char** argv) {
   if (!cli.quiet && !cli.benchmark)
     std::fprintf(stderr, "Rendering %ux%u (%.1fMP) @ %u spp, max depth %u, on %s (%d CUs)\n", rc.width, rc.height,
                  double(out_w) * out_h / 1e6, rc.samples, rc.max_depth, dev, n_cu);
-  std::vector<float> hdr(size_t(std::max(out_w, 1u)) * std::max(out_h, 1u) * 3);
+  const size_t npix = size_t(std::max(out_w, 1u)) * std::max(out_h, 1u);
+  std::vector<uint8_t> rgba(npix * 4), rgbe, normal, albedo;
+  std::vector<float> hdr, depth;
+  lmn_image img;
+  std::memset(&img, 0, sizeof img);
+  img.rgba = rgba.data();
+  if (!cli.save_hdr.empty() && !cli.benchmark) {
+    hdr.resize(npix * 3);
+    rgbe.resize(npix * 4);
+    img.hdr = hdr.data();
+    img.rgbe = rgbe.data();
+  }
+  if (!cli.save_depth.empty() && !cli.benchmark) {
+    depth.resize(npix);
+    img.depth = depth.data();
+  }
+  if (!cli.save_normals.empty() && !cli.benchmark) {
+    normal.resize(npix * 3);
+    img.normal = normal.data();
+  }
+  if (!cli.save_albedo.empty() && !cli.benchmark) {
+    albedo.resize(npix * 3);
+    img.albedo = albedo.data();
+  }
   lmn_stats st;
   std::memset(&st, 0, sizeof st);
-  if (lmn_render(scene, &p, hdr.data(), &st) != 0) die(std::string("render failed: ") + lmn_last_error());
+  if (lmn_render_image(scene, &p, &post, &img, &st) != 0) die(std::string("render failed: ") + lmn_last_error());
   lmn_scene_destroy(scene);
   const double trace_s = st.render_ms / 1e3;
   if (cli.benchmark) {
@@ -323,18 +388,15 @@ int main(int argc, char** argv) {
                  sq, sq);
     return 0;
   }
+  if (post.auto_exposure && !cli.quiet)  // src/render.rs:2595-2605
+    std::fprintf(stderr, "Auto-exposure: %.3f (scene avg luminance: %.4f)\n", img.exposure / post.exposure,
+                 img.log_average);
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
   const double mrays = double(st.paths) / trace_s / 1e6;
   if (!cli.quiet)
     std::fprintf(stderr, "Rendered in %.2fs (trace: %.2fs, %.1f Mrays/s, %llu total rays)\n", secs, trace_s, mrays,
                  (unsigned long long)st.paths);
 
-  // default post chain: exposure, tone map, sRGB, 8-bit RGBA
-  std::vector<uint8_t> rgba(size_t(out_w) * out_h * 4);
-  for (size_t i = 0; i < size_t(out_w) * out_h; ++i) {
-    for (int k = 0; k < 3; ++k) rgba[i * 4 + k] = srgb8(tone(rc.tone_map, double(hdr[i * 3 + k]) * rc.exposure));
-    rgba[i * 4 + 3] = 255;
-  }
   const std::string& out = cli.output;
   bool ppm = out == "-" || (out.size() >= 4 && out.compare(out.size() - 4, 4, ".ppm") == 0);
   if (ppm) {
@@ -369,11 +431,12 @@ int main(int argc, char** argv) {
     else if (!cli.quiet)
       std::fprintf(stderr, "Stats saved to %s\n", cli.save_json.c_str());
   }
+  if (!cli.save_depth.empty()) save_pass(cli.save_depth, "depth", out_w, out_h, depth_rgba(depth));
+  if (!cli.save_normals.empty()) save_pass(cli.save_normals, "normal", out_w, out_h, rgb_to_rgba(normal));
+  if (!cli.save_albedo.empty()) save_pass(cli.save_albedo, "albedo", out_w, out_h, rgb_to_rgba(albedo));
   if (!cli.save_hdr.empty()) {
-    std::vector<float> exposed(hdr.size());
-    for (size_t i = 0; i < hdr.size(); ++i) exposed[i] = float(double(hdr[i]) * rc.exposure);
-    if (!write_rgbe(cli.save_hdr, out_w, out_h, exposed)) die("failed to save HDR '" + cli.save_hdr + "'");
-    if (!cli.quiet) std::fprintf(stderr, "Saved HDR to %s\n", cli.save_hdr.c_str());
+    if (!write_rgbe(cli.save_hdr, out_w, out_h, rgbe)) die("failed to save HDR '" + cli.save_hdr + "'");
+    std::fprintf(stderr, "Saved HDR to %s\n", cli.save_hdr.c_str());
   }
   return 0;
 }

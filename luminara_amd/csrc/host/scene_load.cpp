@@ -192,6 +192,7 @@ class Loader {
     c.has_crop = 0;
     c.exposure = 1.0;
     c.tone_map = 0;
+    c.auto_exposure = 0;
     c.background = lmn_background{};
     c.background.kind = LMN_BG_SKY;
     const TomlValue* r = root.get("render");
@@ -204,6 +205,10 @@ class Loader {
     if (auto v = r->get("max_depth")) c.max_depth = uint32_t(uint_field(*v, ctx + ".max_depth", 0xffffffffu));
     if (auto v = r->get("seed")) c.seed = uint_field(*v, ctx + ".seed", ~0ull);
     opt_num(*r, "exposure", ctx, c.exposure);
+    if (const TomlValue* ae = r->get("auto_exposure")) {  // src/scene.rs:870-872
+      if (ae->kind != TomlValue::BOOLEAN) fail(LMN_ERR_PARSE, "render.auto_exposure: expected a boolean");
+      c.auto_exposure = ae->boolean ? 1u : 0u;
+    }
     bool present = false;
     std::string pf = opt_str(*r, "pixel_filter", ctx, present);
     if (present) {  // src/scene.rs:906-913

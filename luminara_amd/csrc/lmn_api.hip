@@ -1187,4 +1187,231 @@ int lmn_adjoint(const lmn_scene* scene, const lmn_render_params* params, const l
   return rc;
 }
 
+
+// ---- image outputs: AOV ray, post chain, RGBE -------------------------------
+namespace {
+
+int pick_stream(void* s, hipStream_t& out) {
+  if (s) {
+    out = static_cast<hipStream_t>(s);
+    return LMN_OK;
+  }
+  Workspace& W = workspace();
+  std::lock_guard<std::mutex> g(W.mu);
+  int rc = W.init();
+  if (rc != LMN_OK) return rc;
+  out = W.stream;
+  return LMN_OK;
+}
+
+int aov_enqueue(const lmn_scene* scene, const lmn_render_params* params, const FrameSetup& fs, float* d_depth,
+                uint8_t* d_normal, uint8_t* d_albedo, hipStream_t st) {
+  if (params->intersect_mode > LMN_ISECT_REFERENCE) return set_err(LMN_ERR_INVALID, "lmn_aov: unknown intersect mode");
+  if (params->path == LMN_PATH_PLAIN_FP64) {
+    SceneView<double> S = scene->viewD;
+    S.exact_marched = params->intersect_mode == LMN_ISECT_EXACT;
+    launch_aov_fp64(S, scene->host.featD, fs.F, d_depth, d_normal, d_albedo, st);
+  } else {
+    SceneView<float> S = scene->viewF;
+    S.exact_marched = params->intersect_mode == LMN_ISECT_EXACT;
+    launch_aov_fp32(S, scene->host.featF, fs.F, d_depth, d_normal, d_albedo, st);
+  }
+  HIP_TRY(hipGetLastError());
+  return LMN_OK;
+}
+
+// Enqueues the post chain; with auto-exposure, scratch holds the log-average
+// luminance and the applied exposure when the stream reaches the end.
+int post_enqueue(const float* d_hdr, uint64_t n, const lmn_post_params* post, const float* d_fog, uint8_t* d_rgba,
+                 float* d_hdr_exposed, DevBuf& scratch, hipStream_t st) {
+  if (!post) return set_err(LMN_ERR_INVALID, "lmn_post: null post parameters");
+  if (post->tone_map > LMN_TONE_NONE) return set_err(LMN_ERR_INVALID, "lmn_post: unknown tone map");
+  const bool fog = post->depth_fog > 0.0;
+  if (fog && !d_fog) return set_err(LMN_ERR_INVALID, "lmn_post: depth fog needs the depth output of lmn_aov");
+  PostArgs A{};
+  A.hdr = d_hdr;
+  A.fog = fog ? d_fog : nullptr;
+  A.exposure = post->exposure;
+  A.fog_density = post->depth_fog;
+  for (int k = 0; k < 3; ++k) A.fog_rgb[k] = post->fog_color[k];
+  A.tone = post->tone_map;
+  A.n = n;
+  A.rgba = d_rgba;
+  A.hdr_exposed = d_hdr_exposed;
+  if (post->auto_exposure) {
+    HIP_TRY(scratch.ensure((2 + kLogLumBlocks) * sizeof(double)));
+    launch_log_luminance(d_hdr, n, post->exposure, scratch.as<double>(), st);
+    A.exposure_dev = scratch.as<double>();
+  }
+  if (d_rgba || d_hdr_exposed) launch_post(A, st);
+  HIP_TRY(hipGetLastError());
+  return LMN_OK;
+}
+
+int read_exposure(const lmn_post_params* post, const DevBuf& scratch, hipStream_t st, double& used,
+                  double* log_average = nullptr) {
+  used = post->exposure;
+  if (log_average) *log_average = 0.0;
+  if (post->auto_exposure) {
+    double two[2];
+    HIP_TRY(hipMemcpyAsync(two, scratch.p, sizeof two, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    used = two[1];
+    if (log_average) *log_average = two[0];
+  }
+  return LMN_OK;
+}
+
+int to_host(void* host, const DevBuf& d, size_t bytes) {
+  if (host && bytes) HIP_TRY(hipMemcpy(host, d.p, bytes, hipMemcpyDeviceToHost));
+  return LMN_OK;
+}
+
+}  // namespace
+
+int lmn_aov_device(const lmn_scene* scene, const lmn_render_params* params, float* d_depth, uint8_t* d_normal,
+                   uint8_t* d_albedo) {
+  if (!scene || !params) return set_err(LMN_ERR_INVALID, "lmn_aov: null argument");
+  FrameSetup fs;
+  int rc = frame_setup(params, fs);
+  hipStream_t st = nullptr;
+  if (rc == LMN_OK) rc = pick_stream(params->stream, st);
+  if (rc == LMN_OK) rc = aov_enqueue(scene, params, fs, d_depth, d_normal, d_albedo, st);
+  if (rc != LMN_OK) return rc;
+  if (!params->stream) HIP_TRY(hipStreamSynchronize(st));
+  return LMN_OK;
+}
+
+int lmn_aov(const lmn_scene* scene, const lmn_render_params* params, float* depth, uint8_t* normal,
+            uint8_t* albedo) {
+  if (!scene || !params) return set_err(LMN_ERR_INVALID, "lmn_aov: null argument");
+  FrameSetup fs;
+  int rc = frame_setup(params, fs);
+  if (rc != LMN_OK) return rc;
+  const size_t n = fs.npix;
+  DevBuf dd, dn, da;
+  if (depth) HIP_TRY(dd.ensure(n * 4));
+  if (normal) HIP_TRY(dn.ensure(n * 3));
+  if (albedo) HIP_TRY(da.ensure(n * 3));
+  rc = lmn_aov_device(scene, params, dd.as<float>(), dn.as<uint8_t>(), da.as<uint8_t>());
+  if (rc == LMN_OK) rc = to_host(depth, dd, n * 4);
+  if (rc == LMN_OK) rc = to_host(normal, dn, n * 3);
+  if (rc == LMN_OK) rc = to_host(albedo, da, n * 3);
+  return rc;
+}
+
+int lmn_post_device(const float* d_hdr, uint32_t width, uint32_t height, const lmn_post_params* post,
+                    const float* d_fog_depth, uint8_t* d_rgba, float* d_hdr_exposed, double* exposure_used,
+                    void* stream) {
+  if (!d_hdr || !post || width == 0 || height == 0) return set_err(LMN_ERR_INVALID, "lmn_post: bad argument");
+  hipStream_t st = nullptr;
+  int rc = pick_stream(stream, st);
+  DevBuf scratch;
+  if (rc == LMN_OK) rc = post_enqueue(d_hdr, uint64_t(width) * height, post, d_fog_depth, d_rgba, d_hdr_exposed, scratch, st);
+  double used = 0.0;
+  if (rc == LMN_OK) rc = read_exposure(post, scratch, st, used);  // waits for the stream with auto-exposure
+  if (rc != LMN_OK) return rc;
+  if (!stream) HIP_TRY(hipStreamSynchronize(st));
+  if (exposure_used) *exposure_used = used;
+  return LMN_OK;
+}
+
+int lmn_post(const float* hdr, uint32_t width, uint32_t height, const lmn_post_params* post, const float* fog_depth,
+             uint8_t* rgba, float* hdr_exposed, double* exposure_used) {
+  if (!hdr || !post || width == 0 || height == 0) return set_err(LMN_ERR_INVALID, "lmn_post: bad argument");
+  const size_t n = size_t(width) * height;
+  const bool fog = post->depth_fog > 0.0;
+  if (fog && !fog_depth) return set_err(LMN_ERR_INVALID, "lmn_post: depth fog needs the depth output of lmn_aov");
+  DevBuf dh, df, dr, dx;
+  HIP_TRY(dh.ensure(n * 12));
+  HIP_TRY(hipMemcpy(dh.p, hdr, n * 12, hipMemcpyHostToDevice));
+  if (fog) {
+    HIP_TRY(df.ensure(n * 4));
+    HIP_TRY(hipMemcpy(df.p, fog_depth, n * 4, hipMemcpyHostToDevice));
+  }
+  if (rgba) HIP_TRY(dr.ensure(n * 4));
+  if (hdr_exposed) HIP_TRY(dx.ensure(n * 12));
+  int rc = lmn_post_device(dh.as<float>(), width, height, post, df.as<float>(), dr.as<uint8_t>(), dx.as<float>(),
+                           exposure_used, nullptr);
+  if (rc == LMN_OK) rc = to_host(rgba, dr, n * 4);
+  if (rc == LMN_OK) rc = to_host(hdr_exposed, dx, n * 12);
+  return rc;
+}
+
+int lmn_encode_rgbe_device(const float* d_rgb, uint64_t n_pixels, uint8_t* d_rgbe, void* stream) {
+  if ((!d_rgb || !d_rgbe) && n_pixels) return set_err(LMN_ERR_INVALID, "lmn_encode_rgbe: null argument");
+  hipStream_t st = nullptr;
+  int rc = pick_stream(stream, st);
+  if (rc != LMN_OK) return rc;
+  launch_rgbe(d_rgb, n_pixels, d_rgbe, st);
+  HIP_TRY(hipGetLastError());
+  if (!stream) HIP_TRY(hipStreamSynchronize(st));
+  return LMN_OK;
+}
+
+int lmn_encode_rgbe(const float* rgb, uint64_t n_pixels, uint8_t* rgbe) {
+  if ((!rgb || !rgbe) && n_pixels) return set_err(LMN_ERR_INVALID, "lmn_encode_rgbe: null argument");
+  if (n_pixels == 0) return LMN_OK;
+  DevBuf di, dout;
+  HIP_TRY(di.ensure(n_pixels * 12));
+  HIP_TRY(dout.ensure(n_pixels * 4));
+  HIP_TRY(hipMemcpy(di.p, rgb, n_pixels * 12, hipMemcpyHostToDevice));
+  int rc = lmn_encode_rgbe_device(di.as<float>(), n_pixels, dout.as<uint8_t>(), nullptr);
+  if (rc == LMN_OK) rc = to_host(rgbe, dout, n_pixels * 4);
+  return rc;
+}
+
+int lmn_render_image(const lmn_scene* scene, const lmn_render_params* params, const lmn_post_params* post,
+                     lmn_image* img, lmn_stats* stats) {
+  if (!scene || !params || !post || !img) return set_err(LMN_ERR_INVALID, "lmn_render_image: null argument");
+  FrameSetup fs;
+  int rc = frame_setup(params, fs);
+  if (rc != LMN_OK) return rc;
+  const size_t n = fs.npix;
+  hipStream_t st = nullptr;
+  rc = pick_stream(params->stream, st);
+  if (rc != LMN_OK) return rc;
+  DevBuf hdr, dep, nrm, alb, rgba, hx, rgbe, scratch;
+  HIP_TRY(hdr.ensure(n * 12));
+  rc = lmn_render_device(scene, params, hdr.as<float>(), stats);
+  if (rc != LMN_OK) return rc;
+  const bool fog = post->depth_fog > 0.0;
+  if (fog || img->depth) HIP_TRY(dep.ensure(n * 4));
+  if (img->normal) HIP_TRY(nrm.ensure(n * 3));
+  if (img->albedo) HIP_TRY(alb.ensure(n * 3));
+  if (img->rgba) HIP_TRY(rgba.ensure(n * 4));
+  if (img->hdr || img->rgbe) HIP_TRY(hx.ensure(n * 12));
+  if (img->rgbe) HIP_TRY(rgbe.ensure(n * 4));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return set_err(LMN_ERR_DEVICE, "lmn_render_image: hipEventCreate failed");
+  }
+  (void)hipEventRecord(e0, st);
+  if (dep.p || nrm.p || alb.p) rc = aov_enqueue(scene, params, fs, dep.as<float>(), nrm.as<uint8_t>(), alb.as<uint8_t>(), st);
+  if (rc == LMN_OK) rc = post_enqueue(hdr.as<float>(), n, post, dep.as<float>(), rgba.as<uint8_t>(), hx.as<float>(), scratch, st);
+  if (rc == LMN_OK && img->rgbe) launch_rgbe(hx.as<float>(), n, rgbe.as<uint8_t>(), st);
+  if (rc == LMN_OK && img->depth) launch_depth_pass(dep.as<float>(), n, st);
+  hipError_t e = hipSuccess;
+  if (rc == LMN_OK) e = hipGetLastError();
+  if (rc == LMN_OK && e == hipSuccess) e = hipEventRecord(e1, st);
+  if (rc == LMN_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (rc == LMN_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != LMN_OK) return rc;
+  if (e != hipSuccess) return set_err(LMN_ERR_DEVICE, std::string("lmn_render_image: ") + hipGetErrorString(e));
+  img->post_ms = ms;
+  rc = read_exposure(post, scratch, st, img->exposure, &img->log_average);
+  if (rc == LMN_OK) rc = to_host(img->rgba, rgba, n * 4);
+  if (rc == LMN_OK) rc = to_host(img->hdr, hx, n * 12);
+  if (rc == LMN_OK) rc = to_host(img->rgbe, rgbe, n * 4);
+  if (rc == LMN_OK) rc = to_host(img->depth, dep, n * 4);
+  if (rc == LMN_OK) rc = to_host(img->normal, nrm, n * 3);
+  if (rc == LMN_OK) rc = to_host(img->albedo, alb, n * 3);
+  return rc;
+}
+
 }  // extern "C"

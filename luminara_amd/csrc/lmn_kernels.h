@@ -147,4 +147,32 @@ void launch_trace_fp32(const SceneView<float>& S, uint32_t feat, const lmn_ray* 
 void launch_trace_fp64(const SceneView<double>& S, uint32_t feat, const lmn_ray* rays, lmn_hit* hits, size_t n,
                        hipStream_t s);
 
+// AOV passes / depth-fog buffer: one pixel-centre ray per crop pixel (lmn_plain.hip).
+void launch_aov_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, float* depth, uint8_t* normal,
+                     uint8_t* albedo, hipStream_t s);
+void launch_aov_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, float* depth, uint8_t* normal,
+                     uint8_t* albedo, hipStream_t s);
+
+// Post chain and HDR encoding (lmn_post.hip).
+struct PostArgs {
+  const float* hdr;           // n * 3 linear HDR
+  const float* fog;           // n hit distances (+inf = escaped), null = no fog
+  const double* exposure_dev; // [1] = exposure from the automatic exposure, null = `exposure`
+  double exposure;
+  double fog_density;
+  double fog_rgb[3];
+  uint32_t tone;              // LMN_TONE_*
+  uint32_t pad;
+  uint64_t n;
+  uint8_t* rgba;              // n * 4
+  float* hdr_exposed;         // n * 3, or null
+};
+constexpr uint32_t kLogLumBlocks = 1024;
+void launch_post(const PostArgs& A, hipStream_t s);
+// scratch: 2 + kLogLumBlocks doubles; scratch[0] = log-average luminance,
+// scratch[1] = exposure * key factor.
+void launch_log_luminance(const float* hdr, uint64_t n, double exposure, double* scratch, hipStream_t s);
+void launch_rgbe(const float* rgb, uint64_t n, uint8_t* out, hipStream_t s);
+void launch_depth_pass(float* depth, uint64_t n, hipStream_t s);
+
 }  // namespace lmn

@@ -188,6 +188,16 @@ LMN_HD uint64_t frame_key(uint64_t seed, uint32_t scene_key) {
 LMN_HD uint64_t path_key(uint64_t fkey, uint64_t pixel, uint32_t sample) {
   return mix64(fkey ^ mix64((pixel << 24) ^ uint64_t(sample) ^ 0x8CB92BA72F3D8DD7ULL));
 }
+// Rust float semantics of the image outputs: f64::clamp keeps a NaN, and
+// `f64 as u8` saturates, truncates toward zero and maps NaN to 0.
+LMN_HD double clamp_keep_nan(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+LMN_HD uint8_t sat_u8(double x) { return !(x > 0.0) ? uint8_t(0) : (x >= 255.0 ? uint8_t(255) : uint8_t(x)); }
+
+// Stream of a pixel's single AOV / depth-fog ray (the reference seeds it per
+// pixel, apart from the sample streams).
+LMN_HD uint64_t aov_key(uint64_t fkey, uint64_t pixel) {
+  return mix64(fkey ^ mix64(pixel ^ 0x3C6EF372FE94F82BULL) ^ 0xA4093822299F31D0ULL);
+}
 
 template <class R> LMN_HD R u01_from(uint64_t x);
 template <> LMN_HD double u01_from<double>(uint64_t x) { return double(x >> 11) * 0x1.0p-53; }

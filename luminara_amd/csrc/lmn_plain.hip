@@ -127,6 +127,61 @@ __global__ void k_trace(SceneView<R> S, const lmn_ray* __restrict__ rays, lmn_hi
   hits[i] = h;
 }
 
+// ---------------------------------------------------------------------------
+// AOV passes and the depth-fog buffer (src/render.rs:3508-3584, :2634-2665):
+// one ray per crop pixel through the pixel centre, u = (i + 0.5) / (W - 1),
+// v = (y + 0.5) / (H - 1), from the pixel's own stream; closest hit over
+// [0.001, inf).  depth: the hit's t (units of the unnormalised camera
+// direction), +inf where the ray escapes; normal: the facing normal mapped to
+// bytes by (n * 0.5 + 0.5) * 255; albedo: the attenuation of one scatter, or,
+// where the material does not scatter, its emission divided by max(max(e), 1);
+// normal and albedo bytes are 0 where the ray escapes.
+template <class R, uint32_t FEAT>
+__global__ __launch_bounds__(128) void k_aov(SceneView<R> S, FrameDev F, float* __restrict__ depth,
+                                             uint8_t* __restrict__ normal, uint8_t* __restrict__ albedo) {
+  uint32_t npix = F.crop_w * F.crop_h;
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  uint32_t gi = p % F.crop_w + F.crop_x, gj = p / F.crop_w + F.crop_y;
+  Rng rng;
+  rng.key = aov_key(F.fkey, uint64_t(gj) * F.full_w + gi);
+  rng.n = 0;
+  R uc = (R(gi) + R(0.5)) / R(F.full_w - 1);
+  R vc = (R(F.full_h - 1 - gj) + R(0.5)) / R(F.full_h - 1);
+  V3<R> o, d;
+  camera_ray(S.cam, uc, vc, rng, o, d);
+  PrivateStack<R> st;
+  R t;
+  uint32_t idx;
+  float dep = INFINITY;
+  uint8_t nb[3] = {0, 0, 0}, ab[3] = {0, 0, 0};
+  if (trace<R, FEAT, false>(S, o, d, R(0.001), R(INFINITY), t, idx, st)) {
+    Surf<R> sf;
+    surface<R, FEAT>(S, idx, o, d, t, sf);
+    dep = float(t);
+    const double n3[3] = {double(sf.n.x), double(sf.n.y), double(sf.n.z)};
+    for (int k = 0; k < 3; ++k) nb[k] = sat_u8(clamp_keep_nan((n3[k] * 0.5 + 0.5) * 255.0, 0.0, 255.0));
+    if (albedo) {
+      const MaterialT<R>& m = S.materials[sf.mat];
+      V3<R> ndir, att;
+      if (scatter(S, m, d, sf, rng, ndir, att)) {
+        const double a3[3] = {double(att.x), double(att.y), double(att.z)};
+        for (int k = 0; k < 3; ++k) ab[k] = sat_u8(clamp_keep_nan(a3[k], 0.0, 1.0) * 255.0);
+      } else {
+        V3<R> e = emitted(S, m, sf.u, sf.v);
+        const double e3[3] = {double(e.x), double(e.y), double(e.z)};
+        double mx = fmax(fmax(e3[0], fmax(e3[1], e3[2])), 1.0);
+        for (int k = 0; k < 3; ++k) ab[k] = sat_u8(clamp_keep_nan(e3[k] / mx * 255.0, 0.0, 255.0));
+      }
+    }
+  }
+  if (depth) depth[p] = dep;
+  if (normal)
+    for (int k = 0; k < 3; ++k) normal[3 * size_t(p) + k] = nb[k];
+  if (albedo)
+    for (int k = 0; k < 3; ++k) albedo[3 * size_t(p) + k] = ab[k];
+}
+
 // FP32 feature sets (FEAT_MARCH implies FEAT_F64) and FP64 feature sets
 // (FEAT_F64 is meaningless in FP64: only transforms and marched kinds).
 #define LMN_FEAT_SWITCH(feat, CALL) \
@@ -169,6 +224,22 @@ void launch_trace_fp32(const SceneView<float>& S, uint32_t feat, const lmn_ray* 
 void launch_trace_fp64(const SceneView<double>& S, uint32_t feat, const lmn_ray* rays, lmn_hit* hits, size_t n,
                        hipStream_t s) {
 #define CALL(f) hipLaunchKernelGGL((k_trace<double, f>), dim3((n + 127) / 128), dim3(128), 0, s, S, rays, hits, n)
+  LMN_FEAT64_SWITCH(feat, CALL)
+#undef CALL
+}
+void launch_aov_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, float* depth, uint8_t* normal,
+                     uint8_t* albedo, hipStream_t s) {
+  uint32_t npix = F.crop_w * F.crop_h;
+#define CALL(f) \
+  hipLaunchKernelGGL((k_aov<float, f>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, depth, normal, albedo)
+  LMN_FEAT_SWITCH(feat, CALL)
+#undef CALL
+}
+void launch_aov_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, float* depth, uint8_t* normal,
+                     uint8_t* albedo, hipStream_t s) {
+  uint32_t npix = F.crop_w * F.crop_h;
+#define CALL(f) \
+  hipLaunchKernelGGL((k_aov<double, f>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, depth, normal, albedo)
   LMN_FEAT64_SWITCH(feat, CALL)
 #undef CALL
 }

@@ -74,6 +74,20 @@ def lib():
                                      D3, D3, D3, D3]
         L.oracle_reflectance.restype = C.c_double
         L.oracle_reflectance.argtypes = [C.c_double, C.c_double]
+        V = C.c_void_p
+        L.oracle_aov.restype = C.c_int
+        L.oracle_aov.argtypes = [C.POINTER(_abi.SceneDesc), C.POINTER(OracleParams), V, V, V]
+        L.oracle_post.restype = C.c_int
+        L.oracle_post.argtypes = [V, C.c_uint64, C.c_double, C.c_uint32, C.c_uint32, V, C.c_double, D3, V, V,
+                                  C.POINTER(C.c_double)]
+        L.oracle_log_average_luminance.restype = C.c_double
+        L.oracle_log_average_luminance.argtypes = [V, C.c_uint64]
+        L.oracle_tone.restype = C.c_double
+        L.oracle_tone.argtypes = [C.c_uint32, C.c_double]
+        L.oracle_srgb8.restype = C.c_uint32
+        L.oracle_srgb8.argtypes = [C.c_double]
+        L.oracle_rgbe.restype = None
+        L.oracle_rgbe.argtypes = [V, C.c_uint64, V]
         _lib = L
     return _lib
 
@@ -203,3 +217,57 @@ def adjoint(world, config, dl_dimage, texture=-1, emission_material=-1, threads=
                                 tex.ctypes.data_as(dp) if tex is not None else None, rough.ctypes.data_as(dp),
                                 C.byref(em)))
     return tex, rough[:d.n_materials], em.value
+
+
+def _vp(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def aov(world, config, rng_mode=RNG_COUNTER, threads=0, scene_key=0, exact=True, albedo=True):
+    """AOV passes on the CPU (render.rs:3508-3584): (depth, normal, albedo);
+    depth +inf where the pixel's ray escapes."""
+    p = _params(config, rng_mode=rng_mode, threads=threads, scene_key=scene_key, exact=exact)
+    fw, fh = max(config.width, 1), max(config.height, 1)
+    cw, ch = fw, fh
+    if config.crop is not None:
+        cw, ch = min(config.crop[2], fw), min(config.crop[3], fh)
+    d = np.empty((ch, cw), np.float32)
+    n = np.empty((ch, cw, 3), np.uint8)
+    a = np.empty((ch, cw, 3), np.uint8) if albedo else None
+    _check(lib().oracle_aov(world.desc, C.byref(p), _vp(d), _vp(n), _vp(a)))
+    return d, n, a
+
+
+def post(hdr, exposure=1.0, tone_map=0, auto_exposure=False, fog_depth=None, fog_density=0.0,
+         fog_color=(0.8, 0.85, 0.9)):
+    """Default post chain on the CPU: (rgba, exposed HDR, exposure used)."""
+    hdr = np.ascontiguousarray(hdr, dtype=np.float32)
+    shape = hdr.shape[:-1]
+    n = hdr.size // 3
+    rgba = np.empty(shape + (4,), np.uint8)
+    hx = np.empty(hdr.shape, np.float32)
+    fog = None if fog_depth is None else np.ascontiguousarray(fog_depth, dtype=np.float32)
+    ex = C.c_double()
+    _check(lib().oracle_post(_vp(hdr), n, exposure, tone_map, 1 if auto_exposure else 0, _vp(fog), fog_density,
+                             _d3(fog_color), _vp(rgba), _vp(hx), C.byref(ex)))
+    return rgba, hx, ex.value
+
+
+def log_average_luminance(hdr):
+    hdr = np.ascontiguousarray(hdr, dtype=np.float32)
+    return lib().oracle_log_average_luminance(_vp(hdr), hdr.size // 3)
+
+
+def tone(mode, x):
+    return lib().oracle_tone(mode, x)
+
+
+def srgb8(x):
+    return lib().oracle_srgb8(x)
+
+
+def rgbe(rgb):
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    out = np.empty(rgb.shape[:-1] + (4,), np.uint8)
+    lib().oracle_rgbe(_vp(rgb), rgb.size // 3, _vp(out))
+    return out

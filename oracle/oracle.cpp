@@ -1907,4 +1907,200 @@ int oracle_adjoint(const lmn_scene_desc* desc, const oracle_params* p, int32_t t
   return LMN_OK;
 }
 
+
+// ---- image outputs: AOV passes, post chain, Radiance pixels ----------------
+
+namespace {
+
+// Rust float semantics: f64::clamp keeps a NaN; `as u8` saturates, truncates
+// toward zero and maps NaN to 0.
+double clamp_nan(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+uint8_t as_u8(double x) {
+  if (!(x > 0.0)) return 0;
+  if (x >= 255.0) return 255;
+  return uint8_t(x);
+}
+// The GPU's per-pixel AOV stream (counter mode).
+uint64_t aov_key(uint64_t fkey, uint64_t pixel) {
+  return splitmix_out(fkey ^ splitmix_out(pixel ^ 0x3C6EF372FE94F82BULL) ^ 0xA4093822299F31D0ULL);
+}
+
+double aces_tonemap(double x) {  // render.rs:3608-3616
+  x = std::fmax(x, 0.0);
+  double a = 2.51, b = 0.03, c = 2.43, d = 0.59, e = 0.14;
+  return clamp_nan((x * (a * x + b)) / (x * (c * x + d) + e), 0.0, 1.0);
+}
+double reinhard_tonemap(double x) {  // render.rs:3619-3622
+  x = std::fmax(x, 0.0);
+  return x / (1.0 + x);
+}
+double uc2(double v) {  // render.rs:3629-3637
+  double a = 0.15, b = 0.50, c = 0.10, d = 0.20, e = 0.02, f = 0.30;
+  return ((v * (a * v + c * b) + d * e) / (v * (a * v + b) + d * f)) - e / f;
+}
+double filmic_tonemap(double x) {  // render.rs:3626-3641
+  x = std::fmax(x, 0.0);
+  double w = 11.2;
+  return clamp_nan(uc2(x) / uc2(w), 0.0, 1.0);
+}
+double tone_fn(uint32_t mode, double x) {  // render.rs:2626-2632
+  switch (mode) {
+    case 0: return aces_tonemap(x);
+    case 1: return reinhard_tonemap(x);
+    case 2: return filmic_tonemap(x);
+    default: return clamp_nan(x, 0.0, 1.0);
+  }
+}
+uint8_t gamma_correct(double x) {  // render.rs:3649-3658, gamma 0, no dither
+  x = clamp_nan(x, 0.0, 1.0);
+  double s = x <= 0.0031308 ? 12.92 * x : 1.055 * std::pow(x, 1.0 / 2.4) - 0.055;
+  return as_u8(clamp_nan(s + 0.0, 0.0, 0.999) * 256.0);
+}
+
+}  // namespace
+
+// compute_log_average_luminance (render.rs:525-541), one sequential sum.
+double oracle_log_average_luminance(const float* hdr, uint64_t n) {
+  double sum = 0.0;
+  for (uint64_t i = 0; i < n; ++i) {
+    double x = hdr[3 * i], y = hdr[3 * i + 1], z = hdr[3 * i + 2];
+    double lum = 0.2126 * x + 0.7152 * y + 0.0722 * z;
+    sum += std::log(1e-4 + std::fmax(lum, 0.0));
+  }
+  if (n == 0) return 0.0;
+  return std::exp(sum / double(n));
+}
+
+double oracle_tone(uint32_t mode, double x) { return tone_fn(mode, x); }
+uint32_t oracle_srgb8(double x) { return gamma_correct(x); }
+
+// Exposure (render.rs:2594-2608), hdr_data (:2610-2624), and the per-pixel
+// loop of :3099-3202 with vignette 1, depth fog, no white balance / hue /
+// saturation / grading / dither / brightness: tone curve, gamma_correct.
+int oracle_post(const float* hdr, uint64_t n, double exposure, uint32_t tone_map, uint32_t auto_exposure,
+                const float* fog_depth, double fog_density, const double fog_rgb[3], uint8_t* rgba,
+                float* hdr_exposed, double* exposure_used) {
+  if (!hdr || (fog_density > 0.0 && (!fog_depth || !fog_rgb))) {
+    g_err = "null argument";
+    return LMN_ERR_INVALID;
+  }
+  double ex = exposure;
+  if (auto_exposure) {
+    double log_avg = oracle_log_average_luminance(hdr, n);
+    double auto_exp = log_avg > 1e-6 ? 0.18 / log_avg : 1.0;
+    ex = auto_exp * exposure;
+  }
+  if (exposure_used) *exposure_used = ex;
+  for (uint64_t i = 0; i < n; ++i) {
+    double vignette_factor = 1.0;
+    double c[3];
+    for (int k = 0; k < 3; ++k) c[k] = double(hdr[3 * i + k]) * ex * vignette_factor;
+    if (hdr_exposed)
+      for (int k = 0; k < 3; ++k) hdr_exposed[3 * i + k] = float(double(hdr[3 * i + k]) * ex);
+    if (fog_density > 0.0) {
+      double depth = double(fog_depth[i]);
+      double fog_factor = 1.0 - std::exp(-fog_density * depth);
+      for (int k = 0; k < 3; ++k) c[k] = c[k] * (1.0 - fog_factor) + fog_rgb[k] * fog_factor;
+    }
+    if (rgba) {
+      for (int k = 0; k < 3; ++k) rgba[4 * i + k] = gamma_correct(tone_fn(tone_map, c[k]));
+      rgba[4 * i + 3] = 255;
+    }
+  }
+  return LMN_OK;
+}
+
+// write_radiance_hdr's pixel loop (main.rs:964-986): f32 arithmetic, log2f.
+void oracle_rgbe(const float* rgb, uint64_t n, uint8_t* out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    float r = std::fmax(rgb[i * 3], 0.0f), g = std::fmax(rgb[i * 3 + 1], 0.0f), b = std::fmax(rgb[i * 3 + 2], 0.0f);
+    float max_val = std::fmax(std::fmax(r, g), b);
+    uint8_t* o = out + 4 * i;
+    if (max_val < 1e-32f) {
+      o[0] = o[1] = o[2] = o[3] = 0;
+      continue;
+    }
+    float l = std::ceil(std::log2(max_val));  // f32 log2, then ceil
+    int32_t e = l >= 2147483647.0f ? 2147483647 : int32_t(l);  // `as i32` saturates
+    float p2 = std::ldexp(1.0f, e > 1024 ? 1024 : e);  // 2.0_f32.powi(e): exact, inf above 2^127
+    float scale = 256.0f / p2;
+    o[0] = as_u8(double(std::fmin(r * scale, 255.0f)));
+    o[1] = as_u8(double(std::fmin(g * scale, 255.0f)));
+    o[2] = as_u8(double(std::fmin(b * scale, 255.0f)));
+    o[3] = uint8_t(uint32_t(e) + 128u);  // `(exp + 128) as u8` wraps
+  }
+}
+
+// AOV passes (render.rs:3508-3584); the same ray gives the fog depth of
+// :2634-2665.  Rows in parallel.  depth is +inf where the ray escapes (the fog
+// buffer); the reference's depth_pass holds 0 there.
+int oracle_aov(const lmn_scene_desc* desc, const oracle_params* p, float* depth, uint8_t* normal, uint8_t* albedo) {
+  if (!desc || !p) {
+    g_err = "null argument";
+    return LMN_ERR_INVALID;
+  }
+  World w;
+  if (!build_world(*desc, p->exact_marched != 0, w)) return LMN_ERR_INVALID;
+  uint32_t fw = std::max<uint32_t>(p->width, 1), fh = std::max<uint32_t>(p->height, 1);
+  uint32_t cx = 0, cy = 0, cw = fw, ch = fh;
+  if (p->has_crop) {
+    cx = std::min(p->crop[0], fw);
+    cy = std::min(p->crop[1], fh);
+    cw = std::min(p->crop[2], fw);
+    ch = std::min(p->crop[3], fh);
+  }
+  uint64_t fkey = frame_key(p->seed, p->scene_key);
+  std::atomic<uint32_t> next_row{0};
+  unsigned nt = p->threads ? p->threads : std::max(1u, std::thread::hardware_concurrency());
+  auto worker = [&]() {
+    for (;;) {
+      uint32_t j = next_row.fetch_add(1);
+      if (j >= ch) break;
+      uint32_t gj = j + cy;
+      double y = double(fh - 1 - gj);
+      for (uint32_t i = 0; i < cw; ++i) {
+        uint32_t gi = i + cx;
+        double u = (double(gi) + 0.5) / double(fw - 1);
+        double v = (y + 0.5) / double(fh - 1);
+        uint64_t pix = uint64_t(gj) * fw + gi;
+        CounterRng crng(aov_key(fkey, pix));
+        XoshiroRng xrng(pix * p->seed);
+        Rng& rng = p->rng_mode == ORACLE_RNG_REFERENCE ? static_cast<Rng&>(xrng) : static_cast<Rng&>(crng);
+        Ray r = w.get_ray(u, v, rng);
+        size_t o = size_t(j) * cw + i;
+        float dep = INFINITY;
+        uint8_t nb[3] = {0, 0, 0}, ab[3] = {0, 0, 0};
+        HitRecord h;
+        if (w.hit(r, 0.001, INFINITY, h)) {
+          dep = float(h.t);
+          double n3[3] = {h.n.x, h.n.y, h.n.z};
+          for (int k = 0; k < 3; ++k) nb[k] = as_u8(clamp_nan((n3[k] * 0.5 + 0.5) * 255.0, 0.0, 255.0));
+          if (albedo) {
+            Ray sc;
+            Vec3 att;
+            if (h.mat->scatter(r, h, rng, sc, att)) {
+              double a3[3] = {att.x, att.y, att.z};
+              for (int k = 0; k < 3; ++k) ab[k] = as_u8(clamp_nan(a3[k], 0.0, 1.0) * 255.0);
+            } else {
+              Vec3 e = h.mat->emitted(h.u, h.v);
+              double max_e = std::fmax(std::fmax(e.x, std::fmax(e.y, e.z)), 1.0);
+              double e3[3] = {e.x, e.y, e.z};
+              for (int k = 0; k < 3; ++k) ab[k] = as_u8(clamp_nan(e3[k] / max_e * 255.0, 0.0, 255.0));
+            }
+          }
+        }
+        if (depth) depth[o] = dep;
+        for (int k = 0; k < 3; ++k) {
+          if (normal) normal[3 * o + k] = nb[k];
+          if (albedo) albedo[3 * o + k] = ab[k];
+        }
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned k = 0; k < nt; ++k) th.emplace_back(worker);
+  for (auto& t : th) t.join();
+  return LMN_OK;
+}
+
 }  // extern "C"

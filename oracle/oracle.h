@@ -93,6 +93,22 @@ int oracle_adjoint(const lmn_scene_desc* desc, const oracle_params* p, int32_t t
                    const double* dldi, double* g_tex, double* g_rough, double* g_em);
 double oracle_reflectance(double cosine, double ratio);
 
+/* Image outputs.  oracle_aov: the AOV passes of render.rs:3508-3584 (params:
+ * width, height, crop, seed, scene_key, rng_mode, exact_marched, threads);
+ * depth is +inf where the pixel's ray escapes (the fog buffer of
+ * :2634-2665).  oracle_post: exposure (automatic when asked, :2594-2608),
+ * hdr_data (:2610-2624), depth fog and the default per-pixel chain
+ * (:3099-3202, :3606-3658) on an FP32 framebuffer of n pixels.
+ * oracle_rgbe: the pixel loop of write_radiance_hdr (main.rs:964-986). */
+int oracle_aov(const lmn_scene_desc* desc, const oracle_params* p, float* depth, uint8_t* normal, uint8_t* albedo);
+int oracle_post(const float* hdr, uint64_t n, double exposure, uint32_t tone_map, uint32_t auto_exposure,
+                const float* fog_depth, double fog_density, const double fog_rgb[3], uint8_t* rgba,
+                float* hdr_exposed, double* exposure_used);
+double oracle_log_average_luminance(const float* hdr, uint64_t n);
+double oracle_tone(uint32_t mode, double x);
+uint32_t oracle_srgb8(double x);
+void oracle_rgbe(const float* rgb, uint64_t n, uint8_t* out);
+
 #ifdef __cplusplus
 }
 #endif

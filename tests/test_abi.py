@@ -38,3 +38,6 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(_abi.RenderParams) == 88
     assert C.sizeof(_abi.Ray) == 64
     assert C.sizeof(_abi.Hit) == 64
+    assert C.sizeof(_abi.RenderConfig) == 152
+    assert C.sizeof(_abi.PostParams) == 48
+    assert C.sizeof(_abi.Image) == 72

@@ -13,6 +13,7 @@ import pytest
 
 import luminara_amd as lm
 from luminara_amd import build, workloads
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -52,17 +53,6 @@ def read_rgbe(path):
     return e[..., :3] * scale
 
 
-def srgb8(x):
-    x = np.clip(x, 0.0, 1.0)
-    s = np.where(x <= 0.0031308, 12.92 * x, 1.055 * np.power(x, 1.0 / 2.4) - 0.055)
-    return (np.clip(s, 0.0, 0.999) * 256.0).astype(np.int64)
-
-
-def aces(x):
-    x = np.maximum(x, 0.0)
-    return np.clip((x * (2.51 * x + 0.03)) / (x * (2.43 * x + 0.59) + 0.14), 0.0, 1.0)
-
-
 def test_cli_outputs_match_the_library_render(gpu, tmp_path):
     scene = tmp_path / "cornell.toml"
     scene.write_text(workloads.cornell_box(64, 64, 16, 8))
@@ -72,16 +62,20 @@ def test_cli_outputs_match_the_library_render(gpu, tmp_path):
     subprocess.run(args + ["-o", str(ppm)], check=True, timeout=120)
     cfg, cam, world = lm.load_scene(scene.read_text())
     cfg.width, cfg.height, cfg.samples_per_pixel, cfg.max_depth, cfg.seed = 48, 40, 9, 6, 7
-    rows = lm.render(cfg, cam, world).rows.astype(np.float64)
+    rows = lm.render(cfg, cam, world).rows
     img = read_png(png)
     assert img.shape == (40, 48, 4) and np.all(img[..., 3] == 255)
-    expect = srgb8(aces(rows * cfg.exposure))
-    # the C++ and numpy powers may round the last bit differently
-    assert np.abs(img[..., :3].astype(np.int64) - expect).max() <= 1
-    assert (img[..., :3] == expect).mean() > 0.999
+    expect, exposed, _ = oracle.post(rows, exposure=cfg.exposure, tone_map=cfg.tone_map)
+    # the GPU's and the host's pow may round the last bit apart
+    assert np.abs(img[..., :3].astype(np.int64) - expect[..., :3]).max() <= 1
+    assert (img[..., :3] == expect[..., :3]).mean() > 0.999
     assert np.array_equal(read_ppm(ppm), img[..., :3].astype(np.int64))
+    raw = open(hdr, "rb").read()
+    assert raw.startswith(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 40 +X 48\n")
+    assert np.array_equal(np.frombuffer(raw.split(b"+X 48\n", 1)[1], np.uint8).reshape(40, 48, 4),
+                          oracle.rgbe(exposed))
     h = read_rgbe(hdr)
-    ref = rows * cfg.exposure
+    ref = rows.astype(np.float64) * cfg.exposure
     m = ref.max(axis=2, keepdims=True)
     assert np.all(np.abs(h - ref) <= m / 128.0 + 1e-30)
     st = json.loads(js.read_text())
@@ -97,3 +91,27 @@ def test_cli_preview_and_benchmark(gpu, tmp_path):
     assert read_png(png).shape == (16, 16, 4)
     r = subprocess.run([build.CLI, str(scene), "--benchmark"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "Benchmark result:" in r.stderr and "Mrays/s" in r.stderr
+
+
+def test_cli_passes_fog_and_tone_flags(gpu, tmp_path):
+    scene = tmp_path / "cornell.toml"
+    scene.write_text(workloads.cornell_box(64, 64, 16, 8))
+    dp, npth, ap, png = (tmp_path / n for n in ("d.png", "n.png", "a.png", "o.png"))
+    args = [build.CLI, str(scene), "-w", "40", "-H", "40", "-s", "4", "-d", "4", "--seed", "3", "-q",
+            "--tone-map", "reinhard", "--depth-fog", "0.001", "--fog-color", "0.2,0.3,0.4", "-o", str(png),
+            "--save-depth", str(dp), "--save-normals", str(npth), "--save-albedo", str(ap)]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Saved depth pass to" in r.stderr and "Saved albedo pass to" in r.stderr
+    cfg, cam, world = lm.load_scene(scene.read_text())
+    cfg.width, cfg.height, cfg.samples_per_pixel, cfg.max_depth, cfg.seed = 40, 40, 4, 4, 3
+    cfg.tone_map, cfg.depth_fog, cfg.depth_fog_color = 1, 0.001, (0.2, 0.3, 0.4)
+    img = lm.render_image(cfg, world, save_depth=True, save_normals=True, save_albedo=True)
+    assert np.array_equal(read_png(png), img.pixels)
+    assert np.array_equal(read_png(npth)[..., :3], img.normal_pass)
+    assert np.array_equal(read_png(ap)[..., :3], img.albedo_pass)
+    # depth image (src/main.rs:856-876): depth * 255 / max depth, truncated
+    d = img.depth_pass
+    norm = np.float32(255.0) / d.max() if d.max() > 0 else np.float32(1.0)
+    want = np.clip(d * norm, 0, 255).astype(np.uint8)
+    assert np.array_equal(read_png(dp)[..., 0], want)
