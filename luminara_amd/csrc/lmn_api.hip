@@ -1243,7 +1243,10 @@ int post_enqueue(const float* d_hdr, uint64_t n, const lmn_post_params* post, co
     launch_log_luminance(d_hdr, n, post->exposure, scratch.as<double>(), st);
     A.exposure_dev = scratch.as<double>();
   }
-  if (d_rgba || d_hdr_exposed) launch_post(A, st);
+  if (d_rgba || d_hdr_exposed) {
+    HIP_TRY(ensure_post_tables());
+    launch_post(A, st);
+  }
   HIP_TRY(hipGetLastError());
   return LMN_OK;
 }

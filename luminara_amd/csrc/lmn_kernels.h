@@ -162,12 +162,14 @@ struct PostArgs {
   double fog_density;
   double fog_rgb[3];
   uint32_t tone;              // LMN_TONE_*
-  uint32_t pad;
+  uint32_t vec;               // set by launch_post: every buffer 16-B aligned
   uint64_t n;
   uint8_t* rgba;              // n * 4
   float* hdr_exposed;         // n * 3, or null
 };
 constexpr uint32_t kLogLumBlocks = 1024;
+// Uploads the sRGB quantisation thresholds to the current device (once).
+hipError_t ensure_post_tables();
 void launch_post(const PostArgs& A, hipStream_t s);
 // scratch: 2 + kLogLumBlocks doubles; scratch[0] = log-average luminance,
 // scratch[1] = exposure * key factor.
