@@ -176,6 +176,9 @@ typedef struct lmn_render_config {
   uint32_t tone_map;      /* LMN_TONE_*: 0 ACES, 1 Reinhard, 2 Filmic, 3 none */
   uint32_t auto_exposure; /* [render] auto_exposure (src/scene.rs:870-872)      */
   lmn_background background;
+  uint32_t adaptive;      /* [render] adaptive (src/scene.rs:914-916)            */
+  uint32_t pad;
+  double adaptive_threshold; /* [render] adaptive_threshold, default 0.03        */
 } lmn_render_config;
 
 /* ---- scene descriptor ---------------------------------------------------- */
@@ -220,12 +223,23 @@ typedef struct lmn_render_params {
   uint32_t flags;          /* LMN_FLAG_*                                  */
   uint32_t grid_blocks;    /* workgroups of the wavefront kernels, 0 = library pick */
   void* stream;            /* hipStream_t, NULL = library stream          */
+  double adaptive_threshold; /* LMN_FLAG_ADAPTIVE: stop a pixel once the standard error
+                                of its luminance is below this (RenderConfig default 0.03) */
+  double time_limit;       /* seconds, 0 = none (RenderConfig.time_limit): rows begun after
+                              it take one sample through the pixel centre; one frame only,
+                              wavefront path only                                          */
 } lmn_render_params;
 #define LMN_FLAG_STAGE_TIMING 1u /* time each wavefront stage with HIP events */
 #define LMN_FLAG_NO_GRAPH 2u     /* launch wavefront iterations directly, not as a captured hipGraph */
+#define LMN_FLAG_ADAPTIVE 4u     /* adaptive sampling (src/render.rs:1626-1677): each pixel takes
+                                    stratified samples in order until n >= max(spp/4, 16), n > 1
+                                    and sqrt(var(luminance) / n) < adaptive_threshold; its value is
+                                    the unweighted mean of the samples taken (no pixel filter) */
 
 typedef struct lmn_stats {
-  uint64_t paths;          /* W*H*ceil(sqrt(spp))^2 (src/render.rs:1589)  */
+  uint64_t paths;          /* samples averaged into the image, the reference's total_rays:
+                              W*H*ceil(sqrt(spp))^2 (src/render.rs:1589), fewer with
+                              adaptive sampling or a time limit                       */
   uint64_t camera_rays;
   uint64_t extension_rays; /* closest-hit queries after the camera ray    */
   uint64_t shadow_rays;    /* NEE visibility queries                      */
@@ -237,6 +251,9 @@ typedef struct lmn_stats {
   double shadow_ms;
   double regen_ms;
   double march_ms;         /* the marched kinds' extend and shadow kernels */
+  uint64_t samples_traced; /* camera samples traced (adaptive rounds trace a few past the stop) */
+  uint32_t rounds;         /* wavefront runs: adaptive rounds, time-limit bands              */
+  uint32_t expired_rows;   /* rows rendered with one centre sample after the time limit      */
 } lmn_stats;
 
 /* ---- ray-batch parity harness -------------------------------------------- */

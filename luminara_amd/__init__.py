@@ -39,6 +39,9 @@ class RenderConfig:
     exposure: float = 1.0
     tone_map: int = 0
     auto_exposure: bool = False
+    adaptive: bool = False
+    adaptive_threshold: float = 0.03
+    time_limit: float = 0.0
     depth_fog: float = 0.0
     depth_fog_color: Tuple[float, float, float] = (0.8, 0.85, 0.9)
 
@@ -150,7 +153,8 @@ def load_scene(toml_text: str, base_dir: Optional[str] = None):
     c = out.contents.config
     cfg = RenderConfig(width=c.width, height=c.height, samples_per_pixel=c.samples, max_depth=c.max_depth,
                        seed=c.seed, pixel_filter=c.filter, exposure=c.exposure, tone_map=c.tone_map,
-                       auto_exposure=bool(c.auto_exposure))
+                       auto_exposure=bool(c.auto_exposure), adaptive=bool(c.adaptive),
+                       adaptive_threshold=c.adaptive_threshold)
     return cfg, out.contents.camera, world
 
 
@@ -172,8 +176,10 @@ def make_params(config: RenderConfig, path="wavefront", scene_key=0, intersect_m
     p.stack_lds = stack_lds
     p.block_size = block_size
     p.grid_blocks = grid_blocks
-    p.flags = flags
+    p.flags = flags | (_abi.FLAG_ADAPTIVE if config.adaptive else 0)
     p.stream = stream
+    p.adaptive_threshold = config.adaptive_threshold
+    p.time_limit = config.time_limit
     return p
 
 

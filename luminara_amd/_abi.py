@@ -75,7 +75,8 @@ class RenderConfig(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("samples", C.c_uint32),
                 ("max_depth", C.c_uint32), ("seed", C.c_uint64), ("filter", C.c_uint32),
                 ("has_crop", C.c_uint32), ("crop", C.c_uint32 * 4), ("exposure", C.c_double),
-                ("tone_map", C.c_uint32), ("auto_exposure", C.c_uint32), ("background", Background)]
+                ("tone_map", C.c_uint32), ("auto_exposure", C.c_uint32), ("background", Background),
+                ("adaptive", C.c_uint32), ("pad", C.c_uint32), ("adaptive_threshold", C.c_double)]
 
 
 class SceneDesc(C.Structure):
@@ -92,7 +93,8 @@ class RenderParams(C.Structure):
                 ("has_crop", C.c_uint32), ("crop", C.c_uint32 * 4), ("path", C.c_uint32),
                 ("intersect_mode", C.c_uint32), ("scene_key", C.c_uint32), ("pool_paths", C.c_uint32),
                 ("block_size", C.c_uint32), ("stack_lds", C.c_uint32), ("flags", C.c_uint32),
-                ("grid_blocks", C.c_uint32), ("stream", C.c_void_p)]
+                ("grid_blocks", C.c_uint32), ("stream", C.c_void_p), ("adaptive_threshold", C.c_double),
+                ("time_limit", C.c_double)]
 
 FLAG_STAGE_TIMING = 1
 
@@ -101,7 +103,8 @@ class Stats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("camera_rays", C.c_uint64), ("extension_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("render_ms", C.c_double), ("iterations", C.c_uint32),
                 ("extend_launches", C.c_uint32), ("extend_ms", C.c_double), ("shade_ms", C.c_double),
-                ("shadow_ms", C.c_double), ("regen_ms", C.c_double), ("march_ms", C.c_double)]
+                ("shadow_ms", C.c_double), ("regen_ms", C.c_double), ("march_ms", C.c_double),
+                ("samples_traced", C.c_uint64), ("rounds", C.c_uint32), ("expired_rows", C.c_uint32)]
 
 
 class Ray(C.Structure):
@@ -119,6 +122,7 @@ class TraceCounts(C.Structure):
 
 
 FLAG_NO_GRAPH = 2
+FLAG_ADAPTIVE = 4
 
 
 class Tuning(C.Structure):

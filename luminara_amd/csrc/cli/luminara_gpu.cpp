@@ -34,9 +34,9 @@ struct Cli {
   std::string scene, output = "output.png", save_hdr, save_json, save_depth, save_normals, save_albedo, tone_map;
   long width = -1, height = -1, samples = -1, depth = -1, threads = -1;
   long long seed = -1;
-  double exposure = -1.0, depth_fog = -1.0;
+  double exposure = -1.0, depth_fog = -1.0, time_limit = -1.0, adaptive_threshold = -1.0;
   double fog_color[3] = {-1.0, -1.0, -1.0};
-  bool quiet = false, preview = false, benchmark = false, help = false, auto_exposure = false;
+  bool quiet = false, preview = false, benchmark = false, help = false, auto_exposure = false, adaptive = false;
 };
 
 [[noreturn]] void die(const std::string& msg) {
@@ -104,6 +104,12 @@ Cli parse(int argc, char** argv) {
       if (const char* v = next()) parse_num(v, c.threads);
     } else if (a == "-e" || a == "--exposure") {
       if (const char* v = next()) parse_num(v, c.exposure);
+    } else if (a == "--adaptive") {
+      c.adaptive = true;
+    } else if (a == "--adaptive-threshold") {
+      if (const char* v = next()) parse_num(v, c.adaptive_threshold);
+    } else if (a == "--time-limit" || a == "--max-time") {
+      if (const char* v = next()) parse_num(v, c.time_limit);
     } else if (a == "--auto-exposure") {
       c.auto_exposure = true;
     } else if (a == "--tone-map") {
@@ -242,6 +248,9 @@ void usage() {
                "      --seed N            random seed\n"
                "  -e, --exposure X        exposure multiplier\n"
                "      --auto-exposure     exposure from the log-average luminance\n"
+               "      --adaptive          adaptive sampling (stop a pixel once converged)\n"
+               "      --adaptive-threshold X  standard-error threshold [0.03]\n"
+               "      --time-limit SECS   rows begun after SECS take one sample\n"
                "      --tone-map NAME     aces, reinhard, filmic (uncharted2) or none\n"
                "      --depth-fog N       atmospheric depth fog density (e.g. 0.1)\n"
                "      --fog-color R,G,B   fog colour (0-1 each, default: 0.8,0.85,0.9)\n"
@@ -292,6 +301,8 @@ int main(int argc, char** argv) {
     rc.max_depth = std::min<uint32_t>(rc.max_depth, 10);
   }
   if (cli.auto_exposure) rc.auto_exposure = 1;
+  if (cli.adaptive) rc.adaptive = 1;  // src/main.rs:526-534
+  if (cli.adaptive_threshold >= 0.0) rc.adaptive_threshold = cli.adaptive_threshold;
   lmn_post_params post;
   std::memset(&post, 0, sizeof post);
   post.exposure = rc.exposure;
@@ -343,6 +354,9 @@ int main(int argc, char** argv) {
   std::memcpy(p.crop, rc.crop, sizeof p.crop);
   p.path = LMN_PATH_WAVEFRONT_FP32;
   p.intersect_mode = LMN_ISECT_EXACT;
+  if (rc.adaptive) p.flags |= LMN_FLAG_ADAPTIVE;
+  p.adaptive_threshold = rc.adaptive_threshold;
+  p.time_limit = cli.time_limit > 0.0 ? cli.time_limit : 0.0;
   uint32_t out_w = rc.width, out_h = rc.height;
   if (rc.has_crop) {
     out_w = std::min(rc.crop[2], rc.width > rc.crop[0] ? rc.width - rc.crop[0] : 0u);
@@ -387,6 +401,19 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "  %llu primary rays, %u spp (stratified %ux%u)\n", (unsigned long long)st.paths, sq * sq,
                  sq, sq);
     return 0;
+  }
+  if (!cli.quiet) {  // src/render.rs:1762-1773
+    const uint64_t npx = uint64_t(out_w) * out_h;
+    const uint32_t sq = uint32_t(std::ceil(std::sqrt(double(rc.samples))));
+    if (rc.adaptive) {
+      const double max_rays = double(npx) * sq * sq;
+      std::fprintf(stderr, "Adaptive: %llu rays (%.1f avg spp, max %u, %.1f%% saved)\n",
+                   (unsigned long long)st.paths, double(st.paths) / double(npx), sq * sq,
+                   100.0 * (1.0 - double(st.paths) / max_rays));
+    } else {
+      std::fprintf(stderr, "Primary rays: %llu (%u spp, %ux%u stratified)\n", (unsigned long long)st.paths, sq * sq,
+                   sq, sq);
+    }
   }
   if (post.auto_exposure && !cli.quiet)  // src/render.rs:2595-2605
     std::fprintf(stderr, "Auto-exposure: %.3f (scene avg luminance: %.4f)\n", img.exposure / post.exposure,

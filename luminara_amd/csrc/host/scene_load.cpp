@@ -193,6 +193,9 @@ class Loader {
     c.exposure = 1.0;
     c.tone_map = 0;
     c.auto_exposure = 0;
+    c.adaptive = 0;
+    c.pad = 0;
+    c.adaptive_threshold = 0.03;
     c.background = lmn_background{};
     c.background.kind = LMN_BG_SKY;
     const TomlValue* r = root.get("render");
@@ -232,12 +235,11 @@ class Loader {
       else
         c.tone_map = 0;
     }
-    for (const char* k : {"adaptive", "time_limit"}) {
-      const TomlValue* v = r->get(k);
-      if (!v) continue;
-      bool on = (v->kind == TomlValue::BOOLEAN && v->boolean) || (v->is_number() && v->as_double() > 0.0);
-      if (on) fail(LMN_ERR_UNSUPPORTED, std::string("render.") + k + " (adaptive / time-limited sampling) is not on the GPU path");
+    if (const TomlValue* ad = r->get("adaptive")) {  // src/scene.rs:914-916
+      if (ad->kind != TomlValue::BOOLEAN) fail(LMN_ERR_PARSE, "render.adaptive: expected a boolean");
+      c.adaptive = ad->boolean ? 1u : 0u;
     }
+    opt_num(*r, "adaptive_threshold", ctx, c.adaptive_threshold);  // :917-919
     if (const TomlValue* bg = r->get("background")) background(*bg, c.background);
   }
 

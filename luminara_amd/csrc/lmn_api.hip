@@ -331,6 +331,10 @@ int frame_setup(const lmn_render_params* p, FrameSetup& fs) {
   F.max_depth = p->max_depth;
   F.filter = p->filter;
   F.fkey = frame_key(p->seed, p->scene_key);
+  F.mode = FRAME_STRATIFIED;
+  F.pix_list = nullptr;
+  F.n_list = 0;
+  F.pad = 0;
   fs.npix = cw * ch;
   F.n_items = uint64_t(fs.npix) * F.spp_a;
   F.item_base = 0;
@@ -474,40 +478,50 @@ int resolve_wave_config(lmn_scene* const* scenes, const lmn_render_params* const
 // one list (scene by scene, sample pass by sample pass), so one set of
 // kernel launches per iteration serves the whole batch.  Items are grouped in
 // chunks of whole sample passes (at most kMaxChunkItems samples buffered).
-int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* params, const FrameSetup* fs,
-                     float* const* d_out, size_t n, hipStream_t st, lmn_stats* stats, TraceCounters* d_tc) {
+constexpr uint64_t kMaxChunkItems = 1ull << 27;  // 2 GiB of float4 samples
+
+struct WaveTotals {
+  uint64_t ext = 0, sh = 0, cam = 0, iters = 0;
+  double t_stage[6] = {0, 0, 0, 0, 0, 0};
+  uint32_t n_launch = 0;
+};
+
+// Appends the parts of sample passes [s0, s1) of frame F over n_list listed
+// pixels (list null: the whole crop), each at most kMaxChunkItems items.
+void add_parts(std::vector<FrameDev>& parts, FrameDev F, const uint32_t* list, uint32_t n_pix, uint32_t s0,
+               uint32_t s1) {
+  F.pix_list = list;
+  F.n_list = list ? n_pix : 0;
+  uint32_t step = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(s1 - s0, kMaxChunkItems / n_pix)));
+  for (uint32_t a = s0; a < s1; a += step) {
+    uint32_t b = std::min(s1, a + step);
+    FrameDev P = F;
+    P.item_base = uint64_t(a) * n_pix;
+    P.n_items = uint64_t(b - a) * n_pix;
+    parts.push_back(P);
+  }
+}
+
+// Runs the wavefront over `parts`; after each chunk drains, resolve(part,
+// samples of the part) folds the part's samples (sample-major over its pixels).
+template <class Resolve>
+int wavefront_run(lmn_scene* const* scenes, const lmn_render_params* const* params, const FrameSetup* fs, size_t n,
+                  const std::vector<FrameDev>& parts, hipStream_t st, WaveTotals& tot, TraceCounters* d_tc,
+                  Resolve&& resolve) {
+  if (parts.empty()) return LMN_OK;
   Workspace& W = workspace();
   const lmn_render_params* p0 = params[0];
   uint64_t total = 0;
-  uint32_t max_npix = 0, feat = 0;
-  for (size_t i = 0; i < n; ++i) {
-    if (fs[i].black) continue;
-    total += fs[i].F.n_items;
-    max_npix = std::max(max_npix, fs[i].npix);
-    feat |= scenes[i]->host.featF;
+  uint32_t feat = 0;
+  for (const FrameDev& P : parts) {
+    total += P.n_items;
+    feat |= scenes[P.scene]->host.featF;
   }
   if (feat & FEAT_MARCH) feat |= FEAT_F64;
   WaveConfig wc;
   int rc = resolve_wave_config(scenes, params, fs, n, total, wc);
   if (rc != LMN_OK) return rc;
   const uint32_t pool = wc.pool, ss = wc.stack;
-  const uint64_t kMaxChunkItems = 1ull << 27;  // 2 GiB of float4 samples
-  // parts: whole sample passes of one scene
-  std::vector<FrameDev> parts;
-  for (size_t i = 0; i < n; ++i) {
-    if (fs[i].black) continue;
-    FrameDev F = fs[i].F;
-    F.scene = uint32_t(i);
-    uint32_t npix = fs[i].npix;
-    uint32_t spp_chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(F.spp_a, kMaxChunkItems / npix)));
-    for (uint32_t s0 = 0; s0 < F.spp_a; s0 += spp_chunk) {
-      uint32_t s1 = std::min(F.spp_a, s0 + spp_chunk);
-      FrameDev P = F;
-      P.item_base = uint64_t(s0) * npix;
-      P.n_items = uint64_t(s1 - s0) * npix;
-      parts.push_back(P);
-    }
-  }
   // chunks: consecutive parts up to kMaxChunkItems items
   std::vector<std::pair<size_t, size_t>> chunks;
   uint64_t chunk_max = 0;
@@ -544,10 +558,6 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
     HIP_TRY(W.mq_ovf.ensure(size_t(pool) * 4));
     if (march_kinds & 2u) HIP_TRY(W.mq_brk.ensure(size_t(pool / 2 + 1) * sizeof(MarchBracket)));
   }
-  // per-scene accumulators, scene views, parts
-  std::vector<size_t> acc_off(n + 1, 0);
-  for (size_t i = 0; i < n; ++i) acc_off[i + 1] = acc_off[i] + (fs[i].black ? 0 : fs[i].npix);
-  HIP_TRY(W.acc.ensure(std::max<size_t>(acc_off[n], 1) * 32));
   std::vector<SceneView<float>> views(n);
   for (size_t i = 0; i < n; ++i) {
     views[i] = scenes[i]->viewF;
@@ -583,16 +593,11 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
                   march_kinds,
                   0u};
   L.tc = d_tc;
-  for (size_t i = 0; i < n; ++i)
-    if (!fs[i].black) launch_zero_acc(W.acc.as<double4>() + acc_off[i], fs[i].npix, st);
-  uint64_t sum_ext = 0, sum_sh = 0, sum_cam = 0, iters = 0;
   const int block = Workspace::kBlock;
   const bool timing = (p0->flags & LMN_FLAG_STAGE_TIMING) != 0;
   // Stage timing records events between the stages; events recorded by graph
   // nodes cannot be timed, so timed renders launch directly.
   bool use_graph = !(p0->flags & LMN_FLAG_NO_GRAPH) && !timing && !W.graph_failed[0];
-  double t_stage[6] = {0, 0, 0, 0, 0, 0};
-  uint32_t n_launch = 0;
   // parts of a chunk: uploaded in one copy, chunk after chunk
   std::vector<uint64_t> begins;
   for (auto [pb, pe] : chunks) {
@@ -648,16 +653,16 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
           for (int g = 0; g < 6; ++g) {
             float ms = 0.f;
             HIP_TRY(hipEventElapsedTime(&ms, W.stage_ev[k][g], W.stage_ev[k][g + 1]));
-            t_stage[g] += ms;
+            tot.t_stage[g] += ms;
           }
-        n_launch += block;
+        tot.n_launch += block;
       }
       const WaveCounters& c = *W.host_ctr;
       if (c.active == 0 && c.next_item == c.n_items) {
-        sum_ext += c.stat_extend;
-        sum_sh += c.stat_shadow;
-        sum_cam += c.stat_camera;
-        iters += c.iterations;
+        tot.ext += c.stat_extend;
+        tot.sh += c.stat_shadow;
+        tot.cam += c.stat_camera;
+        tot.iters += c.iterations;
         break;
       }
       if (std::chrono::steady_clock::now() - t_start > std::chrono::seconds(600)) {
@@ -666,27 +671,216 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
       }
     }
     if (gexec) (void)hipGraphExecDestroy(gexec);
-    for (size_t q = pb; q < pe; ++q) {
-      const FrameDev& F = parts[q];
-      uint32_t npix = fs[F.scene].npix;
-      launch_resolve(L.samples + begins[q - pb], W.acc.as<double4>() + acc_off[F.scene], npix,
-                     uint32_t(F.n_items / npix), st);
-    }
+    for (size_t q = pb; q < pe; ++q) resolve(parts[q], L.samples + begins[q - pb]);
   }
-  for (size_t i = 0; i < n; ++i)
-    if (!fs[i].black) launch_finalize(W.acc.as<double4>() + acc_off[i], d_out[i], fs[i].npix, st);
+  HIP_TRY(hipGetLastError());
+  return LMN_OK;
+}
+
+// Pixel lists and per-pixel state of the adaptive and time-limited modes.
+struct FrameModeBufs {
+  DevBuf ident;  // 0 .. npix-1 (row bands are slices of it)
+  DevBuf list[2], count, state, taken;
+};
+
+// Adaptive rounds over the listed pixels of every scene i with lists[i]
+// non-empty: all of a round's pixels have taken the same number of samples,
+// so a round is one sample range per scene; the pixels still sampling after
+// it are compacted into the other list.
+int adaptive_rounds(lmn_scene* const* scenes, const lmn_render_params* const* params, const FrameSetup* fs,
+                    size_t n, std::vector<FrameModeBufs>& mb, std::vector<uint32_t>& n_list, hipStream_t st,
+                    WaveTotals& tot, TraceCounters* d_tc, uint32_t& rounds) {
+  std::vector<uint32_t> taken(n, 0), cur(n, 0);
+  for (;;) {
+    std::vector<FrameDev> parts;
+    std::vector<uint32_t> s_end(n, 0);
+    for (size_t i = 0; i < n; ++i) {
+      if (fs[i].black || n_list[i] == 0 || taken[i] >= fs[i].F.spp_a) continue;
+      const FrameDev& F = fs[i].F;
+      uint32_t first = std::min(F.spp_a, adaptive_min_samples(F.spp_a));
+      uint32_t s1 = taken[i] == 0 ? first : std::min(F.spp_a, taken[i] + std::max<uint32_t>(F.sqrt_spp, 4));
+      FrameDev G = F;
+      G.scene = uint32_t(i);
+      add_parts(parts, G, mb[i].list[cur[i]].as<uint32_t>(), n_list[i], taken[i], s1);
+      s_end[i] = s1;
+    }
+    if (parts.empty()) break;
+    ++rounds;
+    int rc = wavefront_run(scenes, params, fs, n, parts, st, tot, d_tc, [&](const FrameDev& P, const float4* smp) {
+      const lmn_render_params* p = params[P.scene];
+      launch_resolve_adaptive(smp, mb[P.scene].state.as<AdaptState>(), P.pix_list, P.n_list,
+                              uint32_t(P.n_items / P.n_list), adaptive_min_samples(P.spp_a), p->adaptive_threshold,
+                              st);
+    });
+    if (rc != LMN_OK) return rc;
+    for (size_t i = 0; i < n; ++i) {
+      if (s_end[i] == 0) continue;
+      taken[i] = s_end[i];
+      HIP_TRY(hipMemsetAsync(mb[i].count.p, 0, 4, st));
+      launch_adapt_compact(mb[i].state.as<AdaptState>(), mb[i].list[cur[i]].as<uint32_t>(), n_list[i],
+                           fs[i].F.spp_a, mb[i].list[cur[i] ^ 1].as<uint32_t>(), mb[i].count.as<uint32_t>(), st);
+      HIP_TRY(hipMemcpyAsync(&n_list[i], mb[i].count.p, 4, hipMemcpyDeviceToHost, st));
+      cur[i] ^= 1;
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    // the compacted lists now sit in list[cur]; keep them in list[0]
+    for (size_t i = 0; i < n; ++i)
+      if (cur[i] && n_list[i]) {
+        HIP_TRY(hipMemcpyAsync(mb[i].list[0].p, mb[i].list[1].p, size_t(n_list[i]) * 4, hipMemcpyDeviceToDevice, st));
+        cur[i] = 0;
+      } else {
+        cur[i] = 0;
+      }
+  }
+  return LMN_OK;
+}
+
+int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* params, const FrameSetup* fs,
+                     float* const* d_out, size_t n, hipStream_t st, lmn_stats* stats, TraceCounters* d_tc,
+                     std::chrono::steady_clock::time_point t0) {
+  Workspace& W = workspace();
+  WaveTotals tot;
+  bool adaptive = false, timed = false;
+  for (size_t i = 0; i < n; ++i) {
+    adaptive = adaptive || (params[i]->flags & LMN_FLAG_ADAPTIVE);
+    timed = timed || params[i]->time_limit > 0.0;
+  }
+  if (timed && n > 1) return set_err(LMN_ERR_UNSUPPORTED, "render: a time limit applies to one frame, not a batch");
+  std::vector<size_t> acc_off(n + 1, 0);
+  for (size_t i = 0; i < n; ++i) acc_off[i + 1] = acc_off[i] + (fs[i].black ? 0 : fs[i].npix);
+  uint64_t taken_total = 0;
+  uint32_t rounds = 0, expired_rows = 0;
+  if (!adaptive && !timed) {
+    std::vector<FrameDev> parts;
+    for (size_t i = 0; i < n; ++i) {
+      if (fs[i].black) continue;
+      FrameDev F = fs[i].F;
+      F.scene = uint32_t(i);
+      add_parts(parts, F, nullptr, fs[i].npix, 0, F.spp_a);
+    }
+    HIP_TRY(W.acc.ensure(std::max<size_t>(acc_off[n], 1) * 32));
+    for (size_t i = 0; i < n; ++i)
+      if (!fs[i].black) launch_zero_acc(W.acc.as<double4>() + acc_off[i], fs[i].npix, st);
+    int rc = wavefront_run(scenes, params, fs, n, parts, st, tot, d_tc, [&](const FrameDev& P, const float4* smp) {
+      launch_resolve(smp, W.acc.as<double4>() + acc_off[P.scene], fs[P.scene].npix,
+                     uint32_t(P.n_items / fs[P.scene].npix), st);
+    });
+    if (rc != LMN_OK) return rc;
+    for (size_t i = 0; i < n; ++i)
+      if (!fs[i].black) {
+        launch_finalize(W.acc.as<double4>() + acc_off[i], d_out[i], fs[i].npix, st);
+        taken_total += uint64_t(fs[i].npix) * fs[i].F.spp_a;
+      }
+    rounds = 1;
+  } else {
+    // adaptive and / or time-limited: pixel lists per scene
+    std::vector<FrameModeBufs> mb(n);
+    std::vector<uint32_t> n_list(n, 0);
+    HIP_TRY(W.acc.ensure(std::max<size_t>(acc_off[n], 1) * 32));
+    for (size_t i = 0; i < n; ++i) {
+      if (fs[i].black) continue;
+      const uint32_t npix = fs[i].npix;
+      HIP_TRY(mb[i].ident.ensure(size_t(npix) * 4));
+      HIP_TRY(mb[i].list[0].ensure(size_t(npix) * 4));
+      HIP_TRY(mb[i].list[1].ensure(size_t(npix) * 4));
+      HIP_TRY(mb[i].count.ensure(8));
+      HIP_TRY(mb[i].state.ensure(size_t(npix) * sizeof(AdaptState)));
+      HIP_TRY(mb[i].taken.ensure(8));
+      launch_adapt_init(mb[i].state.as<AdaptState>(), mb[i].ident.as<uint32_t>(), npix, st);
+      launch_zero_acc(W.acc.as<double4>() + acc_off[i], npix, st);
+    }
+    if (!timed) {
+      for (size_t i = 0; i < n; ++i)
+        if (!fs[i].black) {
+          HIP_TRY(hipMemcpyAsync(mb[i].list[0].p, mb[i].ident.p, size_t(fs[i].npix) * 4, hipMemcpyDeviceToDevice, st));
+          n_list[i] = fs[i].npix;
+        }
+      int rc = adaptive_rounds(scenes, params, fs, n, mb, n_list, st, tot, d_tc, rounds);
+      if (rc != LMN_OK) return rc;
+    } else if (!fs[0].black) {
+      // Time limit (src/render.rs:1600-1625): rows in bands; once the limit
+      // has passed at the start of a band, that band and every later one take
+      // one sample through each pixel centre.
+      const FrameSetup& f0 = fs[0];
+      const uint32_t cw = f0.F.crop_w, ch = f0.F.crop_h;
+      const uint32_t n_bands = std::min<uint32_t>(ch, 16);
+      const double limit = params[0]->time_limit;
+      for (uint32_t b = 0; b < n_bands; ++b) {
+        const uint32_t r0 = uint32_t(uint64_t(ch) * b / n_bands), r1 = uint32_t(uint64_t(ch) * (b + 1) / n_bands);
+        const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        const uint32_t* band = mb[0].ident.as<uint32_t>() + size_t(r0) * cw;
+        if (elapsed > limit) {
+          const uint32_t rows = ch - r0, cnt = rows * cw;
+          FrameDev F = f0.F;
+          F.scene = 0;
+          F.mode = FRAME_CENTER;
+          std::vector<FrameDev> parts;
+          add_parts(parts, F, band, cnt, 0, 1);
+          int rc = wavefront_run(scenes, params, fs, n, parts, st, tot, d_tc, [&](const FrameDev& P, const float4* smp) {
+            if (adaptive)
+              launch_resolve_adaptive(smp, mb[0].state.as<AdaptState>(), P.pix_list, P.n_list, 1, 0xffffffffu, 0.0,
+                                      st);
+            else
+              launch_resolve_list(smp, W.acc.as<double4>(), P.pix_list, P.n_list, 1, st);
+          });
+          if (rc != LMN_OK) return rc;
+          expired_rows = rows;
+          ++rounds;
+          break;
+        }
+        const uint32_t cnt = (r1 - r0) * cw;
+        if (adaptive) {
+          std::vector<uint32_t> nl(1, cnt);
+          HIP_TRY(hipMemcpyAsync(mb[0].list[0].p, band, size_t(cnt) * 4, hipMemcpyDeviceToDevice, st));
+          int rc = adaptive_rounds(scenes, params, fs, n, mb, nl, st, tot, d_tc, rounds);
+          if (rc != LMN_OK) return rc;
+        } else {
+          FrameDev F = f0.F;
+          F.scene = 0;
+          std::vector<FrameDev> parts;
+          add_parts(parts, F, band, cnt, 0, F.spp_a);
+          int rc = wavefront_run(scenes, params, fs, n, parts, st, tot, d_tc, [&](const FrameDev& P, const float4* smp) {
+            launch_resolve_list(smp, W.acc.as<double4>(), P.pix_list, P.n_list, uint32_t(P.n_items / P.n_list), st);
+          });
+          if (rc != LMN_OK) return rc;
+          ++rounds;
+        }
+      }
+    }
+    for (size_t i = 0; i < n; ++i) {
+      if (fs[i].black) continue;
+      if (adaptive) {
+        HIP_TRY(hipMemsetAsync(mb[i].taken.p, 0, 8, st));
+        launch_adapt_finalize(mb[i].state.as<AdaptState>(), d_out[i], fs[i].npix,
+                              mb[i].taken.as<unsigned long long>(), st);
+        unsigned long long t = 0;
+        HIP_TRY(hipMemcpyAsync(&t, mb[i].taken.p, 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        taken_total += t;
+      } else {
+        launch_finalize(W.acc.as<double4>() + acc_off[i], d_out[i], fs[i].npix, st);
+        taken_total += uint64_t(fs[i].npix - uint64_t(expired_rows) * fs[i].F.crop_w) * fs[i].F.spp_a +
+                       uint64_t(expired_rows) * fs[i].F.crop_w;
+      }
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+  }
   HIP_TRY(hipGetLastError());
   if (stats) {
-    stats->camera_rays = sum_cam;
-    stats->extension_rays = sum_ext - sum_cam;
-    stats->shadow_rays = sum_sh;
-    stats->iterations = uint32_t(iters);
-    stats->extend_launches = n_launch;
-    stats->extend_ms = t_stage[0];
-    stats->march_ms = t_stage[1] + t_stage[4];
-    stats->shade_ms = t_stage[2];
-    stats->shadow_ms = t_stage[3];
-    stats->regen_ms = t_stage[5];
+    stats->camera_rays = tot.cam;
+    stats->extension_rays = tot.ext - tot.cam;
+    stats->shadow_rays = tot.sh;
+    stats->iterations = uint32_t(tot.iters);
+    stats->extend_launches = tot.n_launch;
+    stats->extend_ms = tot.t_stage[0];
+    stats->march_ms = tot.t_stage[1] + tot.t_stage[4];
+    stats->shade_ms = tot.t_stage[2];
+    stats->shadow_ms = tot.t_stage[3];
+    stats->regen_ms = tot.t_stage[5];
+    stats->paths = taken_total;
+    stats->samples_traced = tot.cam;
+    stats->rounds = rounds;
+    stats->expired_rows = expired_rows;
   }
   return LMN_OK;
 }
@@ -695,9 +889,49 @@ int render_plain(lmn_scene* s, const lmn_render_params* p, const FrameSetup& fs,
                  lmn_stats* stats, bool fp64) {
   Workspace& W = workspace();
   uint32_t npix = fs.npix;
-  HIP_TRY(W.acc.ensure(size_t(npix) * 32));
+  if (p->time_limit > 0.0)
+    return set_err(LMN_ERR_UNSUPPORTED, "render: a time limit is supported on the wavefront path only");
   HIP_TRY(W.rays.ensure(16));
   HIP_TRY(hipMemsetAsync(W.rays.p, 0, 16, st));
+  if (p->flags & LMN_FLAG_ADAPTIVE) {
+    // one sample pass per launch; pixels that have stopped skip the rest
+    DevBuf state, list, taken;
+    HIP_TRY(state.ensure(size_t(npix) * sizeof(AdaptState)));
+    HIP_TRY(list.ensure(size_t(npix) * 4));
+    HIP_TRY(taken.ensure(8));
+    HIP_TRY(hipMemsetAsync(taken.p, 0, 8, st));
+    launch_adapt_init(state.as<AdaptState>(), list.as<uint32_t>(), npix, st);
+    const uint32_t mins = adaptive_min_samples(fs.F.spp_a);
+    for (uint32_t s0 = 0; s0 < fs.F.spp_a; ++s0) {
+      if (fp64) {
+        SceneView<double> S = s->viewD;
+        S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
+        launch_plain_adaptive_fp64(S, s->host.featD, fs.F, state.as<AdaptState>(), s0, s0 + 1, mins,
+                                   p->adaptive_threshold, W.rays.as<unsigned long long>(), st);
+      } else {
+        SceneView<float> S = s->viewF;
+        S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
+        launch_plain_adaptive_fp32(S, s->host.featF, fs.F, state.as<AdaptState>(), s0, s0 + 1, mins,
+                                   p->adaptive_threshold, W.rays.as<unsigned long long>(), st);
+      }
+    }
+    launch_adapt_finalize(state.as<AdaptState>(), d_out, npix, taken.as<unsigned long long>(), st);
+    HIP_TRY(hipGetLastError());
+    unsigned long long t = 0, r[2];
+    HIP_TRY(hipMemcpyAsync(&t, taken.p, 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(r, W.rays.p, 16, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (stats) {
+      stats->paths = t;
+      stats->samples_traced = t;
+      stats->camera_rays = t;
+      stats->extension_rays = r[0] - t;
+      stats->shadow_rays = r[1];
+      stats->rounds = fs.F.spp_a;
+    }
+    return LMN_OK;
+  }
+  HIP_TRY(W.acc.ensure(size_t(npix) * 32));
   launch_zero_acc(W.acc.as<double4>(), npix, st);
   for (uint32_t s0 = 0; s0 < fs.F.spp_a; ++s0) {
     if (fp64) {
@@ -720,6 +954,9 @@ int render_plain(lmn_scene* s, const lmn_render_params* p, const FrameSetup& fs,
     stats->extension_rays = r[0] - stats->camera_rays;
     stats->shadow_rays = r[1];
     stats->iterations = 0;
+    stats->paths = stats->camera_rays;
+    stats->samples_traced = stats->camera_rays;
+    stats->rounds = fs.F.spp_a;
   }
   return LMN_OK;
 }
@@ -729,6 +966,7 @@ int render_plain(lmn_scene* s, const lmn_render_params* p, const FrameSetup& fs,
 int render_common(lmn_scene* const* scenes, const lmn_render_params* const* params, float* const* d_out, size_t n,
                   lmn_stats* stats, TraceCounters* d_tc) {
   if (!scenes || !params || !d_out || n == 0) return set_err(LMN_ERR_INVALID, "render: null argument");
+  const auto t0 = std::chrono::steady_clock::now();  // the time limit counts from here
   std::vector<FrameSetup> fs(n);
   for (size_t i = 0; i < n; ++i) {
     if (!scenes[i] || !params[i] || !d_out[i]) return set_err(LMN_ERR_INVALID, "render: null argument");
@@ -747,7 +985,7 @@ int render_common(lmn_scene* const* scenes, const lmn_render_params* const* para
   for (size_t i = 0; i < n; ++i)
     if (fs[i].black) HIP_TRY(hipMemsetAsync(d_out[i], 0, size_t(fs[i].npix) * 12, st));
   if (p0->path == LMN_PATH_WAVEFRONT_FP32) {
-    rc = render_wavefront(scenes, params, fs.data(), d_out, n, st, stats, d_tc);
+    rc = render_wavefront(scenes, params, fs.data(), d_out, n, st, stats, d_tc, t0);
   } else if (p0->path == LMN_PATH_PLAIN_FP64 || p0->path == LMN_PATH_PLAIN_FP32) {
     for (size_t i = 0; i < n && rc == LMN_OK; ++i) {
       if (fs[i].black) continue;
@@ -758,6 +996,9 @@ int render_common(lmn_scene* const* scenes, const lmn_render_params* const* para
         stats->camera_rays += one.camera_rays;
         stats->extension_rays += one.extension_rays;
         stats->shadow_rays += one.shadow_rays;
+        stats->paths += one.paths;
+        stats->samples_traced += one.samples_traced;
+        stats->rounds += one.rounds;
       }
     }
   } else {
@@ -769,8 +1010,8 @@ int render_common(lmn_scene* const* scenes, const lmn_render_params* const* para
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, W.ev0, W.ev1));
   if (stats) {
-    stats->paths = 0;
-    for (size_t i = 0; i < n; ++i) stats->paths += uint64_t(fs[i].npix) * fs[i].F.spp_a;
+    for (size_t i = 0; i < n; ++i)  // black frames: the loop still counts its samples
+      if (fs[i].black) stats->paths += uint64_t(fs[i].npix) * fs[i].F.spp_a;
     stats->render_ms = ms;
   }
   return LMN_OK;

@@ -115,12 +115,65 @@ void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA
                            hipEvent_t* ev = nullptr);
 void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s);
 void launch_resolve(const float4* samples, double4* acc, uint32_t npix, uint32_t n_samples, hipStream_t s);
+// Same over listed pixels: samples[s * n_list + e] belongs to crop pixel list[e].
+void launch_resolve_list(const float4* samples, double4* acc, const uint32_t* list, uint32_t n_list,
+                         uint32_t n_samples, hipStream_t s);
+// Adaptive sampling: per-pixel running state of the sequential stopping rule.
+struct AdaptState {
+  double mx, my, mz;  // running mean of the clamped colour
+  double mean_lum, m2;  // Welford state of the clamped luminance
+  uint32_t n;          // samples taken
+  uint32_t done;       // the standard error fell below the threshold
+};
+// One sample into the adaptive state, in the reference's arithmetic and
+// order (src/render.rs:1641-1668): the luminance clamp at 100, Welford's
+// update, the running mean, the test after n >= min_samples and n > 1.
+LMN_HD void adapt_take(AdaptState& a, double x, double y, double z, uint32_t min_samples, double threshold) {
+#pragma clang fp contract(off)
+  double lum0 = 0.2126 * x + 0.7152 * y + 0.0722 * z;
+  if (lum0 > 100.0) {
+    double f = 100.0 / lum0;
+    x *= f;
+    y *= f;
+    z *= f;
+  }
+  a.n += 1;
+  const double n = double(a.n);
+  double lum = 0.2126 * x + 0.7152 * y + 0.0722 * z;
+  double d1 = lum - a.mean_lum;
+  a.mean_lum += d1 / n;
+  double d2 = lum - a.mean_lum;
+  a.m2 += d1 * d2;
+  a.mx = a.mx + (x - a.mx) / n;
+  a.my = a.my + (y - a.my) / n;
+  a.mz = a.mz + (z - a.mz) / n;
+  if (a.n >= min_samples && a.n > 1) {
+    double var = a.m2 / double(a.n - 1);
+    if (sqrt(var / n) < threshold) a.done = 1;
+  }
+}
+// max(spp/4, 16) (src/render.rs:1592)
+LMN_HD uint32_t adaptive_min_samples(uint32_t spp_a) { return spp_a / 4 > 16 ? spp_a / 4 : 16; }
+void launch_resolve_adaptive(const float4* samples, AdaptState* state, const uint32_t* list, uint32_t n_list,
+                             uint32_t n_samples, uint32_t min_samples, double threshold, hipStream_t s);
+void launch_adapt_compact(const AdaptState* state, const uint32_t* list_in, uint32_t n_in, uint32_t spp,
+                          uint32_t* list_out, uint32_t* count, hipStream_t s);
+void launch_adapt_finalize(const AdaptState* state, float* out, uint32_t npix, unsigned long long* taken,
+                           hipStream_t s);
+void launch_adapt_init(AdaptState* state, uint32_t* list, uint32_t npix, hipStream_t s);
 void launch_zero_acc(double4* acc, uint32_t npix, hipStream_t s);
 void launch_finalize(const double4* acc, float* out, uint32_t npix, hipStream_t s);
 void launch_plain_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, double4* acc, uint32_t s0,
                        uint32_t s1, unsigned long long* rays, hipStream_t s);
 void launch_plain_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, double4* acc, uint32_t s0,
                        uint32_t s1, unsigned long long* rays, hipStream_t s);
+struct AdaptState;
+void launch_plain_adaptive_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, AdaptState* state,
+                                uint32_t s0, uint32_t s1, uint32_t min_samples, double threshold,
+                                unsigned long long* rays, hipStream_t s);
+void launch_plain_adaptive_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, AdaptState* state,
+                                uint32_t s0, uint32_t s1, uint32_t min_samples, double threshold,
+                                unsigned long long* rays, hipStream_t s);
 // Path-replay adjoint (lmn_adjoint.hip)
 struct AdjDev {
   int32_t texture;            // parameter texture, -1 = none

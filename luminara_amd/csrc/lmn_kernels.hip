@@ -676,12 +676,13 @@ __global__ void k_counters_init(WaveCounters* ctr, uint64_t n_items) {
 // Resolve: per-sample luminance clamp (src/render.rs:1691-1697) and filter
 // weighting, accumulated per pixel in FP64 in sample order.
 __global__ void k_resolve(const float4* __restrict__ samples, double4* __restrict__ acc, uint32_t npix,
-                          uint32_t n_samples) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npix) return;
+                          uint32_t n_samples, const uint32_t* __restrict__ list) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= npix) return;
+  const uint32_t p = list ? list[e] : e;
   double4 a = acc[p];
   for (uint32_t s = 0; s < n_samples; ++s) {
-    float4 v = samples[size_t(s) * npix + p];
+    float4 v = samples[size_t(s) * npix + e];
     double x = v.x, y = v.y, z = v.z, w = v.w;
     double lum = 0.2126 * x + 0.7152 * y + 0.0722 * z;
     if (lum > 100.0) {
@@ -696,6 +697,53 @@ __global__ void k_resolve(const float4* __restrict__ samples, double4* __restric
     a.w += w;
   }
   acc[p] = a;
+}
+
+// Adaptive sampling (src/render.rs:1626-1677): each listed pixel takes its
+// round's samples in stratum order, luminance-clamped, into a Welford update
+// of the luminance and a running mean of the colour, and stops at the first
+// n >= max(spp/4, 16), n > 1 whose standard error sqrt(var / n) is below the
+// threshold.  Samples of the round after that point are left unused, so the
+// pixel's value is the mean of exactly the samples the sequential loop takes.
+__global__ void k_resolve_adaptive(const float4* __restrict__ samples, AdaptState* __restrict__ state,
+                                   const uint32_t* __restrict__ list, uint32_t n_list, uint32_t n_samples,
+                                   uint32_t min_samples, double threshold) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_list) return;
+  const uint32_t p = list[e];
+  AdaptState a = state[p];
+  for (uint32_t s = 0; s < n_samples && !a.done; ++s) {
+    float4 v = samples[size_t(s) * n_list + e];
+    adapt_take(a, v.x, v.y, v.z, min_samples, threshold);
+  }
+  state[p] = a;
+}
+// Pixels still sampling after a round: listed again (any order: a pixel's
+// samples depend on its index and sample number only).
+__global__ void k_adapt_compact(const AdaptState* __restrict__ state, const uint32_t* __restrict__ list_in,
+                                uint32_t n_in, uint32_t spp, uint32_t* __restrict__ list_out,
+                                uint32_t* __restrict__ count) {
+  uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_in) return;
+  const uint32_t p = list_in[e];
+  const AdaptState a = state[p];
+  if (!a.done && a.n < spp) list_out[atomicAdd(count, 1u)] = p;
+}
+__global__ void k_adapt_finalize(const AdaptState* __restrict__ state, float* __restrict__ out, uint32_t npix,
+                                 unsigned long long* __restrict__ taken) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  const AdaptState a = state[p];
+  out[size_t(p) * 3 + 0] = float(a.mx);
+  out[size_t(p) * 3 + 1] = float(a.my);
+  out[size_t(p) * 3 + 2] = float(a.mz);
+  atomicAdd(taken, (unsigned long long)a.n);
+}
+__global__ void k_adapt_init(AdaptState* __restrict__ state, uint32_t* __restrict__ list, uint32_t npix) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  state[p] = AdaptState{0.0, 0.0, 0.0, 0.0, 0.0, 0u, 0u};
+  list[p] = p;
 }
 
 __global__ void k_zero_acc(double4* acc, uint32_t npix) {
@@ -837,7 +885,32 @@ void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s) {
   hipLaunchKernelGGL(k_counters_init, dim3(1), dim3(64), 0, s, ctr, n_items);
 }
 void launch_resolve(const float4* samples, double4* acc, uint32_t npix, uint32_t n_samples, hipStream_t s) {
-  hipLaunchKernelGGL(k_resolve, dim3((npix + 255) / 256), dim3(256), 0, s, samples, acc, npix, n_samples);
+  hipLaunchKernelGGL(k_resolve, dim3((npix + 255) / 256), dim3(256), 0, s, samples, acc, npix, n_samples,
+                     (const uint32_t*)nullptr);
+}
+void launch_resolve_list(const float4* samples, double4* acc, const uint32_t* list, uint32_t n_list,
+                         uint32_t n_samples, hipStream_t s) {
+  if (n_list)
+    hipLaunchKernelGGL(k_resolve, dim3((n_list + 255) / 256), dim3(256), 0, s, samples, acc, n_list, n_samples, list);
+}
+void launch_resolve_adaptive(const float4* samples, AdaptState* state, const uint32_t* list, uint32_t n_list,
+                             uint32_t n_samples, uint32_t min_samples, double threshold, hipStream_t s) {
+  if (n_list)
+    hipLaunchKernelGGL(k_resolve_adaptive, dim3((n_list + 255) / 256), dim3(256), 0, s, samples, state, list, n_list,
+                       n_samples, min_samples, threshold);
+}
+void launch_adapt_compact(const AdaptState* state, const uint32_t* list_in, uint32_t n_in, uint32_t spp,
+                          uint32_t* list_out, uint32_t* count, hipStream_t s) {
+  if (n_in)
+    hipLaunchKernelGGL(k_adapt_compact, dim3((n_in + 255) / 256), dim3(256), 0, s, state, list_in, n_in, spp,
+                       list_out, count);
+}
+void launch_adapt_finalize(const AdaptState* state, float* out, uint32_t npix, unsigned long long* taken,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_adapt_finalize, dim3((npix + 255) / 256), dim3(256), 0, s, state, out, npix, taken);
+}
+void launch_adapt_init(AdaptState* state, uint32_t* list, uint32_t npix, hipStream_t s) {
+  hipLaunchKernelGGL(k_adapt_init, dim3((npix + 255) / 256), dim3(256), 0, s, state, list, npix);
 }
 void launch_zero_acc(double4* acc, uint32_t npix, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_acc, dim3((npix + 255) / 256), dim3(256), 0, s, acc, npix);

@@ -7,6 +7,69 @@ namespace lmn {
 // ---------------------------------------------------------------------------
 // Plain megakernel: one thread per pixel, paths run to completion one after
 // another, private-memory traversal stack, FP64 (or FP32) throughout.
+
+// One sample (work item k) of ray_color (src/render.rs:1472-1537): the
+// unclamped radiance and the filter weight.
+template <class R, uint32_t FEAT>
+__device__ V3<R> plain_sample(const SceneView<R>& S, const FrameDev& F, uint64_t k, R& w,
+                              unsigned long long& n_ext, unsigned long long& n_sh) {
+  Rng rng;
+  V3<R> o, d;
+  uint32_t pix;
+  generate_item<R>(F, S.cam, k, rng, o, d, w, pix);
+  V3<R> L = {R(0), R(0), R(0)}, beta = {R(1), R(1), R(1)};
+  bool skip = false;
+  for (uint32_t depth = F.max_depth; depth > 0; --depth) {
+    PrivateStack<R> st;
+    R t;
+    uint32_t idx;
+    ++n_ext;
+    if (!trace<R, FEAT, false>(S, o, d, R(0.001), R(INFINITY), t, idx, st)) {
+      L = L + hadamard(beta, background(S.bg, d));
+      break;
+    }
+    Surf<R> sf;
+    surface<R, FEAT>(S, idx, o, d, t, sf);
+    const MaterialT<R>& m = S.materials[sf.mat];
+    V3<R> em = skip ? V3<R>{R(0), R(0), R(0)} : emitted(S, m, sf.u, sf.v);
+    V3<R> ndir, att;
+    if (!scatter(S, m, d, sf, rng, ndir, att)) {
+      L = L + hadamard(beta, em);
+      break;
+    }
+    R surv = clamp_r(max3(att), R(0.05), R(1));
+    bool rr = depth < 47;
+    if (rr && rng.next<R>() > surv) {
+      L = L + hadamard(beta, em);
+      break;
+    }
+    R rrw = rr ? R(1) / surv : R(1);
+    bool use_nee = !is_specular(m) && S.n_lights > 0;
+    V3<R> direct = {R(0), R(0), R(0)};
+    bool nee_nan = false;
+    if (use_nee) {
+      V3<R> ld, c;
+      R dist;
+      if (nee_sample(S, sf.p, sf.n, att, rng, ld, dist, c)) {
+        nee_nan = has_nan(c);
+        PrivateStack<R> st2;
+        R t2;
+        uint32_t i2;
+        ++n_sh;
+        if (!trace<R, FEAT, true>(S, sf.p, ld, R(0.001), dist - R(0.001), t2, i2, st2)) direct = c;
+      }
+    }
+    V3<R> attw = att * rrw;
+    if (has_nan(em) || nee_nan || has_nan(attw)) break;  // level result NaN -> 0
+    L = L + hadamard(beta, em + direct);
+    beta = hadamard(beta, attw);
+    o = sf.p;
+    d = ndir;
+    skip = use_nee;
+  }
+  return L;
+}
+
 template <class R, uint32_t FEAT>
 __global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, double4* __restrict__ acc,
                                                uint32_t s_begin, uint32_t s_end, unsigned long long* rays) {
@@ -16,61 +79,8 @@ __global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, doubl
   double4 a = acc[p];
   unsigned long long n_ext = 0, n_sh = 0;
   for (uint32_t s = s_begin; s < s_end; ++s) {
-    Rng rng;
-    V3<R> o, d;
     R w;
-    uint32_t pix;
-    generate_item<R>(F, S.cam, uint64_t(s) * npix + p, rng, o, d, w, pix);
-    V3<R> L = {R(0), R(0), R(0)}, beta = {R(1), R(1), R(1)};
-    bool skip = false;
-    for (uint32_t depth = F.max_depth; depth > 0; --depth) {
-      PrivateStack<R> st;
-      R t;
-      uint32_t idx;
-      ++n_ext;
-      if (!trace<R, FEAT, false>(S, o, d, R(0.001), R(INFINITY), t, idx, st)) {
-        L = L + hadamard(beta, background(S.bg, d));
-        break;
-      }
-      Surf<R> sf;
-      surface<R, FEAT>(S, idx, o, d, t, sf);
-      const MaterialT<R>& m = S.materials[sf.mat];
-      V3<R> em = skip ? V3<R>{R(0), R(0), R(0)} : emitted(S, m, sf.u, sf.v);
-      V3<R> ndir, att;
-      if (!scatter(S, m, d, sf, rng, ndir, att)) {
-        L = L + hadamard(beta, em);
-        break;
-      }
-      R surv = clamp_r(max3(att), R(0.05), R(1));
-      bool rr = depth < 47;
-      if (rr && rng.next<R>() > surv) {
-        L = L + hadamard(beta, em);
-        break;
-      }
-      R rrw = rr ? R(1) / surv : R(1);
-      bool use_nee = !is_specular(m) && S.n_lights > 0;
-      V3<R> direct = {R(0), R(0), R(0)};
-      bool nee_nan = false;
-      if (use_nee) {
-        V3<R> ld, c;
-        R dist;
-        if (nee_sample(S, sf.p, sf.n, att, rng, ld, dist, c)) {
-          nee_nan = has_nan(c);
-          PrivateStack<R> st2;
-          R t2;
-          uint32_t i2;
-          ++n_sh;
-          if (!trace<R, FEAT, true>(S, sf.p, ld, R(0.001), dist - R(0.001), t2, i2, st2)) direct = c;
-        }
-      }
-      V3<R> attw = att * rrw;
-      if (has_nan(em) || nee_nan || has_nan(attw)) break;  // level result NaN -> 0
-      L = L + hadamard(beta, em + direct);
-      beta = hadamard(beta, attw);
-      o = sf.p;
-      d = ndir;
-      skip = use_nee;
-    }
+    V3<R> L = plain_sample<R, FEAT>(S, F, uint64_t(s) * npix + p, w, n_ext, n_sh);
     double x = double(L.x), y = double(L.y), z = double(L.z);
     double lum = 0.2126 * x + 0.7152 * y + 0.0722 * z;
     if (lum > 100.0) {
@@ -86,6 +96,29 @@ __global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, doubl
     a.w += ww;
   }
   acc[p] = a;
+  if (rays) {
+    atomicAdd(&rays[0], n_ext);
+    atomicAdd(&rays[1], n_sh);
+  }
+}
+
+// Adaptive sampling, one pixel per thread in sample order: sample s of a
+// pixel still sampling goes into its state (src/render.rs:1626-1677).
+template <class R, uint32_t FEAT>
+__global__ __launch_bounds__(128) void k_plain_adaptive(SceneView<R> S, FrameDev F, AdaptState* __restrict__ state,
+                                                        uint32_t s_begin, uint32_t s_end, uint32_t min_samples,
+                                                        double threshold, unsigned long long* rays) {
+  uint32_t npix = F.crop_w * F.crop_h;
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  AdaptState a = state[p];
+  unsigned long long n_ext = 0, n_sh = 0;
+  for (uint32_t s = s_begin; s < s_end && !a.done; ++s) {
+    R w;
+    V3<R> L = plain_sample<R, FEAT>(S, F, uint64_t(s) * npix + p, w, n_ext, n_sh);
+    adapt_take(a, double(L.x), double(L.y), double(L.z), min_samples, threshold);
+  }
+  state[p] = a;
   if (rays) {
     atomicAdd(&rays[0], n_ext);
     atomicAdd(&rays[1], n_sh);
@@ -212,6 +245,26 @@ void launch_plain_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev&
                        uint32_t s1, unsigned long long* rays, hipStream_t s) {
   uint32_t npix = F.crop_w * F.crop_h;
 #define CALL(f) hipLaunchKernelGGL((k_plain<float, f>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, acc, s0, s1, rays)
+  LMN_FEAT_SWITCH(feat, CALL)
+#undef CALL
+}
+void launch_plain_adaptive_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, AdaptState* state,
+                                uint32_t s0, uint32_t s1, uint32_t min_samples, double threshold,
+                                unsigned long long* rays, hipStream_t s) {
+  uint32_t npix = F.crop_w * F.crop_h;
+#define CALL(f)                                                                                          \
+  hipLaunchKernelGGL((k_plain_adaptive<double, f>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, state, s0, s1, \
+                     min_samples, threshold, rays)
+  LMN_FEAT64_SWITCH(feat, CALL)
+#undef CALL
+}
+void launch_plain_adaptive_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, AdaptState* state,
+                                uint32_t s0, uint32_t s1, uint32_t min_samples, double threshold,
+                                unsigned long long* rays, hipStream_t s) {
+  uint32_t npix = F.crop_w * F.crop_h;
+#define CALL(f)                                                                                         \
+  hipLaunchKernelGGL((k_plain_adaptive<float, f>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, state, s0, s1, \
+                     min_samples, threshold, rays)
   LMN_FEAT_SWITCH(feat, CALL)
 #undef CALL
 }

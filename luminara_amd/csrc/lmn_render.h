@@ -916,7 +916,11 @@ template <class R> LMN_HD R filter_weight(uint32_t f, R dx, R dy) {
 
 // Per-frame constants of the pixel loop (src/render.rs:1571-1596).  In a
 // wavefront chunk one FrameDev describes one part: a range of whole sample
-// passes [item_base / npix, item_base / npix + n_items / npix) of `scene`.
+// passes [item_base / n, item_base / n + n_items / n) of `scene` over its n
+// pixels: the whole crop (pix_list null, n = crop_w * crop_h) or the crop
+// pixels listed in pix_list (n = n_list; adaptive rounds, row bands).
+enum : uint32_t { FRAME_STRATIFIED = 0, FRAME_CENTER = 1 };
+constexpr uint32_t kCenterSample = 0xffffffffu;  // stream of a pixel's single centre sample
 struct FrameDev {
   uint32_t full_w, full_h;
   uint32_t crop_x, crop_y, crop_w, crop_h;
@@ -926,20 +930,37 @@ struct FrameDev {
   uint64_t n_items;    // work items of the part (whole frame: crop_w * crop_h * spp_a)
   uint64_t item_base;  // first frame-local work item of the part
   uint32_t scene;      // index into the batch's scene views
+  uint32_t mode;       // FRAME_STRATIFIED, or FRAME_CENTER: one sample through the pixel centre
+  const uint32_t* pix_list;  // crop pixel of each list entry, or null (every crop pixel in order)
+  uint32_t n_list;
   uint32_t pad;
 };
 
-// Generates work item k = sample * (crop_w*crop_h) + crop pixel: its camera
-// ray, RNG stream and filter weight (src/render.rs:1683-1690).
+// Generates work item k = sample * n + entry (n pixels, see FrameDev): its
+// camera ray, RNG stream and filter weight (src/render.rs:1683-1690).  In
+// FRAME_CENTER mode the ray goes through the pixel centre, without jitter
+// draws, with weight 1: the sample a row gets once the time limit has passed
+// (src/render.rs:1614-1624).
 template <class R>
 LMN_HD void generate_item(const FrameDev& F, const CameraT<R>& C, uint64_t k, Rng& rng, V3<R>& o, V3<R>& d, R& w,
                           uint32_t& pixel) {
-  uint64_t npix = uint64_t(F.crop_w) * F.crop_h;
-  uint32_t s = uint32_t(k / npix);
-  uint32_t pl = uint32_t(k - uint64_t(s) * npix);
+  uint64_t n = F.pix_list ? uint64_t(F.n_list) : uint64_t(F.crop_w) * F.crop_h;
+  uint32_t s = uint32_t(k / n);
+  uint32_t e = uint32_t(k - uint64_t(s) * n);
+  uint32_t pl = F.pix_list ? F.pix_list[e] : e;
   uint32_t ci = pl % F.crop_w, cj = pl / F.crop_w;
   uint32_t gi = ci + F.crop_x, gj = cj + F.crop_y;
   uint64_t gpix = uint64_t(gj) * F.full_w + gi;
+  if (F.mode == FRAME_CENTER) {
+    rng.key = path_key(F.fkey, gpix, kCenterSample);
+    rng.n = 0;
+    R uc = (R(gi) + R(0.5)) / R(F.full_w - 1);
+    R vc = (R(F.full_h - 1 - gj) + R(0.5)) / R(F.full_h - 1);
+    camera_ray(C, uc, vc, rng, o, d);
+    w = R(1);
+    pixel = pl;
+    return;
+  }
   rng.key = path_key(F.fkey, gpix, s);
   rng.n = 0;
   uint32_t sy = s / F.sqrt_spp, sx = s - sy * F.sqrt_spp;

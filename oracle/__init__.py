@@ -27,7 +27,8 @@ class OracleParams(C.Structure):
                 ("seed", C.c_uint64), ("filter", C.c_uint32), ("has_crop", C.c_uint32), ("crop", C.c_uint32 * 4),
                 ("rng_mode", C.c_uint32), ("scene_key", C.c_uint32), ("threads", C.c_uint32),
                 ("exact_marched", C.c_uint32), ("row_begin", C.c_uint32), ("row_end", C.c_uint32),
-                ("row_step", C.c_uint32), ("pad", C.c_uint32)]
+                ("row_step", C.c_uint32), ("adaptive", C.c_uint32), ("adaptive_threshold", C.c_double),
+                ("time_limit", C.c_double)]
 
 
 class OracleStats(C.Structure):
@@ -123,6 +124,9 @@ def render(world, config, rng_mode=RNG_COUNTER, threads=0, scene_key=0, exact=Tr
         cw, ch = min(config.crop[2], fw), min(config.crop[3], fh)
     p.rng_mode, p.scene_key, p.threads = rng_mode, scene_key, threads
     p.exact_marched = 1 if exact else 0
+    p.adaptive = 1 if getattr(config, "adaptive", False) else 0
+    p.adaptive_threshold = getattr(config, "adaptive_threshold", 0.03)
+    p.time_limit = getattr(config, "time_limit", 0.0)
     rb, re = rows if rows is not None else (0, ch)
     re = min(re, ch)
     p.row_begin, p.row_end, p.row_step = rb, re, row_step

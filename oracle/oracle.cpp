@@ -1627,7 +1627,9 @@ int oracle_render(const lmn_scene_desc* desc, const oracle_params* p, double* me
   uint32_t sq = uint32_t(std::ceil(std::sqrt(double(p->spp))));
   uint64_t fkey = frame_key(p->seed, p->scene_key);
   std::atomic<uint32_t> next_row{0};
-  std::atomic<uint64_t> n_ext{0}, n_sh{0};
+  std::atomic<uint64_t> n_ext{0}, n_sh{0}, n_cam{0};
+  std::atomic<bool> time_expired{false};
+  const uint32_t adaptive_min_samples = p->adaptive ? std::max(sq * sq / 4, 16u) : 0u;  // render.rs:1592
   unsigned nt = p->threads ? p->threads : std::max(1u, std::thread::hardware_concurrency());
   auto t0 = std::chrono::steady_clock::now();
   auto worker = [&]() {
@@ -1635,13 +1637,85 @@ int oracle_render(const lmn_scene_desc* desc, const oracle_params* p, double* me
     for (;;) {
       uint32_t r = next_row.fetch_add(1);
       if (r >= n_rows) break;
+      // render.rs:1600-1605: checked once per row
+      if (p->time_limit > 0.0 &&
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > p->time_limit)
+        time_expired.store(true);
+      const bool expired = time_expired.load();
       uint32_t j = rb + r * rs;
       uint32_t gj = j + cy;
       double y = double(fh - 1 - gj);
       std::unique_ptr<XoshiroRng> row_rng;
       if (p->rng_mode == ORACLE_RNG_REFERENCE) row_rng = std::make_unique<XoshiroRng>(uint64_t(gj) * p->seed);
+      uint64_t row_rays = 0;
       for (uint32_t i = 0; i < cw; ++i) {
         uint32_t gi = i + cx;
+        size_t idx = (size_t(r) * cw + i) * 3;
+        if (expired) {  // render.rs:1614-1624: one sample through the pixel centre
+          CounterRng crng(path_key(fkey, uint64_t(gj) * fw + gi, 0xffffffffu));
+          Rng& rng = row_rng ? static_cast<Rng&>(*row_rng) : static_cast<Rng&>(crng);
+          double u = (double(gi) + 0.5) / double(fw - 1);
+          double v = (y + 0.5) / double(fh - 1);
+          Ray ray = w.get_ray(u, v, rng);
+          Vec3 smp = ray_color(w, ray, rng, p->max_depth, false, cnt);
+          ++row_rays;
+          double lum = 0.2126 * smp.x + 0.7152 * smp.y + 0.0722 * smp.z;
+          Vec3 c = lum > 100.0 ? smp * (100.0 / lum) : smp;
+          mean[idx] = c.x;
+          mean[idx + 1] = c.y;
+          mean[idx + 2] = c.z;
+          if (var)
+            for (int k = 0; k < 3; ++k) var[idx + k] = 0.0;
+          continue;
+        }
+        if (p->adaptive) {  // render.rs:1626-1677
+          Vec3 mn;
+          double m2_lum = 0.0, mean_lum = 0.0;
+          uint32_t nn = 0;
+          double m1[3] = {0, 0, 0}, m2[3] = {0, 0, 0};
+          bool stop = false;
+          for (uint32_t sy = 0; sy < sq && !stop; ++sy) {
+            for (uint32_t sx = 0; sx < sq; ++sx) {
+              CounterRng crng(path_key(fkey, uint64_t(gj) * fw + gi, sy * sq + sx));
+              Rng& rng = row_rng ? static_cast<Rng&>(*row_rng) : static_cast<Rng&>(crng);
+              double u = (double(gi) + (double(sx) + rng.next_f64()) / double(sq)) / double(fw - 1);
+              double v = (y + (double(sy) + rng.next_f64()) / double(sq)) / double(fh - 1);
+              Ray ray = w.get_ray(u, v, rng);
+              Vec3 smp = ray_color(w, ray, rng, p->max_depth, false, cnt);
+              double luminance = 0.2126 * smp.x + 0.7152 * smp.y + 0.0722 * smp.z;
+              Vec3 clamped = luminance > 100.0 ? smp * (100.0 / luminance) : smp;
+              nn += 1;
+              double lum = 0.2126 * clamped.x + 0.7152 * clamped.y + 0.0722 * clamped.z;
+              double delta_lum = lum - mean_lum;
+              mean_lum += delta_lum / double(nn);
+              double delta_lum2 = lum - mean_lum;
+              m2_lum += delta_lum * delta_lum2;
+              mn = Vec3(mn.x + (clamped.x - mn.x) / double(nn), mn.y + (clamped.y - mn.y) / double(nn),
+                        mn.z + (clamped.z - mn.z) / double(nn));
+              double cv[3] = {clamped.x, clamped.y, clamped.z};
+              for (int k = 0; k < 3; ++k) {  // Welford per channel, for the statistical tests
+                double dlt = cv[k] - m1[k];
+                m1[k] += dlt / nn;
+                m2[k] += dlt * (cv[k] - m1[k]);
+              }
+              if (nn >= adaptive_min_samples && nn > 1) {
+                double variance = m2_lum / double(nn - 1);
+                double std_error = std::sqrt(variance / double(nn));
+                if (std_error < p->adaptive_threshold) {
+                  stop = true;
+                  break;
+                }
+              }
+            }
+          }
+          row_rays += nn;
+          mean[idx] = mn.x;
+          mean[idx + 1] = mn.y;
+          mean[idx + 2] = mn.z;
+          if (var)
+            for (int k = 0; k < 3; ++k) var[idx + k] = nn > 1 ? m2[k] / (nn - 1) : 0.0;
+          continue;
+        }
         Vec3 color;
         double tw = 0.0;
         double m1[3] = {0, 0, 0}, m2[3] = {0, 0, 0};
@@ -1670,13 +1744,14 @@ int oracle_render(const lmn_scene_desc* desc, const oracle_params* p, double* me
             }
           }
         Vec3 out = tw > 0.0 ? color / tw : color;
-        size_t idx = (size_t(r) * cw + i) * 3;
+        row_rays += uint64_t(sq) * sq;
         mean[idx] = out.x;
         mean[idx + 1] = out.y;
         mean[idx + 2] = out.z;
         if (var)
           for (int k = 0; k < 3; ++k) var[idx + k] = ns > 1 ? m2[k] / (ns - 1) : 0.0;
       }
+      n_cam += row_rays;
     }
     n_ext += cnt.ext;
     n_sh += cnt.shadow;
@@ -1686,8 +1761,8 @@ int oracle_render(const lmn_scene_desc* desc, const oracle_params* p, double* me
   for (auto& t : th) t.join();
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (st) {
-    st->camera_rays = uint64_t(n_rows) * cw * sq * sq;
-    st->extension_rays = n_ext.load() - (sq ? st->camera_rays : 0);
+    st->camera_rays = n_cam.load();
+    st->extension_rays = n_ext.load() - st->camera_rays;
     st->shadow_rays = n_sh.load();
     st->render_secs = secs;
     st->threads = nt;

@@ -45,11 +45,13 @@ typedef struct oracle_params {
   uint32_t row_begin;      /* render only rows [row_begin, row_end) of the crop, */
   uint32_t row_end;        /* every row_step-th one; row_end = 0 means all rows, */
   uint32_t row_step;       /* row_step 0 means 1                                 */
-  uint32_t pad;
+  uint32_t adaptive;       /* adaptive sampling (render.rs:1626-1677)            */
+  double adaptive_threshold;
+  double time_limit;       /* seconds, 0 = none (render.rs:1600-1625)            */
 } oracle_params;
 
 typedef struct oracle_stats {
-  uint64_t camera_rays, extension_rays, shadow_rays;
+  uint64_t camera_rays, extension_rays, shadow_rays; /* camera_rays: the reference's total_rays */
   double render_secs; /* pixel loop only (src/render.rs:1596 -> :1746) */
   uint32_t threads;
   uint32_t pad;

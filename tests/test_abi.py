@@ -35,9 +35,10 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(_abi.Light) == 136
     assert C.sizeof(_abi.Background) == 88
     assert C.sizeof(_abi.Camera) == 152
-    assert C.sizeof(_abi.RenderParams) == 88
+    assert C.sizeof(_abi.RenderParams) == 104
     assert C.sizeof(_abi.Ray) == 64
     assert C.sizeof(_abi.Hit) == 64
-    assert C.sizeof(_abi.RenderConfig) == 152
+    assert C.sizeof(_abi.RenderConfig) == 168
     assert C.sizeof(_abi.PostParams) == 48
     assert C.sizeof(_abi.Image) == 72
+    assert C.sizeof(_abi.Stats) == 104
