@@ -53,7 +53,10 @@ extern "C" {
  *   PLANE           point[3], normal[3] (as given)          src/plane.rs:18
  *   TORUS           center[3], major_r, minor_r             src/torus.rs:24
  *   SUPERELLIPSOID  center[3], scale[3], e1, e2             src/superellipsoid.rs:26
- *   SPRING          center[3], coil_r, tube_r, pitch, turns src/spring.rs:35   */
+ *   SPRING          center[3], coil_r, tube_r, pitch, turns src/spring.rs:35
+ *   DISK            center[3], normal[3] (as given), radius src/disk.rs:16
+ *   CYLINDER        center[3], radius, y_min, y_max         src/cylinder.rs:17
+ *   CONE            center[3], radius, y_min, y_max (apex)  src/cone.rs:18     */
 enum {
   LMN_PRIM_SPHERE = 0,
   LMN_PRIM_QUAD = 1,
@@ -65,7 +68,10 @@ enum {
   LMN_PRIM_TORUS = 7,
   LMN_PRIM_SUPERELLIPSOID = 8,
   LMN_PRIM_SPRING = 9,
-  LMN_PRIM_KIND_COUNT = 10
+  LMN_PRIM_DISK = 10,
+  LMN_PRIM_CYLINDER = 11,
+  LMN_PRIM_CONE = 12,
+  LMN_PRIM_KIND_COUNT = 13
 };
 
 typedef struct lmn_prim {

@@ -157,6 +157,15 @@ static void object_bounds(const lmn_prim& p, double lo[3], double hi[3]) {
       lo[2] = q[2] - r; hi[2] = q[2] + r;
       break;
     }
+    case LMN_PRIM_DISK:  // src/disk.rs:61-65: centre +/- radius on every axis
+      for (int k = 0; k < 3; ++k) { lo[k] = q[k] - q[6]; hi[k] = q[k] + q[6]; }
+      break;
+    case LMN_PRIM_CYLINDER:  // src/cylinder.rs:75-80
+    case LMN_PRIM_CONE:      // src/cone.rs:80-85
+      lo[0] = q[0] - q[3]; hi[0] = q[0] + q[3];
+      lo[1] = q[4]; hi[1] = q[5];
+      lo[2] = q[2] - q[3]; hi[2] = q[2] + q[3];
+      break;
     default:
       for (int k = 0; k < 3; ++k) { lo[k] = -INFINITY; hi[k] = INFINITY; }
   }
@@ -547,6 +556,21 @@ void fill_record(const lmn_prim& p, const XformT<double>* x, PrimT<R>& r, uint32
       }
       break;
     }
+    case LMN_PRIM_DISK: {  // Disk::new (src/disk.rs:16-28): unit normal, (0, 1, 0) when near zero
+      double n[3] = {q[3], q[4], q[5]};
+      if (std::fabs(n[0]) < 1e-8 && std::fabs(n[1]) < 1e-8 && std::fabs(n[2]) < 1e-8) {
+        n[0] = 0; n[1] = 1; n[2] = 0;
+      } else {
+        double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (double& c : n) c = c / l;
+      }
+      for (int k = 0; k < 3; ++k) {
+        v[k] = q[k];
+        v[3 + k] = n[k];
+      }
+      v[6] = q[6];
+      break;
+    }
     default:
       for (int k = 0; k < 12; ++k) v[k] = q[k];
   }
@@ -563,6 +587,7 @@ void fill_record(const lmn_prim& p, const XformT<double>* x, PrimT<R>& r, uint32
 // hit-distance tolerance (e.g. a r = 1000 ground sphere).
 bool fp64_island(const lmn_prim& p) {
   if (p.kind == LMN_PRIM_TORUS || p.kind == LMN_PRIM_SUPERELLIPSOID || p.kind == LMN_PRIM_SPRING) return true;
+  if (p.kind >= LMN_PRIM_DISK) return true;  // disk, cylinder, cone: second BVH, FP64
   if (p.kind == LMN_PRIM_SPHERE) return std::fabs(p.p[3]) >= 128.0;
   return false;
 }
@@ -668,13 +693,14 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
     }
 
   // main BVH over the analytic kinds, a second BVH over the marched kinds
-  // (torus, superellipsoid, spring), the unbounded planes last
+  // (torus, superellipsoid, spring) and the FP64 quadrics (disk, cylinder,
+  // cone), the unbounded planes last
   std::vector<uint32_t> bounded, marched, planes;
   for (uint32_t i = 0; i < desc.n_prims; ++i) {
     uint32_t k = desc.prims[i].kind;
     if (k == LMN_PRIM_PLANE)
       planes.push_back(i);
-    else if (k >= LMN_PRIM_TORUS)
+    else if (k >= LMN_PRIM_TORUS)  // marched kinds and the quadrics
       marched.push_back(i);
     else
       bounded.push_back(i);

@@ -581,10 +581,60 @@ class Loader {
     }
     reject_if_present(root, "hemisphere", "(hemisphere)");
     reject_if_present(root, "fog", "(constant medium)");
-    reject_if_present(root, "cone", "(cone)");
+    // cones and cylinders: y from center.y to center.y + height, then
+    // RotateX/Y/Z and Translate wrappers (src/scene.rs:1364-1375, :1422-1456)
+    auto axial = [&](const TomlValue& s, const std::string& ctx, uint32_t kind) {
+      double c[3];
+      req_vec3(s, "center", ctx, c);
+      double r = req_num(s, "radius", ctx), h = req_num(s, "height", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      if (s.get("bump_strength") || s.get("normal_map"))
+        fail(LMN_ERR_UNSUPPORTED, ctx + " bump/normal maps are not on the GPU path");
+      std::vector<lmn_xform_op> ops;
+      double a;
+      if (opt_num(s, "rotate_x", ctx, a)) ops.push_back(op(LMN_XF_ROTATE_X, a));
+      if (opt_num(s, "rotate_y", ctx, a)) ops.push_back(op(LMN_XF_ROTATE_Y, a));
+      if (opt_num(s, "rotate_z", ctx, a)) ops.push_back(op(LMN_XF_ROTATE_Z, a));
+      double t[3];
+      if (opt_vec3(s, "translate", ctx, t)) ops.push_back(op(LMN_XF_TRANSLATE, t[0], t[1], t[2]));
+      int32_t xf = ops.empty() ? -1 : int32_t(add_transform(ops));
+      lmn_prim& p = add_prim(kind, mat, xf, new_object());
+      for (int k = 0; k < 3; ++k) p.p[k] = c[k];
+      p.p[3] = r;
+      p.p[4] = c[1];
+      p.p[5] = c[1] + h;
+    };
+    for (auto& s : table_array(root, "cone")) axial(s, "cone", LMN_PRIM_CONE);
     reject_if_present(root, "annulus", "(annulus)");
-    reject_if_present(root, "disk", "(disk)");
-    reject_if_present(root, "cylinder", "(cylinder)");
+    for (auto& s : table_array(root, "disk")) {  // src/scene.rs:1388-1420
+      const std::string ctx = "disk";
+      double c[3], n[3];
+      req_vec3(s, "center", ctx, c);
+      req_vec3(s, "normal", ctx, n);
+      double r = req_num(s, "radius", ctx);
+      MatInfo mi;
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material", &mi);
+      if (mi.emissive) {  // an emissive disk is an NEE light (normal.unit(), as given)
+        lmn_light l{};
+        l.kind = LMN_LIGHT_DISK;
+        l.material = mat;
+        double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; ++k) {
+          l.origin[k] = c[k];
+          l.normal[k] = n[k] / len;
+          l.emission[k] = mi.emission[k];
+        }
+        l.radius = r;
+        out_.lights.push_back(l);
+      }
+      lmn_prim& p = add_prim(LMN_PRIM_DISK, mat, -1, new_object());
+      for (int k = 0; k < 3; ++k) {
+        p.p[k] = c[k];
+        p.p[3 + k] = n[k];
+      }
+      p.p[6] = r;
+    }
+    for (auto& s : table_array(root, "cylinder")) axial(s, "cylinder", LMN_PRIM_CYLINDER);
     reject_if_present(root, "prism", "(prism)");
     reject_if_present(root, "wedge", "(wedge)");
     reject_if_present(root, "pyramid", "(pyramid)");

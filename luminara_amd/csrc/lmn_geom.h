@@ -23,8 +23,14 @@ namespace lmn {
 
 enum : uint32_t {
   K_SPHERE = 0, K_QUAD = 1, K_RECT_XY = 2, K_RECT_XZ = 3, K_RECT_YZ = 4, K_TRIANGLE = 5,
-  K_PLANE = 6, K_TORUS = 7, K_SUPERELLIPSOID = 8, K_SPRING = 9
+  K_PLANE = 6, K_TORUS = 7, K_SUPERELLIPSOID = 8, K_SPRING = 9, K_DISK = 10, K_CYLINDER = 11, K_CONE = 12
 };
+// Torus, superellipsoid, spring: the marched kinds (own BVH, FP64 roots).
+LMN_HD bool is_marched_kind(uint32_t k) { return k >= K_TORUS && k <= K_SPRING; }
+// Disk, cylinder, cone: rare analytic kinds kept in the marched kinds' BVH
+// and evaluated there in FP64, out of line, so the main traversal carries
+// none of their code.
+LMN_HD bool is_quadric_kind(uint32_t k) { return k >= K_DISK; }
 constexpr uint32_t KIND_MASK = 0x1f;
 constexpr uint32_t FLAG_FP64 = 0x20;  // FP32 path evaluates this primitive in FP64
 constexpr uint32_t LEAF_BIT = 0x80000000u;
@@ -35,6 +41,7 @@ constexpr uint32_t XF_NONE = 0xffffffffu;
 //   RECT_* a0 a1 b0 b1 k | TRIANGLE v0.xyz e1.xyz e2.xyz
 //   PLANE point.xyz normal.xyz uaxis.xyz vaxis.xyz | TORUS c.xyz R r
 //   SUPERELLIPSOID c.xyz s.xyz e1 e2 | SPRING c.xyz coil tube pitch turns
+//   DISK c.xyz unit-normal.xyz r | CYLINDER, CONE c.xyz r y_min y_max
 template <class R>
 struct alignas(16) PrimT {
   R p[13];
@@ -940,6 +947,86 @@ LMN_HD V3<double> marched_normal(uint32_t kind, const double* p, V3<double> wp) 
 
 // Object-space intersection of one primitive in precision R.  Marched kinds
 // are always FP64 and go through hit_marched_fp64.
+// ---- disk, cylinder, cone (FP64, out of line) ---------------------------------
+// Disk::hit (src/disk.rs:31-59): plane test with a parallel cull, then the
+// radius.  A NaN distance is a miss (the reference would accept it).
+LMN_HD bool hit_disk(const double* p, V3<double> o, V3<double> d, double tmin, double tmax, double& t) {
+  const V3<double> c = {p[0], p[1], p[2]}, n = {p[3], p[4], p[5]};
+  double denom = dot(n, d);
+  if (m_abs(denom) < 1e-8) return false;
+  double tt = dot(c - o, n) / denom;
+  if (!(tt >= tmin && tt <= tmax)) return false;
+  V3<double> off = (o + d * tt) - c;
+  if (length_squared(off) > p[6] * p[6]) return false;
+  t = tt;
+  return true;
+}
+// Cylinder::hit / Cone::hit (src/cylinder.rs:23-73, src/cone.rs:23-78):
+// quadratic in the XZ offset (the cone's radius shrinking to its apex at
+// y_max), near root first, then the far one, each within [tmin, tmax] and
+// [y_min, y_max].  A NaN root (a ray along the axis) is a miss.
+LMN_HD bool hit_cylinder_cone(bool cone, const double* p, V3<double> o, V3<double> d, double tmin, double tmax,
+                              double& t) {
+  const double height = p[5] - p[4];
+  double k = 0.0, oy = 0.0;
+  if (cone) {
+    if (m_abs(height) < 1e-10) return false;
+    k = (p[3] / height) * (p[3] / height);
+    oy = o.y - p[5];
+  }
+  const double ox = o.x - p[0], oz = o.z - p[2];
+  double a, half_b, c;
+  if (cone) {
+    a = d.x * d.x + d.z * d.z - k * d.y * d.y;
+    half_b = ox * d.x + oz * d.z - k * oy * d.y;
+    c = ox * ox + oz * oz - k * oy * oy;
+  } else {
+    a = d.x * d.x + d.z * d.z;
+    half_b = ox * d.x + oz * d.z;
+    c = ox * ox + oz * oz - p[3] * p[3];
+  }
+  double disc = half_b * half_b - a * c;
+  if (disc < 0.0) return false;
+  double sq = m_sqrt(disc);
+  for (int s = -1; s <= 1; s += 2) {
+    double root = (-half_b + double(s) * sq) / a;
+    if (!(root >= tmin && root <= tmax)) continue;
+    double y = o.y + root * d.y;
+    if (y < p[4] || y > p[5]) continue;
+    t = root;
+    return true;
+  }
+  return false;
+}
+LMN_COLD bool hit_quadric_fp64(uint32_t kind, const double* p, V3<double> o, V3<double> d, double tmin, double tmax,
+                               double& t) {
+  if (kind == K_DISK) return hit_disk(p, o, d, tmin, tmax, t);
+  return hit_cylinder_cone(kind == K_CONE, p, o, d, tmin, tmax, t);
+}
+// Outward normal and (u, v) at object-space point q of the hit.
+LMN_COLD void quadric_surface(uint32_t kind, const double* p, V3<double> q, V3<double>& n, double& u, double& v) {
+  const double pi = 3.14159265358979323846;
+  const V3<double> c = {p[0], p[1], p[2]};
+  if (kind == K_DISK) {  // src/disk.rs:48-56
+    V3<double> off = q - c;
+    n = {p[3], p[4], p[5]};
+    u = m_atan2(off.z, off.x) / (2.0 * pi) + 0.5;
+    v = m_sqrt(length_squared(off)) / p[6];
+    return;
+  }
+  const double height = p[5] - p[4];
+  if (kind == K_CYLINDER) {  // src/cylinder.rs:51-64
+    n = {(q.x - c.x) / p[3], 0.0, (q.z - c.z) / p[3]};
+    v = m_abs(height) < 1e-10 ? 0.5 : (q.y - p[4]) / height;
+  } else {  // src/cone.rs:60-72
+    double r_at_y = p[3] * (p[5] - q.y) / height;
+    double rr = fmax_r(r_at_y, 1e-10);
+    n = unit(V3<double>{(q.x - c.x) / rr, p[3] / height, (q.z - c.z) / rr});
+    v = (q.y - p[4]) / height;
+  }
+  u = (m_atan2(-(q.z - c.z), q.x - c.x) + pi) / (2.0 * pi);
+}
+
 template <class R>
 LMN_HD bool hit_prim_analytic(uint32_t kind, const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t) {
   switch (kind) {
@@ -970,7 +1057,8 @@ LMN_COLD bool hit_marched_fp64(uint32_t kind, const double* p, V3<double> o, V3<
 LMN_HD bool hit_prim_fp64(uint32_t kind, const double* p, V3<double> o, V3<double> d, double tmin, double tmax,
                           bool exact, double& t) {
   if (kind == K_SPHERE) return hit_sphere<double>(p, o, d, tmin, tmax, t);
-  if (kind >= K_TORUS) return hit_marched_fp64(kind, p, o, d, tmin, tmax, exact, t);
+  if (is_marched_kind(kind)) return hit_marched_fp64(kind, p, o, d, tmin, tmax, exact, t);
+  if (is_quadric_kind(kind)) return hit_quadric_fp64(kind, p, o, d, tmin, tmax, t);
   return hit_prim_analytic<double>(kind, p, o, d, tmin, tmax, t);
 }
 

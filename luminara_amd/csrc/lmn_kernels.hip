@@ -254,7 +254,12 @@ __global__ __launch_bounds__(BS) void k_march_collect(BatchDev B, PathSoA cur, H
             over = true;
             return;
           }
-          uint32_t k = (S.prims[rec].kind_mat & KIND_MASK) - K_TORUS;
+          const uint32_t kind = S.prims[rec].kind_mat & KIND_MASK;
+          if (is_quadric_kind(kind)) {  // disk, cylinder, cone: the fallback traces the ray whole
+            over = true;
+            return;
+          }
+          uint32_t k = kind - K_TORUS;
           c[nc++] = rec | (k << 30);
           nk0 += k == 0u;
           nk1 += k == 1u;

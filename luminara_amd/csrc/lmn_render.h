@@ -143,7 +143,7 @@ LMN_HD bool prim_is64(uint32_t km) {
     return (km & FLAG_FP64) != 0;
   } else {
     if (!(FEAT & FEAT_MARCH)) return false;
-    return (km & KIND_MASK) >= K_TORUS;
+    return (km & KIND_MASK) >= K_TORUS;  // marched kinds and quadrics (second BVH)
   }
 }
 
@@ -173,7 +173,7 @@ LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t
     double t64;
     if constexpr ((FEAT & FEAT_MARCH) != 0)
       hit = hit_prim_fp64(kind, S.prims64[idx].p, o64, d64, double(tmin), double(closest), S.exact_marched != 0, t64);
-    else  // without marched kinds only spheres are FP64 records
+    else  // the main BVH's FP64 records are spheres (the second BVH holds the rest)
       hit = hit_sphere<double>(S.prims64[idx].p, o64, d64, double(tmin), double(closest), t64);
     t = R(t64);
     if (hit && !(t <= closest)) hit = false;  // rounding back to R must not leave the range
@@ -545,9 +545,13 @@ LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_r
     const double* p64 = S.prims64[idx].p;
     V3<double> q = oo64 + dd64 * double(t_true);
     double u64, v64;
-    if ((FEAT & FEAT_MARCH) && kind >= K_TORUS) {
+    if ((FEAT & FEAT_MARCH) && is_marched_kind(kind)) {
       n = cvt<R>(marched_normal_fp64(kind, p64, q, S.exact_marched != 0));
       marched_uv(kind, p64, q, u64, v64);
+    } else if (is_quadric_kind(kind)) {
+      V3<double> n64;
+      quadric_surface(kind, p64, q, n64, u64, v64);
+      n = cvt<R>(n64);
     } else {  // FP64 sphere: only the centre offset needs FP64, the map does not
       V3<R> nn = cvt<R>(prim_normal_analytic<double>(kind, p64, q));
       n = nn;

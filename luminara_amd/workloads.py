@@ -91,6 +91,44 @@ def cornell_box(width=256, height=256, spp=64, depth=8, seed=42):
     return "\n".join(L)
 
 
+def quadrics_scene(width=64, height=48, spp=16, depth=8, seed=42):
+    """Disks, cylinders and cones (src/disk.rs, src/cylinder.rs, src/cone.rs)
+    on a ground quad: rotated and translated cones, an emissive disk that the
+    loader registers as an NEE light, a tilted disk, spheres between them."""
+    rng = SeededRng(seed, 9)
+    L = _header(width, height, spp, depth, seed, '{ type = "sky" }',
+                {"look_from": [0.0, 3.0, 9.0], "look_at": [0.0, 0.8, 0.0], "vfov": 40, "aperture": 0.0,
+                 "focus_dist": 1.0})
+    L += ["[[quad]]", "q = [-6.0, 0.0, -6.0]", "u = [12.0, 0.0, 0.0]", "v = [0.0, 0.0, 12.0]",
+          'material = { type = "lambertian", color = [0.6, 0.6, 0.6] }', ""]
+    for i in range(6):
+        c = [-3.0 + 1.2 * i, 0.0, -1.0 + 0.8 * rng.uniform()]
+        col = [0.2 + 0.7 * rng.uniform() for _ in range(3)]
+        L += ["[[sphere]]", f"center = {_v([c[0], 0.3, c[2] + 1.6])}", "radius = 0.3",
+              f"material = {_mat({'type': 'lambertian', 'color': col})}", ""]
+    for i in range(4):
+        c = [-2.4 + 1.6 * i, 0.0, -0.5 * rng.uniform()]
+        col = [0.2 + 0.7 * rng.uniform() for _ in range(3)]
+        L += ["[[cone]]", f"center = {_v(c)}", f"radius = {0.35 + 0.2 * rng.uniform()!r}",
+              f"height = {0.8 + 0.8 * rng.uniform()!r}", f"material = {_mat({'type': 'lambertian', 'color': col})}"]
+        if i % 2:
+            L += [f"rotate_y = {20.0 * i!r}", f"translate = {_v([0.1 * i, 0.0, 0.2])}"]
+        L += [""]
+    L += ["[[disk]]", "center = [0.0, 3.5, 0.5]", "normal = [0.0, -1.0, 0.0]", "radius = 0.8",
+          'material = { type = "emissive", color = [1.0, 0.9, 0.8], intensity = 6.0 }', ""]
+    L += ["[[disk]]", "center = [2.5, 0.9, -1.2]", "normal = [-0.4, 0.3, 1.0]", "radius = 0.7",
+          'material = { type = "metal", color = [0.8, 0.8, 0.9], fuzz = 0.1 }', ""]
+    for i in range(3):
+        c = [-2.0 + 2.0 * i, 0.0, -2.2]
+        col = [0.2 + 0.7 * rng.uniform() for _ in range(3)]
+        L += ["[[cylinder]]", f"center = {_v(c)}", f"radius = {0.3 + 0.2 * rng.uniform()!r}",
+              f"height = {1.0 + rng.uniform()!r}", f"material = {_mat({'type': 'lambertian', 'color': col})}"]
+        if i == 1:
+            L += ["rotate_z = 12.0", "translate = [0.0, 0.1, 0.0]"]
+        L += [""]
+    return "\n".join(L)
+
+
 def random_spheres(width=1920, height=1080, spp=128, depth=50, seed=42, grid=11):
     """Random-sphere scene: ground sphere r=1000 (albedo 0.5), a grid of
     r=0.2 spheres at integer x,z in [-grid, grid) with U[0,0.9] jitter

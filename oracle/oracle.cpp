@@ -463,6 +463,102 @@ struct AxisRect : Hittable {  // rect.rs:24-125
   }
 };
 
+struct Disk : Hittable {  // disk.rs:8-66
+  Vec3 c, n;  // n: unit, (0, 1, 0) when near zero (Disk::new)
+  double r;
+  const Material* m;
+  int64_t id;
+  bool hit(const Ray& ray, double tmin, double tmax, HitRecord& h) const override {
+    double denom = n.dot(ray.d);
+    if (std::fabs(denom) < 1e-8) return false;
+    double t = (c - ray.o).dot(n) / denom;
+    if (t < tmin || t > tmax) return false;
+    Vec3 point = ray.at(t);
+    Vec3 offset = point - c;
+    if (offset.length_squared() > r * r) return false;
+    double rr = offset.length() / r;
+    double theta = std::atan2(offset.z, offset.x);
+    h.u = theta / (2.0 * M_PI) + 0.5;
+    h.v = rr;
+    h.t = t;
+    h.p = point;
+    h.mat = m;
+    h.prim = id;
+    set_face(h, ray, n);
+    return true;
+  }
+  bool bbox(Aabb& b) const override {
+    Vec3 e(r, r, r);
+    b = {c - e, c + e};
+    return true;
+  }
+};
+
+// Cylinder (cylinder.rs:8-81) and Cone (cone.rs:8-86): Y-axis aligned,
+// y in [y_min, y_max]; the cone's apex at y_max.
+struct AxialQuadric : Hittable {
+  bool cone;
+  Vec3 c;
+  double radius, y_min, y_max;
+  const Material* m;
+  int64_t id;
+  bool hit(const Ray& ray, double tmin, double tmax, HitRecord& h) const override {
+    double height = y_max - y_min;
+    double k = 0.0, oy = 0.0;
+    if (cone) {
+      if (std::fabs(height) < 1e-10) return false;
+      k = (radius / height) * (radius / height);
+      oy = ray.o.y - y_max;
+    }
+    double dx = ray.d.x, dz = ray.d.z, dy = ray.d.y;
+    double ox = ray.o.x - c.x, oz = ray.o.z - c.z;
+    double a, half_b, cc;
+    if (cone) {
+      a = dx * dx + dz * dz - k * dy * dy;
+      half_b = ox * dx + oz * dz - k * oy * dy;
+      cc = ox * ox + oz * oz - k * oy * oy;
+    } else {
+      a = dx * dx + dz * dz;
+      half_b = ox * dx + oz * dz;
+      cc = ox * ox + oz * oz - radius * radius;
+    }
+    double disc = half_b * half_b - a * cc;
+    if (disc < 0.0) return false;
+    double sq = std::sqrt(disc);
+    for (double sign : {-1.0, 1.0}) {
+      double root = (-half_b + sign * sq) / a;
+      if (root < tmin || root > tmax) continue;
+      double y = ray.o.y + root * ray.d.y;
+      if (y < y_min || y > y_max) continue;
+      Vec3 point = ray.at(root);
+      Vec3 outward;
+      if (cone) {
+        double r_at_y = radius * (y_max - y) / height;
+        double nx = (point.x - c.x) / std::fmax(r_at_y, 1e-10);
+        double nz = (point.z - c.z) / std::fmax(r_at_y, 1e-10);
+        outward = Vec3(nx, radius / height, nz).unit();
+        h.v = (y - y_min) / height;
+      } else {
+        outward = Vec3((point.x - c.x) / radius, 0.0, (point.z - c.z) / radius);
+        h.v = std::fabs(height) < 1e-10 ? 0.5 : (y - y_min) / height;
+      }
+      double theta = std::atan2(-(point.z - c.z), point.x - c.x) + M_PI;
+      h.u = theta / (2.0 * M_PI);
+      h.t = root;
+      h.p = point;
+      h.mat = m;
+      h.prim = id;
+      set_face(h, ray, outward);
+      return true;
+    }
+    return false;
+  }
+  bool bbox(Aabb& b) const override {
+    b = {Vec3(c.x - radius, y_min, c.z - radius), Vec3(c.x + radius, y_max, c.z + radius)};
+    return true;
+  }
+};
+
 struct Triangle : Hittable {  // triangle.rs:23-70 / obj.rs:260-319
   Vec3 v0, v1, v2;
   const Material* m;
@@ -1440,6 +1536,28 @@ HPtr make_leaf(const lmn_scene_desc& d, uint32_t i, const World& w, bool exact) 
       s->id = id;
       s->exact = exact;
       return s;
+    }
+    case LMN_PRIM_DISK: {
+      auto dk = std::make_unique<Disk>();
+      dk->c = v3(q);
+      Vec3 n = v3(q + 3);
+      dk->n = (std::fabs(n.x) < 1e-8 && std::fabs(n.y) < 1e-8 && std::fabs(n.z) < 1e-8) ? Vec3(0, 1, 0) : n.unit();
+      dk->r = q[6];
+      dk->m = m;
+      dk->id = id;
+      return dk;
+    }
+    case LMN_PRIM_CYLINDER:
+    case LMN_PRIM_CONE: {
+      auto a = std::make_unique<AxialQuadric>();
+      a->cone = p.kind == LMN_PRIM_CONE;
+      a->c = v3(q);
+      a->radius = q[3];
+      a->y_min = q[4];
+      a->y_max = q[5];
+      a->m = m;
+      a->id = id;
+      return a;
     }
     default: {
       auto s = std::make_unique<Spring>();

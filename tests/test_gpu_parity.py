@@ -64,6 +64,7 @@ def compare_hits(rays, ho, hg, fp32):
 SCENES = {
     "cornell": (lambda: workloads.cornell_box(32, 32, 4, 8), [5, 5, -600], [550, 550, 550]),
     "spheres": (lambda: workloads.random_spheres(32, 18, 4, 50, grid=11), [-12, 0.01, -12], [12, 3, 12]),
+    "quadrics": (lambda: workloads.quadrics_scene(32, 24, 4, 8), [-4, 0.05, -3], [4, 3, 3]),
 }
 
 
@@ -96,7 +97,7 @@ def render_pair(text, **over):
     return cfg, cam, world, ref, var, ost
 
 
-@pytest.mark.parametrize("scene", ["cornell", "spheres"])
+@pytest.mark.parametrize("scene", ["cornell", "spheres", "quadrics"])
 def test_fp64_kernel_matches_oracle(gpu, scene):
     text = SCENES[scene][0]()
     cfg, cam, world, ref, var, ost = render_pair(text, width=48, height=32, samples_per_pixel=9)
@@ -108,7 +109,7 @@ def test_fp64_kernel_matches_oracle(gpu, scene):
     assert r.stats["shadow_rays"] == ost["shadow_rays"]
 
 
-@pytest.mark.parametrize("scene", ["cornell", "spheres"])
+@pytest.mark.parametrize("scene", ["cornell", "spheres", "quadrics"])
 def test_fp32_wavefront_statistical_parity(gpu, scene):
     text = SCENES[scene][0]()
     cfg, cam, world, ref, var, ost = render_pair(text, width=64, height=48, samples_per_pixel=16)
