@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread"]
 
 
 def _run(cmd):
@@ -58,7 +58,7 @@ def build(verbose=False):
                 print(out)
     objs = [j[-1] for j in jobs]
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "cli", "luminara_gpu.cpp"), "-o", CLI,
           f"-L{OUT_DIR}", "-llmn", "-lz", "-Wl,-rpath,$ORIGIN"])

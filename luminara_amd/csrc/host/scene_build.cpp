@@ -2,9 +2,13 @@
 #include "scene_build.h"
 
 #include <algorithm>
+#include <atomic>
+#include <memory>
+#include <thread>
 #include <functional>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 
@@ -211,6 +215,17 @@ struct Builder {
   BvhOptions opt;
   uint32_t depth_max = 0, leaves = 0;
 
+  // Subtrees reached at depth `split_depth` with at least `par_min`
+  // primitives are deferred to `tasks` and built on worker threads.
+  struct Task {
+    uint32_t id;
+    size_t b, e;
+    uint32_t depth;
+  };
+  std::vector<Task>* tasks = nullptr;
+  uint32_t split_depth = 0;
+  size_t par_min = 0;
+
   Builder(const std::vector<double>& lo, const std::vector<double>& hi) : blo(lo), bhi(hi) {}
 
   void bounds_of(size_t b, size_t e, double lo[3], double hi[3], double clo[3], double chi[3]) {
@@ -239,6 +254,10 @@ struct Builder {
   // Children are allocated as adjacent pairs at even indices (root at 0,
   // index 1 unused), so a pair of FP32 nodes is one aligned 64-byte block.
   void build(uint32_t id, size_t b, size_t e, uint32_t depth) {
+    if (tasks && depth == split_depth && e - b >= par_min) {
+      tasks->push_back({id, b, e, depth});
+      return;
+    }
     double lo[3], hi[3], clo[3], chi[3];
     bounds_of(b, e, lo, hi, clo, chi);
     for (int k = 0; k < 3; ++k) {
@@ -348,6 +367,60 @@ struct Builder {
     nodes[id].b = uint32_t(best_axis < 0 ? 0 : best_axis);
     build(c, b, mid, depth + 1);
     build(c + 1, mid, e, depth + 1);
+  }
+
+  // The same tree as build(0, 0, n, 0): the top levels on this thread, the
+  // subtrees below them on worker threads (each in a builder of its own over
+  // a copy of its references, splitting exactly as here), then merged in
+  // depth-first order.  Node numbering differs from the one-thread build;
+  // the tree does not.
+  void build_parallel(size_t n) {
+    unsigned nt = std::min(64u, std::max(1u, std::thread::hardware_concurrency()));
+    if (const char* env = std::getenv("LMN_BUILD_THREADS")) nt = unsigned(std::max(1, std::atoi(env)));
+    if (n < 65536 || nt < 2) {
+      build(0, 0, n, 0);
+      return;
+    }
+    std::vector<Task> T;
+    tasks = &T;
+    split_depth = 7;
+    par_min = 1024;
+    build(0, 0, n, 0);
+    tasks = nullptr;
+    std::vector<std::unique_ptr<Builder>> sub(T.size());
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+      for (;;) {
+        size_t k = next.fetch_add(1);
+        if (k >= T.size()) break;
+        sub[k] = std::make_unique<Builder>(blo, bhi);
+        Builder& S = *sub[k];
+        S.opt = opt;
+        S.refs.assign(refs.begin() + T[k].b, refs.begin() + T[k].e);
+        S.nodes.resize(2);
+        S.nodes[1].b = LEAF_BIT;
+        S.build(0, 0, S.refs.size(), T[k].depth);
+      }
+    };
+    std::vector<std::thread> th;
+    for (unsigned k = 0; k < std::min<size_t>(nt, T.size()); ++k) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+    for (size_t k = 0; k < T.size(); ++k) {
+      const Builder& S = *sub[k];
+      const uint32_t G = uint32_t(nodes.size()), O = uint32_t(order.size());
+      auto moved = [&](NodeD x) {
+        if (x.b & LEAF_BIT)
+          x.a += O;
+        else
+          x.a = G + (x.a - 2);
+        return x;
+      };
+      nodes[T[k].id] = moved(S.nodes[0]);
+      for (size_t i = 2; i < S.nodes.size(); ++i) nodes.push_back(moved(S.nodes[i]));
+      order.insert(order.end(), S.order.begin(), S.order.end());
+      depth_max = std::max(depth_max, S.depth_max);
+      leaves += S.leaves;
+    }
   }
 };
 
@@ -724,7 +797,7 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
     if (!ids.empty()) {
       B.nodes.resize(2);  // root + unused slot keeps child pairs 64-byte aligned
       B.nodes[1].b = LEAF_BIT;  // empty leaf, never referenced
-      B.build(0, 0, ids.size(), 0);
+      B.build_parallel(ids.size());
     }
     for (NodeD& n : B.nodes)
       if (n.b & LEAF_BIT) n.a += rec_base;

@@ -178,3 +178,19 @@ def test_pow_pos_accuracy(harness):
     assert err.max() < 4.0
     tiny = ~fin & np.isfinite(ref)
     assert np.all(np.abs(out[tiny] - ref[tiny]) <= 1e-300)
+
+
+def test_parallel_bvh_build_gives_the_same_hits(harness, monkeypatch):
+    """The BVH of a large mesh is built on worker threads below its top
+    levels; the tree, and so every hit, is the one-thread build's."""
+    cfg, cam, world = lm.load_scene(workloads.batch_scene(3, seed=5)[0])
+    v, f = workloads.displaced_sphere(64, [0.0, 1.2, 0.0], 1.0, 77)   # 81,920 triangles
+    world.add_triangles(v, f, material=world.add_material(workloads._microfacet([0.7, 0.6, 0.5], 0.3)))
+    rays = random_rays(20000, [-2.5, 0.05, -2.5], [2.5, 3.0, 3.0], seed=11)
+    monkeypatch.setenv("LMN_BUILD_THREADS", "1")
+    one, _ = host_trace(harness, world, rays)
+    monkeypatch.setenv("LMN_BUILD_THREADS", "8")
+    many, _ = host_trace(harness, world, rays)
+    assert (one["prim"] >= 0).sum() > 2000
+    assert np.array_equal(one["prim"], many["prim"]) and np.array_equal(one["t"], many["t"])
+    assert np.array_equal(one["normal"], many["normal"])
