@@ -19,9 +19,9 @@ LMN_ERR_DEVICE = 5
 LMN_ERR_NOMEM = 6
 
 PRIM_KINDS = ["sphere", "quad", "rect_xy", "rect_xz", "rect_yz", "triangle", "plane", "torus",
-              "superellipsoid", "spring"]
+              "superellipsoid", "spring", "disk", "cylinder", "cone"]
 (PRIM_SPHERE, PRIM_QUAD, PRIM_RECT_XY, PRIM_RECT_XZ, PRIM_RECT_YZ, PRIM_TRIANGLE, PRIM_PLANE,
- PRIM_TORUS, PRIM_SUPERELLIPSOID, PRIM_SPRING) = range(10)
+ PRIM_TORUS, PRIM_SUPERELLIPSOID, PRIM_SPRING, PRIM_DISK, PRIM_CYLINDER, PRIM_CONE) = range(13)
 MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_EMISSIVE, MAT_MICROFACET = range(5)
 LIGHT_SPHERE, LIGHT_RECT, LIGHT_DISK = range(3)
 BG_SKY, BG_SOLID, BG_GRADIENT, BG_SUN = range(4)

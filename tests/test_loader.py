@@ -92,9 +92,14 @@ def test_geometry_outside_the_path_is_rejected_by_name():  # src/scene.rs:2304-2
     _, _, world = lm.load_scene(base)
     assert len(world.prims) == 1
     with pytest.raises(_abi.LuminaraError) as e:
-        lm.load_scene(base + '[[cylinder]]\ncenter = [0.0, 0.0, 0.0]\nradius = 0.5\nheight = 1.0\n'
+        lm.load_scene(base + '[[annulus]]\ncenter = [0.0, 0.0, 0.0]\nnormal = [0.0, 1.0, 0.0]\n'
+                             'inner_radius = 0.2\nouter_radius = 0.5\n'
                              'material = { type = "metal", color = [0.5, 0.5, 0.5] }\n')
-    assert e.value.code == _abi.LMN_ERR_UNSUPPORTED and "cylinder" in str(e.value)
+    assert e.value.code == _abi.LMN_ERR_UNSUPPORTED and "annulus" in str(e.value)
+    # cylinders, cones and disks are on the path, after spheres in load order
+    _, _, world = lm.load_scene(base + '[[cylinder]]\ncenter = [0.0, 0.0, 0.0]\nradius = 0.5\nheight = 1.0\n'
+                                       'material = { type = "metal", color = [0.5, 0.5, 0.5] }\n')
+    assert [p.kind for p in world.prims] == [_abi.PRIM_SPHERE, _abi.PRIM_CYLINDER]
 
 
 def test_defaults_match_render_config_default():  # src/render.rs:422-431
