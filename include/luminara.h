@@ -56,7 +56,13 @@ extern "C" {
  *   SPRING          center[3], coil_r, tube_r, pitch, turns src/spring.rs:35
  *   DISK            center[3], normal[3] (as given), radius src/disk.rs:16
  *   CYLINDER        center[3], radius, y_min, y_max         src/cylinder.rs:17
- *   CONE            center[3], radius, y_min, y_max (apex)  src/cone.rs:18     */
+ *   CONE            center[3], radius, y_min, y_max (apex)  src/cone.rs:18
+ *   ELLIPSOID       center[3], radii[3]                     src/ellipsoid.rs:14
+ *   HEMISPHERE      center[3], radius (the half y >= c.y)   src/hemisphere.rs:14
+ *   ANNULUS         center[3], normal[3] (as given), inner_r, outer_r
+ *                                                           src/annulus.rs:17
+ * A capsule (src/capsule.rs:15-53) is three prims of one object: CYLINDER
+ * (y from center.y to center.y + height), then the bottom and top SPHERE. */
 enum {
   LMN_PRIM_SPHERE = 0,
   LMN_PRIM_QUAD = 1,
@@ -71,7 +77,10 @@ enum {
   LMN_PRIM_DISK = 10,
   LMN_PRIM_CYLINDER = 11,
   LMN_PRIM_CONE = 12,
-  LMN_PRIM_KIND_COUNT = 13
+  LMN_PRIM_ELLIPSOID = 13,
+  LMN_PRIM_HEMISPHERE = 14,
+  LMN_PRIM_ANNULUS = 15,
+  LMN_PRIM_KIND_COUNT = 16
 };
 
 typedef struct lmn_prim {

@@ -170,6 +170,16 @@ static void object_bounds(const lmn_prim& p, double lo[3], double hi[3]) {
       lo[1] = q[4]; hi[1] = q[5];
       lo[2] = q[2] - q[3]; hi[2] = q[2] + q[3];
       break;
+    case LMN_PRIM_ELLIPSOID:  // src/ellipsoid.rs:69-71: centre +/- radii
+      for (int k = 0; k < 3; ++k) { lo[k] = q[k] - q[3 + k]; hi[k] = q[k] + q[3 + k]; }
+      break;
+    case LMN_PRIM_HEMISPHERE:  // src/hemisphere.rs:56-61: the upper half only
+      for (int k = 0; k < 3; ++k) { lo[k] = q[k] - q[3]; hi[k] = q[k] + q[3]; }
+      lo[1] = q[1];
+      break;
+    case LMN_PRIM_ANNULUS:  // src/annulus.rs:78-81: centre +/- outer radius
+      for (int k = 0; k < 3; ++k) { lo[k] = q[k] - q[7]; hi[k] = q[k] + q[7]; }
+      break;
     default:
       for (int k = 0; k < 3; ++k) { lo[k] = -INFINITY; hi[k] = INFINITY; }
   }
@@ -629,7 +639,8 @@ void fill_record(const lmn_prim& p, const XformT<double>* x, PrimT<R>& r, uint32
       }
       break;
     }
-    case LMN_PRIM_DISK: {  // Disk::new (src/disk.rs:16-28): unit normal, (0, 1, 0) when near zero
+    case LMN_PRIM_DISK:       // Disk::new (src/disk.rs:16-28) and Annulus::new
+    case LMN_PRIM_ANNULUS: {  // (src/annulus.rs:17-36): unit normal, (0, 1, 0) when near zero
       double n[3] = {q[3], q[4], q[5]};
       if (std::fabs(n[0]) < 1e-8 && std::fabs(n[1]) < 1e-8 && std::fabs(n[2]) < 1e-8) {
         n[0] = 0; n[1] = 1; n[2] = 0;
@@ -642,6 +653,7 @@ void fill_record(const lmn_prim& p, const XformT<double>* x, PrimT<R>& r, uint32
         v[3 + k] = n[k];
       }
       v[6] = q[6];
+      v[7] = q[7];
       break;
     }
     default:
@@ -660,7 +672,7 @@ void fill_record(const lmn_prim& p, const XformT<double>* x, PrimT<R>& r, uint32
 // hit-distance tolerance (e.g. a r = 1000 ground sphere).
 bool fp64_island(const lmn_prim& p) {
   if (p.kind == LMN_PRIM_TORUS || p.kind == LMN_PRIM_SUPERELLIPSOID || p.kind == LMN_PRIM_SPRING) return true;
-  if (p.kind >= LMN_PRIM_DISK) return true;  // disk, cylinder, cone: second BVH, FP64
+  if (p.kind >= LMN_PRIM_DISK) return true;  // disk ... annulus: second BVH, FP64
   if (p.kind == LMN_PRIM_SPHERE) return std::fabs(p.p[3]) >= 128.0;
   return false;
 }

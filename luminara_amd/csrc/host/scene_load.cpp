@@ -557,8 +557,40 @@ class Loader {
       p.p[6] = e1;
       p.p[7] = e2;
     }
-    reject_if_present(root, "capsule", "(capsule)");
-    reject_if_present(root, "ellipsoid", "(ellipsoid)");
+    for (auto& s : table_array(root, "capsule")) {  // src/scene.rs:1308-1316, src/capsule.rs:15-53
+      const std::string ctx = "capsule";
+      double c[3];
+      req_vec3(s, "center", ctx, c);
+      double r = req_num(s, "radius", ctx), h = req_num(s, "height", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      // one world-list entry: the cylinder body, then the bottom and top spheres
+      uint32_t obj = new_object();
+      lmn_prim& body = add_prim(LMN_PRIM_CYLINDER, mat, -1, obj);
+      for (int k = 0; k < 3; ++k) body.p[k] = c[k];
+      body.p[3] = r;
+      body.p[4] = c[1];
+      body.p[5] = c[1] + h;
+      for (int cap = 0; cap < 2; ++cap) {
+        lmn_prim& sp = add_prim(LMN_PRIM_SPHERE, mat, -1, obj);
+        sp.p[0] = c[0];
+        sp.p[1] = cap ? c[1] + h : c[1];
+        sp.p[2] = c[2];
+        sp.p[3] = r;
+      }
+    }
+    for (auto& s : table_array(root, "ellipsoid")) {  // src/scene.rs:1318-1327
+      const std::string ctx = "ellipsoid";
+      double c[3], radii[3];
+      req_vec3(s, "center", ctx, c);
+      req_vec3(s, "radii", ctx, radii);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      if (s.get("bump_strength")) fail(LMN_ERR_UNSUPPORTED, "ellipsoid bump maps are not on the GPU path");
+      lmn_prim& p = add_prim(LMN_PRIM_ELLIPSOID, mat, -1, new_object());
+      for (int k = 0; k < 3; ++k) {
+        p.p[k] = c[k];
+        p.p[3 + k] = radii[k];
+      }
+    }
     for (auto& s : table_array(root, "light")) {  // src/scene.rs:1329-1344
       const std::string ctx = "light";
       double pos[3], col[3] = {1, 1, 1};
@@ -579,7 +611,16 @@ class Loader {
       double e[3] = {col[0] * intensity, col[1] * intensity, col[2] * intensity};
       push_light_sphere(pos, radius, e, uint32_t(out_.materials.size() - 1));
     }
-    reject_if_present(root, "hemisphere", "(hemisphere)");
+    for (auto& s : table_array(root, "hemisphere")) {  // src/scene.rs:1346-1349
+      const std::string ctx = "hemisphere";
+      double c[3];
+      req_vec3(s, "center", ctx, c);
+      double r = req_num(s, "radius", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      lmn_prim& p = add_prim(LMN_PRIM_HEMISPHERE, mat, -1, new_object());
+      for (int k = 0; k < 3; ++k) p.p[k] = c[k];
+      p.p[3] = r;
+    }
     reject_if_present(root, "fog", "(constant medium)");
     // cones and cylinders: y from center.y to center.y + height, then
     // RotateX/Y/Z and Translate wrappers (src/scene.rs:1364-1375, :1422-1456)
@@ -605,7 +646,21 @@ class Loader {
       p.p[5] = c[1] + h;
     };
     for (auto& s : table_array(root, "cone")) axial(s, "cone", LMN_PRIM_CONE);
-    reject_if_present(root, "annulus", "(annulus)");
+    for (auto& s : table_array(root, "annulus")) {  // src/scene.rs:1377-1386
+      const std::string ctx = "annulus";
+      double c[3], n[3];
+      req_vec3(s, "center", ctx, c);
+      req_vec3(s, "normal", ctx, n);
+      double ri = req_num(s, "inner_radius", ctx), ro = req_num(s, "outer_radius", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      lmn_prim& p = add_prim(LMN_PRIM_ANNULUS, mat, -1, new_object());
+      for (int k = 0; k < 3; ++k) {
+        p.p[k] = c[k];
+        p.p[3 + k] = n[k];
+      }
+      p.p[6] = ri;
+      p.p[7] = ro;
+    }
     for (auto& s : table_array(root, "disk")) {  // src/scene.rs:1388-1420
       const std::string ctx = "disk";
       double c[3], n[3];

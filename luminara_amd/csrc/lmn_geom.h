@@ -23,11 +23,13 @@ namespace lmn {
 
 enum : uint32_t {
   K_SPHERE = 0, K_QUAD = 1, K_RECT_XY = 2, K_RECT_XZ = 3, K_RECT_YZ = 4, K_TRIANGLE = 5,
-  K_PLANE = 6, K_TORUS = 7, K_SUPERELLIPSOID = 8, K_SPRING = 9, K_DISK = 10, K_CYLINDER = 11, K_CONE = 12
+  K_PLANE = 6, K_TORUS = 7, K_SUPERELLIPSOID = 8, K_SPRING = 9, K_DISK = 10, K_CYLINDER = 11, K_CONE = 12,
+  K_ELLIPSOID = 13, K_HEMISPHERE = 14, K_ANNULUS = 15
 };
 // Torus, superellipsoid, spring: the marched kinds (own BVH, FP64 roots).
 LMN_HD bool is_marched_kind(uint32_t k) { return k >= K_TORUS && k <= K_SPRING; }
-// Disk, cylinder, cone: rare analytic kinds kept in the marched kinds' BVH
+// Disk, cylinder, cone, ellipsoid, hemisphere, annulus: rare analytic kinds
+// kept in the marched kinds' BVH
 // and evaluated there in FP64, out of line, so the main traversal carries
 // none of their code.
 LMN_HD bool is_quadric_kind(uint32_t k) { return k >= K_DISK; }
@@ -42,6 +44,7 @@ constexpr uint32_t XF_NONE = 0xffffffffu;
 //   PLANE point.xyz normal.xyz uaxis.xyz vaxis.xyz | TORUS c.xyz R r
 //   SUPERELLIPSOID c.xyz s.xyz e1 e2 | SPRING c.xyz coil tube pitch turns
 //   DISK c.xyz unit-normal.xyz r | CYLINDER, CONE c.xyz r y_min y_max
+//   ELLIPSOID c.xyz radii.xyz | HEMISPHERE c.xyz r | ANNULUS c.xyz unit-normal.xyz r_in r_out
 template <class R>
 struct alignas(16) PrimT {
   R p[13];
@@ -998,10 +1001,63 @@ LMN_HD bool hit_cylinder_cone(bool cone, const double* p, V3<double> o, V3<doubl
   }
   return false;
 }
+// Ellipsoid::hit (src/ellipsoid.rs:21-67): the unit-sphere quadratic of the
+// ray scaled by 1/radii, near root first, then the far one.
+LMN_HD bool hit_ellipsoid(const double* p, V3<double> o, V3<double> d, double tmin, double tmax, double& t) {
+  const V3<double> inv = {1.0 / p[3], 1.0 / p[4], 1.0 / p[5]};
+  const V3<double> oc = {(o.x - p[0]) * inv.x, (o.y - p[1]) * inv.y, (o.z - p[2]) * inv.z};
+  const V3<double> dd = {d.x * inv.x, d.y * inv.y, d.z * inv.z};
+  double a = length_squared(dd), half_b = dot(oc, dd), c = length_squared(oc) - 1.0;
+  double disc = half_b * half_b - a * c;
+  if (disc < 0.0) return false;
+  double sq = m_sqrt(disc);
+  double root = (-half_b - sq) / a;
+  if (!(root >= tmin && root <= tmax)) {
+    root = (-half_b + sq) / a;
+    if (!(root >= tmin && root <= tmax)) return false;
+  }
+  t = root;
+  return true;
+}
+// Hemisphere::hit (src/hemisphere.rs:21-54): the sphere's roots in order,
+// each kept only in range and on the upper half (y >= c.y).
+LMN_HD bool hit_hemisphere(const double* p, V3<double> o, V3<double> d, double tmin, double tmax, double& t) {
+  const V3<double> oc = o - V3<double>{p[0], p[1], p[2]};
+  double a = length_squared(d), half_b = dot(oc, d), c = length_squared(oc) - p[3] * p[3];
+  double disc = half_b * half_b - a * c;
+  if (disc < 0.0) return false;
+  double sq = m_sqrt(disc);
+  for (int s = -1; s <= 1; s += 2) {
+    double root = (-half_b + double(s) * sq) / a;
+    if (!(root >= tmin && root <= tmax)) continue;
+    if (o.y + root * d.y < p[1]) continue;
+    t = root;
+    return true;
+  }
+  return false;
+}
+// Annulus::hit (src/annulus.rs:39-76): plane test with a parallel cull,
+// then inner_r^2 <= |offset|^2 <= outer_r^2.
+LMN_HD bool hit_annulus(const double* p, V3<double> o, V3<double> d, double tmin, double tmax, double& t) {
+  const V3<double> c = {p[0], p[1], p[2]}, n = {p[3], p[4], p[5]};
+  double denom = dot(n, d);
+  if (m_abs(denom) < 1e-8) return false;
+  double tt = dot(c - o, n) / denom;
+  if (!(tt >= tmin && tt <= tmax)) return false;
+  double dist_sq = length_squared((o + d * tt) - c);
+  if (dist_sq < p[6] * p[6] || dist_sq > p[7] * p[7]) return false;
+  t = tt;
+  return true;
+}
 LMN_COLD bool hit_quadric_fp64(uint32_t kind, const double* p, V3<double> o, V3<double> d, double tmin, double tmax,
                                double& t) {
-  if (kind == K_DISK) return hit_disk(p, o, d, tmin, tmax, t);
-  return hit_cylinder_cone(kind == K_CONE, p, o, d, tmin, tmax, t);
+  switch (kind) {
+    case K_DISK: return hit_disk(p, o, d, tmin, tmax, t);
+    case K_ELLIPSOID: return hit_ellipsoid(p, o, d, tmin, tmax, t);
+    case K_HEMISPHERE: return hit_hemisphere(p, o, d, tmin, tmax, t);
+    case K_ANNULUS: return hit_annulus(p, o, d, tmin, tmax, t);
+    default: return hit_cylinder_cone(kind == K_CONE, p, o, d, tmin, tmax, t);
+  }
 }
 // Outward normal and (u, v) at object-space point q of the hit.
 LMN_COLD void quadric_surface(uint32_t kind, const double* p, V3<double> q, V3<double>& n, double& u, double& v) {
@@ -1012,6 +1068,22 @@ LMN_COLD void quadric_surface(uint32_t kind, const double* p, V3<double> q, V3<d
     n = {p[3], p[4], p[5]};
     u = m_atan2(off.z, off.x) / (2.0 * pi) + 0.5;
     v = m_sqrt(length_squared(off)) / p[6];
+    return;
+  }
+  if (kind == K_ANNULUS) {  // src/annulus.rs:61-74: u angular, v radial
+    V3<double> off = q - c;
+    n = {p[3], p[4], p[5]};
+    u = m_atan2(off.z, off.x) / (2.0 * pi) + 0.5;
+    v = (m_sqrt(length_squared(off)) - p[6]) / (p[7] - p[6]);
+    return;
+  }
+  if (kind == K_ELLIPSOID || kind == K_HEMISPHERE) {  // src/ellipsoid.rs:56-64, src/hemisphere.rs:41-46
+    if (kind == K_ELLIPSOID)
+      n = unit(V3<double>{(q.x - c.x) / (p[3] * p[3]), (q.y - c.y) / (p[4] * p[4]), (q.z - c.z) / (p[5] * p[5])});
+    else
+      n = (q - c) / p[3];
+    u = (m_atan2(-n.z, n.x) + pi) / (2.0 * pi);
+    v = m_acos(-n.y) / pi;
     return;
   }
   const double height = p[5] - p[4];

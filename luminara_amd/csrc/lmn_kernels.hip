@@ -255,7 +255,7 @@ __global__ __launch_bounds__(BS) void k_march_collect(BatchDev B, PathSoA cur, H
             return;
           }
           const uint32_t kind = S.prims[rec].kind_mat & KIND_MASK;
-          if (is_quadric_kind(kind)) {  // disk, cylinder, cone: the fallback traces the ray whole
+          if (is_quadric_kind(kind)) {  // disk ... annulus: the fallback traces the ray whole
             over = true;
             return;
           }

@@ -92,9 +92,13 @@ def cornell_box(width=256, height=256, spp=64, depth=8, seed=42):
 
 
 def quadrics_scene(width=64, height=48, spp=16, depth=8, seed=42):
-    """Disks, cylinders and cones (src/disk.rs, src/cylinder.rs, src/cone.rs)
-    on a ground quad: rotated and translated cones, an emissive disk that the
-    loader registers as an NEE light, a tilted disk, spheres between them."""
+    """Disks, cylinders, cones, an ellipsoid, a hemisphere, an annulus and a
+    capsule (src/disk.rs, src/cylinder.rs, src/cone.rs, src/ellipsoid.rs,
+    src/hemisphere.rs, src/annulus.rs, src/capsule.rs) on a ground quad:
+    rotated and translated cones, an emissive disk that the loader registers
+    as an NEE light, a tilted disk, spheres between them, and in the front row
+    the ellipsoid, a raised hemisphere open below, a tilted metal annulus and
+    the capsule."""
     rng = SeededRng(seed, 9)
     L = _header(width, height, spp, depth, seed, '{ type = "sky" }',
                 {"look_from": [0.0, 3.0, 9.0], "look_at": [0.0, 0.8, 0.0], "vfov": 40, "aperture": 0.0,
@@ -126,6 +130,15 @@ def quadrics_scene(width=64, height=48, spp=16, depth=8, seed=42):
         if i == 1:
             L += ["rotate_z = 12.0", "translate = [0.0, 0.1, 0.0]"]
         L += [""]
+    col = [[0.2 + 0.7 * rng.uniform() for _ in range(3)] for _ in range(3)]
+    L += ["[[capsule]]", "center = [2.6, 0.35, 2.0]", "radius = 0.3", "height = 0.9",
+          f"material = {_mat({'type': 'lambertian', 'color': col[0]})}", ""]
+    L += ["[[ellipsoid]]", "center = [-2.5, 0.5, 2.2]", "radii = [0.6, 0.4, 0.3]",
+          f"material = {_mat({'type': 'lambertian', 'color': col[1]})}", ""]
+    L += ["[[hemisphere]]", "center = [-0.8, 0.5, 2.3]", "radius = 0.45",
+          f"material = {_mat({'type': 'lambertian', 'color': col[2]})}", ""]
+    L += ["[[annulus]]", "center = [0.9, 0.8, 2.3]", "normal = [0.3, 1.0, -0.6]", "inner_radius = 0.2",
+          "outer_radius = 0.55", 'material = { type = "metal", color = [0.9, 0.8, 0.6], fuzz = 0.05 }', ""]
     return "\n".join(L)
 
 

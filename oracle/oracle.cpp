@@ -559,6 +559,119 @@ struct AxialQuadric : Hittable {
   }
 };
 
+// Ellipsoid (ellipsoid.rs:8-72): the unit-sphere quadratic after dividing the
+// ray by the radii; the normal is the unit gradient.
+struct Ellipsoid : Hittable {
+  Vec3 c, radii;
+  const Material* m;
+  int64_t id;
+  bool hit(const Ray& ray, double tmin, double tmax, HitRecord& h) const override {
+    Vec3 inv(1.0 / radii.x, 1.0 / radii.y, 1.0 / radii.z);
+    Vec3 oc((ray.o.x - c.x) * inv.x, (ray.o.y - c.y) * inv.y, (ray.o.z - c.z) * inv.z);
+    Vec3 dir(ray.d.x * inv.x, ray.d.y * inv.y, ray.d.z * inv.z);
+    double a = dir.length_squared();
+    double hb = oc.dot(dir);
+    double cc = oc.length_squared() - 1.0;
+    double disc = hb * hb - a * cc;
+    if (disc < 0.0) return false;
+    double sq = std::sqrt(disc);
+    double root = (-hb - sq) / a;
+    if (root < tmin || root > tmax) {
+      root = (-hb + sq) / a;
+      if (root < tmin || root > tmax) return false;
+    }
+    Vec3 p = ray.at(root);
+    Vec3 on = Vec3((p.x - c.x) / (radii.x * radii.x), (p.y - c.y) / (radii.y * radii.y),
+                   (p.z - c.z) / (radii.z * radii.z)).unit();
+    h.u = (std::atan2(-on.z, on.x) + M_PI) / (2.0 * M_PI);
+    h.v = std::acos(-on.y) / M_PI;
+    h.p = p;
+    h.t = root;
+    h.mat = m;
+    h.prim = id;
+    set_face(h, ray, on);
+    return true;
+  }
+  bool bbox(Aabb& b) const override {
+    b = {c - radii, c + radii};
+    return true;
+  }
+};
+
+// Hemisphere (hemisphere.rs:8-62): the sphere's roots, nearest first, each
+// kept only on the upper half y >= c.y.
+struct Hemisphere : Hittable {
+  Vec3 c;
+  double r;
+  const Material* m;
+  int64_t id;
+  bool hit(const Ray& ray, double tmin, double tmax, HitRecord& h) const override {
+    Vec3 oc = ray.o - c;
+    double a = ray.d.length_squared();
+    double hb = oc.dot(ray.d);
+    double cc = oc.length_squared() - r * r;
+    double disc = hb * hb - a * cc;
+    if (disc < 0.0) return false;
+    double sq = std::sqrt(disc);
+    for (double sign : {-1.0, 1.0}) {
+      double root = (-hb + sign * sq) / a;
+      if (root < tmin || root > tmax) continue;
+      Vec3 p = ray.at(root);
+      if (p.y < c.y) continue;
+      Vec3 on = (p - c) / r;
+      h.u = (std::atan2(-on.z, on.x) + M_PI) / (2.0 * M_PI);
+      h.v = std::acos(-on.y) / M_PI;
+      h.p = p;
+      h.t = root;
+      h.mat = m;
+      h.prim = id;
+      set_face(h, ray, on);
+      return true;
+    }
+    return false;
+  }
+  bool bbox(Aabb& b) const override {
+    Vec3 rr(r, r, r);
+    Vec3 lo = c - rr;
+    lo.y = c.y;
+    b = {lo, c + rr};
+    return true;
+  }
+};
+
+// Annulus (annulus.rs:8-82): plane test with a parallel cull, then the ring
+// inner_r^2 <= |offset|^2 <= outer_r^2; u angular, v radial.
+struct Annulus : Hittable {
+  Vec3 c, n;  // n: unit, (0, 1, 0) when near zero (Annulus::new)
+  double r_in, r_out;
+  const Material* m;
+  int64_t id;
+  bool hit(const Ray& ray, double tmin, double tmax, HitRecord& h) const override {
+    double denom = n.dot(ray.d);
+    if (std::fabs(denom) < 1e-8) return false;
+    double t = (c - ray.o).dot(n) / denom;
+    if (t < tmin || t > tmax) return false;
+    Vec3 point = ray.at(t);
+    Vec3 offset = point - c;
+    double dist_sq = offset.length_squared();
+    if (dist_sq < r_in * r_in || dist_sq > r_out * r_out) return false;
+    double rr = std::sqrt(dist_sq);
+    h.u = std::atan2(offset.z, offset.x) / (2.0 * M_PI) + 0.5;
+    h.v = (rr - r_in) / (r_out - r_in);
+    h.t = t;
+    h.p = point;
+    h.mat = m;
+    h.prim = id;
+    set_face(h, ray, n);
+    return true;
+  }
+  bool bbox(Aabb& b) const override {
+    Vec3 e(r_out, r_out, r_out);
+    b = {c - e, c + e};
+    return true;
+  }
+};
+
 struct Triangle : Hittable {  // triangle.rs:23-70 / obj.rs:260-319
   Vec3 v0, v1, v2;
   const Material* m;
@@ -1546,6 +1659,33 @@ HPtr make_leaf(const lmn_scene_desc& d, uint32_t i, const World& w, bool exact) 
       dk->m = m;
       dk->id = id;
       return dk;
+    }
+    case LMN_PRIM_ELLIPSOID: {
+      auto e = std::make_unique<Ellipsoid>();
+      e->c = v3(q);
+      e->radii = v3(q + 3);
+      e->m = m;
+      e->id = id;
+      return e;
+    }
+    case LMN_PRIM_HEMISPHERE: {
+      auto hs = std::make_unique<Hemisphere>();
+      hs->c = v3(q);
+      hs->r = q[3];
+      hs->m = m;
+      hs->id = id;
+      return hs;
+    }
+    case LMN_PRIM_ANNULUS: {
+      auto a = std::make_unique<Annulus>();
+      a->c = v3(q);
+      Vec3 n = v3(q + 3);
+      a->n = (std::fabs(n.x) < 1e-8 && std::fabs(n.y) < 1e-8 && std::fabs(n.z) < 1e-8) ? Vec3(0, 1, 0) : n.unit();
+      a->r_in = q[6];
+      a->r_out = q[7];
+      a->m = m;
+      a->id = id;
+      return a;
     }
     case LMN_PRIM_CYLINDER:
     case LMN_PRIM_CONE: {

@@ -92,14 +92,35 @@ def test_geometry_outside_the_path_is_rejected_by_name():  # src/scene.rs:2304-2
     _, _, world = lm.load_scene(base)
     assert len(world.prims) == 1
     with pytest.raises(_abi.LuminaraError) as e:
-        lm.load_scene(base + '[[annulus]]\ncenter = [0.0, 0.0, 0.0]\nnormal = [0.0, 1.0, 0.0]\n'
-                             'inner_radius = 0.2\nouter_radius = 0.5\n'
-                             'material = { type = "metal", color = [0.5, 0.5, 0.5] }\n')
-    assert e.value.code == _abi.LMN_ERR_UNSUPPORTED and "annulus" in str(e.value)
+        lm.load_scene(base + '[[fog]]\ncenter = [0.0, 1.0, 0.0]\nradius = 2.0\ndensity = 0.5\n'
+                             'color = [0.8, 0.8, 0.8]\n')
+    assert e.value.code == _abi.LMN_ERR_UNSUPPORTED and "fog" in str(e.value)
     # cylinders, cones and disks are on the path, after spheres in load order
     _, _, world = lm.load_scene(base + '[[cylinder]]\ncenter = [0.0, 0.0, 0.0]\nradius = 0.5\nheight = 1.0\n'
                                        'material = { type = "metal", color = [0.5, 0.5, 0.5] }\n')
     assert [p.kind for p in world.prims] == [_abi.PRIM_SPHERE, _abi.PRIM_CYLINDER]
+
+
+def test_capsule_ellipsoid_hemisphere_annulus_load_order():  # src/scene.rs:1308-1386, :2341-2385
+    mat = 'material = { type = "lambertian", color = [0.5, 0.5, 0.5] }\n'
+    body = ('[[annulus]]\ncenter = [0.0, 0.0, 0.0]\nnormal = [0.0, 1.0, 0.0]\ninner_radius = 0.5\n'
+            'outer_radius = 1.0\n' + mat +
+            '[[hemisphere]]\ncenter = [0.0, 0.0, 0.0]\nradius = 1.0\n' + mat +
+            '[[ellipsoid]]\ncenter = [0.0, 0.0, 0.0]\nradii = [1.0, 0.5, 0.7]\n' + mat +
+            '[[capsule]]\ncenter = [1.0, 2.0, 3.0]\nradius = 0.3\nheight = 1.0\n' + mat)
+    _, _, world = lm.load_scene(body)
+    # fixed type order: capsule, ellipsoid, (light), hemisphere, ..., annulus;
+    # a capsule is one world-list entry of a cylinder and two spheres
+    assert [p.kind for p in world.prims] == [_abi.PRIM_CYLINDER, _abi.PRIM_SPHERE, _abi.PRIM_SPHERE,
+                                             _abi.PRIM_ELLIPSOID, _abi.PRIM_HEMISPHERE, _abi.PRIM_ANNULUS]
+    assert [p.object for p in world.prims] == [0, 0, 0, 1, 2, 3]
+    cyl, bottom, top = world.prims[:3]
+    assert list(cyl.p[:6]) == [1.0, 2.0, 3.0, 0.3, 2.0, 3.0]
+    assert list(bottom.p[:4]) == [1.0, 2.0, 3.0, 0.3] and list(top.p[:4]) == [1.0, 3.0, 3.0, 0.3]
+    assert list(world.prims[5].p[:8]) == [0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.5, 1.0]
+    with pytest.raises(_abi.LuminaraError) as e:
+        lm.load_scene('[[ellipsoid]]\ncenter = [0.0, 0.0, 0.0]\nradii = [1.0, 0.5, 0.7]\nbump_strength = 0.2\n' + mat)
+    assert e.value.code == _abi.LMN_ERR_UNSUPPORTED and "bump" in str(e.value)
 
 
 def test_defaults_match_render_config_default():  # src/render.rs:422-431
