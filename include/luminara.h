@@ -61,8 +61,17 @@ extern "C" {
  *   HEMISPHERE      center[3], radius (the half y >= c.y)   src/hemisphere.rs:14
  *   ANNULUS         center[3], normal[3] (as given), inner_r, outer_r
  *                                                           src/annulus.rs:17
+ *   FACET           v0[3], v1[3], v2[3], normal[3] (unit, outward as the
+ *                   reference orients it): one triangle of a prism or pyramid
+ *                                   src/prism.rs:40-126, src/pyramid.rs:29-97
+ *   WEDGE_FACE      min[3], max[3], face (0 bottom, 1 back, 2 left, 3 right,
+ *                   4 ramp)                                 src/wedge.rs:22-120
+ *   ROUNDED_BOX     center[3], half_size[3], radius         src/rounded_box.rs:10
+ *   MOBIUS          center[3], radius, width, thickness     src/mobius.rs:11
  * A capsule (src/capsule.rs:15-53) is three prims of one object: CYLINDER
- * (y from center.y to center.y + height), then the bottom and top SPHERE. */
+ * (y from center.y to center.y + height), then the bottom and top SPHERE.
+ * A prism is eight FACETs of one object (bottom, top, then two per side), a
+ * pyramid six (two for the base, four sides), a wedge five WEDGE_FACEs. */
 enum {
   LMN_PRIM_SPHERE = 0,
   LMN_PRIM_QUAD = 1,
@@ -80,7 +89,11 @@ enum {
   LMN_PRIM_ELLIPSOID = 13,
   LMN_PRIM_HEMISPHERE = 14,
   LMN_PRIM_ANNULUS = 15,
-  LMN_PRIM_KIND_COUNT = 16
+  LMN_PRIM_FACET = 16,
+  LMN_PRIM_WEDGE_FACE = 17,
+  LMN_PRIM_ROUNDED_BOX = 18,
+  LMN_PRIM_MOBIUS = 19,
+  LMN_PRIM_KIND_COUNT = 20
 };
 
 typedef struct lmn_prim {

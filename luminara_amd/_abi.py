@@ -19,10 +19,12 @@ LMN_ERR_DEVICE = 5
 LMN_ERR_NOMEM = 6
 
 PRIM_KINDS = ["sphere", "quad", "rect_xy", "rect_xz", "rect_yz", "triangle", "plane", "torus",
-              "superellipsoid", "spring", "disk", "cylinder", "cone", "ellipsoid", "hemisphere", "annulus"]
+              "superellipsoid", "spring", "disk", "cylinder", "cone", "ellipsoid", "hemisphere", "annulus",
+              "facet", "wedge_face", "rounded_box", "mobius"]
 (PRIM_SPHERE, PRIM_QUAD, PRIM_RECT_XY, PRIM_RECT_XZ, PRIM_RECT_YZ, PRIM_TRIANGLE, PRIM_PLANE,
  PRIM_TORUS, PRIM_SUPERELLIPSOID, PRIM_SPRING, PRIM_DISK, PRIM_CYLINDER, PRIM_CONE,
- PRIM_ELLIPSOID, PRIM_HEMISPHERE, PRIM_ANNULUS) = range(16)
+ PRIM_ELLIPSOID, PRIM_HEMISPHERE, PRIM_ANNULUS, PRIM_FACET, PRIM_WEDGE_FACE, PRIM_ROUNDED_BOX,
+ PRIM_MOBIUS) = range(20)
 MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_EMISSIVE, MAT_MICROFACET = range(5)
 LIGHT_SPHERE, LIGHT_RECT, LIGHT_DISK = range(3)
 BG_SKY, BG_SOLID, BG_GRADIENT, BG_SUN = range(4)

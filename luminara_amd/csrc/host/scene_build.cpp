@@ -180,6 +180,23 @@ static void object_bounds(const lmn_prim& p, double lo[3], double hi[3]) {
     case LMN_PRIM_ANNULUS:  // src/annulus.rs:78-81: centre +/- outer radius
       for (int k = 0; k < 3; ++k) { lo[k] = q[k] - q[7]; hi[k] = q[k] + q[7]; }
       break;
+    case LMN_PRIM_FACET:  // one triangle of a prism or pyramid: its vertices, padded
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(std::min(q[k], q[3 + k]), q[6 + k]) - 1e-4;
+        hi[k] = std::max(std::max(q[k], q[3 + k]), q[6 + k]) + 1e-4;
+      }
+      break;
+    case LMN_PRIM_ROUNDED_BOX:  // src/rounded_box.rs:81-87: centre +/- (half size + radius)
+      for (int k = 0; k < 3; ++k) { lo[k] = q[k] - (q[3 + k] + q[6]); hi[k] = q[k] + (q[3 + k] + q[6]); }
+      break;
+    case LMN_PRIM_MOBIUS: {  // src/mobius.rs:113-120
+      double e[3] = {q[3] + q[4] + q[5], q[4] + q[5], q[3] + q[4] + q[5]};
+      for (int k = 0; k < 3; ++k) { lo[k] = q[k] - e[k]; hi[k] = q[k] + e[k]; }
+      break;
+    }
+    case LMN_PRIM_WEDGE_FACE:  // src/wedge.rs:123-125: the wedge's box, padded
+      for (int k = 0; k < 3; ++k) { lo[k] = q[k] - 1e-4; hi[k] = q[3 + k] + 1e-4; }
+      break;
     default:
       for (int k = 0; k < 3; ++k) { lo[k] = -INFINITY; hi[k] = INFINITY; }
   }
@@ -656,6 +673,14 @@ void fill_record(const lmn_prim& p, const XformT<double>* x, PrimT<R>& r, uint32
       v[7] = q[7];
       break;
     }
+    case LMN_PRIM_FACET:  // v0, e1 = v1 - v0, e2 = v2 - v0 (src/prism.rs:104-105), normal
+      for (int k = 0; k < 3; ++k) {
+        v[k] = q[k];
+        v[3 + k] = q[3 + k] - q[k];
+        v[6 + k] = q[6 + k] - q[k];
+        v[9 + k] = q[9 + k];
+      }
+      break;
     default:
       for (int k = 0; k < 12; ++k) v[k] = q[k];
   }

@@ -9,6 +9,7 @@
 // rejected with LMN_ERR_UNSUPPORTED naming the feature.
 #include "scene_load.h"
 
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -524,8 +525,33 @@ class Loader {
       p.p[3] = R;
       p.p[4] = r;
     }
-    reject_if_present(root, "rounded_box", "(SDF rounded box)");
-    reject_if_present(root, "mobius", "(SDF Mobius strip)");
+    for (auto& s : table_array(root, "rounded_box")) {  // src/scene.rs:1264-1272
+      const std::string ctx = "rounded_box";
+      double c[3], hs[3];
+      req_vec3(s, "center", ctx, c);
+      req_vec3(s, "half_size", ctx, hs);
+      double r = req_num(s, "radius", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      lmn_prim& p = add_prim(LMN_PRIM_ROUNDED_BOX, mat, -1, new_object());
+      for (int k = 0; k < 3; ++k) {
+        p.p[k] = c[k];
+        p.p[3 + k] = hs[k];
+      }
+      p.p[6] = r;
+    }
+    for (auto& s : table_array(root, "mobius")) {  // src/scene.rs:1274-1283 (thickness default 0.05)
+      const std::string ctx = "mobius";
+      double c[3], th = 0.05;
+      req_vec3(s, "center", ctx, c);
+      double r = req_num(s, "radius", ctx), w = req_num(s, "width", ctx);
+      opt_num(s, "thickness", ctx, th);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      lmn_prim& p = add_prim(LMN_PRIM_MOBIUS, mat, -1, new_object());
+      for (int k = 0; k < 3; ++k) p.p[k] = c[k];
+      p.p[3] = r;
+      p.p[4] = w;
+      p.p[5] = th;
+    }
     for (auto& s : table_array(root, "spring")) {
       const std::string ctx = "spring";
       double c[3];
@@ -690,9 +716,90 @@ class Loader {
       p.p[6] = r;
     }
     for (auto& s : table_array(root, "cylinder")) axial(s, "cylinder", LMN_PRIM_CYLINDER);
-    reject_if_present(root, "prism", "(prism)");
-    reject_if_present(root, "wedge", "(wedge)");
-    reject_if_present(root, "pyramid", "(pyramid)");
+    // RotateX/Y/Z then Translate wrappers of a prism, wedge or pyramid
+    // (src/scene.rs:1465-1468, :1475-1478, :1486-1489)
+    auto rot_translate = [&](const TomlValue& s, const std::string& ctx) -> int32_t {
+      std::vector<lmn_xform_op> ops;
+      double a, t[3];
+      if (opt_num(s, "rotate_x", ctx, a)) ops.push_back(op(LMN_XF_ROTATE_X, a));
+      if (opt_num(s, "rotate_y", ctx, a)) ops.push_back(op(LMN_XF_ROTATE_Y, a));
+      if (opt_num(s, "rotate_z", ctx, a)) ops.push_back(op(LMN_XF_ROTATE_Z, a));
+      if (opt_vec3(s, "translate", ctx, t)) ops.push_back(op(LMN_XF_TRANSLATE, t[0], t[1], t[2]));
+      return ops.empty() ? -1 : int32_t(add_transform(ops));
+    };
+    typedef std::array<double, 3> P3;
+    auto facet = [&](uint32_t mat, int32_t xf, uint32_t obj, const P3& a, const P3& b, const P3& c, const P3& n) {
+      lmn_prim& p = add_prim(LMN_PRIM_FACET, mat, xf, obj);
+      for (int k = 0; k < 3; ++k) {
+        p.p[k] = a[k];
+        p.p[3 + k] = b[k];
+        p.p[6 + k] = c[k];
+        p.p[9 + k] = n[k];
+      }
+    };
+    auto unit3 = [](const P3& v) {
+      double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      return P3{v[0] / l, v[1] / l, v[2] / l};
+    };
+    auto cross3 = [](const P3& a, const P3& b) {
+      return P3{a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    };
+    auto sub3 = [](const P3& a, const P3& b) { return P3{a[0] - b[0], a[1] - b[1], a[2] - b[2]}; };
+    for (auto& s : table_array(root, "prism")) {  // src/scene.rs:1450-1470, src/prism.rs:18-85
+      const std::string ctx = "prism";
+      double c[3];
+      req_vec3(s, "center", ctx, c);
+      double side = req_num(s, "side", ctx), h = req_num(s, "height", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      if (s.get("bump_strength") || s.get("normal_map"))
+        fail(LMN_ERR_UNSUPPORTED, "prism bump/normal maps are not on the GPU path");
+      int32_t xf = rot_translate(s, ctx);
+      uint32_t obj = new_object();
+      double r = side / std::sqrt(3.0);  // circumradius of the equilateral base
+      const P3 bot[3] = {P3{c[0], c[1], c[2] + r}, P3{c[0] - side / 2.0, c[1], c[2] - r / 2.0},
+                         P3{c[0] + side / 2.0, c[1], c[2] - r / 2.0}};
+      P3 top[3];
+      for (int i = 0; i < 3; ++i) top[i] = P3{bot[i][0], bot[i][1] + h, bot[i][2]};
+      facet(mat, xf, obj, bot[0], bot[1], bot[2], P3{0.0, -1.0, 0.0});
+      facet(mat, xf, obj, top[0], top[2], top[1], P3{0.0, 1.0, 0.0});
+      for (int i = 0; i < 3; ++i) {  // side normal edge x up, as the reference forms it
+        int j = (i + 1) % 3;
+        P3 n = unit3(cross3(sub3(bot[j], bot[i]), P3{0.0, 1.0, 0.0}));
+        facet(mat, xf, obj, bot[i], bot[j], top[j], n);
+        facet(mat, xf, obj, bot[i], top[j], top[i], n);
+      }
+    }
+    for (auto& s : table_array(root, "wedge")) {  // src/scene.rs:1472-1480, src/wedge.rs:22-120
+      const std::string ctx = "wedge";
+      double mn[3], mx[3];
+      req_vec3(s, "min", ctx, mn);
+      req_vec3(s, "max", ctx, mx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      int32_t xf = rot_translate(s, ctx);
+      uint32_t obj = new_object();
+      for (int face = 0; face < 5; ++face) {
+        lmn_prim& p = add_prim(LMN_PRIM_WEDGE_FACE, mat, xf, obj);
+        for (int k = 0; k < 3; ++k) {
+          p.p[k] = mn[k];
+          p.p[3 + k] = mx[k];
+        }
+        p.p[6] = face;
+      }
+    }
+    for (auto& s : table_array(root, "pyramid")) {  // src/scene.rs:1482-1491, src/pyramid.rs:52-97
+      const std::string ctx = "pyramid";
+      double c[3];
+      req_vec3(s, "center", ctx, c);
+      double hb = req_num(s, "base_size", ctx) / 2.0, h = req_num(s, "height", ctx);
+      uint32_t mat = material(req(s, "material", ctx), ctx + ".material");
+      int32_t xf = rot_translate(s, ctx);
+      uint32_t obj = new_object();
+      const P3 b0{c[0] - hb, c[1], c[2] - hb}, b1{c[0] + hb, c[1], c[2] - hb}, b2{c[0] + hb, c[1], c[2] + hb},
+          b3{c[0] - hb, c[1], c[2] + hb}, apex{c[0], c[1] + h, c[2]};
+      const P3 tri[6][3] = {{b0, b2, b1}, {b0, b3, b2}, {b0, b1, apex}, {b1, b2, apex}, {b2, b3, apex}, {b3, b0, apex}};
+      for (auto& t : tri)  // normal unit(e1 x e2), as the reference forms it at each hit
+        facet(mat, xf, obj, t[0], t[1], t[2], unit3(cross3(sub3(t[1], t[0]), sub3(t[2], t[0]))));
+    }
     for (auto& s : table_array(root, "triangle")) {
       const std::string ctx = "triangle";
       double v[9];

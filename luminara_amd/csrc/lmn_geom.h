@@ -24,12 +24,14 @@ namespace lmn {
 enum : uint32_t {
   K_SPHERE = 0, K_QUAD = 1, K_RECT_XY = 2, K_RECT_XZ = 3, K_RECT_YZ = 4, K_TRIANGLE = 5,
   K_PLANE = 6, K_TORUS = 7, K_SUPERELLIPSOID = 8, K_SPRING = 9, K_DISK = 10, K_CYLINDER = 11, K_CONE = 12,
-  K_ELLIPSOID = 13, K_HEMISPHERE = 14, K_ANNULUS = 15
+  K_ELLIPSOID = 13, K_HEMISPHERE = 14, K_ANNULUS = 15, K_FACET = 16, K_WEDGE_FACE = 17,
+  K_ROUNDED_BOX = 18, K_MOBIUS = 19
 };
 // Torus, superellipsoid, spring: the marched kinds (own BVH, FP64 roots).
 LMN_HD bool is_marched_kind(uint32_t k) { return k >= K_TORUS && k <= K_SPRING; }
-// Disk, cylinder, cone, ellipsoid, hemisphere, annulus: rare analytic kinds
-// kept in the marched kinds' BVH
+// Disk, cylinder, cone, ellipsoid, hemisphere, annulus, prism and pyramid
+// facets, wedge faces, and the sphere-traced rounded box and Mobius strip:
+// rare kinds kept in the marched kinds' BVH
 // and evaluated there in FP64, out of line, so the main traversal carries
 // none of their code.
 LMN_HD bool is_quadric_kind(uint32_t k) { return k >= K_DISK; }
@@ -45,6 +47,8 @@ constexpr uint32_t XF_NONE = 0xffffffffu;
 //   SUPERELLIPSOID c.xyz s.xyz e1 e2 | SPRING c.xyz coil tube pitch turns
 //   DISK c.xyz unit-normal.xyz r | CYLINDER, CONE c.xyz r y_min y_max
 //   ELLIPSOID c.xyz radii.xyz | HEMISPHERE c.xyz r | ANNULUS c.xyz unit-normal.xyz r_in r_out
+//   FACET v0.xyz e1.xyz e2.xyz normal.xyz | WEDGE_FACE min.xyz max.xyz face
+//   ROUNDED_BOX c.xyz half.xyz r | MOBIUS c.xyz radius half-width thickness
 template <class R>
 struct alignas(16) PrimT {
   R p[13];
@@ -1049,20 +1053,245 @@ LMN_HD bool hit_annulus(const double* p, V3<double> o, V3<double> d, double tmin
   t = tt;
   return true;
 }
+// One triangle of a prism or pyramid: the solids' own Moller-Trumbore
+// (src/prism.rs:100-126, src/pyramid.rs:29-50: cull |a| < 1e-10, t > 1e-10)
+// under their strict range t_min < t < closest (src/prism.rs:50-78,
+// src/pyramid.rs:86-94), so that of two facets at one t the first kept wins.
+// Returns the barycentric (u, v) too.
+LMN_HD bool facet_mt(const double* p, V3<double> o, V3<double> d, double& t, double& u, double& v) {
+  const V3<double> v0 = {p[0], p[1], p[2]}, e1 = {p[3], p[4], p[5]}, e2 = {p[6], p[7], p[8]};
+  V3<double> h = cross(d, e2);
+  double a = dot(e1, h);
+  if (m_abs(a) < 1e-10) return false;
+  double f = 1.0 / a;
+  V3<double> s = o - v0;
+  u = f * dot(s, h);
+  if (!(u >= 0.0 && u <= 1.0)) return false;
+  V3<double> q = cross(s, e1);
+  v = f * dot(d, q);
+  if (v < 0.0 || u + v > 1.0) return false;
+  t = f * dot(e2, q);
+  return t > 1e-10;
+}
+LMN_HD bool hit_facet(const double* p, V3<double> o, V3<double> d, double tmin, double tmax, double& t) {
+  double tt, u, v;
+  if (!facet_mt(p, o, d, tt, u, v) || !(tt > tmin && tt < tmax)) return false;
+  t = tt;
+  return true;
+}
+// One face of a wedge (src/wedge.rs:22-120): the box [min, max] whose top is
+// a ramp rising along +z from min.y at min.z to max.y at max.z.  Faces in the
+// reference's order: 0 bottom, 1 back (z = max.z), 2 left, 3 right (the
+// triangles under the ramp), 4 the ramp; each under the strict range
+// t_min < t < closest.
+LMN_HD double wedge_ramp_y(const double* p, double z) { return (z - p[2]) / (p[5] - p[2]) * (p[4] - p[1]) + p[1]; }
+LMN_HD V3<double> wedge_ramp_normal(const double* p) {
+  return unit(V3<double>{0.0, 1.0, -((p[4] - p[1]) / (p[5] - p[2]))});
+}
+LMN_HD bool hit_wedge_face(const double* p, V3<double> o, V3<double> d, double tmin, double tmax, double& t) {
+  const int face = int(p[6]);
+  const V3<double> lo = {p[0], p[1], p[2]}, hi = {p[3], p[4], p[5]};
+  double tt;
+  if (face == 4) {
+    const V3<double> n = wedge_ramp_normal(p);
+    double denom = dot(d, n);
+    if (!(m_abs(denom) > 1e-10)) return false;
+    tt = dot(lo - o, n) / denom;
+  } else {
+    const int axis = face == 0 ? 1 : face == 1 ? 2 : 0;
+    const double dc = axis == 0 ? d.x : axis == 1 ? d.y : d.z;
+    const double oc = axis == 0 ? o.x : axis == 1 ? o.y : o.z;
+    const double plane = face == 0 ? lo.y : face == 1 ? hi.z : face == 2 ? lo.x : hi.x;
+    if (!(m_abs(dc) > 1e-10)) return false;
+    tt = (plane - oc) / dc;
+  }
+  if (!(tt > tmin && tt < tmax)) return false;
+  const V3<double> q = o + d * tt;
+  bool in;
+  switch (face) {
+    case 0: in = q.x >= lo.x && q.x <= hi.x && q.z >= lo.z && q.z <= hi.z; break;
+    case 1: in = q.x >= lo.x && q.x <= hi.x && q.y >= lo.y && q.y <= hi.y; break;
+    case 2:
+    case 3: in = q.z >= lo.z && q.z <= hi.z && q.y >= lo.y && q.y <= wedge_ramp_y(p, q.z); break;
+    default:
+      in = q.x >= lo.x && q.x <= hi.x && q.z >= lo.z && q.z <= hi.z && m_abs(q.y - wedge_ramp_y(p, q.z)) < 0.01;
+  }
+  if (!in) return false;
+  t = tt;
+  return true;
+}
+// ---- sphere-traced kinds: rounded box, Mobius strip -------------------------
+// RoundedBox's signed distance (src/rounded_box.rs:30-35), at p relative to
+// the centre.
+LMN_HD double sdf_rounded_box(const double* p, V3<double> q) {
+  const double qx = m_abs(q.x) - p[3], qy = m_abs(q.y) - p[4], qz = m_abs(q.z) - p[5];
+  const double ox = fmax(qx, 0.0), oy = fmax(qy, 0.0), oz = fmax(qz, 0.0);
+  const double outside = m_sqrt(ox * ox + oy * oy + oz * oz);
+  const double inside = fmin(fmax(qx, fmax(qy, qz)), 0.0);
+  return outside + inside - p[6];
+}
+// Mobius's distance estimate (src/mobius.rs:37-80): the nearest of the
+// strip's segments at the point's angle and the angles pi either side.
+LMN_HD double sdf_mobius(const double* p, V3<double> q) {
+  const double r = p[3], w = p[4], th = p[5], pi = 3.14159265358979323846;
+  const double angle = m_atan2(q.z, q.x);
+  double best = __builtin_inf();
+  for (int k = -1; k <= 1; ++k) {
+    const double u = angle + double(k) * pi;
+    const double cos_u = m_cos(u), sin_u = m_sin(u), cos_half = m_cos(u / 2.0), sin_half = m_sin(u / 2.0);
+    const double cx = r * cos_u, cz = r * sin_u;
+    const V3<double> dir = unit(V3<double>{cos_u * cos_half, sin_half, sin_u * cos_half});
+    const double vp = (q.x - cx) * dir.x + q.y * dir.y + (q.z - cz) * dir.z;
+    const double vc = fmin(fmax(vp, -w), w);
+    const double dx = q.x - (cx + vc * dir.x), dy = q.y - vc * dir.y, dz = q.z - (cz + vc * dir.z);
+    const double dist = m_sqrt(dx * dx + dy * dy + dz * dz) - th;
+    if (dist < best) best = dist;
+  }
+  return best;
+}
+LMN_HD double sdf_traced(uint32_t kind, const double* p, V3<double> q) {
+  return kind == K_ROUNDED_BOX ? sdf_rounded_box(p, q) : sdf_mobius(p, q);
+}
+// The reference's sphere trace (src/rounded_box.rs:38-76, src/mobius.rs:84-108):
+// from t_min, accept where the distance is below tol, else step max(dist,
+// floor) in t (the direction unnormalised), up to n steps or past t_max.
+// The Mobius strip is traced this way in both intersection modes.
+LMN_HD bool hit_traced_reference(uint32_t kind, const double* p, V3<double> o, V3<double> d, double tmin,
+                                 double tmax, double& t) {
+  const V3<double> oc = o - V3<double>{p[0], p[1], p[2]};
+  const bool box = kind == K_ROUNDED_BOX;
+  const int steps = box ? 256 : 512;
+  const double tol = box ? 1e-5 : 1e-4, floor = box ? 1e-4 : p[5] * 0.3;
+  double tt = tmin;
+  for (int i = 0; i < steps; ++i) {
+    if (tt > tmax) break;
+    double dist = sdf_traced(kind, p, oc + d * tt);
+    if (dist < tol) {
+      t = tt;
+      return true;
+    }
+    tt += fmax(dist, floor);
+  }
+  return false;
+}
+// Gradient of the rounded box's distance at q (unit; the outward normal).
+LMN_HD V3<double> grad_rounded_box(const double* p, V3<double> q) {
+  const double qx = m_abs(q.x) - p[3], qy = m_abs(q.y) - p[4], qz = m_abs(q.z) - p[5];
+  const double sx = q.x < 0.0 ? -1.0 : 1.0, sy = q.y < 0.0 ? -1.0 : 1.0, sz = q.z < 0.0 ? -1.0 : 1.0;
+  if (qx > 0.0 || qy > 0.0 || qz > 0.0) {
+    const V3<double> o = {fmax(qx, 0.0) * sx, fmax(qy, 0.0) * sy, fmax(qz, 0.0) * sz};
+    return o / length(o);
+  }
+  if (qx >= qy && qx >= qz) return {sx, 0.0, 0.0};
+  if (qy >= qz) return {0.0, sy, 0.0};
+  return {0.0, 0.0, sz};
+}
+// Exact mode for the rounded box, whose distance is exact: trace in units of
+// length (t step = |dist| / |d|, never across the surface), outward from
+// t_min or, for an origin inside, to the exit, until the distance is below
+// 1e-7 of the box's size; then Newton steps on dist(t) with the analytic
+// gradient (the distance is C1 across the face, edge and corner pieces) to
+// full precision.  A trace that runs out of steps (a ray grazing the
+// surface) is a miss.
+LMN_HD bool hit_rounded_box_exact(const double* p, V3<double> o, V3<double> d, double tmin, double tmax, double& t) {
+  const V3<double> oc = o - V3<double>{p[0], p[1], p[2]};
+  const double len = length(d), inv_len = 1.0 / len;
+  const double tol = 1e-7 * (p[3] + p[4] + p[5] + p[6]);
+  double tt = tmin;
+  const double sgn = sdf_rounded_box(p, oc + d * tt) < 0.0 ? -1.0 : 1.0;
+  int i = 0;
+  for (; i < 2048; ++i) {
+    if (!(tt <= tmax)) return false;
+    double dist = sgn * sdf_rounded_box(p, oc + d * tt);
+    if (dist < tol) break;
+    tt += dist * inv_len;
+  }
+  if (i == 2048) return false;
+  for (int k = 0; k < 4; ++k) {
+    const V3<double> q = oc + d * tt;
+    const double g = sdf_rounded_box(p, q), dg = dot(grad_rounded_box(p, q), d);
+    if (!(m_abs(dg) > 1e-6 * len)) break;
+    const double step = g / dg;
+    if (!(m_abs(step) <= 2.0 * tol * inv_len + 1e-12 * m_abs(tt))) break;  // stay at this root
+    tt -= step;
+  }
+  if (!(tt >= tmin && tt <= tmax)) return false;
+  t = tt;
+  return true;
+}
+// Central-difference normal (step 1e-5 for the box, 1e-4 for the strip) and
+// the reference's (u, v) at q relative to the centre (src/rounded_box.rs:47-69,
+// src/mobius.rs:92-103).
+LMN_HD void traced_surface(uint32_t kind, const double* p, V3<double> q, V3<double>& n, double& u, double& v) {
+  const double eps = kind == K_ROUNDED_BOX ? 1e-5 : 1e-4;
+  const double nx = sdf_traced(kind, p, q + V3<double>{eps, 0.0, 0.0}) - sdf_traced(kind, p, q - V3<double>{eps, 0.0, 0.0});
+  const double ny = sdf_traced(kind, p, q + V3<double>{0.0, eps, 0.0}) - sdf_traced(kind, p, q - V3<double>{0.0, eps, 0.0});
+  const double nz = sdf_traced(kind, p, q + V3<double>{0.0, 0.0, eps}) - sdf_traced(kind, p, q - V3<double>{0.0, 0.0, eps});
+  n = unit(V3<double>{nx, ny, nz});
+  if (kind == K_MOBIUS) {
+    u = m_atan2(q.z, q.x) / (2.0 * 3.14159265358979323846) + 0.5;
+    v = q.y / (p[4] + p[5]) * 0.5 + 0.5;
+    return;
+  }
+  const V3<double> total = {p[3] + p[6], p[4] + p[6], p[5] + p[6]};
+  const double ax = m_abs(n.x), ay = m_abs(n.y), az = m_abs(n.z);
+  if (ax > ay && ax > az) {
+    u = (q.z / total.z + 1.0) * 0.5;
+    v = (q.y / total.y + 1.0) * 0.5;
+  } else if (ay > az) {
+    u = (q.x / total.x + 1.0) * 0.5;
+    v = (q.z / total.z + 1.0) * 0.5;
+  } else {
+    u = (q.x / total.x + 1.0) * 0.5;
+    v = (q.y / total.y + 1.0) * 0.5;
+  }
+}
+
 LMN_COLD bool hit_quadric_fp64(uint32_t kind, const double* p, V3<double> o, V3<double> d, double tmin, double tmax,
-                               double& t) {
+                               bool exact, double& t) {
   switch (kind) {
+    case K_ROUNDED_BOX:
+      return exact ? hit_rounded_box_exact(p, o, d, tmin, tmax, t) : hit_traced_reference(kind, p, o, d, tmin, tmax, t);
+    case K_MOBIUS: return hit_traced_reference(kind, p, o, d, tmin, tmax, t);
     case K_DISK: return hit_disk(p, o, d, tmin, tmax, t);
     case K_ELLIPSOID: return hit_ellipsoid(p, o, d, tmin, tmax, t);
     case K_HEMISPHERE: return hit_hemisphere(p, o, d, tmin, tmax, t);
     case K_ANNULUS: return hit_annulus(p, o, d, tmin, tmax, t);
+    case K_FACET: return hit_facet(p, o, d, tmin, tmax, t);
+    case K_WEDGE_FACE: return hit_wedge_face(p, o, d, tmin, tmax, t);
     default: return hit_cylinder_cone(kind == K_CONE, p, o, d, tmin, tmax, t);
   }
 }
-// Outward normal and (u, v) at object-space point q of the hit.
-LMN_COLD void quadric_surface(uint32_t kind, const double* p, V3<double> q, V3<double>& n, double& u, double& v) {
+// Outward normal and (u, v) at object-space point q = o + d t of the hit.
+LMN_COLD void quadric_surface(uint32_t kind, const double* p, V3<double> o, V3<double> d, V3<double> q,
+                              V3<double>& n, double& u, double& v) {
   const double pi = 3.14159265358979323846;
   const V3<double> c = {p[0], p[1], p[2]};
+  if (kind == K_ROUNDED_BOX || kind == K_MOBIUS) {
+    traced_surface(kind, p, q - c, n, u, v);
+    return;
+  }
+  if (kind == K_FACET) {  // the stored normal; (u, v) from the same ray's Moller-Trumbore
+    double tt;
+    n = {p[9], p[10], p[11]};
+    facet_mt(p, o, d, tt, u, v);
+    return;
+  }
+  if (kind == K_WEDGE_FACE) {  // src/wedge.rs:37-116: per-face normal and (u, v)
+    const double width = p[3] - p[0], height = p[4] - p[1], depth = p[5] - p[2];
+    switch (int(p[6])) {
+      case 0: n = {0.0, -1.0, 0.0}; u = (q.x - p[0]) / width; v = (q.z - p[2]) / depth; break;
+      case 1: n = {0.0, 0.0, 1.0}; u = (q.x - p[0]) / width; v = (q.y - p[1]) / height; break;
+      case 2:
+      case 3:
+        n = {int(p[6]) == 2 ? -1.0 : 1.0, 0.0, 0.0};
+        u = (q.z - p[2]) / depth;
+        v = (q.y - p[1]) / height;
+        break;
+      default: n = wedge_ramp_normal(p); u = (q.x - p[0]) / width; v = (q.z - p[2]) / depth;
+    }
+    return;
+  }
   if (kind == K_DISK) {  // src/disk.rs:48-56
     V3<double> off = q - c;
     n = {p[3], p[4], p[5]};
@@ -1130,7 +1359,7 @@ LMN_HD bool hit_prim_fp64(uint32_t kind, const double* p, V3<double> o, V3<doubl
                           bool exact, double& t) {
   if (kind == K_SPHERE) return hit_sphere<double>(p, o, d, tmin, tmax, t);
   if (is_marched_kind(kind)) return hit_marched_fp64(kind, p, o, d, tmin, tmax, exact, t);
-  if (is_quadric_kind(kind)) return hit_quadric_fp64(kind, p, o, d, tmin, tmax, t);
+  if (is_quadric_kind(kind)) return hit_quadric_fp64(kind, p, o, d, tmin, tmax, exact, t);
   return hit_prim_analytic<double>(kind, p, o, d, tmin, tmax, t);
 }
 

@@ -550,7 +550,7 @@ LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_r
       marched_uv(kind, p64, q, u64, v64);
     } else if (is_quadric_kind(kind)) {
       V3<double> n64;
-      quadric_surface(kind, p64, q, n64, u64, v64);
+      quadric_surface(kind, p64, oo64, dd64, q, n64, u64, v64);
       n = cvt<R>(n64);
     } else {  // FP64 sphere: only the centre offset needs FP64, the map does not
       V3<R> nn = cvt<R>(prim_normal_analytic<double>(kind, p64, q));

@@ -98,7 +98,8 @@ def quadrics_scene(width=64, height=48, spp=16, depth=8, seed=42):
     rotated and translated cones, an emissive disk that the loader registers
     as an NEE light, a tilted disk, spheres between them, and in the front row
     the ellipsoid, a raised hemisphere open below, a tilted metal annulus and
-    the capsule."""
+    the capsule; a rotated prism, a pyramid and a rotated, translated wedge
+    (src/prism.rs, src/pyramid.rs, src/wedge.rs) at the sides and back."""
     rng = SeededRng(seed, 9)
     L = _header(width, height, spp, depth, seed, '{ type = "sky" }',
                 {"look_from": [0.0, 3.0, 9.0], "look_at": [0.0, 0.8, 0.0], "vfov": 40, "aperture": 0.0,
@@ -139,6 +140,44 @@ def quadrics_scene(width=64, height=48, spp=16, depth=8, seed=42):
           f"material = {_mat({'type': 'lambertian', 'color': col[2]})}", ""]
     L += ["[[annulus]]", "center = [0.9, 0.8, 2.3]", "normal = [0.3, 1.0, -0.6]", "inner_radius = 0.2",
           "outer_radius = 0.55", 'material = { type = "metal", color = [0.9, 0.8, 0.6], fuzz = 0.05 }', ""]
+    col = [[0.2 + 0.7 * rng.uniform() for _ in range(3)] for _ in range(3)]
+    L += ["[[prism]]", "center = [0.0, 0.0, 0.0]", "side = 0.7", "height = 1.0",
+          f"material = {_mat({'type': 'lambertian', 'color': col[0]})}", "rotate_y = 25.0",
+          "translate = [3.3, 0.0, 0.9]", ""]
+    L += ["[[wedge]]", "min = [-0.4, 0.0, -0.25]", "max = [0.4, 0.8, 0.25]",
+          f"material = {_mat({'type': 'lambertian', 'color': col[1]})}", "rotate_y = 15.0",
+          "translate = [1.0, 0.0, -2.7]", ""]
+    L += ["[[pyramid]]", "center = [-3.3, 0.0, 0.9]", "base_size = 0.9", "height = 1.1",
+          f"material = {_mat({'type': 'lambertian', 'color': col[2]})}", ""]
+    L += ["[[rounded_box]]", "center = [-3.2, 0.45, -1.6]", "half_size = [0.3, 0.25, 0.35]", "radius = 0.12",
+          'material = { type = "metal", color = [0.8, 0.7, 0.6], fuzz = 0.2 }', ""]
+    return "\n".join(L)
+
+
+def traced_scene(width=64, height=48, spp=16, depth=8, seed=42):
+    """The sphere-traced kinds (src/mobius.rs, src/rounded_box.rs): two Mobius
+    strips (one thin, one with the default thickness), rounded boxes of
+    several proportions, on a ground quad under the sky, spheres beside them."""
+    rng = SeededRng(seed, 11)
+    L = _header(width, height, spp, depth, seed, '{ type = "sky" }',
+                {"look_from": [0.0, 2.5, 7.0], "look_at": [0.0, 0.7, 0.0], "vfov": 40, "aperture": 0.0,
+                 "focus_dist": 1.0})
+    L += ["[[quad]]", "q = [-6.0, 0.0, -6.0]", "u = [12.0, 0.0, 0.0]", "v = [0.0, 0.0, 12.0]",
+          'material = { type = "lambertian", color = [0.6, 0.6, 0.6] }', ""]
+    for i in range(3):
+        col = [0.2 + 0.7 * rng.uniform() for _ in range(3)]
+        L += ["[[rounded_box]]", f"center = {_v([-2.0 + 2.0 * i, 0.5, -1.5])}",
+              f"half_size = {_v([0.2 + 0.3 * rng.uniform(), 0.3, 0.2 + 0.3 * rng.uniform()])}",
+              f"radius = {0.05 + 0.15 * rng.uniform()!r}", f"material = {_mat({'type': 'lambertian', 'color': col})}",
+              ""]
+    L += ["[[mobius]]", "center = [-1.2, 0.9, 0.8]", "radius = 0.8", "width = 0.3", "thickness = 0.08",
+          'material = { type = "lambertian", color = [0.8, 0.3, 0.2] }', ""]
+    L += ["[[mobius]]", "center = [1.3, 0.8, 0.6]", "radius = 0.6", "width = 0.25",
+          'material = { type = "metal", color = [0.7, 0.7, 0.8], fuzz = 0.1 }', ""]
+    for i in range(4):
+        col = [0.2 + 0.7 * rng.uniform() for _ in range(3)]
+        L += ["[[sphere]]", f"center = {_v([-3.0 + 2.0 * i, 0.25, 2.0])}", "radius = 0.25",
+              f"material = {_mat({'type': 'lambertian', 'color': col})}", ""]
     return "\n".join(L)
 
 

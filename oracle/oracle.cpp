@@ -672,6 +672,271 @@ struct Annulus : Hittable {
   }
 };
 
+// RoundedBox (rounded_box.rs:10-89) and Mobius (mobius.rs:11-121): signed
+// distance (estimate) sphere-traced from t_min as the reference does it; in
+// exact mode the rounded box, whose distance is exact, is traced in units of
+// length to 1e-10 of its size instead (inward to the exit from an origin
+// inside it); the Mobius strip keeps the reference's trace in both modes.
+struct SdfTraced : Hittable {
+  bool mobius = false;
+  bool exact = false;
+  Vec3 c, half;      // rounded box: centre, half size
+  double r = 0;      // rounded box radius / Mobius radius
+  double w = 0, th = 0;  // Mobius half-width, thickness
+  const Material* m;
+  int64_t id;
+  double sdf(const Vec3& p) const {
+    if (!mobius) {  // rounded_box.rs:30-35
+      Vec3 q(std::fabs(p.x) - half.x, std::fabs(p.y) - half.y, std::fabs(p.z) - half.z);
+      double outside = Vec3(std::fmax(q.x, 0.0), std::fmax(q.y, 0.0), std::fmax(q.z, 0.0)).length();
+      double inside = std::fmin(std::fmax(q.x, std::fmax(q.y, q.z)), 0.0);
+      return outside + inside - r;
+    }
+    double angle = std::atan2(p.z, p.x);  // mobius.rs:37-80
+    double best = INFINITY;
+    for (double offset : {-M_PI, 0.0, M_PI}) {
+      double u = angle + offset;
+      double cos_u = std::cos(u), sin_u = std::sin(u), cos_half = std::cos(u / 2.0), sin_half = std::sin(u / 2.0);
+      double cx = r * cos_u, cz = r * sin_u;
+      Vec3 strip_dir = Vec3(cos_u * cos_half, sin_half, sin_u * cos_half).unit();
+      Vec3 dp(p.x - cx, p.y, p.z - cz);
+      double v_param = dp.x * strip_dir.x + dp.y * strip_dir.y + dp.z * strip_dir.z;
+      double v_clamped = std::fmin(std::fmax(v_param, -w), w);
+      double dx = p.x - (cx + v_clamped * strip_dir.x);
+      double dy = p.y - v_clamped * strip_dir.y;
+      double dz = p.z - (cz + v_clamped * strip_dir.z);
+      double dist = std::sqrt(dx * dx + dy * dy + dz * dz) - th;
+      if (dist < best) best = dist;
+    }
+    return best;
+  }
+  bool hit(const Ray& ray, double tmin, double tmax, HitRecord& h) const override {
+    Vec3 o = ray.o - c;
+    double t = tmin;
+    bool found = false;
+    if (exact && !mobius) {
+      // trace in units of length to 1e-7 of the size, then Newton on the
+      // distance along the ray with its analytic gradient
+      double len = ray.d.length(), inv_len = 1.0 / len;
+      double tol = 1e-7 * (half.x + half.y + half.z + r);
+      double sgn = sdf(o + ray.d * t) < 0.0 ? -1.0 : 1.0;
+      for (int i = 0; i < 2048 && t <= tmax; ++i) {
+        double dist = sgn * sdf(o + ray.d * t);
+        if (dist < tol) {
+          found = true;
+          break;
+        }
+        t += dist * inv_len;
+      }
+      for (int k = 0; found && k < 4; ++k) {
+        Vec3 q = o + ray.d * t;
+        Vec3 g;  // gradient: outward from the inner box, or the nearest face's axis inside it
+        Vec3 a(std::fabs(q.x) - half.x, std::fabs(q.y) - half.y, std::fabs(q.z) - half.z);
+        Vec3 sg(q.x < 0 ? -1.0 : 1.0, q.y < 0 ? -1.0 : 1.0, q.z < 0 ? -1.0 : 1.0);
+        if (a.x > 0 || a.y > 0 || a.z > 0) {
+          Vec3 out(std::fmax(a.x, 0.0) * sg.x, std::fmax(a.y, 0.0) * sg.y, std::fmax(a.z, 0.0) * sg.z);
+          g = out / out.length();
+        } else if (a.x >= a.y && a.x >= a.z) {
+          g = Vec3(sg.x, 0, 0);
+        } else if (a.y >= a.z) {
+          g = Vec3(0, sg.y, 0);
+        } else {
+          g = Vec3(0, 0, sg.z);
+        }
+        double dg = g.dot(ray.d);
+        if (!(std::fabs(dg) > 1e-6 * len)) break;
+        double step = sdf(q) / dg;
+        if (!(std::fabs(step) <= 2.0 * tol * inv_len + 1e-12 * std::fabs(t))) break;
+        t -= step;
+      }
+      if (found && !(t >= tmin && t <= tmax)) found = false;
+    } else {
+      int steps = mobius ? 512 : 256;
+      double tol = mobius ? 1e-4 : 1e-5, floor = mobius ? th * 0.3 : 1e-4;
+      for (int i = 0; i < steps; ++i) {
+        if (t > tmax) break;
+        double dist = sdf(o + ray.d * t);
+        if (dist < tol) {
+          found = true;
+          break;
+        }
+        t += std::fmax(dist, floor);
+      }
+    }
+    if (!found) return false;
+    Vec3 point = ray.at(t);
+    Vec3 lp = point - c;
+    double eps = mobius ? 1e-4 : 1e-5;
+    double nx = sdf(lp + Vec3(eps, 0.0, 0.0)) - sdf(lp - Vec3(eps, 0.0, 0.0));
+    double ny = sdf(lp + Vec3(0.0, eps, 0.0)) - sdf(lp - Vec3(0.0, eps, 0.0));
+    double nz = sdf(lp + Vec3(0.0, 0.0, eps)) - sdf(lp - Vec3(0.0, 0.0, eps));
+    Vec3 normal = Vec3(nx, ny, nz).unit();
+    if (mobius) {
+      h.u = std::atan2(lp.z, lp.x) / (2.0 * M_PI) + 0.5;
+      h.v = lp.y / (w + th) * 0.5 + 0.5;
+    } else {
+      Vec3 total = half + Vec3(r, r, r);
+      Vec3 an(std::fabs(normal.x), std::fabs(normal.y), std::fabs(normal.z));
+      if (an.x > an.y && an.x > an.z) {
+        h.u = (lp.z / total.z + 1.0) * 0.5;
+        h.v = (lp.y / total.y + 1.0) * 0.5;
+      } else if (an.y > an.z) {
+        h.u = (lp.x / total.x + 1.0) * 0.5;
+        h.v = (lp.z / total.z + 1.0) * 0.5;
+      } else {
+        h.u = (lp.x / total.x + 1.0) * 0.5;
+        h.v = (lp.y / total.y + 1.0) * 0.5;
+      }
+    }
+    h.p = point;
+    h.t = t;
+    h.mat = m;
+    h.prim = id;
+    set_face(h, ray, normal);
+    return true;
+  }
+  bool bbox(Aabb& b) const override {
+    Vec3 e = mobius ? Vec3(r + w + th, w + th, r + w + th) : half + Vec3(r, r, r);
+    b = {c - e, c + e};
+    return true;
+  }
+};
+
+// ray_triangle / ray_tri of the faceted solids (prism.rs:100-126,
+// pyramid.rs:29-50): Moller-Trumbore with cull |a| < 1e-10 and t > 1e-10.
+static bool solid_ray_triangle(const Ray& ray, const Vec3& v0, const Vec3& v1, const Vec3& v2, double& t, double& u,
+                               double& v) {
+  Vec3 edge1 = v1 - v0;
+  Vec3 edge2 = v2 - v0;
+  Vec3 h = ray.d.cross(edge2);
+  double a = edge1.dot(h);
+  if (std::fabs(a) < 1e-10) return false;
+  double f = 1.0 / a;
+  Vec3 s = ray.o - v0;
+  u = f * s.dot(h);
+  if (!(u >= 0.0 && u <= 1.0)) return false;
+  Vec3 q = s.cross(edge1);
+  v = f * ray.d.dot(q);
+  if (v < 0.0 || u + v > 1.0) return false;
+  t = f * edge2.dot(q);
+  return t > 1e-10;
+}
+
+// Prism (prism.rs:39-98) and Pyramid (pyramid.rs:52-111): a fixed list of
+// triangles, each with the normal the reference gives its face, tested in
+// order under the strict range t_min < t < closest; the box is the solid's.
+struct FacetSolid : Hittable {
+  struct Facet {
+    Vec3 a, b, c, n;
+  };
+  std::vector<Facet> facets;
+  Aabb box;
+  const Material* m;
+  int64_t id;  // id of the first facet; facet k reports id + k
+  bool hit(const Ray& ray, double tmin, double tmax, HitRecord& h) const override {
+    double closest = tmax;
+    int best = -1;
+    double bu = 0, bv = 0;
+    for (size_t k = 0; k < facets.size(); ++k) {
+      double t, u, v;
+      if (solid_ray_triangle(ray, facets[k].a, facets[k].b, facets[k].c, t, u, v) && t > tmin && t < closest) {
+        closest = t;
+        best = int(k);
+        bu = u;
+        bv = v;
+      }
+    }
+    if (best < 0) return false;
+    h.p = ray.at(closest);
+    h.t = closest;
+    h.u = bu;
+    h.v = bv;
+    h.mat = m;
+    h.prim = id + best;
+    set_face(h, ray, facets[best].n);
+    return true;
+  }
+  bool bbox(Aabb& b) const override {
+    b = box;
+    return true;
+  }
+};
+
+// Wedge (wedge.rs:8-127): bottom, back, left, right and ramp faces in that
+// order under the strict range t_min < t < closest.
+struct Wedge : Hittable {
+  Vec3 mn, mx;
+  const Material* m;
+  int64_t id;  // id of the bottom face; face k reports id + k
+  bool hit(const Ray& ray, double tmin, double tmax, HitRecord& h) const override {
+    double closest = tmax;
+    int best = -1;
+    Vec3 bn;
+    double bu = 0, bv = 0;
+    const double depth = mx.z - mn.z, height = mx.y - mn.y, width = mx.x - mn.x;
+    auto take = [&](int face, double t, const Vec3& n, double u, double v) {
+      closest = t;
+      best = face;
+      bn = n;
+      bu = u;
+      bv = v;
+    };
+    if (std::fabs(ray.d.y) > 1e-10) {  // bottom
+      double t = (mn.y - ray.o.y) / ray.d.y;
+      if (t > tmin && t < closest) {
+        Vec3 p = ray.at(t);
+        if (p.x >= mn.x && p.x <= mx.x && p.z >= mn.z && p.z <= mx.z)
+          take(0, t, Vec3(0.0, -1.0, 0.0), (p.x - mn.x) / width, (p.z - mn.z) / depth);
+      }
+    }
+    if (std::fabs(ray.d.z) > 1e-10) {  // back
+      double t = (mx.z - ray.o.z) / ray.d.z;
+      if (t > tmin && t < closest) {
+        Vec3 p = ray.at(t);
+        if (p.x >= mn.x && p.x <= mx.x && p.y >= mn.y && p.y <= mx.y)
+          take(1, t, Vec3(0.0, 0.0, 1.0), (p.x - mn.x) / width, (p.y - mn.y) / height);
+      }
+    }
+    for (int side = 0; side < 2; ++side) {  // left, right
+      if (!(std::fabs(ray.d.x) > 1e-10)) break;
+      double xs = side ? mx.x : mn.x;
+      double t = (xs - ray.o.x) / ray.d.x;
+      if (t > tmin && t < closest) {
+        Vec3 p = ray.at(t);
+        double ramp_y = (p.z - mn.z) / depth * height + mn.y;
+        if (p.z >= mn.z && p.z <= mx.z && p.y >= mn.y && p.y <= ramp_y)
+          take(2 + side, t, Vec3(side ? 1.0 : -1.0, 0.0, 0.0), (p.z - mn.z) / depth, (p.y - mn.y) / height);
+      }
+    }
+    double slope = height / depth;  // ramp
+    Vec3 normal = Vec3(0.0, 1.0, -slope).unit();
+    double denom = ray.d.dot(normal);
+    if (std::fabs(denom) > 1e-10) {
+      double t = (mn - ray.o).dot(normal) / denom;
+      if (t > tmin && t < closest) {
+        Vec3 p = ray.at(t);
+        if (p.x >= mn.x && p.x <= mx.x && p.z >= mn.z && p.z <= mx.z) {
+          double expected_y = (p.z - mn.z) / depth * height + mn.y;
+          if (std::fabs(p.y - expected_y) < 0.01) take(4, t, normal, (p.x - mn.x) / width, (p.z - mn.z) / depth);
+        }
+      }
+    }
+    if (best < 0) return false;
+    h.p = ray.at(closest);
+    h.t = closest;
+    h.u = bu;
+    h.v = bv;
+    h.mat = m;
+    h.prim = id + best;
+    set_face(h, ray, bn);
+    return true;
+  }
+  bool bbox(Aabb& b) const override {
+    b = {mn, mx};
+    return true;
+  }
+};
+
 struct Triangle : Hittable {  // triangle.rs:23-70 / obj.rs:260-319
   Vec3 v0, v1, v2;
   const Material* m;
@@ -1660,6 +1925,24 @@ HPtr make_leaf(const lmn_scene_desc& d, uint32_t i, const World& w, bool exact) 
       dk->id = id;
       return dk;
     }
+    case LMN_PRIM_ROUNDED_BOX:
+    case LMN_PRIM_MOBIUS: {
+      auto sd = std::make_unique<SdfTraced>();
+      sd->mobius = p.kind == LMN_PRIM_MOBIUS;
+      sd->exact = exact;
+      sd->c = v3(q);
+      if (sd->mobius) {
+        sd->r = q[3];
+        sd->w = q[4];
+        sd->th = q[5];
+      } else {
+        sd->half = v3(q + 3);
+        sd->r = q[6];
+      }
+      sd->m = m;
+      sd->id = id;
+      return sd;
+    }
     case LMN_PRIM_ELLIPSOID: {
       auto e = std::make_unique<Ellipsoid>();
       e->c = v3(q);
@@ -1790,7 +2073,31 @@ bool build_world(const lmn_scene_desc& d, bool exact, World& w) {
       return false;
     }
     HPtr obj;
-    if (j - i == 1) {
+    if (d.prims[i].kind == LMN_PRIM_FACET) {  // a prism or pyramid
+      auto fs = std::make_unique<FacetSolid>();
+      for (uint32_t k = i; k < j; ++k) {
+        const double* q = d.prims[k].p;
+        fs->facets.push_back({v3(q), v3(q + 3), v3(q + 6), v3(q + 9)});
+      }
+      // the solid's box: prism.rs:87-97 / pyramid.rs:99-110 span exactly its vertices
+      Vec3 lo = fs->facets[0].a, hi = lo;
+      for (auto& f : fs->facets)
+        for (const Vec3* v : {&f.a, &f.b, &f.c}) {
+          lo = Vec3(std::fmin(lo.x, v->x), std::fmin(lo.y, v->y), std::fmin(lo.z, v->z));
+          hi = Vec3(std::fmax(hi.x, v->x), std::fmax(hi.y, v->y), std::fmax(hi.z, v->z));
+        }
+      fs->box = {lo, hi};
+      fs->m = w.mats[d.prims[i].material].get();
+      fs->id = int64_t(i);
+      obj = std::move(fs);
+    } else if (d.prims[i].kind == LMN_PRIM_WEDGE_FACE) {
+      auto wg = std::make_unique<Wedge>();
+      wg->mn = v3(d.prims[i].p);
+      wg->mx = v3(d.prims[i].p + 3);
+      wg->m = w.mats[d.prims[i].material].get();
+      wg->id = int64_t(i);
+      obj = std::move(wg);
+    } else if (j - i == 1) {
       obj = make_leaf(d, i, w, exact);
     } else {
       auto list = std::make_unique<HittableList>();

@@ -44,6 +44,7 @@ SCENES = {
     "spheres": (lambda: workloads.random_spheres(32, 18, 4, 50, grid=11), None, [-12, 0.01, -12], [12, 3, 12]),
     "mesh": (lambda: workloads.batch_scene(3, seed=5)[0], 12, [-2.5, 0.05, -2.5], [2.5, 3.0, 3.0]),
     "quadrics": (lambda: workloads.quadrics_scene(), None, [-4, 0.05, -3], [4, 3, 3]),
+    "traced": (lambda: workloads.traced_scene(), None, [-3.5, 0.05, -2.5], [3.5, 2.0, 2.5]),
 }
 
 
