@@ -347,7 +347,8 @@ LMN_HD void bvh_walk(const SceneView<R>& S, const typename NodeSel<R>::type* nod
 // Ordered walk of the FP32 4-wide BVH.  A node's (up to) four child boxes
 // are tested together; the hits are sorted by entry distance, the nearest is
 // visited next and the others pushed farthest first, each with its entry
-// distance so that a pop can drop it once a closer hit is known.
+// distance so that a pop can drop it once a closer hit is known (a
+// visibility query, ANY, pushes the child alone).
 template <uint32_t FEAT, bool ANY, class Stack>
 LMN_HD void bvh4_walk(const SceneView<float>& S, V3<float> o, V3<float> d, V3<float> inv, float tmin,
                       float& closest, uint32_t& best, bool& found, Stack& st, uint32_t& nf, uint32_t& pt,
@@ -395,18 +396,13 @@ LMN_HD void bvh4_walk(const SceneView<float>& S, V3<float> o, V3<float> d, V3<fl
       LMN_CSWAP(1, 2)
 #undef LMN_CSWAP
       if (tk[0] != INFINITY) {
-        if (tk[3] != INFINITY) {
-          st.push(ch[3]);
-          st.push(bits_of(tk[3]));
-        }
-        if (tk[2] != INFINITY) {
-          st.push(ch[2]);
-          st.push(bits_of(tk[2]));
-        }
-        if (tk[1] != INFINITY) {
-          st.push(ch[1]);
-          st.push(bits_of(tk[1]));
-        }
+        // a visibility query returns at its first hit, so its range never
+        // shrinks and an entry needs no distance: one word instead of two
+        for (int k = 3; k >= 1; --k)
+          if (tk[k] != INFINITY) {
+            st.push(ch[k]);
+            if (!ANY) st.push(bits_of(tk[k]));
+          }
         ref = ch[0];
         continue;
       }
@@ -423,13 +419,20 @@ LMN_HD void bvh4_walk(const SceneView<float>& S, V3<float> o, V3<float> d, V3<fl
       }
     }
     bool next = false;
-    while (!st.empty()) {
-      float te = float_of(st.pop());
-      uint32_t r = st.pop();
-      if (te <= closest) {
-        ref = r;
+    if (ANY) {
+      if (!st.empty()) {
+        ref = st.pop();
         next = true;
-        break;
+      }
+    } else {
+      while (!st.empty()) {
+        float te = float_of(st.pop());
+        uint32_t r = st.pop();
+        if (te <= closest) {
+          ref = r;
+          next = true;
+          break;
+        }
       }
     }
     if (!next) return;
@@ -548,7 +551,9 @@ LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_r
     if ((FEAT & FEAT_MARCH) && is_marched_kind(kind)) {
       n = cvt<R>(marched_normal_fp64(kind, p64, q, S.exact_marched != 0));
       marched_uv(kind, p64, q, u64, v64);
-    } else if (is_quadric_kind(kind)) {
+    } else if ((FEAT & FEAT_MARCH) && is_quadric_kind(kind)) {
+      // only the second BVH holds these, and only kernels with FEAT_MARCH
+      // walk it: the main traversal's kernels carry none of this (out of line) code
       V3<double> n64;
       quadric_surface(kind, p64, oo64, dd64, q, n64, u64, v64);
       n = cvt<R>(n64);
