@@ -753,6 +753,8 @@ void fill_camera(const lmn_camera& c, CameraT<R>& o) {
   o.vertical = v(c.vertical);
   o.u = v(c.u);
   o.v = v(c.v);
+  double rel[3] = {c.lower_left[0] - c.origin[0], c.lower_left[1] - c.origin[1], c.lower_left[2] - c.origin[2]};
+  o.ll_rel = v(rel);
   o.lens_radius = R(c.lens_radius);
 }
 

@@ -42,6 +42,7 @@ struct BackgroundT {
 template <class R>
 struct CameraT {
   V3<R> origin, lower_left, horizontal, vertical, u, v;
+  V3<R> ll_rel;  // lower_left - origin, formed in FP64
   R lens_radius;
 };
 
@@ -896,7 +897,10 @@ template <class R> LMN_HD void camera_ray(const CameraT<R>& C, R s, R t, Rng& rn
   V3<R> rd = random_in_unit_disk<R>(rng) * C.lens_radius;
   V3<R> off = C.u * rd.x + C.v * rd.y;
   o = C.origin + off;
-  d = C.lower_left + C.horizontal * s + C.vertical * t - C.origin - off;
+  if constexpr (sizeof(R) == 4)  // lower_left - origin cancels: |origin| / focus_dist ulps in FP32
+    d = C.ll_rel + C.horizontal * s + C.vertical * t - off;
+  else  // the reference's order (src/camera.rs:94-100)
+    d = C.lower_left + C.horizontal * s + C.vertical * t - C.origin - off;
   (void)rng.next<R>();
 }
 

@@ -200,3 +200,18 @@ def test_traced_kinds_image_parity(gpu):
         rel = np.linalg.norm(r.rows - ref) / np.linalg.norm(ref)
         print(path, "3sigma pass", frac, "relRMSE", rel, r.stats, ost)
         assert frac >= 0.99
+
+
+def test_fp32_camera_rays_keep_the_light_edges():
+    # FP32 against the FP64 kernel with common random numbers on the Cornell
+    # box (camera 800 units from the scene, focus distance 1): forming the
+    # direction as lower_left - origin in FP32 cancelled to ~1e-4 relative and
+    # moved the light's edges by a thirtieth of a pixel (relRMSE 9.8e-3 at 16
+    # spp, one light sample more or less in every edge pixel); formed in FP64
+    # on the host, the error is that of rays that meet edges (6.1e-4)
+    cfg, cam, world = lm.load_scene(workloads.cornell_box(256, 256, 16, 8, 42))
+    a = lm.render(cfg, cam, world, path="wavefront").rows.astype(np.float64)
+    b = lm.render(cfg, cam, world, path="plain64").rows.astype(np.float64)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("relRMSE(FP32, FP64)", rel)
+    assert rel < 2e-3
