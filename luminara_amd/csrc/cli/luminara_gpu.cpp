@@ -262,7 +262,70 @@ void usage() {
                "      --save-normals FILE normal pass image\n"
                "      --save-albedo FILE  albedo pass image\n"
                "      --save-json FILE    render statistics\n"
-               "      --benchmark         the scene at 400x225, 64 spp; prints Mrays/s\n");
+               "      --benchmark         the demo scene at 400x225, 64 spp; prints Mrays/s\n");
+}
+
+// scene::demo_scene (src/scene.rs): ground sphere r = 1000, the glass,
+// Lambertian and metal r = 1 spheres, a 10 x 10 grid of r = 0.2 spheres (80 %
+// Lambertian albedo U*U, 15 % metal U[0.5, 1] fuzz U[0, 0.5], 5 % glass 1.5)
+// clear of (4, 0.2, 0); camera (13, 2, 3) -> origin, vfov 20, aperture 0.1,
+// focus 10; 800 x 450, 64 spp, depth 50, seed 31337, default sky, no NEE
+// lights.  The reference draws the grid from an OS-seeded generator, so each
+// of its runs renders another instance; this one draws it from a fixed
+// SplitMix64 stream, in the reference's draw order.
+std::string demo_scene_toml() {
+  uint64_t state = 0x9e3779b97f4a7c15ull;
+  auto uniform = [&state]() {
+    uint64_t z = (state += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return double(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  std::ostringstream t;
+  t.precision(17);
+  t << "[render]\nwidth = 800\nheight = 450\nsamples = 64\nmax_depth = 50\nseed = 31337\n\n"
+    << "[camera]\nlook_from = [13.0, 2.0, 3.0]\nlook_at = [0.0, 0.0, 0.0]\nvup = [0.0, 1.0, 0.0]\n"
+    << "vfov = 20.0\naperture = 0.1\nfocus_dist = 10.0\n\n";
+  auto sphere = [&t](double x, double y, double z, double r, const std::string& m) {
+    t << "[[sphere]]\ncenter = [" << x << ", " << y << ", " << z << "]\nradius = " << r << "\nmaterial = " << m
+      << "\n\n";
+  };
+  auto color = [](double r, double g, double b) {
+    std::ostringstream c;
+    c.precision(17);
+    c << "[" << r << ", " << g << ", " << b << "]";
+    return c.str();
+  };
+  sphere(0.0, -1000.0, 0.0, 1000.0, "{ type = \"lambertian\", color = [0.5, 0.5, 0.5] }");
+  sphere(0.0, 1.0, 0.0, 1.0, "{ type = \"dielectric\", refraction_index = 1.5 }");
+  sphere(-4.0, 1.0, 0.0, 1.0, "{ type = \"lambertian\", color = [0.4, 0.2, 0.1] }");
+  sphere(4.0, 1.0, 0.0, 1.0, "{ type = \"metal\", color = [0.7, 0.6, 0.5], fuzz = 0.0 }");
+  for (int a = -5; a < 5; ++a)
+    for (int b = -5; b < 5; ++b) {
+      double choose = uniform();
+      double cx = a + 0.9 * uniform(), cz = b + 0.9 * uniform();
+      double dx = cx - 4.0, dz = cz;
+      if (std::sqrt(dx * dx + dz * dz) <= 0.9) continue;
+      std::string m;
+      if (choose < 0.8) {
+        double c1[3], c2[3];
+        for (double& c : c1) c = uniform();
+        for (double& c : c2) c = uniform();
+        m = "{ type = \"lambertian\", color = " + color(c1[0] * c2[0], c1[1] * c2[1], c1[2] * c2[2]) + " }";
+      } else if (choose < 0.95) {
+        double c[3];
+        for (double& v : c) v = 0.5 + 0.5 * uniform();
+        std::ostringstream f;
+        f.precision(17);
+        f << uniform() * 0.5;
+        m = "{ type = \"metal\", color = " + color(c[0], c[1], c[2]) + ", fuzz = " + f.str() + " }";
+      } else {
+        m = "{ type = \"dielectric\", refraction_index = 1.5 }";
+      }
+      sphere(cx, 0.2, cz, 0.2, m);
+    }
+  return t.str();
 }
 
 }  // namespace
@@ -274,20 +337,37 @@ int main(int argc, char** argv) {
     return 0;
   }
   const auto t_start = std::chrono::steady_clock::now();
-  if (cli.scene.empty()) die("no scene file given (luminara-gpu scene.toml [options])");
-  bool ok = false;
-  std::string text = read_file(cli.scene, ok);
-  if (!ok) die("cannot read scene file '" + cli.scene + "'");
-  std::string base = cli.scene.find('/') == std::string::npos ? "." : cli.scene.substr(0, cli.scene.rfind('/'));
+  std::string text, base = ".";
+  if (cli.scene.empty()) {  // src/main.rs:198-201
+    std::fprintf(stderr, "No scene file provided, rendering demo scene...\n");
+    text = demo_scene_toml();
+  } else {
+    bool ok = false;
+    text = read_file(cli.scene, ok);
+    if (!ok) die("cannot read scene file '" + cli.scene + "'");
+    base = cli.scene.find('/') == std::string::npos ? "." : cli.scene.substr(0, cli.scene.rfind('/'));
+  }
   lmn_scene_desc* desc = nullptr;
   if (lmn_load_scene(text.c_str(), base.c_str(), &desc) != 0)
-    die("invalid scene '" + cli.scene + "': " + lmn_last_error());
+    die("invalid scene '" + (cli.scene.empty() ? std::string("demo") : cli.scene) + "': " + lmn_last_error());
+  if (cli.benchmark) {  // src/main.rs:688-706: the demo scene, whatever scene was given
+    lmn_desc_free(desc);
+    desc = nullptr;
+    if (lmn_load_scene(demo_scene_toml().c_str(), ".", &desc) != 0) die(std::string("demo scene: ") + lmn_last_error());
+    std::fprintf(stderr, "Benchmark: rendering demo scene at 400x225 @ 64 spp...\n");
+  }
   lmn_render_config rc = desc->config;
   lmn_scene* scene = nullptr;
   int rcode = lmn_scene_create(desc, &scene);
   lmn_desc_free(desc);
   if (rcode != 0) die(std::string("scene upload failed: ") + lmn_last_error());
 
+  if (cli.benchmark) {  // the benchmark takes none of the overrides (src/main.rs:690-694)
+    Cli b;
+    b.benchmark = true;
+    b.quiet = true;
+    cli = b;
+  }
   if (cli.width >= 0) rc.width = uint32_t(cli.width);
   if (cli.height >= 0) rc.height = uint32_t(cli.height);
   if (cli.samples >= 0) rc.samples = uint32_t(cli.samples);

@@ -89,8 +89,20 @@ def test_cli_preview_and_benchmark(gpu, tmp_path):
     png = tmp_path / "p.png"
     subprocess.run([build.CLI, str(scene), "-p", "-q", "-o", str(png)], check=True, timeout=120)
     assert read_png(png).shape == (16, 16, 4)
-    r = subprocess.run([build.CLI, str(scene), "--benchmark"], capture_output=True, text=True, timeout=120)
+    # the benchmark renders the demo scene at 400x225, 64 spp, whatever scene is given (src/main.rs:688-706)
+    r = subprocess.run([build.CLI, str(scene), "--benchmark", "-s", "4"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "Benchmark result:" in r.stderr and "Mrays/s" in r.stderr
+    assert "rendering demo scene at 400x225 @ 64 spp" in r.stderr and "5760000 primary rays" in r.stderr
+
+
+def test_cli_demo_scene(gpu, tmp_path):
+    png = tmp_path / "demo.png"
+    r = subprocess.run([build.CLI, "-w", "80", "-H", "45", "-s", "4", "-q", "-o", str(png)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "No scene file provided, rendering demo scene..." in r.stderr
+    img = read_png(png)
+    assert img.shape == (45, 80, 4) and img[..., :3].std() > 10   # sky, ground and spheres
 
 
 def test_cli_passes_fog_and_tone_flags(gpu, tmp_path):
