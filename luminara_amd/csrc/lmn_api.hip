@@ -182,25 +182,31 @@ int lmn_desc_add_triangles(lmn_scene_desc* desc, const double* verts, uint64_t n
   if (!desc || (n_tris && (!verts || !tris))) return set_err(LMN_ERR_INVALID, "lmn_desc_add_triangles: null argument");
   auto* od = reinterpret_cast<OwnedDesc*>(desc);
   if (od->prims.size() + n_tris >= (1ull << 31)) return set_err(LMN_ERR_INVALID, "too many primitives");
+  // every index is checked before the description changes: a failed call
+  // leaves it (and its exported views) exactly as they were
+  for (uint64_t t = 0; t < n_tris; ++t) {
+    if ((materials ? materials[t] : material) >= od->materials.size())
+      return set_err(LMN_ERR_INVALID, "lmn_desc_add_triangles: material out of range");
+    for (int c = 0; c < 3; ++c)
+      if (tris[t * 3 + c] >= n_verts) return set_err(LMN_ERR_INVALID, "lmn_desc_add_triangles: vertex index out of range");
+  }
   try {
     od->prims.reserve(od->prims.size() + n_tris);
-    for (uint64_t t = 0; t < n_tris; ++t) {
-      uint32_t m = materials ? materials[t] : material;
-      if (m >= od->materials.size()) return set_err(LMN_ERR_INVALID, "lmn_desc_add_triangles: material out of range");
-      lmn_prim p{};
-      p.kind = LMN_PRIM_TRIANGLE;
-      p.material = m;
-      p.transform = -1;
-      p.object = od->n_objects++;
-      for (int c = 0; c < 3; ++c) {
-        uint32_t vi = tris[t * 3 + c];
-        if (vi >= n_verts) return set_err(LMN_ERR_INVALID, "lmn_desc_add_triangles: vertex index out of range");
-        for (int k = 0; k < 3; ++k) p.p[c * 3 + k] = verts[size_t(vi) * 3 + k];
-      }
-      od->prims.push_back(p);
-    }
   } catch (const std::bad_alloc&) {
+    od->sync();
     return set_err(LMN_ERR_NOMEM, "out of host memory");
+  }
+  for (uint64_t t = 0; t < n_tris; ++t) {
+    lmn_prim p{};
+    p.kind = LMN_PRIM_TRIANGLE;
+    p.material = materials ? materials[t] : material;
+    p.transform = -1;
+    p.object = od->n_objects++;
+    for (int c = 0; c < 3; ++c) {
+      const uint32_t vi = tris[t * 3 + c];
+      for (int k = 0; k < 3; ++k) p.p[c * 3 + k] = verts[size_t(vi) * 3 + k];
+    }
+    od->prims.push_back(p);  // capacity reserved above: no reallocation
   }
   od->sync();
   return LMN_OK;

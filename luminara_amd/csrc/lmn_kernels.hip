@@ -286,7 +286,9 @@ __global__ __launch_bounds__(BS) void k_march_collect(BatchDev B, PathSoA cur, H
       uint32_t slot = k == 0u ? off0++ : (k == 1u ? off1++ : off2++);
       if (slot < q.cap) q.cand[size_t(k) * q.cap + slot] = fits ? make_uint2(i, c[j] & 0x3fffffffu) : make_uint2(MARCH_SKIP, 0u);
     }
-    if (over || !fits) q.ovf[atomicAdd(&ctr->n_ovf[SHADOW ? 1 : 0], 1u)] = i;
+    // only real rays with candidates that did not fit (or too many) go to the fallback;
+    // lanes past n and rays without candidates have nothing to lose
+    if (i < n && (over || (nc != 0u && !fits))) q.ovf[atomicAdd(&ctr->n_ovf[SHADOW ? 1 : 0], 1u)] = i;
     __syncthreads();  // cnt / base are rewritten next round
   }
 }

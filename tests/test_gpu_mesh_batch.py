@@ -165,11 +165,13 @@ def _torus_row(n):
     return body
 
 
-@pytest.mark.parametrize("pool", [0, 4096])
+@pytest.mark.parametrize("pool", [0, 4096, 4000])
 def test_marched_overflow_fallback(gpu, pool):
     # rays with more candidates than the per-ray limit, and (small pool)
     # candidates beyond the queue capacity, go to the fallback kernel; the
-    # image must still be the plain FP32 kernel's
+    # image must still be the plain FP32 kernel's.  A pool that is not a
+    # multiple of the workgroup size leaves lanes past the pool in the last
+    # workgroup: they must stay off the fallback list.
     cfg, cam, world = lm.load_scene(_torus_row(16))
     cfg.samples_per_pixel = 4
     a = lm.render(cfg, cam, world, path="wavefront", intersect_mode=_abi.ISECT_EXACT, pool_paths=pool)

@@ -248,3 +248,28 @@ def test_cornell_example_geometry():
     assert (cfg.width, cfg.height, cfg.samples_per_pixel) == (600, 600, 256)
     assert len(world.prims) == 18
     assert len(world.lights) >= 1
+
+
+def test_bad_mesh_leaves_the_description_unchanged():
+    # a mesh with an out-of-range vertex or material index is refused before
+    # anything is appended: the description keeps its prims (readable, same
+    # values), its count and object numbering, and takes a good mesh after
+    _, _, world = lm.load_scene(workloads.cornell_box(16, 16, 1, 2, 42))
+    before = [(p.kind, p.material, p.object, tuple(p.p)) for p in world.prims]
+    n_obj = max(p.object for p in world.prims) + 1
+    verts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float64)
+    big = np.zeros((200000, 3), dtype=np.uint32)  # would force a reallocation
+    big[-1] = [0, 1, 3]  # vertex 3 does not exist
+    with pytest.raises(lm.LuminaraError):
+        world.add_triangles(verts, big, material=0)
+    bad_mat = np.zeros((1000, 3), dtype=np.uint32)
+    bad_mat[:] = [0, 1, 2]
+    with pytest.raises(lm.LuminaraError):
+        world.add_triangles(verts, bad_mat, material=10 ** 6)
+    after = [(p.kind, p.material, p.object, tuple(p.p)) for p in world.prims]
+    assert after == before
+    world.add_triangles(verts, np.array([[0, 1, 2]], dtype=np.uint32), material=0)
+    prims = world.prims
+    assert len(prims) == len(before) + 1
+    assert prims[-1].kind == _abi.PRIM_TRIANGLE and prims[-1].object == n_obj
+    np.testing.assert_array_equal(np.array(prims[-1].p[:9]).reshape(3, 3), verts)
