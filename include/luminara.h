@@ -495,6 +495,21 @@ int lmn_post_device(const float* d_hdr, uint32_t width, uint32_t height, const l
                     const float* d_fog_depth, uint8_t* d_rgba, float* d_hdr_exposed, double* exposure_used,
                     void* stream);
 
+/* Allocation-free variant of lmn_post_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as lmn_post_device, bit for bit; the
+ * automatic exposure's scratch (2 + 1024 doubles, 8.2 KB, independent of the
+ * frame size, so no frame is too large for it) is a static device array of
+ * the library, one per GPU, held until the process ends.  stream NULL = the
+ * default (null) stream, not the library's stream.  Calls that overlap on one
+ * GPU share that array and race: serialise them.  The first call on a device
+ * looks the array's address up (hipGetSymbolAddress) and uploads the sRGB
+ * tables into constant memory; no call allocates. */
+int lmn_post_device_noalloc(const float* d_hdr, uint32_t width, uint32_t height, const lmn_post_params* post,
+                            const float* d_fog_depth, uint8_t* d_rgba, float* d_hdr_exposed, double* exposure_used,
+                            void* stream);
+
 /* Radiance RGBE pixels (src/main.rs:964-986): n_pixels*3 floats in, n_pixels*4
  * bytes out (the writer prepends the header "#?RADIANCE\nFORMAT=32-bit_rle_rgbe
  * \n\n-Y H +X W\n"). */

@@ -201,6 +201,8 @@ SIGNATURES = {
                            C.c_void_p, C.POINTER(C.c_double)]),
     "lmn_post_device": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(PostParams), C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
+    "lmn_post_device_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(PostParams), C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]),
     "lmn_encode_rgbe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "lmn_encode_rgbe_device": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
     "lmn_render_image": (C.c_int, [C.c_void_p, C.POINTER(RenderParams), C.POINTER(PostParams), C.POINTER(Image),

@@ -1467,10 +1467,11 @@ int aov_enqueue(const lmn_scene* scene, const lmn_render_params* params, const F
   return LMN_OK;
 }
 
-// Enqueues the post chain; with auto-exposure, scratch holds the log-average
-// luminance and the applied exposure when the stream reaches the end.
-int post_enqueue(const float* d_hdr, uint64_t n, const lmn_post_params* post, const float* d_fog, uint8_t* d_rgba,
-                 float* d_hdr_exposed, DevBuf& scratch, hipStream_t st) {
+// Enqueues the post chain; with auto-exposure, d_scratch (2 + kLogLumBlocks
+// doubles) holds the log-average luminance and the applied exposure when the
+// stream reaches the end.  Allocates nothing.
+int post_enqueue_into(const float* d_hdr, uint64_t n, const lmn_post_params* post, const float* d_fog,
+                      uint8_t* d_rgba, float* d_hdr_exposed, double* d_scratch, hipStream_t st) {
   if (!post) return set_err(LMN_ERR_INVALID, "lmn_post: null post parameters");
   if (post->tone_map > LMN_TONE_NONE) return set_err(LMN_ERR_INVALID, "lmn_post: unknown tone map");
   const bool fog = post->depth_fog > 0.0;
@@ -1486,9 +1487,9 @@ int post_enqueue(const float* d_hdr, uint64_t n, const lmn_post_params* post, co
   A.rgba = d_rgba;
   A.hdr_exposed = d_hdr_exposed;
   if (post->auto_exposure) {
-    HIP_TRY(scratch.ensure((2 + kLogLumBlocks) * sizeof(double)));
-    launch_log_luminance(d_hdr, n, post->exposure, scratch.as<double>(), st);
-    A.exposure_dev = scratch.as<double>();
+    if (!d_scratch) return set_err(LMN_ERR_INVALID, "lmn_post: automatic exposure without scratch");
+    launch_log_luminance(d_hdr, n, post->exposure, d_scratch, st);
+    A.exposure_dev = d_scratch;
   }
   if (d_rgba || d_hdr_exposed) {
     HIP_TRY(ensure_post_tables());
@@ -1498,18 +1499,31 @@ int post_enqueue(const float* d_hdr, uint64_t n, const lmn_post_params* post, co
   return LMN_OK;
 }
 
-int read_exposure(const lmn_post_params* post, const DevBuf& scratch, hipStream_t st, double& used,
-                  double* log_average = nullptr) {
+// The same with the scratch in a library-allocated buffer.
+int post_enqueue(const float* d_hdr, uint64_t n, const lmn_post_params* post, const float* d_fog, uint8_t* d_rgba,
+                 float* d_hdr_exposed, DevBuf& scratch, hipStream_t st) {
+  if (!post) return set_err(LMN_ERR_INVALID, "lmn_post: null post parameters");
+  if (post->auto_exposure) HIP_TRY(scratch.ensure((2 + kLogLumBlocks) * sizeof(double)));
+  return post_enqueue_into(d_hdr, n, post, d_fog, d_rgba, d_hdr_exposed, scratch.as<double>(), st);
+}
+
+int read_exposure_from(const lmn_post_params* post, const double* d_scratch, hipStream_t st, double& used,
+                       double* log_average = nullptr) {
   used = post->exposure;
   if (log_average) *log_average = 0.0;
   if (post->auto_exposure) {
     double two[2];
-    HIP_TRY(hipMemcpyAsync(two, scratch.p, sizeof two, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(two, d_scratch, sizeof two, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     used = two[1];
     if (log_average) *log_average = two[0];
   }
   return LMN_OK;
+}
+
+int read_exposure(const lmn_post_params* post, const DevBuf& scratch, hipStream_t st, double& used,
+                  double* log_average = nullptr) {
+  return read_exposure_from(post, scratch.as<double>(), st, used, log_average);
 }
 
 int to_host(void* host, const DevBuf& d, size_t bytes) {
@@ -1560,6 +1574,28 @@ int lmn_post_device(const float* d_hdr, uint32_t width, uint32_t height, const l
   if (rc == LMN_OK) rc = post_enqueue(d_hdr, uint64_t(width) * height, post, d_fog_depth, d_rgba, d_hdr_exposed, scratch, st);
   double used = 0.0;
   if (rc == LMN_OK) rc = read_exposure(post, scratch, st, used);  // waits for the stream with auto-exposure
+  if (rc != LMN_OK) return rc;
+  if (!stream) HIP_TRY(hipStreamSynchronize(st));
+  if (exposure_used) *exposure_used = used;
+  return LMN_OK;
+}
+
+int lmn_post_device_noalloc(const float* d_hdr, uint32_t width, uint32_t height, const lmn_post_params* post,
+                            const float* d_fog_depth, uint8_t* d_rgba, float* d_hdr_exposed, double* exposure_used,
+                            void* stream) {
+  if (!d_hdr || !post || width == 0 || height == 0) return set_err(LMN_ERR_INVALID, "lmn_post: bad argument");
+  // the default stream when none is given: the library's own stream would be
+  // created (and its pinned counters allocated) on first use
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double* d_scratch = nullptr;
+  size_t cap = 0;
+  if (post->auto_exposure) {
+    HIP_TRY(noalloc_post_scratch(&d_scratch, &cap));
+    if (cap < 2 + size_t(kLogLumBlocks)) return set_err(LMN_ERR_INVALID, "lmn_post_device_noalloc: scratch too small");
+  }
+  int rc = post_enqueue_into(d_hdr, uint64_t(width) * height, post, d_fog_depth, d_rgba, d_hdr_exposed, d_scratch, st);
+  double used = 0.0;
+  if (rc == LMN_OK) rc = read_exposure_from(post, d_scratch, st, used);  // waits for the stream with auto-exposure
   if (rc != LMN_OK) return rc;
   if (!stream) HIP_TRY(hipStreamSynchronize(st));
   if (exposure_used) *exposure_used = used;

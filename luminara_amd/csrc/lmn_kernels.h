@@ -223,6 +223,9 @@ struct PostArgs {
 constexpr uint32_t kLogLumBlocks = 1024;
 // Uploads the sRGB quantisation thresholds to the current device (once).
 hipError_t ensure_post_tables();
+// Device address of the allocation-free post chain's scratch on the current
+// device (2 + kLogLumBlocks doubles, looked up once per device; no allocation).
+hipError_t noalloc_post_scratch(double** out, size_t* doubles);
 void launch_post(const PostArgs& A, hipStream_t s);
 // scratch: 2 + kLogLumBlocks doubles; scratch[0] = log-average luminance,
 // scratch[1] = exposure * key factor.

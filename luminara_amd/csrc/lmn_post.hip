@@ -21,6 +21,13 @@
 
 #pragma clang fp contract(off)
 
+// Scratch of the allocation-free post chain (lmn_post_device_noalloc): the
+// automatic exposure's per-block partial sums and its two results, in the
+// layout launch_log_luminance expects.  Every launch writes all of it before
+// reading it, so nothing needs clearing between calls.  Host code reaches it
+// only through the per-device address from hipGetSymbolAddress below.
+static __device__ __align__(256) double scratch[2 + lmn::kLogLumBlocks];
+
 namespace lmn {
 namespace {
 
@@ -269,6 +276,30 @@ std::mutex g_table_mu;
 std::vector<int> g_table_devices;
 
 }  // namespace
+
+namespace {
+constexpr int kMaxNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_scratch[kMaxNoAllocDevices] = {};
+}  // namespace
+
+hipError_t noalloc_post_scratch(double** out, size_t* doubles) {
+  *out = nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxNoAllocDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(g_noalloc_mu);
+  if (!g_noalloc_scratch[dev]) {  // once per device, with that device current
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_noalloc_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_noalloc_scratch[dev];
+  if (doubles) *doubles = sizeof(scratch) / sizeof(double);
+  return hipSuccess;
+}
 
 hipError_t ensure_post_tables() {
   static const SrgbTable table;
