@@ -36,7 +36,8 @@ def log(*a):
 def measured_traffic(workload, kernel="k_extend"):
     """HBM-side bytes per launch of `kernel` on `workload`, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
-    calibration recorded there), or None when no summary covers it: PMC
+    calibration recorded there), with the summary's path and its record (L2
+    hit rate, VALU lane use), or Nones when no summary covers it: PMC
     counters need a profiler run of their own and cannot be read here."""
     import glob
     import re
@@ -46,7 +47,7 @@ def measured_traffic(workload, kernel="k_extend"):
             continue
         rec = json.loads(m.group(1))
         if rec.get("workload") == workload and rec.get("kernel") == kernel:
-            return rec["bytes_per_launch"], os.path.relpath(path, ROOT), rec.get("l2_hit_rate")
+            return rec["bytes_per_launch"], os.path.relpath(path, ROOT), rec
     return None, None, None
 
 
@@ -214,53 +215,66 @@ def image_leg(scenes, worlds, frame, n_pix, reps):
                     "mrays_per_s": n_pix / (aov_ms / 1e3) / 1e6}}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batch64")
-    ap.add_argument("--scenes", type=int, default=64)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 kernel comparison")
-    ap.add_argument("--no-count", action="store_true", help="skip the instrumented traversal count")
-    ap.add_argument("--cpu-seconds", type=float, default=16.0)
-    ap.add_argument("--stack-lds", type=int, default=0)
-    ap.add_argument("--pool", type=int, default=0)
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering adjoint leg")
-    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults or cache)")
-    ap.add_argument("--no-images", action="store_true", help="skip the post chain / RGBE / AOV timings")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a device-to-device copy of MIB MiB (5 times): the reference kernel that "
-                         "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
-    args = ap.parse_args()
-    if args.copy_calibration:
-        import torch
-        n = args.copy_calibration << 18  # float32 elements
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}))
-        return
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only: every GPU box here has a single "
-                                   "MI355X; run with --gpus 1"}))
-        sys.exit(2)
+def host_cores():
+    """Cores this process may run on: its CPU affinity, capped by a cgroup v2
+    CPU quota when one is set (a container's share of the host) and by
+    OMP_NUM_THREADS when the job's scheduler sets it."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(float(quota) / float(period))))
+    except (OSError, ValueError):
+        pass
+    try:  # a job's CPU share set by its scheduler (the thread-pool convention)
+        n = min(n, max(1, int(os.environ["OMP_NUM_THREADS"])))
+    except (KeyError, ValueError):
+        pass
+    return n
 
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def counter_limiter(rec, hbm_frac):
+    """What holds the kernel below its HBM roofline, read off the committed
+    PMC counters: HBM-side traffic near peak means bandwidth, otherwise the
+    VALU lane use tells divergence from plain latency."""
+    if hbm_frac is None:
+        return None
+    if hbm_frac >= 0.6:
+        return "hbm bandwidth"
+    lanes = rec.get("valu_lane_util") if rec else None
+    if lanes is not None and lanes < 0.5:
+        return f"latency and lane divergence (HBM-side {hbm_frac:.1%} of peak, VALU lanes {lanes:.0%} busy)"
+    return f"latency (HBM-side {hbm_frac:.1%} of peak)"
+
+
+def measure(wl_name, scenes, desc, args, want_images, tag):
+    """Times one workload: the FP32 batched wavefront (graph launches; a
+    second region with stage events), the extend kernel's roofline, the plain
+    FP64 kernel's builds and the FP32-vs-FP64 error."""
     import numpy as np
     import torch
     import luminara_amd as lm
     from luminara_amd import _abi
 
     t0 = time.time()
-    wl_name, scenes, desc = build_workload(args.workload, args.scenes)
     worlds = [w for _, w in scenes]
     nodes = 0
     for w in worlds:
         nodes += w.bvh_info()["nodes"]
-    log(f"[bench] {wl_name}: {desc}, {nodes} BVH nodes, scenes built and uploaded in {time.time() - t0:.1f}s")
+    log(f"[bench:{tag}] {wl_name}: {desc}, {nodes} BVH nodes, scenes built and uploaded in {time.time() - t0:.1f}s")
     # every scene's framebuffer is a view into one device buffer, so the post
     # chain can run over the whole batch's pixels in one launch
     shapes = [lm.output_shape(c) for c, _ in scenes]
@@ -282,7 +296,7 @@ def main():
     tuning = None
     if not args.pool and not args.stack_lds and not args.no_tune:
         tuning = lm.autotune(worlds, params("wavefront"), ptrs)
-        log(f"[bench] autotune: {tuning}")
+        log(f"[bench:{tag}] autotune: {tuning}")
     # Timed region 1 (`value`): the product path, blocks of iterations
     # replayed as hipGraphs.  Timed region 2: the same renders launched
     # directly with HIP events between the stages (events recorded by graph
@@ -310,7 +324,7 @@ def main():
     value = paths * args.steps / elapsed / 1e6
     value_direct = paths * args.steps / elapsed_direct / 1e6
     imgs32 = [o.double().cpu().numpy() for o in outs]
-    log(f"[bench] wavefront FP32: {ms_per_step:.1f} ms/step, {value:.1f} Mpaths/s (graph launches); "
+    log(f"[bench:{tag}] wavefront FP32: {ms_per_step:.1f} ms/step, {value:.1f} Mpaths/s (graph launches); "
         f"{value_direct:.1f} Mpaths/s launched directly with stage events; stats {stats[-1]}")
 
     ext_ms = sum(s["extend_ms"] for s in stats) / args.steps
@@ -319,83 +333,169 @@ def main():
     if not args.no_count:
         # algorithmic bytes of the extend kernel: node and primitive fetches
         # counted by an instrumented run of the same traversal on the same
-        # work, plus per-ray queue traffic; time from HIP events
+        # work, plus the kernel's own reads and writes per ray; time from HIP
+        # events on its stream
         counts = lm.profile_extend(worlds, params("wavefront"), ptrs)
-        io_bytes = 40  # per ray: origin+beta, direction+beta (32 B) and scene id read, hit (8 B) written
-        alg_bytes = (counts["node_fetches"] * counts["node_bytes"] + counts["prim_tests"] * counts["prim_bytes"]
-                     + counts["rays"] * io_bytes)
+        rays, hits = counts["rays"], counts["hits"]
+        # per ray: origin + beta, direction + beta, scene word (3 x 16 B) read;
+        # a hit writes the 32-B surface record, a miss its 16-B half
+        io_bytes = 48 * rays + 32 * hits + 16 * (rays - hits)
+        alg_bytes = counts["node_fetches"] * counts["node_bytes"] + counts["prim_tests"] * counts["prim_bytes"] + io_bytes
         bytes_per_launch = alg_bytes / launches
         avg_launch_s = ext_ms / launches / 1e3
         achieved = bytes_per_launch / avg_launch_s
-        traffic, traffic_src, l2_hit = measured_traffic(wl_name)
+        traffic, traffic_src, rec = measured_traffic(wl_name)
+        hbm_side = None
+        if traffic:
+            hbm_side = {"achieved": traffic / avg_launch_s / 1e9, "frac": traffic / avg_launch_s / HBM_PEAK,
+                        "bytes_per_launch": traffic, "source": traffic_src}
         roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_unit": "bytes per launch",
-                    "traffic_source": traffic_src, "l2_hit_rate": l2_hit, "kernel": "k_extend",
-                    "avg_launch_us": avg_launch_s * 1e6, "launches_per_step": launches,
+                    "traffic_source": traffic_src, "l2_hit_rate": rec.get("l2_hit_rate") if rec else None,
+                    "hbm_side": hbm_side,
+                    "limiter": counter_limiter(rec, hbm_side["frac"] if hbm_side else None),
+                    "valu_lane_util": rec.get("valu_lane_util") if rec else None,
+                    "kernel": "k_extend", "avg_launch_us": avg_launch_s * 1e6, "launches_per_step": launches,
                     "bytes_per_launch": bytes_per_launch,
                     "bytes_model": "128 B per 4-wide BVH node fetched + 64 B per primitive record tested "
-                                   "+ 40 B ray/hit I/O",
-                    "per_ray": {"node_fetches": counts["node_fetches"] / counts["rays"],
-                                "prim_tests": counts["prim_tests"] / counts["rays"]}}
-        log(f"[bench] extend: {counts}, {ext_ms:.2f} ms/step over {launches:.0f} launches, "
-            f"{achieved / 1e9:.0f} GB/s algorithmic")
+                                   "+ 48 B ray read + 32 B (hit) or 16 B (miss) record written",
+                    "per_ray": {"node_fetches": counts["node_fetches"] / rays, "prim_tests": counts["prim_tests"] / rays,
+                                "hit_fraction": hits / rays}}
+        log(f"[bench:{tag}] extend: {counts}, {ext_ms:.2f} ms/step over {launches:.0f} launches, "
+            f"{achieved / 1e9:.0f} GB/s algorithmic; HBM side {hbm_side}")
 
-    extra = {"value_direct_launch": value_direct,
-             "stage_ms": {"extend": ext_ms, "shade": sum(s["shade_ms"] for s in stats) / args.steps,
-                          "shadow": sum(s["shadow_ms"] for s in stats) / args.steps,
-                          "regen": sum(s["regen_ms"] for s in stats) / args.steps,
-                          "marched_kinds": sum(s["march_ms"] for s in stats) / args.steps}}
+    res = {"workload": wl_name, "value": value, "unit": "Mpaths/s", "ms_per_step": ms_per_step,
+           "paths_per_step": paths, "bvh_nodes": nodes, "launch": tuning, "roofline": roofline,
+           "value_direct_launch": value_direct, "max_depth": scenes[0][0].max_depth,
+           "mrays_per_s": (stats[-1]["camera_rays"] + stats[-1]["extension_rays"] + stats[-1]["shadow_rays"])
+                          / (ms_per_step / 1e3) / 1e6,
+           "stage_ms": {"extend": ext_ms, "shade": sum(s["shade_ms"] for s in stats) / args.steps,
+                        "shadow": sum(s["shadow_ms"] for s in stats) / args.steps,
+                        "regen": sum(s["regen_ms"] for s in stats) / args.steps,
+                        "marched_kinds": sum(s["march_ms"] for s in stats) / args.steps}}
+    res.update(desc)
     if not args.no_fp64:
-        p64 = params("plain64")
+        # the plain one-path-per-thread FP64 kernel, every path its own
+        # thread; built for 4, 3 and 2 waves per SIMD (register budgets),
+        # each warmed up, then timed twice: the fastest build is the
+        # denominator.  All builds give the same image.
         outs64 = [torch.empty_like(o) for o in outs]
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        lm.render_batch_device(worlds, p64, [o.data_ptr() for o in outs64])
-        torch.cuda.synchronize()
-        t64 = time.perf_counter() - t1
+        p64 = [o.data_ptr() for o in outs64]
+        builds = {}
+        for waves in (4, 3, 2):
+            pp = params("plain64", waves << _abi.FLAG_PLAIN_WAVES_SHIFT)
+            lm.render_batch_device(worlds, pp, p64)
+            torch.cuda.synchronize()
+            best = math.inf
+            for _ in range(2):
+                t1 = time.perf_counter()
+                lm.render_batch_device(worlds, pp, p64)
+                torch.cuda.synchronize()
+                best = min(best, time.perf_counter() - t1)
+            builds[waves] = paths / best / 1e6
+        fp64_rate = max(builds.values())
         imgs64 = [o.double().cpu().numpy() for o in outs64]
         num = sum(float(np.sum((a - b) ** 2)) for a, b in zip(imgs32, imgs64))
         den = sum(float(np.sum(b ** 2)) for b in imgs64)
         per = [float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)) for a, b in zip(imgs32, imgs64)]
-        fp64_rate = paths / t64 / 1e6
-        extra.update({"plain_fp64_mpaths_per_s": fp64_rate, "speedup_vs_plain_fp64": value / fp64_rate,
-                      "relrmse_fp32_vs_fp64": (num / den) ** 0.5, "relrmse_fp32_vs_fp64_max_scene": max(per),
-                      "relrmse_per_scene": [float(f"{x:.3g}") for x in per]})
-        log(f"[bench] plain FP64: {t64 * 1e3:.1f} ms ({fp64_rate:.1f} Mpaths/s), relRMSE(FP32, FP64) = "
-            f"{(num / den) ** 0.5:.3e} (worst scene {max(per):.3e})")
-
-    images = None
-    if not args.no_images:
+        rel = (num / den) ** 0.5
+        res.update({"plain_fp64_mpaths_per_s": fp64_rate,
+                    "plain_fp64_builds": {f"{w}_waves_per_simd": v for w, v in builds.items()},
+                    "speedup_vs_plain_fp64": value / fp64_rate, "relrmse_fp32_vs_fp64": rel,
+                    "relrmse_fp32_vs_fp64_max_scene": max(per),
+                    "rmse_gate_1e-3": {"aggregate": rel <= 1e-3, "every_scene": max(per) <= 1e-3}})
+        if len(per) > 1:
+            res["relrmse_per_scene"] = [float(f"{x:.3g}") for x in per]
+        log(f"[bench:{tag}] plain FP64 builds {builds} Mpaths/s; relRMSE(FP32, FP64) = {rel:.3e} "
+            f"(worst scene {max(per):.3e})")
+    if want_images and not args.no_images:
         images = image_leg(scenes, worlds, frame, n_pix, max(args.steps, 3) * 10)
-        log(f"[bench] image outputs: {images}")
-        extra["image_outputs"] = images
+        log(f"[bench:{tag}] image outputs: {images}")
+        res["image_outputs"] = images
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="batch64")
+    ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 kernel comparison")
+    ap.add_argument("--no-count", action="store_true", help="skip the instrumented traversal count")
+    ap.add_argument("--cpu-seconds", type=float, default=16.0)
+    ap.add_argument("--stack-lds", type=int, default=0)
+    ap.add_argument("--pool", type=int, default=0)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering adjoint leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults or cache)")
+    ap.add_argument("--no-images", action="store_true", help="skip the post chain / RGBE / AOV timings")
+    ap.add_argument("--no-mesh-heavy", action="store_true",
+                    help="skip the second leg: the 1.31M-triangle mesh-heavy scene (1080p, 64 spp, depth 12)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (5 times): the reference kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 18  # float32 elements
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 5}))
+        return
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only: every GPU box here has a single "
+                                   "MI355X; run with --gpus 1"}))
+        sys.exit(2)
+
+    # torch before the library: liblmn.so then binds to the HIP runtime torch
+    # loaded (one runtime per process), not a second copy of it
+    import torch  # noqa: F401
+    wl_name, scenes, desc = build_workload(args.workload, args.scenes)
+    main_res = measure(wl_name, scenes, desc, args, True, args.workload)
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_cores()
         cpaths, csecs, frac = cpu_baseline(scenes, args.cpu_seconds, threads)
         cpu = {"value": cpaths / csecs / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(), "cores_visible": os.cpu_count(),
                "sample": f"every k-th row of every scene, k = {round(1 / frac)} ({cpaths} paths, {csecs:.1f} s), "
-                         f"CPU FP64 oracle (restatement of the reference pixel loop), counter RNG"}
+                         f"CPU FP64 oracle (restatement of the reference pixel loop), counter RNG, one thread per "
+                         f"core this job may use (affinity, cgroup quota, OMP_NUM_THREADS)"}
         log(f"[bench] cpu oracle: {cpu}")
+    del scenes
+
+    mesh = None
+    if args.workload != "mesh_heavy" and not args.no_mesh_heavy:
+        mname, mscenes, mdesc = build_workload("mesh_heavy")
+        mesh = measure(mname, mscenes, mdesc, args, False, "mesh_heavy")
+        del mscenes
 
     adj = None
     if not args.no_adjoint:
         adj = adjoint_leg(max(args.steps, 3), 1)
         log(f"[bench] adjoint: {adj}")
 
-    cfg0 = scenes[0][0]
+    r = main_res
     line = {
-        "metric": METRIC, "value": value, "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": r["value"], "unit": "Mpaths/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": dict({"workload": wl_name, "paths_per_step": paths, "bvh_nodes": nodes,
-                        "max_depth": cfg0.max_depth, "launch": tuning}, **desc),
-        "roofline": roofline, "cpu_baseline": cpu, "adjoint": adj,
-        "mrays_per_s": (stats[-1]["camera_rays"] + stats[-1]["extension_rays"] + stats[-1]["shadow_rays"])
-                       / (ms_per_step / 1e3) / 1e6,
+        "config": dict({"workload": wl_name, "paths_per_step": r["paths_per_step"], "bvh_nodes": r["bvh_nodes"],
+                        "max_depth": r["max_depth"], "launch": r["launch"]}, **desc),
+        "roofline": r["roofline"], "cpu_baseline": cpu, "adjoint": adj,
     }
-    line.update(extra)
+    for k, v in r.items():
+        if k not in ("workload", "value", "unit", "ms_per_step", "paths_per_step", "bvh_nodes", "launch",
+                     "roofline", "max_depth") and k not in desc:
+            line[k] = v
+    if mesh is not None:
+        line["mesh_heavy"] = mesh
     print(json.dumps(line), flush=True)
 
 

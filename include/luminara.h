@@ -259,6 +259,12 @@ typedef struct lmn_render_params {
 } lmn_render_params;
 #define LMN_FLAG_STAGE_TIMING 1u /* time each wavefront stage with HIP events */
 #define LMN_FLAG_NO_GRAPH 2u     /* launch wavefront iterations directly, not as a captured hipGraph */
+/* Plain FP64 kernel (LMN_PATH_PLAIN_FP64): the waves per SIMD its build is
+ * held to, in bits 8-10 of flags: 2 (its full register budget, no spills),
+ * 3, or 4 (the default, also for 0 and other values).  Results are
+ * identical; the fastest depends on the scene. */
+#define LMN_FLAG_PLAIN_WAVES_SHIFT 8u
+#define LMN_FLAG_PLAIN_WAVES(n) ((uint32_t)(n) << LMN_FLAG_PLAIN_WAVES_SHIFT)
 #define LMN_FLAG_ADAPTIVE 4u     /* adaptive sampling (src/render.rs:1626-1677): each pixel takes
                                     stratified samples in order until n >= max(spp/4, 16), n > 1
                                     and sqrt(var(luminance) / n) < adaptive_threshold; its value is
@@ -374,6 +380,7 @@ typedef struct lmn_trace_counts {
   uint64_t rays;
   uint32_t node_bytes;
   uint32_t prim_bytes;
+  uint64_t hits;        /* rays of the pass that found a hit */
 } lmn_trace_counts;
 int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lmn_render_params* params,
                        float* const* d_hdr_rgb, lmn_trace_counts* counts);

@@ -121,11 +121,12 @@ class Hit(C.Structure):
 
 class TraceCounts(C.Structure):
     _fields_ = [("node_fetches", C.c_uint64), ("prim_tests", C.c_uint64), ("tri_tests", C.c_uint64),
-                ("rays", C.c_uint64), ("node_bytes", C.c_uint32), ("prim_bytes", C.c_uint32)]
+                ("rays", C.c_uint64), ("node_bytes", C.c_uint32), ("prim_bytes", C.c_uint32), ("hits", C.c_uint64)]
 
 
 FLAG_NO_GRAPH = 2
 FLAG_ADAPTIVE = 4
+FLAG_PLAIN_WAVES_SHIFT = 8  # bits 8-10: waves per SIMD of the plain FP64 kernel build (2, 3, 4)
 
 
 class Tuning(C.Structure):

@@ -485,6 +485,7 @@ int resolve_wave_config(lmn_scene* const* scenes, const lmn_render_params* const
 // kernel launches per iteration serves the whole batch.  Items are grouped in
 // chunks of whole sample passes (at most kMaxChunkItems samples buffered).
 constexpr uint64_t kMaxChunkItems = 1ull << 27;  // 2 GiB of float4 samples
+constexpr uint64_t kPlainChunkItems = 1ull << 26;  // 2 GiB of double4 plain-kernel samples
 
 struct WaveTotals {
   uint64_t ext = 0, sh = 0, cam = 0, iters = 0;
@@ -937,18 +938,26 @@ int render_plain(lmn_scene* s, const lmn_render_params* p, const FrameSetup& fs,
     }
     return LMN_OK;
   }
+  // one path per thread over whole sample passes, at most kPlainChunkItems
+  // paths per launch; each chunk's samples are added in sample order
   HIP_TRY(W.acc.ensure(size_t(npix) * 32));
   launch_zero_acc(W.acc.as<double4>(), npix, st);
-  for (uint32_t s0 = 0; s0 < fs.F.spp_a; ++s0) {
+  const uint32_t passes = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(fs.F.spp_a, kPlainChunkItems / npix)));
+  HIP_TRY(W.samples.ensure(size_t(passes) * npix * sizeof(double4)));
+  for (uint32_t s0 = 0; s0 < fs.F.spp_a; s0 += passes) {
+    const uint32_t s1 = std::min(fs.F.spp_a, s0 + passes);
+    const uint64_t k0 = uint64_t(s0) * npix, nk = uint64_t(s1 - s0) * npix;
     if (fp64) {
       SceneView<double> S = s->viewD;
       S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
-      launch_plain_fp64(S, s->host.featD, fs.F, W.acc.as<double4>(), s0, s0 + 1, W.rays.as<unsigned long long>(), st);
+      launch_plain_fp64(S, s->host.featD, fs.F, W.samples.as<double4>(), k0, nk, W.rays.as<unsigned long long>(),
+                        (p->flags >> LMN_FLAG_PLAIN_WAVES_SHIFT) & 7u, st);
     } else {
       SceneView<float> S = s->viewF;
       S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
-      launch_plain_fp32(S, s->host.featF, fs.F, W.acc.as<double4>(), s0, s0 + 1, W.rays.as<unsigned long long>(), st);
+      launch_plain_fp32(S, s->host.featF, fs.F, W.samples.as<double4>(), k0, nk, W.rays.as<unsigned long long>(), st);
     }
+    launch_plain_resolve(W.samples.as<double4>(), W.acc.as<double4>(), npix, s1 - s0, st);
   }
   launch_finalize(W.acc.as<double4>(), d_out, npix, st);
   HIP_TRY(hipGetLastError());
@@ -1148,6 +1157,7 @@ int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lm
     counts->prim_tests = h.prim_tests;
     counts->tri_tests = h.tri_tests;
     counts->rays = h.rays;
+    counts->hits = h.hits;
     counts->node_bytes = sizeof(Node4F);
     counts->prim_bytes = sizeof(PrimT<float>);
   }

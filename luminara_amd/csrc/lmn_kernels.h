@@ -163,10 +163,14 @@ void launch_adapt_finalize(const AdaptState* state, float* out, uint32_t npix, u
 void launch_adapt_init(AdaptState* state, uint32_t* list, uint32_t npix, hipStream_t s);
 void launch_zero_acc(double4* acc, uint32_t npix, hipStream_t s);
 void launch_finalize(const double4* acc, float* out, uint32_t npix, hipStream_t s);
-void launch_plain_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, double4* acc, uint32_t s0,
-                       uint32_t s1, unsigned long long* rays, hipStream_t s);
-void launch_plain_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, double4* acc, uint32_t s0,
-                       uint32_t s1, unsigned long long* rays, hipStream_t s);
+// Plain kernels, one path per thread: work items [k0, k0 + n) of a frame
+// (k = s * npix + p), radiance and filter weight per item into samples[0, n).
+void launch_plain_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, double4* samples, uint64_t k0,
+                       uint64_t n, unsigned long long* rays, uint32_t waves, hipStream_t s);
+void launch_plain_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, double4* samples, uint64_t k0,
+                       uint64_t n, unsigned long long* rays, hipStream_t s);
+// Adds `passes` whole sample passes of samples (pass-major) into acc per pixel.
+void launch_plain_resolve(const double4* samples, double4* acc, uint32_t npix, uint32_t passes, hipStream_t s);
 struct AdaptState;
 void launch_plain_adaptive_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, AdaptState* state,
                                 uint32_t s0, uint32_t s1, uint32_t min_samples, double threshold,

@@ -70,18 +70,51 @@ __device__ V3<R> plain_sample(const SceneView<R>& S, const FrameDev& F, uint64_t
   return L;
 }
 
-template <class R, uint32_t FEAT>
-__global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, double4* __restrict__ acc,
-                                               uint32_t s_begin, uint32_t s_end, unsigned long long* rays) {
-  uint32_t npix = F.crop_w * F.crop_h;
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+// Sums a thread's ray counts over its wave and adds them with one atomic
+// per wave (one per thread would serialise every path on two addresses).
+__device__ __forceinline__ void add_ray_counts(unsigned long long* rays, unsigned long long n_ext,
+                                               unsigned long long n_sh) {
+  for (int off = 32; off > 0; off >>= 1) {
+    n_ext += __shfl_xor(n_ext, off, 64);
+    n_sh += __shfl_xor(n_sh, off, 64);
+  }
+  if ((threadIdx.x & 63u) == 0u) {
+    if (n_ext) atomicAdd(&rays[0], n_ext);
+    if (n_sh) atomicAdd(&rays[1], n_sh);
+  }
+}
+
+// One path per thread: work item k0 + j (k = s * npix + p, sample s of crop
+// pixel p) for j < n.  The unclamped radiance and the filter weight go to
+// samples[j]; k_plain_resolve adds them per pixel in sample order.  OCC: the
+// waves per SIMD the build is held to (its register budget); the FP64 build
+// takes 250+ VGPRs left alone (2 waves), at 4 it spills some.  Which runs
+// fastest depends on the scene (path length, traversal depth), so the FP64
+// kernel is built for 2, 3 and 4 and the caller picks.
+template <class R, uint32_t FEAT, int OCC>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(OCC, 8))) void k_plain(SceneView<R> S, FrameDev F, double4* __restrict__ samples, uint64_t k0,
+                                               uint64_t n, unsigned long long* rays) {
+  const uint64_t j = uint64_t(blockIdx.x) * 128u + threadIdx.x;
+  unsigned long long n_ext = 0, n_sh = 0;
+  if (j < n) {
+    R w;
+    V3<R> L = plain_sample<R, FEAT>(S, F, k0 + j, w, n_ext, n_sh);
+    samples[j] = make_double4(double(L.x), double(L.y), double(L.z), double(w));
+  }
+  if (rays) add_ray_counts(rays, n_ext, n_sh);  // every lane of the wave takes part
+}
+
+// Per pixel, the samples of `passes` consecutive sample passes in sample
+// order: luminance clamp at 100 (src/render.rs:1692-1697), filter weight,
+// FP64 sums (:1699-1705).
+__global__ __launch_bounds__(256) void k_plain_resolve(const double4* __restrict__ samples, double4* __restrict__ acc,
+                                                       uint32_t npix, uint32_t passes) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p >= npix) return;
   double4 a = acc[p];
-  unsigned long long n_ext = 0, n_sh = 0;
-  for (uint32_t s = s_begin; s < s_end; ++s) {
-    R w;
-    V3<R> L = plain_sample<R, FEAT>(S, F, uint64_t(s) * npix + p, w, n_ext, n_sh);
-    double x = double(L.x), y = double(L.y), z = double(L.z);
+  for (uint32_t s = 0; s < passes; ++s) {
+    const double4 v = samples[size_t(s) * npix + p];
+    double x = v.x, y = v.y, z = v.z;
     double lum = 0.2126 * x + 0.7152 * y + 0.0722 * z;
     if (lum > 100.0) {
       double f = 100.0 / lum;
@@ -89,17 +122,12 @@ __global__ __launch_bounds__(128) void k_plain(SceneView<R> S, FrameDev F, doubl
       y *= f;
       z *= f;
     }
-    double ww = double(w);
-    a.x += x * ww;
-    a.y += y * ww;
-    a.z += z * ww;
-    a.w += ww;
+    a.x += x * v.w;
+    a.y += y * v.w;
+    a.z += z * v.w;
+    a.w += v.w;
   }
   acc[p] = a;
-  if (rays) {
-    atomicAdd(&rays[0], n_ext);
-    atomicAdd(&rays[1], n_sh);
-  }
 }
 
 // Adaptive sampling, one pixel per thread in sample order: sample s of a
@@ -234,19 +262,27 @@ __global__ __launch_bounds__(128) void k_aov(SceneView<R> S, FrameDev F, float* 
     default: CALL(5); break;           \
   }
 
-void launch_plain_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, double4* acc, uint32_t s0,
-                       uint32_t s1, unsigned long long* rays, hipStream_t s) {
-  uint32_t npix = F.crop_w * F.crop_h;
-#define CALL(f) hipLaunchKernelGGL((k_plain<double, f>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, acc, s0, s1, rays)
+void launch_plain_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, double4* samples, uint64_t k0,
+                       uint64_t n, unsigned long long* rays, uint32_t waves, hipStream_t s) {
+  if (!n) return;
+  const dim3 grid(uint32_t((n + 127) / 128));
+#define CALL(f)                                                                                                     \
+  if (waves == 2) hipLaunchKernelGGL((k_plain<double, f, 2>), grid, dim3(128), 0, s, S, F, samples, k0, n, rays);   \
+  else if (waves == 3) hipLaunchKernelGGL((k_plain<double, f, 3>), grid, dim3(128), 0, s, S, F, samples, k0, n, rays); \
+  else hipLaunchKernelGGL((k_plain<double, f, 4>), grid, dim3(128), 0, s, S, F, samples, k0, n, rays);
   LMN_FEAT64_SWITCH(feat, CALL)
 #undef CALL
 }
-void launch_plain_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, double4* acc, uint32_t s0,
-                       uint32_t s1, unsigned long long* rays, hipStream_t s) {
-  uint32_t npix = F.crop_w * F.crop_h;
-#define CALL(f) hipLaunchKernelGGL((k_plain<float, f>), dim3((npix + 127) / 128), dim3(128), 0, s, S, F, acc, s0, s1, rays)
+void launch_plain_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, double4* samples, uint64_t k0,
+                       uint64_t n, unsigned long long* rays, hipStream_t s) {
+  if (!n) return;
+  const dim3 grid(uint32_t((n + 127) / 128));
+#define CALL(f) hipLaunchKernelGGL((k_plain<float, f, 1>), grid, dim3(128), 0, s, S, F, samples, k0, n, rays)
   LMN_FEAT_SWITCH(feat, CALL)
 #undef CALL
+}
+void launch_plain_resolve(const double4* samples, double4* acc, uint32_t npix, uint32_t passes, hipStream_t s) {
+  if (npix && passes) hipLaunchKernelGGL(k_plain_resolve, dim3((npix + 255) / 256), dim3(256), 0, s, samples, acc, npix, passes);
 }
 void launch_plain_adaptive_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, AdaptState* state,
                                 uint32_t s0, uint32_t s1, uint32_t min_samples, double threshold,

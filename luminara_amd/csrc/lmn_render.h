@@ -73,6 +73,7 @@ struct TraceCounters {
   unsigned long long prim_tests;
   unsigned long long tri_tests;
   unsigned long long rays;
+  unsigned long long hits;
 };
 
 // Traversal stack in per-thread private memory (plain kernels).  The FP64
@@ -485,6 +486,7 @@ LMN_HD bool trace_pass(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R& close
     atomicAdd(&cnt->prim_tests, (unsigned long long)pt);
     atomicAdd(&cnt->tri_tests, (unsigned long long)tt);
     atomicAdd(&cnt->rays, 1ull);
+    if (found) atomicAdd(&cnt->hits, 1ull);
   }
 #else
   (void)cnt;

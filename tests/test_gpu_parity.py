@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 import luminara_amd as lm
-from luminara_amd import workloads
+from luminara_amd import _abi, workloads
 import oracle
 
 pytestmark = pytest.mark.gpu
@@ -107,6 +107,32 @@ def test_fp64_kernel_matches_oracle(gpu, scene):
     assert rel < 1e-6
     assert r.stats["extension_rays"] == ost["extension_rays"]
     assert r.stats["shadow_rays"] == ost["shadow_rays"]
+
+
+@pytest.mark.parametrize("waves", [2, 3])
+def test_fp64_kernel_builds_agree(gpu, waves):
+    # the plain FP64 kernel's occupancy builds (register budgets) trace the
+    # same paths: images equal bit for bit, equal ray counts
+    for text, (w, h, spp) in [(SCENES["quadrics"][0](), (48, 32, 9)),
+                              (workloads.cornell_box(256, 256, 64, 8, 42), (256, 256, 64))]:
+        cfg, cam, world = lm.load_scene(text)
+        cfg.width, cfg.height, cfg.samples_per_pixel = w, h, spp
+        a = lm.render(cfg, cam, world, path="plain64")
+        b = lm.render(cfg, cam, world, path="plain64", flags=waves << _abi.FLAG_PLAIN_WAVES_SHIFT)
+        assert np.array_equal(a.rows, b.rows)
+        assert a.stats["extension_rays"] == b.stats["extension_rays"]
+        assert a.stats["shadow_rays"] == b.stats["shadow_rays"]
+
+
+def test_fp64_kernel_chunks_equal_one_launch(gpu):
+    # 4096^2 x 9 spp = 151 M paths: the plain kernels launch whole sample
+    # passes of at most 2^26 paths (4 + 4 + 1 here) and add each launch's
+    # samples in sample order; a 256^2 crop of the same frame fits one launch
+    cfg, cam, world = lm.load_scene(workloads.cornell_box(4096, 4096, 9, 8, 42))
+    full = lm.render(cfg, cam, world, path="plain64")
+    cfg.crop = (1900, 2000, 256, 256)
+    part = lm.render(cfg, cam, world, path="plain64")
+    assert np.array_equal(full.rows[2000:2256, 1900:2156], part.rows)
 
 
 @pytest.mark.parametrize("scene", ["cornell", "spheres", "quadrics"])
