@@ -25,10 +25,11 @@ struct ShadowSoA {
   float4* c;
 };
 
+// The two queue tails that every shading workgroup bumps (survivors, shadow
+// rays) sit on cache lines of their own: same-line atomics serialise in one
+// L2 channel.
 struct WaveCounters {
   uint32_t active;       // paths in the current pool
-  uint32_t next_active;  // survivors compacted into the next pool
-  uint32_t n_shadow;     // shadow rays queued this iteration
   uint32_t pad;
   uint64_t next_item;    // next work item to issue (chunk relative)
   uint64_t n_items;      // work items in the chunk
@@ -40,6 +41,8 @@ struct WaveCounters {
   uint32_t n_ovf[2];      // rays left to the marched fallback, [extend, shadow]
   uint32_t n_brk;         // superellipsoid brackets queued for refinement (extend)
   uint32_t pad2;
+  alignas(256) uint32_t next_active;  // survivors compacted into the next pool
+  alignas(256) uint32_t n_shadow;     // shadow rays queued this iteration
 };
 
 // Candidate queues of the marched pass.  A ray lists the marched

@@ -8,6 +8,8 @@
 //  * ray-batch trace for parity.
 // The estimator is the reference's ray_color / pixel loop
 // (src/render.rs:1472-1537, :1587-1747), unrolled into an iteration.
+#include <algorithm>
+
 #include "lmn_kernels.h"
 
 namespace lmn {
@@ -76,6 +78,18 @@ __device__ __forceinline__ void block_reserve2(uint32_t* c0, uint32_t* c1, bool 
   s1 = lds[2 * NW + 1] + lds[NW + wave] + uint32_t(__popcll(m1 & lt));
 }
 
+// Slot in a device-wide queue for every lane of the wave that wants one:
+// one atomic per wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool want) {
+  const unsigned long long m = __ballot(want);
+  if (m == 0ull) return 0u;
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0u;
+  if (int(__lane_id()) == leader) base = atomicAdd(ctr, uint32_t(__popcll(m)));
+  base = uint32_t(__shfl(int(base), leader));
+  return base + uint32_t(__popcll(m & __lanemask_lt()));
+}
+
 // ---------------------------------------------------------------------------
 // Wave-uniform scene selection.  Paths of one scene are issued together, so
 // a wave almost always holds one scene; the loop peels one scene per pass
@@ -141,8 +155,10 @@ __device__ __forceinline__ void extend_body(const BatchDev& B, const PathSoA& cu
   }
 }
 // The traversal kernels are latency-bound: the register budget is capped at
-// 8 waves per SIMD (64 VGPRs), which outweighs the few spills it costs.
-#define LMN_TRAVERSAL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(8, 8)))
+// 8 waves per SIMD (64 VGPRs), which outweighs the few spills it costs.  With
+// the 32-word LDS stack, LDS holds a CU to 20 waves (5 per SIMD) whatever the
+// registers: the budget is then that of 5 waves (102 VGPRs).
+#define LMN_TRAVERSAL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(SS == 16 ? 8 : 5, SS == 16 ? 8 : 5)))
 
 template <int SS, int BS, uint32_t FEAT>
 __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_extend(BatchDev B, PathSoA cur, HitSoA hit,
@@ -158,33 +174,93 @@ __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_extend_count(Bat
   extend_body<SS, BS, FEAT, true>(B, cur, hit, ctr, spill, tc);
 }
 
-// Shadow rays: any hit over [0.001, dist - 0.001] (src/render.rs:1457-1458).
-// Without marched kinds the contribution of a visible light sample is added
-// here; with them an occluded ray's distance is negated and the marched
-// pass finishes the query.
+// Shadow rays: any hit over [0.001, dist - 0.001] (src/render.rs:1457-1458),
+// main BVH then planes.  Without marched kinds the contribution of a visible
+// light sample is added here; with them an occluded ray's distance is
+// negated and the marched pass finishes the query.
+//
+// Persistent: a grid of about one workgroup per resident slot walks all the
+// stage's rays, and a lane whose ray has ended takes its wave's next ray, so
+// a wave no longer waits for its longest ray before its lanes get new work.
+// Visibility rays end at their first hit and carry no surface work, so the
+// refill pays (mesh-heavy shadow stage 25.6 -> 19.8 ms); closest-hit rays
+// end with a surface record, and running that per refill round cost more
+// than the refill saved (extend kept grid-stride).  The rays of wave w are
+// blocks w, w + W, w + 2W, ... of 64 (W waves in the grid), as a
+// grid-stride kernel deals them out; walk steps run in rounds, and a round
+// ends once a quarter of the lanes that began it have finished.  Every
+// ray's walk is the grid-stride walk, step for step.  Exit: a lane whose
+// next ray lies past the stage's count is done; the wave leaves when all its
+// lanes are.
 template <int SS, int BS, uint32_t FEAT>
-__global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_shadow(BatchDev B, ShadowSoA sh, float4* __restrict__ samples,
-                                               const WaveCounters* __restrict__ ctr, uint32_t* spill) {
+__global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_shadow(BatchDev B, ShadowSoA sh,
+                                                                       float4* __restrict__ samples,
+                                                                       const WaveCounters* __restrict__ ctr,
+                                                                       uint32_t* spill) {
   __shared__ uint32_t lds[SS * BS];
   constexpr bool DEFER = (FEAT & FEAT_MARCH) != 0;
-  uint32_t n = ctr->n_shadow;
-  uint32_t nthreads = gridDim.x * BS;
-  uint32_t gid = blockIdx.x * BS + threadIdx.x;
-  for (uint32_t i = gid; i < n; i += nthreads) {
-    float4 o = sh.o[i], d = sh.d[i], c = sh.c[i];
-    uint32_t sid = __float_as_uint(c.w);
+  constexpr uint32_t MAIN = FEAT & ~FEAT_MARCH;
+  constexpr uint32_t kIdle = 0xffffffffu, kDone = 0xfffffffeu;
+  const uint32_t n = ctr->n_shadow;
+  const uint32_t nthreads = gridDim.x * BS;
+  const uint32_t gid = blockIdx.x * BS + threadIdx.x;
+  const uint64_t n_waves = nthreads / 64u, wave = gid / 64u;
+  uint64_t taken = 0;  // the wave's rays handed out so far (wave-uniform)
+  LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
+  uint32_t i = kIdle, sid = 0, ref = 0, best = 0;
+  V3<float> o = {0.f, 0.f, 0.f}, d = {0.f, 0.f, 0.f}, inv = {0.f, 0.f, 0.f};
+  float closest = 0.f, dist = 0.f;
+  bool found = false;
+  for (;;) {
+    const bool need = i == kIdle;
+    const unsigned long long want = __ballot(need);
+    if (need) {
+      const uint64_t k = taken + uint64_t(__popcll(want & __lanemask_lt()));
+      const uint64_t r = (wave + (k >> 6) * n_waves) * 64u + (k & 63u);
+      i = r < n ? uint32_t(r) : kDone;
+      if (i != kDone) {
+        const float4 o4 = sh.o[i], d4 = sh.d[i];
+        o = f3(o4);
+        d = f3(d4);
+        dist = o4.w;
+        closest = o4.w - 0.001f;
+        sid = __float_as_uint(sh.c[i].w);
+        inv = {1.f / d.x, 1.f / d.y, 1.f / d.z};
+        ref = 0;
+        best = 0;
+        found = false;
+        st.sp = 0;
+      }
+    }
+    taken += uint64_t(__popcll(want));
+    if (__ballot(i != kDone) == 0ull) return;
+    if (i == kDone) continue;
+    bool fin = false;
     for_scene(sid, [&](uint32_t s) {
       const SceneView<float> S = B.views[s];
-      LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
-      float t = o.w - 0.001f;
-      uint32_t idx = 0;
-      bool occluded = trace_pass<float, FEAT & ~FEAT_MARCH, true, PASS_MAIN>(S, f3(o), f3(d), 0.001f, t, idx, st);
+      uint32_t nf = 0, pt = 0, tt = 0;
+      const uint32_t began = uint32_t(__popcll(__ballot(true)));
+      for (;;) {
+        if (!fin)
+          fin = S.n_nodes4 == 0 ||
+                bvh4_step<MAIN, true>(S, o, d, inv, 0.001f, closest, best, found, ref, st, nf, pt, tt, false);
+        const uint32_t walking = uint32_t(__popcll(__ballot(!fin)));
+        if (walking == 0u || 4u * walking <= 3u * began) break;
+      }
+      if (!fin) return;
+      if (!found)  // the unbounded planes after the BVH (src/scene.rs:817-834)
+        for (uint32_t k = 0; k < S.n_planes; ++k)
+          if (test_prim<float, MAIN>(S, S.n_bounded + k, o, d, 0.001f, closest)) {
+            found = true;
+            break;
+          }
       if (DEFER) {
-        if (occluded) sh.o[i].w = -o.w;
-      } else if (!occluded) {
-        add_sample(samples, __float_as_uint(d.w), f3(c));
+        if (found) sh.o[i].w = -dist;
+      } else if (!found) {
+        add_sample(samples, __float_as_uint(sh.d[i].w), f3(sh.c[i]));
       }
     });
+    if (fin) i = kIdle;
   }
 }
 
@@ -369,17 +445,6 @@ __global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_test(BatchDev 
   }
 }
 
-// Slot in a device-wide queue for every lane of the wave that wants one:
-// one atomic per wave.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool want) {
-  const unsigned long long m = __ballot(want);
-  if (m == 0ull) return 0u;
-  const int leader = __ffsll((long long)m) - 1;
-  uint32_t base = 0u;
-  if (int(__lane_id()) == leader) base = atomicAdd(ctr, uint32_t(__popcll(m)));
-  base = uint32_t(__shfl(int(base), leader));
-  return base + uint32_t(__popcll(m & __lanemask_lt()));
-}
 
 // Superellipsoid candidates of the extend stage, exact mode: the interval
 // walk up to the bracketing leaf; brackets are queued for k_march_refine, so
@@ -805,13 +870,28 @@ static void launch_march(const WaveLaunch& L, const BatchDev& B, const PathSoA& 
                      L.samples, L.mq, L.ctr);
 }
 
+// Workgroups of the persistent shadow launch: as many as the device holds
+// at once, at most the grid-stride grid (the spill area is sized for that).
+template <int SS, int BS, uint32_t FEAT>
+static uint32_t persistent_grid(uint32_t grid) {
+  static const uint32_t resident = [] {
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_shadow<SS, BS, FEAT>), BS, 0);
+    return uint32_t(std::max(1, cus) * std::max(1, per_cu));
+  }();
+  return std::max(1u, std::min(grid, resident));
+}
+
 template <int SS, int BS, uint32_t FEAT>
 static void launch_stage_ss(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow,
                             hipEvent_t mid) {
   constexpr uint32_t MAIN = FEAT & ~FEAT_MARCH;
   constexpr bool MARCH = (FEAT & FEAT_MARCH) != 0;
   if (shadow) {
-    hipLaunchKernelGGL((k_shadow<SS, BS, FEAT>), dim3(L.grid), dim3(BS), 0, L.stream, B, L.sh, L.samples, L.ctr, L.spill);
+    hipLaunchKernelGGL((k_shadow<SS, BS, FEAT>), dim3(persistent_grid<SS, BS, FEAT>(L.grid)), dim3(BS), 0, L.stream,
+                       B, L.sh, L.samples, L.ctr, L.spill);
     if (mid) (void)hipEventRecord(mid, L.stream);
     if constexpr (MARCH) launch_march<SS, BS, FEAT, true>(L, B, cur);
     return;
@@ -856,18 +936,17 @@ static void launch_trace_stage(const WaveLaunch& L, const BatchDev& B, const Pat
   LMN_FEAT32_SWITCH(L.feat, CALL)
 #undef CALL
 }
-template <int BS>
-static void launch_shade_bs(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt) {
-#define CALL(f) \
-  hipLaunchKernelGGL((k_shade<BS, f>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, nxt, L.sh, L.samples, L.ctr)
+// Shading workgroups reserve their queue slots with one device atomic per
+// queue and round (block_reserve2); 512 lanes per workgroup halve those
+// atomics against 256 (batch shade stage 113 -> 69 ms); 1024 was slower.
+constexpr int kShadeBlock = 512;
+static void launch_shade(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt) {
+  const uint32_t g = std::max<uint32_t>(1, uint32_t(uint64_t(L.grid) * L.block / kShadeBlock));
+#define CALL(f)                                                                                                    \
+  hipLaunchKernelGGL((k_shade<kShadeBlock, f>), dim3(g), dim3(kShadeBlock), 0, L.stream, B, cur, L.hit, nxt, L.sh, \
+                     L.samples, L.ctr)
   LMN_FEAT32_SWITCH(L.feat, CALL)
 #undef CALL
-}
-static void launch_shade(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt) {
-  if (L.block == 128)
-    launch_shade_bs<128>(L, B, cur, nxt);
-  else
-    launch_shade_bs<256>(L, B, cur, nxt);
 }
 
 void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt,

@@ -351,37 +351,41 @@ LMN_HD void bvh_walk(const SceneView<R>& S, const typename NodeSel<R>::type* nod
 // visited next and the others pushed farthest first, each with its entry
 // distance so that a pop can drop it once a closer hit is known (a
 // visibility query, ANY, pushes the child alone).
+// One step of the 4-wide walk: the node or leaf `ref` names, then the next
+// entry to visit (popped from the stack) into `ref`.  Returns true once the
+// walk has ended (stack empty, or a visibility query's first hit).  A walk
+// is `ref = 0` (the root) followed by steps until one returns true, so a
+// caller may interleave other work (a persistent kernel's ray fetch)
+// between steps.
 template <uint32_t FEAT, bool ANY, class Stack>
-LMN_HD void bvh4_walk(const SceneView<float>& S, V3<float> o, V3<float> d, V3<float> inv, float tmin,
-                      float& closest, uint32_t& best, bool& found, Stack& st, uint32_t& nf, uint32_t& pt,
-                      uint32_t& tt, bool counting) {
-  uint32_t ref = 0;
-  for (;;) {
-    if (!(ref & LEAF4)) {
-      const Node4F& nd = S.nodes4[ref];
-      ++nf;
-      float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
-      uint32_t ch[4];
-      for (int k = 0; k < 4; ++k) {
-        lox[k] = nd.lox[k];
-        loy[k] = nd.loy[k];
-        loz[k] = nd.loz[k];
-        hix[k] = nd.hix[k];
-        hiy[k] = nd.hiy[k];
-        hiz[k] = nd.hiz[k];
-        ch[k] = nd.child[k];
-      }
-      uint32_t nc = nd.n;
-      float tk[4];
-      for (int k = 0; k < 4; ++k) {
-        float tx0 = (lox[k] - o.x) * inv.x, tx1 = (hix[k] - o.x) * inv.x;
-        float ty0 = (loy[k] - o.y) * inv.y, ty1 = (hiy[k] - o.y) * inv.y;
-        float tz0 = (loz[k] - o.z) * inv.z, tz1 = (hiz[k] - o.z) * inv.z;
-        float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-        float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), closest));
-        tk[k] = (uint32_t(k) < nc && tn <= tf) ? tn : INFINITY;
-      }
-      // sorting network on (tk, ch): misses (INFINITY) sink to the end
+LMN_HD bool bvh4_step(const SceneView<float>& S, V3<float> o, V3<float> d, V3<float> inv, float tmin,
+                      float& closest, uint32_t& best, bool& found, uint32_t& ref, Stack& st, uint32_t& nf,
+                      uint32_t& pt, uint32_t& tt, bool counting) {
+  if (!(ref & LEAF4)) {
+    const Node4F& nd = S.nodes4[ref];
+    ++nf;
+    float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
+    uint32_t ch[4];
+    for (int k = 0; k < 4; ++k) {
+      lox[k] = nd.lox[k];
+      loy[k] = nd.loy[k];
+      loz[k] = nd.loz[k];
+      hix[k] = nd.hix[k];
+      hiy[k] = nd.hiy[k];
+      hiz[k] = nd.hiz[k];
+      ch[k] = nd.child[k];
+    }
+    uint32_t nc = nd.n;
+    float tk[4];
+    for (int k = 0; k < 4; ++k) {
+      float tx0 = (lox[k] - o.x) * inv.x, tx1 = (hix[k] - o.x) * inv.x;
+      float ty0 = (loy[k] - o.y) * inv.y, ty1 = (hiy[k] - o.y) * inv.y;
+      float tz0 = (loz[k] - o.z) * inv.z, tz1 = (hiz[k] - o.z) * inv.z;
+      float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+      float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), closest));
+      tk[k] = (uint32_t(k) < nc && tn <= tf) ? tn : INFINITY;
+    }
+    // sorting network on (tk, ch): misses (INFINITY) sink to the end
 #define LMN_CSWAP(a, b)                    \
   if (tk[b] < tk[a]) {                     \
     float tt_ = tk[a];                     \
@@ -391,53 +395,59 @@ LMN_HD void bvh4_walk(const SceneView<float>& S, V3<float> o, V3<float> d, V3<fl
     ch[a] = ch[b];                         \
     ch[b] = cc_;                           \
   }
-      LMN_CSWAP(0, 1)
-      LMN_CSWAP(2, 3)
-      LMN_CSWAP(0, 2)
-      LMN_CSWAP(1, 3)
-      LMN_CSWAP(1, 2)
+    LMN_CSWAP(0, 1)
+    LMN_CSWAP(2, 3)
+    LMN_CSWAP(0, 2)
+    LMN_CSWAP(1, 3)
+    LMN_CSWAP(1, 2)
 #undef LMN_CSWAP
-      if (tk[0] != INFINITY) {
-        // a visibility query returns at its first hit, so its range never
-        // shrinks and an entry needs no distance: one word instead of two
-        for (int k = 3; k >= 1; --k)
-          if (tk[k] != INFINITY) {
-            st.push(ch[k]);
-            if (!ANY) st.push(bits_of(tk[k]));
-          }
-        ref = ch[0];
-        continue;
-      }
-    } else {
-      uint32_t first = ref & LEAF4_FIRST, count = ((ref >> 28) & 7u) + 1u;
-      for (uint32_t k = 0; k < count; ++k) {
-        ++pt;
-        if (counting && (S.prims[first + k].kind_mat & KIND_MASK) == K_TRIANGLE) ++tt;
-        if (test_prim<float, FEAT>(S, first + k, o, d, tmin, closest)) {
-          found = true;
-          best = first + k;
-          if (ANY) return;
+    if (tk[0] != INFINITY) {
+      // a visibility query returns at its first hit, so its range never
+      // shrinks and an entry needs no distance: one word instead of two
+      for (int k = 3; k >= 1; --k)
+        if (tk[k] != INFINITY) {
+          st.push(ch[k]);
+          if (!ANY) st.push(bits_of(tk[k]));
         }
+      ref = ch[0];
+      return false;
+    }
+  } else {
+    uint32_t first = ref & LEAF4_FIRST, count = ((ref >> 28) & 7u) + 1u;
+    for (uint32_t k = 0; k < count; ++k) {
+      ++pt;
+      if (counting && (S.prims[first + k].kind_mat & KIND_MASK) == K_TRIANGLE) ++tt;
+      if (test_prim<float, FEAT>(S, first + k, o, d, tmin, closest)) {
+        found = true;
+        best = first + k;
+        if (ANY) return true;
       }
     }
-    bool next = false;
-    if (ANY) {
-      if (!st.empty()) {
-        ref = st.pop();
-        next = true;
-      }
-    } else {
-      while (!st.empty()) {
-        float te = float_of(st.pop());
-        uint32_t r = st.pop();
-        if (te <= closest) {
-          ref = r;
-          next = true;
-          break;
-        }
+  }
+  if (ANY) {
+    if (!st.empty()) {
+      ref = st.pop();
+      return false;
+    }
+  } else {
+    while (!st.empty()) {
+      float te = float_of(st.pop());
+      uint32_t r = st.pop();
+      if (te <= closest) {
+        ref = r;
+        return false;
       }
     }
-    if (!next) return;
+  }
+  return true;
+}
+
+template <uint32_t FEAT, bool ANY, class Stack>
+LMN_HD void bvh4_walk(const SceneView<float>& S, V3<float> o, V3<float> d, V3<float> inv, float tmin,
+                      float& closest, uint32_t& best, bool& found, Stack& st, uint32_t& nf, uint32_t& pt,
+                      uint32_t& tt, bool counting) {
+  uint32_t ref = 0;
+  while (!bvh4_step<FEAT, ANY>(S, o, d, inv, tmin, closest, best, found, ref, st, nf, pt, tt, counting)) {
   }
 }
 
