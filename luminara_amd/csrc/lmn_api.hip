@@ -484,7 +484,10 @@ int resolve_wave_config(lmn_scene* const* scenes, const lmn_render_params* const
 // one list (scene by scene, sample pass by sample pass), so one set of
 // kernel launches per iteration serves the whole batch.  Items are grouped in
 // chunks of whole sample passes (at most kMaxChunkItems samples buffered).
-constexpr uint64_t kMaxChunkItems = 1ull << 27;  // 2 GiB of float4 samples
+// Every chunk ends with the pool draining (its last iterations run few
+// paths), so chunks are as large as HBM comfortably holds: 2^30 items, 16 GiB
+// of float4 samples (a 64-scene batch of ~1e9 paths is one chunk).
+constexpr uint64_t kMaxChunkItems = 1ull << 30;
 constexpr uint64_t kPlainChunkItems = 1ull << 26;  // 2 GiB of double4 plain-kernel samples
 
 struct WaveTotals {
@@ -1257,7 +1260,7 @@ int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_rend
     return LMN_OK;
   };
   // coordinate descent: pool size, workgroup size, stack depth, grid size
-  for (uint32_t pool : {1u << 21, 1u << 22, 1u << 24, 1u << 25}) {
+  for (uint32_t pool : {1u << 21, 1u << 22, 1u << 24, 1u << 25, 1u << 26}) {
     WaveConfig c = best;
     c.pool = uint32_t(std::min<uint64_t>(pool, std::max<uint64_t>(total, 64)));
     if (c.pool == best.pool) continue;
