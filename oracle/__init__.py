@@ -57,6 +57,13 @@ def lib():
                                      C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.c_double,
                                      C.c_double, C.c_double, C.c_uint64, C.c_uint32, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
+        L.oracle_scatter_batch.restype = C.c_int
+        L.oracle_scatter_batch.argtypes = [C.POINTER(_abi.SceneDesc), C.c_uint32, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.c_double,
+                                           C.c_double, C.c_double, C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]
+        L.oracle_xoshiro.restype = C.c_int
+        L.oracle_xoshiro.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p]
         L.oracle_last_error.restype = C.c_char_p
         D3 = C.POINTER(C.c_double)
         L.oracle_nee.restype = C.c_int
@@ -149,6 +156,27 @@ def scatter(world, material, ray_dir, point, normal, front_face, t, u, v, key, c
     if ok < 0:
         _check(ok)
     return bool(ok), np.array(od[:]), np.array(oa[:]), oc.value
+
+
+def scatter_batch(world, material, ray_dir, point, normal, front_face, n, t=1.0, u=0.5, v=0.5, key0=0):
+    """n scatters at one point (sample k from stream key0 + k): (ok bool[n],
+    directions [n, 3], attenuations [n, 3])."""
+    D = C.c_double * 3
+    dirs, atts = np.zeros((n, 3)), np.zeros((n, 3))
+    ok = np.zeros(n, dtype=np.uint8)
+    _check(lib().oracle_scatter_batch(world.desc, material, D(*ray_dir), D(*point), D(*normal), int(front_face), t,
+                                      u, v, key0, n, dirs.ctypes.data, atts.ctypes.data, ok.ctypes.data))
+    return ok.astype(bool), dirs, atts
+
+
+def xoshiro(n, state=None, seed=0):
+    """The reference stream's generator: n u64 outputs of xoshiro256++ from a
+    raw state (4 u64) or from seed_from_u64(seed), and their f64 draws."""
+    out = np.zeros(n, dtype=np.uint64)
+    f = np.zeros(n)
+    st = None if state is None else np.ascontiguousarray(state, dtype=np.uint64)
+    _check(lib().oracle_xoshiro(None if st is None else st.ctypes.data, seed, n, out.ctypes.data, f.ctypes.data))
+    return out, f
 
 
 def _d3(v):

@@ -2367,6 +2367,60 @@ int oracle_scatter(const lmn_scene_desc* desc, uint32_t material, const double r
   return ok ? 1 : 0;
 }
 
+// n scatters at one surface point, sample k drawn from key0 + k (the
+// material tests' Monte Carlo estimates).
+int oracle_scatter_batch(const lmn_scene_desc* desc, uint32_t material, const double ray_dir[3],
+                         const double point[3], const double normal[3], int front_face, double t, double u, double v,
+                         uint64_t key0, uint32_t n, double* out_dir, double* out_att, uint8_t* out_ok) {
+  World w;
+  if (!desc || material >= desc->n_materials || !out_dir || !out_att || !out_ok) {
+    g_err = "bad argument";
+    return LMN_ERR_INVALID;
+  }
+  if (!build_world(*desc, false, w)) return LMN_ERR_INVALID;
+  Ray r{Vec3(0, 0, 0), v3(ray_dir), 0.0};
+  HitRecord h;
+  h.p = v3(point);
+  h.n = v3(normal);
+  h.front = front_face != 0;
+  h.t = t;
+  h.u = u;
+  h.v = v;
+  h.mat = w.mats[material].get();
+  for (uint32_t k = 0; k < n; ++k) {
+    CounterRng rng(key0 + k, 0);
+    Ray out;
+    Vec3 att;
+    out_ok[k] = h.mat->scatter(r, h, rng, out, att) ? 1 : 0;
+    out_dir[3 * k] = out.d.x;
+    out_dir[3 * k + 1] = out.d.y;
+    out_dir[3 * k + 2] = out.d.z;
+    out_att[3 * k] = att.x;
+    out_att[3 * k + 1] = att.y;
+    out_att[3 * k + 2] = att.z;
+  }
+  return LMN_OK;
+}
+
+// The reference-stream generator (rand 0.9.2 SmallRng = xoshiro256++):
+// n u64 outputs from the raw state s[4] (state != NULL), or from
+// seed_from_u64(seed); f64 draws = (u64 >> 11) * 2^-53 into out_f64 when given.
+int oracle_xoshiro(const uint64_t* state, uint64_t seed, uint32_t n, uint64_t* out_u64, double* out_f64) {
+  if (!out_u64) {
+    g_err = "null argument";
+    return LMN_ERR_INVALID;
+  }
+  XoshiroRng g(seed);
+  if (state)
+    for (int i = 0; i < 4; ++i) g.s[i] = state[i];
+  for (uint32_t k = 0; k < n; ++k) {
+    XoshiroRng c = g;
+    out_u64[k] = g.next_u64();
+    if (out_f64) out_f64[k] = c.next_f64();
+  }
+  return LMN_OK;
+}
+
 // ---- single-function evaluations for the known-answer tests ---------------
 
 int oracle_nee(const lmn_scene_desc* desc, const double point[3], const double normal[3], const double albedo[3],

@@ -20,7 +20,10 @@
  *   ORACLE_RNG_REFERENCE rand 0.9.2 SmallRng = xoshiro256++ seeded by
  *                        SplitMix64 per row (src/render.rs:1608), restated
  *                        from the published algorithm (Blackman & Vigna).
- *                        No reference test pins its output: parity unpinned.
+ *                        The generator is pinned by the published outputs of
+ *                        the reference implementation and of rand's
+ *                        seed_from_u64 (tests/test_oracle_pins.py); the per-row
+ *                        seeding and draw order follow the cited call sites.
  */
 #ifndef LMN_ORACLE_H
 #define LMN_ORACLE_H

@@ -647,8 +647,9 @@ def test_determinism_and_seed_dependence():
 
 
 def test_reference_rng_mode_converges_to_counter_mode():
-    # the xoshiro256++ row streams (parity unpinned: no reference test pins
-    # their values) and the counter streams estimate the same image
+    # the xoshiro256++ row streams (their generator pinned by its published
+    # outputs in test_oracle_pins.py) and the counter streams estimate the
+    # same image
     cfg, world = _render_cfg(FURNACE, spp=256)
     a, va, _ = oracle.render(world, cfg, threads=8)
     b, vb, _ = oracle.render(world, cfg, threads=8, rng_mode=oracle.RNG_REFERENCE)

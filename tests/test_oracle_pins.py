@@ -1,0 +1,368 @@
+"""Pins the oracle's material sampling, backgrounds, texel lookup and
+reference random stream (CPU only, no GPU).
+
+Each check is independent of the oracle's own code: an expectation from a
+deterministic quadrature of the published model, a distribution's closed-form
+CDF, a textbook law (Snell, Schlick, Beer), a closed-form value taken from
+the reference's definition at chosen points, or the published generator's
+known outputs.  The Monte Carlo side draws from the oracle
+(oracle_scatter_batch); bars are 4 standard errors.
+
+References: src/material.rs:59-71 (Lambertian), :89-107 (metal), :138-214
+(dielectric), :248-391 (microfacet GGX); src/render.rs:152-168 (sun);
+src/scene.rs:1119-1131 (sun defaults, sunset); src/texture.rs:463-479
+(image texture); src/vec3.rs:96-122 (random_in_unit_sphere /
+random_unit_vector); rand 0.9.2 SmallRng (xoshiro256++, seed_from_u64).
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import luminara_amd as lm
+from luminara_amd import _abi
+import oracle
+
+UP = np.array([0.0, 1.0, 0.0])
+
+
+def _mat_world(mat_toml):
+    return lm.load_scene(f'[[sphere]]\ncenter = [0.0, -5.0, 0.0]\nradius = 1.0\nmaterial = {mat_toml}\n')[2]
+
+
+def _within(mc, se, expect, k=4.0):
+    mc, se, expect = np.atleast_1d(mc), np.atleast_1d(se), np.atleast_1d(expect)
+    assert np.all(np.abs(mc - expect) <= k * se + 1e-12), (mc, expect, se)
+
+
+# ------------------------------------------------------------------ GGX microfacet
+
+def _schlick(c, f0):
+    return f0 + (1.0 - f0) * (1.0 - c) ** 5
+
+
+def ggx_expected_attenuation(roughness, metallic, albedo, cos_v, n_theta=3000, n_phi=720):
+    """E[attenuation] of one Microfacet::scatter at incidence cos_v, by
+    quadrature of the Cook-Torrance model the sampler estimates (Walter et
+    al. 2007): GGX D with alpha = roughness^2, Schlick-Smith G with
+    k = (roughness + 1)^2 / 8, Schlick F with F0 = lerp(0.04, albedo,
+    metallic), integrated over half vectors m with weight D(m) F(v.m)
+    G(n.v, n.l) (v.m) / (n.v) where l mirrors v about m and n.l > 0, n.m >
+    0.001; plus the diffuse lobe's (1 - F(n.v)) (1 - metallic) albedo.
+    Returns (expectation per channel, integral of D(m) cos(theta_m))."""
+    albedo = np.asarray(albedo, dtype=np.float64)
+    alpha = roughness * roughness
+    f0 = 0.04 * (1.0 - metallic) + albedo * metallic
+    nv = max(cos_v, 0.001)
+    v = np.array([math.sqrt(1.0 - cos_v * cos_v), cos_v, 0.0])
+    x = (np.arange(n_theta) + 0.5) / n_theta  # midpoint rule in x, theta = (pi/2) x^3 (dense near the normal)
+    th = 0.5 * math.pi * x ** 3
+    dth = 1.5 * math.pi * x ** 2 / n_theta
+    ph = (np.arange(n_phi) + 0.5) * (2.0 * math.pi / n_phi)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    W = (np.sin(T) * dth[:, None]) * (2.0 * math.pi / n_phi)
+    m = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], axis=-1)
+    nm = m[..., 1]
+    D = alpha ** 2 / (math.pi * (nm ** 2 * (alpha ** 2 - 1.0) + 1.0) ** 2)
+    vm = m @ v
+    l = 2.0 * vm[..., None] * m - v
+    nl = l[..., 1]
+    k = (roughness + 1.0) ** 2 / 8.0
+    G = (nv / (nv * (1 - k) + k)) * (nl / (nl * (1 - k) + k))
+    ok = (nl > 0.0) & (nm > 0.001) & (vm > 0.0)
+    base = np.where(ok, D * G * vm / nv, 0.0) * W
+    spec = np.array([np.sum(base * _schlick(np.clip(vm, 0.0, 1.0), f0[c])) for c in range(3)])
+    diffuse = (1.0 - _schlick(nv, f0)) * (1.0 - metallic) * albedo
+    return spec + diffuse, float(np.sum(D * nm * W))
+
+
+@pytest.mark.parametrize("roughness,metallic,albedo,cos_v", [
+    (0.3, 1.0, (0.9, 0.6, 0.3), 0.9),
+    (0.6, 1.0, (0.95, 0.8, 0.5), 0.5),
+    (0.8, 0.0, (0.8, 0.8, 0.8), 0.3),
+    (0.5, 0.4, (0.2, 0.7, 0.9), 0.7),
+])
+def test_microfacet_mean_weight_matches_cook_torrance_quadrature(roughness, metallic, albedo, cos_v):
+    expect, norm = ggx_expected_attenuation(roughness, metallic, albedo, cos_v)
+    assert abs(norm - 1.0) < 2e-4  # the quadrature integrates GGX's projected area to 1
+    w = _mat_world(f'{{ type = "pbr", color = [{albedo[0]}, {albedo[1]}, {albedo[2]}], '
+                   f'roughness = {roughness}, metallic = {metallic} }}')
+    v = np.array([math.sqrt(1.0 - cos_v * cos_v), cos_v, 0.0])
+    n = 60000
+    ok, dirs, att = oracle.scatter_batch(w, 0, -v, [0, 0, 0], UP, True, n, key0=1000)
+    att = np.where(ok[:, None], att, 0.0)
+    mean, se = att.mean(axis=0), att.std(axis=0) / math.sqrt(n)
+    _within(mean, se, expect)
+    assert np.all(dirs[ok] @ UP > 0.0)
+
+
+def test_microfacet_half_vectors_follow_ggx():
+    # at normal incidence the half vector of a specular sample is (v + l)/|v + l|;
+    # sampled with density D(m) cos(theta_m), whose CDF in theta is
+    # tan^2 / (alpha^2 + tan^2); n.l > 0 keeps theta_m < 45 degrees.  metallic
+    # = 1: diffuse samples carry zero attenuation, so att > 0 marks specular ones
+    rough = 0.5
+    alpha = rough * rough
+    albedo = (0.9, 0.7, 0.5)
+    w = _mat_world(f'{{ type = "pbr", color = [0.9, 0.7, 0.5], roughness = {rough}, metallic = 1.0 }}')
+    n = 40000
+    ok, dirs, att = oracle.scatter_batch(w, 0, -UP, [0, 0, 0], UP, True, n, key0=7)
+    spec = ok & (att.sum(axis=1) > 0.0)
+    h = dirs[spec] / np.linalg.norm(dirs[spec], axis=1)[:, None] + UP
+    h /= np.linalg.norm(h, axis=1)[:, None]
+    th = np.arccos(np.clip(h[:, 1], -1.0, 1.0))
+    cdf45 = 1.0 / (alpha ** 2 + 1.0)
+
+    def cdf(t):
+        t2 = np.tan(t) ** 2
+        return (t2 / (alpha ** 2 + t2)) / cdf45
+
+    assert stats.kstest(th, cdf).pvalue > 1e-4
+    assert stats.kstest(np.arctan2(h[:, 2], h[:, 0]), stats.uniform(-math.pi, 2 * math.pi).cdf).pvalue > 1e-4
+    # branch and truncation probabilities: p = mean F0 (F(1) = F0); absorbed =
+    # p * (1 - CDF(45 deg)); diffuse (att = 0, ok) = 1 - p
+    p = float(np.mean(albedo))
+    for frac, want in [((~ok).mean(), p * (1.0 - cdf45)), ((ok & ~spec).mean(), 1.0 - p)]:
+        _within(frac, math.sqrt(want * (1 - want) / n), want)
+
+
+# ------------------------------------------------------------------ Lambertian
+
+def test_lambertian_directions_are_cosine_distributed():
+    # n + random_unit_vector: density cos(theta)/pi, so cos^2(theta) ~ U[0,1]
+    # and the azimuth is uniform
+    w = _mat_world('{ type = "lambertian", color = [0.5, 0.5, 0.5] }')
+    ok, dirs, _ = oracle.scatter_batch(w, 0, [0.3, -1.0, 0.2], [0, 0, 0], UP, True, 40000, key0=99)
+    assert ok.all()
+    u = dirs / np.linalg.norm(dirs, axis=1)[:, None]
+    assert stats.kstest(u[:, 1] ** 2, "uniform").pvalue > 1e-4
+    assert stats.kstest(np.arctan2(u[:, 2], u[:, 0]), stats.uniform(-math.pi, 2 * math.pi).cdf).pvalue > 1e-4
+
+
+# ------------------------------------------------------------------ metal fuzz
+
+@pytest.mark.parametrize("fuzz", [0.3, 1.0])
+def test_metal_fuzz_is_a_uniform_ball_around_the_mirror(fuzz):
+    # reflect(unit(d), n) + fuzz * p, p uniform in the unit ball: at normal
+    # incidence the mirror is n, no sample goes below the surface, |d - n| <
+    # fuzz, E[d - n] = 0 and E|d - n|^2 = 3/5 fuzz^2 (the ball's second moment)
+    w = _mat_world(f'{{ type = "metal", color = [0.9, 0.8, 0.7], fuzz = {fuzz} }}')
+    n = 40000
+    ok, dirs, att = oracle.scatter_batch(w, 0, -UP, [0, 0, 0], UP, True, n, key0=5)
+    assert ok.all()
+    np.testing.assert_allclose(att, np.tile([0.9, 0.8, 0.7], (n, 1)))
+    e = dirs - UP
+    r2 = np.sum(e * e, axis=1)
+    assert r2.max() < fuzz * fuzz
+    _within(e.mean(axis=0), e.std(axis=0) / math.sqrt(n), [0.0, 0.0, 0.0])
+    _within(r2.mean(), r2.std() / math.sqrt(n), 0.6 * fuzz * fuzz)
+
+
+# ------------------------------------------------------------------ dielectric
+
+def _schlick_scalar(c, eta):
+    r0 = ((1.0 - eta) / (1.0 + eta)) ** 2
+    return r0 + (1.0 - r0) * (1.0 - c) ** 5
+
+
+@pytest.mark.parametrize("cos_i", [0.9, 0.5, 0.2])
+def test_dielectric_entering_obeys_snell_and_schlick(cos_i):
+    # from outside (front face, normal up, eta = 1/1.5): refracted rays bend
+    # toward the normal with sin_t = sin_i / 1.5 in the plane of incidence;
+    # the reflected share is Schlick's R(cos_i) with r0 = ((1 - eta)/(1 + eta))^2
+    w = _mat_world('{ type = "dielectric", refraction_index = 1.5 }')
+    s_i = math.sqrt(1.0 - cos_i * cos_i)
+    ud = np.array([s_i, -cos_i, 0.0])
+    n = 40000
+    ok, dirs, att = oracle.scatter_batch(w, 0, ud, [0, 0, 0], UP, True, n, key0=11)
+    assert ok.all()
+    refl = dirs[:, 1] > 0.0
+    r = dirs[refl]
+    np.testing.assert_allclose(r, np.tile([s_i, cos_i, 0.0], (len(r), 1)), atol=1e-12)
+    t = dirs[~refl]
+    t /= np.linalg.norm(t, axis=1)[:, None]
+    np.testing.assert_allclose(t[:, 0], s_i / 1.5, atol=1e-12)
+    np.testing.assert_allclose(t[:, 2], 0.0, atol=1e-15)
+    R = _schlick_scalar(cos_i, 1.0 / 1.5)
+    _within(refl.mean(), math.sqrt(R * (1 - R) / n), R)
+
+
+def test_dielectric_leaving_snell_total_reflection_and_beer():
+    # from inside (back face: the normal given faces the ray, eta = 1.5):
+    # below the critical angle sin_t = 1.5 sin_i, above it every ray reflects;
+    # a tinted glass attenuates by exp(-(1 - tint) t) on the way out
+    w = _mat_world('{ type = "dielectric", refraction_index = 1.5, tint = [0.8, 0.6, 0.4] }')
+    down = -UP  # the facing normal of the inside of the upper surface
+    for s_i, tir in [(0.4, False), (0.8, True)]:
+        c_i = math.sqrt(1.0 - s_i * s_i)
+        ud = np.array([s_i, c_i, 0.0])
+        n = 20000
+        ok, dirs, att = oracle.scatter_batch(w, 0, ud, [0, 0, 0], down, False, n, t=2.0, key0=3)
+        assert ok.all()
+        np.testing.assert_allclose(att, np.tile(np.exp(-(1.0 - np.array([0.8, 0.6, 0.4])) * 2.0), (n, 1)),
+                                   rtol=1e-14)
+        refl = dirs[:, 1] < 0.0
+        if tir:
+            assert refl.all()
+        else:
+            t = dirs[~refl] / np.linalg.norm(dirs[~refl], axis=1)[:, None]
+            np.testing.assert_allclose(t[:, 0], 1.5 * s_i, atol=1e-12)
+            R = _schlick_scalar(c_i, 1.5)
+            _within(refl.mean(), math.sqrt(R * (1 - R) / n), R)
+    # entering, the tint is the attenuation itself (no distance)
+    ok, _, att = oracle.scatter_batch(w, 0, [0.0, -1.0, 0.0], [0, 0, 0], UP, True, 10, t=2.0)
+    np.testing.assert_allclose(att, np.tile([0.8, 0.6, 0.4], (10, 1)))
+
+
+def test_dielectric_dispersion_channels():
+    # a uniform channel pick (1/3 each) sets attenuation 3 in that channel and
+    # ior -+ dispersion; the refracted direction obeys Snell for that ior
+    disp = 0.05
+    w = _mat_world(f'{{ type = "dielectric", refraction_index = 1.5, dispersion = {disp} }}')
+    s_i = math.sqrt(1.0 - 0.7 * 0.7)
+    n = 30000
+    ok, dirs, att = oracle.scatter_batch(w, 0, [s_i, -0.7, 0.0], [0, 0, 0], UP, True, n, key0=21)
+    assert ok.all()
+    ch = np.argmax(att, axis=1)
+    np.testing.assert_array_equal(att, 3.0 * np.eye(3)[ch])
+    for c in range(3):
+        _within((ch == c).mean(), math.sqrt(2.0 / 9.0 / n), 1.0 / 3.0)
+        ior = 1.5 + (-disp, 0.0, disp)[c]
+        sel = (ch == c) & (dirs[:, 1] < 0.0)
+        t = dirs[sel] / np.linalg.norm(dirs[sel], axis=1)[:, None]
+        np.testing.assert_allclose(t[:, 0], s_i / ior, atol=1e-12)
+
+
+def test_frosted_dielectric_stays_within_roughness_of_the_mirror():
+    # beyond the critical angle every ray reflects; roughness adds
+    # roughness * p, p in the unit ball
+    w = _mat_world('{ type = "dielectric", refraction_index = 1.5, roughness = 0.3 }')
+    s_i = 0.8
+    ud = np.array([s_i, math.sqrt(1 - s_i * s_i), 0.0])
+    ok, dirs, _ = oracle.scatter_batch(w, 0, ud, [0, 0, 0], -UP, False, 5000, key0=2)
+    mirror = ud - 2.0 * (ud @ -UP) * -UP
+    assert ok.all() and np.linalg.norm(dirs - mirror, axis=1).max() < 0.3
+
+
+# ------------------------------------------------------------------ sun background
+
+def _rotate_toward(a, angle):
+    a = np.asarray(a, dtype=np.float64) / np.linalg.norm(a)
+    p = np.cross(a, [0.0, 0.0, 1.0])
+    p /= np.linalg.norm(p)
+    return math.cos(angle) * a + math.sin(angle) * p
+
+
+def _sky(d, sky):
+    t = 0.5 * (d[1] / np.linalg.norm(d) + 1.0)
+    return (1.0 - t) * np.ones(3) + t * np.asarray(sky)
+
+
+@pytest.mark.parametrize("bg,sun_dir,sun_color,intensity,sky", [
+    ('{ type = "sun", direction = [0.35, 0.6, -0.45] }', [0.35, 0.6, -0.45], (1.0, 0.95, 0.85), 20.0,
+     (0.5, 0.7, 1.0)),
+    ('{ type = "sun", direction = [0.0, 1.0, 1.0], sun_color = [0.9, 0.5, 0.1], intensity = 7.0, '
+     'sky_color = [0.2, 0.3, 0.9] }', [0.0, 1.0, 1.0], (0.9, 0.5, 0.1), 7.0, (0.2, 0.3, 0.9)),
+    ('{ type = "sunset" }', [0.3, 0.15, -1.0], (1.0, 0.6, 0.2), 30.0, (0.8, 0.4, 0.2)),
+])
+def test_sun_background_closed_forms(bg, sun_dir, sun_color, intensity, sky):
+    # Background::Sun: the disk (cos > 0.9995) is sun_color * intensity; the
+    # halo (0.995 < cos <= 0.9995) adds sun_color * intensity * 0.3 * glow to
+    # the sky gradient, glow rising linearly from 0 to 1 across it; beyond it
+    # the gradient lerp(white, sky_color, (y + 1) / 2) of the unit direction
+    _, _, w = lm.load_scene(f"[render]\nbackground = {bg}\n")
+    sc, s = np.asarray(sun_color), np.asarray(sun_dir) / np.linalg.norm(sun_dir)
+    for scale in (1.0, 3.7):  # ray directions are not unit
+        np.testing.assert_allclose(oracle.background(w, scale * s), sc * intensity, rtol=1e-12)
+        d = _rotate_toward(s, math.acos(0.9996))
+        np.testing.assert_allclose(oracle.background(w, scale * d), sc * intensity, rtol=1e-12)
+        for c in (0.997, 0.9951, 0.9994):
+            d = _rotate_toward(s, math.acos(c))
+            glow = (d @ s - 0.995) / (0.9995 - 0.995)
+            np.testing.assert_allclose(oracle.background(w, scale * d), _sky(d, sky) + sc * intensity * 0.3 * glow,
+                                       rtol=1e-9)
+        for c in (0.99, 0.5, -0.8):
+            d = _rotate_toward(s, math.acos(c))
+            np.testing.assert_allclose(oracle.background(w, scale * d), _sky(d, sky), rtol=1e-12)
+
+
+# ------------------------------------------------------------------ image texture lookup
+
+def _texels(h, w):
+    # distinct bytes per texel and channel; values (k + 0.5)/255 so that the
+    # AOV's truncation to bytes gives k back
+    k = np.arange(h * w * 3, dtype=np.float64).reshape(h, w, 3) % 251
+    return k, ((k + 0.5) / 255.0).astype(np.float32)
+
+
+def _textured(world, rgb):
+    t = world.add_texture(rgb)
+    m = world.materials()[0]
+    m.texture = t
+    world.set_material(0, m)
+
+
+@pytest.mark.parametrize("u,v,ij", [
+    (0.0, 0.0, (2, 0)), (1.0, 1.0, (0, 3)), (0.0, 1.0, (0, 0)), (1.0, 0.0, (2, 3)),  # corners, v flipped
+    (-0.5, 2.0, (0, 0)), (1.5, -3.0, (2, 3)),                                         # clamped
+    (0.2499, 0.6666, (1, 0)), (0.25, 0.6667, (0, 1)), (0.74, 0.34, (1, 2)),           # cell edges (w 4, h 3)
+])
+def test_image_texture_lookup_known_answers(u, v, ij):
+    # ImageTexture::value: u, v clamped to [0, 1], v flipped (row 0 = top),
+    # nearest texel, index clamped to the last row / column
+    w = _mat_world('{ type = "lambertian", color = [0.0, 0.0, 0.0] }')
+    k, rgb = _texels(3, 4)
+    _textured(w, rgb)
+    ok, _, att = oracle.scatter_batch(w, 0, -UP, [0, 0, 0], UP, True, 1, u=u, v=v)
+    np.testing.assert_array_equal(att[0], rgb[ij].astype(np.float64))
+
+
+def test_image_texture_on_a_quad_through_the_aov():
+    # a quad q + a u + b v facing the camera reports (u, v) = (a, b); the AOV
+    # albedo bytes of each pixel are those of texel (floor((1 - b) h),
+    # floor(a w)); pixels on a texel boundary are left out
+    from test_post import quad_scene
+    cfg, cam, world = quad_scene('{ type = "lambertian", color = [0.0, 0.0, 0.0] }', width=192, height=144)
+    k, rgb = _texels(5, 7)
+    _textured(world, rgb)
+    _, _, albedo = oracle.aov(world, cfg)
+    o = np.array(cam.origin[:])
+    ll, hz, vt = (np.array(x[:]) for x in (cam.lower_left, cam.horizontal, cam.vertical))
+    j, i = np.mgrid[0:cfg.height, 0:cfg.width]
+    uu = (i + 0.5) / (cfg.width - 1)
+    vv = ((cfg.height - 1 - j) + 0.5) / (cfg.height - 1)
+    d = ll + uu[..., None] * hz + vv[..., None] * vt - o
+    p = o + ((5.0 - o[2]) / d[..., 2])[..., None] * d
+    a, b = (p[..., 0] + 2.0) / 4.0, (p[..., 1] + 1.5) / 3.0
+    inside = (a > 1e-9) & (a < 1 - 1e-9) & (b > 1e-9) & (b < 1 - 1e-9)
+    ti, tj = np.minimum(np.floor(a * 7), 6).astype(int), np.minimum(np.floor((1.0 - b) * 5), 4).astype(int)
+    clear = (np.abs(a * 7 - np.round(a * 7)) > 1e-6) & (np.abs((1 - b) * 5 - np.round((1 - b) * 5)) > 1e-6)
+    sel = inside & clear
+    assert sel.sum() > 2000
+    want = k[np.clip(tj, 0, 4), np.clip(ti, 0, 6)]
+    np.testing.assert_array_equal(albedo[sel], want[sel].astype(np.uint8))
+    assert len(np.unique(want[sel][:, 0])) == 35  # every texel seen
+
+
+# ------------------------------------------------------------------ reference stream
+
+def test_xoshiro256plusplus_reference_outputs():
+    # outputs of the reference implementation (xoshiro256plusplus.c) from
+    # the state {1, 2, 3, 4}, as rand_xoshiro / rand's SmallRng tests record them
+    want = [41943041, 58720359, 3588806011781223, 3591011842654386, 9228616714210784205, 9973669472204895162,
+            14011001112246962877, 12406186145184390807, 15849039046786891736, 10450023813501588000]
+    u, f = oracle.xoshiro(10, state=[1, 2, 3, 4])
+    assert [int(x) for x in u] == want
+    # f64 draws: the top 53 bits times 2^-53
+    np.testing.assert_array_equal(f, np.array([float(x >> 11) * 2.0 ** -53 for x in want]))
+
+
+def test_smallrng_seed_from_u64_outputs():
+    # SmallRng::seed_from_u64(0): the state filled by SplitMix64 (rand 0.9's
+    # value-stability test of seed_from_u64)
+    want = [5987356902031041503, 7051070477665621255, 6633766593972829180, 211316841551650330,
+            9136120204379184874, 379361710973160858, 15813423377499357806, 15596884590815070553,
+            5439680534584881407, 1369371744833522710]
+    u, _ = oracle.xoshiro(10, seed=0)
+    assert [int(x) for x in u] == want
