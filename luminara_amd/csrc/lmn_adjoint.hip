@@ -94,12 +94,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     uint32_t nv = 0;
     V3<R> L = {R(0), R(0), R(0)}, beta = {R(1), R(1), R(1)}, tail = {R(0), R(0), R(0)};
     bool skip = false;
+    uint32_t org = kNoOrigin;  // FP32: the record a bounce ray leaves from, as in the plain kernels
     // ---- replay (the plain kernels' loop, recording vertices)
     for (uint32_t depth = F.max_depth; depth > 0; --depth) {
       PrivateStack<R> st;
       R t;
       uint32_t idx;
-      if (!trace<R, FEAT, false>(S, o, d, R(0.001), R(INFINITY), t, idx, st)) {
+      if (!trace<R, FEAT, false>(S, o, d, R(0.001), R(INFINITY), t, idx, st, nullptr, org)) {
         V3<R> bg = background(S.bg, d);
         L = L + hadamard(beta, bg);
         tail = bg;
@@ -184,6 +185,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
       o = sf.p;
       d = ndir;
       skip = use_nee;
+      if (sizeof(R) == 4) org = idx;
       tail = {R(0), R(0), R(0)};  // depth runs out: ray_color(depth 0) is black
     }
     // ---- the pixel's adjoint: filter weight and luminance clamp (FP64 as the resolve)

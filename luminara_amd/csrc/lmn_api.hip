@@ -72,7 +72,7 @@ hipError_t upload(DevBuf& b, const std::vector<T>& v) {
 struct Workspace {
   std::mutex mu;
   DevBuf st[2][4];  // ob, db, bk, meta for the two pools
-  DevBuf hit, hit_b, sh_o, sh_d, sh_c, samples, acc, spill, ctr, tc, rays, out, views, parts, part_begin;
+  DevBuf hit, hit_b, hit_rec, sh_o, sh_d, sh_c, samples, acc, spill, ctr, tc, rays, out, views, parts, part_begin;
   DevBuf adj_wsum, adj_gtex, adj_part, adj_red, adj_dldi, adj_tex_out;  // path-replay adjoint
   DevBuf mq_cand, mq_key, mq_ovf, mq_brk;  // marched-pass candidate queues
   uint32_t pool = 0;
@@ -549,6 +549,7 @@ int wavefront_run(lmn_scene* const* scenes, const lmn_render_params* const* para
     for (int f = 0; f < 4; ++f) HIP_TRY(W.st[k][f].ensure(size_t(pool) * 16));
   HIP_TRY(W.hit.ensure(size_t(pool) * 16));
   HIP_TRY(W.hit_b.ensure(size_t(pool) * 16));
+  HIP_TRY(W.hit_rec.ensure(size_t(pool) * 4));
   HIP_TRY(W.sh_o.ensure(size_t(pool) * 16));
   HIP_TRY(W.sh_d.ensure(size_t(pool) * 16));
   HIP_TRY(W.sh_c.ensure(size_t(pool) * 16));
@@ -587,7 +588,7 @@ int wavefront_run(lmn_scene* const* scenes, const lmn_render_params* const* para
   L.pool = pool;
   L.stack_lds = ss;
   L.feat = feat;
-  L.hit = HitSoA{W.hit.as<float4>(), W.hit_b.as<float4>()};
+  L.hit = HitSoA{W.hit.as<float4>(), W.hit_b.as<float4>(), W.hit_rec.as<uint32_t>()};
   L.sh = ShadowSoA{W.sh_o.as<float4>(), W.sh_d.as<float4>(), W.sh_c.as<float4>()};
   L.samples = W.samples.as<float4>();
   L.ctr = W.ctr.as<WaveCounters>();

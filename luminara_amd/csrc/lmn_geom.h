@@ -132,7 +132,7 @@ template <class R> LMN_HD R t_report_factor(const XformT<R>& x, V3<R> d_obj, V3<
 
 // Sphere::hit (src/sphere.rs:24-62)
 template <class R>
-LMN_HD bool hit_sphere(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t) {
+LMN_HD bool hit_sphere(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t, bool far_only = false) {
   V3<R> c = {p[0], p[1], p[2]};
   R r = p[3];
   V3<R> oc = o - c;
@@ -154,7 +154,7 @@ LMN_HD bool hit_sphere(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t) {
     R t0 = cc / q, t1 = q / a;
     R tn = t0 < t1 ? t0 : t1, tf = t0 < t1 ? t1 : t0;
     if (q == R(0)) tn = tf = -half_b / a;
-    root = tn;
+    root = far_only ? tf : tn;
     if (root < tmin || root > tmax) {
       root = tf;
       if (root < tmin || root > tmax) return false;
@@ -163,7 +163,7 @@ LMN_HD bool hit_sphere(const R* p, V3<R> o, V3<R> d, R tmin, R tmax, R& t) {
     R disc = half_b * half_b - a * cc;
     if (disc < R(0)) return false;
     R sq = vsqrt(disc);
-    root = (-half_b - sq) / a;
+    root = far_only ? (-half_b + sq) / a : (-half_b - sq) / a;
     if (root < tmin || root > tmax) {
       root = (-half_b + sq) / a;
       if (root < tmin || root > tmax) return false;

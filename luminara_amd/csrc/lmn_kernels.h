@@ -92,6 +92,7 @@ struct BatchDev {
 struct HitSoA {
   float4* a;
   float4* b;
+  uint32_t* rec;  // the hit's primitive record: the origin record of the path's next ray
 };
 constexpr uint32_t HIT_MISS = 0xffffffffu;
 

@@ -114,6 +114,7 @@ __device__ __forceinline__ void store_hit(const SceneView<float>& S, HitSoA hit,
   surface<float, FEAT>(S, idx, o, d, t, s);
   hit.a[i] = make_float4(s.n.x, s.n.y, s.n.z, s.t_true);
   hit.b[i] = make_float4(s.u, s.v, __uint_as_float(s.mat | (s.front ? 0x80000000u : 0u)), s.t);
+  hit.rec[i] = idx;
 }
 __device__ __forceinline__ void store_miss(HitSoA hit, uint32_t i) {
   hit.b[i] = make_float4(0.f, 0.f, __uint_as_float(HIT_MISS), INFINITY);
@@ -140,13 +141,15 @@ __device__ __forceinline__ void extend_body(const BatchDev& B, const PathSoA& cu
   uint32_t gid = blockIdx.x * BS + threadIdx.x;
   for (uint32_t i = gid; i < n; i += nthreads) {
     float4 a = cur.ob[i], b = cur.db[i];
-    uint32_t sid = cur.aux[i].z;
+    const uint4 x = cur.aux[i];
+    const uint32_t sid = x.z, org = x.w;  // org: the record the ray leaves from
     for_scene(sid, [&](uint32_t s) {
       const SceneView<float> S = B.views[s];
       LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
       float t = INFINITY;
       uint32_t idx = 0;
-      bool h = trace_pass<float, FEAT, false, PASS_MAIN>(S, f3(a), f3(b), 0.001f, t, idx, st, COUNT ? tc : nullptr);
+      bool h = trace_pass<float, FEAT, false, PASS_MAIN>(S, f3(a), f3(b), 0.001f, t, idx, st, COUNT ? tc : nullptr,
+                                                         org);
       if (h)
         store_hit<FEAT>(S, hit, i, f3(a), f3(b), t, idx);
       else
@@ -602,6 +605,7 @@ __global__ __launch_bounds__(BS) void k_shade(BatchDev B, PathSoA cur, HitSoA hi
     bool skip = (M.y >> 16) & 1u;
     float4 HA = hit.a[j], HB = hit.b[j];
     uint32_t mword = __float_as_uint(HB.z);
+    const uint32_t rec = mword == HIT_MISS ? kNoOrigin : hit.rec[j];
     V3<float> add = {0.f, 0.f, 0.f};
     bool cont = false, emit = false;
     V3<float> sp_o = {0.f, 0.f, 0.f}, sp_d = {0.f, 0.f, 0.f}, ndir = {0.f, 0.f, 0.f}, nbeta = beta;
@@ -664,7 +668,7 @@ __global__ __launch_bounds__(BS) void k_shade(BatchDev B, PathSoA cur, HitSoA hi
       nxt.ob[pn] = make_float4(sp_o.x, sp_o.y, sp_o.z, nbeta.x);
       nxt.db[pn] = make_float4(ndir.x, ndir.y, ndir.z, nbeta.y);
       nxt.meta[pn] = make_uint4(item, (depth - 1) | (use_nee ? 0x10000u : 0u), M.z, M.w);
-      nxt.aux[pn] = make_uint4(rng.n, __float_as_uint(nbeta.z), sid, 0u);
+      nxt.aux[pn] = make_uint4(rng.n, __float_as_uint(nbeta.z), sid, rec);
     }
   }
 }
@@ -703,7 +707,7 @@ __global__ __launch_bounds__(BS) void k_regen(BatchDev B, PathSoA nxt, float4* _
       nxt.ob[slot] = make_float4(o.x, o.y, o.z, 1.f);
       nxt.db[slot] = make_float4(d.x, d.y, d.z, 1.f);
       nxt.meta[slot] = make_uint4(uint32_t(item), F.max_depth, uint32_t(rng.key), uint32_t(rng.key >> 32));
-      nxt.aux[slot] = make_uint4(rng.n, __float_as_uint(1.f), F.scene, 0u);
+      nxt.aux[slot] = make_uint4(rng.n, __float_as_uint(1.f), F.scene, kNoOrigin);
     });
   }
 }

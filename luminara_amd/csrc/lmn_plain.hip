@@ -19,12 +19,15 @@ __device__ V3<R> plain_sample(const SceneView<R>& S, const FrameDev& F, uint64_t
   generate_item<R>(F, S.cam, k, rng, o, d, w, pix);
   V3<R> L = {R(0), R(0), R(0)}, beta = {R(1), R(1), R(1)};
   bool skip = false;
+  // FP32: the record a bounce ray leaves from (test_prim's origin rule); the
+  // FP64 kernel keeps the reference's t >= 0.001 alone, as the oracle does
+  uint32_t org = kNoOrigin;
   for (uint32_t depth = F.max_depth; depth > 0; --depth) {
     PrivateStack<R> st;
     R t;
     uint32_t idx;
     ++n_ext;
-    if (!trace<R, FEAT, false>(S, o, d, R(0.001), R(INFINITY), t, idx, st)) {
+    if (!trace<R, FEAT, false>(S, o, d, R(0.001), R(INFINITY), t, idx, st, nullptr, org)) {
       L = L + hadamard(beta, background(S.bg, d));
       break;
     }
@@ -66,6 +69,7 @@ __device__ V3<R> plain_sample(const SceneView<R>& S, const FrameDev& F, uint64_t
     o = sf.p;
     d = ndir;
     skip = use_nee;
+    if (sizeof(R) == 4) org = idx;
   }
   return L;
 }

@@ -149,10 +149,32 @@ LMN_HD bool prim_is64(uint32_t km) {
   }
 }
 
+// The record a ray leaves from (FP32 extension rays), or none.
+constexpr uint32_t kNoOrigin = 0xffffffffu;
+
+// A ray that leaves the surface of record `org` cannot meet that surface
+// again if it is planar, and meets a sphere it leaves only at the far root,
+// and only if it heads inward.  That is what the reference's t >= 0.001
+// decides in FP64; an FP32 hit point lies up to a few ulps off the surface,
+// and with the reference's unnormalised scatter directions (n + a unit
+// vector can be arbitrarily short) 0.001 in t can be less than that, so the
+// FP32 path would meet its own surface again.  `sphere_far` is set for an
+// origin sphere; false = skip the record.
+template <class R>
+LMN_HD bool origin_rule(uint32_t kind, const R* p, V3<R> o, V3<R> d, bool& sphere_far) {
+  sphere_far = false;
+  if (kind != K_SPHERE) return kind != K_QUAD && kind != K_RECT_XY && kind != K_RECT_XZ && kind != K_RECT_YZ &&
+                               kind != K_TRIANGLE && kind != K_PLANE;
+  sphere_far = dot(o - V3<R>{p[0], p[1], p[2]}, d) < R(0);
+  return sphere_far;
+}
+
 // Intersects primitive `idx` (transform applied); on a hit in
-// [tmin, closest] updates closest to the reference-reported t.
+// [tmin, closest] updates closest to the reference-reported t.  `org`: the
+// record the ray leaves from (FP32 extension rays, see origin_rule).
 template <class R, uint32_t FEAT>
-LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R tmin, R& closest) {
+LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R tmin, R& closest,
+                      uint32_t org = kNoOrigin) {
   const PrimT<R>& pr = S.prims[idx];
   uint32_t km = pr.kind_mat;
   uint32_t kind = km & KIND_MASK;
@@ -165,6 +187,10 @@ LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t
   }
   R t;
   bool hit;
+  bool far_only = false;
+  if constexpr (sizeof(R) == 4) {
+    if (idx == org && !origin_rule<R>(kind, pr.p, oo, dd, far_only)) return false;
+  }
   if (prim_is64<R, FEAT>(km)) {
     V3<double> o64 = cvt<double>(o), d64 = cvt<double>(d);
     if (has_xf) {
@@ -173,12 +199,16 @@ LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t
       d64 = xf_dir(X64.inv, d64);
     }
     double t64;
-    if constexpr ((FEAT & FEAT_MARCH) != 0)
+    if (far_only)
+      hit = hit_sphere<double>(S.prims64[idx].p, o64, d64, double(tmin), double(closest), t64, true);
+    else if constexpr ((FEAT & FEAT_MARCH) != 0)
       hit = hit_prim_fp64(kind, S.prims64[idx].p, o64, d64, double(tmin), double(closest), S.exact_marched != 0, t64);
     else  // the main BVH's FP64 records are spheres (the second BVH holds the rest)
       hit = hit_sphere<double>(S.prims64[idx].p, o64, d64, double(tmin), double(closest), t64);
     t = R(t64);
     if (hit && !(t <= closest)) hit = false;  // rounding back to R must not leave the range
+  } else if (far_only) {
+    hit = hit_sphere<R>(pr.p, oo, dd, tmin, closest, t, true);
   } else {
     hit = hit_prim_analytic<R>(kind, pr.p, oo, dd, tmin, closest, t);
   }
@@ -297,7 +327,7 @@ LMN_HD void bvh_leaves(const NodeF* nodes, V3<float> o, V3<float> inv, float tmi
 template <class R, uint32_t FEAT, bool ANY, class Stack>
 LMN_HD void bvh_walk(const SceneView<R>& S, const typename NodeSel<R>::type* nodes, V3<R> o, V3<R> d, V3<R> inv,
                      R tmin, R& closest, uint32_t& best, bool& found, Stack& st, uint32_t& nf, uint32_t& pt,
-                     uint32_t& tt, bool counting) {
+                     uint32_t& tt, bool counting, uint32_t org = kNoOrigin) {
   using Node = typename NodeSel<R>::type;
   Node nd = nodes[0];
   ++nf;
@@ -309,7 +339,7 @@ LMN_HD void bvh_walk(const SceneView<R>& S, const typename NodeSel<R>::type* nod
       for (uint32_t k = 0; k < count; ++k) {
         ++pt;
         if (counting && (S.prims[first + k].kind_mat & KIND_MASK) == K_TRIANGLE) ++tt;
-        if (test_prim<R, FEAT>(S, first + k, o, d, tmin, closest)) {
+        if (test_prim<R, FEAT>(S, first + k, o, d, tmin, closest, org)) {
           found = true;
           best = first + k;
           if (ANY) break;
@@ -360,7 +390,7 @@ LMN_HD void bvh_walk(const SceneView<R>& S, const typename NodeSel<R>::type* nod
 template <uint32_t FEAT, bool ANY, class Stack>
 LMN_HD bool bvh4_step(const SceneView<float>& S, V3<float> o, V3<float> d, V3<float> inv, float tmin,
                       float& closest, uint32_t& best, bool& found, uint32_t& ref, Stack& st, uint32_t& nf,
-                      uint32_t& pt, uint32_t& tt, bool counting) {
+                      uint32_t& pt, uint32_t& tt, bool counting, uint32_t org = kNoOrigin) {
   if (!(ref & LEAF4)) {
     const Node4F& nd = S.nodes4[ref];
     ++nf;
@@ -417,7 +447,7 @@ LMN_HD bool bvh4_step(const SceneView<float>& S, V3<float> o, V3<float> d, V3<fl
     for (uint32_t k = 0; k < count; ++k) {
       ++pt;
       if (counting && (S.prims[first + k].kind_mat & KIND_MASK) == K_TRIANGLE) ++tt;
-      if (test_prim<float, FEAT>(S, first + k, o, d, tmin, closest)) {
+      if (test_prim<float, FEAT>(S, first + k, o, d, tmin, closest, org)) {
         found = true;
         best = first + k;
         if (ANY) return true;
@@ -445,9 +475,9 @@ LMN_HD bool bvh4_step(const SceneView<float>& S, V3<float> o, V3<float> d, V3<fl
 template <uint32_t FEAT, bool ANY, class Stack>
 LMN_HD void bvh4_walk(const SceneView<float>& S, V3<float> o, V3<float> d, V3<float> inv, float tmin,
                       float& closest, uint32_t& best, bool& found, Stack& st, uint32_t& nf, uint32_t& pt,
-                      uint32_t& tt, bool counting) {
+                      uint32_t& tt, bool counting, uint32_t org = kNoOrigin) {
   uint32_t ref = 0;
-  while (!bvh4_step<FEAT, ANY>(S, o, d, inv, tmin, closest, best, found, ref, st, nf, pt, tt, counting)) {
+  while (!bvh4_step<FEAT, ANY>(S, o, d, inv, tmin, closest, best, found, ref, st, nf, pt, tt, counting, org)) {
   }
 }
 
@@ -463,23 +493,24 @@ enum : uint32_t { PASS_ALL = 0, PASS_MAIN = 1, PASS_MARCH = 2 };
 // t_max (or a previous pass's hit) and leaves as the closest hit's t.
 template <class R, uint32_t FEAT, bool ANY, uint32_t PASS = PASS_ALL, class Stack>
 LMN_HD bool trace_pass(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R& closest, uint32_t& best, Stack& st,
-                       TraceCounters* cnt = nullptr) {
+                       TraceCounters* cnt = nullptr, uint32_t org = kNoOrigin) {
   bool found = false;
   V3<R> inv = {R(1) / d.x, R(1) / d.y, R(1) / d.z};
   uint32_t nf = 0, pt = 0, tt = 0;
   constexpr uint32_t MAIN_FEAT = FEAT & ~FEAT_MARCH;
   if constexpr (sizeof(R) == 4) {
     if (PASS != PASS_MARCH && S.n_nodes4 > 0)
-      bvh4_walk<MAIN_FEAT, ANY>(S, o, d, inv, tmin, closest, best, found, st, nf, pt, tt, cnt != nullptr);
+      bvh4_walk<MAIN_FEAT, ANY>(S, o, d, inv, tmin, closest, best, found, st, nf, pt, tt, cnt != nullptr, org);
   } else {
     if (PASS != PASS_MARCH && S.n_nodes > 0)
-      bvh_walk<R, MAIN_FEAT, ANY>(S, S.nodes, o, d, inv, tmin, closest, best, found, st, nf, pt, tt, cnt != nullptr);
+      bvh_walk<R, MAIN_FEAT, ANY>(S, S.nodes, o, d, inv, tmin, closest, best, found, st, nf, pt, tt, cnt != nullptr,
+                                  org);
   }
   if (PASS != PASS_MARCH && !(ANY && found)) {
     for (uint32_t k = 0; k < S.n_planes; ++k) {
       uint32_t idx = S.n_bounded + k;
       ++pt;
-      if (test_prim<R, MAIN_FEAT>(S, idx, o, d, tmin, closest)) {
+      if (test_prim<R, MAIN_FEAT>(S, idx, o, d, tmin, closest, org)) {
         found = true;
         best = idx;
         if (ANY) break;
@@ -507,10 +538,10 @@ LMN_HD bool trace_pass(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R& close
 
 template <class R, uint32_t FEAT, bool ANY, class Stack>
 LMN_HD bool trace(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R tmax, R& t_out, uint32_t& idx_out, Stack& st,
-                  TraceCounters* cnt = nullptr) {
+                  TraceCounters* cnt = nullptr, uint32_t org = kNoOrigin) {
   R closest = tmax;
   uint32_t best = 0;
-  bool found = trace_pass<R, FEAT, ANY, PASS_ALL>(S, o, d, tmin, closest, best, st, cnt);
+  bool found = trace_pass<R, FEAT, ANY, PASS_ALL>(S, o, d, tmin, closest, best, st, cnt, org);
   t_out = closest;
   idx_out = best;
   return found;
