@@ -68,6 +68,10 @@ extern "C" {
  *                   4 ramp)                                 src/wedge.rs:22-120
  *   ROUNDED_BOX     center[3], half_size[3], radius         src/rounded_box.rs:10
  *   MOBIUS          center[3], radius, width, thickness     src/mobius.rs:11
+ *   MESH_TRIANGLE   v0[3], v1[3], v2[3], attr: a MeshTriangle with per-vertex
+ *                   normals and/or texture coordinates (an OBJ face with vn /
+ *                   vt indices); p[9] = index into lmn_scene_desc.vertex_attrs
+ *                                                           src/obj.rs:250-313
  * A capsule (src/capsule.rs:15-53) is three prims of one object: CYLINDER
  * (y from center.y to center.y + height), then the bottom and top SPHERE.
  * A prism is eight FACETs of one object (bottom, top, then two per side), a
@@ -93,7 +97,8 @@ enum {
   LMN_PRIM_WEDGE_FACE = 17,
   LMN_PRIM_ROUNDED_BOX = 18,
   LMN_PRIM_MOBIUS = 19,
-  LMN_PRIM_KIND_COUNT = 20
+  LMN_PRIM_MESH_TRIANGLE = 20,
+  LMN_PRIM_KIND_COUNT = 21
 };
 
 typedef struct lmn_prim {
@@ -210,6 +215,18 @@ typedef struct lmn_render_config {
 } lmn_render_config;
 
 /* ---- scene descriptor ---------------------------------------------------- */
+/* Per-vertex attributes of a MESH_TRIANGLE (src/obj.rs:250-313).  With
+ * normals the outward normal at barycentrics (u, v) is
+ * unit(n0 (1 - u - v) + n1 u + n2 v); with uvs the texture coordinates are
+ * uv0 (1 - u - v) + uv1 u + uv2 v (otherwise the geometric normal and (u, v)).
+ * The loader stores the OBJ normals already made unit, as the reference does. */
+typedef struct lmn_vertex_attr {
+  double normal[3][3];
+  double uv[3][2];
+  uint32_t has_normals;
+  uint32_t has_uvs;
+} lmn_vertex_attr;
+
 typedef struct lmn_scene_desc {
   uint32_t n_prims;
   uint32_t n_transforms;
@@ -224,6 +241,8 @@ typedef struct lmn_scene_desc {
   const lmn_light* lights;
   lmn_camera camera;
   lmn_render_config config; /* [render] settings as loaded (CLI applies on top) */
+  uint64_t n_vertex_attrs;
+  const lmn_vertex_attr* vertex_attrs; /* MESH_TRIANGLE attributes */
 } lmn_scene_desc;
 
 /* ---- per-call render parameters ----------------------------------------- */
@@ -335,6 +354,13 @@ int lmn_desc_add_material(lmn_scene_desc* desc, const lmn_material* m, uint32_t*
  * ASCII PLY path for large meshes. */
 int lmn_desc_add_triangles(lmn_scene_desc* desc, const double* verts, uint64_t n_verts, const uint32_t* tris,
                            uint64_t n_tris, uint32_t material, const uint32_t* materials);
+/* The same with per-corner attributes: normals n_tris*9 doubles (the three
+ * corners' normals of each triangle, unit as obj.rs makes them) and/or uvs
+ * n_tris*6 doubles (NULL = absent).  Triangles with an attribute become
+ * MESH_TRIANGLE prims (src/obj.rs:250-313), the rest TRIANGLE prims. */
+int lmn_desc_add_mesh(lmn_scene_desc* desc, const double* verts, uint64_t n_verts, const uint32_t* tris,
+                      uint64_t n_tris, const double* normals, const double* uvs, uint32_t material,
+                      const uint32_t* materials);
 
 /* Flatten, build the SAH BVH and upload a scene to the current HIP device. */
 typedef struct lmn_scene lmn_scene;

@@ -104,6 +104,21 @@ class SceneWorld:
             len(t), material, None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint32))))
         self.invalidate()
 
+    def add_mesh(self, verts: np.ndarray, tris: np.ndarray, normals=None, uvs=None, material: int = 0,
+                 materials=None):
+        """Appends a triangle mesh with per-corner normals (n_tris x 3 x 3) and/or
+        texture coordinates (n_tris x 3 x 2): MESH_TRIANGLE prims."""
+        v = np.ascontiguousarray(verts, dtype=np.float64)
+        t = np.ascontiguousarray(tris, dtype=np.uint32)
+        nrm = None if normals is None else np.ascontiguousarray(normals, dtype=np.float64).reshape(len(t), 9)
+        uv = None if uvs is None else np.ascontiguousarray(uvs, dtype=np.float64).reshape(len(t), 6)
+        m = None if materials is None else np.ascontiguousarray(materials, dtype=np.uint32)
+        check(lib().lmn_desc_add_mesh(
+            self._desc, v.ctypes.data_as(C.POINTER(C.c_double)), len(v), t.ctypes.data_as(C.POINTER(C.c_uint32)),
+            len(t), None if nrm is None else nrm.ctypes.data, None if uv is None else uv.ctypes.data, material,
+            None if m is None else m.ctypes.data_as(C.POINTER(C.c_uint32))))
+        self.invalidate()
+
     def set_texture(self, index, rgb: np.ndarray):
         rgb = np.ascontiguousarray(rgb, dtype=np.float32)
         h, w, _ = rgb.shape

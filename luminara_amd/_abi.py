@@ -20,11 +20,11 @@ LMN_ERR_NOMEM = 6
 
 PRIM_KINDS = ["sphere", "quad", "rect_xy", "rect_xz", "rect_yz", "triangle", "plane", "torus",
               "superellipsoid", "spring", "disk", "cylinder", "cone", "ellipsoid", "hemisphere", "annulus",
-              "facet", "wedge_face", "rounded_box", "mobius"]
+              "facet", "wedge_face", "rounded_box", "mobius", "mesh_triangle"]
 (PRIM_SPHERE, PRIM_QUAD, PRIM_RECT_XY, PRIM_RECT_XZ, PRIM_RECT_YZ, PRIM_TRIANGLE, PRIM_PLANE,
  PRIM_TORUS, PRIM_SUPERELLIPSOID, PRIM_SPRING, PRIM_DISK, PRIM_CYLINDER, PRIM_CONE,
  PRIM_ELLIPSOID, PRIM_HEMISPHERE, PRIM_ANNULUS, PRIM_FACET, PRIM_WEDGE_FACE, PRIM_ROUNDED_BOX,
- PRIM_MOBIUS) = range(20)
+ PRIM_MOBIUS, PRIM_MESH_TRIANGLE) = range(21)
 MAT_LAMBERTIAN, MAT_METAL, MAT_DIELECTRIC, MAT_EMISSIVE, MAT_MICROFACET = range(5)
 LIGHT_SPHERE, LIGHT_RECT, LIGHT_DISK = range(3)
 BG_SKY, BG_SOLID, BG_GRADIENT, BG_SUN = range(4)
@@ -82,12 +82,18 @@ class RenderConfig(C.Structure):
                 ("adaptive", C.c_uint32), ("pad", C.c_uint32), ("adaptive_threshold", C.c_double)]
 
 
+class VertexAttr(C.Structure):
+    _fields_ = [("normal", (C.c_double * 3) * 3), ("uv", (C.c_double * 2) * 3), ("has_normals", C.c_uint32),
+                ("has_uvs", C.c_uint32)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("n_prims", C.c_uint32), ("n_transforms", C.c_uint32), ("n_materials", C.c_uint32),
                 ("n_textures", C.c_uint32), ("n_lights", C.c_uint32), ("n_objects", C.c_uint32),
                 ("prims", C.POINTER(Prim)), ("transforms", C.POINTER(Transform)),
                 ("materials", C.POINTER(Material)), ("textures", C.POINTER(Texture)),
-                ("lights", C.POINTER(Light)), ("camera", Camera), ("config", RenderConfig)]
+                ("lights", C.POINTER(Light)), ("camera", Camera), ("config", RenderConfig),
+                ("n_vertex_attrs", C.c_uint64), ("vertex_attrs", C.POINTER(VertexAttr))]
 
 
 class RenderParams(C.Structure):
@@ -175,6 +181,8 @@ SIGNATURES = {
     "lmn_desc_add_material": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(Material), C.POINTER(C.c_uint32)]),
     "lmn_desc_add_triangles": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_double), C.c_uint64,
                                          C.POINTER(C.c_uint32), C.c_uint64, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "lmn_desc_add_mesh": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_uint32),
+                                    C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "lmn_scene_create": (C.c_int, [C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
     "lmn_scene_destroy": (None, [C.c_void_p]),
     "lmn_scene_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),

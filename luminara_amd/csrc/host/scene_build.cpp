@@ -139,7 +139,8 @@ static void object_bounds(const lmn_prim& p, double lo[3], double hi[3]) {
     case LMN_PRIM_RECT_YZ:  // src/rect.rs:119-124
       lo[1] = q[0]; hi[1] = q[1]; lo[2] = q[2]; hi[2] = q[3]; lo[0] = q[4] - 0.0001; hi[0] = q[4] + 0.0001;
       break;
-    case LMN_PRIM_TRIANGLE:  // src/triangle.rs:66-70
+    case LMN_PRIM_TRIANGLE:       // src/triangle.rs:66-70
+    case LMN_PRIM_MESH_TRIANGLE:  // src/obj.rs:315-319
       for (int k = 0; k < 3; ++k) {
         lo[k] = std::min(std::min(q[k], q[3 + k]), q[6 + k]) - 1e-4;
         hi[k] = std::max(std::max(q[k], q[3 + k]), q[6 + k]) + 1e-4;
@@ -760,7 +761,60 @@ void fill_camera(const lmn_camera& c, CameraT<R>& o) {
 
 }  // namespace
 
+namespace {
+int build_scene_tables(const lmn_scene_desc& desc, const BvhOptions& opt, HostScene& out, std::string& err);
+}  // namespace
+
+// A MESH_TRIANGLE is a triangle to everything but the surface: the tables
+// are built from a copy of the prims with those marked TRIANGLE, and their
+// records then get FLAG_SMOOTH and the index of their attributes.
 int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScene& out, std::string& err) {
+  std::vector<lmn_prim> prims(desc.prims, desc.prims + desc.n_prims);
+  std::vector<int64_t> attr_of(desc.n_prims, -1);
+  for (uint32_t i = 0; i < desc.n_prims; ++i) {
+    if (prims[i].kind != LMN_PRIM_MESH_TRIANGLE) continue;
+    const double a = prims[i].p[9];
+    if (!(a >= 0.0) || a != std::floor(a) || a >= double(desc.n_vertex_attrs) || !desc.vertex_attrs ||
+        a >= 4294967295.0) {
+      err = "prim " + std::to_string(i) + ": vertex attribute index out of range";
+      return LMN_ERR_INVALID;
+    }
+    attr_of[i] = int64_t(a);
+    prims[i].kind = LMN_PRIM_TRIANGLE;
+    prims[i].p[9] = 0.0;
+  }
+  lmn_scene_desc d = desc;
+  d.prims = prims.data();
+  int rc = build_scene_tables(d, opt, out, err);
+  if (rc != LMN_OK) return rc;
+  out.attrF.assign(std::max<uint64_t>(desc.n_vertex_attrs, 1), AttrT<float>{});
+  out.attrD.assign(std::max<uint64_t>(desc.n_vertex_attrs, 1), AttrT<double>{});
+  for (uint64_t i = 0; i < desc.n_vertex_attrs; ++i) {
+    const lmn_vertex_attr& a = desc.vertex_attrs[i];
+    const uint32_t flags = (a.has_normals ? 1u : 0u) | (a.has_uvs ? 2u : 0u);
+    for (int k = 0; k < 9; ++k) {
+      out.attrF[i].n[k] = float(a.normal[k / 3][k % 3]);
+      out.attrD[i].n[k] = a.normal[k / 3][k % 3];
+    }
+    for (int k = 0; k < 6; ++k) {
+      out.attrF[i].uv[k] = float(a.uv[k / 2][k % 2]);
+      out.attrD[i].uv[k] = a.uv[k / 2][k % 2];
+    }
+    out.attrF[i].flags = out.attrD[i].flags = flags;
+  }
+  for (size_t r = 0; r < out.order.size(); ++r) {
+    const int64_t a = attr_of[out.order[r]];
+    if (a < 0) continue;
+    out.primsF[r].kind_mat |= FLAG_SMOOTH;
+    out.primsD[r].kind_mat |= FLAG_SMOOTH;
+    set_attr_index(out.primsF[r], uint32_t(a));
+    set_attr_index(out.primsD[r], uint32_t(a));
+  }
+  return LMN_OK;
+}
+
+namespace {
+int build_scene_tables(const lmn_scene_desc& desc, const BvhOptions& opt, HostScene& out, std::string& err) {
   // validate references
   for (uint32_t i = 0; i < desc.n_prims; ++i) {
     const lmn_prim& p = desc.prims[i];
@@ -936,5 +990,6 @@ int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScen
   fill_camera(desc.camera, out.camD);
   return LMN_OK;
 }
+}  // namespace
 
 }  // namespace lmn

@@ -35,6 +35,8 @@ struct HostScene {
   std::vector<float> texels;
   std::vector<LightT<float>> lightF;
   std::vector<LightT<double>> lightD;
+  std::vector<AttrT<float>> attrF;   // vertex attributes of smooth mesh triangles
+  std::vector<AttrT<double>> attrD;
   BackgroundT<float> bgF;
   BackgroundT<double> bgD;
   CameraT<float> camF;

@@ -33,6 +33,8 @@ void OwnedDesc::sync() {
   desc.materials = materials.data();
   desc.textures = textures.data();
   desc.lights = lights.data();
+  desc.n_vertex_attrs = vertex_attrs.size();
+  desc.vertex_attrs = vertex_attrs.data();
 }
 
 namespace {
@@ -854,17 +856,24 @@ class Loader {
       std::stringstream ss;
       ss << in.rdbuf();
       std::string content = ss.str();
-      if (f.str.size() < 4 || f.str.compare(f.str.size() - 4, 4, ".ply") != 0)
-        fail(LMN_ERR_UNSUPPORTED, "OBJ meshes (smooth normals/uvs) are not on the GPU path: '" + f.str + "'");
       std::vector<double> tv;
+      std::vector<lmn_vertex_attr> ta;
       std::string err;
-      int rc = load_ply_text(content, scale, off, tv, err);
+      const bool ply = f.str.size() >= 4 && f.str.compare(f.str.size() - 4, 4, ".ply") == 0;
+      int rc = ply ? load_ply_text(content, scale, off, tv, err) : load_obj_text(content, scale, off, tv, ta, err);
       if (rc != LMN_OK) fail(rc, err);
       size_t nt = tv.size() / 9;
       out_.prims.reserve(out_.prims.size() + nt);
       for (size_t t = 0; t < nt; ++t) {
-        lmn_prim& p = add_prim(LMN_PRIM_TRIANGLE, mat, -1, new_object());
+        // a face with normals or uvs is a MeshTriangle with attributes; the
+        // rest shade exactly as plain triangles (geometric normal, (u, v))
+        const bool smooth = !ta.empty() && (ta[t].has_normals || ta[t].has_uvs);
+        lmn_prim& p = add_prim(smooth ? LMN_PRIM_MESH_TRIANGLE : LMN_PRIM_TRIANGLE, mat, -1, new_object());
         for (int k = 0; k < 9; ++k) p.p[k] = tv[t * 9 + k];
+        if (smooth) {
+          p.p[9] = double(out_.vertex_attrs.size());
+          out_.vertex_attrs.push_back(ta[t]);
+        }
       }
     }
     for (auto& s : table_array(root, "box")) {  // src/scene.rs:1561-1600
@@ -959,6 +968,111 @@ class Loader {
 };
 
 }  // namespace
+
+int load_obj_text(const std::string& content, double scale, const double offset[3], std::vector<double>& tv,
+                  std::vector<lmn_vertex_attr>& ta, std::string& err) {
+  // src/obj.rs:18-153.  Numbers as Rust parses them: an index is an unsigned
+  // decimal, a coordinate anything strtod takes whole.
+  struct Corner {
+    uint64_t pos, uv, nrm;  // 1-based; 0 = absent
+  };
+  std::vector<double> verts, uvs, nrms;
+  std::vector<std::array<Corner, 3>> faces;
+  std::istringstream in(content);
+  std::string line;
+  size_t line_num = 0;
+  auto fail_line = [&](const std::string& what) {
+    err = "Line " + std::to_string(line_num) + ": " + what;
+    return LMN_ERR_PARSE;
+  };
+  auto num = [](const std::string& w, double& x) {
+    char* end = nullptr;
+    x = std::strtod(w.c_str(), &end);
+    return !w.empty() && *end == 0;
+  };
+  auto index = [](const std::string& w, uint64_t& v) {
+    if (w.empty() || w[0] == '-' || w[0] == '+') return false;
+    char* end = nullptr;
+    v = std::strtoull(w.c_str(), &end, 10);
+    return *end == 0;
+  };
+  while (std::getline(in, line)) {
+    ++line_num;
+    std::istringstream ls(line);
+    std::vector<std::string> parts;
+    std::string w;
+    while (ls >> w) parts.push_back(w);
+    if (parts.empty() || parts[0][0] == '#') continue;
+    const std::string& tag = parts[0];
+    if (tag == "v" || tag == "vn") {
+      if (parts.size() < 4) return fail_line(tag == "v" ? "vertex needs 3 coordinates" : "normal needs 3 components");
+      double x[3];
+      for (int k = 0; k < 3; ++k)
+        if (!num(parts[1 + k], x[k])) return fail_line("invalid float literal");
+      if (tag == "v") {
+        for (int k = 0; k < 3; ++k) verts.push_back(x[k] * scale + offset[k]);
+      } else {  // Vec3::unit: divided by its length
+        double l = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        for (int k = 0; k < 3; ++k) nrms.push_back(x[k] / l);
+      }
+    } else if (tag == "vt") {
+      if (parts.size() < 3) return fail_line("texture coord needs 2 values");
+      double u, v;
+      if (!num(parts[1], u) || !num(parts[2], v)) return fail_line("invalid float literal");
+      uvs.push_back(u);
+      uvs.push_back(v);
+    } else if (tag == "f") {
+      std::vector<Corner> fv;
+      for (size_t i = 1; i < parts.size(); ++i) {
+        std::vector<std::string> seg;
+        size_t b = 0;
+        for (;;) {
+          size_t e = parts[i].find('/', b);
+          seg.push_back(parts[i].substr(b, e == std::string::npos ? std::string::npos : e - b));
+          if (e == std::string::npos) break;
+          b = e + 1;
+        }
+        Corner c{0, 0, 0};
+        if (!index(seg[0], c.pos)) return fail_line("invalid digit found in string");
+        if (seg.size() >= 2 && !seg[1].empty() && !index(seg[1], c.uv)) return fail_line("invalid digit found in string");
+        if (seg.size() >= 3 && !seg[2].empty() && !index(seg[2], c.nrm)) return fail_line("invalid digit found in string");
+        fv.push_back(c);
+      }
+      if (fv.size() < 3) return fail_line("face needs at least 3 vertices");
+      for (size_t i = 1; i + 1 < fv.size(); ++i) faces.push_back({fv[0], fv[i], fv[i + 1]});  // fan
+    }
+  }
+  const uint64_t nv = verts.size() / 3, nt = uvs.size() / 2, nn = nrms.size() / 3;
+  tv.reserve(tv.size() + faces.size() * 9);
+  ta.reserve(ta.size() + faces.size());
+  for (const auto& f : faces) {
+    for (const Corner& c : f)
+      if (c.pos == 0 || c.pos > nv) {
+        err = "Vertex index " + std::to_string(c.pos) + " out of range";
+        return LMN_ERR_PARSE;
+      }
+    for (const Corner& c : f)
+      for (int k = 0; k < 3; ++k) tv.push_back(verts[(c.pos - 1) * 3 + k]);
+    lmn_vertex_attr a{};
+    // attributes only when every corner names one that exists (a 0 index
+    // wraps in the reference's `n - 1` and finds nothing)
+    bool hn = true, hu = true;
+    for (const Corner& c : f) {
+      hn = hn && c.nrm >= 1 && c.nrm <= nn;
+      hu = hu && c.uv >= 1 && c.uv <= nt;
+    }
+    for (int j = 0; j < 3; ++j) {
+      for (int k = 0; k < 3 && hn; ++k) a.normal[j][k] = nrms[(f[j].nrm - 1) * 3 + k];
+      for (int k = 0; k < 2 && hu; ++k) a.uv[j][k] = uvs[(f[j].uv - 1) * 2 + k];
+    }
+    a.has_normals = hn ? 1u : 0u;
+    a.has_uvs = hu ? 1u : 0u;
+    ta.push_back(a);
+  }
+  fprintf(stderr, "Loaded OBJ: %llu vertices, %llu tex coords, %llu normals, %llu triangles\n",
+          (unsigned long long)nv, (unsigned long long)nt, (unsigned long long)nn, (unsigned long long)faces.size());
+  return LMN_OK;
+}
 
 int load_ply_text(const std::string& content, double scale, const double offset[3], std::vector<double>& tv,
                   std::string& err) {

@@ -19,6 +19,7 @@ struct OwnedDesc {
   std::vector<std::vector<float>> texels;
   std::vector<lmn_texture> textures;
   std::vector<lmn_light> lights;
+  std::vector<lmn_vertex_attr> vertex_attrs;
   uint32_t n_objects = 0;
   void sync();  // refresh desc pointers/counts after the vectors change
 };
@@ -30,5 +31,13 @@ int load_scene_toml(const std::string& text, const std::string& base_dir, OwnedD
 // scaled then offset.  Appends triangles to tri_verts (9 doubles each).
 int load_ply_text(const std::string& content, double scale, const double offset[3],
                   std::vector<double>& tri_verts, std::string& err);
+
+// OBJ mesh (src/obj.rs:18-153): v / vt / vn / f lines, fan-triangulated
+// faces, vertices scaled then offset, normals made unit.  Appends 9 vertex
+// coordinates per triangle to tri_verts and, per triangle, its corner
+// normals and uvs (tri_attrs; has_normals / has_uvs 0 where the face lacks
+// them or an index is out of range).
+int load_obj_text(const std::string& content, double scale, const double offset[3],
+                  std::vector<double>& tri_verts, std::vector<lmn_vertex_attr>& tri_attrs, std::string& err);
 
 }  // namespace lmn

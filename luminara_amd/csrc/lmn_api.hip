@@ -103,7 +103,8 @@ Workspace& workspace() {
 
 struct lmn_scene {
   HostScene host;
-  DevBuf nodes4F, nodesF, nodesD, mnodesF, mnodesD, primsF, primsD, xfF, xfD, matF, matD, tex, texels, lightF, lightD;
+  DevBuf nodes4F, nodesF, nodesD, mnodesF, mnodesD, primsF, primsD, xfF, xfD, matF, matD, tex, texels, lightF, lightD,
+      attrF, attrD;
   SceneView<float> viewF{};
   SceneView<double> viewD{};
   lmn_camera camera{};
@@ -177,28 +178,31 @@ int lmn_desc_add_material(lmn_scene_desc* desc, const lmn_material* m, uint32_t*
   return LMN_OK;
 }
 
-int lmn_desc_add_triangles(lmn_scene_desc* desc, const double* verts, uint64_t n_verts, const uint32_t* tris,
-                           uint64_t n_tris, uint32_t material, const uint32_t* materials) {
-  if (!desc || (n_tris && (!verts || !tris))) return set_err(LMN_ERR_INVALID, "lmn_desc_add_triangles: null argument");
+int lmn_desc_add_mesh(lmn_scene_desc* desc, const double* verts, uint64_t n_verts, const uint32_t* tris,
+                      uint64_t n_tris, const double* normals, const double* uvs, uint32_t material,
+                      const uint32_t* materials) {
+  if (!desc || (n_tris && (!verts || !tris))) return set_err(LMN_ERR_INVALID, "lmn_desc_add_mesh: null argument");
   auto* od = reinterpret_cast<OwnedDesc*>(desc);
   if (od->prims.size() + n_tris >= (1ull << 31)) return set_err(LMN_ERR_INVALID, "too many primitives");
   // every index is checked before the description changes: a failed call
   // leaves it (and its exported views) exactly as they were
   for (uint64_t t = 0; t < n_tris; ++t) {
     if ((materials ? materials[t] : material) >= od->materials.size())
-      return set_err(LMN_ERR_INVALID, "lmn_desc_add_triangles: material out of range");
+      return set_err(LMN_ERR_INVALID, "lmn_desc_add_mesh: material out of range");
     for (int c = 0; c < 3; ++c)
-      if (tris[t * 3 + c] >= n_verts) return set_err(LMN_ERR_INVALID, "lmn_desc_add_triangles: vertex index out of range");
+      if (tris[t * 3 + c] >= n_verts) return set_err(LMN_ERR_INVALID, "lmn_desc_add_mesh: vertex index out of range");
   }
+  const bool smooth = normals || uvs;
   try {
     od->prims.reserve(od->prims.size() + n_tris);
+    if (smooth) od->vertex_attrs.reserve(od->vertex_attrs.size() + n_tris);
   } catch (const std::bad_alloc&) {
     od->sync();
     return set_err(LMN_ERR_NOMEM, "out of host memory");
   }
   for (uint64_t t = 0; t < n_tris; ++t) {
     lmn_prim p{};
-    p.kind = LMN_PRIM_TRIANGLE;
+    p.kind = smooth ? LMN_PRIM_MESH_TRIANGLE : LMN_PRIM_TRIANGLE;
     p.material = materials ? materials[t] : material;
     p.transform = -1;
     p.object = od->n_objects++;
@@ -206,10 +210,24 @@ int lmn_desc_add_triangles(lmn_scene_desc* desc, const double* verts, uint64_t n
       const uint32_t vi = tris[t * 3 + c];
       for (int k = 0; k < 3; ++k) p.p[c * 3 + k] = verts[size_t(vi) * 3 + k];
     }
-    od->prims.push_back(p);  // capacity reserved above: no reallocation
+    if (smooth) {
+      lmn_vertex_attr a{};
+      a.has_normals = normals ? 1u : 0u;
+      a.has_uvs = uvs ? 1u : 0u;
+      for (int k = 0; k < 9 && normals; ++k) a.normal[k / 3][k % 3] = normals[t * 9 + k];
+      for (int k = 0; k < 6 && uvs; ++k) a.uv[k / 2][k % 2] = uvs[t * 6 + k];
+      p.p[9] = double(od->vertex_attrs.size());
+      od->vertex_attrs.push_back(a);  // capacity reserved above: no reallocation
+    }
+    od->prims.push_back(p);
   }
   od->sync();
   return LMN_OK;
+}
+
+int lmn_desc_add_triangles(lmn_scene_desc* desc, const double* verts, uint64_t n_verts, const uint32_t* tris,
+                           uint64_t n_tris, uint32_t material, const uint32_t* materials) {
+  return lmn_desc_add_mesh(desc, verts, n_verts, tris, n_tris, nullptr, nullptr, material, materials);
 }
 
 int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
@@ -240,9 +258,11 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   HIP_TRY(upload(s->texels, h.texels));
   HIP_TRY(upload(s->lightF, h.lightF));
   HIP_TRY(upload(s->lightD, h.lightD));
+  HIP_TRY(upload(s->attrF, h.attrF));
+  HIP_TRY(upload(s->attrD, h.attrD));
   s->device_bytes = s->nodes4F.bytes + s->nodesF.bytes + s->nodesD.bytes + s->mnodesF.bytes + s->mnodesD.bytes + s->primsF.bytes + s->primsD.bytes + s->xfF.bytes +
                     s->xfD.bytes + s->matF.bytes + s->matD.bytes + s->tex.bytes + s->texels.bytes +
-                    s->lightF.bytes + s->lightD.bytes;
+                    s->lightF.bytes + s->lightD.bytes + s->attrF.bytes + s->attrD.bytes;
   SceneView<float>& F = s->viewF;
   F.nodes4 = s->nodes4F.as<Node4F>();
   F.n_nodes4 = uint32_t(h.nodes4F.size());
@@ -257,6 +277,7 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   F.textures = s->tex.as<TextureDev>();
   F.texels = s->texels.as<float>();
   F.lights = s->lightF.as<LightT<float>>();
+  F.attrs = s->attrF.as<AttrT<float>>();
   F.n_nodes = uint32_t(h.nodesF.size());
   F.n_bounded = h.n_bounded;
   F.n_planes = h.n_planes;
@@ -277,6 +298,7 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   D.textures = F.textures;
   D.texels = F.texels;
   D.lights = s->lightD.as<LightT<double>>();
+  D.attrs = s->attrD.as<AttrT<double>>();
   D.n_nodes = F.n_nodes;
   D.n_bounded = F.n_bounded;
   D.n_planes = F.n_planes;

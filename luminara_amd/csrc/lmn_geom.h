@@ -37,6 +37,7 @@ LMN_HD bool is_marched_kind(uint32_t k) { return k >= K_TORUS && k <= K_SPRING; 
 LMN_HD bool is_quadric_kind(uint32_t k) { return k >= K_DISK; }
 constexpr uint32_t KIND_MASK = 0x1f;
 constexpr uint32_t FLAG_FP64 = 0x20;  // FP32 path evaluates this primitive in FP64
+constexpr uint32_t FLAG_SMOOTH = 0x40;  // triangle with vertex attributes (p[9]: AttrT index)
 constexpr uint32_t LEAF_BIT = 0x80000000u;
 constexpr uint32_t XF_NONE = 0xffffffffu;
 
@@ -58,6 +59,23 @@ struct alignas(16) PrimT {
 };
 static_assert(sizeof(PrimT<float>) == 64, "FP32 primitive record must be 64 bytes");
 static_assert(sizeof(PrimT<double>) == 128, "FP64 primitive record must be 128 bytes");
+
+// Vertex attributes of a smooth mesh triangle (MeshTriangle, src/obj.rs:250-313):
+// the corners' normals and texture coordinates.
+template <class R>
+struct alignas(16) AttrT {
+  R n[9];          // n0, n1, n2
+  R uv[6];         // uv0, uv1, uv2
+  uint32_t flags;  // 1: normals, 2: uvs
+};
+static_assert(sizeof(AttrT<float>) == 64, "FP32 vertex attributes must be 64 bytes");
+static_assert(sizeof(AttrT<double>) == 128, "FP64 vertex attributes must be 128 bytes");
+// The AttrT index of a FLAG_SMOOTH triangle record: float records keep its
+// bits in p[9], double records its value.
+LMN_HD uint32_t attr_index(const PrimT<float>& r) { return bits_of(r.p[9]); }
+LMN_HD uint32_t attr_index(const PrimT<double>& r) { return uint32_t(r.p[9]); }
+LMN_HD void set_attr_index(PrimT<float>& r, uint32_t i) { r.p[9] = float_of(i); }
+LMN_HD void set_attr_index(PrimT<double>& r, uint32_t i) { r.p[9] = double(i); }
 
 struct NodeF {
   float lo[3];

@@ -59,6 +59,7 @@ struct SceneView {
   const TextureDev* textures;
   const float* texels;
   const LightT<R>* lights;
+  const AttrT<R>* attrs;          // vertex attributes of smooth mesh triangles
   uint32_t n_nodes, n_mnodes, n_bounded, n_planes, n_lights;
   uint32_t n_nodes4;
   BackgroundT<R> bg;
@@ -613,6 +614,21 @@ LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_r
   } else {
     n = prim_normal_analytic<R>(kind, pr.p, oo + dd * t_true);
     prim_uv_analytic<R>(kind, pr.p, oo, dd, t_true, s.u, s.v);
+    if (kind == K_TRIANGLE && (km & FLAG_SMOOTH)) {
+      // MeshTriangle::hit (src/obj.rs:291-303): the normal and texture
+      // coordinates interpolated at the barycentrics (u, v)
+      const AttrT<R>& A = S.attrs[attr_index(pr)];
+      const R bu = s.u, bv = s.v;
+      if (A.flags & 1u) {
+        const V3<R> n0 = {A.n[0], A.n[1], A.n[2]}, n1 = {A.n[3], A.n[4], A.n[5]}, n2 = {A.n[6], A.n[7], A.n[8]};
+        n = unit(n0 * (R(1) - bu - bv) + n1 * bu + n2 * bv);
+      }
+      if (A.flags & 2u) {
+        const R w = R(1) - bu - bv;
+        s.u = A.uv[0] * w + A.uv[2] * bu + A.uv[4] * bv;
+        s.v = A.uv[1] * w + A.uv[3] * bu + A.uv[5] * bv;
+      }
+    }
   }
   bool front = dot(dd, n) < R(0);
   if (!front) n = -n;

@@ -538,3 +538,45 @@ def inverse_rendering_scene(width=512, height=512, spp=32, depth=8, seed=42, tex
         return {"texture": tex, "wall_material": wall, "emission_material": emit, "sphere_materials": sph}
 
     return toml, setup
+
+
+def smooth_mesh_scene(width=64, height=48, spp=16, depth=8, seed=42, freq=10):
+    """Smooth-shaded meshes (OBJ faces with vn / vt, MeshTriangle with
+    attributes, src/obj.rs:250-313): a geodesic sphere with its radial vertex
+    normals and spherical texture coordinates under an image texture, and a
+    displaced sphere with vertex normals only (GGX), on a ground quad lit by
+    a quad light (NEE) under the sky.  Returns (toml, add_meshes); add_meshes
+    appends both meshes and the texture and returns the triangle count."""
+    import numpy as np
+    L = _header(width, height, spp, depth, seed, None,
+                {"look_from": [0.0, 1.6, 5.5], "look_at": [0.0, 0.9, 0.0], "vfov": 40, "aperture": 0.0,
+                 "focus_dist": 1.0})
+    L += ["[[quad]]", "q = [-0.6, 3.5, -0.5]", "u = [1.2, 0.0, 0.0]", "v = [0.0, 0.0, 1.0]",
+          'material = { type = "emissive", color = [1.0, 0.95, 0.9], intensity = 10.0 }', ""]
+    L += ["[[quad]]", "q = [-6.0, 0.0, -6.0]", "u = [12.0, 0.0, 0.0]", "v = [0.0, 0.0, 12.0]",
+          'material = { type = "lambertian", color = [0.6, 0.6, 0.6] }', ""]
+    toml = "\n".join(L)
+
+    def add_meshes(world):
+        from . import _abi
+        u, f = geodesic_sphere(freq)
+        centre = np.array([-0.9, 0.9, 0.0])
+        v = centre + 0.85 * u
+        nrm = u[f]  # (T, 3, 3): the corners' radial normals
+        uv = np.stack([(np.arctan2(-u[:, 2], u[:, 0]) + np.pi) / (2 * np.pi), np.arccos(-u[:, 1]) / np.pi], axis=1)[f]
+        yy, xx = np.mgrid[0:8, 0:16]  # a 16 x 8 checker
+        tex = (((xx + yy) % 2)[..., None] * np.array([0.7, 0.3, 0.1])
+               + np.array([0.2, 0.25, 0.3])).astype(np.float32)
+        t = world.add_texture(tex)
+        m = _abi.Material()
+        m.kind, m.texture, m.intensity, m.ior = _abi.MAT_LAMBERTIAN, t, 1.0, 1.5
+        for c in range(3):
+            m.color[c] = 0.5
+        world.add_mesh(v, f, normals=nrm, uvs=uv, material=world.add_material(m))
+        v2, f2 = displaced_sphere(freq, [1.0, 0.8, 0.2], 0.75, seed)
+        d = v2 - np.array([1.0, 0.8, 0.2])
+        vn = d / np.linalg.norm(d, axis=1)[:, None]
+        world.add_mesh(v2, f2, normals=vn[f2], material=world.add_material(_microfacet([0.8, 0.7, 0.5], 0.3)))
+        return len(f) + len(f2)
+
+    return toml, add_meshes

@@ -941,6 +941,7 @@ struct Triangle : Hittable {  // triangle.rs:23-70 / obj.rs:260-319
   Vec3 v0, v1, v2;
   const Material* m;
   int64_t id;
+  const lmn_vertex_attr* attr = nullptr;  // MeshTriangle's normals / uvs (obj.rs:291-303)
   bool hit(const Ray& r, double tmin, double tmax, HitRecord& h) const override {
     Vec3 e1 = v1 - v0, e2 = v2 - v0;
     Vec3 hh = r.d.cross(e2);
@@ -961,7 +962,15 @@ struct Triangle : Hittable {  // triangle.rs:23-70 / obj.rs:260-319
     h.v = v;
     h.mat = m;
     h.prim = id;
-    set_face(h, r, e1.cross(e2).unit());
+    Vec3 outward = e1.cross(e2).unit();
+    if (attr && attr->has_normals)
+      outward = (v3(attr->normal[0]) * (1.0 - u - v) + v3(attr->normal[1]) * u + v3(attr->normal[2]) * v).unit();
+    if (attr && attr->has_uvs) {
+      double w = 1.0 - u - v;
+      h.u = attr->uv[0][0] * w + attr->uv[1][0] * u + attr->uv[2][0] * v;
+      h.v = attr->uv[0][1] * w + attr->uv[1][1] * u + attr->uv[2][1] * v;
+    }
+    set_face(h, r, outward);
     return true;
   }
   bool bbox(Aabb& b) const override {
@@ -1884,13 +1893,15 @@ HPtr make_leaf(const lmn_scene_desc& d, uint32_t i, const World& w, bool exact) 
       r->id = id;
       return r;
     }
-    case LMN_PRIM_TRIANGLE: {
+    case LMN_PRIM_TRIANGLE:
+    case LMN_PRIM_MESH_TRIANGLE: {
       auto t = std::make_unique<Triangle>();
       t->v0 = v3(q);
       t->v1 = v3(q + 3);
       t->v2 = v3(q + 6);
       t->m = m;
       t->id = id;
+      if (p.kind == LMN_PRIM_MESH_TRIANGLE) t->attr = &d.vertex_attrs[size_t(q[9])];  // checked by build_world
       return t;
     }
     case LMN_PRIM_PLANE: return std::make_unique<Plane>(v3(q), v3(q + 3), m, id);
@@ -2036,6 +2047,12 @@ HPtr wrap(HPtr obj, const lmn_transform& t) {
 }
 
 bool build_world(const lmn_scene_desc& d, bool exact, World& w) {
+  for (uint32_t i = 0; i < d.n_prims; ++i)
+    if (d.prims[i].kind == LMN_PRIM_MESH_TRIANGLE &&
+        !(d.prims[i].p[9] >= 0.0 && d.prims[i].p[9] < double(d.n_vertex_attrs) && d.vertex_attrs)) {
+      g_err = "vertex attribute index out of range";
+      return false;
+    }
   for (uint32_t i = 0; i < d.n_materials; ++i) {
     const lmn_material& m = d.materials[i];
     auto mm = std::make_unique<Material>();

@@ -77,6 +77,7 @@ int host_trace_fp32(const lmn_scene_desc* desc, const lmn_ray* rays, size_t n, u
   S.textures = H.tex.data();
   S.texels = H.texels.data();
   S.lights = H.lightF.data();
+  S.attrs = H.attrF.data();
   S.n_bounded = H.n_bounded;
   S.n_planes = H.n_planes;
   S.n_lights = uint32_t(H.lightF.size());

@@ -43,3 +43,4 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(_abi.Image) == 72
     assert C.sizeof(_abi.Stats) == 104
     assert C.sizeof(_abi.TraceCounts) == 48
+    assert C.sizeof(_abi.VertexAttr) == 128

@@ -195,3 +195,41 @@ def test_parallel_bvh_build_gives_the_same_hits(harness, monkeypatch):
     assert (one["prim"] >= 0).sum() > 2000
     assert np.array_equal(one["prim"], many["prim"]) and np.array_equal(one["t"], many["t"])
     assert np.array_equal(one["normal"], many["normal"])
+
+
+def smooth_world():
+    toml, add_meshes = workloads.smooth_mesh_scene()
+    cfg, cam, world = lm.load_scene(toml)
+    assert add_meshes(world) == 4000
+    return world
+
+
+def check_smooth_hits(rays, ho, hx, fp32):
+    # interpolated normals and texture coordinates (MeshTriangle::hit,
+    # src/obj.rs:260-313) on the hits where both sides found the same
+    # triangle; the mesh triangles are prims 2..4001
+    r = compare_hits(rays, ho, hx, fp32=fp32)
+    same = (ho["prim"] == hx["prim"]) & (ho["prim"] >= 2)
+    assert same.sum() > 2000
+    dots = np.einsum("ij,ij->i", ho["normal"][same], hx["normal"][same])
+    du = np.abs(ho["u"][same] - hx["u"][same])
+    dv = np.abs(ho["v"][same] - hx["v"][same])
+    if fp32:
+        assert (dots > 1 - 1e-5).mean() > 0.999 and dots.min() > 0.99
+        assert (np.maximum(du, dv) < 1e-4).mean() > 0.999
+    else:
+        assert dots.min() > 1 - 1e-12
+        assert np.maximum(du, dv).max() < 1e-9
+    assert (ho["front_face"][same] == hx["front_face"][same]).mean() > 0.999
+    return r
+
+
+def test_smooth_mesh_attributes(harness):
+    world = smooth_world()
+    rays = random_rays(20000, [-2.0, 0.05, -1.5], [2.0, 2.0, 1.5], seed=11)
+    ho = oracle.trace(world, rays)
+    hh, _ = host_trace(harness, world, rays)
+    r = check_smooth_hits(rays, ho, hh, fp32=True)
+    print(r)
+    assert r["bad_t"] == 0
+    assert r["flips"] <= int(5e-4 * r["n"])

@@ -273,3 +273,86 @@ def test_bad_mesh_leaves_the_description_unchanged():
     assert len(prims) == len(before) + 1
     assert prims[-1].kind == _abi.PRIM_TRIANGLE and prims[-1].object == n_obj
     np.testing.assert_array_equal(np.array(prims[-1].p[:9]).reshape(3, 3), verts)
+
+
+# ---------------------------------------------------------------- OBJ meshes (src/obj.rs:18-153)
+
+def _obj_scene(tmp_path, text, scale=None, offset=None, name="m.obj"):
+    path = tmp_path / name
+    path.write_text(text)
+    extra = (f"scale = {scale}\n" if scale is not None else "") + (f"offset = {offset}\n" if offset else "")
+    toml = (f'[[mesh]]\nfile = "{path}"\n{extra}'
+            'material = { type = "lambertian", color = [0.5, 0.5, 0.5] }\n')
+    return lm.load_scene(toml)[2]
+
+
+@pytest.mark.parametrize("text,n", [
+    ("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1 2 3 4\n", 2),                       # test_load_quad_triangulates
+    ("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n", 1),                                   # test_load_triangle
+    ("v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\nf 1/1 2/2 3/3\n", 1),    # test_load_with_uvs
+    ("v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\nvn 0 0 1\nvn 0 0 1\nvn 0 0 1\n"
+     "f 1/1/1 2/2/2 3/3/3\n", 1),                                                  # test_load_full_format
+    ("v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nvn 0 0 1\nvn 0 0 1\nf 1//1 2//2 3//3\n", 1),  # test_load_with_normals
+    ("# comment\nv 0 0 0\nv 1 0 0\nv 0 1 0\n\nf 1 2 3\n", 1),                       # test_obj_with_comments
+])
+def test_obj_reference_cases(tmp_path, text, n):  # src/obj.rs:328-393
+    w = _obj_scene(tmp_path, text)
+    assert len(w.prims) == n
+
+
+def test_obj_scale_offset_and_attributes(tmp_path):  # src/obj.rs:345-356, :114-149
+    # scale then offset on positions only; vn made unit; a face has normals
+    # (uvs) only when every corner names one
+    text = ("v 1 1 1\nv 2 1 1\nv 1 2 1\nv 2 2 1\n"
+            "vt 0.25 0.5\nvt 1 0\nvt 0 1\n"
+            "vn 0 0 2\nvn 0 3 4\nvn 1 0 0\n"
+            "f 1/1/1 2/2/2 3/3/3\n"    # normals and uvs
+            "f 2//2 4//1 3//3\n"       # normals only
+            "f 2/2 4/3 3\n")           # a corner without uv: plain triangle
+    w = _obj_scene(tmp_path, text, scale=2.0, offset="[1.0, 0.0, 0.0]")
+    prims = w.prims
+    assert [p.kind for p in prims] == [_abi.PRIM_MESH_TRIANGLE, _abi.PRIM_MESH_TRIANGLE, _abi.PRIM_TRIANGLE]
+    np.testing.assert_allclose(np.array(prims[0].p[:9]).reshape(3, 3), [[3, 2, 2], [5, 2, 2], [3, 4, 2]])
+    d = w.desc_struct
+    assert d.n_vertex_attrs == 2
+    a0, a1 = d.vertex_attrs[int(prims[0].p[9])], d.vertex_attrs[int(prims[1].p[9])]
+    assert a0.has_normals and a0.has_uvs and a1.has_normals and not a1.has_uvs
+    np.testing.assert_allclose(np.array(a0.normal), [[0, 0, 1], [0, 0.6, 0.8], [1, 0, 0]])
+    np.testing.assert_allclose(np.array(a0.uv), [[0.25, 0.5], [1, 0], [0, 1]])
+    np.testing.assert_allclose(np.array(a1.normal), [[0, 0.6, 0.8], [0, 0, 1], [1, 0, 0]])
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("v 0 0\n", "Line 1: vertex needs 3 coordinates"),
+    ("v 0 0 0\nvt 1\n", "Line 2: texture coord needs 2 values"),
+    ("vn 0 1\n", "Line 1: normal needs 3 components"),
+    ("v 0 0 0\nv 1 0 0\nf 1 2\n", "Line 3: face needs at least 3 vertices"),
+    ("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 4\n", "Vertex index 4 out of range"),
+    ("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 0 1 2\n", "Vertex index 0 out of range"),
+    ("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 -3\n", "Line 4:"),
+])
+def test_obj_errors(tmp_path, text, msg):
+    with pytest.raises(lm.LuminaraError) as e:
+        _obj_scene(tmp_path, text)
+    assert msg in str(e.value)
+
+
+def test_obj_out_of_range_attribute_indices_fall_back(tmp_path):
+    # normals / uvs that do not exist leave the face without them
+    # (obj.rs:114-149: get(n - 1) finds nothing)
+    w = _obj_scene(tmp_path, "v 0 0 0\nv 1 0 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//2 3//1\n")
+    assert [p.kind for p in w.prims] == [_abi.PRIM_TRIANGLE]
+
+
+def test_c_api_mesh_with_attributes():
+    _, _, w = lm.load_scene('[[sphere]]\ncenter = [0.0, -9.0, 0.0]\nradius = 1.0\n'
+                            'material = { type = "lambertian", color = [0.5, 0.5, 0.5] }\n')
+    verts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], dtype=np.float64)
+    tris = np.array([[0, 1, 2], [1, 3, 2]], dtype=np.uint32)
+    nrm = np.tile(np.array([0.0, 0.0, 1.0]), (2, 3, 1))
+    w.add_mesh(verts, tris, normals=nrm, material=0)
+    kinds = [p.kind for p in w.prims]
+    assert kinds[-2:] == [_abi.PRIM_MESH_TRIANGLE] * 2 and w.desc_struct.n_vertex_attrs == 2
+    with pytest.raises(lm.LuminaraError):
+        w.add_mesh(verts, np.array([[0, 1, 7]], dtype=np.uint32), normals=nrm[:1])
+    assert w.desc_struct.n_vertex_attrs == 2 and len(w.prims) == 3

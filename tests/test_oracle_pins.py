@@ -366,3 +366,39 @@ def test_smallrng_seed_from_u64_outputs():
             5439680534584881407, 1369371744833522710]
     u, _ = oracle.xoshiro(10, seed=0)
     assert [int(x) for x in u] == want
+
+
+# ------------------------------------------------------------------ smooth mesh triangles
+
+def test_mesh_triangle_interpolates_normal_and_uv():
+    # MeshTriangle::hit (src/obj.rs:260-313): at barycentrics (u, v) of the
+    # hit (p = v0 + u e1 + v e2, solved here with numpy) the outward normal is
+    # unit(n0 (1 - u - v) + n1 u + n2 v), turned to face the ray, and the
+    # texture coordinates uv0 (1 - u - v) + uv1 u + uv2 v
+    _, _, w = lm.load_scene('[[sphere]]\ncenter = [0.0, -50.0, 0.0]\nradius = 1.0\n'
+                            'material = { type = "lambertian", color = [0.5, 0.5, 0.5] }\n')
+    V = np.array([[0.0, 0.0, 0.0], [2.0, 0.1, 0.0], [0.3, 1.7, 0.4]])
+    N = np.array([[0.0, 0.0, 1.0], [0.6, 0.0, 0.8], [0.0, -0.6, 0.8]])
+    UV = np.array([[0.1, 0.9], [0.8, 0.7], [0.3, 0.05]])
+    w.add_mesh(V, np.array([[0, 1, 2]], dtype=np.uint32), normals=N[None], uvs=UV[None])
+    rng = np.random.default_rng(4)
+    bu = rng.uniform(0.02, 0.9, 400)
+    bv = rng.uniform(0.02, 0.9, 400) * (1 - bu) * 0.95
+    p = V[0] + bu[:, None] * (V[1] - V[0]) + bv[:, None] * (V[2] - V[0])
+    o = p + rng.normal(size=(400, 3)) * 3.0
+    rays = np.concatenate([o, p - o, np.full((400, 1), 0.001), np.full((400, 1), np.inf)], axis=1)
+    h = oracle.trace(w, rays)
+    assert (h["prim"] == 1).all()
+    e1, e2 = V[1] - V[0], V[2] - V[0]
+    A = np.stack([e1, e2], axis=1)
+    uv_solved = np.array([np.linalg.lstsq(A, q - V[0], rcond=None)[0] for q in p])
+    np.testing.assert_allclose(uv_solved, np.stack([bu, bv], 1), atol=1e-12)
+    wgt = 1 - bu - bv
+    n = wgt[:, None] * N[0] + bu[:, None] * N[1] + bv[:, None] * N[2]
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    front = np.einsum("ij,ij->i", p - o, n) < 0
+    n[~front] *= -1
+    np.testing.assert_allclose(h["normal"], n, atol=1e-9)
+    np.testing.assert_array_equal(h["front_face"].astype(bool), front)
+    np.testing.assert_allclose(h["u"], wgt * UV[0, 0] + bu * UV[1, 0] + bv * UV[2, 0], atol=1e-9)
+    np.testing.assert_allclose(h["v"], wgt * UV[0, 1] + bu * UV[1, 1] + bv * UV[2, 1], atol=1e-9)
