@@ -288,13 +288,14 @@ def measure(wl_name, scenes, desc, args, want_images, tag):
 
     def params(path, flags=0):
         return [lm.make_params(c, path=path, scene_key=i, pool_paths=args.pool, stack_lds=args.stack_lds,
-                               flags=flags) for i, (c, _) in enumerate(scenes)]
+                               block_size=args.block, grid_blocks=args.grid, flags=flags)
+                for i, (c, _) in enumerate(scenes)]
 
     # launch configuration (path pool, workgroup size, LDS stack depth, grid)
     # chosen by the library's autotuner on probe renders of this batch; the
     # timed renders below leave those knobs at 0 and take the tuned values
     tuning = None
-    if not args.pool and not args.stack_lds and not args.no_tune:
+    if not (args.pool or args.stack_lds or args.block or args.grid or args.no_tune):
         tuning = lm.autotune(worlds, params("wavefront"), ptrs)
         log(f"[bench:{tag}] autotune: {tuning}")
     # Timed region 1 (`value`): the product path, blocks of iterations
@@ -427,7 +428,9 @@ def main():
     ap.add_argument("--no-count", action="store_true", help="skip the instrumented traversal count")
     ap.add_argument("--cpu-seconds", type=float, default=16.0)
     ap.add_argument("--stack-lds", type=int, default=0)
-    ap.add_argument("--pool", type=int, default=0)
+    ap.add_argument("--pool", type=int, default=0, help="path-pool size (with any launch knob: no autotuning)")
+    ap.add_argument("--block", type=int, default=0, help="workgroup size of the traversal kernels (128 or 256)")
+    ap.add_argument("--grid", type=int, default=0, help="workgroups of the grid-stride kernels")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the inverse-rendering adjoint leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults or cache)")
     ap.add_argument("--no-images", action="store_true", help="skip the post chain / RGBE / AOV timings")
