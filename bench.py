@@ -215,6 +215,50 @@ def image_leg(scenes, worlds, frame, n_pix, reps):
                     "mrays_per_s": n_pix / (aov_ms / 1e3) / 1e6}}
 
 
+def ingest_leg(freq=256):
+    """Scene ingestion of the mesh-heavy scene's 1.31M-triangle displaced
+    icosphere through the TOML `[[mesh]]` path: one ASCII PLY file, loaded
+    from its text (the sidecar written on the way), loaded again from the
+    binary sidecar, and the scene built (SAH BVH, 4-wide collapse, upload)."""
+    import shutil
+    import tempfile
+    import luminara_amd as lm
+    from luminara_amd import workloads
+    v, f = workloads.displaced_sphere(freq, [0.0, 1.75, 0.0], 1.6, 42)
+    tmp = tempfile.mkdtemp(prefix="lmn_ingest_")
+    try:
+        path = os.path.join(tmp, "icosphere.ply")
+        workloads.write_ply(path, v, f)
+        toml = (f'[[mesh]]\nfile = "{path}"\n'
+                'material = { type = "pbr", color = [0.7, 0.7, 0.7], roughness = 0.3 }\n')
+        old = os.environ.get("LMN_MESH_SIDECAR")
+        os.environ["LMN_MESH_SIDECAR"] = "auto"
+        try:
+            t0 = time.perf_counter()
+            _, _, w_text = lm.load_scene(toml)
+            t_text = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            _, _, w_side = lm.load_scene(toml)
+            t_side = time.perf_counter() - t0
+        finally:
+            if old is None:
+                del os.environ["LMN_MESH_SIDECAR"]
+            else:
+                os.environ["LMN_MESH_SIDECAR"] = old
+        n_tri = len(w_side.prims)
+        t0 = time.perf_counter()
+        w_side.device_scene()
+        t_build = time.perf_counter() - t0
+        res = {"triangles": n_tri, "vertices": len(v), "ply_bytes": os.path.getsize(path),
+               "sidecar_bytes": os.path.getsize(path + ".lmnmesh"),
+               "text_load_s": t_text, "sidecar_load_s": t_side, "scene_create_s": t_build,
+               "build_threads": int(os.environ.get("LMN_BUILD_THREADS", "0")) or min(64, os.cpu_count() or 1)}
+        del w_text, w_side
+        return res
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def host_cores():
     """Cores this process may run on: its CPU affinity, capped by a cgroup v2
     CPU quota when one is set (a container's share of the host) and by
@@ -436,6 +480,8 @@ def main():
     ap.add_argument("--no-images", action="store_true", help="skip the post chain / RGBE / AOV timings")
     ap.add_argument("--no-mesh-heavy", action="store_true",
                     help="skip the second leg: the 1.31M-triangle mesh-heavy scene (1080p, 64 spp, depth 12)")
+    ap.add_argument("--no-ingest", action="store_true",
+                    help="skip timing the 1.31M-triangle PLY load (text, binary sidecar) and scene build")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the reference kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -478,6 +524,10 @@ def main():
         mname, mscenes, mdesc = build_workload("mesh_heavy")
         mesh = measure(mname, mscenes, mdesc, args, False, "mesh_heavy")
         del mscenes
+
+    if mesh is not None and not args.no_ingest:
+        mesh["ingest"] = ingest_leg()
+        log(f"[bench] ingest: {mesh['ingest']}")
 
     adj = None
     if not args.no_adjoint:

@@ -18,7 +18,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "liblmn.so")
 CLI = os.path.join(OUT_DIR, "luminara-gpu")
 
-HOST_SOURCES = ["host/toml.cpp", "host/scene_load.cpp", "host/scene_build.cpp"]
+HOST_SOURCES = ["host/toml.cpp", "host/scene_load.cpp", "host/scene_build.cpp", "host/mesh_io.cpp"]
 HIP_SOURCES = ["lmn_api.hip", "lmn_kernels.hip", "lmn_plain.hip", "lmn_adjoint.hip", "lmn_post.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -15,6 +15,7 @@
 #include <fstream>
 #include <sstream>
 
+#include "mesh_io.h"
 #include "toml.h"
 
 namespace lmn {
@@ -851,17 +852,27 @@ class Loader {
       double scale = 1.0, off[3] = {0, 0, 0};
       opt_num(s, "scale", ctx, scale);
       opt_vec3(s, "offset", ctx, off);
-      std::ifstream in(resolve(f.str), std::ios::binary);
-      if (!in.good()) fail(LMN_ERR_IO, "Failed to read mesh '" + f.str + "'");
-      std::stringstream ss;
-      ss << in.rdbuf();
-      std::string content = ss.str();
       std::vector<double> tv;
       std::vector<lmn_vertex_attr> ta;
       std::string err;
       const bool ply = f.str.size() >= 4 && f.str.compare(f.str.size() - 4, 4, ".ply") == 0;
-      int rc = ply ? load_ply_text(content, scale, off, tv, err) : load_obj_text(content, scale, off, tv, ta, err);
-      if (rc != LMN_OK) fail(rc, err);
+      if (ply) {
+        // the binary sidecar when it is current, else the text
+        MeshData mesh;
+        bool from_sidecar = false;
+        int rc = load_ply_file(resolve(f.str), f.str, mesh, from_sidecar, err);
+        if (rc != LMN_OK) fail(rc, err);
+        fprintf(stderr, "Loaded PLY: %zu vertices, %llu faces, %zu triangles%s\n", mesh.n_vertices(),
+                (unsigned long long)mesh.n_faces, mesh.n_triangles(), from_sidecar ? " (binary sidecar)" : "");
+        emit_mesh(mesh, scale, off, tv);
+      } else {
+        std::ifstream in(resolve(f.str), std::ios::binary);
+        if (!in.good()) fail(LMN_ERR_IO, "Failed to read mesh '" + f.str + "'");
+        std::stringstream ss;
+        ss << in.rdbuf();
+        int rc = load_obj_text(ss.str(), scale, off, tv, ta, err);
+        if (rc != LMN_OK) fail(rc, err);
+      }
       size_t nt = tv.size() / 9;
       out_.prims.reserve(out_.prims.size() + nt);
       for (size_t t = 0; t < nt; ++t) {
@@ -1074,115 +1085,24 @@ int load_obj_text(const std::string& content, double scale, const double offset[
   return LMN_OK;
 }
 
+void emit_mesh(const MeshData& mesh, double scale, const double offset[3], std::vector<double>& tv) {
+  // vertices scaled then offset (src/obj.rs:206), each triangle's corners
+  // copied out (src/obj.rs:226-240)
+  std::vector<double> v(mesh.verts.size());
+  for (size_t i = 0; i < mesh.n_vertices(); ++i)
+    for (int k = 0; k < 3; ++k) v[i * 3 + k] = mesh.verts[i * 3 + k] * scale + offset[k];
+  tv.reserve(tv.size() + mesh.tris.size() * 3);
+  for (uint32_t c : mesh.tris)
+    for (int k = 0; k < 3; ++k) tv.push_back(v[size_t(c) * 3 + k]);
+}
+
 int load_ply_text(const std::string& content, double scale, const double offset[3], std::vector<double>& tv,
                   std::string& err) {
   // src/obj.rs:157-246
-  std::istringstream in(content);
-  std::string line;
-  auto trim = [](const std::string& s) {
-    size_t b = s.find_first_not_of(" \t\r\n"), e = s.find_last_not_of(" \t\r\n");
-    return b == std::string::npos ? std::string() : s.substr(b, e - b + 1);
-  };
-  auto split = [](const std::string& s) {
-    std::vector<std::string> out;
-    std::istringstream ss(s);
-    std::string w;
-    while (ss >> w) out.push_back(w);
-    return out;
-  };
-  if (!std::getline(in, line) || trim(line) != "ply") {
-    err = "Not a PLY file (missing 'ply' magic)";
-    return LMN_ERR_PARSE;
-  }
-  size_t nv = 0, nf = 0;
-  bool in_header = true;
-  while (std::getline(in, line)) {
-    std::string t = trim(line);
-    if (t == "end_header") {
-      in_header = false;
-      break;
-    }
-    auto parts = split(t);
-    if (parts.size() >= 3 && parts[0] == "element") {
-      char* end = nullptr;
-      unsigned long long n = std::strtoull(parts[2].c_str(), &end, 10);
-      if (*end || parts[2][0] == '-') {
-        err = std::string("Bad ") + (parts[1] == "vertex" ? "vertex" : "face") + " count";
-        if (parts[1] == "vertex" || parts[1] == "face") return LMN_ERR_PARSE;
-        continue;
-      }
-      if (parts[1] == "vertex") nv = n;
-      if (parts[1] == "face") nf = n;
-    }
-  }
-  if (in_header) {
-    err = "PLY header not terminated (missing 'end_header')";
-    return LMN_ERR_PARSE;
-  }
-  std::vector<double> verts(nv * 3);
-  for (size_t i = 0; i < nv; ++i) {
-    if (!std::getline(in, line)) {
-      err = "Expected vertex " + std::to_string(i) + ", got EOF";
-      return LMN_ERR_PARSE;
-    }
-    auto parts = split(line);
-    if (parts.size() < 3) {
-      err = "Vertex " + std::to_string(i) + ": need at least 3 coordinates";
-      return LMN_ERR_PARSE;
-    }
-    for (int k = 0; k < 3; ++k) {
-      char* end = nullptr;
-      double x = std::strtod(parts[k].c_str(), &end);
-      if (*end) {
-        err = "Vertex " + std::to_string(i) + " coordinate: invalid float literal";
-        return LMN_ERR_PARSE;
-      }
-      verts[i * 3 + k] = x * scale + offset[k];
-    }
-  }
-  tv.reserve(tv.size() + nf * 9);
-  for (size_t i = 0; i < nf; ++i) {
-    if (!std::getline(in, line)) {
-      err = "Expected face " + std::to_string(i) + ", got EOF";
-      return LMN_ERR_PARSE;
-    }
-    auto parts = split(line);
-    if (parts.empty()) {
-      err = "Face " + std::to_string(i) + ": empty line";
-      return LMN_ERR_PARSE;
-    }
-    char* end = nullptr;
-    unsigned long n = std::strtoul(parts[0].c_str(), &end, 10);
-    if (*end || parts[0][0] == '-') {
-      err = "Face " + std::to_string(i) + " vertex count: invalid digit";
-      return LMN_ERR_PARSE;
-    }
-    if (parts.size() < n + 1) {
-      err = "Face " + std::to_string(i) + ": expected " + std::to_string(n) + " vertex indices";
-      return LMN_ERR_PARSE;
-    }
-    std::vector<size_t> idx(n);
-    for (size_t j = 0; j < n; ++j) {
-      unsigned long long x = std::strtoull(parts[j + 1].c_str(), &end, 10);
-      if (*end || parts[j + 1][0] == '-') {
-        err = "Face " + std::to_string(i) + " index " + std::to_string(j + 1) + ": invalid digit";
-        return LMN_ERR_PARSE;
-      }
-      idx[j] = size_t(x);
-    }
-    // fan triangulation (src/obj.rs:226-240); a face with fewer than three
-    // indices adds nothing
-    for (size_t j = 1; j + 1 < idx.size(); ++j) {
-      size_t i0 = idx[0], i1 = idx[j], i2 = idx[j + 1];
-      if (i0 >= nv || i1 >= nv || i2 >= nv) {
-        err = "Face " + std::to_string(i) + ": vertex index out of bounds";
-        return LMN_ERR_PARSE;
-      }
-      for (size_t c : {i0, i1, i2})
-        for (int k = 0; k < 3; ++k) tv.push_back(verts[c * 3 + k]);
-    }
-  }
-  return LMN_OK;
+  MeshData mesh;
+  int rc = parse_ply(content.c_str(), content.size(), mesh, err);
+  if (rc == LMN_OK) emit_mesh(mesh, scale, offset, tv);
+  return rc;
 }
 
 int load_scene_toml(const std::string& text, const std::string& base_dir, OwnedDesc& out, std::string& err) {

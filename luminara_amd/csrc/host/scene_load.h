@@ -32,6 +32,11 @@ int load_scene_toml(const std::string& text, const std::string& base_dir, OwnedD
 int load_ply_text(const std::string& content, double scale, const double offset[3],
                   std::vector<double>& tri_verts, std::string& err);
 
+// Appends a parsed mesh's triangles to tri_verts (9 doubles each), vertices
+// scaled then offset as load_ply does.
+struct MeshData;
+void emit_mesh(const MeshData& mesh, double scale, const double offset[3], std::vector<double>& tri_verts);
+
 // OBJ mesh (src/obj.rs:18-153): v / vt / vn / f lines, fan-triangulated
 // faces, vertices scaled then offset, normals made unit.  Appends 9 vertex
 // coordinates per triangle to tri_verts and, per triangle, its corner
