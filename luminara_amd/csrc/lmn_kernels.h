@@ -30,7 +30,8 @@ struct ShadowSoA {
 // L2 channel.
 struct WaveCounters {
   uint32_t active;       // paths in the current pool
-  uint32_t pad;
+  uint32_t n_surv;       // of which survivors of the last iteration; slots n_surv.. are new camera rays
+  uint64_t cam_item;     // work item of slot n_surv (the camera rays are items cam_item, cam_item + 1, ...)
   uint64_t next_item;    // next work item to issue (chunk relative)
   uint64_t n_items;      // work items in the chunk
   uint64_t stat_extend;  // closest-hit queries issued
@@ -112,9 +113,10 @@ struct WaveLaunch {
   MarchQ mq;           // marched pass (scenes with torus/superellipsoid/spring)
 };
 
-// One wavefront iteration: extend, shade, shadow, regenerate, bookkeeping.
-// With `ev` (7 events), ev[0..6] bracket extend, its marched pass, shade,
-// shadow, its marched pass and regenerate+bookkeeping for per-stage timing.
+// One wavefront iteration: extend, shade, shadow, bookkeeping (new work
+// items into the freed slots).  With `ev` (7 events), ev[0..6] bracket
+// extend, its marched pass, shade, shadow, its marched pass and bookkeeping
+// for per-stage timing.
 void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt,
                            hipEvent_t* ev = nullptr);
 void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s);

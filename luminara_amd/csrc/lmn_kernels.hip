@@ -1,8 +1,9 @@
 // HIP kernels of the render hot path for gfx950 (CDNA4, wave64):
 //  * wavefront FP32 path tracer: extend (closest hit, LDS short stack),
 //    shade (materials, emission, Russian roulette, NEE sampling, wave-ballot
-//    compaction of survivors and shadow rays), shadow (any hit), regenerate
-//    (camera rays for new work items into freed pool slots), bookkeeping;
+//    compaction of survivors and shadow rays), shadow (any hit),
+//    bookkeeping (new work items take the freed pool slots; their camera
+//    rays are generated where extend and shade read them);
 //  * plain one-path-per-thread megakernel in FP64 (or FP32);
 //  * resolve: per-pixel luminance clamp, filter weights and FP64 sums;
 //  * ray-batch trace for parity.
@@ -129,6 +130,68 @@ __device__ __forceinline__ void add_sample(float4* samples, uint32_t item, V3<fl
 }
 
 // ---------------------------------------------------------------------------
+// New camera rays are not stored: the slots after an iteration's survivors
+// stand for work items cam_item, cam_item + 1, ... (src/render.rs:1683-1690),
+// and the stages that read a path recompute its camera ray from the item
+// (generate: the item's part by binary search, its pixel, jitter, lens and
+// time draws).  That saves writing 80 bytes per path in a kernel of its own
+// and reading them back in extend and shade.
+struct CameraPath {
+  V3<float> o, d;
+  float w;           // filter weight of the sample
+  Rng rng;           // after the camera draws
+  uint32_t scene, max_depth;
+};
+__device__ __forceinline__ void camera_path(const BatchDev& B, uint64_t item, CameraPath& c) {
+  // part holding the item: largest p with part_begin[p] <= item.  A wave's
+  // camera slots hold consecutive items, so the binary search runs once, on
+  // the first active lane's item (the smallest), with wave-uniform values;
+  // each lane then steps forward past the part boundaries below its own item
+  // (none, or a few where the wave straddles parts).
+  const uint64_t it0 = (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(item >> 32))))) << 32) |
+                       uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(item))));
+  uint32_t lo = 0, hi = B.n_parts;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (B.part_begin[mid] <= it0)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  while (lo + 1 < B.n_parts && B.part_begin[lo + 1] <= item) ++lo;
+  for_scene(lo, [&](uint32_t part) {
+    const FrameDev F = B.parts[part];
+    uint32_t pixel;
+    generate_item<float>(F, B.views[F.scene].cam, F.item_base + (item - B.part_begin[part]), c.rng, c.o, c.d, c.w,
+                         pixel);
+    c.scene = F.scene;
+    c.max_depth = F.max_depth;
+  });
+}
+
+// The ray of pool slot i: a stored path (i < n_surv) or a new camera ray.
+// org: the record the ray leaves from (kNoOrigin for camera rays).
+__device__ __forceinline__ void pool_ray(const BatchDev& B, const PathSoA& cur, const WaveCounters* ctr, uint32_t i,
+                                         V3<float>& o, V3<float>& d, uint32_t& sid, uint32_t& org) {
+  const uint32_t ns = ctr->n_surv;
+  if (i < ns) {
+    const float4 a = cur.ob[i], b = cur.db[i];
+    const uint4 x = cur.aux[i];
+    o = {a.x, a.y, a.z};
+    d = {b.x, b.y, b.z};
+    sid = x.z;
+    org = x.w;
+  } else {
+    CameraPath c;
+    camera_path(B, ctr->cam_item + (i - ns), c);
+    o = c.o;
+    d = c.d;
+    sid = c.scene;
+    org = kNoOrigin;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Wavefront stages
 
 // Extend, main pass: closest hit over the main BVH and the planes.
@@ -140,18 +203,17 @@ __device__ __forceinline__ void extend_body(const BatchDev& B, const PathSoA& cu
   uint32_t nthreads = gridDim.x * BS;
   uint32_t gid = blockIdx.x * BS + threadIdx.x;
   for (uint32_t i = gid; i < n; i += nthreads) {
-    float4 a = cur.ob[i], b = cur.db[i];
-    const uint4 x = cur.aux[i];
-    const uint32_t sid = x.z, org = x.w;  // org: the record the ray leaves from
+    V3<float> o, d;
+    uint32_t sid, org;  // org: the record the ray leaves from
+    pool_ray(B, cur, ctr, i, o, d, sid, org);
     for_scene(sid, [&](uint32_t s) {
       const SceneView<float> S = B.views[s];
       LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
       float t = INFINITY;
       uint32_t idx = 0;
-      bool h = trace_pass<float, FEAT, false, PASS_MAIN>(S, f3(a), f3(b), 0.001f, t, idx, st, COUNT ? tc : nullptr,
-                                                         org);
+      bool h = trace_pass<float, FEAT, false, PASS_MAIN>(S, o, d, 0.001f, t, idx, st, COUNT ? tc : nullptr, org);
       if (h)
-        store_hit<FEAT>(S, hit, i, f3(a), f3(b), t, idx);
+        store_hit<FEAT>(S, hit, i, o, d, t, idx);
       else
         store_miss(hit, i);
     });
@@ -285,8 +347,9 @@ __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_shadow(BatchDev 
 // direction, scene, range end; `live` false for shadow rays the main pass
 // found occluded.
 template <bool SHADOW>
-__device__ __forceinline__ bool march_ray(const PathSoA& cur, const HitSoA& hit, const ShadowSoA& sh, uint32_t i,
-                                          V3<float>& o, V3<float>& d, uint32_t& sid, float& tmax) {
+__device__ __forceinline__ bool march_ray(const BatchDev& B, const PathSoA& cur, const HitSoA& hit,
+                                          const ShadowSoA& sh, const WaveCounters* ctr, uint32_t i, V3<float>& o,
+                                          V3<float>& d, uint32_t& sid, float& tmax) {
   if (SHADOW) {
     float4 o4 = sh.o[i], d4 = sh.d[i];
     o = f3(o4);
@@ -295,10 +358,9 @@ __device__ __forceinline__ bool march_ray(const PathSoA& cur, const HitSoA& hit,
     tmax = o4.w - 0.001f;
     return o4.w > 0.f;
   }
-  float4 o4 = cur.ob[i], d4 = cur.db[i], h = hit.b[i];
-  o = f3(o4);
-  d = f3(d4);
-  sid = cur.aux[i].z;
+  uint32_t org;
+  pool_ray(B, cur, ctr, i, o, d, sid, org);
+  const float4 h = hit.b[i];
   tmax = __float_as_uint(h.z) == HIT_MISS ? INFINITY : h.w;
   return true;
 }
@@ -321,7 +383,7 @@ __global__ __launch_bounds__(BS) void k_march_collect(BatchDev B, PathSoA cur, H
     V3<float> o, d;
     uint32_t sid;
     float tmax;
-    if (i < n && march_ray<SHADOW>(cur, hit, sh, i, o, d, sid, tmax)) {
+    if (i < n && march_ray<SHADOW>(B, cur, hit, sh, ctr, i, o, d, sid, tmax)) {
       q.key[i] = SHADOW ? 0ull : MARCH_EMPTY;
       for_scene(sid, [&](uint32_t s) {
         const SceneView<float> S = B.views[s];
@@ -400,7 +462,7 @@ __global__ __launch_bounds__(BS) void k_march_screen(BatchDev B, PathSoA cur, Hi
       V3<float> o, d;
       uint32_t sid;
       float tmax;
-      march_ray<SHADOW>(cur, hit, sh, e.x, o, d, sid, tmax);
+      march_ray<SHADOW>(B, cur, hit, sh, ctr, e.x, o, d, sid, tmax);
       for_scene(sid, [&](uint32_t s) {
         const SceneView<float> S = B.views[s];
         const MarchedRay r = marched_ray<FEAT>(S, e.y, o, d);
@@ -434,7 +496,7 @@ __global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_test(BatchDev 
     V3<float> o, d;
     uint32_t sid;
     float tmax;
-    march_ray<SHADOW>(cur, hit, sh, e.x, o, d, sid, tmax);
+    march_ray<SHADOW>(B, cur, hit, sh, ctr, e.x, o, d, sid, tmax);
     for_scene(sid, [&](uint32_t s) {
       const SceneView<float> S = B.views[s];
       float t = tmax;
@@ -468,7 +530,7 @@ __global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_bracket(BatchD
     V3<float> o, d;
     uint32_t sid;
     float tmax;
-    march_ray<false>(cur, hit, sh, e.x, o, d, sid, tmax);
+    march_ray<false>(B, cur, hit, sh, ctr, e.x, o, d, sid, tmax);
     SeBracket br{};
     bool queue = false;
     for_scene(sid, [&](uint32_t s) {
@@ -511,7 +573,7 @@ __global__ __launch_bounds__(BS) void k_march_refine(BatchDev B, PathSoA cur, Hi
     V3<float> o, d;
     uint32_t sid;
     float tmax;
-    march_ray<false>(cur, hit, sh, m.ray, o, d, sid, tmax);
+    march_ray<false>(B, cur, hit, sh, ctr, m.ray, o, d, sid, tmax);
     for_scene(sid, [&](uint32_t s) {
       const SceneView<float> S = B.views[s];
       const MarchedRay r = marched_ray<FEAT>(S, m.rec, o, d);
@@ -538,7 +600,7 @@ __global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_fallback(Batch
     V3<float> o, d;
     uint32_t sid;
     float tmax;
-    march_ray<SHADOW>(cur, hit, sh, i, o, d, sid, tmax);
+    march_ray<SHADOW>(B, cur, hit, sh, ctr, i, o, d, sid, tmax);
     for_scene(sid, [&](uint32_t s) {
       const SceneView<float> S = B.views[s];
       LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
@@ -570,11 +632,12 @@ __global__ __launch_bounds__(BS) void k_march_finish(BatchDev B, PathSoA cur, Hi
     }
     const unsigned long long key = q.key[i];
     if (key == MARCH_EMPTY) continue;
-    const float4 o4 = cur.ob[i], d4 = cur.db[i];
-    const uint32_t sid = cur.aux[i].z;
+    V3<float> o, d;
+    uint32_t sid, org;
+    pool_ray(B, cur, ctr, i, o, d, sid, org);
     for_scene(sid, [&](uint32_t s) {
       const SceneView<float> S = B.views[s];
-      store_hit<FEAT>(S, hit, i, f3(o4), f3(d4), __uint_as_float(uint32_t(key >> 32)), uint32_t(key));
+      store_hit<FEAT>(S, hit, i, o, d, __uint_as_float(uint32_t(key >> 32)), uint32_t(key));
     });
   }
 }
@@ -592,7 +655,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void
                                               ShadowSoA sh, float4* __restrict__ samples,
                                               WaveCounters* __restrict__ ctr) {
   __shared__ uint32_t res_lds[2][2 * (BS / 64) + 2];
-  uint32_t n = ctr->active;
+  const uint32_t n = ctr->active, ns = ctr->n_surv;
+  const uint64_t cam0 = ctr->cam_item;
   uint32_t nthreads = gridDim.x * BS;
   uint32_t round = 0;
   // block-uniform trip count: every lane reaches the reservation barriers
@@ -600,14 +664,37 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     uint32_t i = base + threadIdx.x;
     bool valid = i < n;
     uint32_t j = valid ? i : base;  // invalid lanes read a valid slot and do nothing with it
-    float4 A = cur.ob[j], Bd = cur.db[j];
-    uint4 M = cur.meta[j], X = cur.aux[j];
-    V3<float> o = f3(A), d = f3(Bd), beta = {A.w, Bd.w, __uint_as_float(X.y)};
+    // the path: stored (a survivor) or a new camera ray, recomputed
+    const bool camera = valid && j >= ns;
+    V3<float> o, d, beta = {1.f, 1.f, 1.f};
     Rng rng;
-    rng.key = (uint64_t(M.w) << 32) | M.z;
-    rng.n = X.x;
-    uint32_t item = M.x, depth = M.y & 0xffffu, sid = X.z;
-    bool skip = (M.y >> 16) & 1u;
+    uint32_t item, depth, sid;
+    bool skip = false;
+    float cam_w = 0.f;
+    if (!camera) {
+      const float4 A = cur.ob[j], Bd = cur.db[j];
+      const uint4 M = cur.meta[j], X = cur.aux[j];
+      o = f3(A);
+      d = f3(Bd);
+      beta = {A.w, Bd.w, __uint_as_float(X.y)};
+      rng.key = (uint64_t(M.w) << 32) | M.z;
+      rng.n = X.x;
+      item = M.x;
+      depth = M.y & 0xffffu;
+      sid = X.z;
+      skip = (M.y >> 16) & 1u;
+    } else {
+      CameraPath c;
+      const uint64_t it = cam0 + (j - ns);
+      camera_path(B, it, c);
+      o = c.o;
+      d = c.d;
+      rng = c.rng;
+      item = uint32_t(it);
+      depth = c.max_depth;
+      sid = c.scene;
+      cam_w = c.w;
+    }
     float4 HA = hit.a[j], HB = hit.b[j];
     uint32_t mword = __float_as_uint(HB.z);
     const uint32_t rec = mword == HIT_MISS ? kNoOrigin : hit.rec[j];
@@ -661,7 +748,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void
       nbeta = hadamard(beta, attw);
       cont = depth > 1;
     });
-    if (add.x != 0.f || add.y != 0.f || add.z != 0.f) add_sample(samples, item, add);
+    // a camera path's first vertex starts its sample (radiance, filter
+    // weight); later vertices add to it
+    if (camera)
+      samples[item] = make_float4(add.x, add.y, add.z, cam_w);
+    else if (add.x != 0.f || add.y != 0.f || add.z != 0.f)
+      add_sample(samples, item, add);
     uint32_t ps, pn;
     block_reserve2<BS>(&ctr->n_shadow, &ctr->next_active, emit, cont, ps, pn, res_lds[round & 1u]);
     if (emit) {
@@ -672,51 +764,16 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     if (cont) {
       nxt.ob[pn] = make_float4(sp_o.x, sp_o.y, sp_o.z, nbeta.x);
       nxt.db[pn] = make_float4(ndir.x, ndir.y, ndir.z, nbeta.y);
-      nxt.meta[pn] = make_uint4(item, (depth - 1) | (use_nee ? 0x10000u : 0u), M.z, M.w);
+      nxt.meta[pn] = make_uint4(item, (depth - 1) | (use_nee ? 0x10000u : 0u), uint32_t(rng.key),
+                                uint32_t(rng.key >> 32));
       nxt.aux[pn] = make_uint4(rng.n, __float_as_uint(nbeta.z), sid, rec);
     }
   }
 }
 
-// Regenerate: new work items (camera rays, src/render.rs:1683-1690) into the
-// free slots after the compacted survivors.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_regen(BatchDev B, PathSoA nxt, float4* __restrict__ samples,
-                                              const WaveCounters* __restrict__ ctr, uint32_t pool) {
-  uint32_t na = ctr->next_active;
-  uint64_t left = ctr->n_items - ctr->next_item;
-  uint32_t room = pool - na;
-  uint32_t m = left < room ? uint32_t(left) : room;
-  uint32_t nthreads = gridDim.x * BS;
-  for (uint32_t t = blockIdx.x * BS + threadIdx.x; t < m; t += nthreads) {
-    uint64_t item = ctr->next_item + t;
-    // part holding the item: largest p with part_begin[p] <= item
-    uint32_t lo = 0, hi = B.n_parts;
-    while (hi - lo > 1) {
-      uint32_t mid = (lo + hi) >> 1;
-      if (B.part_begin[mid] <= item)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    for_scene(lo, [&](uint32_t part) {
-      const FrameDev F = B.parts[part];
-      const CameraT<float> cam = B.views[F.scene].cam;
-      Rng rng;
-      V3<float> o, d;
-      float w;
-      uint32_t pixel;
-      generate_item<float>(F, cam, F.item_base + (item - B.part_begin[part]), rng, o, d, w, pixel);
-      samples[item] = make_float4(0.f, 0.f, 0.f, w);
-      uint32_t slot = na + t;
-      nxt.ob[slot] = make_float4(o.x, o.y, o.z, 1.f);
-      nxt.db[slot] = make_float4(d.x, d.y, d.z, 1.f);
-      nxt.meta[slot] = make_uint4(uint32_t(item), F.max_depth, uint32_t(rng.key), uint32_t(rng.key >> 32));
-      nxt.aux[slot] = make_uint4(rng.n, __float_as_uint(1.f), F.scene, kNoOrigin);
-    });
-  }
-}
-
+// Bookkeeping between iterations: the survivors stay in their compacted
+// slots and new work items (camera rays) take the free slots after them, as
+// many as fit in the pool; the stages recompute those rays from their items.
 __global__ void k_book(WaveCounters* ctr, uint32_t pool) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   uint32_t na = ctr->next_active;
@@ -727,6 +784,8 @@ __global__ void k_book(WaveCounters* ctr, uint32_t pool) {
   ctr->stat_shadow += ctr->n_shadow;
   ctr->stat_camera += m;
   ctr->active = na + m;
+  ctr->n_surv = na;
+  ctr->cam_item = ctr->next_item;
   ctr->next_item += m;
   ctr->next_active = 0;
   ctr->n_shadow = 0;
@@ -741,7 +800,8 @@ __global__ void k_counters_init(WaveCounters* ctr, uint64_t n_items) {
   ctr->active = 0;
   ctr->next_active = 0;
   ctr->n_shadow = 0;
-  ctr->pad = 0;
+  ctr->n_surv = 0;
+  ctr->cam_item = 0;
   ctr->next_item = 0;
   ctr->n_items = n_items;
   ctr->stat_extend = 0;
@@ -960,7 +1020,7 @@ static void launch_shade(const WaveLaunch& L, const BatchDev& B, const PathSoA& 
 
 void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt,
                            hipEvent_t* ev) {
-  // events: 0 extend 1 extend (marched) 2 shade 3 shadow 4 shadow (marched) 5 regen 6
+  // events: 0 extend 1 extend (marched) 2 shade 3 shadow 4 shadow (marched) 5 bookkeeping 6
   if (ev) (void)hipEventRecord(ev[0], L.stream);
   launch_trace_stage(L, B, cur, false, ev ? ev[1] : nullptr);
   if (ev) (void)hipEventRecord(ev[2], L.stream);
@@ -968,10 +1028,6 @@ void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA
   if (ev) (void)hipEventRecord(ev[3], L.stream);
   launch_trace_stage(L, B, cur, true, ev ? ev[4] : nullptr);
   if (ev) (void)hipEventRecord(ev[5], L.stream);
-  if (L.block == 128)
-    hipLaunchKernelGGL((k_regen<128>), dim3(L.grid), dim3(128), 0, L.stream, B, nxt, L.samples, L.ctr, L.pool);
-  else
-    hipLaunchKernelGGL((k_regen<256>), dim3(L.grid), dim3(256), 0, L.stream, B, nxt, L.samples, L.ctr, L.pool);
   hipLaunchKernelGGL(k_book, dim3(1), dim3(64), 0, L.stream, L.ctr, L.pool);
   if (ev) (void)hipEventRecord(ev[6], L.stream);
 }
