@@ -451,7 +451,7 @@ int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_rend
  *     lights registered from it).
  * Supported for max_depth <= LMN_ADJOINT_MAX_DEPTH.  Gradient accumulation
  * uses floating-point atomics: results agree run to run to rounding. */
-#define LMN_ADJOINT_MAX_DEPTH 16
+#define LMN_ADJOINT_MAX_DEPTH 50
 typedef struct lmn_adjoint_spec {
   int32_t texture;           /* texture whose texels are parameters, -1 = none        */
   int32_t emission_material; /* emissive material whose intensity is one, -1 = none   */

@@ -141,7 +141,7 @@ class Tuning(C.Structure):
                 ("probe_paths", C.c_uint64), ("candidates", C.c_uint32), ("pad", C.c_uint32)]
 
 
-ADJOINT_MAX_DEPTH = 16
+ADJOINT_MAX_DEPTH = 50
 
 
 class AdjointSpec(C.Structure):

@@ -26,7 +26,12 @@
 
 namespace lmn {
 
-constexpr uint32_t ADJ_MAXV = 16;    // LMN_ADJOINT_MAX_DEPTH
+// Vertex capacity of the replay: a path of max_depth bounces records up to
+// max_depth vertices in private memory.  Two builds: 16 (the common depths,
+// about 1.5 KB of scratch per lane in FP32) and LMN_ADJOINT_MAX_DEPTH = 50,
+// the reference's default depth, for deeper renders.
+constexpr uint32_t ADJ_MAXV_SMALL = 16;
+constexpr uint32_t ADJ_MAXV = LMN_ADJOINT_MAX_DEPTH;
 constexpr uint32_t ADJ_MAX_SLOTS = 256;
 
 enum : uint32_t { AV_CONT = 1u, AV_RRU = 2u, AV_EMP = 4u, AV_LIGHTP = 8u, AV_ARG_SHIFT = 4u };
@@ -74,7 +79,7 @@ __global__ void k_adj_wsum(CameraT<R> C, FrameDev F, double* __restrict__ wsum) 
 
 // Capped at 4 waves per SIMD (128 VGPRs): the replay is latency-bound and
 // the extra wave outweighs the spills (24.1 -> 21.6 ms per 512^2 x 36 spp).
-template <class R, uint32_t FEAT>
+template <class R, uint32_t FEAT, uint32_t MAXV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_adjoint(SceneView<R> S, FrameDev F, AdjDev A,
                                                  const float* __restrict__ dldi, const double* __restrict__ wsum,
                                                  R* __restrict__ g_tex, double* __restrict__ part, uint64_t k0,
@@ -90,7 +95,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     R w;
     uint32_t pix;
     generate_item<R>(F, S.cam, k, rng, o, d, w, pix);
-    AdjVertex<R> V[ADJ_MAXV];
+    AdjVertex<R> V[MAXV];
     uint32_t nv = 0;
     V3<R> L = {R(0), R(0), R(0)}, beta = {R(1), R(1), R(1)}, tail = {R(0), R(0), R(0)};
     bool skip = false;
@@ -296,8 +301,12 @@ void launch_adjoint_fp32(const SceneView<float>& S, uint32_t feat, const FrameDe
                          const float* dldi, const double* wsum, float* g_tex, double* part, uint32_t n_blocks,
                          uint64_t k0, uint64_t k1, hipStream_t s) {
 #define CALL(f)                                                                                              \
-  hipLaunchKernelGGL((k_adjoint<float, f>), dim3(n_blocks), dim3(256), 0, s, S, F, A, dldi, wsum, g_tex, part, \
-                     k0, k1)
+  if (F.max_depth <= ADJ_MAXV_SMALL)                                                                          \
+    hipLaunchKernelGGL((k_adjoint<float, f, ADJ_MAXV_SMALL>), dim3(n_blocks), dim3(256), 0, s, S, F, A, dldi, wsum, \
+                       g_tex, part, k0, k1);                                                                    \
+  else                                                                                                       \
+    hipLaunchKernelGGL((k_adjoint<float, f, ADJ_MAXV>), dim3(n_blocks), dim3(256), 0, s, S, F, A, dldi, wsum, g_tex, \
+                       part, k0, k1)
   LMN_ADJ_FEAT_SWITCH(feat, CALL)
 #undef CALL
 }
@@ -306,8 +315,12 @@ void launch_adjoint_fp64(const SceneView<double>& S, uint32_t feat, const FrameD
                          const float* dldi, const double* wsum, double* g_tex, double* part, uint32_t n_blocks,
                          uint64_t k0, uint64_t k1, hipStream_t s) {
 #define CALL(f)                                                                                               \
-  hipLaunchKernelGGL((k_adjoint<double, f>), dim3(n_blocks), dim3(256), 0, s, S, F, A, dldi, wsum, g_tex, part, \
-                     k0, k1)
+  if (F.max_depth <= ADJ_MAXV_SMALL)                                                                          \
+    hipLaunchKernelGGL((k_adjoint<double, f, ADJ_MAXV_SMALL>), dim3(n_blocks), dim3(256), 0, s, S, F, A, dldi, wsum, \
+                       g_tex, part, k0, k1);                                                                    \
+  else                                                                                                       \
+    hipLaunchKernelGGL((k_adjoint<double, f, ADJ_MAXV>), dim3(n_blocks), dim3(256), 0, s, S, F, A, dldi, wsum, g_tex, \
+                       part, k0, k1)
   LMN_ADJ_FEAT64_SWITCH(feat, CALL)
 #undef CALL
 }

@@ -1332,7 +1332,7 @@ int adjoint_common(const lmn_scene* scene, const lmn_render_params* p, const lmn
   if (!fp64 && p->path != LMN_PATH_WAVEFRONT_FP32 && p->path != LMN_PATH_PLAIN_FP32)
     return set_err(LMN_ERR_INVALID, "lmn_adjoint: unknown path");
   if (p->max_depth > LMN_ADJOINT_MAX_DEPTH)
-    return set_err(LMN_ERR_UNSUPPORTED, "lmn_adjoint: max_depth above LMN_ADJOINT_MAX_DEPTH (16)");
+    return set_err(LMN_ERR_UNSUPPORTED, "lmn_adjoint: max_depth above LMN_ADJOINT_MAX_DEPTH (50)");
   uint32_t n_mat = uint32_t(H.matF.size());
   AdjDev A{};
   A.texture = spec->texture;

@@ -250,10 +250,12 @@ __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_extend_count(Bat
 // Visibility rays end at their first hit and carry no surface work, so the
 // refill pays (mesh-heavy shadow stage 25.6 -> 19.8 ms); closest-hit rays
 // end with a surface record, and running that per refill round cost more
-// than the refill saved (extend kept grid-stride).  The rays of wave w are
+// than the refill saved (extend kept grid-stride; a persistent extend
+// measured 167 -> 253 ms per batch step).  The rays of wave w are
 // blocks w, w + W, w + 2W, ... of 64 (W waves in the grid), as a
 // grid-stride kernel deals them out; walk steps run in rounds, and a round
-// ends once a quarter of the lanes that began it have finished.  Every
+// ends once half of the lanes that began it have finished (a quarter:
+// batch shadow stage 53.7 ms, half: 53.0, an eighth: 57.3).  Every
 // ray's walk is the grid-stride walk, step for step.  Exit: a lane whose
 // next ray lies past the stage's count is done; the wave leaves when all its
 // lanes are.
@@ -310,7 +312,7 @@ __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_shadow(BatchDev 
           fin = S.n_nodes4 == 0 ||
                 bvh4_step<MAIN, true>(S, o, d, inv, 0.001f, closest, best, found, ref, st, nf, pt, tt, false);
         const uint32_t walking = uint32_t(__popcll(__ballot(!fin)));
-        if (walking == 0u || 4u * walking <= 3u * began) break;
+        if (walking == 0u || 2u * walking <= began) break;
       }
       if (!fin) return;
       if (!found)  // the unbounded planes after the BVH (src/scene.rs:817-834)

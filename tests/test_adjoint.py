@@ -105,12 +105,14 @@ def _seeds_agree(fd, adj, rel):
 
 
 def test_roughness_gradient_in_expectation():
-    # one GGX bounce under a gradient sky (no roulette at remaining depth 50)
+    # one GGX bounce under a gradient sky (no roulette at remaining depth 50,
+    # a smooth background): the central difference with common random
+    # numbers is resolved to 0.3 %, and the adjoint agrees within 1 %
     toml = """
 [render]
 width = 16
 height = 16
-samples = 8192
+samples = 16384
 max_depth = 50
 background = { type = "gradient", bottom = [1.0, 0.2, 0.1], top = [0.1, 0.3, 1.0] }
 [camera]
@@ -126,7 +128,7 @@ material = { type = "pbr", color = [0.8, 0.6, 0.4], roughness = 0.35, metallic =
     cfg, _, w = lm.load_scene(toml)
     G = np.random.default_rng(3).normal(size=(16, 16, 3)) + 1.0
     fds, adjs = [], []
-    for seed in range(1, 7):
+    for seed in range(1, 17):
         cfg.seed = seed
         _, gr, _ = oracle.adjoint(w, cfg, G, threads=8)
         vals = []
@@ -140,7 +142,9 @@ material = { type = "pbr", color = [0.8, 0.6, 0.4], roughness = 0.35, metallic =
         w.set_material(0, m)
         fds.append((vals[0] - vals[1]) / 2e-3)
         adjs.append(gr[0])
-    _seeds_agree(fds, adjs, rel=0.03)
+    fd = np.asarray(fds)
+    assert fd.std(ddof=1) / np.sqrt(len(fd)) <= 3e-3 * abs(fd.mean())
+    _seeds_agree(fds, adjs, rel=0.01)
 
 
 def test_texel_gradient_with_roulette_in_expectation():
