@@ -3,6 +3,7 @@ oracle: for each scene, seeded rays and the closest hit (t, descriptor prim
 id, u, v, normal) in both intersect modes -- the reference's marchers and the
 exact roots.  Each fixture carries its scene's TOML text, so it is self-
 contained.  Run from the repository root: python tests/golden/make_golden.py
+[name ...] (all fixtures, or the named ones).
 """
 import os
 import sys
@@ -75,10 +76,20 @@ def scenes():
     # unit directions: the reference's marchers step t by the SDF value
     yield "marched", MARCHED, rays_toward(rng, 1500, [-3, 0.2, 3], [3, 1.5, 5], [-1.8, 0.0, -1.0], [1.8, 1.0, 1.5],
                                           unit=True)
+    # the same scene with every ray component (origin, direction, t_min)
+    # exactly representable in FP32: the FP32 kernels then trace the very
+    # rays the oracle traced, which the reference's marchers need (they are
+    # chaotic in their input: a ray rounded to FP32 takes other steps)
+    rays = rays_toward(np.random.default_rng(2025), 1500, [-3, 0.2, 3], [3, 1.5, 5], [-1.8, 0.0, -1.0],
+                       [1.8, 1.0, 1.5], unit=True)
+    rays[:, :7] = rays[:, :7].astype(np.float32).astype(np.float64)
+    yield "marched_f32", MARCHED, rays
 
 
-def main():
+def main(only=None):
     for name, toml, rays in scenes():
+        if only and name not in only:
+            continue
         _, _, world = lm.load_scene(toml)
         out = {"toml": np.array(toml), "rays": rays}
         for mode, exact in (("reference", False), ("exact", True)):
@@ -93,4 +104,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])  # optional fixture names: write only those
