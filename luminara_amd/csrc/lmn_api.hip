@@ -362,7 +362,7 @@ int frame_setup(const lmn_render_params* p, FrameSetup& fs) {
   F.mode = FRAME_STRATIFIED;
   F.pix_list = nullptr;
   F.n_list = 0;
-  F.pad = 0;
+  F.tile_w = 0;
   fs.npix = cw * ch;
   F.n_items = uint64_t(fs.npix) * F.spp_a;
   F.item_base = 0;
@@ -520,6 +520,12 @@ struct WaveTotals {
 
 // Appends the parts of sample passes [s0, s1) of frame F over n_list listed
 // pixels (list null: the whole crop), each at most kMaxChunkItems items.
+// The whole-frame wavefront issues a frame's pixels in 8x8 tiles when the
+// crop's sides are multiples of 8 (see tile_pixel), else in row order.
+static uint32_t frame_tile_w(const FrameDev& F) {
+  return F.crop_w % 8u == 0u && F.crop_h % 8u == 0u ? F.crop_w : 0u;
+}
+
 void add_parts(std::vector<FrameDev>& parts, FrameDev F, const uint32_t* list, uint32_t n_pix, uint32_t s0,
                uint32_t s1) {
   F.pix_list = list;
@@ -789,6 +795,7 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
       if (fs[i].black) continue;
       FrameDev F = fs[i].F;
       F.scene = uint32_t(i);
+      F.tile_w = frame_tile_w(F);
       add_parts(parts, F, nullptr, fs[i].npix, 0, F.spp_a);
     }
     HIP_TRY(W.acc.ensure(std::max<size_t>(acc_off[n], 1) * 32));
@@ -796,7 +803,7 @@ int render_wavefront(lmn_scene* const* scenes, const lmn_render_params* const* p
       if (!fs[i].black) launch_zero_acc(W.acc.as<double4>() + acc_off[i], fs[i].npix, st);
     int rc = wavefront_run(scenes, params, fs, n, parts, st, tot, d_tc, [&](const FrameDev& P, const float4* smp) {
       launch_resolve(smp, W.acc.as<double4>() + acc_off[P.scene], fs[P.scene].npix,
-                     uint32_t(P.n_items / fs[P.scene].npix), st);
+                     uint32_t(P.n_items / fs[P.scene].npix), P.tile_w, st);
     });
     if (rc != LMN_OK) return rc;
     for (size_t i = 0; i < n; ++i)

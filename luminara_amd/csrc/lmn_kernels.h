@@ -120,7 +120,10 @@ struct WaveLaunch {
 void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt,
                            hipEvent_t* ev = nullptr);
 void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s);
-void launch_resolve(const float4* samples, double4* acc, uint32_t npix, uint32_t n_samples, hipStream_t s);
+// tile_w: the frame's entries run through 8x8 tiles of a crop this wide (0:
+// row order), as the work items were issued
+void launch_resolve(const float4* samples, double4* acc, uint32_t npix, uint32_t n_samples, uint32_t tile_w,
+                    hipStream_t s);
 // Same over listed pixels: samples[s * n_list + e] belongs to crop pixel list[e].
 void launch_resolve_list(const float4* samples, double4* acc, const uint32_t* list, uint32_t n_list,
                          uint32_t n_samples, hipStream_t s);

@@ -819,10 +819,10 @@ __global__ void k_counters_init(WaveCounters* ctr, uint64_t n_items) {
 // Resolve: per-sample luminance clamp (src/render.rs:1691-1697) and filter
 // weighting, accumulated per pixel in FP64 in sample order.
 __global__ void k_resolve(const float4* __restrict__ samples, double4* __restrict__ acc, uint32_t npix,
-                          uint32_t n_samples, const uint32_t* __restrict__ list) {
+                          uint32_t n_samples, const uint32_t* __restrict__ list, uint32_t tile_w) {
   uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= npix) return;
-  const uint32_t p = list ? list[e] : e;
+  const uint32_t p = list ? list[e] : tile_w ? tile_pixel(e, tile_w) : e;
   double4 a = acc[p];
   for (uint32_t s = 0; s < n_samples; ++s) {
     float4 v = samples[size_t(s) * npix + e];
@@ -1037,14 +1037,16 @@ void launch_wave_iteration(const WaveLaunch& L, const BatchDev& B, const PathSoA
 void launch_counters_init(WaveCounters* ctr, uint64_t n_items, hipStream_t s) {
   hipLaunchKernelGGL(k_counters_init, dim3(1), dim3(64), 0, s, ctr, n_items);
 }
-void launch_resolve(const float4* samples, double4* acc, uint32_t npix, uint32_t n_samples, hipStream_t s) {
+void launch_resolve(const float4* samples, double4* acc, uint32_t npix, uint32_t n_samples, uint32_t tile_w,
+                    hipStream_t s) {
   hipLaunchKernelGGL(k_resolve, dim3((npix + 255) / 256), dim3(256), 0, s, samples, acc, npix, n_samples,
-                     (const uint32_t*)nullptr);
+                     (const uint32_t*)nullptr, tile_w);
 }
 void launch_resolve_list(const float4* samples, double4* acc, const uint32_t* list, uint32_t n_list,
                          uint32_t n_samples, hipStream_t s) {
   if (n_list)
-    hipLaunchKernelGGL(k_resolve, dim3((n_list + 255) / 256), dim3(256), 0, s, samples, acc, n_list, n_samples, list);
+    hipLaunchKernelGGL(k_resolve, dim3((n_list + 255) / 256), dim3(256), 0, s, samples, acc, n_list, n_samples, list,
+                       0u);
 }
 void launch_resolve_adaptive(const float4* samples, AdaptState* state, const uint32_t* list, uint32_t n_list,
                              uint32_t n_samples, uint32_t min_samples, double threshold, hipStream_t s) {

@@ -1005,8 +1005,18 @@ struct FrameDev {
   uint32_t mode;       // FRAME_STRATIFIED, or FRAME_CENTER: one sample through the pixel centre
   const uint32_t* pix_list;  // crop pixel of each list entry, or null (every crop pixel in order)
   uint32_t n_list;
-  uint32_t pad;
+  uint32_t tile_w;  // without a list: 0 = crop pixels in row order; else the crop width, pixels in 8x8 tiles
 };
+
+// Crop pixel of entry e when the entries run through 8x8 tiles, the tiles
+// in row order (w: crop width, a multiple of 8; the crop height too): the 64
+// work items of a wave are then the pixels of one tile, whose camera rays
+// traverse nearly the same nodes.
+LMN_HD uint32_t tile_pixel(uint32_t e, uint32_t w) {
+  const uint32_t tile = e >> 6, in = e & 63u, tiles_x = w >> 3;
+  const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+  return ((ty << 3) | (in >> 3)) * w + ((tx << 3) | (in & 7u));
+}
 
 // Generates work item k = sample * n + entry (n pixels, see FrameDev): its
 // camera ray, RNG stream and filter weight (src/render.rs:1683-1690).  In
@@ -1019,7 +1029,7 @@ LMN_HD void generate_item(const FrameDev& F, const CameraT<R>& C, uint64_t k, Rn
   uint64_t n = F.pix_list ? uint64_t(F.n_list) : uint64_t(F.crop_w) * F.crop_h;
   uint32_t s = uint32_t(k / n);
   uint32_t e = uint32_t(k - uint64_t(s) * n);
-  uint32_t pl = F.pix_list ? F.pix_list[e] : e;
+  uint32_t pl = F.pix_list ? F.pix_list[e] : F.tile_w ? tile_pixel(e, F.tile_w) : e;
   uint32_t ci = pl % F.crop_w, cj = pl / F.crop_w;
   uint32_t gi = ci + F.crop_x, gj = cj + F.crop_y;
   uint64_t gpix = uint64_t(gj) * F.full_w + gi;
