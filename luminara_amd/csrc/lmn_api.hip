@@ -1234,10 +1234,11 @@ int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_rend
   }
   if (total == 0) return set_err(LMN_ERR_INVALID, "lmn_autotune: nothing to render");
   const std::string key = tune_key(sc.data(), orig.data(), fs.data(), n_scenes);
-  // probe: the same frames at reduced spp, about 2^28 paths in all (several
+  // probe: the same frames at reduced spp, about 2^29 paths in all (eight
   // fills of the largest pool, so that the drain at the end of a render
-  // weighs about as little as in the full one)
-  const double scale = std::min(1.0, double(1ull << 28) / double(total));
+  // weighs about as little as in the full one; with 2^28 the probes
+  // favoured smaller pools than the full render does)
+  const double scale = std::min(1.0, double(1ull << 29) / double(total));
   std::vector<lmn_render_params> pp(params, params + n_scenes);
   uint64_t probe_paths = 0;
   for (size_t i = 0; i < n_scenes; ++i) {
@@ -1289,27 +1290,44 @@ int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_rend
     }
     return LMN_OK;
   };
-  // coordinate descent: pool size, workgroup size, stack depth, grid size
-  for (uint32_t pool : {1u << 21, 1u << 22, 1u << 24, 1u << 25, 1u << 26}) {
-    WaveConfig c = best;
-    c.pool = uint32_t(std::min<uint64_t>(pool, std::max<uint64_t>(total, 64)));
-    if (c.pool == best.pool) continue;
-    if ((rc = consider(with_grid(c, 1.0))) != LMN_OK) return rc;
-  }
-  {
-    WaveConfig c = best;
-    c.block = best.block == 256 ? 128 : 256;
-    if ((rc = consider(with_grid(c, 1.0))) != LMN_OK) return rc;
-  }
+  // pool size and workgroup size together (they interact: the grid follows
+  // both), then the grid size up to one thread per pool path, then the LDS
+  // stack depth, then the grid again around the winner
+  const WaveConfig start = best;
+  for (uint32_t pool : {1u << 21, 1u << 22, 1u << 24, 1u << 25, 1u << 26})
+    for (uint32_t block : {128u, 256u}) {
+      WaveConfig c = start;
+      c.pool = uint32_t(std::min<uint64_t>(pool, std::max<uint64_t>(total, 64)));
+      c.block = block;
+      c = with_grid(c, 1.0);
+      if (c.pool == best.pool && c.block == best.block && c.grid == best.grid) continue;
+      if ((rc = consider(c)) != LMN_OK) return rc;
+    }
+  auto grid_pass = [&](std::initializer_list<double> fs) -> int {
+    const WaveConfig base = best;
+    for (double f : fs) {
+      WaveConfig c = with_grid(base, f);
+      if (c.grid == best.grid) continue;
+      int r = consider(c);
+      if (r != LMN_OK) return r;
+    }
+    return LMN_OK;
+  };
+  if ((rc = grid_pass({0.25, 0.5, 2.0, 4.0, 8.0, 16.0})) != LMN_OK) return rc;
   {
     WaveConfig c = best;
     c.stack = best.stack == 16 ? 32 : 16;
     if ((rc = consider(c)) != LMN_OK) return rc;
   }
-  for (double f : {0.125, 0.25, 0.5, 2.0, 4.0, 8.0}) {
-    WaveConfig c = with_grid(best, f);
-    if (c.grid == best.grid) continue;
-    if ((rc = consider(c)) != LMN_OK) return rc;
+  {
+    // the grid again, at half and twice the winner's
+    const WaveConfig base = best;
+    for (uint32_t g : {base.grid / 2, base.grid * 2}) {
+      WaveConfig c = base;
+      c.grid = std::max<uint32_t>(1, std::min<uint32_t>(g, (c.pool + c.block - 1) / c.block));
+      if (c.grid == best.grid) continue;
+      if ((rc = consider(c)) != LMN_OK) return rc;
+    }
   }
   tune_cache().store(key, best);
   out->pool_paths = best.pool;
