@@ -302,7 +302,8 @@ typedef struct lmn_stats {
   double extend_ms;        /* summed stage times (LMN_FLAG_STAGE_TIMING):  */
   double shade_ms;         /* extend and shadow are the main BVH's kernels */
   double shadow_ms;
-  double regen_ms;
+  double regen_ms;         /* the bookkeeping between iterations (new work items take the
+                              freed slots; extend and shade generate their camera rays)    */
   double march_ms;         /* the marched kinds' extend and shadow kernels */
   uint64_t samples_traced; /* camera samples traced (adaptive rounds trace a few past the stop) */
   uint32_t rounds;         /* wavefront runs: adaptive rounds, time-limit bands              */
