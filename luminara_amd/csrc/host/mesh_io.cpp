@@ -8,6 +8,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <atomic>
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -325,7 +326,10 @@ bool write_sidecar_for(const std::string& mesh_path, const MeshData& mesh, uint6
   // written under a temporary name and renamed into place, so a reader never
   // sees a partial sidecar
   const std::string final_path = sidecar_path(mesh_path);
-  const std::string tmp = final_path + ".tmp" + std::to_string(getpid());
+  // unique per process and writer, so that loaders on several threads never
+  // share a temporary file
+  static std::atomic<uint64_t> writers{0};
+  const std::string tmp = final_path + ".tmp" + std::to_string(getpid()) + "." + std::to_string(writers++);
   FILE* f = std::fopen(tmp.c_str(), "wb");
   if (!f) return false;
   bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 &&
