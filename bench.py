@@ -402,7 +402,7 @@ def measure(wl_name, scenes, desc, args, want_images, tag):
                     "valu_lane_util": rec.get("valu_lane_util") if rec else None,
                     "kernel": "k_extend", "avg_launch_us": avg_launch_s * 1e6, "launches_per_step": launches,
                     "bytes_per_launch": bytes_per_launch,
-                    "bytes_model": "128 B per 4-wide BVH node fetched + 64 B per primitive record tested "
+                    "bytes_model": f"{counts['node_bytes']} B per 4-wide BVH node fetched + 64 B per primitive record tested "
                                    "+ 48 B ray read + 32 B (hit) or 16 B (miss) record written",
                     "per_ray": {"node_fetches": counts["node_fetches"] / rays, "prim_tests": counts["prim_tests"] / rays,
                                 "hit_fraction": hits / rays}}

@@ -452,6 +452,8 @@ struct Builder {
   }
 };
 
+}  // namespace
+
 // Collapses the FP32 binary BVH (sibling-pair layout) into a 4-wide one: a
 // 4-wide node starts from a binary node's two children and keeps opening the
 // child of largest surface area until it holds four, the usual top-down
@@ -597,6 +599,72 @@ uint32_t collapse_bvh4(const std::vector<NodeF>& n2, std::vector<Node4F>& n4) {
   }
   return worst;
 }
+
+// Quantises the 4-wide nodes' child boxes onto a per-node grid (Node4Q).
+// Axis a of a node gets the origin org = the lowest child plane and the
+// smallest power-of-two step s with 253 s >= the node's extent; a child's
+// lower plane index is the largest q with org + q s below its FP32 plane,
+// the upper one the smallest q with org + q s above it (strictly: an index
+// landing exactly on the plane moves one step out, so the slab distance
+// the kernel forms as fma(q, s / d, (org - o) / d) has a whole step of slack
+// against its own rounding; only q = 0, the origin plane itself, is exact).
+// The comparisons run in long double, exact for the FP32 differences here.
+void quantize_bvh4(const std::vector<Node4F>& in, std::vector<Node4Q>& out) {
+  out.assign(in.size(), Node4Q{});
+  for (size_t i = 0; i < in.size(); ++i) {
+    const Node4F& f = in[i];
+    Node4Q& q = out[i];
+    const uint32_t n = f.n;
+    const float* lo[3] = {f.lox, f.loy, f.loz};
+    const float* hi[3] = {f.hix, f.hiy, f.hiz};
+    uint32_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0}, meta = n << 24;
+    for (int a = 0; a < 3; ++a) {
+      float org = lo[a][0], top = hi[a][0];
+      for (uint32_t k = 1; k < n; ++k) {
+        org = std::min(org, lo[a][k]);
+        top = std::max(top, hi[a][k]);
+      }
+      const long double ext = (long double)top - (long double)org;
+      int e = -126;  // smallest normal step: a flat axis has every index 0
+      if (ext > 0) {
+        int x;
+        std::frexp(ext / 253, &x);  // 2^(x-1) <= ext / 253 < 2^x
+        e = std::min(std::max(x - 1, -126), 127);
+        while (e > -126 && std::ldexp((long double)253, e - 1) >= ext) --e;
+        while (e < 127 && std::ldexp((long double)253, e) < ext) ++e;
+      }
+      const long double step = std::ldexp((long double)1, e);
+      q.org[a] = org;
+      meta |= uint32_t(e + 127) << (8 * a);
+      for (uint32_t k = 0; k < n; ++k) {
+        const long double dl = (long double)lo[a][k] - (long double)org;
+        const long double dh = (long double)hi[a][k] - (long double)org;
+        long double ql = std::floor(dl / step), qh = std::ceil(dh / step);
+        ql = std::min(std::max(ql, (long double)0), (long double)255);
+        qh = std::min(std::max(qh, (long double)0), (long double)255);
+        while (ql > 0 && ql * step > dl) ql -= 1;
+        if (ql > 0 && ql * step == dl) ql -= 1;
+        while (qh < 255 && qh * step < dh) qh += 1;
+        if (qh < 255 && qh * step == dh) qh += 1;
+        qlo[a] |= uint32_t(ql) << (8 * k);
+        qhi[a] |= uint32_t(qh) << (8 * k);
+      }
+    }
+    q.qlo_x = qlo[0];
+    q.qlo_y = qlo[1];
+    q.qlo_z = qlo[2];
+    q.qhi_x = qhi[0];
+    q.qhi_y = qhi[1];
+    q.qhi_z = qhi[2];
+    q.child0 = f.child[0];
+    q.child1 = f.child[1];
+    q.child2 = f.child[2];
+    q.child3 = f.child[3];
+    q.meta = meta;
+  }
+}
+
+namespace {
 
 float round_down(double x) {
   float f = float(x);
@@ -924,7 +992,11 @@ int build_scene_tables(const lmn_scene_desc& desc, const BvhOptions& opt, HostSc
     err = "too many primitives for the 4-wide BVH";
     return LMN_ERR_INVALID;
   }
-  out.stack4_words = collapse_bvh4(out.nodesF, out.nodes4F);
+  {
+    std::vector<Node4F> wide;
+    out.stack4_words = collapse_bvh4(out.nodesF, wide);
+    quantize_bvh4(wide, out.nodes4);
+  }
   out.n_bounded = uint32_t(bounded.size() + marched.size());
   out.n_planes = uint32_t(planes.size());
   for (uint32_t p : planes) out.order.push_back(p);

@@ -22,8 +22,8 @@ struct BvhOptions {
 };
 
 struct HostScene {
-  std::vector<Node4F> nodes4F;  // main BVH collapsed to 4-wide (FP32 path)
-  uint32_t stack4_words = 0;      // worst-case traversal stack of nodes4F, 32-bit words
+  std::vector<Node4Q> nodes4;     // main BVH collapsed to 4-wide, quantised boxes (FP32 path)
+  uint32_t stack4_words = 0;      // worst-case traversal stack of nodes4, 32-bit words
   std::vector<NodeF> nodesF;   // main BVH
   std::vector<NodeD> nodesD;
   std::vector<NodeF> mnodesF;  // BVH over the marched kinds
@@ -54,6 +54,12 @@ struct HostScene {
 
 // Returns LMN_OK or an error code with a message.
 int build_host_scene(const lmn_scene_desc& desc, const BvhOptions& opt, HostScene& out, std::string& err);
+
+// The FP32 binary BVH collapsed to 4-wide nodes (returns the worst-case
+// traversal stack in 32-bit words), and those nodes' boxes quantised into
+// the device layout.
+uint32_t collapse_bvh4(const std::vector<NodeF>& n2, std::vector<Node4F>& n4);
+void quantize_bvh4(const std::vector<Node4F>& in, std::vector<Node4Q>& out);
 
 // World-space bounds of a descriptor primitive (transform applied).
 void prim_world_bounds(const lmn_scene_desc& desc, const lmn_prim& p, double lo[3], double hi[3]);

@@ -103,7 +103,7 @@ Workspace& workspace() {
 
 struct lmn_scene {
   HostScene host;
-  DevBuf nodes4F, nodesF, nodesD, mnodesF, mnodesD, primsF, primsD, xfF, xfD, matF, matD, tex, texels, lightF, lightD,
+  DevBuf nodes4, nodesF, nodesD, mnodesF, mnodesD, primsF, primsD, xfF, xfD, matF, matD, tex, texels, lightF, lightD,
       attrF, attrD;
   SceneView<float> viewF{};
   SceneView<double> viewD{};
@@ -243,7 +243,7 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   HostScene& h = s->host;
   if (h.stack4_words > uint32_t(kPrivateStackWords32))
     return set_err(LMN_ERR_UNSUPPORTED, "BVH too deep for the FP32 traversal stack");
-  HIP_TRY(upload(s->nodes4F, h.nodes4F));
+  HIP_TRY(upload(s->nodes4, h.nodes4));
   HIP_TRY(upload(s->nodesF, h.nodesF));
   HIP_TRY(upload(s->nodesD, h.nodesD));
   HIP_TRY(upload(s->mnodesF, h.mnodesF));
@@ -260,12 +260,12 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   HIP_TRY(upload(s->lightD, h.lightD));
   HIP_TRY(upload(s->attrF, h.attrF));
   HIP_TRY(upload(s->attrD, h.attrD));
-  s->device_bytes = s->nodes4F.bytes + s->nodesF.bytes + s->nodesD.bytes + s->mnodesF.bytes + s->mnodesD.bytes + s->primsF.bytes + s->primsD.bytes + s->xfF.bytes +
+  s->device_bytes = s->nodes4.bytes + s->nodesF.bytes + s->nodesD.bytes + s->mnodesF.bytes + s->mnodesD.bytes + s->primsF.bytes + s->primsD.bytes + s->xfF.bytes +
                     s->xfD.bytes + s->matF.bytes + s->matD.bytes + s->tex.bytes + s->texels.bytes +
                     s->lightF.bytes + s->lightD.bytes + s->attrF.bytes + s->attrD.bytes;
   SceneView<float>& F = s->viewF;
-  F.nodes4 = s->nodes4F.as<Node4F>();
-  F.n_nodes4 = uint32_t(h.nodes4F.size());
+  F.nodes4 = s->nodes4.as<Node4Q>();
+  F.n_nodes4 = uint32_t(h.nodes4.size());
   F.nodes = s->nodesF.as<NodeF>();
   F.mnodes = s->mnodesF.as<NodeF>();
   F.n_mnodes = uint32_t(h.mnodesF.size());
@@ -1191,7 +1191,7 @@ int lmn_profile_extend(const lmn_scene* const* scenes, size_t n_scenes, const lm
     counts->tri_tests = h.tri_tests;
     counts->rays = h.rays;
     counts->hits = h.hits;
-    counts->node_bytes = sizeof(Node4F);
+    counts->node_bytes = sizeof(Node4Q);
     counts->prim_bytes = sizeof(PrimT<float>);
   }
   (void)hipFree(d_tc);

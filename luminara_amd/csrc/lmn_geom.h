@@ -7,7 +7,8 @@
 //             internal: a = first child, b = split axis; leaf: a = first
 //             prim, b = LEAF_BIT | prim count.  The FP64 path traverses it.
 //   nodes4  : the FP32 path's 4-wide BVH, collapsed from the binary one:
-//             128-byte nodes holding the boxes of up to four children.
+//             64-byte nodes holding the boxes of up to four children,
+//             quantised to 8 bits per plane on a per-node grid (Node4Q).
 //             Two nodes map onto the same primitive records.
 //   prims   : primitive records reordered into leaf order (64 B FP32 /
 //             128 B FP64), so a leaf's primitives are one contiguous fetch.
@@ -90,9 +91,9 @@ struct alignas(16) NodeD {
   uint32_t a, b;
 };
 static_assert(sizeof(NodeD) == 64, "FP64 BVH node must be 64 bytes");
-// FP32 4-wide BVH node: the boxes of up to four children in SoA order (one
-// node is eight 16-byte loads, two 64-byte lines) and the child entries: a
-// node index, or a leaf LEAF4 | (count - 1) << 28 | first primitive record.
+// FP32 4-wide BVH node as the host builds it, before quantisation (Node4Q):
+// the FP32 boxes of up to four children in SoA order and the child entries:
+// a node index, or a leaf LEAF4 | (count - 1) << 28 | first primitive record.
 struct alignas(128) Node4F {
   float lox[4], loy[4], loz[4];
   float hix[4], hiy[4], hiz[4];
@@ -101,6 +102,27 @@ struct alignas(128) Node4F {
   uint32_t pad[3];
 };
 static_assert(sizeof(Node4F) == 128, "FP32 BVH4 node must be 128 bytes");
+// The FP32 path's 4-wide node as the device reads it: 64 bytes, four 16-byte
+// loads.  Child boxes are quantised to 8 bits per plane on a per-node grid
+// of step 2^e per axis: plane(q) = fma(q, 2^e, org), rounded outward on the
+// host so that every quantised box contains its child's FP32 box, with at
+// least one grid step to spare except where the plane is the grid origin
+// itself (there the plane and its slab distance are the unquantised ones).
+//   w0: org.x org.y org.z qlo.x      w1: qlo.y qlo.z qhi.x qhi.y
+//   w2: qhi.z child0 child1 child2   w3: child3 meta  pad pad
+// qlo.a / qhi.a hold child k's lower / upper plane index of axis a in byte
+// k; meta holds the biased FP32 exponent of axis a's grid step in byte a
+// and the number of children in use in byte 3.  Every word the box test
+// needs sits in a different 16-byte quarter, so all four loads are issued
+// together and the node costs one memory round trip.
+struct alignas(64) Node4Q {
+  float org[3];
+  uint32_t qlo_x;
+  uint32_t qlo_y, qlo_z, qhi_x, qhi_y;
+  uint32_t qhi_z, child0, child1, child2;
+  uint32_t child3, meta, pad0, pad1;
+};
+static_assert(sizeof(Node4Q) == 64, "quantised FP32 BVH4 node must be 64 bytes");
 constexpr uint32_t LEAF4 = 0x80000000u;
 constexpr uint32_t LEAF4_FIRST = 0x0fffffffu;  // records addressable by a leaf entry
 constexpr uint32_t LEAF4_MAX = 8;              // primitives per leaf entry

@@ -292,7 +292,7 @@ __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_shadow(BatchDev 
         dist = o4.w;
         closest = o4.w - 0.001f;
         sid = __float_as_uint(sh.c[i].w);
-        inv = {1.f / d.x, 1.f / d.y, 1.f / d.z};
+        inv = slab_inv(d);
         ref = 0;
         best = 0;
         found = false;

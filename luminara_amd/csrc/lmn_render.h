@@ -48,7 +48,7 @@ struct CameraT {
 
 template <class R>
 struct SceneView {
-  const Node4F* nodes4;                     // main BVH, 4-wide (FP32 path)
+  const Node4Q* nodes4;                     // main BVH, 4-wide, quantised boxes (FP32 path)
   const typename NodeSel<R>::type* nodes;   // main BVH (no marched kinds)
   const typename NodeSel<R>::type* mnodes;  // BVH over the marched kinds
   const PrimT<R>* prims;          // bounded prims (leaf order) then planes
@@ -377,6 +377,71 @@ LMN_HD void bvh_walk(const SceneView<R>& S, const typename NodeSel<R>::type* nod
   }
 }
 
+// The four 16-byte quarters of 4-wide node `ref`.  On the device they are
+// global-memory loads off the scene's node array (a scalar base and a 32-bit
+// offset), all four issued before any is used.
+struct Words4 {
+  uint32_t x, y, z, w;
+};
+LMN_HD void node4_words(const Node4Q* nodes, uint32_t ref, Words4 q[4]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(1))) const uint4 G4;
+  G4* p = (G4*)((__attribute__((address_space(1))) const char*)nodes + (ref << 6));
+  for (int k = 0; k < 4; ++k) {
+    const uint4 v = p[k];
+    q[k] = {v.x, v.y, v.z, v.w};
+  }
+#else
+  __builtin_memcpy(q, nodes + ref, sizeof(Node4Q));
+#endif
+}
+
+// Inverse direction for the FP32 slab tests: components clamped to 2^80 in
+// magnitude, so that an axis-parallel ray gets a finite factor (the 4-wide
+// nodes form slab distances as fma(q, step / d, (org - o) / d), where an
+// infinite factor would meet 0 * inf).  The clamped box test is at least as
+// permissive as the unclamped one.
+LMN_HD V3<float> slab_inv(V3<float> d) {
+  constexpr float C = 1.2089258196146292e24f;  // 2^80
+  return {fminf(fmaxf(1.f / d.x, -C), C), fminf(fmaxf(1.f / d.y, -C), C), fminf(fmaxf(1.f / d.z, -C), C)};
+}
+
+// Entry distances of a 4-wide node's children (INFINITY where the ray
+// misses the child's box within [tmin, closest]) and their entries.
+LMN_HD void node4_children(const Words4 w[4], V3<float> o, V3<float> inv, float tmin, float closest, float tk[4],
+                           uint32_t ch[4]) {
+  const uint32_t meta = w[3].y, nc = meta >> 24;
+  // slab distance of grid plane q on axis a: fma(q, a_a, b_a), with
+  // a_a = step_a / d_a (exact: step_a is a power of two) and
+  // b_a = (org_a - o_a) / d_a
+  const float ax = float_of((meta & 0xffu) << 23) * inv.x;
+  const float ay = float_of(((meta >> 8) & 0xffu) << 23) * inv.y;
+  const float az = float_of(((meta >> 16) & 0xffu) << 23) * inv.z;
+  const float bx = (float_of(w[0].x) - o.x) * inv.x;
+  const float by = (float_of(w[0].y) - o.y) * inv.y;
+  const float bz = (float_of(w[0].z) - o.z) * inv.z;
+  // near and far plane bytes per axis: the lower planes are near where the
+  // direction component is positive
+  const uint32_t lox = w[0].w, loy = w[1].x, loz = w[1].y, hix = w[1].z, hiy = w[1].w, hiz = w[2].x;
+  const bool sx = inv.x < 0.f, sy = inv.y < 0.f, sz = inv.z < 0.f;
+  const uint32_t nx = sx ? hix : lox, fx = sx ? lox : hix;
+  const uint32_t ny = sy ? hiy : loy, fy = sy ? loy : hiy;
+  const uint32_t nz = sz ? hiz : loz, fz = sz ? loz : hiz;
+  ch[0] = w[2].y;
+  ch[1] = w[2].z;
+  ch[2] = w[2].w;
+  ch[3] = w[3].x;
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t sh = 8u * uint32_t(k);
+    const float tnx = fmaf(float((nx >> sh) & 0xffu), ax, bx), tfx = fmaf(float((fx >> sh) & 0xffu), ax, bx);
+    const float tny = fmaf(float((ny >> sh) & 0xffu), ay, by), tfy = fmaf(float((fy >> sh) & 0xffu), ay, by);
+    const float tnz = fmaf(float((nz >> sh) & 0xffu), az, bz), tfz = fmaf(float((fz >> sh) & 0xffu), az, bz);
+    const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+    const float tf = fminf(fminf(tfx, tfy), fminf(tfz, closest));
+    tk[k] = (uint32_t(k) < nc && tn <= tf) ? tn : INFINITY;
+  }
+}
+
 // Ordered walk of the FP32 4-wide BVH.  A node's (up to) four child boxes
 // are tested together; the hits are sorted by entry distance, the nearest is
 // visited next and the others pushed farthest first, each with its entry
@@ -387,35 +452,18 @@ LMN_HD void bvh_walk(const SceneView<R>& S, const typename NodeSel<R>::type* nod
 // walk has ended (stack empty, or a visibility query's first hit).  A walk
 // is `ref = 0` (the root) followed by steps until one returns true, so a
 // caller may interleave other work (a persistent kernel's ray fetch)
-// between steps.
+// between steps.  `inv` is slab_inv(d).
 template <uint32_t FEAT, bool ANY, class Stack>
 LMN_HD bool bvh4_step(const SceneView<float>& S, V3<float> o, V3<float> d, V3<float> inv, float tmin,
                       float& closest, uint32_t& best, bool& found, uint32_t& ref, Stack& st, uint32_t& nf,
                       uint32_t& pt, uint32_t& tt, bool counting, uint32_t org = kNoOrigin) {
   if (!(ref & LEAF4)) {
-    const Node4F& nd = S.nodes4[ref];
+    Words4 w[4];
+    node4_words(S.nodes4, ref, w);
     ++nf;
-    float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
-    uint32_t ch[4];
-    for (int k = 0; k < 4; ++k) {
-      lox[k] = nd.lox[k];
-      loy[k] = nd.loy[k];
-      loz[k] = nd.loz[k];
-      hix[k] = nd.hix[k];
-      hiy[k] = nd.hiy[k];
-      hiz[k] = nd.hiz[k];
-      ch[k] = nd.child[k];
-    }
-    uint32_t nc = nd.n;
     float tk[4];
-    for (int k = 0; k < 4; ++k) {
-      float tx0 = (lox[k] - o.x) * inv.x, tx1 = (hix[k] - o.x) * inv.x;
-      float ty0 = (loy[k] - o.y) * inv.y, ty1 = (hiy[k] - o.y) * inv.y;
-      float tz0 = (loz[k] - o.z) * inv.z, tz1 = (hiz[k] - o.z) * inv.z;
-      float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-      float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), closest));
-      tk[k] = (uint32_t(k) < nc && tn <= tf) ? tn : INFINITY;
-    }
+    uint32_t ch[4];
+    node4_children(w, o, inv, tmin, closest, tk, ch);
     // sorting network on (tk, ch): misses (INFINITY) sink to the end
 #define LMN_CSWAP(a, b)                    \
   if (tk[b] < tk[a]) {                     \
@@ -496,7 +544,11 @@ template <class R, uint32_t FEAT, bool ANY, uint32_t PASS = PASS_ALL, class Stac
 LMN_HD bool trace_pass(const SceneView<R>& S, V3<R> o, V3<R> d, R tmin, R& closest, uint32_t& best, Stack& st,
                        TraceCounters* cnt = nullptr, uint32_t org = kNoOrigin) {
   bool found = false;
-  V3<R> inv = {R(1) / d.x, R(1) / d.y, R(1) / d.z};
+  V3<R> inv;
+  if constexpr (sizeof(R) == 4)
+    inv = slab_inv(d);
+  else
+    inv = {R(1) / d.x, R(1) / d.y, R(1) / d.z};
   uint32_t nf = 0, pt = 0, tt = 0;
   constexpr uint32_t MAIN_FEAT = FEAT & ~FEAT_MARCH;
   if constexpr (sizeof(R) == 4) {
