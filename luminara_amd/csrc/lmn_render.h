@@ -170,13 +170,39 @@ LMN_HD bool origin_rule(uint32_t kind, const R* p, V3<R> o, V3<R> d, bool& spher
   return sphere_far;
 }
 
+// A primitive record.  On the device, where PIN is set, an FP32 record is
+// four 16-byte global loads issued together and all taken before the kind
+// (in the last quarter) is decoded, so a record costs one memory round trip
+// rather than a load of the kind followed by loads of the fields its test
+// reads.  The copy lives in `local`.  Without PIN the record is read in place, field
+// by field: the kernels with FP64 islands or transforms have no registers to
+// spare for the copy (it cost them spills in the traversal loop).
+template <class R, bool PIN>
+LMN_HD const PrimT<R>& prim_record(const PrimT<R>* prims, uint32_t idx, PrimT<R>& local) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(R) == 4 && PIN) {
+    typedef __attribute__((address_space(1))) const uint4 G4;
+    G4* q = (G4*)((__attribute__((address_space(1))) const char*)prims + (uint64_t(idx) << 6));
+    uint4 w[4] = {q[0], q[1], q[2], q[3]};
+    for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(w[k].x), "+v"(w[k].y), "+v"(w[k].z), "+v"(w[k].w));
+    __builtin_memcpy(&local, w, sizeof local);
+    return local;
+  }
+#endif
+  (void)local;
+  return prims[idx];
+}
+// FP32 kernels whose records are copied whole (see prim_record).
+template <uint32_t FEAT> constexpr bool kPinRecords = (FEAT & (FEAT_F64 | FEAT_XF)) == 0;
+
 // Intersects primitive `idx` (transform applied); on a hit in
 // [tmin, closest] updates closest to the reference-reported t.  `org`: the
 // record the ray leaves from (FP32 extension rays, see origin_rule).
 template <class R, uint32_t FEAT>
 LMN_HD bool test_prim(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R tmin, R& closest,
                       uint32_t org = kNoOrigin) {
-  const PrimT<R>& pr = S.prims[idx];
+  PrimT<R> local;
+  const PrimT<R>& pr = prim_record<R, kPinRecords<FEAT>>(S.prims, idx, local);
   uint32_t km = pr.kind_mat;
   uint32_t kind = km & KIND_MASK;
   bool has_xf = (FEAT & FEAT_XF) && pr.xform != XF_NONE;
@@ -623,7 +649,8 @@ template <class R> LMN_HD V3<R> surface_point(V3<R> o, V3<R> d, R t) { return o 
 
 template <class R, uint32_t FEAT>
 LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_rep, Surf<R>& s) {
-  const PrimT<R>& pr = S.prims[idx];
+  PrimT<R> local;
+  const PrimT<R>& pr = prim_record<R, kPinRecords<FEAT>>(S.prims, idx, local);
   uint32_t km = pr.kind_mat, kind = km & KIND_MASK;
   bool has_xf = (FEAT & FEAT_XF) && pr.xform != XF_NONE;
   V3<R> oo = o, dd = d;
