@@ -1004,6 +1004,7 @@ int build_scene_tables(const lmn_scene_desc& desc, const BvhOptions& opt, HostSc
   out.primsF.resize(nrec);
   out.primsD.resize(nrec);
   out.featF = out.featD = 0;
+  out.islands_main = false;
   out.march_kinds = 0;
   for (size_t i = 0; i < nrec; ++i) {
     const lmn_prim& p = desc.prims[out.order[i]];
@@ -1012,6 +1013,7 @@ int build_scene_tables(const lmn_scene_desc& desc, const BvhOptions& opt, HostSc
     fill_record<double>(p, nullptr, out.primsD[i], out.order[i], false);
     if (p.transform >= 0) out.featF |= FEAT_XF, out.featD |= FEAT_XF;
     if (isl) out.featF |= FEAT_F64;
+    if (isl && p.kind < LMN_PRIM_TORUS) out.islands_main = true;  // an FP64 record in the main BVH
     if (p.kind >= LMN_PRIM_TORUS) out.featF |= FEAT_MARCH, out.featD |= FEAT_MARCH;
     if (p.kind == LMN_PRIM_TORUS) out.march_kinds |= 1u;
     if (p.kind == LMN_PRIM_SUPERELLIPSOID) out.march_kinds |= 2u;

@@ -46,6 +46,7 @@ struct HostScene {
   CameraT<double> camD;
   uint32_t n_bounded = 0, n_planes = 0;
   uint32_t featF = 0, featD = 0;  // FEAT_* of the FP32 and FP64 paths
+  bool islands_main = false;       // FP64 records in the main BVH (not only in the marched kinds' BVH)
   uint32_t march_kinds = 0;        // bit k: marched kind K_TORUS + k present
   uint32_t tree_depth = 0, n_leaves = 0;
   std::vector<uint32_t> order;  // order[i] = descriptor prim index of record i

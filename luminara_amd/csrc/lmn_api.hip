@@ -551,10 +551,14 @@ int wavefront_run(lmn_scene* const* scenes, const lmn_render_params* const* para
   const lmn_render_params* p0 = params[0];
   uint64_t total = 0;
   uint32_t feat = 0;
+  bool islands_main = false;
   for (const FrameDev& P : parts) {
     total += P.n_items;
     feat |= scenes[P.scene]->host.featF;
+    islands_main = islands_main || scenes[P.scene]->host.islands_main;
   }
+  // the main BVHs' features: FEAT_F64 only for FP64 islands there
+  const uint32_t feat_main = (feat & FEAT_XF) | (islands_main ? FEAT_F64 : 0u);
   if (feat & FEAT_MARCH) feat |= FEAT_F64;
   WaveConfig wc;
   int rc = resolve_wave_config(scenes, params, fs, n, total, wc);
@@ -616,6 +620,7 @@ int wavefront_run(lmn_scene* const* scenes, const lmn_render_params* const* para
   L.pool = pool;
   L.stack_lds = ss;
   L.feat = feat;
+  L.feat_main = feat_main;
   L.hit = HitSoA{W.hit.as<float4>(), W.hit_b.as<float4>(), W.hit_rec.as<uint32_t>()};
   L.sh = ShadowSoA{W.sh_o.as<float4>(), W.sh_d.as<float4>(), W.sh_c.as<float4>()};
   L.samples = W.samples.as<float4>();

@@ -104,6 +104,7 @@ struct WaveLaunch {
   uint32_t pool;       // path pool size
   uint32_t stack_lds;  // LDS short-stack entries per lane
   uint32_t feat;       // FEAT_* specialisation of the scene
+  uint32_t feat_main;  // FEAT_* of the main BVH's records (FEAT_F64: FP64 islands there)
   HitSoA hit;          // per path: the closest hit's surface (extend -> shade)
   ShadowSoA sh;
   float4* samples;

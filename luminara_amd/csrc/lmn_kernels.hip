@@ -259,14 +259,15 @@ __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_extend_count(Bat
 // ray's walk is the grid-stride walk, step for step.  Exit: a lane whose
 // next ray lies past the stage's count is done; the wave leaves when all its
 // lanes are.
-template <int SS, int BS, uint32_t FEAT>
+// MAIN: the features of the main BVH's walk (FEAT without FEAT_MARCH, and
+// without FEAT_F64 where no scene has FP64 islands in its main BVH).
+template <int SS, int BS, uint32_t FEAT, uint32_t MAIN>
 __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_shadow(BatchDev B, ShadowSoA sh,
                                                                        float4* __restrict__ samples,
                                                                        const WaveCounters* __restrict__ ctr,
                                                                        uint32_t* spill) {
   __shared__ uint32_t lds[SS * BS];
   constexpr bool DEFER = (FEAT & FEAT_MARCH) != 0;
-  constexpr uint32_t MAIN = FEAT & ~FEAT_MARCH;
   constexpr uint32_t kIdle = 0xffffffffu, kDone = 0xfffffffeu;
   const uint32_t n = ctr->n_shadow;
   const uint32_t nthreads = gridDim.x * BS;
@@ -943,35 +944,48 @@ static void launch_march(const WaveLaunch& L, const BatchDev& B, const PathSoA& 
 
 // Workgroups of the persistent shadow launch: as many as the device holds
 // at once, at most the grid-stride grid (the spill area is sized for that).
-template <int SS, int BS, uint32_t FEAT>
+template <int SS, int BS, uint32_t FEAT, uint32_t MAIN>
 static uint32_t persistent_grid(uint32_t grid) {
   static const uint32_t resident = [] {
     int dev = 0, cus = 0, per_cu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_shadow<SS, BS, FEAT>), BS, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
+                                                       reinterpret_cast<const void*>(&k_shadow<SS, BS, FEAT, MAIN>), BS, 0);
     return uint32_t(std::max(1, cus) * std::max(1, per_cu));
   }();
   return std::max(1u, std::min(grid, resident));
 }
 
+// The main passes of one feature set: MAIN is FEAT without FEAT_MARCH.
+template <int SS, int BS, uint32_t FEAT, uint32_t MAIN>
+static void launch_main_pass(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow) {
+  if (shadow)
+    hipLaunchKernelGGL((k_shadow<SS, BS, FEAT, MAIN>), dim3(persistent_grid<SS, BS, FEAT, MAIN>(L.grid)), dim3(BS), 0,
+                       L.stream, B, L.sh, L.samples, L.ctr, L.spill);
+  else if (L.tc)
+    hipLaunchKernelGGL((k_extend_count<SS, BS, MAIN>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.ctr,
+                       L.spill, L.tc);
+  else
+    hipLaunchKernelGGL((k_extend<SS, BS, MAIN>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.ctr, L.spill);
+}
 template <int SS, int BS, uint32_t FEAT>
 static void launch_stage_ss(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, bool shadow,
                             hipEvent_t mid) {
   constexpr uint32_t MAIN = FEAT & ~FEAT_MARCH;
   constexpr bool MARCH = (FEAT & FEAT_MARCH) != 0;
+  // FEAT_MARCH brings FEAT_F64 for the marched pass; the main passes carry
+  // the FP64 sphere test (and the registers it takes) only for scenes with
+  // FP64 islands in their main BVH
+  if (MARCH && (MAIN & FEAT_F64) && !(L.feat_main & FEAT_F64))
+    launch_main_pass<SS, BS, FEAT, MAIN & ~FEAT_F64>(L, B, cur, shadow);
+  else
+    launch_main_pass<SS, BS, FEAT, MAIN>(L, B, cur, shadow);
   if (shadow) {
-    hipLaunchKernelGGL((k_shadow<SS, BS, FEAT>), dim3(persistent_grid<SS, BS, FEAT>(L.grid)), dim3(BS), 0, L.stream,
-                       B, L.sh, L.samples, L.ctr, L.spill);
     if (mid) (void)hipEventRecord(mid, L.stream);
     if constexpr (MARCH) launch_march<SS, BS, FEAT, true>(L, B, cur);
     return;
   }
-  if (L.tc)
-    hipLaunchKernelGGL((k_extend_count<SS, BS, MAIN>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.ctr,
-                       L.spill, L.tc);
-  else
-    hipLaunchKernelGGL((k_extend<SS, BS, MAIN>), dim3(L.grid), dim3(BS), 0, L.stream, B, cur, L.hit, L.ctr, L.spill);
   if (mid) (void)hipEventRecord(mid, L.stream);
   if constexpr (MARCH) launch_march<SS, BS, FEAT, false>(L, B, cur);
 }
