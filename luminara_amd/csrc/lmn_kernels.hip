@@ -722,7 +722,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void
       s.front = (mword >> 31) != 0;
       s.mat = mword & 0x7fffffffu;
       s.prim_id = 0;
-      const MaterialT<float>& m = S.materials[s.mat];
+      MaterialT<float> mcopy;
+      const MaterialT<float>& m = record_copy<true>(S.materials, s.mat, mcopy);
       V3<float> em = skip ? V3<float>{0.f, 0.f, 0.f} : emitted(S, m, s.u, s.v);
       V3<float> att;
       if (!scatter(S, m, d, s, rng, ndir, att)) {

@@ -11,7 +11,7 @@ enum : uint32_t { L_SPHERE = 0, L_RECT = 1, L_DISK = 2 };
 enum : uint32_t { BG_SKY = 0, BG_SOLID = 1, BG_GRADIENT = 2, BG_SUN = 3 };
 
 template <class R>
-struct MaterialT {
+struct alignas(16) MaterialT {
   uint32_t kind;
   int32_t tex;
   R color[3];
@@ -26,7 +26,7 @@ struct TextureDev {
 };
 
 template <class R>
-struct LightT {
+struct alignas(16) LightT {
   uint32_t kind, mat;  // mat: emissive material it was registered from, ~0u if none
   V3<R> origin, u, v, normal, emission;
   R radius, area;
@@ -168,6 +168,30 @@ LMN_HD bool origin_rule(uint32_t kind, const R* p, V3<R> o, V3<R> d, bool& spher
                                kind != K_TRIANGLE && kind != K_PLANE;
   sphere_far = dot(o - V3<R>{p[0], p[1], p[2]}, d) < R(0);
   return sphere_far;
+}
+
+// Record i of a device table of 16-byte-aligned records, copied whole into
+// `local` on the device: its 16-byte quarters are global loads issued
+// together and all taken before any field is read, so a record whose fields
+// are read behind a test of its kind word costs one memory round trip, not
+// one per field.  On the host (and with PIN false) the record is read in place.
+template <bool PIN, class T>
+LMN_HD const T& record_copy(const T* table, uint32_t i, T& local) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (PIN) {
+    static_assert(sizeof(T) % 16 == 0 && alignof(T) >= 16, "records are whole 16-byte quarters");
+    constexpr int N = int(sizeof(T) / 16);
+    typedef __attribute__((address_space(1))) const uint4 G4;
+    G4* q = (G4*)((__attribute__((address_space(1))) const char*)table + uint64_t(i) * sizeof(T));
+    uint4 w[N];
+    for (int k = 0; k < N; ++k) w[k] = q[k];
+    for (int k = 0; k < N; ++k) asm volatile("" : "+v"(w[k].x), "+v"(w[k].y), "+v"(w[k].z), "+v"(w[k].w));
+    __builtin_memcpy(&local, w, sizeof(T));
+    return local;
+  }
+#endif
+  (void)local;
+  return table[i];
 }
 
 // A primitive record.  On the device, where PIN is set, an FP32 record is
@@ -944,7 +968,8 @@ LMN_HD bool nee_sample(const SceneView<R>& S, V3<R> hp, V3<R> normal, V3<R> albe
   uint32_t li = uint32_t(rng.next<R>() * R(nl));
   li = li < nl - 1 ? li : nl - 1;
   if (light) *light = li;
-  const LightT<R>& L = S.lights[li];
+  LightT<R> lcopy;
+  const LightT<R>& L = record_copy<sizeof(R) == 4>(S.lights, li, lcopy);
   const R pi = R(3.14159265358979323846);
   V3<R> lp, ln, em = L.emission;
   R area;
