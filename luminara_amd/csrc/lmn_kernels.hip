@@ -91,6 +91,31 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool want) {
   return base + uint32_t(__popcll(m & __lanemask_lt()));
 }
 
+// The launch's tables (scene views, frame parts, part boundaries) are written
+// by the host before the launch and only read by kernels: they are read
+// through the constant address space, so a wave-uniform index becomes a
+// scalar load through the scalar cache instead of a vector load through L1
+// (the camera-ray part search is a chain of such loads).
+template <class T>
+__device__ __forceinline__ T uniform_load(const T* table, uint64_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ((const __attribute__((address_space(4))) T*)table)[i];
+#else
+  return table[i];
+#endif
+}
+
+// The scene view of a traversal kernel: scalar loads, except in the kernels
+// with FP64 islands or transforms, whose traversal loops have no registers
+// to spare (held in scalar registers the view cost them spills).
+template <uint32_t FEAT>
+__device__ __forceinline__ SceneView<float> scene_view(const BatchDev& B, uint32_t s) {
+  if constexpr (kPinRecords<FEAT>)
+    return uniform_load(B.views, s);
+  else
+    return B.views[s];
+}
+
 // ---------------------------------------------------------------------------
 // Wave-uniform scene selection.  Paths of one scene are issued together, so
 // a wave almost always holds one scene; the loop peels one scene per pass
@@ -153,16 +178,17 @@ __device__ __forceinline__ void camera_path(const BatchDev& B, uint64_t item, Ca
   uint32_t lo = 0, hi = B.n_parts;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (B.part_begin[mid] <= it0)
+    if (uniform_load(B.part_begin, mid) <= it0)
       lo = mid;
     else
       hi = mid;
   }
-  while (lo + 1 < B.n_parts && B.part_begin[lo + 1] <= item) ++lo;
+  while (lo + 1 < B.n_parts && uniform_load(B.part_begin, lo + 1) <= item) ++lo;
   for_scene(lo, [&](uint32_t part) {
-    const FrameDev F = B.parts[part];
+    const FrameDev F = uniform_load(B.parts, part);
+    const CameraT<float> cam = uniform_load(B.views, F.scene).cam;
     uint32_t pixel;
-    generate_item<float>(F, B.views[F.scene].cam, F.item_base + (item - B.part_begin[part]), c.rng, c.o, c.d, c.w,
+    generate_item<float>(F, cam, F.item_base + (item - uniform_load(B.part_begin, part)), c.rng, c.o, c.d, c.w,
                          pixel);
     c.scene = F.scene;
     c.max_depth = F.max_depth;
@@ -207,7 +233,7 @@ __device__ __forceinline__ void extend_body(const BatchDev& B, const PathSoA& cu
     uint32_t sid, org;  // org: the record the ray leaves from
     pool_ray(B, cur, ctr, i, o, d, sid, org);
     for_scene(sid, [&](uint32_t s) {
-      const SceneView<float> S = B.views[s];
+      const SceneView<float> S = scene_view<FEAT>(B, s);
       LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
       float t = INFINITY;
       uint32_t idx = 0;
@@ -305,7 +331,7 @@ __global__ __launch_bounds__(BS) LMN_TRAVERSAL_OCCUPANCY void k_shadow(BatchDev 
     if (i == kDone) continue;
     bool fin = false;
     for_scene(sid, [&](uint32_t s) {
-      const SceneView<float> S = B.views[s];
+      const SceneView<float> S = scene_view<MAIN>(B, s);
       uint32_t nf = 0, pt = 0, tt = 0;
       const uint32_t began = uint32_t(__popcll(__ballot(true)));
       for (;;) {
@@ -389,7 +415,7 @@ __global__ __launch_bounds__(BS) void k_march_collect(BatchDev B, PathSoA cur, H
     if (i < n && march_ray<SHADOW>(B, cur, hit, sh, ctr, i, o, d, sid, tmax)) {
       q.key[i] = SHADOW ? 0ull : MARCH_EMPTY;
       for_scene(sid, [&](uint32_t s) {
-        const SceneView<float> S = B.views[s];
+        const SceneView<float> S = uniform_load(B.views, s);
         if (S.n_mnodes == 0) return;
         LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
         V3<float> inv = {1.f / d.x, 1.f / d.y, 1.f / d.z};
@@ -467,7 +493,7 @@ __global__ __launch_bounds__(BS) void k_march_screen(BatchDev B, PathSoA cur, Hi
       float tmax;
       march_ray<SHADOW>(B, cur, hit, sh, ctr, e.x, o, d, sid, tmax);
       for_scene(sid, [&](uint32_t s) {
-        const SceneView<float> S = B.views[s];
+        const SceneView<float> S = uniform_load(B.views, s);
         const MarchedRay r = marched_ray<FEAT>(S, e.y, o, d);
         keep = S.exact_marched == 0 || se_may_cross(S.prims64[e.y].p, r.o64, r.d64, 0.001, double(tmax), 8);
       });
@@ -501,7 +527,7 @@ __global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_test(BatchDev 
     float tmax;
     march_ray<SHADOW>(B, cur, hit, sh, ctr, e.x, o, d, sid, tmax);
     for_scene(sid, [&](uint32_t s) {
-      const SceneView<float> S = B.views[s];
+      const SceneView<float> S = uniform_load(B.views, s);
       float t = tmax;
       if (test_marched<KIND, FEAT, SHADOW>(S, e.y, o, d, 0.001f, t)) {
         if (SHADOW)
@@ -537,7 +563,7 @@ __global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_bracket(BatchD
     SeBracket br{};
     bool queue = false;
     for_scene(sid, [&](uint32_t s) {
-      const SceneView<float> S = B.views[s];
+      const SceneView<float> S = uniform_load(B.views, s);
       float t = tmax;
       if (S.exact_marched == 0) {
         if (test_marched<K_SUPERELLIPSOID, FEAT>(S, e.y, o, d, 0.001f, t))
@@ -554,7 +580,7 @@ __global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_bracket(BatchD
       continue;
     }
     for_scene(sid, [&](uint32_t s) {  // queue full: refine here
-      const SceneView<float> S = B.views[s];
+      const SceneView<float> S = uniform_load(B.views, s);
       const MarchedRay r = marched_ray<FEAT>(S, e.y, o, d);
       double t64;
       float t = tmax;
@@ -578,7 +604,7 @@ __global__ __launch_bounds__(BS) void k_march_refine(BatchDev B, PathSoA cur, Hi
     float tmax;
     march_ray<false>(B, cur, hit, sh, ctr, m.ray, o, d, sid, tmax);
     for_scene(sid, [&](uint32_t s) {
-      const SceneView<float> S = B.views[s];
+      const SceneView<float> S = uniform_load(B.views, s);
       const MarchedRay r = marched_ray<FEAT>(S, m.rec, o, d);
       double t64;
       float t = tmax;
@@ -605,7 +631,7 @@ __global__ __launch_bounds__(BS) LMN_MARCH_OCCUPANCY void k_march_fallback(Batch
     float tmax;
     march_ray<SHADOW>(B, cur, hit, sh, ctr, i, o, d, sid, tmax);
     for_scene(sid, [&](uint32_t s) {
-      const SceneView<float> S = B.views[s];
+      const SceneView<float> S = uniform_load(B.views, s);
       LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
       float t = tmax;
       uint32_t idx = 0;
@@ -639,7 +665,7 @@ __global__ __launch_bounds__(BS) void k_march_finish(BatchDev B, PathSoA cur, Hi
     uint32_t sid, org;
     pool_ray(B, cur, ctr, i, o, d, sid, org);
     for_scene(sid, [&](uint32_t s) {
-      const SceneView<float> S = B.views[s];
+      const SceneView<float> S = uniform_load(B.views, s);
       store_hit<FEAT>(S, hit, i, o, d, __uint_as_float(uint32_t(key >> 32)), uint32_t(key));
     });
   }
@@ -708,7 +734,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     float sp_dist = 0.f;
     bool use_nee = false;
     if (valid) for_scene(sid, [&](uint32_t sc) {
-      const SceneView<float> S = B.views[sc];
+      const SceneView<float> S = uniform_load(B.views, sc);
       if (mword == HIT_MISS) {
         add = hadamard(beta, background(S.bg, d));
         return;
