@@ -47,36 +47,61 @@ __device__ __forceinline__ V3<float> f3(float4 a) { return {a.x, a.y, a.z}; }
 // ballot counts go to LDS, one thread takes the block's total from each
 // global counter (one atomic per queue per block and round instead of one
 // per wave: the counters are device-wide, so their atomics are the
-// contended resource), and every lane gets its slot.  All lanes of the
-// block must call it; `lds` holds 2 * (BS / 64) + 2 words and is reused
-// after the last barrier of the next call, so a caller alternates two.
+// contended resource), and every lane gets its slot.  All lanes of the block
+// must call it.  The second queue's slots are ordered by a 3-bit key
+// within the block's range (stable within a key): the paths that survive a
+// bounce go to the next pool grouped by the octant of their new direction,
+// so the next extend's waves hold rays of one octant from nearby pixels and
+// walk the tree more alike.  64 lanes of wave 0 scan the 8 x 8 (key, wave)
+// counts; `lds` holds 64 + BS / 64 + 2 words and is reused after the
+// last barrier of the next call, so a caller alternates two.
 template <int BS>
-__device__ __forceinline__ void block_reserve2(uint32_t* c0, uint32_t* c1, bool w0, bool w1, uint32_t& s0,
-                                               uint32_t& s1, uint32_t* lds) {
+__device__ __forceinline__ void block_reserve2_keyed(uint32_t* c0, uint32_t* c1, bool w0, bool w1, uint32_t key1,
+                                                     uint32_t& s0, uint32_t& s1, uint32_t* lds) {
   constexpr int NW = BS / 64;
-  uint32_t wave = threadIdx.x / 64u;
-  unsigned long long m0 = __ballot(w0), m1 = __ballot(w1);
-  if (__lane_id() == 0) {
-    lds[wave] = uint32_t(__popcll(m0));
-    lds[NW + wave] = uint32_t(__popcll(m1));
+  static_assert(NW * 8 == 64, "one scan lane per (key, wave) pair");
+  uint32_t* cnt1 = lds;            // [key][wave] counts, then exclusive offsets
+  uint32_t* cnt0 = lds + 64;       // [wave] counts of the first queue, then offsets
+  uint32_t* base = lds + 64 + NW;  // the block's first slot in each queue
+  const uint32_t wave = threadIdx.x / 64u;
+  const uint64_t lt = __lanemask_lt();
+  const unsigned long long m0 = __ballot(w0);
+  // the lanes of each key from one ballot per key bit (wave-uniform masks)
+  const unsigned long long mv = __ballot(w1), b0 = __ballot((key1 & 1u) != 0u), b1 = __ballot((key1 & 2u) != 0u),
+                           b2 = __ballot((key1 & 4u) != 0u);
+  if (__lane_id() < 8u) {
+    const uint32_t k = __lane_id();
+    const unsigned long long m =
+        mv & ((k & 1u) ? b0 : ~b0) & ((k & 2u) ? b1 : ~b1) & ((k & 4u) ? b2 : ~b2);
+    cnt1[k * NW + wave] = uint32_t(__popcll(m));
   }
+  const unsigned long long mine =
+      mv & ((key1 & 1u) ? b0 : ~b0) & ((key1 & 2u) ? b1 : ~b1) & ((key1 & 4u) ? b2 : ~b2);
+  if (__lane_id() == 0) cnt0[wave] = uint32_t(__popcll(m0));
   __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t0 = 0, t1 = 0;
-    for (int k = 0; k < NW; ++k) {
-      uint32_t a = lds[k], b = lds[NW + k];
-      lds[k] = t0;
-      lds[NW + k] = t1;
-      t0 += a;
-      t1 += b;
+  if (wave == 0) {
+    const uint32_t lane = __lane_id();
+    const uint32_t v = cnt1[lane];
+    uint32_t x = v;  // inclusive scan over the 64 (key, wave) counts, key-major
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = uint32_t(__shfl_up(int(x), off));
+      if (int(lane) >= off) x += y;
     }
-    lds[2 * NW] = t0 ? atomicAdd(c0, t0) : 0u;
-    lds[2 * NW + 1] = t1 ? atomicAdd(c1, t1) : 0u;
+    cnt1[lane] = x - v;
+    if (lane == 63) base[1] = x ? atomicAdd(c1, x) : 0u;
+    if (lane == 0) {
+      uint32_t t0 = 0;
+      for (int k = 0; k < NW; ++k) {
+        const uint32_t a = cnt0[k];
+        cnt0[k] = t0;
+        t0 += a;
+      }
+      base[0] = t0 ? atomicAdd(c0, t0) : 0u;
+    }
   }
   __syncthreads();
-  uint64_t lt = __lanemask_lt();
-  s0 = lds[2 * NW] + lds[wave] + uint32_t(__popcll(m0 & lt));
-  s1 = lds[2 * NW + 1] + lds[NW + wave] + uint32_t(__popcll(m1 & lt));
+  s0 = base[0] + cnt0[wave] + uint32_t(__popcll(m0 & lt));
+  s1 = base[1] + cnt1[(key1 & 7u) * NW + wave] + uint32_t(__popcll(mine & lt));
 }
 
 // Slot in a device-wide queue for every lane of the wave that wants one:
@@ -683,7 +708,7 @@ template <int BS, uint32_t FEAT>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void k_shade(BatchDev B, PathSoA cur, HitSoA hit, PathSoA nxt,
                                               ShadowSoA sh, float4* __restrict__ samples,
                                               WaveCounters* __restrict__ ctr) {
-  __shared__ uint32_t res_lds[2][2 * (BS / 64) + 2];
+  __shared__ uint32_t res_lds[2][64 + (BS / 64) + 2];
   const uint32_t n = ctr->active, ns = ctr->n_surv;
   const uint64_t cam0 = ctr->cam_item;
   uint32_t nthreads = gridDim.x * BS;
@@ -785,7 +810,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void
     else if (add.x != 0.f || add.y != 0.f || add.z != 0.f)
       add_sample(samples, item, add);
     uint32_t ps, pn;
-    block_reserve2<BS>(&ctr->n_shadow, &ctr->next_active, emit, cont, ps, pn, res_lds[round & 1u]);
+    const uint32_t octant = (ndir.x < 0.f ? 1u : 0u) | (ndir.y < 0.f ? 2u : 0u) | (ndir.z < 0.f ? 4u : 0u);
+    block_reserve2_keyed<BS>(&ctr->n_shadow, &ctr->next_active, emit, cont, octant, ps, pn, res_lds[round & 1u]);
     if (emit) {
       sh.o[ps] = make_float4(sp_o.x, sp_o.y, sp_o.z, sp_dist);
       sh.d[ps] = make_float4(sp_d.x, sp_d.y, sp_d.z, __uint_as_float(item));
@@ -1049,7 +1075,7 @@ static void launch_trace_stage(const WaveLaunch& L, const BatchDev& B, const Pat
 #undef CALL
 }
 // Shading workgroups reserve their queue slots with one device atomic per
-// queue and round (block_reserve2); 512 lanes per workgroup halve those
+// queue and round (block_reserve2_keyed); 512 lanes per workgroup halve those
 // atomics against 256 (batch shade stage 113 -> 69 ms); 1024 was slower.
 constexpr int kShadeBlock = 512;
 static void launch_shade(const WaveLaunch& L, const BatchDev& B, const PathSoA& cur, const PathSoA& nxt) {
