@@ -292,16 +292,29 @@ def cpu_model():
 
 def counter_limiter(rec, hbm_frac):
     """What holds the kernel below its HBM roofline, read off the committed
-    PMC counters: HBM-side traffic near peak means bandwidth, otherwise the
-    VALU lane use tells divergence from plain latency."""
+    PMC counters: HBM-side traffic near peak means bandwidth; otherwise VALU
+    instruction issue and the texture-data (load return) unit's busy share
+    tell an issue-bound kernel from a latency-bound one, and the VALU lane
+    use how much of that issue divergence wastes."""
     if hbm_frac is None:
         return None
     if hbm_frac >= 0.6:
         return "hbm bandwidth"
-    lanes = rec.get("valu_lane_util") if rec else None
+    rec = rec or {}
+    lanes, issue, td = rec.get("valu_lane_util"), rec.get("valu_issue"), rec.get("td_busy")
+    parts = [f"HBM-side {hbm_frac:.1%} of peak"]
+    if issue is not None:
+        parts.append(f"VALU issue {issue:.0%} of cycles")
+    if td is not None:
+        parts.append(f"load return (TD) {td:.0%} busy")
+    if lanes is not None:
+        parts.append(f"VALU lanes {lanes:.0%} active")
+    busy = [n for n, v in (("VALU issue", issue), ("load return", td)) if v is not None and v >= 0.75]
+    if busy:
+        return " and ".join(busy) + " with divergent lanes (" + ", ".join(parts) + ")"
     if lanes is not None and lanes < 0.5:
-        return f"latency and lane divergence (HBM-side {hbm_frac:.1%} of peak, VALU lanes {lanes:.0%} busy)"
-    return f"latency (HBM-side {hbm_frac:.1%} of peak)"
+        return "latency and lane divergence (" + ", ".join(parts) + ")"
+    return "latency (" + ", ".join(parts) + ")"
 
 
 def measure(wl_name, scenes, desc, args, want_images, tag):
