@@ -792,7 +792,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void
       use_nee = !is_specular(m) && S.n_lights > 0;
       V3<float> c = {0.f, 0.f, 0.f};
       bool nee_ok = false;
-      if (use_nee) nee_ok = nee_sample(S, s.p, s.n, att, rng, sp_d, sp_dist, c);
+      if (use_nee) nee_ok = nee_sample<float, true>(S, s.p, s.n, att, rng, sp_d, sp_dist, c);
       V3<float> attw = att * rrw;
       // NaN guard of one recursion level (src/render.rs:1526-1531)
       if (has_nan(em) || (nee_ok && has_nan(c)) || has_nan(attw)) return;

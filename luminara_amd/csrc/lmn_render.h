@@ -961,7 +961,10 @@ template <class R> LMN_HD void build_onb(V3<R> n, V3<R>& a, V3<R>& b) {
 // Returns false where the reference returns zero before its shadow query.
 // `geo` receives the contribution per unit albedo and emission (the scalar
 // factor of the estimate) and `light` the light picked.
-template <class R>
+// PIN: read the light record whole (record_copy), for the wavefront shading
+// kernel; the one-path-per-thread kernels, at their register limit, read it
+// in place.
+template <class R, bool PIN = false>
 LMN_HD bool nee_sample(const SceneView<R>& S, V3<R> hp, V3<R> normal, V3<R> albedo, Rng& rng, V3<R>& dir_out,
                        R& dist_out, V3<R>& contrib, R* geo = nullptr, uint32_t* light = nullptr) {
   uint32_t nl = S.n_lights;
@@ -969,7 +972,7 @@ LMN_HD bool nee_sample(const SceneView<R>& S, V3<R> hp, V3<R> normal, V3<R> albe
   li = li < nl - 1 ? li : nl - 1;
   if (light) *light = li;
   LightT<R> lcopy;
-  const LightT<R>& L = record_copy<sizeof(R) == 4>(S.lights, li, lcopy);
+  const LightT<R>& L = record_copy<PIN && sizeof(R) == 4>(S.lights, li, lcopy);
   const R pi = R(3.14159265358979323846);
   V3<R> lp, ln, em = L.emission;
   R area;
