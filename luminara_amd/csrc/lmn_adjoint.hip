@@ -77,14 +77,34 @@ __global__ void k_adj_wsum(CameraT<R> C, FrameDev F, double* __restrict__ wsum) 
   wsum[p] = s;
 }
 
+// The replay's traversal stack: FP32 walks keep their first kAdjLdsStack
+// words in LDS and the rest in a global spill area ([depth][thread]), as the
+// wavefront kernels do; the FP64 binary walk keeps its private stack.
+constexpr int kAdjLdsStack = 16;
+constexpr int kAdjBlock = 256;
+template <class R>
+struct AdjStack {
+  using type = PrivateStack<R>;
+  static __device__ type make(uint32_t*, uint32_t*, uint32_t, uint32_t) { return type{}; }
+};
+template <>
+struct AdjStack<float> {
+  using type = LdsStack<kAdjLdsStack, kAdjBlock>;
+  static __device__ type make(uint32_t* lds, uint32_t* spill, uint32_t gid, uint32_t nthreads) {
+    return type{lds, spill, threadIdx.x, gid, nthreads, 0};
+  }
+};
+
 // Capped at 4 waves per SIMD (128 VGPRs): the replay is latency-bound and
 // the extra wave outweighs the spills (24.1 -> 21.6 ms per 512^2 x 36 spp).
 template <class R, uint32_t FEAT, uint32_t MAXV>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_adjoint(SceneView<R> S, FrameDev F, AdjDev A,
+__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_adjoint(SceneView<R> S, FrameDev F, AdjDev A,
                                                  const float* __restrict__ dldi, const double* __restrict__ wsum,
                                                  R* __restrict__ g_tex, double* __restrict__ part, uint64_t k0,
-                                                 uint64_t k1) {
+                                                 uint64_t k1, uint32_t* __restrict__ spill) {
   __shared__ R s_g[ADJ_MAX_SLOTS + 1];  // roughness slots, then the emission sum
+  __shared__ uint32_t st_lds[sizeof(R) == 4 ? kAdjLdsStack * kAdjBlock : 1];
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
   for (uint32_t i = threadIdx.x; i <= A.n_slots; i += blockDim.x) s_g[i] = R(0);
   __syncthreads();
   R g_em = R(0);
@@ -102,7 +122,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     uint32_t org = kNoOrigin;  // FP32: the record a bounce ray leaves from, as in the plain kernels
     // ---- replay (the plain kernels' loop, recording vertices)
     for (uint32_t depth = F.max_depth; depth > 0; --depth) {
-      PrivateStack<R> st;
+      typename AdjStack<R>::type st = AdjStack<R>::make(st_lds, spill, gid, nthreads);
       R t;
       uint32_t idx;
       if (!trace<R, FEAT, false>(S, o, d, R(0.001), R(INFINITY), t, idx, st, nullptr, org)) {
@@ -299,28 +319,28 @@ void launch_adj_wsum_fp64(const CameraT<double>& C, const FrameDev& F, double* w
 
 void launch_adjoint_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, const AdjDev& A,
                          const float* dldi, const double* wsum, float* g_tex, double* part, uint32_t n_blocks,
-                         uint64_t k0, uint64_t k1, hipStream_t s) {
+                         uint64_t k0, uint64_t k1, uint32_t* spill, hipStream_t s) {
 #define CALL(f)                                                                                              \
   if (F.max_depth <= ADJ_MAXV_SMALL)                                                                          \
-    hipLaunchKernelGGL((k_adjoint<float, f, ADJ_MAXV_SMALL>), dim3(n_blocks), dim3(256), 0, s, S, F, A, dldi, wsum, \
-                       g_tex, part, k0, k1);                                                                    \
+    hipLaunchKernelGGL((k_adjoint<float, f, ADJ_MAXV_SMALL>), dim3(n_blocks), dim3(kAdjBlock), 0, s, S, F, A, dldi, wsum, \
+                       g_tex, part, k0, k1, spill);                                                                    \
   else                                                                                                       \
-    hipLaunchKernelGGL((k_adjoint<float, f, ADJ_MAXV>), dim3(n_blocks), dim3(256), 0, s, S, F, A, dldi, wsum, g_tex, \
-                       part, k0, k1)
+    hipLaunchKernelGGL((k_adjoint<float, f, ADJ_MAXV>), dim3(n_blocks), dim3(kAdjBlock), 0, s, S, F, A, dldi, wsum, g_tex, \
+                       part, k0, k1, spill)
   LMN_ADJ_FEAT_SWITCH(feat, CALL)
 #undef CALL
 }
 
 void launch_adjoint_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, const AdjDev& A,
                          const float* dldi, const double* wsum, double* g_tex, double* part, uint32_t n_blocks,
-                         uint64_t k0, uint64_t k1, hipStream_t s) {
+                         uint64_t k0, uint64_t k1, uint32_t* spill, hipStream_t s) {
 #define CALL(f)                                                                                               \
   if (F.max_depth <= ADJ_MAXV_SMALL)                                                                          \
-    hipLaunchKernelGGL((k_adjoint<double, f, ADJ_MAXV_SMALL>), dim3(n_blocks), dim3(256), 0, s, S, F, A, dldi, wsum, \
-                       g_tex, part, k0, k1);                                                                    \
+    hipLaunchKernelGGL((k_adjoint<double, f, ADJ_MAXV_SMALL>), dim3(n_blocks), dim3(kAdjBlock), 0, s, S, F, A, dldi, wsum, \
+                       g_tex, part, k0, k1, spill);                                                                    \
   else                                                                                                       \
-    hipLaunchKernelGGL((k_adjoint<double, f, ADJ_MAXV>), dim3(n_blocks), dim3(256), 0, s, S, F, A, dldi, wsum, g_tex, \
-                       part, k0, k1)
+    hipLaunchKernelGGL((k_adjoint<double, f, ADJ_MAXV>), dim3(n_blocks), dim3(kAdjBlock), 0, s, S, F, A, dldi, wsum, g_tex, \
+                       part, k0, k1, spill)
   LMN_ADJ_FEAT64_SWITCH(feat, CALL)
 #undef CALL
 }

@@ -74,6 +74,7 @@ struct Workspace {
   DevBuf st[2][4];  // ob, db, bk, meta for the two pools
   DevBuf hit, hit_b, hit_rec, sh_o, sh_d, sh_c, samples, acc, spill, ctr, tc, rays, out, views, parts, part_begin;
   DevBuf adj_wsum, adj_gtex, adj_part, adj_red, adj_dldi, adj_tex_out;  // path-replay adjoint
+  DevBuf adj_spill;  // FP32 adjoint traversal stack beyond its LDS words, [depth][thread]
   DevBuf mq_cand, mq_key, mq_ovf, mq_brk;  // marched-pass candidate queues
   uint32_t pool = 0;
   bool graph_failed[2] = {false, false};  // capture refused once [without, with stage events]: launch directly
@@ -1406,6 +1407,10 @@ int adjoint_common(const lmn_scene* scene, const lmn_render_params* p, const lmn
   HIP_TRY(W.adj_gtex.ensure(std::max<size_t>(n_texels * 3 * (fp64 ? 8 : 4), 16)));
   HIP_TRY(W.adj_part.ensure(size_t(n_blocks) * width * 8));
   HIP_TRY(W.adj_red.ensure(size_t(width) * 8));
+  // the FP32 replay's traversal stack beyond its LDS words, per thread of the
+  // grid: the 4-wide walk's worst case, or the binary marched walk's depth cap
+  const uint32_t spill_words = std::max(kMaxTreeDepth, H.stack4_words);
+  if (!fp64) HIP_TRY(W.adj_spill.ensure(size_t(spill_words) * n_blocks * 256 * 4));
   HIP_TRY(hipEventRecord(W.ev0, st));
   launch_adj_zero_f64(W.adj_part.as<double>(), size_t(n_blocks) * width, st);
   if (fp64)
@@ -1426,12 +1431,12 @@ int adjoint_common(const lmn_scene* scene, const lmn_render_params* p, const lmn
         SceneView<double> S = scene->viewD;
         S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
         launch_adjoint_fp64(S, H.featD, fs.F, A, d_dldi, W.adj_wsum.as<double>(), W.adj_gtex.as<double>(),
-                            W.adj_part.as<double>(), n_blocks, k0, k1, st);
+                            W.adj_part.as<double>(), n_blocks, k0, k1, W.adj_spill.as<uint32_t>(), st);
       } else {
         SceneView<float> S = scene->viewF;
         S.exact_marched = p->intersect_mode == LMN_ISECT_EXACT;
         launch_adjoint_fp32(S, H.featF, fs.F, A, d_dldi, W.adj_wsum.as<double>(), W.adj_gtex.as<float>(),
-                            W.adj_part.as<double>(), n_blocks, k0, k1, st);
+                            W.adj_part.as<double>(), n_blocks, k0, k1, W.adj_spill.as<uint32_t>(), st);
       }
     }
   }

@@ -76,6 +76,31 @@ constexpr unsigned long long MARCH_EMPTY = ~0ull;
 constexpr uint32_t MARCH_SKIP = 0xffffffffu;  // reserved slot left unused
 constexpr uint32_t MARCH_MAX_CAND = 16;
 
+// LDS short stack with spill to a global per-thread region.  The first SS
+// entries of each thread live in LDS ([SS][BS] layout, bank-conflict free);
+// deeper entries go to `spill` ([depth][threads] layout, coalesced).
+template <int SS, int BS>
+struct LdsStack {
+  uint32_t* lds;
+  uint32_t* spill;
+  uint32_t tid;
+  uint32_t slot;
+  uint32_t nthreads;
+  int sp;
+  __device__ void push(uint32_t v) {
+    if (sp < SS)
+      lds[sp * BS + tid] = v;
+    else
+      spill[size_t(sp - SS) * nthreads + slot] = v;
+    ++sp;
+  }
+  __device__ uint32_t pop() {
+    --sp;
+    return sp < SS ? lds[sp * BS + tid] : spill[size_t(sp - SS) * nthreads + slot];
+  }
+  __device__ bool empty() const { return sp == 0; }
+};
+
 // Scenes of one wavefront launch: per-scene views (device array), the
 // chunk's parts (FrameDev each) and their first work items (n_parts + 1
 // prefix, chunk-relative).
@@ -200,10 +225,10 @@ void launch_adj_wsum_fp32(const CameraT<float>& C, const FrameDev& F, double* ws
 void launch_adj_wsum_fp64(const CameraT<double>& C, const FrameDev& F, double* wsum, hipStream_t s);
 void launch_adjoint_fp32(const SceneView<float>& S, uint32_t feat, const FrameDev& F, const AdjDev& A,
                          const float* dldi, const double* wsum, float* g_tex, double* part, uint32_t n_blocks,
-                         uint64_t k0, uint64_t k1, hipStream_t s);
+                         uint64_t k0, uint64_t k1, uint32_t* spill, hipStream_t s);
 void launch_adjoint_fp64(const SceneView<double>& S, uint32_t feat, const FrameDev& F, const AdjDev& A,
                          const float* dldi, const double* wsum, double* g_tex, double* part, uint32_t n_blocks,
-                         uint64_t k0, uint64_t k1, hipStream_t s);
+                         uint64_t k0, uint64_t k1, uint32_t* spill, hipStream_t s);
 void launch_adj_reduce(const double* part, uint32_t n_blocks, uint32_t width, double* out, hipStream_t s);
 void launch_adj_zero_f32(float* p, size_t n, hipStream_t s);
 void launch_adj_zero_f64(double* p, size_t n, hipStream_t s);

@@ -15,32 +15,6 @@
 
 namespace lmn {
 
-// ---------------------------------------------------------------------------
-// LDS short stack with spill to a global per-thread region.  The first SS
-// entries of each thread live in LDS ([SS][BS] layout, bank-conflict free);
-// deeper entries go to `spill` ([depth][threads] layout, coalesced).
-template <int SS, int BS>
-struct LdsStack {
-  uint32_t* lds;
-  uint32_t* spill;
-  uint32_t tid;
-  uint32_t slot;
-  uint32_t nthreads;
-  int sp;
-  __device__ void push(uint32_t v) {
-    if (sp < SS)
-      lds[sp * BS + tid] = v;
-    else
-      spill[size_t(sp - SS) * nthreads + slot] = v;
-    ++sp;
-  }
-  __device__ uint32_t pop() {
-    --sp;
-    return sp < SS ? lds[sp * BS + tid] : spill[size_t(sp - SS) * nthreads + slot];
-  }
-  __device__ bool empty() const { return sp == 0; }
-};
-
 __device__ __forceinline__ V3<float> f3(float4 a) { return {a.x, a.y, a.z}; }
 
 // Block-aggregated slot reservation in two queues at once: each wave's
