@@ -89,7 +89,7 @@ struct AdjStack {
 };
 template <>
 struct AdjStack<float> {
-  using type = LdsStack<kAdjLdsStack, kAdjBlock>;
+  using type = LdsStack<kAdjLdsStack, kAdjBlock, true>;
   static __device__ type make(uint32_t* lds, uint32_t* spill, uint32_t gid, uint32_t nthreads) {
     return type{lds, spill, threadIdx.x, gid, nthreads, 0};
   }

@@ -77,26 +77,101 @@ constexpr uint32_t MARCH_SKIP = 0xffffffffu;  // reserved slot left unused
 constexpr uint32_t MARCH_MAX_CAND = 16;
 
 // LDS short stack with spill to a global per-thread region.  The first SS
-// entries of each thread live in LDS ([SS][BS] layout, bank-conflict free);
-// deeper entries go to `spill` ([depth][threads] layout, coalesced).
-template <int SS, int BS>
+// words of each thread live in LDS, deeper ones in `spill` (coalesced across
+// the threads of the grid).  A closest-hit walk pushes (entry, distance)
+// pairs: with PAIRS the two words of a pair sit side by side per lane (word w
+// of lane t at [w / 2][t][w % 2]), so an entry is one 8-byte access to LDS
+// or to the spill area, with one address and one depth check; without PAIRS
+// word w of lane t is at [w][t], conflict-free for the one-word entries of a
+// visibility walk.  SS is even, so a pair never straddles LDS and spill.
+template <int SS, int BS, bool PAIRS = false>
 struct LdsStack {
+  static_assert(SS % 2 == 0, "pairs must not straddle LDS and spill");
   uint32_t* lds;
   uint32_t* spill;
   uint32_t tid;
   uint32_t slot;
   uint32_t nthreads;
   int sp;
+  // word w of this lane: its index in LDS (w < SS) or in the spill area
+  __device__ uint32_t lds_index(int w) const {
+    return PAIRS ? uint32_t(w >> 1) * 2u * BS + 2u * tid + uint32_t(w & 1) : uint32_t(w) * BS + tid;
+  }
+  __device__ size_t spill_index(int w) const {
+    const size_t d = size_t(w - SS);
+    return PAIRS ? (d >> 1) * 2 * nthreads + 2 * size_t(slot) + (d & 1) : d * nthreads + slot;
+  }
+  // Accesses through address-space-qualified pointers: the LDS and spill
+  // branches then stay LDS and global accesses (from two generic pointers
+  // the compiler forms one flat access to whichever the branch picks).
+#if defined(__HIP_DEVICE_COMPILE__)
+  template <class T>
+  __device__ __attribute__((address_space(3))) T* L(uint32_t i) const {
+    return (__attribute__((address_space(3))) T*)(lds + i);
+  }
+  template <class T>
+  __device__ __attribute__((address_space(1))) T* G(size_t i) const {
+    return (__attribute__((address_space(1))) T*)(spill + i);
+  }
+#else
+  template <class T>
+  __device__ T* L(uint32_t i) const { return reinterpret_cast<T*>(lds + i); }
+  template <class T>
+  __device__ T* G(size_t i) const { return reinterpret_cast<T*>(spill + i); }
+#endif
   __device__ void push(uint32_t v) {
     if (sp < SS)
-      lds[sp * BS + tid] = v;
+      *L<uint32_t>(lds_index(sp)) = v;
     else
-      spill[size_t(sp - SS) * nthreads + slot] = v;
+      *G<uint32_t>(spill_index(sp)) = v;
     ++sp;
   }
   __device__ uint32_t pop() {
     --sp;
-    return sp < SS ? lds[sp * BS + tid] : spill[size_t(sp - SS) * nthreads + slot];
+    uint32_t v;
+    if (sp < SS)
+      v = *L<uint32_t>(lds_index(sp));
+    else
+      v = *G<uint32_t>(spill_index(sp));
+    return v;
+  }
+  // a pair (a first, then b): pop2 returns them as pushed
+  __device__ void push2(uint32_t a, uint32_t b) {
+    if constexpr (PAIRS) {
+      if (sp < SS)
+        *L<uint2>(lds_index(sp)) = make_uint2(a, b);
+      else
+        *G<uint2>(spill_index(sp)) = make_uint2(a, b);
+      sp += 2;
+    } else {
+      push(a);
+      push(b);
+    }
+  }
+  __device__ void pop2(uint32_t& a, uint32_t& b) {
+    if constexpr (PAIRS) {
+      sp -= 2;
+      uint2 v;
+      if (sp < SS) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        // one LDS read, waited on in the same block: written as a plain load
+        // the compiler merges it with the spill branch's into one flat load
+        uint64_t x;
+        const uint32_t addr = uint32_t(size_t(L<uint2>(lds_index(sp))));
+        asm volatile("ds_read_b64 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(x) : "v"(addr) : "memory");
+        v = make_uint2(uint32_t(x), uint32_t(x >> 32));
+#else
+        v = *L<uint2>(lds_index(sp));
+#endif
+      } else {
+        v = *G<uint2>(spill_index(sp));
+      }
+      a = v.x;
+      b = v.y;
+    } else {
+      b = pop();
+      a = pop();
+    }
   }
   __device__ bool empty() const { return sp == 0; }
 };

@@ -233,7 +233,7 @@ __device__ __forceinline__ void extend_body(const BatchDev& B, const PathSoA& cu
     pool_ray(B, cur, ctr, i, o, d, sid, org);
     for_scene(sid, [&](uint32_t s) {
       const SceneView<float> S = scene_view<FEAT>(B, s);
-      LdsStack<SS, BS> st{lds, spill, threadIdx.x, gid, nthreads, 0};
+      LdsStack<SS, BS, kPinRecords<FEAT>> st{lds, spill, threadIdx.x, gid, nthreads, 0};
       float t = INFINITY;
       uint32_t idx = 0;
       bool h = trace_pass<float, FEAT, false, PASS_MAIN>(S, o, d, 0.001f, t, idx, st, COUNT ? tc : nullptr, org);

@@ -90,6 +90,16 @@ struct PrivateStackN {
   int sp = 0;
   LMN_HD void push(uint32_t v) { s[sp++] = v; }
   LMN_HD uint32_t pop() { return s[--sp]; }
+  LMN_HD void push2(uint32_t a, uint32_t b) {
+    s[sp] = a;
+    s[sp + 1] = b;
+    sp += 2;
+  }
+  LMN_HD void pop2(uint32_t& a, uint32_t& b) {
+    sp -= 2;
+    a = s[sp];
+    b = s[sp + 1];
+  }
   LMN_HD bool empty() const { return sp == 0; }
 };
 template <class R>
@@ -535,8 +545,10 @@ LMN_HD bool bvh4_step(const SceneView<float>& S, V3<float> o, V3<float> d, V3<fl
       // shrinks and an entry needs no distance: one word instead of two
       for (int k = 3; k >= 1; --k)
         if (tk[k] != INFINITY) {
-          st.push(ch[k]);
-          if (!ANY) st.push(bits_of(tk[k]));
+          if (ANY)
+            st.push(ch[k]);
+          else
+            st.push2(ch[k], bits_of(tk[k]));
         }
       ref = ch[0];
       return false;
@@ -560,8 +572,9 @@ LMN_HD bool bvh4_step(const SceneView<float>& S, V3<float> o, V3<float> d, V3<fl
     }
   } else {
     while (!st.empty()) {
-      float te = float_of(st.pop());
-      uint32_t r = st.pop();
+      uint32_t r, tb;
+      st.pop2(r, tb);
+      const float te = float_of(tb);
       if (te <= closest) {
         ref = r;
         return false;
