@@ -285,6 +285,8 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   F.n_lights = uint32_t(h.lightF.size());
   F.bg = h.bgF;
   F.cam = h.camF;
+  F.textured = 0;
+  for (const MaterialT<float>& m : h.matF) F.textured |= m.tex >= 0 ? 1u : 0u;
   SceneView<double>& D = s->viewD;
   D.nodes4 = nullptr;
   D.n_nodes4 = 0;
@@ -298,6 +300,7 @@ int lmn_scene_create(const lmn_scene_desc* desc, lmn_scene** out) {
   D.materials = s->matD.as<MaterialT<double>>();
   D.textures = F.textures;
   D.texels = F.texels;
+  D.textured = F.textured;
   D.lights = s->lightD.as<LightT<double>>();
   D.attrs = s->attrD.as<AttrT<double>>();
   D.n_nodes = F.n_nodes;

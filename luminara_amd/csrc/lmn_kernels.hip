@@ -136,7 +136,7 @@ template <uint32_t FEAT>
 __device__ __forceinline__ void store_hit(const SceneView<float>& S, HitSoA hit, uint32_t i, V3<float> o, V3<float> d,
                                           float t, uint32_t idx) {
   Surf<float> s;
-  surface<float, FEAT>(S, idx, o, d, t, s);
+  surface<float, FEAT>(S, idx, o, d, t, s, S.textured != 0);
   hit.a[i] = make_float4(s.n.x, s.n.y, s.n.z, s.t_true);
   hit.b[i] = make_float4(s.u, s.v, __uint_as_float(s.mat | (s.front ? 0x80000000u : 0u)), s.t);
   hit.rec[i] = idx;

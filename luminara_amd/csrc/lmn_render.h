@@ -65,7 +65,7 @@ struct SceneView {
   BackgroundT<R> bg;
   CameraT<R> cam;
   uint32_t exact_marched;  // LMN_ISECT_EXACT for marched prims
-  uint32_t pad;
+  uint32_t textured;       // a material reads an image texture (surface (u, v) are used)
 };
 
 // Traversal statistics of an instrumented launch (bytes per unit of work).
@@ -684,8 +684,12 @@ struct Surf {
 // that forms it, so that they round alike).
 template <class R> LMN_HD V3<R> surface_point(V3<R> o, V3<R> d, R t) { return o + d * t; }
 
+// want_uv = false: (u, v) are left 0 unless the record needs them itself (a
+// smooth triangle's barycentrics interpolate its normal) -- for callers whose
+// scene reads no image texture, where nothing else looks at them.
 template <class R, uint32_t FEAT>
-LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_rep, Surf<R>& s) {
+LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_rep, Surf<R>& s,
+                    bool want_uv = true) {
   PrimT<R> local;
   const PrimT<R>& pr = prim_record<R, kPinRecords<FEAT>>(S.prims, idx, local);
   uint32_t km = pr.kind_mat, kind = km & KIND_MASK;
@@ -722,14 +726,17 @@ LMN_HD void surface(const SceneView<R>& S, uint32_t idx, V3<R> o, V3<R> d, R t_r
       V3<R> nn = cvt<R>(prim_normal_analytic<double>(kind, p64, q));
       n = nn;
       const R pi = R(3.14159265358979323846);
-      u64 = double((m_atan2(-nn.z, nn.x) + pi) / (R(2) * pi));
-      v64 = double(m_acos(-nn.y) / pi);
+      u64 = want_uv ? double((m_atan2(-nn.z, nn.x) + pi) / (R(2) * pi)) : 0.0;
+      v64 = want_uv ? double(m_acos(-nn.y) / pi) : 0.0;
     }
     s.u = R(u64);
     s.v = R(v64);
   } else {
     n = prim_normal_analytic<R>(kind, pr.p, oo + dd * t_true);
-    prim_uv_analytic<R>(kind, pr.p, oo, dd, t_true, s.u, s.v);
+    if (want_uv || (kind == K_TRIANGLE && (km & FLAG_SMOOTH)))
+      prim_uv_analytic<R>(kind, pr.p, oo, dd, t_true, s.u, s.v);
+    else
+      s.u = s.v = R(0);
     if (kind == K_TRIANGLE && (km & FLAG_SMOOTH)) {
       // MeshTriangle::hit (src/obj.rs:291-303): the normal and texture
       // coordinates interpolated at the barycentrics (u, v)
