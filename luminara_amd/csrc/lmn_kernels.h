@@ -7,10 +7,12 @@
 namespace lmn {
 
 // Path pool, structure of arrays (16 B per path per field):
-//   ob   = origin.xyz, beta.x     db   = direction.xyz, beta.y
+//   ob   = origin.xyz, scene index (bits)
+//   db   = direction.xyz, origin record (bits: the primitive the ray leaves)
 //   meta = work item, remaining depth | skip-emission flag << 16,
 //          rng key lo, rng key hi
-//   aux  = rng counter, beta.z (bit pattern), scene index, spare
+//   aux  = rng counter, beta.xyz (bit patterns)
+// Extend reads a stored ray from ob and db alone.
 struct PathSoA {
   float4* ob;
   float4* db;

@@ -201,11 +201,10 @@ __device__ __forceinline__ void pool_ray(const BatchDev& B, const PathSoA& cur, 
   const uint32_t ns = ctr->n_surv;
   if (i < ns) {
     const float4 a = cur.ob[i], b = cur.db[i];
-    const uint4 x = cur.aux[i];
     o = {a.x, a.y, a.z};
     d = {b.x, b.y, b.z};
-    sid = x.z;
-    org = x.w;
+    sid = __float_as_uint(a.w);
+    org = __float_as_uint(b.w);
   } else {
     CameraPath c;
     camera_path(B, ctr->cam_item + (i - ns), c);
@@ -704,12 +703,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void
       const uint4 M = cur.meta[j], X = cur.aux[j];
       o = f3(A);
       d = f3(Bd);
-      beta = {A.w, Bd.w, __uint_as_float(X.y)};
+      beta = {__uint_as_float(X.y), __uint_as_float(X.z), __uint_as_float(X.w)};
       rng.key = (uint64_t(M.w) << 32) | M.z;
       rng.n = X.x;
       item = M.x;
       depth = M.y & 0xffffu;
-      sid = X.z;
+      sid = __float_as_uint(A.w);
       skip = (M.y >> 16) & 1u;
     } else {
       CameraPath c;
@@ -792,11 +791,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6, 6))) void
       sh.c[ps] = make_float4(cb.x, cb.y, cb.z, __uint_as_float(sid));
     }
     if (cont) {
-      nxt.ob[pn] = make_float4(sp_o.x, sp_o.y, sp_o.z, nbeta.x);
-      nxt.db[pn] = make_float4(ndir.x, ndir.y, ndir.z, nbeta.y);
+      nxt.ob[pn] = make_float4(sp_o.x, sp_o.y, sp_o.z, __uint_as_float(sid));
+      nxt.db[pn] = make_float4(ndir.x, ndir.y, ndir.z, __uint_as_float(rec));
       nxt.meta[pn] = make_uint4(item, (depth - 1) | (use_nee ? 0x10000u : 0u), uint32_t(rng.key),
                                 uint32_t(rng.key >> 32));
-      nxt.aux[pn] = make_uint4(rng.n, __float_as_uint(nbeta.z), sid, rec);
+      nxt.aux[pn] = make_uint4(rng.n, __float_as_uint(nbeta.x), __float_as_uint(nbeta.y), __float_as_uint(nbeta.z));
     }
   }
 }
