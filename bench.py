@@ -395,9 +395,10 @@ def measure(wl_name, scenes, desc, args, want_images, tag):
         # events on its stream
         counts = lm.profile_extend(worlds, params("wavefront"), ptrs)
         rays, hits = counts["rays"], counts["hits"]
-        # per ray: origin + beta, direction + beta, scene word (3 x 16 B) read;
-        # a hit writes the 32-B surface record, a miss its 16-B half
-        io_bytes = 48 * rays + 32 * hits + 16 * (rays - hits)
+        # a stored (bounce) ray reads its origin and direction quarters (32 B;
+        # a camera ray is generated in the kernel from the launch tables, one
+        # per path); a hit writes the 32-B surface record, a miss its 16-B half
+        io_bytes = 32 * max(rays - paths, 0) + 32 * hits + 16 * (rays - hits)
         alg_bytes = counts["node_fetches"] * counts["node_bytes"] + counts["prim_tests"] * counts["prim_bytes"] + io_bytes
         bytes_per_launch = alg_bytes / launches
         avg_launch_s = ext_ms / launches / 1e3
@@ -416,7 +417,7 @@ def measure(wl_name, scenes, desc, args, want_images, tag):
                     "kernel": "k_extend", "avg_launch_us": avg_launch_s * 1e6, "launches_per_step": launches,
                     "bytes_per_launch": bytes_per_launch,
                     "bytes_model": f"{counts['node_bytes']} B per 4-wide BVH node fetched + 64 B per primitive record tested "
-                                   "+ 48 B ray read + 32 B (hit) or 16 B (miss) record written",
+                                   "+ 32 B read per stored (bounce) ray + 32 B (hit) or 16 B (miss) record written",
                     "per_ray": {"node_fetches": counts["node_fetches"] / rays, "prim_tests": counts["prim_tests"] / rays,
                                 "hit_fraction": hits / rays}}
         log(f"[bench:{tag}] extend: {counts}, {ext_ms:.2f} ms/step over {launches:.0f} launches, "
