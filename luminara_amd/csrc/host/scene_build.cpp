@@ -995,6 +995,10 @@ int build_scene_tables(const lmn_scene_desc& desc, const BvhOptions& opt, HostSc
   {
     std::vector<Node4F> wide;
     out.stack4_words = collapse_bvh4(out.nodesF, wide);
+    if (wide.size() > (size_t(1) << 26)) {  // the kernels form a node's byte offset in 32 bits
+      err = "too many BVH nodes for the 4-wide tree (at most 2^26)";
+      return LMN_ERR_INVALID;
+    }
     quantize_bvh4(wide, out.nodes4);
   }
   out.n_bounded = uint32_t(bounded.size() + marched.size());

@@ -439,7 +439,8 @@ LMN_HD void bvh_walk(const SceneView<R>& S, const typename NodeSel<R>::type* nod
 
 // The four 16-byte quarters of 4-wide node `ref`.  On the device they are
 // global-memory loads off the scene's node array (a scalar base and a 32-bit
-// offset), all four issued before any is used.
+// offset: the host builder caps a tree at 2^26 nodes), all four issued
+// before any is used.
 struct Words4 {
   uint32_t x, y, z, w;
 };
