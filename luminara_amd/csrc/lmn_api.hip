@@ -1303,7 +1303,7 @@ int lmn_autotune(const lmn_scene* const* scenes, size_t n_scenes, const lmn_rend
   // both), then the grid size up to one thread per pool path, then the LDS
   // stack depth, then the grid again around the winner
   const WaveConfig start = best;
-  for (uint32_t pool : {1u << 21, 1u << 22, 1u << 24, 1u << 25, 1u << 26})
+  for (uint32_t pool : {1u << 21, 1u << 22, 1u << 24, 1u << 25, 1u << 26, 1u << 27})
     for (uint32_t block : {128u, 256u}) {
       WaveConfig c = start;
       c.pool = uint32_t(std::min<uint64_t>(pool, std::max<uint64_t>(total, 64)));
