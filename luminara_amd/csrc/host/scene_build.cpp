@@ -635,7 +635,7 @@ void quantize_bvh4(const std::vector<Node4F>& in, std::vector<Node4Q>& out) {
       }
       const long double step = std::ldexp((long double)1, e);
       q.org[a] = org;
-      meta |= uint32_t(e + 127) << (8 * a);
+      meta |= uint32_t(uint8_t(int8_t(e))) << (8 * a);
       for (uint32_t k = 0; k < n; ++k) {
         const long double dl = (long double)lo[a][k] - (long double)org;
         const long double dh = (long double)hi[a][k] - (long double)org;

@@ -111,8 +111,8 @@ static_assert(sizeof(Node4F) == 128, "FP32 BVH4 node must be 128 bytes");
 //   w0: org.x org.y org.z qlo.x      w1: qlo.y qlo.z qhi.x qhi.y
 //   w2: qhi.z child0 child1 child2   w3: child3 meta  pad pad
 // qlo.a / qhi.a hold child k's lower / upper plane index of axis a in byte
-// k; meta holds the biased FP32 exponent of axis a's grid step in byte a
-// and the number of children in use in byte 3.  Every word the box test
+// k; meta holds the exponent e of axis a's grid step 2^e as a signed byte
+// in byte a, and the number of children in use in byte 3.  Every word the box test
 // needs sits in a different 16-byte quarter, so all four loads are issued
 // together and the node costs one memory round trip.
 struct alignas(64) Node4Q {

@@ -103,6 +103,10 @@ struct LdsStack {
     const size_t d = size_t(w - SS);
     return PAIRS ? (d >> 1) * 2 * nthreads + 2 * size_t(slot) + (d & 1) : d * nthreads + slot;
   }
+  // the same for the pair at word w, which push2/pop2 keep even: one
+  // multiply-add off a per-lane base
+  __device__ uint32_t pair_lds_index(int w) const { return uint32_t(w) * BS + 2u * tid; }
+  __device__ size_t pair_spill_index(int w) const { return size_t(w - SS) * nthreads + 2 * size_t(slot); }
   // Accesses through address-space-qualified pointers: the LDS and spill
   // branches then stay LDS and global accesses (from two generic pointers
   // the compiler forms one flat access to whichever the branch picks).
@@ -141,9 +145,9 @@ struct LdsStack {
   __device__ void push2(uint32_t a, uint32_t b) {
     if constexpr (PAIRS) {
       if (sp < SS)
-        *L<uint2>(lds_index(sp)) = make_uint2(a, b);
+        *L<uint2>(pair_lds_index(sp)) = make_uint2(a, b);
       else
-        *G<uint2>(spill_index(sp)) = make_uint2(a, b);
+        *G<uint2>(pair_spill_index(sp)) = make_uint2(a, b);
       sp += 2;
     } else {
       push(a);
@@ -159,14 +163,14 @@ struct LdsStack {
         // one LDS read, waited on in the same block: written as a plain load
         // the compiler merges it with the spill branch's into one flat load
         uint64_t x;
-        const uint32_t addr = uint32_t(size_t(L<uint2>(lds_index(sp))));
+        const uint32_t addr = uint32_t(size_t(L<uint2>(pair_lds_index(sp))));
         asm volatile("ds_read_b64 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(x) : "v"(addr) : "memory");
         v = make_uint2(uint32_t(x), uint32_t(x >> 32));
 #else
-        v = *L<uint2>(lds_index(sp));
+        v = *L<uint2>(pair_lds_index(sp));
 #endif
       } else {
-        v = *G<uint2>(spill_index(sp));
+        v = *G<uint2>(pair_spill_index(sp));
       }
       a = v.x;
       b = v.y;

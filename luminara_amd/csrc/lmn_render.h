@@ -456,7 +456,6 @@ LMN_HD void node4_words(const Node4Q* nodes, uint32_t ref, Words4 q[4]) {
   __builtin_memcpy(q, nodes + ref, sizeof(Node4Q));
 #endif
 }
-
 // Inverse direction for the FP32 slab tests: components clamped to 2^80 in
 // magnitude, so that an axis-parallel ray gets a finite factor (the 4-wide
 // nodes form slab distances as fma(q, step / d, (org - o) / d), where an
@@ -475,9 +474,9 @@ LMN_HD void node4_children(const Words4 w[4], V3<float> o, V3<float> inv, float 
   // slab distance of grid plane q on axis a: fma(q, a_a, b_a), with
   // a_a = step_a / d_a (exact: step_a is a power of two) and
   // b_a = (org_a - o_a) / d_a
-  const float ax = float_of((meta & 0xffu) << 23) * inv.x;
-  const float ay = float_of(((meta >> 8) & 0xffu) << 23) * inv.y;
-  const float az = float_of(((meta >> 16) & 0xffu) << 23) * inv.z;
+  const float ax = ldexpf(inv.x, int(int8_t(meta & 0xffu)));
+  const float ay = ldexpf(inv.y, int(int8_t((meta >> 8) & 0xffu)));
+  const float az = ldexpf(inv.z, int(int8_t((meta >> 16) & 0xffu)));
   const float bx = (float_of(w[0].x) - o.x) * inv.x;
   const float by = (float_of(w[0].y) - o.y) * inv.y;
   const float bz = (float_of(w[0].z) - o.z) * inv.z;

@@ -123,7 +123,7 @@ int host_quant_check(const lmn_scene_desc* desc, const lmn_ray* rays, size_t n, 
     const uint32_t lo[3] = {g.qlo_x, g.qlo_y, g.qlo_z}, hi[3] = {g.qhi_x, g.qhi_y, g.qhi_z};
     for (uint32_t k = 0; k < f.n; ++k)
       for (int a = 0; a < 3; ++a) {
-        const long double step = std::ldexp((long double)1, int((g.meta >> (8 * a)) & 0xffu) - 127);
+        const long double step = std::ldexp((long double)1, int(int8_t((g.meta >> (8 * a)) & 0xffu)));
         const long double pl = (long double)g.org[a] + ((lo[a] >> (8 * k)) & 0xffu) * step;
         const long double ph = (long double)g.org[a] + ((hi[a] >> (8 * k)) & 0xffu) * step;
         const float flo = (&f.lox[0])[a * 4 + k], fhi = (&f.hix[0])[a * 4 + k];
