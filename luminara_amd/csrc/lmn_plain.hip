@@ -116,8 +116,7 @@ __global__ __launch_bounds__(256) void k_plain_resolve(const double4* __restrict
   const uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p >= npix) return;
   double4 a = acc[p];
-  for (uint32_t s = 0; s < passes; ++s) {
-    const double4 v = samples[size_t(s) * npix + p];
+  auto add = [&](const double4& v) {
     double x = v.x, y = v.y, z = v.z;
     double lum = 0.2126 * x + 0.7152 * y + 0.0722 * z;
     if (lum > 100.0) {
@@ -130,7 +129,18 @@ __global__ __launch_bounds__(256) void k_plain_resolve(const double4* __restrict
     a.y += y * v.w;
     a.z += z * v.w;
     a.w += v.w;
+  };
+  // the sample loads go out eight at a time (one memory round trip per
+  // eight samples); the sums keep sample order
+  uint32_t s = 0;
+  for (; s + 8u <= passes; s += 8u) {
+    double4 v[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; ++k) v[k] = samples[size_t(s + k) * npix + p];
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; ++k) add(v[k]);
   }
+  for (; s < passes; ++s) add(samples[size_t(s) * npix + p]);
   acc[p] = a;
 }
 
