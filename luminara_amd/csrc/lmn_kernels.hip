@@ -138,6 +138,29 @@ __device__ __forceinline__ void for_scene(uint32_t sid, F&& body) {
     }
   }
 }
+// The same with the pass's index in a scalar register inside the body, so
+// that the records it selects are read with scalar loads (for the camera
+// records of a wave's camera rays, which come from one part but for the
+// waves that straddle a part boundary).  Two things keep the index scalar:
+// - the loop ends on a wave-uniform condition, so the body runs inside the
+//   pass with the lanes of one index (a body on a divergent loop exit runs
+//   after the loop, once per lane's own index);
+// - inside the branch the compiler knows s0 == sid and would pass on the
+//   per-lane sid; the body takes a copy made opaque by an empty asm.
+template <class F>
+__device__ __forceinline__ void for_scene_scalar(uint32_t sid, F&& body) {
+  bool done = false;
+  do {
+    const uint64_t waiting = __builtin_amdgcn_ballot_w64(!done);
+    const uint32_t s0 = uint32_t(__builtin_amdgcn_readlane(int(sid), int(__builtin_ctzll(waiting))));
+    uint32_t su = s0;
+    asm volatile("" : "+s"(su));
+    if (!done && sid == s0) {
+      body(su);
+      done = true;
+    }
+  } while (__builtin_amdgcn_ballot_w64(!done) != 0);
+}
 
 // Surface record of a closest hit (see HitSoA).
 template <uint32_t FEAT>
@@ -174,6 +197,10 @@ struct CameraPath {
   Rng rng;           // after the camera draws
   uint32_t scene, max_depth;
 };
+// SCALAR: the part's frame and camera records through scalar loads (the
+// extend kernel, whose load-return path the traversal keeps busy); the
+// other callers read them per lane, which spares their scalar registers.
+template <bool SCALAR = false>
 __device__ __forceinline__ void camera_path(const BatchDev& B, uint64_t item, CameraPath& c) {
   // part holding the item: largest p with part_begin[p] <= item.  A wave's
   // camera slots hold consecutive items, so the binary search runs once, on
@@ -191,7 +218,7 @@ __device__ __forceinline__ void camera_path(const BatchDev& B, uint64_t item, Ca
       hi = mid;
   }
   while (lo + 1 < B.n_parts && uniform_load(B.part_begin, lo + 1) <= item) ++lo;
-  for_scene(lo, [&](uint32_t part) {
+  auto gen = [&](uint32_t part) {
     const FrameDev F = uniform_load(B.parts, part);
     const CameraT<float> cam = uniform_load(B.views, F.scene).cam;
     uint32_t pixel;
@@ -199,11 +226,16 @@ __device__ __forceinline__ void camera_path(const BatchDev& B, uint64_t item, Ca
                          pixel);
     c.scene = F.scene;
     c.max_depth = F.max_depth;
-  });
+  };
+  if constexpr (SCALAR)
+    for_scene_scalar(lo, gen);
+  else
+    for_scene(lo, gen);
 }
 
 // The ray of pool slot i: a stored path (i < n_surv) or a new camera ray.
 // org: the record the ray leaves from (kNoOrigin for camera rays).
+template <bool SCALAR = false>
 __device__ __forceinline__ void pool_ray(const BatchDev& B, const PathSoA& cur, const WaveCounters* ctr, uint32_t i,
                                          V3<float>& o, V3<float>& d, uint32_t& sid, uint32_t& org) {
   const uint32_t ns = ctr->n_surv;
@@ -215,7 +247,7 @@ __device__ __forceinline__ void pool_ray(const BatchDev& B, const PathSoA& cur, 
     org = __float_as_uint(b.w);
   } else {
     CameraPath c;
-    camera_path(B, ctr->cam_item + (i - ns), c);
+    camera_path<SCALAR>(B, ctr->cam_item + (i - ns), c);
     o = c.o;
     d = c.d;
     sid = c.scene;
@@ -237,7 +269,7 @@ __device__ __forceinline__ void extend_body(const BatchDev& B, const PathSoA& cu
   for (uint32_t i = gid; i < n; i += nthreads) {
     V3<float> o, d;
     uint32_t sid, org;  // org: the record the ray leaves from
-    pool_ray(B, cur, ctr, i, o, d, sid, org);
+    pool_ray<true>(B, cur, ctr, i, o, d, sid, org);
     for_scene(sid, [&](uint32_t s) {
       const SceneView<float> S = scene_view<FEAT>(B, s);
       LdsStack<SS, BS, kPinRecords<FEAT>> st{lds, spill, threadIdx.x, gid, nthreads, 0};
